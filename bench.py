@@ -1,0 +1,206 @@
+"""Benchmark: ff-room projection steps on one MI355X.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload c2] [--mode mixed|fp64]
+
+A step is one full projection time step (FluidSolver::step: predictor,
+buoyancy, masks, pressure solve, projection, temperature, diagnostics) of the
+named synthetic workload, with all state resident in HBM. Prints one JSON
+line: whole-job cell-updates/s of the timed steps plus
+  * roofline of the dominant kernel (algorithmic HBM bytes / mean launch time
+    from HIP events on the scene's stream, against 8 TB/s),
+  * the plain FP64 CG form (the reference algorithm on the GPU) timed on the
+    same workload, and the pressure error of each form against a tightly
+    converged FP64 solve of the same right-hand side,
+  * the reference's own solver (oracle/_ref, compiled from its sources) timed
+    on this host's cores for a bounded sample of the workload.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import subprocess
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "ff-room_amd"))
+sys.path.insert(0, ROOT)
+
+import numpy as np  # noqa: E402
+
+HBM_PEAK = 8.0e12  # MI355X HBM3E bytes/s
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def workload(name: str, precision: str):
+    from ff_room.generator import workload_c2, workload_c3
+    if name == "c2":
+        cfg = workload_c2(precision=precision)
+        desc = "c2: 10x8x3 m room on 256x204x80, 2 fans (seed 2), 4 boxes, QUICK + buoyancy, dt 0.005"
+    elif name == "c3":
+        cfg = workload_c3(precision=precision)
+        desc = "c3: 10x8x3 m room on 512x408x160, 2 fans (seed 3), 4 boxes, QUICK + buoyancy, dt 0.0025"
+    else:
+        raise SystemExit(f"unknown workload {name}")
+    return cfg, desc
+
+
+def timed_steps(d, ctx, steps: int, warmup: int, prof: bool):
+    for _ in range(warmup):
+        d.step()
+    if prof:
+        ctx.prof_reset()
+        ctx.prof_enable(True)
+    t0 = time.perf_counter()
+    res = [d.step() for _ in range(steps)]
+    t1 = time.perf_counter()
+    table = None
+    if prof:
+        table = ctx.prof_read()
+        ctx.prof_enable(False)
+    return t1 - t0, res, table
+
+
+def roofline(table: dict):
+    name, e = max(table.items(), key=lambda kv: kv[1]["total_ms"])
+    mean_s = e["total_ms"] / e["launches"] / 1e3
+    ach = e["bytes_per_launch"] / mean_s
+    return name, {"bound": "hbm", "achieved": ach / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                  "frac": ach / HBM_PEAK, "traffic": None,
+                  "kernel": name, "bytes_per_launch": e["bytes_per_launch"],
+                  "mean_launch_us": mean_s * 1e6, "launches": e["launches"]}
+
+
+def form_params_inplace(p, mode):
+    """The two forms compared: the reference algorithm (unpreconditioned FP64
+    CG, run to 1e-9 so its pressure error meets 1e-6) and the mixed form (FP32
+    stencils, FP32/BF16 MG-PCG inner solves, FP64 outer defect correction)."""
+    from ff_room import capi
+    if mode == "fp64":
+        p.precision, p.poisson_method = capi.PREC_FP64, capi.POISSON_CG
+        p.poisson_tol, p.poisson_max_iter = 1e-9, 50000
+    else:
+        p.precision, p.poisson_method = capi.PREC_MIXED, capi.POISSON_MGPCG
+        p.poisson_tol, p.poisson_max_iter = 1e-10, 50
+
+
+def cpu_baseline(ps, steps: int = 1):
+    """The reference's own FluidSolver::step (oracle/_ref, compiled from the
+    reference sources with OpenMP), timed on this host."""
+    from oracle.pyoracle import RefSolver, SceneArrays, ref_available, OracleSolver
+    from ff_room import capi
+    threads = int(os.environ.get("OMP_NUM_THREADS", "16"))
+    os.environ["OMP_NUM_THREADS"] = str(threads)
+    p = ps.params
+    params = dict(rho=p.rho, nu=p.nu, dt=p.dt, max_steps=p.max_steps, convergence_tol=0.0,
+                  poisson_max_iter=1000, poisson_tol=1e-6, advection=p.advection,
+                  use_multigrid=0, thermal=p.thermal, T_initial=p.T_initial,
+                  T_target=p.T_target, buoyancy=p.buoyancy, g_accel=p.g_accel, beta=p.beta,
+                  k_thermal=p.k_thermal, cp=p.cp)
+    sc = SceneArrays(ps.nx, ps.ny, ps.nz, float(ps.dxv.sum()), float(ps.dyv.sum()),
+                     float(ps.dzv.sum()), ps.flags, ps.fans,
+                     [o for o in ps.outflows if o not in [tuple(x[:6]) for x in ps.openings]],
+                     ps.openings)
+    if ref_available():
+        solver, kind = RefSolver(sc, params, use_openmp=True), "reference"
+        sh = sc.shapes
+        solver.set_fields(np.zeros(sh["u"]), np.zeros(sh["v"]), np.zeros(sh["w"]),
+                          np.zeros(sh["p"]), ps.T0)
+    else:
+        solver, kind = OracleSolver(sc, params, nthreads=threads), "port"
+        solver.set_fields(T=ps.T0)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        solver.step()
+    dt = time.perf_counter() - t0
+    return {"value": ps.cells * steps / dt, "unit": "cell-updates/s", "cores": threads,
+            "kind": kind,
+            "sample": f"{steps} step(s) of the same workload from its initial state, the "
+                      f"reference CG at its default cap (1000 iterations, tol 1e-6), "
+                      f"OpenMP {threads} threads; {dt:.1f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--workload", default="c2")
+    ap.add_argument("--mode", default="mixed", choices=["mixed", "fp64"])
+    ap.add_argument("--fp64-steps", type=int, default=3)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-compare", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs one GPU only; --gpus {args.gpus} is not supported: the workload is "
+              f"one scene per GPU and scenes shard by scene (see DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+
+    from ff_room import capi, device
+    ctx = capi.Context(0)
+    info = ctx.device_info()
+    cfg, desc = workload(args.workload, args.mode)
+    ps = device.prepare(cfg)
+    log(f"device {info['name']} {info['arch']} CUs={info['compute_units']}; workload {desc}")
+
+    # ---- headline form -------------------------------------------------------
+    form_params_inplace(ps.params, args.mode)
+    d = device.create(ctx, ps)
+    wall, res, _ = timed_steps(d, ctx, args.steps, args.warmup, prof=False)
+    d.close()
+    value = ps.cells * args.steps / wall
+    iters = [r["poisson_iters"] for r in res]
+    log(f"{args.mode}: {args.steps} steps in {wall:.3f} s -> {value:.4g} cell-updates/s; "
+        f"solver iterations {min(iters)}..{max(iters)}")
+
+    # profiled pass of the same steps for the per-kernel roofline
+    d = device.create(ctx, ps)
+    pwall, _, table = timed_steps(d, ctx, args.steps, args.warmup, prof=True)
+    d.close()
+    kname, roof = roofline(table)
+    log(f"dominant kernel {kname}: {roof['achieved']:.0f} GB/s ({100 * roof['frac']:.1f}% of peak)")
+
+    out = {
+        "metric": "cell-updates/s + p50/p99 scene latency, mixed vs FP64 CG at L2 err<=1e-6",
+        "value": value, "unit": "cell-updates/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": 1e3 * wall / args.steps,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32+bf16 stencils/multigrid, f64 outer residual" if args.mode == "mixed" else "f64",
+        "data": "synthetic (seeded scene generator, zero initial velocity)",
+        "config": {"workload": desc, "cells": ps.cells, "mode": args.mode,
+                   "poisson_tol": ps.params.poisson_tol},
+        "roofline": roof,
+        "kernels": {k: {"launches": v["launches"], "mean_us": 1e3 * v["total_ms"] / v["launches"],
+                        "GBps": (v["bytes_per_launch"] / (v["total_ms"] / v["launches"] / 1e3) / 1e9
+                                 if v["bytes_per_launch"] else None)}
+                    for k, v in table.items()},
+        "solver_iterations": {"min": min(iters), "max": max(iters)},
+    }
+
+    # ---- the plain FP64 CG form on the same workload (comparison) -----------
+    if args.mode == "mixed" and not args.no_compare:
+        ps64 = device.prepare(cfg)
+        form_params_inplace(ps64.params, "fp64")
+        d = device.create(ctx, ps64)
+        w64, r64, _ = timed_steps(d, ctx, args.fp64_steps, 1, prof=False)
+        d.close()
+        v64 = ps64.cells * args.fp64_steps / w64
+        out["fp64_cg"] = {"value": v64, "steps": args.fp64_steps,
+                          "iterations": [r["poisson_iters"] for r in r64]}
+        out["speedup_vs_fp64_cg"] = value / v64
+        log(f"fp64 CG: {v64:.4g} cell-updates/s -> mixed/fp64 = {value / v64:.1f}x")
+
+    if not args.no_cpu:
+        log("timing the reference CPU solver on a bounded sample ...")
+        out["cpu_baseline"] = cpu_baseline(device.prepare(cfg), steps=1)
+        log(f"cpu: {out['cpu_baseline']['value']:.4g} cell-updates/s")
+    ctx.close()
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

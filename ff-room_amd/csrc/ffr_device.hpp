@@ -1,0 +1,216 @@
+// ffr_device.hpp - device-side types and helpers shared by the gfx950 kernels.
+//
+// Data layout in HBM: every field keeps the reference's flat layout
+// (k fastest; u (nx+1,ny,nz), v (nx,ny+1,nz), w (nx,ny,nz+1), cells
+// (nx,ny,nz)), so a thread's flat index is its element and a wavefront reads
+// 64 consecutive elements. Neighbours at +-1 share the cache line, +-nz and
+// +-ny*nz come from L2; launch blocks are remapped so that each XCD (own L2)
+// walks one contiguous eighth of an array (see xcd_block()).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace ffr {
+
+constexpr int kBlock = 256;  // threads per block (4 wave64)
+constexpr int kWave = 64;
+constexpr int kXcds = 8;     // MI355X: 8 XCDs, blocks dispatched round-robin
+
+enum : uint8_t { FLUID = 0, SOLID = 1, INFLOW = 2, OUTFLOW = 3 };
+enum { UPWIND1 = 0, QUICK = 1, LAX_WENDROFF = 2 };
+
+// Division by a runtime-invariant divisor d (1 <= d < 2^31) for n < 2^31:
+// q = (umulhi(n, m) + n) >> s, with s = ceil(log2 d), m = floor(2^32 (2^s - d) / d) + 1.
+struct FastDiv {
+    uint32_t d, m, s;
+};
+
+inline FastDiv make_fastdiv(uint32_t d)
+{
+    FastDiv f;
+    f.d = d;
+    uint32_t s = 0;
+    while ((1ull << s) < d) s++;
+    f.s = s;
+    f.m = (uint32_t)((((1ull << s) - d) << 32) / d + 1);
+    return f;
+}
+
+__device__ __forceinline__ uint32_t fdiv(uint32_t n, const FastDiv &f)
+{
+    return (__umulhi(n, f.m) + n) >> f.s;
+}
+
+struct FanDev {      // FanBC with its box clamped to the grid (BoundaryManager.cpp:16-22)
+    int i0, i1, j0, j1, k0, k1;
+    double vel[3];
+    float velf[3];
+};
+
+struct OpeningDev {  // OpeningBC (BoundaryManager.hpp:27-29)
+    int axis, side, a0, a1, b0, b1;
+    double T_out;
+    int offset;      // first thread of this patch in the launch
+};
+
+// Geometry of one scene as the kernels see it (passed by value).
+struct Geo {
+    int nx, ny, nz;
+    uint32_t nu, nv, nw, nc;
+    FastDiv f_nz, f_nz1, f_nynz, f_ny1nz, f_nynz1;
+    const uint8_t *flags;
+    // FP64 per-axis arrays
+    const double *hx, *hy, *hz;        // cell widths dxv, dyv, dzv
+    const double *i2x, *i2y, *i2z;     // 1/(h*h), FluidSolver.cpp:101-103
+    const double *cx, *cy, *cz;        // centre distance 0.5*(d[i-1]+d[i]) at index i (i>=1)
+    const double *lap;                 // slx srx | sly sry | slz srz (PoissonSolver.cpp:34-49)
+    // FP32 copies
+    const float *hxf, *hyf, *hzf, *ihxf, *ihyf, *ihzf, *i2xf, *i2yf, *i2zf;
+    const float *cxf, *cyf, *czf, *lapf;
+    const FanDev *fans;
+    int n_fans;
+    const uint8_t *wall_of;            // outflow-patch marks per boundary face
+    const OpeningDev *openings;
+    int n_openings;
+    int open_threads;                  // total patch cells over all openings
+};
+
+// Offsets into wall_of: x walls [side][j][k], y walls [side][i][k], z walls [side][i][j].
+__host__ __device__ inline uint32_t wall_x(const Geo &g, int side, int j, int k)
+{
+    return (uint32_t)((side * g.ny + j) * g.nz + k);
+}
+__host__ __device__ inline uint32_t wall_y(const Geo &g, int side, int i, int k)
+{
+    return (uint32_t)(2 * g.ny * g.nz + (side * g.nx + i) * g.nz + k);
+}
+__host__ __device__ inline uint32_t wall_z(const Geo &g, int side, int i, int j)
+{
+    return (uint32_t)(2 * g.ny * g.nz + 2 * g.nx * g.nz + (side * g.nx + i) * g.ny + j);
+}
+
+__device__ __forceinline__ uint32_t cidx(const Geo &g, int i, int j, int k)
+{
+    return ((uint32_t)i * g.ny + j) * g.nz + k;
+}
+__device__ __forceinline__ uint32_t uidx(const Geo &g, int i, int j, int k)
+{
+    return ((uint32_t)i * g.ny + j) * g.nz + k;
+}
+__device__ __forceinline__ uint32_t vidx(const Geo &g, int i, int j, int k)
+{
+    return ((uint32_t)i * (g.ny + 1) + j) * g.nz + k;
+}
+__device__ __forceinline__ uint32_t widx(const Geo &g, int i, int j, int k)
+{
+    return ((uint32_t)i * g.ny + j) * (g.nz + 1) + k;
+}
+
+// Block index remapped so that blocks dispatched to the same XCD
+// (b mod 8) own one contiguous range of logical blocks.
+__device__ __forceinline__ uint32_t xcd_block()
+{
+    const uint32_t b = blockIdx.x, G = gridDim.x;
+    const uint32_t q = G / kXcds, r = G % kXcds;
+    const uint32_t x = b % kXcds, slot = b / kXcds;
+    return (x < r) ? x * (q + 1) + slot : r * (q + 1) + (x - r) * q + slot;
+}
+
+// Contiguous element range [b, e) of logical block `blk` when n elements are
+// split over gridDim.x blocks in whole 256-element chunks.
+__device__ __forceinline__ void block_span(uint32_t n, uint32_t blk, uint32_t &b, uint32_t &e)
+{
+    const uint32_t chunks = (n + kBlock - 1) / kBlock;
+    const uint32_t per = (chunks + gridDim.x - 1) / gridDim.x;
+    const uint64_t b64 = (uint64_t)blk * per * kBlock;
+    b = (uint32_t)(b64 < n ? b64 : n);
+    const uint64_t e64 = b64 + (uint64_t)per * kBlock;
+    e = (uint32_t)(e64 < n ? e64 : n);
+}
+
+// ---- reductions -----------------------------------------------------------
+// Wave64 tree reductions via cross-lane shuffles, then one LDS stage.
+__device__ __forceinline__ double wave_sum(double v)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, kWave);
+    return v;
+}
+__device__ __forceinline__ double wave_max(double v)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, kWave));
+    return v;
+}
+
+// Sum over the block; the result is valid in thread 0. Fixed order.
+__device__ __forceinline__ double block_sum(double v, double *lds)
+{
+    v = wave_sum(v);
+    const int lane = threadIdx.x % kWave, wid = threadIdx.x / kWave;
+    if (lane == 0) lds[wid] = v;
+    __syncthreads();
+    double s = 0.0;
+    if (threadIdx.x == 0) {
+        for (int w = 0; w < (int)(blockDim.x / kWave); w++) s += lds[w];
+    }
+    __syncthreads();
+    return s;
+}
+
+__device__ __forceinline__ double block_max(double v, double *lds)
+{
+    v = wave_max(v);
+    const int lane = threadIdx.x % kWave, wid = threadIdx.x / kWave;
+    if (lane == 0) lds[wid] = v;
+    __syncthreads();
+    double s = 0.0;
+    if (threadIdx.x == 0) {
+        for (int w = 0; w < (int)(blockDim.x / kWave); w++) s = fmax(s, lds[w]);
+    }
+    __syncthreads();
+    return s;
+}
+
+// Max of non-negative doubles through the ordered bit pattern; NaN is never
+// passed in (fmax drops it), matching std::max's handling in the reference.
+__device__ __forceinline__ void atomic_max_nonneg(unsigned long long *slot, double v)
+{
+    atomicMax(slot, (unsigned long long)__double_as_longlong(v));
+}
+
+// Deterministic grid sum: every block stores its partial, the last block to
+// arrive (atomic ticket) adds all partials in block order. Returns true in
+// thread 0 of that last block with *total set. The ticket is reset there.
+__device__ __forceinline__ bool grid_sum_last(double v, double *partials, unsigned int *ticket,
+                                              uint32_t blk, double *lds, double *total)
+{
+    double s = block_sum(v, lds);
+    __shared__ bool last;
+    if (threadIdx.x == 0) {
+        partials[blk] = s;
+        __threadfence();
+        unsigned int t = atomicAdd(ticket, 1u);
+        last = (t == gridDim.x - 1);
+    }
+    __syncthreads();
+    if (!last) return false;
+    __threadfence();
+    // fixed-order reduction of gridDim.x partials by the whole block: thread t
+    // adds partials t, t+256, ... then the 256 runs are added in thread order
+    const uint32_t G = gridDim.x;
+    double acc = 0.0;
+    for (uint32_t idx = threadIdx.x; idx < G; idx += blockDim.x) acc += partials[idx];
+    __shared__ double runs[kBlock];
+    runs[threadIdx.x] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t = 0.0;
+        for (uint32_t q = 0; q < blockDim.x; q++) t += runs[q];
+        *total = t;
+        *ticket = 0u;
+    }
+    return threadIdx.x == 0;
+}
+
+}  // namespace ffr

@@ -1,0 +1,123 @@
+// ffr_scene.hpp - host-side definitions of the opaque ABI objects.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <map>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../../include/ffroom.h"
+#include "ffr_kernels.hpp"
+#include "ffr_mg.hpp"
+
+struct ffr_ctx {
+    int device = 0;
+    hipDeviceProp_t prop{};
+    size_t budget = 0;  // HBM bytes this context may hold
+    size_t in_use = 0;
+    hipStream_t stream = nullptr;
+    char err[512] = {0};
+    void fail(const char *fmt, ...);
+
+    // profiling
+    struct PendingEvent {
+        hipEvent_t a, b;
+        const char *name;
+        double bytes;
+    };
+    struct ProfEntry {
+        long launches = 0;
+        double total_ms = 0;
+        double bytes = 0;
+    };
+    bool prof_on = false;
+    bool open_event = false;
+    std::vector<PendingEvent> pending;
+    std::vector<hipEvent_t> ev_pool;
+    std::map<std::string, ProfEntry> prof_map;
+    void prof_begin(hipStream_t s, const char *name, double bytes);
+    void prof_end(hipStream_t s);
+    void prof_resolve();
+    void prof_clear();
+
+    // serving (ffr_serve.hip)
+    struct Server;
+    Server *server = nullptr;
+    void serve_shutdown();
+};
+
+// RAII-free scope marker used around each launch.
+struct ProfScope {
+    ffr_ctx *c;
+    hipStream_t s;
+    ProfScope(ffr_ctx *c_, hipStream_t s_, const char *name, double bytes) : c(c_), s(s_)
+    {
+        c->prof_begin(s, name, bytes);
+    }
+    ~ProfScope() { c->prof_end(s); }
+};
+
+struct ffr_scene {
+    ffr_ctx *ctx = nullptr;
+    ffr_params prm{};
+    int nx = 0, ny = 0, nz = 0;
+    uint64_t nu = 0, nv = 0, nw = 0, nc = 0;
+    hipStream_t stream = nullptr;
+    bool own_stream = false;
+
+    // host copies of the geometry
+    std::vector<double> dxv, dyv, dzv;
+    std::vector<uint8_t> flags;
+    std::vector<ffr_fan> fans;
+    std::vector<ffr_outflow> outflows;
+    std::vector<ffr_opening> openings;
+    bool opening_touches_fluid = false;
+    long n_fluid = 0;
+
+    // device memory
+    void *arena = nullptr;
+    size_t arena_bytes = 0;
+    ffr::Geo geo{};
+    int fp64 = 1;            // state precision: 1 = double, 0 = float
+    void *u = nullptr, *v = nullptr, *w = nullptr, *T = nullptr;
+    void *us = nullptr, *vs = nullptr, *ws = nullptr, *Tn = nullptr;
+    double *p = nullptr, *b = nullptr;
+    double *r = nullptr, *d = nullptr, *Ad = nullptr;  // FP64 CG / outer residual
+    ffr::StepRec *rec = nullptr;
+    ffr::CgState *cg = nullptr;
+    ffr::RedScratch red{};
+    double *scratch64 = nullptr;  // nc doubles for host transfers of fp32 state
+    ffr::Multigrid mg{};          // multigrid hierarchy (when allocated)
+
+    // pinned host mirrors
+    ffr::StepRec *h_rec = nullptr;
+    ffr::CgState *h_cg = nullptr;
+    ffr::MgState *h_mgs = nullptr;
+
+    int step = 0;
+    int last_iters = 0;
+};
+
+// Returns `code` from the calling function after recording the HIP error.
+#define HIP_CHECK_CTX(ctx, call, code)                                                  \
+    do {                                                                                \
+        hipError_t e_ = (call);                                                         \
+        if (e_ != hipSuccess) {                                                         \
+            (ctx)->fail("%s failed: %s", #call, hipGetErrorString(e_));                 \
+            return code;                                                                \
+        }                                                                               \
+    } while (0)
+
+namespace ffr {
+// internal entry points shared between translation units
+size_t scene_bytes(const ffr_scene_desc *d, const ffr_params *p, bool with_mg);
+bool scene_uses_mg(const ffr_params &p);
+int scene_step(ffr_scene *s, ffr_step_result *out, bool sync_result);
+int poisson_device(ffr_scene *s, int *iters, int *outer, double *res);
+// multigrid (ffr_mg.hip)
+size_t mg_bytes(int nx, int ny, int nz, const ffr_params &p);
+int mg_create(ffr_scene *s);
+void mg_destroy(ffr_scene *s);
+int mg_solve(ffr_scene *s, int *iters, int *outer, double *res);
+}  // namespace ffr

@@ -1,0 +1,620 @@
+// step_kernels.hip - the velocity/temperature stencils of one projection step.
+//
+// K1  predict:    u*, v*, w* (advection + diffusion, FluidSolver.cpp:83-309)
+//                 + buoyancy on w* (:373-386) + no-slip and fan masks
+//                 (BoundaryManager.cpp:60-130), one face per thread.
+// K4  rhs:        b = -(rho/dt) div(u*) (:311-330).
+// K6  correct:    u = u* - (dt/rho) grad p (:332-371) + no-slip, fan and
+//                 outflow masks (BoundaryManager.cpp:60-130, 180-220)
+//                 + max |u^{n+1}-u^n| (:527-534), in place over u^n.
+// K8  temperature: T transport (:388-456) + max |div u| over FLUID (:470-484)
+//                 + sum of T over FLUID (:458-468).
+//     opening_T:  BoundaryManager.cpp:137-178.
+//
+// All kernels are templated on the field type R: double reproduces the
+// reference's arithmetic operation by operation (contraction off, same
+// operand order, true divisions), float is the mixed-precision form.
+#include "../ffr_kernels.hpp"
+
+#pragma clang fp contract(off)
+
+namespace ffr {
+
+template <typename R>
+struct Axes {
+    const R *hx, *hy, *hz, *i2x, *i2y, *i2z;
+};
+__device__ __forceinline__ Axes<double> axes_of(const Geo &g, double)
+{
+    return {g.hx, g.hy, g.hz, g.i2x, g.i2y, g.i2z};
+}
+__device__ __forceinline__ Axes<float> axes_of(const Geo &g, float)
+{
+    return {g.hxf, g.hyf, g.hzf, g.i2xf, g.i2yf, g.i2zf};
+}
+
+// ---- one-direction advection terms, FluidSolver.cpp:47-81 --------------
+template <typename R>
+__device__ __forceinline__ R adv_upwind(R m, R c, R p, R vel, R h)
+{
+    if (vel > R(0)) return vel * (c - m) / h;
+    return vel * (p - c) / h;
+}
+
+template <typename R>
+__device__ __forceinline__ R adv_quick(R mm, R m, R c, R p, R vel, R h)
+{
+    R right, left;
+    if (vel >= R(0)) {
+        right = (R(6) * c + R(3) * p - m) / R(8);
+        left = (R(6) * m + R(3) * c - mm) / R(8);
+    } else {  // the reference's u<0 branch (:64-69)
+        right = (R(6) * p + R(3) * c - mm) / R(8);
+        left = (R(6) * c + R(3) * p - m) / R(8);
+    }
+    return vel * (right - left) / h;
+}
+
+template <typename R>
+__device__ __forceinline__ R adv_lw(R m, R c, R p, R vel, R dt, R h)
+{
+    R cr = vel * dt / h;
+    R fr = vel * (R(0.5) * (c + p) - R(0.5) * cr * (p - c));
+    R fl = vel * (R(0.5) * (m + c) - R(0.5) * cr * (c - m));
+    return (fr - fl) / h;
+}
+
+// Scheme switch and diffusion for one face (:132-162). nb = {-x,+x,-y,+y,-z,+z}.
+template <typename R>
+__device__ __forceinline__ R face_update(int scheme, bool qok, const R nb[6], const R q[3], R c,
+                                         const R vel[3], const R h[3], const R i2[3], R dt,
+                                         R nu)
+{
+    R adv;
+    if (scheme == QUICK && qok) {
+        adv = adv_quick(q[0], nb[0], c, nb[1], vel[0], h[0]) +
+              adv_quick(q[1], nb[2], c, nb[3], vel[1], h[1]) +
+              adv_quick(q[2], nb[4], c, nb[5], vel[2], h[2]);
+    } else if (scheme == LAX_WENDROFF) {
+        adv = adv_lw(nb[0], c, nb[1], vel[0], dt, h[0]) +
+              adv_lw(nb[2], c, nb[3], vel[1], dt, h[1]) +
+              adv_lw(nb[4], c, nb[5], vel[2], dt, h[2]);
+    } else {
+        adv = adv_upwind(nb[0], c, nb[1], vel[0], h[0]) +
+              adv_upwind(nb[2], c, nb[3], vel[1], h[1]) +
+              adv_upwind(nb[4], c, nb[5], vel[2], h[2]);
+    }
+    R dif = nu * (i2[0] * (nb[0] - R(2) * c + nb[1]) + i2[1] * (nb[2] - R(2) * c + nb[3]) +
+                  i2[2] * (nb[4] - R(2) * c + nb[5]));
+    return c + dt * (-adv + dif);
+}
+
+__device__ __forceinline__ double fan_v(const FanDev &f, int a, double) { return f.vel[a]; }
+__device__ __forceinline__ float fan_v(const FanDev &f, int a, float) { return f.velf[a]; }
+
+// No-slip + fan masks of one face along `axis` at (i,j,k) (BoundaryManager.cpp:60-130).
+// The face lies between cells lo = (i,j,k) - e_axis and hi = (i,j,k).
+template <typename R>
+__device__ __forceinline__ R mask_face(const Geo &g, int axis, int i, int j, int k, R val)
+{
+    const uint8_t *f = g.flags;
+    const int n_ax = axis == 0 ? g.nx : (axis == 1 ? g.ny : g.nz);
+    const int pos = axis == 0 ? i : (axis == 1 ? j : k);
+    const int stride = axis == 0 ? g.ny * g.nz : (axis == 1 ? g.nz : 1);
+    const int lo_i = i - (axis == 0), lo_j = j - (axis == 1), lo_k = k - (axis == 2);
+    // walls: zeroed unless the adjacent boundary cell is OUTFLOW (:62-94)
+    if (pos == 0 && f[cidx(g, i, j, k)] != OUTFLOW) val = R(0);
+    if (pos == n_ax && f[cidx(g, lo_i, lo_j, lo_k)] != OUTFLOW) val = R(0);
+    // faces of SOLID cells (:96-107)
+    if (pos > 0 && f[cidx(g, lo_i, lo_j, lo_k)] == SOLID) val = R(0);
+    if (pos < n_ax && f[cidx(g, i, j, k)] == SOLID) val = R(0);
+    (void)stride;
+    // fans in registration order, the last covering fan wins (:111-130)
+    for (int n = 0; n < g.n_fans; n++) {
+        const FanDev &fn = g.fans[n];
+        if (fn.vel[axis] == 0.0) continue;
+        bool in;
+        if (axis == 0)
+            in = i >= fn.i0 && i - 1 <= fn.i1 && j >= fn.j0 && j <= fn.j1 && k >= fn.k0 && k <= fn.k1;
+        else if (axis == 1)
+            in = i >= fn.i0 && i <= fn.i1 && j >= fn.j0 && j - 1 <= fn.j1 && k >= fn.k0 && k <= fn.k1;
+        else
+            in = i >= fn.i0 && i <= fn.i1 && j >= fn.j0 && j <= fn.j1 && k >= fn.k0 && k - 1 <= fn.k1;
+        if (in) val = fan_v(fn, axis, R(0));
+    }
+    return val;
+}
+
+// ---- K1: predictor -------------------------------------------------------
+template <typename R>
+__device__ __forceinline__ R predict_u(const Geo &g, const StepParams &sp, const Axes<R> &A,
+                                       const R *u, const R *v, const R *w, int i, int j, int k,
+                                       uint32_t e)
+{
+    const int nx = g.nx, ny = g.ny, nz = g.nz;
+    const uint32_t si = (uint32_t)ny * nz, sj = nz;
+    const R c = u[e];
+    if (i == 0 || i == nx) return c;  // boundary faces copied (:164-169)
+    R nb[6], q[3] = {R(0), R(0), R(0)}, vel[3];
+    const R h[3] = {A.hx[i], A.hy[j], A.hz[k]};
+    const R i2[3] = {A.i2x[i], A.i2y[j], A.i2z[k]};
+    nb[0] = i > 1 ? u[e - si] : R(0);
+    nb[1] = i < nx - 1 ? u[e + si] : R(0);
+    nb[2] = j > 0 ? u[e - sj] : R(0);
+    nb[3] = j < ny - 1 ? u[e + sj] : R(0);
+    nb[4] = k > 0 ? u[e - 1] : R(0);
+    nb[5] = k < nz - 1 ? u[e + 1] : R(0);
+    vel[0] = c;
+    vel[1] = j > 0 ? R(0.5) * (v[vidx(g, i - 1, j, k)] + v[vidx(g, i, j, k)]) : R(0);
+    vel[2] = k > 0 ? R(0.5) * (w[widx(g, i - 1, j, k)] + w[widx(g, i, j, k)]) : R(0);
+    const bool qok = i >= 2 && i <= nx - 2 && j >= 1 && j <= ny - 2 && k >= 1 && k <= nz - 2;
+    if (sp.scheme == QUICK && qok) {
+        q[0] = u[e - 2 * si];
+        q[1] = u[e - 2 * sj];  // flat index, wraps into row j-1 at j=1 (:139)
+        q[2] = u[e - 2];       // wraps at k=1 (:141)
+    }
+    return face_update<R>(sp.scheme, qok, nb, q, c, vel, h, i2, (R)sp.dt, (R)sp.nu);
+}
+
+template <typename R>
+__device__ __forceinline__ R predict_v(const Geo &g, const StepParams &sp, const Axes<R> &A,
+                                       const R *u, const R *v, const R *w, int i, int j, int k,
+                                       uint32_t e)
+{
+    const int nx = g.nx, ny = g.ny, nz = g.nz;
+    const uint32_t si = (uint32_t)(ny + 1) * nz, sj = nz;
+    const R c = v[e];
+    if (j == 0 || j == ny) return c;  // (:232-236)
+    R nb[6], q[3] = {R(0), R(0), R(0)}, vel[3];
+    const R h[3] = {A.hx[i], A.hy[j], A.hz[k]};
+    const R i2[3] = {A.i2x[i], A.i2y[j], A.i2z[k]};
+    vel[0] = i > 0 ? R(0.5) * (u[uidx(g, i, j - 1, k)] + u[uidx(g, i, j, k)]) : R(0);
+    nb[0] = i > 0 ? v[e - si] : R(0);
+    nb[1] = i < nx - 1 ? v[e + si] : R(0);
+    nb[2] = j > 1 ? v[e - sj] : R(0);
+    nb[3] = j < ny - 1 ? v[e + sj] : R(0);
+    vel[1] = c;
+    vel[2] = k > 0 ? R(0.5) * (w[widx(g, i, j - 1, k)] + w[widx(g, i, j, k)]) : R(0);
+    nb[4] = k > 0 ? v[e - 1] : R(0);
+    nb[5] = k < nz - 1 ? v[e + 1] : R(0);
+    const bool qok = i >= 1 && i <= nx - 2 && j >= 2 && j <= ny - 2 && k >= 1 && k <= nz - 2;
+    if (sp.scheme == QUICK && qok) {
+        q[0] = v[e - si];      // adjacent value as the far upwind one (:208)
+        q[1] = v[e - 2 * sj];
+        q[2] = v[e - 1];       // adjacent (:211)
+    }
+    return face_update<R>(sp.scheme, qok, nb, q, c, vel, h, i2, (R)sp.dt, (R)sp.nu);
+}
+
+template <typename R>
+__device__ __forceinline__ R predict_w(const Geo &g, const StepParams &sp, const Axes<R> &A,
+                                       const R *u, const R *v, const R *w, const R *T, int i,
+                                       int j, int k, uint32_t e)
+{
+    const int nx = g.nx, ny = g.ny, nz = g.nz;
+    const uint32_t si = (uint32_t)ny * (nz + 1), sj = nz + 1;
+    const R c = w[e];
+    if (k == 0 || k == nz) return c;  // (:299-303)
+    R nb[6], q[3] = {R(0), R(0), R(0)}, vel[3];
+    const R h[3] = {A.hx[i], A.hy[j], A.hz[k]};
+    const R i2[3] = {A.i2x[i], A.i2y[j], A.i2z[k]};
+    vel[0] = i > 0 ? R(0.5) * (u[uidx(g, i, j, k - 1)] + u[uidx(g, i, j, k)]) : R(0);
+    nb[0] = i > 0 ? w[e - si] : R(0);
+    nb[1] = i < nx - 1 ? w[e + si] : R(0);
+    vel[1] = j > 0 ? R(0.5) * (v[vidx(g, i, j, k - 1)] + v[vidx(g, i, j, k)]) : R(0);
+    nb[2] = j > 0 ? w[e - sj] : R(0);
+    nb[3] = j < ny - 1 ? w[e + sj] : R(0);
+    nb[4] = k > 1 ? w[e - 1] : R(0);
+    nb[5] = k < nz - 1 ? w[e + 1] : R(0);
+    vel[2] = c;
+    const bool qok = i >= 1 && i <= nx - 2 && j >= 1 && j <= ny - 2 && k >= 2 && k <= nz - 2;
+    if (sp.scheme == QUICK && qok) {
+        q[0] = w[e - si];  // adjacent (:275)
+        q[1] = w[e - sj];  // adjacent (:276)
+        q[2] = w[e - 2];
+    }
+    R val = face_update<R>(sp.scheme, qok, nb, q, c, vel, h, i2, (R)sp.dt, (R)sp.nu);
+    if (sp.buoyancy) {  // (:373-386), interior z-faces, before the BC masks
+        const uint32_t cc = cidx(g, i, j, k);
+        const R lo = T[cc - 1], hi = T[cc];
+        val += (R)sp.buoy_coeff * (R(0.5) * (lo + hi) - (R)sp.T_ref);
+    }
+    return val;
+}
+
+template <typename R>
+__global__ void __launch_bounds__(kBlock) k_predict(Geo g, StepParams sp, const R *u, const R *v,
+                                                    const R *w, const R *T, R *us, R *vs, R *ws,
+                                                    StepRec *rec)
+{
+    if (blockIdx.x == 0 && threadIdx.x == 0) {  // start-of-step diagnostics reset
+        rec->vmax_bits = 0ull;
+        rec->divmax_bits = 0ull;
+        rec->tsum = 0.0;
+        rec->nonfinite = 0;
+    }
+    const Axes<R> A = axes_of(g, R(0));
+    uint32_t b, end;
+    block_span(g.nu + g.nv + g.nw, xcd_block(), b, end);
+    for (uint32_t e0 = b + threadIdx.x; e0 < end; e0 += kBlock) {
+        if (e0 < g.nu) {
+            const uint32_t e = e0;
+            const uint32_t i = fdiv(e, g.f_nynz), rem = e - i * (g.ny * g.nz);
+            const uint32_t j = fdiv(rem, g.f_nz), k = rem - j * g.nz;
+            R val = predict_u<R>(g, sp, A, u, v, w, i, j, k, e);
+            us[e] = mask_face<R>(g, 0, i, j, k, val);
+        } else if (e0 < g.nu + g.nv) {
+            const uint32_t e = e0 - g.nu;
+            const uint32_t i = fdiv(e, g.f_ny1nz), rem = e - i * ((g.ny + 1) * g.nz);
+            const uint32_t j = fdiv(rem, g.f_nz), k = rem - j * g.nz;
+            R val = predict_v<R>(g, sp, A, u, v, w, i, j, k, e);
+            vs[e] = mask_face<R>(g, 1, i, j, k, val);
+        } else {
+            const uint32_t e = e0 - g.nu - g.nv;
+            const uint32_t i = fdiv(e, g.f_nynz1), rem = e - i * (g.ny * (g.nz + 1));
+            const uint32_t j = fdiv(rem, g.f_nz1), k = rem - j * (g.nz + 1);
+            R val = predict_w<R>(g, sp, A, u, v, w, T, i, j, k, e);
+            ws[e] = mask_face<R>(g, 2, i, j, k, val);
+        }
+    }
+}
+
+template <typename R>
+void launch_predict(hipStream_t s, const Geo &g, const StepParams &sp, const R *u, const R *v,
+                    const R *w, const R *T, R *us, R *vs, R *ws, StepRec *rec)
+{
+    const uint64_t n = (uint64_t)g.nu + g.nv + g.nw;
+    hipLaunchKernelGGL(k_predict<R>, dim3(grid_for(n)), dim3(kBlock), 0, s, g, sp, u, v, w, T,
+                       us, vs, ws, rec);
+}
+
+// ---- K4: Poisson right-hand side ----------------------------------------
+template <typename R>
+__global__ void __launch_bounds__(kBlock) k_rhs(Geo g, StepParams sp, const R *us, const R *vs,
+                                                const R *ws, double *b)
+{
+    uint32_t sb, se;
+    block_span(g.nc, xcd_block(), sb, se);
+    for (uint32_t c = sb + threadIdx.x; c < se; c += kBlock) {
+        if (g.flags[c] == SOLID) {
+            b[c] = 0.0;
+            continue;
+        }
+        const uint32_t i = fdiv(c, g.f_nynz), rem = c - i * (g.ny * g.nz);
+        const uint32_t j = fdiv(rem, g.f_nz), k = rem - j * g.nz;
+        const double du = (double)us[c + g.ny * g.nz] - (double)us[c];
+        const uint32_t ev = vidx(g, i, j, k), ew = widx(g, i, j, k);
+        const double dv = (double)vs[ev + g.nz] - (double)vs[ev];
+        const double dw = (double)ws[ew + 1] - (double)ws[ew];
+        const double div = du / g.hx[i] + dv / g.hy[j] + dw / g.hz[k];
+        b[c] = sp.rhs_factor * div;
+    }
+}
+
+template <typename R>
+void launch_rhs(hipStream_t s, const Geo &g, const StepParams &sp, const R *us, const R *vs,
+                const R *ws, double *b)
+{
+    hipLaunchKernelGGL(k_rhs<R>, dim3(grid_for(g.nc)), dim3(kBlock), 0, s, g, sp, us, vs, ws, b);
+}
+
+// ---- K6: projection + post-projection masks + velocity change ----------
+__device__ __forceinline__ double pgrad(const Geo &g, const double *p, int axis, int i, int j,
+                                        int k)
+{
+    const uint32_t c = cidx(g, i, j, k);
+    if (axis == 0) return (p[c] - p[c - g.ny * g.nz]) / g.cx[i];
+    if (axis == 1) return (p[c] - p[c - g.nz]) / g.cy[j];
+    return (p[c] - p[c - 1]) / g.cz[k];
+}
+
+// Value of a face after correction, no-slip and fans (before outflow copy).
+template <typename R>
+__device__ __forceinline__ R corrected(const Geo &g, const StepParams &sp, const R *fs,
+                                       const double *p, int axis, int i, int j, int k, uint32_t e)
+{
+    R x = fs[e];
+    const int n_ax = axis == 0 ? g.nx : (axis == 1 ? g.ny : g.nz);
+    const int pos = axis == 0 ? i : (axis == 1 ? j : k);
+    if (pos >= 1 && pos <= n_ax - 1) {
+        const double gp = pgrad(g, p, axis, i, j, k);
+        x = x - (R)(sp.dt_rho * gp);
+    }
+    return mask_face<R>(g, axis, i, j, k, x);
+}
+
+// fp64: x -= dt_rho*gp exactly as the reference (:345); the generic version
+// rounds dt_rho*gp to R first, which for R = double is the same operation.
+
+template <typename R>
+__global__ void __launch_bounds__(kBlock) k_correct(Geo g, StepParams sp, const R *us,
+                                                    const R *vs, const R *ws, const double *p,
+                                                    R *u, R *v, R *w, StepRec *rec)
+{
+    __shared__ double lds[kBlock / kWave];
+    double dv = 0.0;
+    bool bad = false;
+    uint32_t sb, se;
+    block_span(g.nu + g.nv + g.nw, xcd_block(), sb, se);
+    for (uint32_t e0 = sb + threadIdx.x; e0 < se; e0 += kBlock) {
+        R val, old;
+        if (e0 < g.nu) {
+            const uint32_t e = e0;
+            const uint32_t i = fdiv(e, g.f_nynz), rem = e - i * (g.ny * g.nz);
+            const uint32_t j = fdiv(rem, g.f_nz), k = rem - j * g.nz;
+            if (i == 0 && g.wall_of[wall_x(g, 0, j, k)])
+                val = corrected<R>(g, sp, us, p, 0, 1, j, k, uidx(g, 1, j, k));
+            else if (i == (uint32_t)g.nx && g.wall_of[wall_x(g, 1, j, k)])
+                val = corrected<R>(g, sp, us, p, 0, g.nx - 1, j, k, uidx(g, g.nx - 1, j, k));
+            else
+                val = corrected<R>(g, sp, us, p, 0, i, j, k, e);
+            old = u[e];
+            u[e] = val;
+        } else if (e0 < g.nu + g.nv) {
+            const uint32_t e = e0 - g.nu;
+            const uint32_t i = fdiv(e, g.f_ny1nz), rem = e - i * ((g.ny + 1) * g.nz);
+            const uint32_t j = fdiv(rem, g.f_nz), k = rem - j * g.nz;
+            if (j == 0 && g.wall_of[wall_y(g, 0, i, k)])
+                val = corrected<R>(g, sp, vs, p, 1, i, 1, k, vidx(g, i, 1, k));
+            else if (j == (uint32_t)g.ny && g.wall_of[wall_y(g, 1, i, k)])
+                val = corrected<R>(g, sp, vs, p, 1, i, g.ny - 1, k, vidx(g, i, g.ny - 1, k));
+            else
+                val = corrected<R>(g, sp, vs, p, 1, i, j, k, e);
+            old = v[e];
+            v[e] = val;
+        } else {
+            const uint32_t e = e0 - g.nu - g.nv;
+            const uint32_t i = fdiv(e, g.f_nynz1), rem = e - i * (g.ny * (g.nz + 1));
+            const uint32_t j = fdiv(rem, g.f_nz1), k = rem - j * (g.nz + 1);
+            if (k == 0 && g.wall_of[wall_z(g, 0, i, j)])
+                val = corrected<R>(g, sp, ws, p, 2, i, j, 1, widx(g, i, j, 1));
+            else if (k == (uint32_t)g.nz && g.wall_of[wall_z(g, 1, i, j)])
+                val = corrected<R>(g, sp, ws, p, 2, i, j, g.nz - 1, widx(g, i, j, g.nz - 1));
+            else
+                val = corrected<R>(g, sp, ws, p, 2, i, j, k, e);
+            old = w[e];
+            w[e] = val;
+        }
+        dv = fmax(dv, fabs((double)val - (double)old));  // fmax drops NaN as std::max does
+        bad |= !isfinite((double)val);
+    }
+    if (bad) rec->nonfinite = 1;
+    const double m = block_max(dv, lds);
+    if (threadIdx.x == 0 && m > 0.0) atomic_max_nonneg(&rec->vmax_bits, m);
+}
+
+template <typename R>
+void launch_correct(hipStream_t s, const Geo &g, const StepParams &sp, const R *us, const R *vs,
+                    const R *ws, const double *p, R *u, R *v, R *w, StepRec *rec)
+{
+    const uint64_t n = (uint64_t)g.nu + g.nv + g.nw;
+    hipLaunchKernelGGL(k_correct<R>, dim3(grid_for(n)), dim3(kBlock), 0, s, g, sp, us, vs, ws, p,
+                       u, v, w, rec);
+}
+
+// ---- divergence (FluidSolver.cpp:470-484) --------------------------------
+template <typename R>
+__device__ __forceinline__ double cell_div(const Geo &g, const R *u, const R *v, const R *w,
+                                           int i, int j, int k, uint32_t c)
+{
+    const uint32_t ev = vidx(g, i, j, k), ew = widx(g, i, j, k);
+    return fabs(((double)u[c + g.ny * g.nz] - (double)u[c]) / g.hx[i] +
+                ((double)v[ev + g.nz] - (double)v[ev]) / g.hy[j] +
+                ((double)w[ew + 1] - (double)w[ew]) / g.hz[k]);
+}
+
+template <typename R>
+__global__ void __launch_bounds__(kBlock) k_divmax(Geo g, const R *u, const R *v, const R *w,
+                                                   StepRec *rec)
+{
+    __shared__ double lds[kBlock / kWave];
+    double d = 0.0;
+    uint32_t sb, se;
+    block_span(g.nc, xcd_block(), sb, se);
+    for (uint32_t c = sb + threadIdx.x; c < se; c += kBlock) {
+        if (g.flags[c] != FLUID) continue;
+        const uint32_t i = fdiv(c, g.f_nynz), rem = c - i * (g.ny * g.nz);
+        const uint32_t j = fdiv(rem, g.f_nz), k = rem - j * g.nz;
+        d = fmax(d, cell_div(g, u, v, w, i, j, k, c));  // drops NaN like std::max
+    }
+    const double m = block_max(d, lds);
+    if (threadIdx.x == 0 && m > 0.0) atomic_max_nonneg(&rec->divmax_bits, m);
+}
+
+template <typename R>
+void launch_divmax(hipStream_t s, const Geo &g, const R *u, const R *v, const R *w, StepRec *rec)
+{
+    hipLaunchKernelGGL(k_divmax<R>, dim3(grid_for(g.nc)), dim3(kBlock), 0, s, g, u, v, w, rec);
+}
+
+// ---- K8: temperature transport (FluidSolver.cpp:388-456) ---------------
+template <typename R>
+__global__ void __launch_bounds__(kBlock) k_temperature(Geo g, StepParams sp, const R *u,
+                                                        const R *v, const R *w, const R *T,
+                                                        R *Tn, StepRec *rec, RedScratch red)
+{
+    __shared__ double lds[kBlock / kWave];
+    const uint32_t blk = xcd_block();
+    double dmax = 0.0, tsum = 0.0;
+    bool bad = false;
+    const int nx = g.nx, ny = g.ny, nz = g.nz;
+    const Axes<R> A = axes_of(g, R(0));
+    uint32_t sb, se;
+    block_span(g.nc, blk, sb, se);
+    for (uint32_t c = sb + threadIdx.x; c < se; c += kBlock) {
+        if (g.flags[c] != FLUID) {  // non-FLUID cells are frozen (:402-405)
+            Tn[c] = T[c];
+            continue;
+        }
+        const uint32_t i = fdiv(c, g.f_nynz), rem = c - i * (ny * nz);
+        const uint32_t j = fdiv(rem, g.f_nz), k = rem - j * nz;
+        const R hx = A.hx[i], hy = A.hy[j], hz = A.hz[k];
+        const R Tc = T[c];
+        const uint32_t ev = vidx(g, i, j, k), ew = widx(g, i, j, k);
+        const R u0 = u[c], u1 = u[c + ny * nz], v0 = v[ev], v1 = v[ev + nz], w0 = w[ew],
+                w1 = w[ew + 1];
+        const R uc = R(0.5) * (u0 + u1), vc = R(0.5) * (v0 + v1), wc = R(0.5) * (w0 + w1);
+        // neighbour T: Neumann (Tc) outside the domain and at SOLID (:417-422)
+        auto tn = [&](int a, int bb, int d) -> R {
+            if (a < 0 || a >= nx || bb < 0 || bb >= ny || d < 0 || d >= nz) return Tc;
+            const uint32_t q = cidx(g, a, bb, d);
+            return g.flags[q] == SOLID ? Tc : T[q];
+        };
+        const int ii = i, jj = j, kk = k;
+        const R xm = tn(ii - 1, jj, kk), xp = tn(ii + 1, jj, kk);
+        const R ym = tn(ii, jj - 1, kk), yp = tn(ii, jj + 1, kk);
+        const R zm = tn(ii, jj, kk - 1), zp = tn(ii, jj, kk + 1);
+        const R dt = (R)sp.dt;
+        R adv;
+        if (sp.scheme == QUICK && ii >= 1 && ii <= nx - 2 && jj >= 1 && jj <= ny - 2 && kk >= 1 &&
+            kk <= nz - 2) {
+            const R xmm = tn(ii - 2, jj, kk), ymm = tn(ii, jj - 2, kk), zmm = tn(ii, jj, kk - 2);
+            adv = adv_quick(xmm, xm, Tc, xp, uc, hx) + adv_quick(ymm, ym, Tc, yp, vc, hy) +
+                  adv_quick(zmm, zm, Tc, zp, wc, hz);
+        } else if (sp.scheme == LAX_WENDROFF) {
+            adv = adv_lw(xm, Tc, xp, uc, dt, hx) + adv_lw(ym, Tc, yp, vc, dt, hy) +
+                  adv_lw(zm, Tc, zp, wc, dt, hz);
+        } else {
+            adv = adv_upwind(xm, Tc, xp, uc, hx) + adv_upwind(ym, Tc, yp, vc, hy) +
+                  adv_upwind(zm, Tc, zp, wc, hz);
+        }
+        const R al = (R)sp.alpha_T;
+        const R dif = al * ((xm - R(2) * Tc + xp) / (hx * hx) + (ym - R(2) * Tc + yp) / (hy * hy) +
+                            (zm - R(2) * Tc + zp) / (hz * hz));
+        const R tnew = Tc + dt * (-adv + dif);
+        Tn[c] = tnew;
+        tsum += (double)tnew;
+        bad |= !isfinite((double)tnew);
+        dmax = fmax(dmax, fabs(((double)u1 - (double)u0) / g.hx[i] +
+                               ((double)v1 - (double)v0) / g.hy[j] +
+                               ((double)w1 - (double)w0) / g.hz[k]));
+    }
+    if (bad) rec->nonfinite = 1;
+    const double m = block_max(dmax, lds);
+    if (threadIdx.x == 0 && m > 0.0) atomic_max_nonneg(&rec->divmax_bits, m);
+    double total;
+    if (grid_sum_last(tsum, red.partials, &red.tickets[0], blk, lds, &total)) rec->tsum = total;
+}
+
+template <typename R>
+void launch_temperature(hipStream_t s, const Geo &g, const StepParams &sp, const R *u,
+                        const R *v, const R *w, const R *T, R *Tn, StepRec *rec, RedScratch red)
+{
+    hipLaunchKernelGGL(k_temperature<R>, dim3(grid_for(g.nc)), dim3(kBlock), 0, s, g, sp, u, v,
+                       w, T, Tn, rec, red);
+}
+
+// Sum of T over FLUID cells (used when an opening overwrites FLUID cells).
+template <typename R>
+__global__ void __launch_bounds__(kBlock) k_tsum(Geo g, const R *T, StepRec *rec, RedScratch red)
+{
+    __shared__ double lds[kBlock / kWave];
+    const uint32_t blk = xcd_block();
+    double t = 0.0;
+    uint32_t sb, se;
+    block_span(g.nc, blk, sb, se);
+    for (uint32_t c = sb + threadIdx.x; c < se; c += kBlock)
+        if (g.flags[c] == FLUID) t += (double)T[c];
+    double total;
+    if (grid_sum_last(t, red.partials, &red.tickets[0], blk, lds, &total)) rec->tsum = total;
+}
+
+template <typename R>
+void launch_tsum(hipStream_t s, const Geo &g, const R *T, StepRec *rec, RedScratch red)
+{
+    hipLaunchKernelGGL(k_tsum<R>, dim3(grid_for(g.nc)), dim3(kBlock), 0, s, g, T, rec, red);
+}
+
+// ---- opening temperatures (BoundaryManager.cpp:137-178) -----------------
+// One block walks the openings in registration order, so overlapping
+// patches resolve exactly as the reference's sequential loop does.
+template <typename R>
+__global__ void __launch_bounds__(kBlock) k_opening_T(Geo g, const R *u, const R *v, const R *w,
+                                                      R *T)
+{
+    const int nx = g.nx, ny = g.ny, nz = g.nz;
+    for (int n = 0; n < g.n_openings; n++) {
+        const OpeningDev o = g.openings[n];
+        const int na = o.a1 - o.a0 + 1, nb = o.b1 - o.b0 + 1;
+        const int cnt = na * nb;
+        for (int t = threadIdx.x; t < cnt; t += blockDim.x) {
+            const int a = o.a0 + t / nb, b = o.b0 + t % nb;
+            double vf;
+            uint32_t wall, inner;
+            const int sd = o.side;
+            if (o.axis == 0) {
+                vf = (double)u[uidx(g, sd ? nx : 0, a, b)];
+                wall = cidx(g, sd ? nx - 1 : 0, a, b);
+                inner = cidx(g, sd ? nx - 2 : 1, a, b);
+            } else if (o.axis == 1) {
+                vf = (double)v[vidx(g, a, sd ? ny : 0, b)];
+                wall = cidx(g, a, sd ? ny - 1 : 0, b);
+                inner = cidx(g, a, sd ? ny - 2 : 1, b);
+            } else {
+                vf = (double)w[widx(g, a, b, sd ? nz : 0)];
+                wall = cidx(g, a, b, sd ? nz - 1 : 0);
+                inner = cidx(g, a, b, sd ? nz - 2 : 1);
+            }
+            const bool incoming = sd ? (vf <= 0.0) : (vf >= 0.0);
+            T[wall] = incoming ? (R)o.T_out : T[inner];
+        }
+        __syncthreads();
+    }
+}
+
+template <typename R>
+void launch_opening_T(hipStream_t s, const Geo &g, const R *u, const R *v, const R *w, R *T)
+{
+    if (g.n_openings == 0) return;
+    hipLaunchKernelGGL(k_opening_T<R>, dim3(1), dim3(kBlock), 0, s, g, u, v, w, T);
+}
+
+// ---- precision conversion ------------------------------------------------
+template <typename R>
+__global__ void k_convert(const double *src, R *dst, uint64_t n)
+{
+    for (uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x; i < n;
+         i += (uint64_t)gridDim.x * kBlock)
+        dst[i] = (R)src[i];
+}
+template <typename R>
+__global__ void k_convert_back(const R *src, double *dst, uint64_t n)
+{
+    for (uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x; i < n;
+         i += (uint64_t)gridDim.x * kBlock)
+        dst[i] = (double)src[i];
+}
+template <typename R>
+void launch_convert(hipStream_t s, const double *src, R *dst, uint64_t n)
+{
+    if (n) hipLaunchKernelGGL(k_convert<R>, dim3(grid_for(n)), dim3(kBlock), 0, s, src, dst, n);
+}
+template <typename R>
+void launch_convert_back(hipStream_t s, const R *src, double *dst, uint64_t n)
+{
+    if (n) hipLaunchKernelGGL(k_convert_back<R>, dim3(grid_for(n)), dim3(kBlock), 0, s, src, dst, n);
+}
+
+#define FFR_INSTANTIATE(R)                                                                      \
+    template void launch_predict<R>(hipStream_t, const Geo &, const StepParams &, const R *,   \
+                                    const R *, const R *, const R *, R *, R *, R *, StepRec *); \
+    template void launch_correct<R>(hipStream_t, const Geo &, const StepParams &, const R *,   \
+                                    const R *, const R *, const double *, R *, R *, R *,       \
+                                    StepRec *);                                                \
+    template void launch_divmax<R>(hipStream_t, const Geo &, const R *, const R *, const R *,  \
+                                   StepRec *);                                                 \
+    template void launch_temperature<R>(hipStream_t, const Geo &, const StepParams &,          \
+                                        const R *, const R *, const R *, const R *, R *,       \
+                                        StepRec *, RedScratch);                                \
+    template void launch_tsum<R>(hipStream_t, const Geo &, const R *, StepRec *, RedScratch);  \
+    template void launch_opening_T<R>(hipStream_t, const Geo &, const R *, const R *,          \
+                                      const R *, R *);                                         \
+    template void launch_rhs<R>(hipStream_t, const Geo &, const StepParams &, const R *,       \
+                                const R *, const R *, double *);                               \
+    template void launch_convert<R>(hipStream_t, const double *, R *, uint64_t);               \
+    template void launch_convert_back<R>(hipStream_t, const R *, double *, uint64_t);
+
+FFR_INSTANTIATE(double)
+FFR_INSTANTIATE(float)
+
+}  // namespace ffr

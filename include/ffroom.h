@@ -1,0 +1,215 @@
+/* ffroom.h - C ABI of the MI355X (gfx950) projection-step library.
+ *
+ * The host C++ classes (Grid, BoundaryManager, PoissonSolver,
+ * MultigridPoissonSolver, FluidSolver) and the pybind11 module
+ * `_ffroom_core` are thin shims over these calls; a foreign host (ctypes,
+ * cgo, JNI) can bind them directly. Plain C types only: no C++ objects, no
+ * torch tensors and no exceptions cross this boundary. Every function
+ * returns FFR_OK (0) or a negative ffr_status; ffr_last_error() gives the
+ * message of the last failure on a context.
+ *
+ * Host buffers use the reference layout (ddd3h/ff-room cpp/src/core/Grid.hpp:
+ * 57-65, k fastest): u (nx+1,ny,nz), v (nx,ny+1,nz), w (nx,ny,nz+1),
+ * p/T/cell_type (nx,ny,nz); fields are double, cell_type is uint8.
+ *
+ * Threading: a context and its scenes are used from one host thread at a time.
+ */
+#ifndef FFROOM_H
+#define FFROOM_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FFR_ABI_VERSION 1
+
+typedef struct ffr_ctx ffr_ctx;
+typedef struct ffr_scene ffr_scene;
+
+typedef enum {
+    FFR_OK = 0,
+    FFR_E_INVALID = -1,     /* bad argument or descriptor */
+    FFR_E_DEVICE = -2,      /* HIP runtime error */
+    FFR_E_NOMEM = -3,       /* device allocation or HBM budget exceeded */
+    FFR_E_STATE = -4,       /* call not valid in the object's state */
+    FFR_E_NUMERIC = -5,     /* non-finite values detected */
+    FFR_E_UNSUPPORTED = -6, /* option not available for this scene */
+    FFR_E_NOT_FOUND = -7    /* unknown ticket */
+} ffr_status;
+
+/* CellType, Grid.hpp:9-14 */
+enum { FFR_FLUID = 0, FFR_SOLID = 1, FFR_INFLOW = 2, FFR_OUTFLOW = 3 };
+/* AdvectionScheme, FluidSolver.hpp:9 */
+enum { FFR_UPWIND1 = 0, FFR_QUICK = 1, FFR_LAX_WENDROFF = 2 };
+/* Arithmetic of the state and stencils. */
+enum {
+    FFR_PREC_FP64 = 0,  /* every field and kernel in FP64: the reference algorithm */
+    FFR_PREC_MIXED = 1  /* FP32 fields/stencils, FP32+BF16 multigrid, FP64 outer residual */
+};
+/* Pressure solver. */
+enum {
+    FFR_POISSON_AUTO = 0,  /* reference choice: CG, or MG-PCG when use_multigrid */
+    FFR_POISSON_CG = 1,    /* unpreconditioned CG, PoissonSolver.cpp:93-158 */
+    FFR_POISSON_MGPCG = 2  /* geometric-multigrid-preconditioned CG */
+};
+
+/* FanBC, BoundaryManager.hpp:8-13 */
+typedef struct {
+    int i_min, i_max, j_min, j_max, k_min, k_max;
+    double vel[3];
+} ffr_fan;
+
+/* OutflowBC, BoundaryManager.hpp:17-22 */
+typedef struct {
+    int axis, side, a_min, a_max, b_min, b_max;
+} ffr_outflow;
+
+/* OpeningBC, BoundaryManager.hpp:27-29 */
+typedef struct {
+    int axis, side, a_min, a_max, b_min, b_max;
+    double T_outside;
+} ffr_opening;
+
+/* Geometry and boundary lists of one scene (what Grid + BoundaryManager
+ * hold). `outflows` lists every registered outflow patch, openings included,
+ * as BoundaryManager::outflows() does (BoundaryManager.cpp:24-44, 132-135). */
+typedef struct {
+    int nx, ny, nz;
+    const double *dxv, *dyv, *dzv;  /* cell widths, sizes nx, ny, nz */
+    const uint8_t *cell_type;       /* nx*ny*nz */
+    const ffr_fan *fans;
+    int n_fans;
+    const ffr_outflow *outflows;
+    int n_outflows;
+    const ffr_opening *openings;
+    int n_openings;
+} ffr_scene_desc;
+
+/* FluidSolverParams (FluidSolver.hpp:11-38) + PoissonSolverParams
+ * (PoissonSolver.hpp:14-18), then optional extensions. ffr_params_default()
+ * gives the reference defaults and the reference algorithm. */
+typedef struct {
+    double rho, nu, dt;
+    int max_steps;
+    double convergence_tol;
+    int poisson_max_iter;
+    double poisson_tol;
+    int advection;
+    int use_openmp;    /* accepted for compatibility; no effect on the GPU */
+    int use_multigrid; /* pressure at OUTFLOW cells pinned to 0 (PoissonSolver.cpp:201) */
+    int thermal;
+    double T_initial, T_target;
+    int buoyancy;
+    double g_accel, beta, k_thermal, cp;
+    /* extensions */
+    int precision;       /* FFR_PREC_* */
+    int poisson_method;  /* FFR_POISSON_* */
+    int mg_levels;       /* multigrid levels incl. the finest; 0 = automatic (<= 5) */
+    int mg_bf16;         /* store multigrid levels >= 1 in bfloat16 */
+    int mg_pre_sweeps;   /* red-black Gauss-Seidel sweeps per side of the V-cycle */
+    int mg_coarse_sweeps;/* symmetric sweeps on the coarsest level */
+    double inner_tol;    /* mixed: relative residual target of each inner FP32 solve */
+    int inner_max_iter;  /* mixed: iteration cap of each inner solve */
+} ffr_params;
+
+/* StepResult, FluidSolver.hpp:40-47, plus solver statistics. */
+typedef struct {
+    int step;
+    double velocity_change;   /* max |u^{n+1}-u^n| / dt over all faces */
+    double pressure_residual; /* relative residual of the pressure solve */
+    double divergence_max;    /* max |div u| over FLUID cells */
+    int converged;
+    double T_mean;
+    int poisson_iters;        /* CG / inner PCG iterations of this step */
+    int outer_iters;          /* FP64 defect-correction passes (mixed) */
+    int nonfinite;            /* 1 when a NaN/Inf appeared in u, v, w or T */
+} ffr_step_result;
+
+typedef struct {
+    double residual; /* relative residual ||b-Ax|| / ||b|| (SOLID rows excluded) */
+    int iterations;
+    int outer_iterations;
+} ffr_solve_info;
+
+typedef struct {
+    char name[64];
+    char arch[32];
+    int compute_units;
+    int xcds;
+    size_t hbm_bytes;
+    size_t l2_bytes;
+    size_t lds_bytes_per_cu;
+    int clock_khz;
+} ffr_device_info;
+
+/* Per-kernel timing gathered with HIP events on the launching stream. */
+typedef struct {
+    char name[48];
+    long launches;
+    double total_ms;
+    double bytes_per_launch; /* algorithmic bytes of the last launch */
+} ffr_prof_entry;
+
+typedef void (*ffr_step_callback)(const ffr_step_result *, void *user);
+
+/* ---- library / context ------------------------------------------------ */
+int ffr_abi_version(void);
+const char *ffr_status_string(int status);
+void ffr_params_default(ffr_params *p);
+int ffr_ctx_create(int device, size_t hbm_budget_bytes, ffr_ctx **out);
+void ffr_ctx_destroy(ffr_ctx *ctx);
+const char *ffr_last_error(const ffr_ctx *ctx);
+int ffr_device_query(ffr_ctx *ctx, ffr_device_info *out);
+size_t ffr_ctx_bytes_in_use(const ffr_ctx *ctx);
+
+/* ---- scenes ------------------------------------------------------------ */
+/* Device bytes a scene with this geometry and these parameters occupies. */
+size_t ffr_scene_footprint(const ffr_scene_desc *desc, const ffr_params *params);
+/* Copies the geometry to the device and allocates the state (zero fields). */
+int ffr_scene_create(ffr_ctx *ctx, const ffr_scene_desc *desc, const ffr_params *params,
+                     ffr_scene **out);
+void ffr_scene_destroy(ffr_scene *scene);
+int ffr_scene_set_params(ffr_scene *scene, const ffr_params *params);
+/* Host <-> device state copies; a NULL pointer leaves that field alone. */
+int ffr_scene_upload(ffr_scene *scene, const double *u, const double *v, const double *w,
+                     const double *p, const double *T);
+int ffr_scene_download(ffr_scene *scene, double *u, double *v, double *w, double *p,
+                       double *T);
+int ffr_scene_step_count(const ffr_scene *scene);
+int ffr_scene_set_step_count(ffr_scene *scene, int step);
+
+/* ---- the projection step (FluidSolver.cpp:486-560) --------------------- */
+int ffr_step(ffr_scene *scene, ffr_step_result *out);
+int ffr_run(ffr_scene *scene, int max_steps, ffr_step_callback cb, void *user,
+            ffr_step_result *last);
+int ffr_divergence_max(ffr_scene *scene, double *out); /* FluidSolver.cpp:470-484 */
+int ffr_T_mean(ffr_scene *scene, double *out);         /* FluidSolver.cpp:458-468 */
+
+/* ---- pressure operator and solve (PoissonSolver.cpp:22-158) ------------ */
+/* Ax = A x for the scene's flags and spacings (host buffers, nx*ny*nz). */
+int ffr_apply_laplacian(ffr_scene *scene, const double *x, double *Ax);
+/* Solves A p = rhs with the scene's poisson settings; p is written back. */
+int ffr_poisson_solve(ffr_scene *scene, const double *rhs, double *p, ffr_solve_info *info);
+
+/* ---- batching and serving ---------------------------------------------- */
+/* Queue `max_steps` steps (stopping early when velocity_change < conv_tol,
+ * or on the thermal target) of a scene; steps of all queued scenes overlap
+ * on the device. */
+int ffr_submit(ffr_ctx *ctx, ffr_scene *scene, int max_steps, uint64_t *ticket);
+/* Advances queued work; *done is set when the ticket's scene finished. */
+int ffr_poll(ffr_ctx *ctx, uint64_t ticket, ffr_step_result *last, int *done);
+/* Runs every queued ticket to completion. */
+int ffr_drain(ffr_ctx *ctx);
+
+/* ---- profiling ---------------------------------------------------------- */
+int ffr_prof_enable(ffr_ctx *ctx, int on);
+int ffr_prof_reset(ffr_ctx *ctx);
+int ffr_prof_read(ffr_ctx *ctx, ffr_prof_entry *out, int cap, int *n_out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
