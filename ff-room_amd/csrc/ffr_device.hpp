@@ -179,38 +179,31 @@ __device__ __forceinline__ void atomic_max_nonneg(unsigned long long *slot, doub
     atomicMax(slot, (unsigned long long)__double_as_longlong(v));
 }
 
-// Deterministic grid sum: every block stores its partial, the last block to
-// arrive (atomic ticket) adds all partials in block order. Returns true in
-// thread 0 of that last block with *total set. The ticket is reset there.
-__device__ __forceinline__ bool grid_sum_last(double v, double *partials, unsigned int *ticket,
-                                              uint32_t blk, double *lds, double *total)
+// Grid reductions without inter-block synchronisation: every block stores
+// its partial (block_partial), and the consumer - the next kernel on the
+// stream, where the kernel boundary makes the partials visible - re-adds all
+// of them in one fixed order (sum_partials), so every block of the consumer
+// obtains the same bits. (A last-block-done scheme would need a device-scope
+// fence per block, which on gfx950 writes back and invalidates the L2.)
+__device__ __forceinline__ void block_partial(double v, double *partials, uint32_t blk,
+                                              double *lds)
 {
-    double s = block_sum(v, lds);
-    __shared__ bool last;
-    if (threadIdx.x == 0) {
-        partials[blk] = s;
-        __threadfence();
-        unsigned int t = atomicAdd(ticket, 1u);
-        last = (t == gridDim.x - 1);
-    }
-    __syncthreads();
-    if (!last) return false;
-    __threadfence();
-    // fixed-order reduction of gridDim.x partials by the whole block: thread t
-    // adds partials t, t+256, ... then the 256 runs are added in thread order
-    const uint32_t G = gridDim.x;
+    const double s = block_sum(v, lds);
+    if (threadIdx.x == 0) partials[blk] = s;
+}
+
+// Sum of partials[0..G) in a fixed order, returned to every thread.
+__device__ __forceinline__ double sum_partials(const double *partials, uint32_t G, double *lds)
+{
     double acc = 0.0;
-    for (uint32_t idx = threadIdx.x; idx < G; idx += blockDim.x) acc += partials[idx];
-    __shared__ double runs[kBlock];
-    runs[threadIdx.x] = acc;
+    for (uint32_t i = threadIdx.x; i < G; i += blockDim.x) acc += partials[i];
+    const double s = block_sum(acc, lds);
+    __shared__ double bcast;
+    if (threadIdx.x == 0) bcast = s;
     __syncthreads();
-    if (threadIdx.x == 0) {
-        double t = 0.0;
-        for (uint32_t q = 0; q < blockDim.x; q++) t += runs[q];
-        *total = t;
-        *ticket = 0u;
-    }
-    return threadIdx.x == 0;
+    const double r = bcast;
+    __syncthreads();
+    return r;
 }
 
 }  // namespace ffr

@@ -22,17 +22,24 @@ struct StepRec {
 };
 
 // State of the device-resident reference CG (PoissonSolver.cpp:93-158).
+// rr is double-buffered by iteration parity: the kernel of iteration `it`
+// reads rr[it & 1] while its block 0 writes rr[(it + 1) & 1].
 struct CgState {
-    double rr, bnorm, alpha, beta, res, dAd;
-    int iter, done, iters, max_iter;
+    double rr[2];
+    double bnorm, res;
+    int done, iters, max_iter, pad;
     double tol;
 };
 
-// Reduction scratch of one stream: block partials and arrival tickets.
+// Reduction scratch of one stream: partial sums, kSlots independent slots of
+// kMaxBlocksCap doubles (a producer kernel writes one slot, its consumer
+// reads it).
 struct RedScratch {
-    double *partials;       // >= max grid size
-    unsigned int *tickets;  // a few independent counters, zero at rest
+    double *partials;
+    unsigned int *tickets;  // unused counters kept zero
 };
+constexpr int kSlots = 6;
+enum { SLOT_INIT = 0, SLOT_SPMV = 1, SLOT_UPDATE = 2, SLOT_TSUM = 3, SLOT_MG_A = 4, SLOT_MG_B = 5 };
 
 // Upper bound on blocks per launch: one resident wave of 256-thread blocks
 // over 256 CUs at 8 blocks each; larger arrays loop inside each block over a
@@ -63,6 +70,8 @@ void launch_temperature(hipStream_t s, const Geo &g, const StepParams &sp, const
                         RedScratch red);
 template <typename R>
 void launch_tsum(hipStream_t s, const Geo &g, const R *T, StepRec *rec, RedScratch red);
+// rec->tsum := sum of the SLOT_TSUM partials written by temperature / tsum.
+void launch_step_finalize(hipStream_t s, const Geo &g, StepRec *rec, RedScratch red);
 template <typename R>
 void launch_opening_T(hipStream_t s, const Geo &g, const R *u, const R *v, const R *w, R *T);
 template <typename R>
@@ -78,14 +87,13 @@ void launch_convert_back(hipStream_t s, const R *src, double *dst, uint64_t n);
 // ||b|| < 1e-15 or max_iter == 0.
 void launch_cg_init(hipStream_t s, const Geo &g, const double *b, double *x, double *r,
                     double *d, CgState *cg, RedScratch red, int max_iter, double tol);
-// One CG iteration: Ad = A d (+ d.Ad), x/r update (+ r.r), d = r + beta d.
-void launch_cg_iteration(hipStream_t s, const Geo &g, double *x, double *r, double *d,
-                         double *Ad, CgState *cg, RedScratch red);
+// Iteration `it` of CG: Ad = A d (+ d.Ad), x/r update (+ r.r), d = r + beta d.
 void launch_cg_spmv(hipStream_t s, const Geo &g, const double *d, double *Ad, CgState *cg,
                     RedScratch red);
 void launch_cg_update(hipStream_t s, uint32_t n, double *x, double *r, const double *d,
-                      const double *Ad, CgState *cg, RedScratch red);
-void launch_cg_dir(hipStream_t s, uint32_t n, double *d, const double *r, CgState *cg);
+                      const double *Ad, CgState *cg, RedScratch red, int it);
+void launch_cg_dir(hipStream_t s, uint32_t n, double *d, const double *r, CgState *cg,
+                   RedScratch red, int it);
 // Plain A*x (PoissonSolver.cpp:22-91) in FP64.
 void launch_laplacian(hipStream_t s, const Geo &g, const double *x, double *Ax);
 

@@ -1,15 +1,295 @@
-// ffr_mg.hip - multigrid-preconditioned CG (see ffr_mg.hpp).
+// ffr_mg.hip - host side of the multigrid-preconditioned pressure solve
+// (method in ffr_mg.hpp): hierarchy set-up on the device and the enqueue
+// logic of the outer FP64 / inner FP32 iteration.
 #include "ffr_scene.hpp"
+#include "ffr_mg_launch.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
 
 namespace ffr {
 
-size_t mg_bytes(int, int, int, const ffr_params &) { return 0; }
-int mg_create(ffr_scene *) { return FFR_OK; }
-void mg_destroy(ffr_scene *) {}
-int mg_solve(ffr_scene *s, int *, int *, double *)
+namespace {
+
+struct Dims {
+    int levels;
+    int n[kMaxLevels][3];
+};
+
+Dims level_dims(int nx, int ny, int nz, const ffr_params &p)
 {
-    s->ctx->fail("MG-PCG not available yet");
-    return FFR_E_UNSUPPORTED;
+    Dims d{};
+    const int cap = p.mg_levels > 0 ? std::min(p.mg_levels, kMaxLevels) : kMaxLevels;
+    d.n[0][0] = nx; d.n[0][1] = ny; d.n[0][2] = nz;
+    d.levels = 1;
+    // coarsen while every extent is at least 4 cells
+    while (d.levels < cap) {
+        const int *f = d.n[d.levels - 1];
+        if (std::min({f[0], f[1], f[2]}) < 4) break;
+        for (int a = 0; a < 3; a++) d.n[d.levels][a] = (f[a] + 1) / 2;
+        d.levels++;
+    }
+    return d;
+}
+
+// arena layout (sizing pass when base == nullptr)
+size_t mg_layout(char *base, const Dims &D, bool bf16, Multigrid *M, MgState **state)
+{
+    size_t off = 0;
+    auto take = [&](size_t bytes) -> char * {
+        char *ptr = base ? base + off : nullptr;
+        off = (off + bytes + 255) & ~(size_t)255;
+        return ptr;
+    };
+    const uint32_t n0 = (uint32_t)D.n[0][0] * D.n[0][1] * D.n[0][2];
+    MgState *st = (MgState *)take(sizeof(MgState));
+    float *e = (float *)take(4ull * n0), *ri = (float *)take(4ull * n0);
+    float *pi = (float *)take(4ull * n0), *qi = (float *)take(4ull * n0);
+    float *z0 = (float *)take(4ull * n0);
+    if (M) {
+        M->e = e; M->ri = ri; M->pi = pi; M->qi = qi;
+        M->lv[0].z = z0;
+        M->lv[0].r = ri;
+        M->lv[0].active = nullptr;
+        M->lv[0].cx = M->lv[0].cy = M->lv[0].cz = M->lv[0].diag = M->lv[0].rdiag = nullptr;
+        M->lv[0].bf16 = 0;
+    }
+    for (int l = 1; l < D.levels; l++) {
+        const uint32_t n = (uint32_t)D.n[l][0] * D.n[l][1] * D.n[l][2];
+        const size_t vs = bf16 ? 2 : 4;
+        uint8_t *act = (uint8_t *)take(n);
+        float *cx = (float *)take(4ull * n), *cy = (float *)take(4ull * n), *cz = (float *)take(4ull * n);
+        float *dg = (float *)take(4ull * n), *rd = (float *)take(4ull * n);
+        void *z = take(vs * n), *r = take(vs * n);
+        if (M) {
+            MgLevel &L = M->lv[l];
+            L.active = act; L.cx = cx; L.cy = cy; L.cz = cz; L.diag = dg; L.rdiag = rd;
+            L.z = z; L.r = r; L.bf16 = bf16 ? 1 : 0;
+        }
+    }
+    if (state) *state = st;
+    return off;
+}
+
+bool uniform(const std::vector<double> &h)
+{
+    for (double x : h)
+        if (std::fabs(x - h[0]) > 1e-12 * std::fabs(h[0])) return false;
+    return true;
+}
+
+}  // namespace
+
+size_t mg_bytes(int nx, int ny, int nz, const ffr_params &p)
+{
+    const Dims D = level_dims(nx, ny, nz, p);
+    return mg_layout(nullptr, D, p.mg_bf16 != 0, nullptr, nullptr);
+}
+
+static MgState *g_unused_state;  // placeholder for sizing calls
+
+int mg_create(ffr_scene *s)
+{
+    if (s->mg.levels) return FFR_OK;
+    if (!uniform(s->dxv) || !uniform(s->dyv) || !uniform(s->dzv)) {
+        s->ctx->fail("multigrid-preconditioned CG needs uniform spacing per axis (stretched grids "
+                     "use the CG solver)");
+        return FFR_E_UNSUPPORTED;
+    }
+    const Dims D = level_dims(s->nx, s->ny, s->nz, s->prm);
+    Multigrid &M = s->mg;
+    const bool bf16 = s->prm.mg_bf16 != 0;
+    const size_t bytes = mg_layout(nullptr, D, bf16, nullptr, nullptr);
+    if (s->ctx->in_use + bytes > s->ctx->budget) {
+        s->ctx->fail("multigrid hierarchy needs %zu bytes beyond the context budget", bytes);
+        return FFR_E_NOMEM;
+    }
+    if (hipMalloc(&M.arena, bytes) != hipSuccess) {
+        s->ctx->fail("hipMalloc of %zu bytes for the multigrid hierarchy failed", bytes);
+        return FFR_E_NOMEM;
+    }
+    M.bytes = bytes;
+    s->ctx->in_use += bytes;
+    MgState *st = nullptr;
+    mg_layout((char *)M.arena, D, bf16, &M, &st);
+    s->mgs = st;
+    M.levels = D.levels;
+    for (int l = 0; l < D.levels; l++) {
+        MgLevel &L = M.lv[l];
+        L.nx = D.n[l][0]; L.ny = D.n[l][1]; L.nz = D.n[l][2];
+        L.n = (uint32_t)L.nx * L.ny * L.nz;
+        L.f_nz = make_fastdiv(L.nz);
+        L.f_nynz = make_fastdiv(L.ny * L.nz);
+    }
+    M.pre_sweeps = std::max(1, s->prm.mg_pre_sweeps);
+    M.coarse_sweeps = std::max(1, s->prm.mg_coarse_sweeps);
+    hipStream_t str = s->stream;
+    HIP_CHECK_CTX(s->ctx, hipMemsetAsync(M.arena, 0, bytes, str), FFR_E_DEVICE);
+    for (int l = 0; l + 1 < D.levels; l++) mg_launch_build(str, s->geo, M.lv[l], M.lv[l + 1], l == 0);
+    HIP_CHECK_CTX(s->ctx, hipStreamSynchronize(str), FFR_E_DEVICE);
+    (void)g_unused_state;
+    return FFR_OK;
+}
+
+void mg_destroy(ffr_scene *s)
+{
+    if (s->mg.arena) {
+        hipFree(s->mg.arena);
+        s->ctx->in_use -= s->mg.bytes;
+        s->mg.arena = nullptr;
+    }
+    s->mg.levels = 0;
+}
+
+// ---- V-cycle enqueue (z_0 = M r_0 on level 0) --------------------------------
+template <typename TC>
+static void vcycle(ffr_scene *s, int l)
+{
+    Multigrid &M = s->mg;
+    const Geo &g = s->geo;
+    hipStream_t st = s->stream;
+    const MgState *m = s->mgs;
+    const MgLevel &L = M.lv[l];
+    const bool l0 = l == 0;
+    const double n = (double)L.n;
+    if (l == M.levels - 1) {
+        ProfScope ps(s->ctx, st, l0 ? "mg_coarse_l0" : "mg_coarse", 0);
+        if (l0) mg_launch_coarse<float>(st, g, L, true, M.coarse_sweeps, m);
+        else mg_launch_coarse<TC>(st, g, L, false, M.coarse_sweeps, m);
+        return;
+    }
+    // algorithmic bytes per level-0 pass: z read, r read, flags, z write
+    const double bsm = l0 ? n * (4 + 4 + 1 + 4) : n * (2 * sizeof(TC) + 1 + 20 + sizeof(TC));
+    for (int sw = 0; sw < M.pre_sweeps; sw++) {
+        {
+            ProfScope ps(s->ctx, st, l0 ? "mg_smooth_l0" : "mg_smooth", bsm);
+            if (l0) mg_launch_smooth<float>(st, g, L, true, 0, sw == 0, m);
+            else mg_launch_smooth<TC>(st, g, L, false, 0, sw == 0, m);
+        }
+        {
+            ProfScope ps(s->ctx, st, l0 ? "mg_smooth_l0" : "mg_smooth", bsm);
+            if (l0) mg_launch_smooth<float>(st, g, L, true, 1, 0, m);
+            else mg_launch_smooth<TC>(st, g, L, false, 1, 0, m);
+        }
+    }
+    {
+        ProfScope ps(s->ctx, st, l0 ? "mg_restrict_l0" : "mg_restrict", 0);
+        if (l0) mg_launch_restrict<float, TC>(st, g, L, M.lv[l + 1], true, m);
+        else mg_launch_restrict<TC, TC>(st, g, L, M.lv[l + 1], false, m);
+    }
+    vcycle<TC>(s, l + 1);
+    {
+        ProfScope ps(s->ctx, st, l0 ? "mg_prolong_l0" : "mg_prolong", 0);
+        if (l0) mg_launch_prolong<float, TC>(st, g, L, M.lv[l + 1], true, m);
+        else mg_launch_prolong<TC, TC>(st, g, L, M.lv[l + 1], false, m);
+    }
+    for (int sw = 0; sw < M.pre_sweeps; sw++) {
+        {
+            ProfScope ps(s->ctx, st, l0 ? "mg_smooth_l0" : "mg_smooth", bsm);
+            if (l0) mg_launch_smooth<float>(st, g, L, true, 1, 0, m);
+            else mg_launch_smooth<TC>(st, g, L, false, 1, 0, m);
+        }
+        {
+            ProfScope ps(s->ctx, st, l0 ? "mg_smooth_l0" : "mg_smooth", bsm);
+            if (l0) mg_launch_smooth<float>(st, g, L, true, 0, 0, m);
+            else mg_launch_smooth<TC>(st, g, L, false, 0, 0, m);
+        }
+    }
+}
+
+static void precondition(ffr_scene *s)
+{
+    if (s->mg.levels > 1 && s->mg.lv[1].bf16) vcycle<__hip_bfloat16>(s, 0);
+    else vcycle<float>(s, 0);
+}
+
+static void inner_iteration(ffr_scene *s, int it)
+{
+    Multigrid &M = s->mg;
+    const Geo &g = s->geo;
+    hipStream_t st = s->stream;
+    const double n = (double)g.nc;
+    {
+        ProfScope ps(s->ctx, st, "pcg_spmv", n * (4 + 1 + 4));
+        pcg_launch_spmv(st, g, M.pi, M.qi, s->mgs, s->red);
+    }
+    {
+        ProfScope ps(s->ctx, st, "pcg_update", n * 24);
+        pcg_launch_update(st, g.nc, M.e, M.ri, M.pi, M.qi, s->mgs, s->red, it);
+    }
+    pcg_launch_check(st, g.nc, s->mgs, s->red, it);
+    precondition(s);
+    {
+        ProfScope ps(s->ctx, st, "pcg_dot_rz", n * 8);
+        pcg_launch_dot_rz(st, g.nc, M.ri, (const float *)M.lv[0].z, s->mgs, s->red);
+    }
+    {
+        ProfScope ps(s->ctx, st, "pcg_dir", n * 12);
+        pcg_launch_dir(st, g.nc, (const float *)M.lv[0].z, M.pi, s->mgs, s->red, it);
+    }
+}
+
+int mg_solve(ffr_scene *s, int *iters, int *outer, double *res)
+{
+    if (!s->mg.levels) {
+        int rc = mg_create(s);
+        if (rc != FFR_OK) return rc;
+    }
+    Multigrid &M = s->mg;
+    const Geo &g = s->geo;
+    hipStream_t st = s->stream;
+    MgState *h = s->h_mgs;
+    const double n = (double)g.nc;
+    const int max_outer = 50;
+    const int max_inner = std::max(1, s->prm.inner_max_iter);
+    const int total_cap = std::max(1, s->prm.poisson_max_iter);
+    double *X[2] = {s->p, s->d};
+    {
+        ProfScope ps(s->ctx, st, "dc_init", n * (8 + 8 + 4 + 1));
+        dc_launch_init(st, g, s->b, s->p, M.e, s->red);
+    }
+    dc_launch_start(st, g, s->mgs, s->red, s->prm.poisson_tol, max_outer, max_inner, s->prm.inner_tol);
+    int passes = 0;
+    for (int o = 0; o <= max_outer; o++) {
+        {
+            ProfScope ps(s->ctx, st, "dc_residual", n * (8 + 8 + 4 + 1 + 8 + 4));
+            dc_launch_residual(st, g, s->b, X[passes & 1], M.e, X[(passes + 1) & 1], M.ri, s->mgs, s->red);
+        }
+        dc_launch_check(st, g, s->mgs, s->red);
+        HIP_CHECK_CTX(s->ctx, hipMemcpyAsync(h, s->mgs, sizeof(MgState), hipMemcpyDeviceToHost, st), FFR_E_DEVICE);
+        HIP_CHECK_CTX(s->ctx, hipStreamSynchronize(st), FFR_E_DEVICE);
+        if (h->bnorm < 1e-15) break;  // nothing ran: x is 0 in X[0]
+        passes++;
+        if (h->solve_done || h->total_iters >= total_cap) break;
+        // inner solve: z = M r, p = z, then CG steps until the inner test stops it
+        precondition(s);
+        pcg_launch_dot_rz(st, g.nc, M.ri, (const float *)M.lv[0].z, s->mgs, s->red);
+        pcg_launch_start(st, g.nc, (const float *)M.lv[0].z, M.pi, M.e, s->mgs, s->red);
+        const int slot = std::min(o, 15);
+        int ahead = M.inner_pred[slot] > 0 ? M.inner_pred[slot] : 6;
+        ahead = std::min(ahead, max_inner);
+        int it = 0;
+        for (; it < ahead; it++) inner_iteration(s, it);
+        HIP_CHECK_CTX(s->ctx, hipMemcpyAsync(h, s->mgs, sizeof(MgState), hipMemcpyDeviceToHost, st), FFR_E_DEVICE);
+        HIP_CHECK_CTX(s->ctx, hipStreamSynchronize(st), FFR_E_DEVICE);
+        while (!h->inner_done && it < max_inner) {
+            inner_iteration(s, it++);
+            HIP_CHECK_CTX(s->ctx, hipMemcpyAsync(h, s->mgs, sizeof(MgState), hipMemcpyDeviceToHost, st), FFR_E_DEVICE);
+            HIP_CHECK_CTX(s->ctx, hipStreamSynchronize(st), FFR_E_DEVICE);
+        }
+        M.inner_pred[slot] = h->inner_it;
+    }
+    {
+        ProfScope ps(s->ctx, st, "dc_finalize", n * (8 + 8 + 1));
+        dc_launch_finalize(st, g, s->b, X[passes & 1], s->p, s->prm.use_multigrid);
+    }
+    HIP_CHECK_CTX(s->ctx, hipMemcpyAsync(h, s->mgs, sizeof(MgState), hipMemcpyDeviceToHost, st), FFR_E_DEVICE);
+    HIP_CHECK_CTX(s->ctx, hipStreamSynchronize(st), FFR_E_DEVICE);
+    *iters = h->total_iters;
+    *outer = passes;
+    *res = h->res;
+    return FFR_OK;
 }
 
 }  // namespace ffr

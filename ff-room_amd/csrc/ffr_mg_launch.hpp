@@ -1,0 +1,41 @@
+// ffr_mg_launch.hpp - launchers of the multigrid and defect-correction kernels.
+#pragma once
+#include "ffr_kernels.hpp"
+#include "ffr_mg.hpp"
+
+namespace ffr {
+void mg_launch_build(hipStream_t s, const Geo &g, const MgLevel &F, const MgLevel &C, bool fine0);
+template <typename T>
+void mg_launch_smooth(hipStream_t s, const Geo &g, const MgLevel &L, bool l0, int color,
+                      int zero_start, const MgState *m);
+template <typename TF, typename TC>
+void mg_launch_restrict(hipStream_t s, const Geo &g, const MgLevel &F, const MgLevel &C, bool l0,
+                        const MgState *m);
+template <typename TF, typename TC>
+void mg_launch_prolong(hipStream_t s, const Geo &g, const MgLevel &F, const MgLevel &C, bool l0,
+                       const MgState *m);
+template <typename T>
+void mg_launch_coarse(hipStream_t s, const Geo &g, const MgLevel &L, bool l0, int sweeps,
+                      const MgState *m);
+void pcg_launch_spmv(hipStream_t s, const Geo &g, const float *p, float *q, const MgState *m,
+                     RedScratch red);
+void pcg_launch_update(hipStream_t s, uint32_t n, float *e, float *r, const float *p,
+                       const float *q, MgState *m, RedScratch red, int it);
+void pcg_launch_check(hipStream_t s, uint32_t n, MgState *m, RedScratch red, int it);
+void pcg_launch_dot_rz(hipStream_t s, uint32_t n, const float *r, const float *z, const MgState *m,
+                       RedScratch red);
+void pcg_launch_start(hipStream_t s, uint32_t n, const float *z, float *p, float *e, MgState *m,
+                      RedScratch red);
+void pcg_launch_dir(hipStream_t s, uint32_t n, const float *z, float *p, MgState *m,
+                    RedScratch red, int it);
+void dc_launch_init(hipStream_t s, const Geo &g, const double *b, double *x, float *e,
+                    RedScratch red);
+void dc_launch_start(hipStream_t s, const Geo &g, MgState *m, RedScratch red, double outer_tol,
+                     int max_outer, int max_inner, double inner_tol);
+void dc_launch_residual(hipStream_t s, const Geo &g, const double *b, const double *xin,
+                        const float *e, double *xout, float *r32, const MgState *m,
+                        RedScratch red);
+void dc_launch_check(hipStream_t s, const Geo &g, MgState *m, RedScratch red);
+void dc_launch_finalize(hipStream_t s, const Geo &g, const double *b, const double *x, double *p,
+                        int pin_outflow);
+}  // namespace ffr

@@ -128,7 +128,7 @@ size_t layout(char *base, const ffr_scene_desc *d, const ffr_params *p, ArenaPtr
     a.scratch = (p->precision == FFR_PREC_MIXED) ? (double *)take(8 * std::max({nu, nv, nw})) : nullptr;
     a.rec = (StepRec *)take(sizeof(StepRec));
     a.cg = (CgState *)take(sizeof(CgState));
-    a.partials = (double *)take(sizeof(double) * kMaxBlocksCap);
+    a.partials = (double *)take(sizeof(double) * kMaxBlocksCap * kSlots);
     a.tickets = (unsigned int *)take(sizeof(unsigned int) * 16);
     if (out) *out = a;
     return off;
@@ -500,8 +500,8 @@ static int cg_fp64(ffr_scene *s, int *iters, double *res)
         const int nrun = std::min(chunk, max_iter - enq);
         for (int m = 0; m < nrun; m++) {
             { ProfScope ps(s->ctx, st, "cg_spmv", nc * 17); launch_cg_spmv(st, g, s->d, s->Ad, s->cg, s->red); }
-            { ProfScope ps(s->ctx, st, "cg_update", nc * 48); launch_cg_update(st, g.nc, s->p, s->r, s->d, s->Ad, s->cg, s->red); }
-            { ProfScope ps(s->ctx, st, "cg_dir", nc * 24); launch_cg_dir(st, g.nc, s->d, s->r, s->cg); }
+            { ProfScope ps(s->ctx, st, "cg_update", nc * 48); launch_cg_update(st, g.nc, s->p, s->r, s->d, s->Ad, s->cg, s->red, enq + m); }
+            { ProfScope ps(s->ctx, st, "cg_dir", nc * 24); launch_cg_dir(st, g.nc, s->d, s->r, s->cg, s->red, enq + m); }
         }
         enq += nrun;
         HIP_CHECK_CTX(s->ctx, hipMemcpyAsync(s->h_cg, s->cg, sizeof(CgState), hipMemcpyDeviceToHost, st), FFR_E_DEVICE);
@@ -578,6 +578,7 @@ static int step_impl(ffr_scene *s, ffr_step_result *out, bool sync_result)
             ProfScope ps(ctx, st, "tsum", nc * sz + nc);
             launch_tsum<R>(st, g, (R *)s->T, s->rec, s->red);
         }
+        launch_step_finalize(st, g, s->rec, s->red);
     } else {
         ProfScope ps(ctx, st, "divmax", faces * sz + nc);
         launch_divmax<R>(st, g, u, v, w, s->rec);
@@ -662,6 +663,7 @@ int ffr_T_mean(ffr_scene *s, double *out)
     hipStream_t st = s->stream;
     if (s->fp64) launch_tsum<double>(st, s->geo, (double *)s->T, s->rec, s->red);
     else launch_tsum<float>(st, s->geo, (float *)s->T, s->rec, s->red);
+    launch_step_finalize(st, s->geo, s->rec, s->red);
     HIP_CHECK_CTX(s->ctx, hipMemcpyAsync(s->h_rec, s->rec, sizeof(StepRec), hipMemcpyDeviceToHost, st), FFR_E_DEVICE);
     HIP_CHECK_CTX(s->ctx, hipStreamSynchronize(st), FFR_E_DEVICE);
     *out = s->n_fluid > 0 ? s->h_rec->tsum / (double)s->n_fluid : s->prm.T_initial;  // :458-468

@@ -89,6 +89,7 @@ struct ffr_scene {
     ffr::RedScratch red{};
     double *scratch64 = nullptr;  // nc doubles for host transfers of fp32 state
     ffr::Multigrid mg{};          // multigrid hierarchy (when allocated)
+    ffr::MgState *mgs = nullptr;  // device scalars of the multigrid solve
 
     // pinned host mirrors
     ffr::StepRec *h_rec = nullptr;

@@ -68,10 +68,11 @@ void launch_laplacian(hipStream_t s, const Geo &g, const double *x, double *Ax)
     hipLaunchKernelGGL(k_laplacian, dim3(grid_for(g.nc)), dim3(kBlock), 0, s, g, x, Ax);
 }
 
-// x = 0, r = d = b (SOLID rows 0), rr = r.r (PoissonSolver.cpp:96-117)
+__device__ __forceinline__ double *slot(RedScratch red, int k) { return red.partials + k * kMaxBlocksCap; }
+
+// x = 0, r = d = b (SOLID rows 0), partials of r.r (PoissonSolver.cpp:96-117)
 __global__ void __launch_bounds__(kBlock) k_cg_init(Geo g, const double *b, double *x, double *r,
-                                                    double *d, CgState *cg, RedScratch red,
-                                                    int max_iter, double tol)
+                                                    double *d, RedScratch red)
 {
     __shared__ double lds[kBlock / kWave];
     const uint32_t blk = xcd_block();
@@ -85,39 +86,45 @@ __global__ void __launch_bounds__(kBlock) k_cg_init(Geo g, const double *b, doub
         d[c] = rv;
         rr += rv * rv;
     }
-    double total;
-    if (grid_sum_last(rr, red.partials, &red.tickets[1], blk, lds, &total)) {
-        cg->max_iter = max_iter;
-        cg->tol = tol;
-        const double bn = sqrt(total);
-        cg->rr = total;
-        cg->bnorm = bn;
-        cg->iter = 0;
-        cg->alpha = cg->beta = 0.0;
-        if (bn < 1e-15) {
-            cg->done = 1;
-            cg->iters = 0;
-            cg->res = 0.0;
-        } else if (cg->max_iter <= 0) {
-            cg->done = 1;
-            cg->iters = cg->max_iter;
-            cg->res = sqrt(total) / bn;
-        } else {
-            cg->done = 0;
-            cg->iters = 0;
-            cg->res = 1.0;
-        }
+    block_partial(rr, slot(red, SLOT_INIT), blk, lds);
+}
+
+// ||b||, early exits of :113-117 and max_iter <= 0 (one block)
+__global__ void __launch_bounds__(kBlock) k_cg_start(CgState *cg, RedScratch red, uint32_t G,
+                                                     int max_iter, double tol)
+{
+    __shared__ double lds[kBlock / kWave];
+    const double total = sum_partials(slot(red, SLOT_INIT), G, lds);
+    if (threadIdx.x != 0) return;
+    const double bn = sqrt(total);
+    cg->max_iter = max_iter;
+    cg->tol = tol;
+    cg->rr[0] = total;
+    cg->bnorm = bn;
+    if (bn < 1e-15) {
+        cg->done = 1;
+        cg->iters = 0;
+        cg->res = 0.0;
+    } else if (max_iter <= 0) {
+        cg->done = 1;
+        cg->iters = max_iter;
+        cg->res = sqrt(total) / bn;
+    } else {
+        cg->done = 0;
+        cg->iters = 0;
+        cg->res = 1.0;
     }
 }
 
 void launch_cg_init(hipStream_t s, const Geo &g, const double *b, double *x, double *r,
                     double *d, CgState *cg, RedScratch red, int max_iter, double tol)
 {
-    hipLaunchKernelGGL(k_cg_init, dim3(grid_for(g.nc)), dim3(kBlock), 0, s, g, b, x, r, d, cg,
-                       red, max_iter, tol);
+    const unsigned G = grid_for(g.nc);
+    hipLaunchKernelGGL(k_cg_init, dim3(G), dim3(kBlock), 0, s, g, b, x, r, d, red);
+    hipLaunchKernelGGL(k_cg_start, dim3(1), dim3(kBlock), 0, s, cg, red, G, max_iter, tol);
 }
 
-// Ad = A d and dAd = d.Ad; alpha = rr / dAd, or stop when |dAd| < 1e-30 (:120-129)
+// Ad = A d and partials of d.Ad (:120-126)
 __global__ void __launch_bounds__(kBlock) k_cg_spmv(Geo g, const double *d, double *Ad,
                                                     CgState *cg, RedScratch red)
 {
@@ -140,17 +147,7 @@ __global__ void __launch_bounds__(kBlock) k_cg_spmv(Geo g, const double *d, doub
         Ad[c] = a;
         dq += dc * a;
     }
-    double dAd;
-    if (grid_sum_last(dq, red.partials, &red.tickets[2], blk, lds, &dAd)) {
-        cg->dAd = dAd;
-        if (fabs(dAd) < 1e-30) {  // breakdown: the reference reports max_iter (:127, 155-157)
-            cg->done = 1;
-            cg->iters = cg->max_iter;
-            cg->res = sqrt(cg->rr) / cg->bnorm;
-        } else {
-            cg->alpha = cg->rr / dAd;
-        }
-    }
+    block_partial(dq, slot(red, SLOT_SPMV), blk, lds);
 }
 
 void launch_cg_spmv(hipStream_t s, const Geo &g, const double *d, double *Ad, CgState *cg,
@@ -159,15 +156,26 @@ void launch_cg_spmv(hipStream_t s, const Geo &g, const double *d, double *Ad, Cg
     hipLaunchKernelGGL(k_cg_spmv, dim3(grid_for(g.nc)), dim3(kBlock), 0, s, g, d, Ad, cg, red);
 }
 
-// x += alpha d, r -= alpha Ad, rr_new = r.r; convergence test and beta (:130-152)
+// alpha = rr / d.Ad (stop when |d.Ad| < 1e-30, :127-129); x += alpha d,
+// r -= alpha Ad and partials of r.r (:130-138)
 __global__ void __launch_bounds__(kBlock) k_cg_update(uint32_t n, double *x, double *r,
                                                       const double *d, const double *Ad,
-                                                      CgState *cg, RedScratch red)
+                                                      CgState *cg, RedScratch red, int it)
 {
     __shared__ double lds[kBlock / kWave];
     if (cg->done) return;
-    const double alpha = cg->alpha;
     const uint32_t blk = xcd_block();
+    const double dAd = sum_partials(slot(red, SLOT_SPMV), gridDim.x, lds);
+    const double rr_old = cg->rr[it & 1];
+    if (fabs(dAd) < 1e-30) {  // breakdown: the reference reports max_iter (:127, 155-157)
+        if (blk == 0 && threadIdx.x == 0) {
+            cg->done = 1;
+            cg->iters = cg->max_iter;
+            cg->res = sqrt(rr_old) / cg->bnorm;
+        }
+        return;
+    }
+    const double alpha = rr_old / dAd;
     double rr = 0.0;
     uint32_t sb, se;
     block_span(n, blk, sb, se);
@@ -177,56 +185,52 @@ __global__ void __launch_bounds__(kBlock) k_cg_update(uint32_t n, double *x, dou
         r[c] = rv;
         rr += rv * rv;
     }
-    double rr_new;
-    if (grid_sum_last(rr, red.partials, &red.tickets[3], blk, lds, &rr_new)) {
-        const double res = sqrt(rr_new) / cg->bnorm;
-        const int it = cg->iter + 1;
-        if (res < cg->tol) {
+    block_partial(rr, slot(red, SLOT_UPDATE), blk, lds);
+}
+
+void launch_cg_update(hipStream_t s, uint32_t n, double *x, double *r, const double *d,
+                      const double *Ad, CgState *cg, RedScratch red, int it)
+{
+    hipLaunchKernelGGL(k_cg_update, dim3(grid_for(n)), dim3(kBlock), 0, s, n, x, r, d, Ad, cg,
+                       red, it);
+}
+
+// res = sqrt(rr_new)/||b|| and the stopping test (:140-145); beta, d = r + beta d (:147-152)
+__global__ void __launch_bounds__(kBlock) k_cg_dir(uint32_t n, double *d, const double *r,
+                                                   CgState *cg, RedScratch red, int it)
+{
+    __shared__ double lds[kBlock / kWave];
+    if (cg->done) return;
+    const uint32_t blk = xcd_block();
+    const double rr_new = sum_partials(slot(red, SLOT_UPDATE), gridDim.x, lds);
+    const double res = sqrt(rr_new) / cg->bnorm;
+    const bool lead = blk == 0 && threadIdx.x == 0;
+    if (res < cg->tol) {
+        if (lead) {
             cg->done = 1;
-            cg->iters = it;
+            cg->iters = it + 1;
             cg->res = res;
-        } else {
-            cg->beta = rr_new / cg->rr;
-            cg->rr = rr_new;
-            cg->iter = it;
-            if (it >= cg->max_iter) {
-                cg->done = 1;
-                cg->iters = cg->max_iter;
-                cg->res = sqrt(rr_new) / cg->bnorm;
-            }
+        }
+        return;
+    }
+    const double beta = rr_new / cg->rr[it & 1];
+    uint32_t sb, se;
+    block_span(n, blk, sb, se);
+    for (uint32_t c = sb + threadIdx.x; c < se; c += kBlock) d[c] = r[c] + beta * d[c];
+    if (lead) {
+        cg->rr[(it + 1) & 1] = rr_new;
+        if (it + 1 >= cg->max_iter) {
+            cg->done = 1;
+            cg->iters = cg->max_iter;
+            cg->res = sqrt(rr_new) / cg->bnorm;
         }
     }
 }
 
-void launch_cg_update(hipStream_t s, uint32_t n, double *x, double *r, const double *d,
-                      const double *Ad, CgState *cg, RedScratch red)
+void launch_cg_dir(hipStream_t s, uint32_t n, double *d, const double *r, CgState *cg,
+                   RedScratch red, int it)
 {
-    hipLaunchKernelGGL(k_cg_update, dim3(grid_for(n)), dim3(kBlock), 0, s, n, x, r, d, Ad, cg,
-                       red);
-}
-
-// d = r + beta d (:147-151)
-__global__ void __launch_bounds__(kBlock) k_cg_dir(uint32_t n, double *d, const double *r,
-                                                   CgState *cg)
-{
-    if (cg->done) return;
-    const double beta = cg->beta;
-    uint32_t sb, se;
-    block_span(n, xcd_block(), sb, se);
-    for (uint32_t c = sb + threadIdx.x; c < se; c += kBlock) d[c] = r[c] + beta * d[c];
-}
-
-void launch_cg_dir(hipStream_t s, uint32_t n, double *d, const double *r, CgState *cg)
-{
-    hipLaunchKernelGGL(k_cg_dir, dim3(grid_for(n)), dim3(kBlock), 0, s, n, d, r, cg);
-}
-
-void launch_cg_iteration(hipStream_t s, const Geo &g, double *x, double *r, double *d,
-                         double *Ad, CgState *cg, RedScratch red)
-{
-    launch_cg_spmv(s, g, d, Ad, cg, red);
-    launch_cg_update(s, g.nc, x, r, d, Ad, cg, red);
-    launch_cg_dir(s, g.nc, d, r, cg);
+    hipLaunchKernelGGL(k_cg_dir, dim3(grid_for(n)), dim3(kBlock), 0, s, n, d, r, cg, red, it);
 }
 
 }  // namespace ffr

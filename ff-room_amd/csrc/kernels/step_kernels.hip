@@ -492,8 +492,7 @@ __global__ void __launch_bounds__(kBlock) k_temperature(Geo g, StepParams sp, co
     if (bad) rec->nonfinite = 1;
     const double m = block_max(dmax, lds);
     if (threadIdx.x == 0 && m > 0.0) atomic_max_nonneg(&rec->divmax_bits, m);
-    double total;
-    if (grid_sum_last(tsum, red.partials, &red.tickets[0], blk, lds, &total)) rec->tsum = total;
+    block_partial(tsum, red.partials + SLOT_TSUM * kMaxBlocksCap, blk, lds);
 }
 
 template <typename R>
@@ -515,14 +514,26 @@ __global__ void __launch_bounds__(kBlock) k_tsum(Geo g, const R *T, StepRec *rec
     block_span(g.nc, blk, sb, se);
     for (uint32_t c = sb + threadIdx.x; c < se; c += kBlock)
         if (g.flags[c] == FLUID) t += (double)T[c];
-    double total;
-    if (grid_sum_last(t, red.partials, &red.tickets[0], blk, lds, &total)) rec->tsum = total;
+    block_partial(t, red.partials + SLOT_TSUM * kMaxBlocksCap, blk, lds);
 }
 
 template <typename R>
 void launch_tsum(hipStream_t s, const Geo &g, const R *T, StepRec *rec, RedScratch red)
 {
     hipLaunchKernelGGL(k_tsum<R>, dim3(grid_for(g.nc)), dim3(kBlock), 0, s, g, T, rec, red);
+}
+
+// rec->tsum from the per-block partials of temperature / tsum (one block).
+__global__ void __launch_bounds__(kBlock) k_step_finalize(StepRec *rec, RedScratch red, uint32_t G)
+{
+    __shared__ double lds[kBlock / kWave];
+    const double t = sum_partials(red.partials + SLOT_TSUM * kMaxBlocksCap, G, lds);
+    if (threadIdx.x == 0) rec->tsum = t;
+}
+
+void launch_step_finalize(hipStream_t s, const Geo &g, StepRec *rec, RedScratch red)
+{
+    hipLaunchKernelGGL(k_step_finalize, dim3(1), dim3(kBlock), 0, s, rec, red, grid_for(g.nc));
 }
 
 // ---- opening temperatures (BoundaryManager.cpp:137-178) -----------------

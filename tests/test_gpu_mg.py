@@ -1,0 +1,172 @@
+"""The multigrid-preconditioned pressure solve and the mixed-precision step.
+
+Pins: the dense solution of the same linear system (tiny grids), the FP64
+reference CG converged tightly (oracle), the manufactured solution, and the
+FP64 GPU step for the mixed-precision step's trajectory error. Tolerances are
+stated per test.
+"""
+import numpy as np
+import pytest
+
+from ff_room import capi
+from ff_room.generator import random_scene, workload_c2
+from helpers import capi_params, device_scene, rel_l2
+from oracle.pyoracle import OUTFLOW, SOLID, OracleSolver, SceneArrays
+from oracle.scene_flags import build_scene
+from scenes import BASIC_ROOM, SMALL_LW, SMALL_THERMAL, solver_params
+
+pytestmark = pytest.mark.gpu
+
+
+def _rhs(sc, seed=3):
+    b = np.random.default_rng(seed).standard_normal(sc.nx * sc.ny * sc.nz)
+    b[sc.flags == SOLID] = 0.0
+    return b
+
+
+def _dense(sc):
+    o = OracleSolver(sc, {}, init=False)
+    n = sc.nx * sc.ny * sc.nz
+    A = np.zeros((n, n))
+    for c in range(n):
+        e = np.zeros(n)
+        e[c] = 1.0
+        A[:, c] = o.laplacian(e)
+    return A
+
+
+@pytest.mark.parametrize("bf16", [0, 1])
+@pytest.mark.parametrize("precision", [capi.PREC_FP64, capi.PREC_MIXED])
+def test_mgpcg_solves_the_reference_system(ctx, bf16, precision):
+    """CG semantics (OUTFLOW pressure back-substituted): equals the dense solve."""
+    rng = np.random.default_rng(1)
+    nx, ny, nz = 9, 7, 6
+    f = rng.choice([0, 0, 0, 0, 0, 0, SOLID, 2], size=(nx, ny, nz)).astype(np.uint8)
+    f[-1] = OUTFLOW
+    sc = SceneArrays(nx, ny, nz, 1.8, 1.4, 1.2, f.reshape(-1))
+    A = _dense(sc)
+    b = _rhs(sc)
+    x = np.linalg.solve(A, b)
+    d = device_scene(ctx, sc, capi_params({}, poisson_method=capi.POISSON_MGPCG, poisson_tol=1e-11,
+                                          mg_bf16=bf16, precision=precision, poisson_max_iter=500))
+    p, info = d.poisson(b)
+    assert info["residual"] < 1e-11
+    assert rel_l2(p, x) < 1e-9
+
+
+def test_mgpcg_pinned_outflow_is_the_reference_multigrid_system(ctx):
+    """use_multigrid: OUTFLOW pressure pinned to 0 (PoissonSolver.cpp:201)."""
+    rng = np.random.default_rng(2)
+    nx, ny, nz = 8, 6, 5
+    f = rng.choice([0, 0, 0, 0, 0, SOLID], size=(nx, ny, nz)).astype(np.uint8)
+    f[:, -1, :] = OUTFLOW
+    sc = SceneArrays(nx, ny, nz, 1.6, 1.2, 1.0, f.reshape(-1))
+    A = _dense(sc)
+    b = _rhs(sc, 4)
+    out = f.reshape(-1) == OUTFLOW
+    A[out] = 0.0
+    A[out, out] = 1.0
+    bp = b.copy()
+    bp[out] = 0.0
+    x = np.linalg.solve(A, bp)
+    d = device_scene(ctx, sc, capi_params({}, use_multigrid=1, poisson_tol=1e-11,
+                                          poisson_max_iter=500))
+    p, info = d.poisson(b)
+    assert np.all(p[out] == 0.0)
+    assert rel_l2(p, x) < 1e-9
+
+
+@pytest.mark.parametrize("cfg", [BASIC_ROOM, SMALL_THERMAL])
+def test_mgpcg_matches_tight_reference_cg(ctx, cfg):
+    sc = build_scene(cfg)
+    b = _rhs(sc, 9)
+    o = OracleSolver(sc, {}, init=False)
+    o.poisson_cg(b, 20000, 1e-13)
+    d = device_scene(ctx, sc, capi_params({}, poisson_method=capi.POISSON_MGPCG, poisson_tol=1e-12,
+                                          precision=capi.PREC_MIXED, poisson_max_iter=500))
+    p, info = d.poisson(b)
+    nonsolid = sc.flags != SOLID
+    assert rel_l2(p[nonsolid], o.p[nonsolid]) < 1e-9
+    assert info["outer_iterations"] >= 2  # FP32 inner solves need FP64 correction
+
+
+def test_mgpcg_manufactured_second_order(ctx):
+    from test_oracle import manufactured
+    errs = []
+    for N in (16, 32):
+        sc, rhs, pe, inner = manufactured(N)
+        d = device_scene(ctx, sc, capi_params({}, poisson_method=capi.POISSON_MGPCG,
+                                              poisson_tol=1e-10, precision=capi.PREC_MIXED))
+        p, info = d.poisson(rhs)
+        assert info["residual"] < 1e-10
+        errs.append(np.max(np.abs(p[inner] - pe[inner])))
+    assert errs[0] < 0.01 and errs[0] / errs[1] > 3.5
+
+
+def _workload_scene(grid, seed=2):
+    from ff_room import device
+    cfg = workload_c2(grid=grid, seed=seed)
+    return device.prepare(cfg)
+
+
+def test_mixed_pressure_error_meets_1e6(ctx):
+    """The headline criterion at a reduced c2 grid: the mixed solve's pressure
+    on a step's right-hand side is within relative L2 1e-6 (non-SOLID cells)
+    of FP64 CG converged to 1e-12."""
+    from ff_room import device
+    ps = _workload_scene((64, 52, 20))
+    p64 = capi.default_params(**{k: getattr(ps.params, k) for k, _ in capi.Params._fields_})
+    p64.precision, p64.poisson_method, p64.poisson_tol, p64.poisson_max_iter = 0, 1, 1e-12, 100000
+    d = device.create(ctx, ps, p64)
+    for _ in range(3):
+        d.step()
+    f = d.download()
+    b = d.laplacian(f["p"])  # a realistic right-hand side: A p of a step's pressure
+    b[ps.flags == SOLID] = 0.0
+    x_ref, _ = d.poisson(b)
+    pm = capi.default_params(**{k: getattr(ps.params, k) for k, _ in capi.Params._fields_})
+    pm.precision, pm.poisson_method, pm.poisson_tol, pm.poisson_max_iter = 1, 2, 1e-10, 500
+    dm = device.create(ctx, ps, pm)
+    x, info = dm.poisson(b)
+    m = ps.flags != SOLID
+    assert rel_l2(x[m], x_ref[m]) < 1e-6
+    assert info["residual"] < 1e-10
+
+
+@pytest.mark.parametrize("cfg,steps,tol", [(BASIC_ROOM, 20, 1e-3), (SMALL_THERMAL, 20, 1e-3)])
+def test_mixed_step_tracks_fp64_step(ctx, cfg, steps, tol):
+    """FP32 stencils + mixed pressure solve against the FP64 GPU step:
+    relative L2 of u, v, w, T after 20 steps within 1e-3 (FP32 rounding of the
+    state accumulating over the steps; the pressure solve itself is at 1e-10)."""
+    sc = build_scene(cfg)
+    params = solver_params(cfg)
+    params["convergence_tol"] = 0.0
+    o = OracleSolver(sc, params)
+    d64 = device_scene(ctx, sc, capi_params(params, poisson_tol=1e-10, poisson_max_iter=20000))
+    dmx = device_scene(ctx, sc, capi_params(params, precision=capi.PREC_MIXED, poisson_tol=1e-10,
+                                            poisson_max_iter=500))
+    for d in (d64, dmx):
+        d.upload(T=o.T)
+    for _ in range(steps):
+        r64, rmx = d64.step(), dmx.step()
+    f64, fmx = d64.download(), dmx.download()
+    for k in ("u", "v", "w", "T"):
+        assert rel_l2(fmx[k], f64[k]) < tol, (k, rel_l2(fmx[k], f64[k]))
+    nonsolid = sc.flags != SOLID
+    assert rel_l2(fmx["p"][nonsolid], f64["p"][nonsolid]) < 10 * tol
+    assert rmx["step"] == steps and not rmx["nonfinite"]
+
+
+def test_mixed_deterministic(ctx):
+    sc = build_scene(SMALL_THERMAL)
+    params = solver_params(SMALL_THERMAL)
+    outs = []
+    for _ in range(2):
+        d = device_scene(ctx, sc, capi_params(params, precision=capi.PREC_MIXED, poisson_tol=1e-10,
+                                              poisson_max_iter=500))
+        d.upload(T=np.full(sc.nx * sc.ny * sc.nz, params["T_initial"]))
+        rs = [d.step() for _ in range(4)]
+        outs.append((d.download(), rs))
+    for k in outs[0][0]:
+        assert np.array_equal(outs[0][0][k], outs[1][0][k]), k
+    assert outs[0][1] == outs[1][1]
