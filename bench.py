@@ -6,8 +6,9 @@ A step is one full projection time step (FluidSolver::step: predictor,
 buoyancy, masks, pressure solve, projection, temperature, diagnostics) of the
 named synthetic workload, with all state resident in HBM. Prints one JSON
 line: whole-job cell-updates/s of the timed steps plus
-  * roofline of the dominant kernel (algorithmic HBM bytes / mean launch time
-    from HIP events on the scene's stream, against 8 TB/s),
+  * roofline of the dominant kernel (algorithmic HBM bytes / mean kernel
+    duration from dispatch begin/end HIP events on the scene's stream, against
+    8 TB/s; HBM traffic from the committed rocprofv3 counter summary),
   * the plain FP64 CG form (the reference algorithm on the GPU) timed on the
     same workload, and the pressure error of each form against a tightly
     converged FP64 solve of the same right-hand side,
@@ -65,14 +66,39 @@ def timed_steps(d, ctx, steps: int, warmup: int, prof: bool):
     return t1 - t0, res, table
 
 
-def roofline(table: dict):
+TIMED_KERNELS = ("predict", "rhs", "cg_spmv", "cg_update", "cg_dir", "mg_smooth_l0",
+                 "mg_restrict_l0", "pcg_spmv", "pcg_update", "dc_residual")
+
+
+def traffic_from_profiles(kernel: str, workload: str):
+    """HBM-side bytes per launch measured with rocprofv3 --pmc (FETCH_SIZE +
+    WRITE_SIZE, calibrated on a copy kernel), when a summary is committed."""
+    path = os.path.join(ROOT, "profiles", f"pmc_{kernel}_{workload}.txt")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("traffic_bytes_per_launch:"):
+            return float(line.split(":", 1)[1])
+    return None
+
+
+def roofline(table: dict, d, workload: str):
+    """Dominant kernel by total time in the profiled steps; its mean duration
+    over 200 launches, each timed by its own dispatch begin/end events on the
+    scene's stream (the duration a kernel trace reports)."""
     name, e = max(table.items(), key=lambda kv: kv[1]["total_ms"])
-    mean_s = e["total_ms"] / e["launches"] / 1e3
-    ach = e["bytes_per_launch"] / mean_s
+    mean_events_s = e["total_ms"] / e["launches"] / 1e3
+    bytes_per = e["bytes_per_launch"]
+    mean_s, how = mean_events_s, "per-launch events during the steps"
+    if name in TIMED_KERNELS:
+        ms, b = d.prof_kernel(name, 200)
+        mean_s, bytes_per, how = ms / 1e3, b, "200 launches, dispatch begin/end events"
+    ach = bytes_per / mean_s
     return name, {"bound": "hbm", "achieved": ach / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-                  "frac": ach / HBM_PEAK, "traffic": None,
-                  "kernel": name, "bytes_per_launch": e["bytes_per_launch"],
-                  "mean_launch_us": mean_s * 1e6, "launches": e["launches"]}
+                  "frac": ach / HBM_PEAK, "traffic": traffic_from_profiles(name, workload),
+                  "kernel": name, "bytes_per_launch": bytes_per,
+                  "mean_launch_us": mean_s * 1e6, "timing": how,
+                  "launches_per_step": None}
 
 
 def form_params_inplace(p, mode):
@@ -134,10 +160,12 @@ def main():
     ap.add_argument("--fp64-steps", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-compare", action="store_true")
+    ap.add_argument("--kernel-only", default=None,
+                    help="time one kernel (for rocprofv3 --pmc runs) and exit")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py runs one GPU only; --gpus {args.gpus} is not supported: the workload is "
-              f"one scene per GPU and scenes shard by scene (see DESIGN.md)", file=sys.stderr)
+        print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported: scenes are "
+              f"independent and shard one scene (or one slab of a scene) per GPU", file=sys.stderr)
         sys.exit(2)
 
     from ff_room import capi, device
@@ -147,8 +175,21 @@ def main():
     ps = device.prepare(cfg)
     log(f"device {info['name']} {info['arch']} CUs={info['compute_units']}; workload {desc}")
 
-    # ---- headline form -------------------------------------------------------
     form_params_inplace(ps.params, args.mode)
+    if args.kernel_only:
+        # the named kernel, then the library's plain FP64 copy of nx*ny*nz
+        # doubles, whose memory counters calibrate the kernel's
+        d = device.create(ctx, ps)
+        for _ in range(2):
+            d.step()
+        for name in (args.kernel_only, "copy"):
+            ms, nbytes = d.prof_kernel(name, 20)
+            print(json.dumps({"kernel": name, "mean_us": ms * 1e3, "bytes": nbytes}), flush=True)
+        d.close()
+        ctx.close()
+        return
+
+    # ---- headline form -------------------------------------------------------
     d = device.create(ctx, ps)
     wall, res, _ = timed_steps(d, ctx, args.steps, args.warmup, prof=False)
     d.close()
@@ -160,8 +201,9 @@ def main():
     # profiled pass of the same steps for the per-kernel roofline
     d = device.create(ctx, ps)
     pwall, _, table = timed_steps(d, ctx, args.steps, args.warmup, prof=True)
+    kname, roof = roofline(table, d, args.workload)
+    roof["launches_per_step"] = table[kname]["launches"] / args.steps
     d.close()
-    kname, roof = roofline(table)
     log(f"dominant kernel {kname}: {roof['achieved']:.0f} GB/s ({100 * roof['frac']:.1f}% of peak)")
 
     out = {

@@ -18,6 +18,7 @@
 
 namespace ffr {
 unsigned g_max_blocks = 2048;
+thread_local ProfEvents g_prof_events;
 }
 
 using namespace ffr;
@@ -74,9 +75,10 @@ void ffr_params_default(ffr_params *p)
     p->poisson_method = FFR_POISSON_AUTO;
     p->mg_levels = 0;
     p->mg_bf16 = 1;
-    p->mg_pre_sweeps = 1;
+    p->mg_pre_sweeps = 2;
+    p->mg_level_sweeps = 2;
     p->mg_coarse_sweeps = 8;
-    p->inner_tol = 1e-4;
+    p->inner_tol = 1e-3;
     p->inner_max_iter = 60;
 }
 
@@ -163,15 +165,21 @@ void ffr_ctx::prof_begin(hipStream_t s, const char *name, double bytes)
     }
     pe.name = name;
     pe.bytes = bytes;
-    hipEventRecord(pe.a, s);
     pending.push_back(pe);
     open_event = true;
+    // the next launch on this thread records a/b at its dispatch begin/end
+    ffr::g_prof_events.start = pe.a;
+    ffr::g_prof_events.stop = pe.b;
 }
 
 void ffr_ctx::prof_end(hipStream_t s)
 {
     if (!prof_on || !open_event) return;
-    hipEventRecord(pending.back().b, s);
+    if (ffr::g_prof_events.start) {  // the scope launched nothing: record an empty interval
+        ffr::g_prof_events = ffr::ProfEvents{};
+        hipEventRecord(pending.back().a, s);
+        hipEventRecord(pending.back().b, s);
+    }
     open_event = false;
     if (pending.size() > 4096) prof_resolve();
 }

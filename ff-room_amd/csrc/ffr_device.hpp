@@ -7,6 +7,7 @@
 // +-ny*nz come from L2; launch blocks are remapped so that each XCD (own L2)
 // walks one contiguous eighth of an array (see xcd_block()).
 #pragma once
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -53,6 +54,23 @@ struct OpeningDev {  // OpeningBC (BoundaryManager.hpp:27-29)
     int offset;      // first thread of this patch in the launch
 };
 
+// Per-cell neighbour code of the pressure operator (built once per scene):
+//   bit 0      the cell is an unknown (FLUID or INFLOW)
+//   bits 1..6  neighbour -x,+x,-y,+y,-z,+z exists and is an unknown (couples)
+//   bits 7..12 that neighbour is OUTFLOW (adds 2*scale to the diagonal)
+//   bit 13     red-black colour (i+j+k) & 1
+//   bit 14     the cell is OUTFLOW, bit 15 the cell is SOLID
+// Walls and SOLID neighbours set neither bit (Neumann), exactly the cases of
+// PoissonSolver.cpp:52-87.
+enum : uint16_t {
+    NB_SELF = 1u << 0,
+    NB_UNK0 = 1u << 1,
+    NB_OUT0 = 1u << 7,
+    NB_COLOR = 1u << 13,
+    NB_SELF_OUT = 1u << 14,
+    NB_SELF_SOLID = 1u << 15
+};
+
 // Geometry of one scene as the kernels see it (passed by value).
 struct Geo {
     int nx, ny, nz;
@@ -67,6 +85,10 @@ struct Geo {
     // FP32 copies
     const float *hxf, *hyf, *hzf, *ihxf, *ihyf, *ihzf, *i2xf, *i2yf, *i2zf;
     const float *cxf, *cyf, *czf, *lapf;
+    const uint16_t *nbm;               // neighbour codes (see NB_*)
+    double sxd, syd, szd;              // uniform-grid coupling 1/h^2 per axis (0 if not uniform)
+    float sx, sy, sz;
+    int uniform;
     const FanDev *fans;
     int n_fans;
     const uint8_t *wall_of;            // outflow-patch marks per boundary face
@@ -206,4 +228,24 @@ __device__ __forceinline__ double sum_partials(const double *partials, uint32_t 
     return r;
 }
 
+// Profiling hook: when a profiler scope has armed a start/stop event pair,
+// the next launch records them at the dispatch's own begin and end
+// (hipExtLaunchKernelGGL), i.e. the kernel's execution time without the
+// inter-kernel gap that events placed around a launch would include.
+struct ProfEvents {
+    hipEvent_t start = nullptr, stop = nullptr;
+};
+extern thread_local ProfEvents g_prof_events;
+
 }  // namespace ffr
+
+#define FFR_LAUNCH(kernel, grid, block, shm, stream, ...)                                   \
+    do {                                                                                    \
+        if (ffr::g_prof_events.start) {                                                     \
+            hipEvent_t a_ = ffr::g_prof_events.start, b_ = ffr::g_prof_events.stop;        \
+            ffr::g_prof_events = ffr::ProfEvents{};                                         \
+            hipExtLaunchKernelGGL(kernel, grid, block, shm, stream, a_, b_, 0, __VA_ARGS__); \
+        } else {                                                                            \
+            hipLaunchKernelGGL(kernel, grid, block, shm, stream, __VA_ARGS__);              \
+        }                                                                                   \
+    } while (0)

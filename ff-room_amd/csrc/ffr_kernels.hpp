@@ -82,6 +82,9 @@ void launch_convert(hipStream_t s, const double *src, R *dst, uint64_t n);
 template <typename R>
 void launch_convert_back(hipStream_t s, const R *src, double *dst, uint64_t n);
 
+// plain FP64 copy (calibration of memory counters)
+void launch_copy64(hipStream_t s, const double *src, double *dst, uint64_t n);
+
 // ---- reference CG in FP64 (cg_kernels.hip) -----------------------------
 // b -> (x = 0, r = d = b with SOLID rows zeroed), ||b||; cg->done set when
 // ||b|| < 1e-15 or max_iter == 0.

@@ -123,6 +123,7 @@ int mg_create(ffr_scene *s)
         L.f_nynz = make_fastdiv(L.ny * L.nz);
     }
     M.pre_sweeps = std::max(1, s->prm.mg_pre_sweeps);
+    M.level_sweeps = std::max(1, s->prm.mg_level_sweeps);
     M.coarse_sweeps = std::max(1, s->prm.mg_coarse_sweeps);
     hipStream_t str = s->stream;
     HIP_CHECK_CTX(s->ctx, hipMemsetAsync(M.arena, 0, bytes, str), FFR_E_DEVICE);
@@ -143,6 +144,17 @@ void mg_destroy(ffr_scene *s)
 }
 
 // ---- V-cycle enqueue (z_0 = M r_0 on level 0) --------------------------------
+// kernel names per level for the profiler (bytes differ per level)
+static const char *kSmoothName[kMaxLevels] = {"mg_smooth_l0", "mg_smooth_l1", "mg_smooth_l2",
+                                              "mg_smooth_l3", "mg_smooth_l4", "mg_smooth_l5",
+                                              "mg_smooth_l6", "mg_smooth_l7"};
+static const char *kRestrictName[kMaxLevels] = {"mg_restrict_l0", "mg_restrict_l1", "mg_restrict_l2",
+                                                "mg_restrict_l3", "mg_restrict_l4", "mg_restrict_l5",
+                                                "mg_restrict_l6", "mg_restrict_l7"};
+static const char *kProlongName[kMaxLevels] = {"mg_prolong_l0", "mg_prolong_l1", "mg_prolong_l2",
+                                               "mg_prolong_l3", "mg_prolong_l4", "mg_prolong_l5",
+                                               "mg_prolong_l6", "mg_prolong_l7"};
+
 template <typename TC>
 static void vcycle(ffr_scene *s, int l)
 {
@@ -159,39 +171,44 @@ static void vcycle(ffr_scene *s, int l)
         else mg_launch_coarse<TC>(st, g, L, false, M.coarse_sweeps, m);
         return;
     }
-    // algorithmic bytes per level-0 pass: z read, r read, flags, z write
-    const double bsm = l0 ? n * (4 + 4 + 1 + 4) : n * (2 * sizeof(TC) + 1 + 20 + sizeof(TC));
-    for (int sw = 0; sw < M.pre_sweeps; sw++) {
+    // algorithmic bytes of one colour pass: z and r read, the code (level 0,
+    // 2 B) or operator (coarse: active + 3 couplings + rdiag), z written
+    const double bsm = l0 ? n * (4 + 4 + 2 + 4) : n * (2 * sizeof(TC) + 1 + 16 + sizeof(TC));
+    const double bres = l0 ? n * (4 + 4 + 2) + M.lv[l + 1].n * (double)sizeof(TC)
+                           : n * (2 * sizeof(TC) + 1 + 16) + M.lv[l + 1].n * (double)sizeof(TC);
+    const double bpro = n * (2.0 * (l0 ? 4 : sizeof(TC)) + (l0 ? 2 : 1)) + M.lv[l + 1].n * (double)sizeof(TC);
+    const int sweeps = l0 ? M.pre_sweeps : M.level_sweeps;
+    for (int sw = 0; sw < sweeps; sw++) {
         {
-            ProfScope ps(s->ctx, st, l0 ? "mg_smooth_l0" : "mg_smooth", bsm);
+            ProfScope ps(s->ctx, st, kSmoothName[l], bsm);
             if (l0) mg_launch_smooth<float>(st, g, L, true, 0, sw == 0, m);
             else mg_launch_smooth<TC>(st, g, L, false, 0, sw == 0, m);
         }
         {
-            ProfScope ps(s->ctx, st, l0 ? "mg_smooth_l0" : "mg_smooth", bsm);
+            ProfScope ps(s->ctx, st, kSmoothName[l], bsm);
             if (l0) mg_launch_smooth<float>(st, g, L, true, 1, 0, m);
             else mg_launch_smooth<TC>(st, g, L, false, 1, 0, m);
         }
     }
     {
-        ProfScope ps(s->ctx, st, l0 ? "mg_restrict_l0" : "mg_restrict", 0);
+        ProfScope ps(s->ctx, st, kRestrictName[l], bres);
         if (l0) mg_launch_restrict<float, TC>(st, g, L, M.lv[l + 1], true, m);
         else mg_launch_restrict<TC, TC>(st, g, L, M.lv[l + 1], false, m);
     }
     vcycle<TC>(s, l + 1);
     {
-        ProfScope ps(s->ctx, st, l0 ? "mg_prolong_l0" : "mg_prolong", 0);
+        ProfScope ps(s->ctx, st, kProlongName[l], bpro);
         if (l0) mg_launch_prolong<float, TC>(st, g, L, M.lv[l + 1], true, m);
         else mg_launch_prolong<TC, TC>(st, g, L, M.lv[l + 1], false, m);
     }
-    for (int sw = 0; sw < M.pre_sweeps; sw++) {
+    for (int sw = 0; sw < sweeps; sw++) {
         {
-            ProfScope ps(s->ctx, st, l0 ? "mg_smooth_l0" : "mg_smooth", bsm);
+            ProfScope ps(s->ctx, st, kSmoothName[l], bsm);
             if (l0) mg_launch_smooth<float>(st, g, L, true, 1, 0, m);
             else mg_launch_smooth<TC>(st, g, L, false, 1, 0, m);
         }
         {
-            ProfScope ps(s->ctx, st, l0 ? "mg_smooth_l0" : "mg_smooth", bsm);
+            ProfScope ps(s->ctx, st, kSmoothName[l], bsm);
             if (l0) mg_launch_smooth<float>(st, g, L, true, 0, 0, m);
             else mg_launch_smooth<TC>(st, g, L, false, 0, 0, m);
         }
@@ -290,6 +307,50 @@ int mg_solve(ffr_scene *s, int *iters, int *outer, double *res)
     *outer = passes;
     *res = h->res;
     return FFR_OK;
+}
+
+double mg_prof_launch(ffr_scene *s, const char *name, std::function<void()> *launch)
+{
+    Multigrid &M = s->mg;
+    const Geo g = s->geo;
+    hipStream_t st = s->stream;
+    MgState *m = s->mgs;
+    const double n = (double)g.nc;
+    const std::string k(name);
+    const MgLevel L0 = M.lv[0];
+    if (k == "mg_smooth_l0") {
+        *launch = [=] { mg_launch_smooth<float>(st, g, L0, true, 1, 0, m); };
+        return n * (4 + 4 + 2 + 4);
+    }
+    if (k == "mg_restrict_l0" && M.levels > 1) {
+        const MgLevel L1 = M.lv[1];
+        const bool bf = L1.bf16;
+        const double nc1 = (double)L1.n * (bf ? 2 : 4);
+        if (bf) *launch = [=] { mg_launch_restrict<float, __hip_bfloat16>(st, g, L0, L1, true, m); };
+        else *launch = [=] { mg_launch_restrict<float, float>(st, g, L0, L1, true, m); };
+        return n * (4 + 4 + 2) + nc1;
+    }
+    if (k == "pcg_spmv") {
+        float *p = M.pi, *q = M.qi;
+        RedScratch red = s->red;
+        *launch = [=] { pcg_launch_spmv(st, g, p, q, m, red); };
+        return n * (4 + 2 + 4);
+    }
+    if (k == "pcg_update") {
+        float *e = M.e, *r = M.ri, *p = M.pi, *q = M.qi;
+        RedScratch red = s->red;
+        *launch = [=] { pcg_launch_update(st, g.nc, e, r, p, q, m, red, 0); };
+        return n * 24;
+    }
+    if (k == "dc_residual") {
+        double *b = s->b, *x0 = s->p, *x1 = s->d;
+        float *e = M.e, *r = M.ri;
+        RedScratch red = s->red;
+        *launch = [=] { dc_launch_residual(st, g, b, x0, e, x1, r, m, red); };
+        return n * (8 + 8 + 4 + 2 + 8 + 4);
+    }
+    *launch = [] {};
+    return 0.0;
 }
 
 }  // namespace ffr

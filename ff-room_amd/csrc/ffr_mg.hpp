@@ -55,7 +55,7 @@ struct Multigrid {
     size_t bytes = 0;
     // level-0 vectors of the inner PCG (FP32) and the outer correction
     float *e, *ri, *pi, *qi;
-    int pre_sweeps = 1, coarse_sweeps = 8;
+    int pre_sweeps = 2, level_sweeps = 2, coarse_sweeps = 8;
     int inner_pred[16] = {0};  // inner iterations seen per outer pass (launch prediction)
 };
 

@@ -4,7 +4,9 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <functional>
 #include <new>
+#include <string>
 #include <vector>
 
 using namespace ffr;
@@ -84,6 +86,7 @@ bool valid_params(const ffr_params *p, char *err, size_t errn)
 
 struct ArenaPtrs {
     uint8_t *flags;
+    uint16_t *nbm;
     double *ax64;
     float *ax32;
     FanDev *fans;
@@ -112,6 +115,7 @@ size_t layout(char *base, const ffr_scene_desc *d, const ffr_params *p, ArenaPtr
     const size_t s = (p->precision == FFR_PREC_MIXED) ? sizeof(float) : sizeof(double);
     ArenaPtrs a{};
     a.flags = (uint8_t *)take(nc);
+    a.nbm = (uint16_t *)take(2 * nc);
     a.ax64 = (double *)take(sizeof(double) * 8 * na);
     a.ax32 = (float *)take(sizeof(float) * 10 * na);
     a.fans = (FanDev *)take(sizeof(FanDev) * std::max(1, d->n_fans));
@@ -317,9 +321,36 @@ int ffr_scene_create(ffr_ctx *ctx, const ffr_scene_desc *d, const ffr_params *pr
                 if (s->flags[c] == FLUID) s->opening_touches_fluid = true;
             }
 
+    // neighbour codes of the pressure operator
+    std::vector<uint16_t> nbm(s->nc);
+    {
+        const uint8_t *f = s->flags.data();
+        auto unk = [](uint8_t t) { return t == FLUID || t == INFLOW; };
+        for (int i = 0; i < nx; i++)
+            for (int j = 0; j < ny; j++)
+                for (int k = 0; k < nz; k++) {
+                    const long c = ((long)i * ny + j) * nz + k;
+                    uint16_t m = 0;
+                    if (unk(f[c])) m |= NB_SELF;
+                    if (f[c] == OUTFLOW) m |= NB_SELF_OUT;
+                    if (f[c] == SOLID) m |= NB_SELF_SOLID;
+                    if ((i + j + k) & 1) m |= NB_COLOR;
+                    const long nb[6] = {i > 0 ? c - (long)ny * nz : -1, i < nx - 1 ? c + (long)ny * nz : -1,
+                                        j > 0 ? c - nz : -1, j < ny - 1 ? c + nz : -1,
+                                        k > 0 ? c - 1 : -1, k < nz - 1 ? c + 1 : -1};
+                    for (int q = 0; q < 6; q++) {
+                        if (nb[q] < 0) continue;
+                        if (unk(f[nb[q]])) m |= (uint16_t)(NB_UNK0 << q);
+                        else if (f[nb[q]] == OUTFLOW) m |= (uint16_t)(NB_OUT0 << q);
+                    }
+                    nbm[c] = m;
+                }
+    }
+
     hipStream_t st = s->stream;
     bool ok = true;
     ok &= hipMemcpyAsync(a.flags, s->flags.data(), s->nc, hipMemcpyHostToDevice, st) == hipSuccess;
+    ok &= hipMemcpyAsync(a.nbm, nbm.data(), s->nc * 2, hipMemcpyHostToDevice, st) == hipSuccess;
     ok &= hipMemcpyAsync(a.ax64, ax64.data(), ax64.size() * 8, hipMemcpyHostToDevice, st) == hipSuccess;
     ok &= hipMemcpyAsync(a.ax32, ax32.data(), ax32.size() * 4, hipMemcpyHostToDevice, st) == hipSuccess;
     ok &= hipMemcpyAsync(a.fans, fans.data(), fans.size() * sizeof(FanDev), hipMemcpyHostToDevice, st) == hipSuccess;
@@ -347,6 +378,22 @@ int ffr_scene_create(ffr_ctx *ctx, const ffr_scene_desc *d, const ffr_params *pr
     g.f_ny1nz = make_fastdiv((ny + 1) * nz);
     g.f_nynz1 = make_fastdiv(ny * (nz + 1));
     g.flags = a.flags;
+    g.nbm = a.nbm;
+    {
+        auto uni = [](const std::vector<double> &h) {
+            for (double x : h)
+                if (std::fabs(x - h[0]) > 1e-12 * std::fabs(h[0])) return false;
+            return true;
+        };
+        g.uniform = uni(s->dxv) && uni(s->dyv) && uni(s->dzv);
+        // 1/(h*h) is what PoissonSolver.cpp:34-49 gives every face of a uniform axis
+        g.sxd = g.uniform ? 1.0 / (s->dxv[0] * s->dxv[0]) : 0.0;
+        g.syd = g.uniform ? 1.0 / (s->dyv[0] * s->dyv[0]) : 0.0;
+        g.szd = g.uniform ? 1.0 / (s->dzv[0] * s->dzv[0]) : 0.0;
+        g.sx = (float)g.sxd;
+        g.sy = (float)g.syd;
+        g.sz = (float)g.szd;
+    }
     {
         const double *b64 = a.ax64;
         g.hx = b64; g.hy = b64 + nx; g.hz = b64 + nx + ny; b64 += na;
@@ -679,6 +726,69 @@ int ffr_apply_laplacian(ffr_scene *s, const double *x, double *Ax)
     launch_laplacian(st, s->geo, s->d, s->Ad);
     HIP_CHECK_CTX(s->ctx, hipMemcpyAsync(Ax, s->Ad, s->nc * 8, hipMemcpyDeviceToHost, st), FFR_E_DEVICE);
     HIP_CHECK_CTX(s->ctx, hipStreamSynchronize(st), FFR_E_DEVICE);
+    return FFR_OK;
+}
+
+int ffr_prof_kernel(ffr_scene *s, const char *name, int reps, double *mean_ms, double *bytes)
+{
+    if (!s || !name || reps <= 0) return FFR_E_INVALID;
+    hipSetDevice(s->ctx->device);
+    hipStream_t st = s->stream;
+    const Geo &g = s->geo;
+    const double nc = (double)s->nc, faces = (double)(s->nu + s->nv + s->nw);
+    const double sz = s->fp64 ? 8.0 : 4.0;
+    const StepParams sp = make_step_params(s->prm);
+    std::string k(name);
+    std::function<void()> launch;
+    double b = 0.0;
+    const bool mg = s->mg.levels > 0;
+    if (k == "predict") {
+        b = faces * 2 * sz + nc + (s->prm.buoyancy ? nc * sz : 0);
+        if (s->fp64)
+            launch = [&] { launch_predict<double>(st, g, sp, (double *)s->u, (double *)s->v, (double *)s->w, (double *)s->T, (double *)s->us, (double *)s->vs, (double *)s->ws, s->rec); };
+        else
+            launch = [&] { launch_predict<float>(st, g, sp, (float *)s->u, (float *)s->v, (float *)s->w, (float *)s->T, (float *)s->us, (float *)s->vs, (float *)s->ws, s->rec); };
+    } else if (k == "rhs") {
+        b = faces * sz + nc + nc * 8;
+        if (s->fp64) launch = [&] { launch_rhs<double>(st, g, sp, (double *)s->us, (double *)s->vs, (double *)s->ws, s->b); };
+        else launch = [&] { launch_rhs<float>(st, g, sp, (float *)s->us, (float *)s->vs, (float *)s->ws, s->b); };
+    } else if (k == "cg_spmv" || k == "cg_update" || k == "cg_dir") {
+        // keep the CG state live for the timed launches
+        HIP_CHECK_CTX(s->ctx, hipMemsetAsync(&s->cg->done, 0, sizeof(int), st), FFR_E_DEVICE);
+        if (k == "cg_spmv") { b = nc * 17; launch = [&] { launch_cg_spmv(st, g, s->d, s->Ad, s->cg, s->red); }; }
+        else if (k == "cg_update") { b = nc * 48; launch = [&] { launch_cg_update(st, g.nc, s->p, s->r, s->d, s->Ad, s->cg, s->red, 0); }; }
+        else { b = nc * 24; launch = [&] { launch_cg_dir(st, g.nc, s->d, s->r, s->cg, s->red, 0); }; }
+    } else if (k == "copy") {  // plain FP64 copy b -> r (counter calibration)
+        b = nc * 16;
+        launch = [&] { launch_copy64(st, s->b, s->r, s->nc); };
+    } else if (mg && (k == "mg_smooth_l0" || k == "mg_restrict_l0" || k == "pcg_spmv" ||
+                      k == "pcg_update" || k == "dc_residual")) {
+        HIP_CHECK_CTX(s->ctx, hipMemsetAsync(&s->mgs->solve_done, 0, 2 * sizeof(int), st), FFR_E_DEVICE);
+        b = mg_prof_launch(s, k.c_str(), &launch);
+    } else {
+        s->ctx->fail("ffr_prof_kernel: unknown kernel '%s' for this scene", name);
+        return FFR_E_INVALID;
+    }
+    // each launch timed by its own dispatch begin/end events
+    std::vector<hipEvent_t> ev(2 * (size_t)reps);
+    for (auto &e : ev) HIP_CHECK_CTX(s->ctx, hipEventCreate(&e), FFR_E_DEVICE);
+    launch();  // warm
+    for (int r = 0; r < reps; r++) {
+        g_prof_events.start = ev[2 * r];
+        g_prof_events.stop = ev[2 * r + 1];
+        launch();
+        g_prof_events = ProfEvents{};
+    }
+    HIP_CHECK_CTX(s->ctx, hipStreamSynchronize(st), FFR_E_DEVICE);
+    double total = 0.0;
+    for (int r = 0; r < reps; r++) {
+        float ms = 0.f;
+        hipEventElapsedTime(&ms, ev[2 * r], ev[2 * r + 1]);
+        total += ms;
+    }
+    for (auto &e : ev) hipEventDestroy(e);
+    if (mean_ms) *mean_ms = total / reps;
+    if (bytes) *bytes = b;
     return FFR_OK;
 }
 

@@ -2,6 +2,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <functional>
 #include <map>
 #include <memory>
 #include <string>
@@ -121,4 +122,7 @@ size_t mg_bytes(int nx, int ny, int nz, const ffr_params &p);
 int mg_create(ffr_scene *s);
 void mg_destroy(ffr_scene *s);
 int mg_solve(ffr_scene *s, int *iters, int *outer, double *res);
+// sets *launch to one launch of a level-0 multigrid / inner-PCG / outer kernel
+// on the scene's buffers and returns its algorithmic bytes
+double mg_prof_launch(ffr_scene *s, const char *name, std::function<void()> *launch);
 }  // namespace ffr

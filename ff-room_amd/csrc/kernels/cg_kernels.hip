@@ -10,62 +10,62 @@
 
 namespace ffr {
 
-// Row c of A (PoissonSolver.cpp:52-87): neighbours in the reference order
-// -x,+x,-y,+y,-z,+z; OUTFLOW adds 2*scale to the diagonal, SOLID and the
-// domain walls add nothing, other cells couple with -scale.
-__device__ __forceinline__ double lap_row(const Geo &g, const double *x, int i, int j, int k,
-                                          uint32_t c)
+// Row c of A (PoissonSolver.cpp:52-87) from the cell's neighbour code:
+// faces in the reference order -x,+x,-y,+y,-z,+z; an unknown neighbour
+// couples with -scale, an OUTFLOW neighbour adds 2*scale to the diagonal,
+// walls and SOLID add nothing. UNI: every face of an axis has the same
+// scale 1/h^2 (uniform spacing), so no index decode is needed.
+template <bool UNI>
+__device__ __forceinline__ double lap_row(const Geo &g, const double *__restrict__ x, uint32_t c,
+                                          uint16_t m)
 {
-    const uint8_t *f = g.flags;
-    const int nx = g.nx, ny = g.ny, nz = g.nz;
-    const uint32_t si = (uint32_t)ny * nz, sj = nz;
-    const double *L = g.lap;
-    double diag = 0.0, off = 0.0;
-#define FFR_FACE(cond, nc, sc)                          \
-    if (cond) {                                         \
-        const uint8_t t = f[nc];                        \
-        const double s_ = (sc);                         \
-        if (t == OUTFLOW) diag += 2.0 * s_;             \
-        else if (t != SOLID) { diag += s_; off += s_ * x[nc]; } \
+    const uint32_t si = (uint32_t)g.ny * g.nz, sj = g.nz;
+    double sxl, sxr, syl, syr, szl, szr;
+    if (UNI) {
+        sxl = sxr = g.sxd;
+        syl = syr = g.syd;
+        szl = szr = g.szd;
+    } else {
+        const uint32_t i = fdiv(c, g.f_nynz), rem = c - i * si;
+        const uint32_t j = fdiv(rem, g.f_nz), k = rem - j * sj;
+        const double *L = g.lap;
+        sxl = L[i]; sxr = L[g.nx + i];
+        syl = L[2 * g.nx + j]; syr = L[2 * g.nx + g.ny + j];
+        szl = L[2 * g.nx + 2 * g.ny + k]; szr = L[2 * g.nx + 2 * g.ny + g.nz + k];
     }
-    FFR_FACE(i > 0, c - si, L[i])
-    FFR_FACE(i < nx - 1, c + si, L[nx + i])
-    FFR_FACE(j > 0, c - sj, L[2 * nx + j])
-    FFR_FACE(j < ny - 1, c + sj, L[2 * nx + ny + j])
-    FFR_FACE(k > 0, c - 1, L[2 * nx + 2 * ny + k])
-    FFR_FACE(k < nz - 1, c + 1, L[2 * nx + 2 * ny + nz + k])
+    double diag = 0.0, off = 0.0;
+#define FFR_FACE(q, nc, sc)                                                   \
+    if (m & (NB_UNK0 << q)) { diag += (sc); off += (sc) * x[nc]; }           \
+    else if (m & (NB_OUT0 << q)) diag += 2.0 * (sc);
+    FFR_FACE(0, c - si, sxl)
+    FFR_FACE(1, c + si, sxr)
+    FFR_FACE(2, c - sj, syl)
+    FFR_FACE(3, c + sj, syr)
+    FFR_FACE(4, c - 1, szl)
+    FFR_FACE(5, c + 1, szr)
 #undef FFR_FACE
     if (diag < 1e-30) diag = 1.0;
     return diag * x[c] - off;
 }
 
-__device__ __forceinline__ void decode_cell(const Geo &g, uint32_t c, int &i, int &j, int &k)
-{
-    const uint32_t ii = fdiv(c, g.f_nynz), rem = c - ii * (g.ny * g.nz);
-    const uint32_t jj = fdiv(rem, g.f_nz);
-    i = ii;
-    j = jj;
-    k = rem - jj * g.nz;
-}
-
-__global__ void __launch_bounds__(kBlock) k_laplacian(Geo g, const double *x, double *Ax)
+template <bool UNI>
+__global__ void __launch_bounds__(kBlock) k_laplacian(Geo g, const double *__restrict__ x,
+                                                      double *__restrict__ Ax)
 {
     uint32_t sb, se;
     block_span(g.nc, xcd_block(), sb, se);
     for (uint32_t c = sb + threadIdx.x; c < se; c += kBlock) {
-        if (g.flags[c] == SOLID) {
-            Ax[c] = x[c];
-            continue;
-        }
-        int i, j, k;
-        decode_cell(g, c, i, j, k);
-        Ax[c] = lap_row(g, x, i, j, k, c);
+        const uint16_t m = g.nbm[c];
+        Ax[c] = (m & NB_SELF_SOLID) ? x[c] : lap_row<UNI>(g, x, c, m);
     }
 }
 
 void launch_laplacian(hipStream_t s, const Geo &g, const double *x, double *Ax)
 {
-    hipLaunchKernelGGL(k_laplacian, dim3(grid_for(g.nc)), dim3(kBlock), 0, s, g, x, Ax);
+    if (g.uniform)
+        FFR_LAUNCH(k_laplacian<true>, dim3(grid_for(g.nc)), dim3(kBlock), 0, s, g, x, Ax);
+    else
+        FFR_LAUNCH(k_laplacian<false>, dim3(grid_for(g.nc)), dim3(kBlock), 0, s, g, x, Ax);
 }
 
 __device__ __forceinline__ double *slot(RedScratch red, int k) { return red.partials + k * kMaxBlocksCap; }
@@ -120,13 +120,15 @@ void launch_cg_init(hipStream_t s, const Geo &g, const double *b, double *x, dou
                     double *d, CgState *cg, RedScratch red, int max_iter, double tol)
 {
     const unsigned G = grid_for(g.nc);
-    hipLaunchKernelGGL(k_cg_init, dim3(G), dim3(kBlock), 0, s, g, b, x, r, d, red);
-    hipLaunchKernelGGL(k_cg_start, dim3(1), dim3(kBlock), 0, s, cg, red, G, max_iter, tol);
+    FFR_LAUNCH(k_cg_init, dim3(G), dim3(kBlock), 0, s, g, b, x, r, d, red);
+    FFR_LAUNCH(k_cg_start, dim3(1), dim3(kBlock), 0, s, cg, red, G, max_iter, tol);
 }
 
 // Ad = A d and partials of d.Ad (:120-126)
-__global__ void __launch_bounds__(kBlock) k_cg_spmv(Geo g, const double *d, double *Ad,
-                                                    CgState *cg, RedScratch red)
+template <bool UNI>
+__global__ void __launch_bounds__(kBlock) k_cg_spmv(Geo g, const double *__restrict__ d,
+                                                    double *__restrict__ Ad, CgState *cg,
+                                                    RedScratch red)
 {
     __shared__ double lds[kBlock / kWave];
     if (cg->done) return;
@@ -135,15 +137,9 @@ __global__ void __launch_bounds__(kBlock) k_cg_spmv(Geo g, const double *d, doub
     uint32_t sb, se;
     block_span(g.nc, blk, sb, se);
     for (uint32_t c = sb + threadIdx.x; c < se; c += kBlock) {
-        double a;
+        const uint16_t m = g.nbm[c];
         const double dc = d[c];
-        if (g.flags[c] == SOLID) {
-            a = dc;
-        } else {
-            int i, j, k;
-            decode_cell(g, c, i, j, k);
-            a = lap_row(g, d, i, j, k, c);
-        }
+        const double a = (m & NB_SELF_SOLID) ? dc : lap_row<UNI>(g, d, c, m);
         Ad[c] = a;
         dq += dc * a;
     }
@@ -153,13 +149,18 @@ __global__ void __launch_bounds__(kBlock) k_cg_spmv(Geo g, const double *d, doub
 void launch_cg_spmv(hipStream_t s, const Geo &g, const double *d, double *Ad, CgState *cg,
                     RedScratch red)
 {
-    hipLaunchKernelGGL(k_cg_spmv, dim3(grid_for(g.nc)), dim3(kBlock), 0, s, g, d, Ad, cg, red);
+    if (g.uniform)
+        FFR_LAUNCH(k_cg_spmv<true>, dim3(grid_for(g.nc)), dim3(kBlock), 0, s, g, d, Ad, cg, red);
+    else
+        FFR_LAUNCH(k_cg_spmv<false>, dim3(grid_for(g.nc)), dim3(kBlock), 0, s, g, d, Ad, cg, red);
 }
 
 // alpha = rr / d.Ad (stop when |d.Ad| < 1e-30, :127-129); x += alpha d,
 // r -= alpha Ad and partials of r.r (:130-138)
-__global__ void __launch_bounds__(kBlock) k_cg_update(uint32_t n, double *x, double *r,
-                                                      const double *d, const double *Ad,
+__global__ void __launch_bounds__(kBlock) k_cg_update(uint32_t n, double *__restrict__ x,
+                                                      double *__restrict__ r,
+                                                      const double *__restrict__ d,
+                                                      const double *__restrict__ Ad,
                                                       CgState *cg, RedScratch red, int it)
 {
     __shared__ double lds[kBlock / kWave];
@@ -191,12 +192,13 @@ __global__ void __launch_bounds__(kBlock) k_cg_update(uint32_t n, double *x, dou
 void launch_cg_update(hipStream_t s, uint32_t n, double *x, double *r, const double *d,
                       const double *Ad, CgState *cg, RedScratch red, int it)
 {
-    hipLaunchKernelGGL(k_cg_update, dim3(grid_for(n)), dim3(kBlock), 0, s, n, x, r, d, Ad, cg,
+    FFR_LAUNCH(k_cg_update, dim3(grid_for(n)), dim3(kBlock), 0, s, n, x, r, d, Ad, cg,
                        red, it);
 }
 
 // res = sqrt(rr_new)/||b|| and the stopping test (:140-145); beta, d = r + beta d (:147-152)
-__global__ void __launch_bounds__(kBlock) k_cg_dir(uint32_t n, double *d, const double *r,
+__global__ void __launch_bounds__(kBlock) k_cg_dir(uint32_t n, double *__restrict__ d,
+                                                   const double *__restrict__ r,
                                                    CgState *cg, RedScratch red, int it)
 {
     __shared__ double lds[kBlock / kWave];
@@ -230,7 +232,7 @@ __global__ void __launch_bounds__(kBlock) k_cg_dir(uint32_t n, double *d, const 
 void launch_cg_dir(hipStream_t s, uint32_t n, double *d, const double *r, CgState *cg,
                    RedScratch red, int it)
 {
-    hipLaunchKernelGGL(k_cg_dir, dim3(grid_for(n)), dim3(kBlock), 0, s, n, d, r, cg, red, it);
+    FFR_LAUNCH(k_cg_dir, dim3(grid_for(n)), dim3(kBlock), 0, s, n, d, r, cg, red, it);
 }
 
 }  // namespace ffr

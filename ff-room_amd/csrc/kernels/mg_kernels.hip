@@ -30,58 +30,29 @@ __device__ __forceinline__ void decode(uint32_t c, const FastDiv &fnynz, const F
     k = rem - jj * nz;
 }
 
-// ---- level-0 operator in FP32: the reference row (PoissonSolver.cpp:52-87)
-// on an unknown cell; returns diag and the neighbour sum.
-template <typename T>
-__device__ __forceinline__ void row0(const Geo &g, const T *x, int i, int j, int k, uint32_t c,
-                                     float &diag, float &off)
+// ---- level-0 operator in FP32 from the neighbour code (uniform spacing):
+// the reference row (PoissonSolver.cpp:52-87) of an unknown cell.
+__device__ __forceinline__ float diag0m(const Geo &g, uint16_t m)
 {
-    const uint8_t *f = g.flags;
-    const float *L = g.lapf;
-    const int nx = g.nx, ny = g.ny, nz = g.nz;
-    const uint32_t si = (uint32_t)ny * nz, sj = nz;
-    float dg = 0.f, of = 0.f;
-#define FFR_NB(cond, nc, sc)                                    \
-    if (cond) {                                                 \
-        const uint8_t t = f[nc];                                \
-        const float s_ = (sc);                                  \
-        if (t == OUTFLOW) dg += 2.f * s_;                       \
-        else if (t != SOLID) { dg += s_; of += s_ * ldv(x, nc); } \
-    }
-    FFR_NB(i > 0, c - si, L[i])
-    FFR_NB(i < nx - 1, c + si, L[nx + i])
-    FFR_NB(j > 0, c - sj, L[2 * nx + j])
-    FFR_NB(j < ny - 1, c + sj, L[2 * nx + ny + j])
-    FFR_NB(k > 0, c - 1, L[2 * nx + 2 * ny + k])
-    FFR_NB(k < nz - 1, c + 1, L[2 * nx + 2 * ny + nz + k])
-#undef FFR_NB
-    if (dg < 1e-30f) dg = 1.f;
-    diag = dg;
-    off = of;
+    const float cx = (float)(__popc(m & (3u << 1)) + 2 * __popc(m & (3u << 7)));
+    const float cy = (float)(__popc(m & (3u << 3)) + 2 * __popc(m & (3u << 9)));
+    const float cz = (float)(__popc(m & (3u << 5)) + 2 * __popc(m & (3u << 11)));
+    const float d = g.sx * cx + g.sy * cy + g.sz * cz;
+    return d < 1e-30f ? 1.f : d;
 }
 
-// diagonal of the level-0 row alone
-__device__ __forceinline__ float diag0(const Geo &g, int i, int j, int k, uint32_t c)
+template <typename T>
+__device__ __forceinline__ float off0m(const Geo &g, const T *__restrict__ x, uint32_t c, uint16_t m)
 {
-    const uint8_t *f = g.flags;
-    const float *L = g.lapf;
-    const int nx = g.nx, ny = g.ny, nz = g.nz;
-    const uint32_t si = (uint32_t)ny * nz, sj = nz;
-    float dg = 0.f;
-#define FFR_NB(cond, nc, sc)                          \
-    if (cond) {                                       \
-        const uint8_t t = f[nc];                      \
-        if (t == OUTFLOW) dg += 2.f * (sc);           \
-        else if (t != SOLID) dg += (sc);              \
-    }
-    FFR_NB(i > 0, c - si, L[i])
-    FFR_NB(i < nx - 1, c + si, L[nx + i])
-    FFR_NB(j > 0, c - sj, L[2 * nx + j])
-    FFR_NB(j < ny - 1, c + sj, L[2 * nx + ny + j])
-    FFR_NB(k > 0, c - 1, L[2 * nx + 2 * ny + k])
-    FFR_NB(k < nz - 1, c + 1, L[2 * nx + 2 * ny + nz + k])
-#undef FFR_NB
-    return dg < 1e-30f ? 1.f : dg;
+    const uint32_t si = (uint32_t)g.ny * g.nz, sj = g.nz;
+    float ox = 0.f, oy = 0.f, oz = 0.f;
+    if (m & (NB_UNK0 << 0)) ox += ldv(x, c - si);
+    if (m & (NB_UNK0 << 1)) ox += ldv(x, c + si);
+    if (m & (NB_UNK0 << 2)) oy += ldv(x, c - sj);
+    if (m & (NB_UNK0 << 3)) oy += ldv(x, c + sj);
+    if (m & (NB_UNK0 << 4)) oz += ldv(x, c - 1);
+    if (m & (NB_UNK0 << 5)) oz += ldv(x, c + 1);
+    return g.sx * ox + g.sy * oy + g.sz * oz;
 }
 
 // stored operator of a coarse level: returns the neighbour sum
@@ -103,13 +74,9 @@ __device__ __forceinline__ float rowc(const MgLevel &L, const T *x, int i, int j
 // coupling of a fine cell to its lower neighbour along `axis` (0 unless both unknown)
 __device__ __forceinline__ float coupling0(const Geo &g, int axis, int i, int j, int k)
 {
-    const int pos = axis == 0 ? i : (axis == 1 ? j : k);
-    if (pos == 0) return 0.f;
-    const uint32_t c = cidx(g, i, j, k);
-    const uint32_t st = axis == 0 ? g.ny * g.nz : (axis == 1 ? g.nz : 1);
-    if (!unknown0(g.flags[c]) || !unknown0(g.flags[c - st])) return 0.f;
-    const int off = axis == 0 ? 0 : (axis == 1 ? 2 * g.nx : 2 * g.nx + 2 * g.ny);
-    return g.lapf[off + pos];
+    const uint16_t m = g.nbm[cidx(g, i, j, k)];
+    if (!(m & NB_SELF) || !(m & (NB_UNK0 << (2 * axis)))) return 0.f;
+    return axis == 0 ? g.sx : (axis == 1 ? g.sy : g.sz);
 }
 __device__ __forceinline__ float couplingc(const MgLevel &F, int axis, int i, int j, int k)
 {
@@ -136,9 +103,9 @@ __global__ void __launch_bounds__(kBlock) k_mg_build(Geo g, MgLevel F, MgLevel C
                     float dg;
                     bool act;
                     if (FINE0) {
-                        const uint32_t c = cidx(g, a, b, d);
-                        act = unknown0(g.flags[c]);
-                        dg = act ? diag0(g, a, b, d, c) : 0.f;
+                        const uint16_t m = g.nbm[cidx(g, a, b, d)];
+                        act = (m & NB_SELF) != 0;
+                        dg = act ? diag0m(g, m) : 0.f;
                     } else {
                         const uint32_t c = ((uint32_t)a * F.ny + b) * F.nz + d;
                         act = F.active[c] != 0;
@@ -181,42 +148,41 @@ __global__ void __launch_bounds__(kBlock) k_mg_build(Geo g, MgLevel F, MgLevel C
 // z := (r + sum coupling * z_nb) / diag on unknowns of `color`; with
 // zero_start the pass begins from z = 0 and also clears the other colour.
 template <typename T, bool L0>
-__global__ void __launch_bounds__(kBlock) k_mg_smooth(Geo g, MgLevel L, T *z, const T *r,
-                                                      int color, int zero_start, const MgState *m)
+__global__ void __launch_bounds__(kBlock) k_mg_smooth(Geo g, MgLevel L, T *__restrict__ z,
+                                                      const T *__restrict__ r, int color,
+                                                      int zero_start, const MgState *m)
 {
     if (stopped(m)) return;
     uint32_t sb, se;
     block_span(L.n, xcd_block(), sb, se);
     for (uint32_t c = sb + threadIdx.x; c < se; c += kBlock) {
+        if (L0) {
+            const uint16_t nb = g.nbm[c];
+            const int col = (nb & NB_COLOR) ? 1 : 0;
+            if (col != color) {
+                if (zero_start) stv(z, c, 0.f);
+                continue;
+            }
+            if (!(nb & NB_SELF)) {
+                stv(z, c, 0.f);
+                continue;
+            }
+            const float of = zero_start ? 0.f : off0m(g, z, c, nb);
+            stv(z, c, (ldv(r, c) + of) / diag0m(g, nb));
+            continue;
+        }
         int i, j, k;
         decode(c, L.f_nynz, L.f_nz, L.ny, L.nz, i, j, k);
-        const bool mine = ((i + j + k) & 1) == color;
-        if (!mine) {
+        if (((i + j + k) & 1) != color) {
             if (zero_start) stv(z, c, 0.f);
             continue;
         }
-        float znew;
-        if (L0) {
-            if (!unknown0(g.flags[c])) {
-                stv(z, c, 0.f);
-                continue;
-            }
-            if (zero_start) {  // all neighbours start at 0
-                znew = ldv(r, c) / diag0(g, i, j, k, c);
-            } else {
-                float dg, of;
-                row0<T>(g, z, i, j, k, c, dg, of);
-                znew = (ldv(r, c) + of) / dg;
-            }
-        } else {
-            if (!L.active[c]) {
-                stv(z, c, 0.f);
-                continue;
-            }
-            const float of = zero_start ? 0.f : rowc<T>(L, z, i, j, k, c);
-            znew = (ldv(r, c) + of) * L.rdiag[c];
+        if (!L.active[c]) {
+            stv(z, c, 0.f);
+            continue;
         }
-        stv(z, c, znew);
+        const float of = zero_start ? 0.f : rowc<T>(L, z, i, j, k, c);
+        stv(z, c, (ldv(r, c) + of) * L.rdiag[c]);
     }
 }
 
@@ -239,10 +205,9 @@ __global__ void __launch_bounds__(kBlock) k_mg_restrict(Geo g, MgLevel F, MgLeve
                 for (int d = 2 * kc; d <= k1; d++) {
                     const uint32_t c = ((uint32_t)a * F.ny + b) * F.nz + d;
                     if (L0) {
-                        if (!unknown0(g.flags[c])) continue;
-                        float dg, of;
-                        row0<TF>(g, z, a, b, d, c, dg, of);
-                        sum += ldv(r, c) - (dg * ldv(z, c) - of);
+                        const uint16_t nb = g.nbm[c];
+                        if (!(nb & NB_SELF)) continue;
+                        sum += ldv(r, c) - (diag0m(g, nb) * ldv(z, c) - off0m(g, z, c, nb));
                     } else {
                         if (!F.active[c]) continue;
                         const float of = rowc<TF>(F, z, a, b, d, c);
@@ -262,7 +227,7 @@ __global__ void __launch_bounds__(kBlock) k_mg_prolong(Geo g, MgLevel F, MgLevel
     uint32_t sb, se;
     block_span(F.n, xcd_block(), sb, se);
     for (uint32_t c = sb + threadIdx.x; c < se; c += kBlock) {
-        const bool act = L0 ? unknown0(g.flags[c]) : F.active[c] != 0;
+        const bool act = L0 ? (g.nbm[c] & NB_SELF) != 0 : F.active[c] != 0;
         if (!act) continue;
         int i, j, k;
         decode(c, F.f_nynz, F.f_nz, F.ny, F.nz, i, j, k);
@@ -288,10 +253,9 @@ __global__ void __launch_bounds__(1024) k_mg_coarse(Geo g, MgLevel L, T *z, cons
             decode(c, L.f_nynz, L.f_nz, L.ny, L.nz, i, j, k);
             if (((i + j + k) & 1) != color) continue;
             if (L0) {
-                if (!unknown0(g.flags[c])) continue;
-                float dg, of;
-                row0<T>(g, z, i, j, k, c, dg, of);
-                stv(z, c, (ldv(r, c) + of) / dg);
+                const uint16_t nb = g.nbm[c];
+                if (!(nb & NB_SELF)) continue;
+                stv(z, c, (ldv(r, c) + off0m(g, z, c, nb)) / diag0m(g, nb));
             } else {
                 if (!L.active[c]) continue;
                 stv(z, c, (ldv(r, c) + rowc<T>(L, z, i, j, k, c)) * L.rdiag[c]);
@@ -314,13 +278,11 @@ __global__ void __launch_bounds__(kBlock) k_pcg_spmv(Geo g, const float *p, floa
     block_span(g.nc, blk, sb, se);
     for (uint32_t c = sb + threadIdx.x; c < se; c += kBlock) {
         float qv = 0.f;
-        if (unknown0(g.flags[c])) {
-            int i, j, k;
-            decode(c, g.f_nynz, g.f_nz, g.ny, g.nz, i, j, k);
-            float dg, of;
-            row0<float>(g, p, i, j, k, c, dg, of);
-            qv = dg * p[c] - of;
-            acc += (double)p[c] * (double)qv;
+        const uint16_t nb = g.nbm[c];
+        if (nb & NB_SELF) {
+            const float pc = p[c];
+            qv = diag0m(g, nb) * pc - off0m(g, p, c, nb);
+            acc += (double)pc * (double)qv;
         }
         q[c] = qv;
     }
@@ -450,46 +412,53 @@ __global__ void __launch_bounds__(kBlock) k_dc_start(MgState *m, RedScratch red,
     m->res = 0.0;
 }
 
-// x_out = x_in + e on unknowns; r = b - A x_out (FP64, the reference row);
-// r32 = (float) r; partials of ||r||^2
-__global__ void __launch_bounds__(kBlock) k_dc_residual(Geo g, const double *b, const double *xin,
-                                                        const float *e, double *xout, float *r32,
-                                                        const MgState *m, RedScratch red)
+// FP64 reference row (PoissonSolver.cpp:52-87) on a uniform grid from the
+// neighbour code; neighbour values come from `val` (x + e for the outer
+// residual, x for the OUTFLOW back substitution).
+template <typename F>
+__device__ __forceinline__ void row64(const Geo &g, uint32_t c, uint16_t m, F val, double &diag,
+                                      double &off)
+{
+    const uint32_t si = (uint32_t)g.ny * g.nz, sj = g.nz;
+    double dg = 0.0, of = 0.0;
+#define FFR_FACE(q, nc, sc)                                                   \
+    if (m & (NB_UNK0 << q)) { dg += (sc); of += (sc) * val(nc); }            \
+    else if (m & (NB_OUT0 << q)) dg += 2.0 * (sc);
+    FFR_FACE(0, c - si, g.sxd)
+    FFR_FACE(1, c + si, g.sxd)
+    FFR_FACE(2, c - sj, g.syd)
+    FFR_FACE(3, c + sj, g.syd)
+    FFR_FACE(4, c - 1, g.szd)
+    FFR_FACE(5, c + 1, g.szd)
+#undef FFR_FACE
+    diag = dg < 1e-30 ? 1.0 : dg;
+    off = of;
+}
+
+// x_out = x_in + e on unknowns; r = b - A x_out (FP64); r32 = (float) r;
+// partials of ||r||^2
+__global__ void __launch_bounds__(kBlock) k_dc_residual(Geo g, const double *__restrict__ b,
+                                                        const double *__restrict__ xin,
+                                                        const float *__restrict__ e,
+                                                        double *__restrict__ xout,
+                                                        float *__restrict__ r32, const MgState *m,
+                                                        RedScratch red)
 {
     __shared__ double lds[kBlock / kWave];
     if (m->solve_done) return;
     const uint32_t blk = xcd_block();
     double acc = 0.0;
-    const uint8_t *f = g.flags;
-    const double *L = g.lap;
-    const int nx = g.nx, ny = g.ny, nz = g.nz;
-    const uint32_t si = (uint32_t)ny * nz, sj = nz;
     uint32_t sb, se;
     block_span(g.nc, blk, sb, se);
     for (uint32_t c = sb + threadIdx.x; c < se; c += kBlock) {
-        if (!unknown0(f[c])) {
+        const uint16_t nb = g.nbm[c];
+        if (!(nb & NB_SELF)) {
             xout[c] = 0.0;
             r32[c] = 0.f;
             continue;
         }
-        int i, j, k;
-        decode(c, g.f_nynz, g.f_nz, ny, nz, i, j, k);
-        double diag = 0.0, off = 0.0;
-#define FFR_NB(cond, nc, sc)                                                   \
-        if (cond) {                                                            \
-            const uint8_t t = f[nc];                                           \
-            const double s_ = (sc);                                            \
-            if (t == OUTFLOW) diag += 2.0 * s_;                                \
-            else if (t != SOLID) { diag += s_; off += s_ * (xin[nc] + (double)e[nc]); } \
-        }
-        FFR_NB(i > 0, c - si, L[i])
-        FFR_NB(i < nx - 1, c + si, L[nx + i])
-        FFR_NB(j > 0, c - sj, L[2 * nx + j])
-        FFR_NB(j < ny - 1, c + sj, L[2 * nx + ny + j])
-        FFR_NB(k > 0, c - 1, L[2 * nx + 2 * ny + k])
-        FFR_NB(k < nz - 1, c + 1, L[2 * nx + 2 * ny + nz + k])
-#undef FFR_NB
-        if (diag < 1e-30) diag = 1.0;
+        double diag, off;
+        row64(g, c, nb, [&](uint32_t q) { return xin[q] + (double)e[q]; }, diag, off);
         const double xc = xin[c] + (double)e[c];
         const double rv = b[c] - (diag * xc - off);
         xout[c] = xc;
@@ -520,43 +489,24 @@ __global__ void __launch_bounds__(kBlock) k_dc_check(MgState *m, RedScratch red,
 
 // p: unknowns from x; OUTFLOW rows solved from their neighbours (or pinned
 // to 0 for the reference multigrid's convention); SOLID cells 0.
-__global__ void __launch_bounds__(kBlock) k_dc_finalize(Geo g, const double *b, const double *x,
-                                                        double *p, int pin_outflow)
+__global__ void __launch_bounds__(kBlock) k_dc_finalize(Geo g, const double *__restrict__ b,
+                                                        const double *x, double *p,
+                                                        int pin_outflow)
 {
-    const uint8_t *f = g.flags;
-    const double *L = g.lap;
-    const int nx = g.nx, ny = g.ny, nz = g.nz;
-    const uint32_t si = (uint32_t)ny * nz, sj = nz;
     uint32_t sb, se;
     block_span(g.nc, xcd_block(), sb, se);
     for (uint32_t c = sb + threadIdx.x; c < se; c += kBlock) {
-        const uint8_t t0 = f[c];
-        if (unknown0(t0)) {
+        const uint16_t nb = g.nbm[c];
+        if (nb & NB_SELF) {
             p[c] = x[c];
             continue;
         }
-        if (t0 == SOLID || pin_outflow) {
+        if ((nb & NB_SELF_SOLID) || pin_outflow) {
             p[c] = 0.0;
             continue;
         }
-        int i, j, k;
-        decode(c, g.f_nynz, g.f_nz, ny, nz, i, j, k);
-        double diag = 0.0, off = 0.0;
-#define FFR_NB(cond, nc, sc)                                     \
-        if (cond) {                                              \
-            const uint8_t t = f[nc];                             \
-            const double s_ = (sc);                              \
-            if (t == OUTFLOW) diag += 2.0 * s_;                  \
-            else if (t != SOLID) { diag += s_; off += s_ * x[nc]; } \
-        }
-        FFR_NB(i > 0, c - si, L[i])
-        FFR_NB(i < nx - 1, c + si, L[nx + i])
-        FFR_NB(j > 0, c - sj, L[2 * nx + j])
-        FFR_NB(j < ny - 1, c + sj, L[2 * nx + ny + j])
-        FFR_NB(k > 0, c - 1, L[2 * nx + 2 * ny + k])
-        FFR_NB(k < nz - 1, c + 1, L[2 * nx + 2 * ny + nz + k])
-#undef FFR_NB
-        if (diag < 1e-30) diag = 1.0;
+        double diag, off;
+        row64(g, c, nb, [&](uint32_t q) { return x[q]; }, diag, off);
         p[c] = (b[c] + off) / diag;
     }
 }
@@ -565,9 +515,9 @@ __global__ void __launch_bounds__(kBlock) k_dc_finalize(Geo g, const double *b, 
 void mg_launch_build(hipStream_t s, const Geo &g, const MgLevel &F, const MgLevel &C, bool fine0)
 {
     if (fine0)
-        hipLaunchKernelGGL(k_mg_build<true>, dim3(grid_for(C.n)), dim3(kBlock), 0, s, g, F, C);
+        FFR_LAUNCH(k_mg_build<true>, dim3(grid_for(C.n)), dim3(kBlock), 0, s, g, F, C);
     else
-        hipLaunchKernelGGL(k_mg_build<false>, dim3(grid_for(C.n)), dim3(kBlock), 0, s, g, F, C);
+        FFR_LAUNCH(k_mg_build<false>, dim3(grid_for(C.n)), dim3(kBlock), 0, s, g, F, C);
 }
 
 template <typename T>
@@ -577,10 +527,10 @@ void mg_launch_smooth(hipStream_t s, const Geo &g, const MgLevel &L, bool l0, in
     T *z = (T *)L.z;
     const T *r = (const T *)L.r;
     if (l0)
-        hipLaunchKernelGGL((k_mg_smooth<T, true>), dim3(grid_for(L.n)), dim3(kBlock), 0, s, g, L, z, r,
+        FFR_LAUNCH((k_mg_smooth<T, true>), dim3(grid_for(L.n)), dim3(kBlock), 0, s, g, L, z, r,
                            color, zero_start, m);
     else
-        hipLaunchKernelGGL((k_mg_smooth<T, false>), dim3(grid_for(L.n)), dim3(kBlock), 0, s, g, L, z,
+        FFR_LAUNCH((k_mg_smooth<T, false>), dim3(grid_for(L.n)), dim3(kBlock), 0, s, g, L, z,
                            r, color, zero_start, m);
 }
 
@@ -589,10 +539,10 @@ void mg_launch_restrict(hipStream_t s, const Geo &g, const MgLevel &F, const MgL
                         const MgState *m)
 {
     if (l0)
-        hipLaunchKernelGGL((k_mg_restrict<TF, TC, true>), dim3(grid_for(C.n)), dim3(kBlock), 0, s, g, F,
+        FFR_LAUNCH((k_mg_restrict<TF, TC, true>), dim3(grid_for(C.n)), dim3(kBlock), 0, s, g, F,
                            C, (const TF *)F.z, (const TF *)F.r, (TC *)C.r, m);
     else
-        hipLaunchKernelGGL((k_mg_restrict<TF, TC, false>), dim3(grid_for(C.n)), dim3(kBlock), 0, s, g,
+        FFR_LAUNCH((k_mg_restrict<TF, TC, false>), dim3(grid_for(C.n)), dim3(kBlock), 0, s, g,
                            F, C, (const TF *)F.z, (const TF *)F.r, (TC *)C.r, m);
 }
 
@@ -601,10 +551,10 @@ void mg_launch_prolong(hipStream_t s, const Geo &g, const MgLevel &F, const MgLe
                        const MgState *m)
 {
     if (l0)
-        hipLaunchKernelGGL((k_mg_prolong<TF, TC, true>), dim3(grid_for(F.n)), dim3(kBlock), 0, s, g, F,
+        FFR_LAUNCH((k_mg_prolong<TF, TC, true>), dim3(grid_for(F.n)), dim3(kBlock), 0, s, g, F,
                            C, (TF *)F.z, (const TC *)C.z, m);
     else
-        hipLaunchKernelGGL((k_mg_prolong<TF, TC, false>), dim3(grid_for(F.n)), dim3(kBlock), 0, s, g,
+        FFR_LAUNCH((k_mg_prolong<TF, TC, false>), dim3(grid_for(F.n)), dim3(kBlock), 0, s, g,
                            F, C, (TF *)F.z, (const TC *)C.z, m);
 }
 
@@ -613,10 +563,10 @@ void mg_launch_coarse(hipStream_t s, const Geo &g, const MgLevel &L, bool l0, in
                       const MgState *m)
 {
     if (l0)
-        hipLaunchKernelGGL((k_mg_coarse<T, true>), dim3(1), dim3(1024), 0, s, g, L, (T *)L.z,
+        FFR_LAUNCH((k_mg_coarse<T, true>), dim3(1), dim3(1024), 0, s, g, L, (T *)L.z,
                            (const T *)L.r, sweeps, m);
     else
-        hipLaunchKernelGGL((k_mg_coarse<T, false>), dim3(1), dim3(1024), 0, s, g, L, (T *)L.z,
+        FFR_LAUNCH((k_mg_coarse<T, false>), dim3(1), dim3(1024), 0, s, g, L, (T *)L.z,
                            (const T *)L.r, sweeps, m);
 }
 
@@ -633,55 +583,55 @@ template void mg_launch_coarse<__hip_bfloat16>(hipStream_t, const Geo &, const M
 
 void pcg_launch_spmv(hipStream_t s, const Geo &g, const float *p, float *q, const MgState *m, RedScratch red)
 {
-    hipLaunchKernelGGL(k_pcg_spmv, dim3(grid_for(g.nc)), dim3(kBlock), 0, s, g, p, q, m, red);
+    FFR_LAUNCH(k_pcg_spmv, dim3(grid_for(g.nc)), dim3(kBlock), 0, s, g, p, q, m, red);
 }
 void pcg_launch_update(hipStream_t s, uint32_t n, float *e, float *r, const float *p, const float *q,
                        MgState *m, RedScratch red, int it)
 {
-    hipLaunchKernelGGL(k_pcg_update, dim3(grid_for(n)), dim3(kBlock), 0, s, n, e, r, p, q, m, red, it);
+    FFR_LAUNCH(k_pcg_update, dim3(grid_for(n)), dim3(kBlock), 0, s, n, e, r, p, q, m, red, it);
 }
 void pcg_launch_check(hipStream_t s, uint32_t n, MgState *m, RedScratch red, int it)
 {
-    hipLaunchKernelGGL(k_pcg_check, dim3(1), dim3(kBlock), 0, s, m, red, grid_for(n), it);
+    FFR_LAUNCH(k_pcg_check, dim3(1), dim3(kBlock), 0, s, m, red, grid_for(n), it);
 }
 void pcg_launch_dot_rz(hipStream_t s, uint32_t n, const float *r, const float *z, const MgState *m,
                        RedScratch red)
 {
-    hipLaunchKernelGGL(k_pcg_dot_rz, dim3(grid_for(n)), dim3(kBlock), 0, s, n, r, z, m, red);
+    FFR_LAUNCH(k_pcg_dot_rz, dim3(grid_for(n)), dim3(kBlock), 0, s, n, r, z, m, red);
 }
 void pcg_launch_start(hipStream_t s, uint32_t n, const float *z, float *p, float *e, MgState *m,
                       RedScratch red)
 {
-    hipLaunchKernelGGL(k_pcg_start, dim3(grid_for(n)), dim3(kBlock), 0, s, n, z, p, e, m, red);
+    FFR_LAUNCH(k_pcg_start, dim3(grid_for(n)), dim3(kBlock), 0, s, n, z, p, e, m, red);
 }
 void pcg_launch_dir(hipStream_t s, uint32_t n, const float *z, float *p, MgState *m, RedScratch red, int it)
 {
-    hipLaunchKernelGGL(k_pcg_dir, dim3(grid_for(n)), dim3(kBlock), 0, s, n, z, p, m, red, it);
+    FFR_LAUNCH(k_pcg_dir, dim3(grid_for(n)), dim3(kBlock), 0, s, n, z, p, m, red, it);
 }
 void dc_launch_init(hipStream_t s, const Geo &g, const double *b, double *x, float *e, RedScratch red)
 {
-    hipLaunchKernelGGL(k_dc_init, dim3(grid_for(g.nc)), dim3(kBlock), 0, s, g, b, x, e, red);
+    FFR_LAUNCH(k_dc_init, dim3(grid_for(g.nc)), dim3(kBlock), 0, s, g, b, x, e, red);
 }
 void dc_launch_start(hipStream_t s, const Geo &g, MgState *m, RedScratch red, double outer_tol,
                      int max_outer, int max_inner, double inner_tol)
 {
-    hipLaunchKernelGGL(k_dc_start, dim3(1), dim3(kBlock), 0, s, m, red, grid_for(g.nc), outer_tol,
+    FFR_LAUNCH(k_dc_start, dim3(1), dim3(kBlock), 0, s, m, red, grid_for(g.nc), outer_tol,
                        max_outer, max_inner, inner_tol);
 }
 void dc_launch_residual(hipStream_t s, const Geo &g, const double *b, const double *xin, const float *e,
                         double *xout, float *r32, const MgState *m, RedScratch red)
 {
-    hipLaunchKernelGGL(k_dc_residual, dim3(grid_for(g.nc)), dim3(kBlock), 0, s, g, b, xin, e, xout, r32,
+    FFR_LAUNCH(k_dc_residual, dim3(grid_for(g.nc)), dim3(kBlock), 0, s, g, b, xin, e, xout, r32,
                        m, red);
 }
 void dc_launch_check(hipStream_t s, const Geo &g, MgState *m, RedScratch red)
 {
-    hipLaunchKernelGGL(k_dc_check, dim3(1), dim3(kBlock), 0, s, m, red, grid_for(g.nc));
+    FFR_LAUNCH(k_dc_check, dim3(1), dim3(kBlock), 0, s, m, red, grid_for(g.nc));
 }
 void dc_launch_finalize(hipStream_t s, const Geo &g, const double *b, const double *x, double *p,
                         int pin_outflow)
 {
-    hipLaunchKernelGGL(k_dc_finalize, dim3(grid_for(g.nc)), dim3(kBlock), 0, s, g, b, x, p, pin_outflow);
+    FFR_LAUNCH(k_dc_finalize, dim3(grid_for(g.nc)), dim3(kBlock), 0, s, g, b, x, p, pin_outflow);
 }
 
 }  // namespace ffr

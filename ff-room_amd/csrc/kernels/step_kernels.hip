@@ -264,7 +264,7 @@ void launch_predict(hipStream_t s, const Geo &g, const StepParams &sp, const R *
                     const R *w, const R *T, R *us, R *vs, R *ws, StepRec *rec)
 {
     const uint64_t n = (uint64_t)g.nu + g.nv + g.nw;
-    hipLaunchKernelGGL(k_predict<R>, dim3(grid_for(n)), dim3(kBlock), 0, s, g, sp, u, v, w, T,
+    FFR_LAUNCH(k_predict<R>, dim3(grid_for(n)), dim3(kBlock), 0, s, g, sp, u, v, w, T,
                        us, vs, ws, rec);
 }
 
@@ -295,7 +295,7 @@ template <typename R>
 void launch_rhs(hipStream_t s, const Geo &g, const StepParams &sp, const R *us, const R *vs,
                 const R *ws, double *b)
 {
-    hipLaunchKernelGGL(k_rhs<R>, dim3(grid_for(g.nc)), dim3(kBlock), 0, s, g, sp, us, vs, ws, b);
+    FFR_LAUNCH(k_rhs<R>, dim3(grid_for(g.nc)), dim3(kBlock), 0, s, g, sp, us, vs, ws, b);
 }
 
 // ---- K6: projection + post-projection masks + velocity change ----------
@@ -388,7 +388,7 @@ void launch_correct(hipStream_t s, const Geo &g, const StepParams &sp, const R *
                     const R *ws, const double *p, R *u, R *v, R *w, StepRec *rec)
 {
     const uint64_t n = (uint64_t)g.nu + g.nv + g.nw;
-    hipLaunchKernelGGL(k_correct<R>, dim3(grid_for(n)), dim3(kBlock), 0, s, g, sp, us, vs, ws, p,
+    FFR_LAUNCH(k_correct<R>, dim3(grid_for(n)), dim3(kBlock), 0, s, g, sp, us, vs, ws, p,
                        u, v, w, rec);
 }
 
@@ -424,7 +424,7 @@ __global__ void __launch_bounds__(kBlock) k_divmax(Geo g, const R *u, const R *v
 template <typename R>
 void launch_divmax(hipStream_t s, const Geo &g, const R *u, const R *v, const R *w, StepRec *rec)
 {
-    hipLaunchKernelGGL(k_divmax<R>, dim3(grid_for(g.nc)), dim3(kBlock), 0, s, g, u, v, w, rec);
+    FFR_LAUNCH(k_divmax<R>, dim3(grid_for(g.nc)), dim3(kBlock), 0, s, g, u, v, w, rec);
 }
 
 // ---- K8: temperature transport (FluidSolver.cpp:388-456) ---------------
@@ -499,7 +499,7 @@ template <typename R>
 void launch_temperature(hipStream_t s, const Geo &g, const StepParams &sp, const R *u,
                         const R *v, const R *w, const R *T, R *Tn, StepRec *rec, RedScratch red)
 {
-    hipLaunchKernelGGL(k_temperature<R>, dim3(grid_for(g.nc)), dim3(kBlock), 0, s, g, sp, u, v,
+    FFR_LAUNCH(k_temperature<R>, dim3(grid_for(g.nc)), dim3(kBlock), 0, s, g, sp, u, v,
                        w, T, Tn, rec, red);
 }
 
@@ -520,7 +520,7 @@ __global__ void __launch_bounds__(kBlock) k_tsum(Geo g, const R *T, StepRec *rec
 template <typename R>
 void launch_tsum(hipStream_t s, const Geo &g, const R *T, StepRec *rec, RedScratch red)
 {
-    hipLaunchKernelGGL(k_tsum<R>, dim3(grid_for(g.nc)), dim3(kBlock), 0, s, g, T, rec, red);
+    FFR_LAUNCH(k_tsum<R>, dim3(grid_for(g.nc)), dim3(kBlock), 0, s, g, T, rec, red);
 }
 
 // rec->tsum from the per-block partials of temperature / tsum (one block).
@@ -533,7 +533,7 @@ __global__ void __launch_bounds__(kBlock) k_step_finalize(StepRec *rec, RedScrat
 
 void launch_step_finalize(hipStream_t s, const Geo &g, StepRec *rec, RedScratch red)
 {
-    hipLaunchKernelGGL(k_step_finalize, dim3(1), dim3(kBlock), 0, s, rec, red, grid_for(g.nc));
+    FFR_LAUNCH(k_step_finalize, dim3(1), dim3(kBlock), 0, s, rec, red, grid_for(g.nc));
 }
 
 // ---- opening temperatures (BoundaryManager.cpp:137-178) -----------------
@@ -577,7 +577,7 @@ template <typename R>
 void launch_opening_T(hipStream_t s, const Geo &g, const R *u, const R *v, const R *w, R *T)
 {
     if (g.n_openings == 0) return;
-    hipLaunchKernelGGL(k_opening_T<R>, dim3(1), dim3(kBlock), 0, s, g, u, v, w, T);
+    FFR_LAUNCH(k_opening_T<R>, dim3(1), dim3(kBlock), 0, s, g, u, v, w, T);
 }
 
 // ---- precision conversion ------------------------------------------------
@@ -598,12 +598,25 @@ __global__ void k_convert_back(const R *src, double *dst, uint64_t n)
 template <typename R>
 void launch_convert(hipStream_t s, const double *src, R *dst, uint64_t n)
 {
-    if (n) hipLaunchKernelGGL(k_convert<R>, dim3(grid_for(n)), dim3(kBlock), 0, s, src, dst, n);
+    if (n) FFR_LAUNCH(k_convert<R>, dim3(grid_for(n)), dim3(kBlock), 0, s, src, dst, n);
 }
 template <typename R>
 void launch_convert_back(hipStream_t s, const R *src, double *dst, uint64_t n)
 {
-    if (n) hipLaunchKernelGGL(k_convert_back<R>, dim3(grid_for(n)), dim3(kBlock), 0, s, src, dst, n);
+    if (n) FFR_LAUNCH(k_convert_back<R>, dim3(grid_for(n)), dim3(kBlock), 0, s, src, dst, n);
+}
+
+__global__ void __launch_bounds__(kBlock) k_copy64(const double *__restrict__ src,
+                                                   double *__restrict__ dst, uint32_t n)
+{
+    uint32_t sb, se;
+    block_span(n, xcd_block(), sb, se);
+    for (uint32_t c = sb + threadIdx.x; c < se; c += kBlock) dst[c] = src[c];
+}
+
+void launch_copy64(hipStream_t s, const double *src, double *dst, uint64_t n)
+{
+    FFR_LAUNCH(k_copy64, dim3(grid_for(n)), dim3(kBlock), 0, s, src, dst, (uint32_t)n);
 }
 
 #define FFR_INSTANTIATE(R)                                                                      \

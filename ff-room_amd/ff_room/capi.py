@@ -64,7 +64,8 @@ class Params(C.Structure):
                 ("buoyancy", C.c_int), ("g_accel", C.c_double), ("beta", C.c_double),
                 ("k_thermal", C.c_double), ("cp", C.c_double),
                 ("precision", C.c_int), ("poisson_method", C.c_int), ("mg_levels", C.c_int),
-                ("mg_bf16", C.c_int), ("mg_pre_sweeps", C.c_int), ("mg_coarse_sweeps", C.c_int),
+                ("mg_bf16", C.c_int), ("mg_pre_sweeps", C.c_int), ("mg_level_sweeps", C.c_int),
+                ("mg_coarse_sweeps", C.c_int),
                 ("inner_tol", C.c_double), ("inner_max_iter", C.c_int)]
 
 
@@ -127,6 +128,7 @@ SIGNATURES = {
     "ffr_prof_enable": (_I, [_P, C.c_int]),
     "ffr_prof_reset": (_I, [_P]),
     "ffr_prof_read": (_I, [_P, C.POINTER(ProfEntry), C.c_int, C.POINTER(C.c_int)]),
+    "ffr_prof_kernel": (_I, [_P, C.c_char_p, C.c_int, _dp, _dp]),
 }
 
 _lib = None
@@ -325,6 +327,13 @@ class DeviceScene:
         d = C.c_double()
         self.ctx.check(self.ctx._l.ffr_T_mean(self.h, C.byref(d)), "ffr_T_mean")
         return d.value
+
+    def prof_kernel(self, name: str, reps: int = 200):
+        """(mean launch ms, algorithmic bytes) of `reps` back-to-back launches."""
+        ms, b = C.c_double(), C.c_double()
+        self.ctx.check(self.ctx._l.ffr_prof_kernel(self.h, name.encode(), reps, C.byref(ms),
+                                                   C.byref(b)), "ffr_prof_kernel")
+        return ms.value, b.value
 
     def step_count(self) -> int:
         return self.ctx._l.ffr_scene_step_count(self.h)

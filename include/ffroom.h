@@ -109,7 +109,8 @@ typedef struct {
     int poisson_method;  /* FFR_POISSON_* */
     int mg_levels;       /* multigrid levels incl. the finest; 0 = automatic (<= 5) */
     int mg_bf16;         /* store multigrid levels >= 1 in bfloat16 */
-    int mg_pre_sweeps;   /* red-black Gauss-Seidel sweeps per side of the V-cycle */
+    int mg_pre_sweeps;   /* red-black Gauss-Seidel sweeps per side of the V-cycle, level 0 */
+    int mg_level_sweeps; /* the same on levels 1 .. coarsest-1 */
     int mg_coarse_sweeps;/* symmetric sweeps on the coarsest level */
     double inner_tol;    /* mixed: relative residual target of each inner FP32 solve */
     int inner_max_iter;  /* mixed: iteration cap of each inner solve */
@@ -208,6 +209,16 @@ int ffr_drain(ffr_ctx *ctx);
 int ffr_prof_enable(ffr_ctx *ctx, int on);
 int ffr_prof_reset(ffr_ctx *ctx);
 int ffr_prof_read(ffr_ctx *ctx, ffr_prof_entry *out, int cap, int *n_out);
+/* Times `reps` launches of one kernel of the step on the scene's current
+ * buffers, each between the begin and end timestamps of its own dispatch
+ * (the figure a kernel trace reports): the mean kernel duration, and the
+ * kernel's algorithmic bytes per launch. Kernels: "predict", "rhs",
+ * "cg_spmv", "cg_update", "cg_dir", "mg_smooth_l0", "mg_restrict_l0",
+ * "pcg_spmv", "pcg_update", "dc_residual", and "copy" (a plain FP64 copy
+ * of nx*ny*nz doubles, to calibrate memory counters). The timed kernels overwrite
+ * solver scratch (not u, v, w, T): call between steps, not inside one. */
+int ffr_prof_kernel(ffr_scene *scene, const char *name, int reps, double *mean_ms,
+                    double *bytes_per_launch);
 
 #ifdef __cplusplus
 }
