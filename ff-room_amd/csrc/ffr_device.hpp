@@ -228,6 +228,34 @@ __device__ __forceinline__ double sum_partials(const double *partials, uint32_t 
     return r;
 }
 
+// ---- raw buffer access ------------------------------------------------------
+// A buffer resource over n elements: element offsets are 32-bit, and a read
+// whose offset lies outside [0, n) returns 0 (the hardware range check), so a
+// gather at an array end needs no clamping. An index that underflowed as an
+// unsigned value ((c - s) >> 1 with c < s) maps to a byte offset of at least
+// 2^32 - 2s, outside any array below 2 GiB. Word 3 is the gfx9-family data
+// format (32-bit, untyped).
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t buf_rsrc(const void *p, uint32_t bytes)
+{
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(p), (short)0, (int)bytes, 0x00020000);
+}
+__device__ __forceinline__ float buf_ldf(__amdgpu_buffer_rsrc_t r, uint32_t i)
+{
+    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, (int)(i * 4u), 0, 0));
+}
+__device__ __forceinline__ uint16_t buf_ldu16(__amdgpu_buffer_rsrc_t r, uint32_t i)
+{
+    return __builtin_amdgcn_raw_buffer_load_b16(r, (int)(i * 2u), 0, 0);
+}
+__device__ __forceinline__ uint8_t buf_ldu8(__amdgpu_buffer_rsrc_t r, uint32_t i)
+{
+    return __builtin_amdgcn_raw_buffer_load_b8(r, (int)i, 0, 0);
+}
+__device__ __forceinline__ void buf_stf(__amdgpu_buffer_rsrc_t r, uint32_t i, float v)
+{
+    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(uint32_t, v), r, (int)(i * 4u), 0, 0);
+}
+
 // Profiling hook: when a profiler scope has armed a start/stop event pair,
 // the next launch records them at the dispatch's own begin and end
 // (hipExtLaunchKernelGGL), i.e. the kernel's execution time without the

@@ -43,14 +43,18 @@ size_t mg_layout(char *base, const Dims &D, bool bf16, Multigrid *M, MgState **s
         return ptr;
     };
     const uint32_t n0 = (uint32_t)D.n[0][0] * D.n[0][1] * D.n[0][2];
+    const uint32_t H = (n0 + 1) / 2;  // split layout: two colour arrays of H
     MgState *st = (MgState *)take(sizeof(MgState));
-    float *e = (float *)take(4ull * n0), *ri = (float *)take(4ull * n0);
-    float *pi = (float *)take(4ull * n0), *qi = (float *)take(4ull * n0);
-    float *z0 = (float *)take(4ull * n0);
+    float *e = (float *)take(8ull * H), *ri = (float *)take(8ull * H);
+    float *pi = (float *)take(8ull * H), *qi = (float *)take(8ull * H);
+    float *z0 = (float *)take(8ull * H);
+    uint16_t *nbs = (uint16_t *)take(4ull * H);
     if (M) {
         M->e = e; M->ri = ri; M->pi = pi; M->qi = qi;
         M->lv[0].z = z0;
         M->lv[0].r = ri;
+        M->lv[0].nbs = nbs;
+        M->lv[0].H = H;
         M->lv[0].active = nullptr;
         M->lv[0].cx = M->lv[0].cy = M->lv[0].cz = M->lv[0].diag = M->lv[0].rdiag = nullptr;
         M->lv[0].bf16 = 0;
@@ -122,11 +126,33 @@ int mg_create(ffr_scene *s)
         L.f_nz = make_fastdiv(L.nz);
         L.f_nynz = make_fastdiv(L.ny * L.nz);
     }
+    {
+        MgLevel &L = M.lv[0];
+        L.par_nz = (L.nz % 2) == 0;
+        L.par_ny = (L.ny % 2) == 0;
+        L.f_hz = make_fastdiv(L.par_nz ? L.nz / 2 : 1);
+        L.f_hyz = make_fastdiv(L.par_ny ? L.ny * L.nz / 2 : 1);
+    }
+    // the small levels that fit one workgroup's LDS (z and r of each) run as
+    // a single kernel; level 0 never does (it is FP32 in the split layout)
+    {
+        const size_t vs = bf16 ? 2 : 4;
+        M.tail_first = -1;
+        M.tail_lds = 0;
+        size_t acc = 0;
+        for (int l = D.levels - 1; l >= 1; l--) {
+            acc += tail_level_bytes(M.lv[l].n, vs);
+            if (acc > kTailLdsMax) break;
+            M.tail_first = l;
+            M.tail_lds = acc;
+        }
+    }
     M.pre_sweeps = std::max(1, s->prm.mg_pre_sweeps);
     M.level_sweeps = std::max(1, s->prm.mg_level_sweeps);
     M.coarse_sweeps = std::max(1, s->prm.mg_coarse_sweeps);
     hipStream_t str = s->stream;
     HIP_CHECK_CTX(s->ctx, hipMemsetAsync(M.arena, 0, bytes, str), FFR_E_DEVICE);
+    mg_launch_split_codes(str, s->geo, M.lv[0]);
     for (int l = 0; l + 1 < D.levels; l++) mg_launch_build(str, s->geo, M.lv[l], M.lv[l + 1], l == 0);
     HIP_CHECK_CTX(s->ctx, hipStreamSynchronize(str), FFR_E_DEVICE);
     (void)g_unused_state;
@@ -165,6 +191,22 @@ static void vcycle(ffr_scene *s, int l)
     const MgLevel &L = M.lv[l];
     const bool l0 = l == 0;
     const double n = (double)L.n;
+    if (l == M.tail_first) {
+        TailLevels tl{};
+        uint32_t off = 0;
+        for (int k = l; k < M.levels; k++) {
+            tl.lv[k] = M.lv[k];
+            tl.off[k] = off;
+            off += (uint32_t)tail_level_bytes(M.lv[k].n, sizeof(TC));
+        }
+        tl.first = l;
+        tl.last = M.levels - 1;
+        tl.sweeps = M.level_sweeps;
+        tl.coarse_sweeps = M.coarse_sweeps;
+        ProfScope ps(s->ctx, st, "mg_tail", 0);
+        mg_launch_tail<TC>(st, tl, M.tail_lds, m);
+        return;
+    }
     if (l == M.levels - 1) {
         ProfScope ps(s->ctx, st, l0 ? "mg_coarse_l0" : "mg_coarse", 0);
         if (l0) mg_launch_coarse<float>(st, g, L, true, M.coarse_sweeps, m);
@@ -173,7 +215,8 @@ static void vcycle(ffr_scene *s, int l)
     }
     // algorithmic bytes of one colour pass: z and r read, the code (level 0,
     // 2 B) or operator (coarse: active + 3 couplings + rdiag), z written
-    const double bsm = l0 ? n * (4 + 4 + 2 + 4) : n * (2 * sizeof(TC) + 1 + 16 + sizeof(TC));
+    // (level 0, split layout: r, code and z of the colour, z of the other colour)
+    const double bsm = l0 ? n * (4 + 4 + 2 + 4) / 2 : n * (2 * sizeof(TC) + 1 + 16 + sizeof(TC));
     const double bres = l0 ? n * (4 + 4 + 2) + M.lv[l + 1].n * (double)sizeof(TC)
                            : n * (2 * sizeof(TC) + 1 + 16) + M.lv[l + 1].n * (double)sizeof(TC);
     const double bpro = n * (2.0 * (l0 ? 4 : sizeof(TC)) + (l0 ? 2 : 1)) + M.lv[l + 1].n * (double)sizeof(TC);
@@ -229,7 +272,7 @@ static void inner_iteration(ffr_scene *s, int it)
     const double n = (double)g.nc;
     {
         ProfScope ps(s->ctx, st, "pcg_spmv", n * (4 + 1 + 4));
-        pcg_launch_spmv(st, g, M.pi, M.qi, s->mgs, s->red);
+        pcg_launch_spmv(st, g, M.lv[0], M.pi, M.qi, s->mgs, s->red);
     }
     {
         ProfScope ps(s->ctx, st, "pcg_update", n * 24);
@@ -320,7 +363,7 @@ double mg_prof_launch(ffr_scene *s, const char *name, std::function<void()> *lau
     const MgLevel L0 = M.lv[0];
     if (k == "mg_smooth_l0") {
         *launch = [=] { mg_launch_smooth<float>(st, g, L0, true, 1, 0, m); };
-        return n * (4 + 4 + 2 + 4);
+        return (double)(g.nc - L0.H) * (4 + 4 + 2 + 4);
     }
     if (k == "mg_restrict_l0" && M.levels > 1) {
         const MgLevel L1 = M.lv[1];
@@ -333,7 +376,7 @@ double mg_prof_launch(ffr_scene *s, const char *name, std::function<void()> *lau
     if (k == "pcg_spmv") {
         float *p = M.pi, *q = M.qi;
         RedScratch red = s->red;
-        *launch = [=] { pcg_launch_spmv(st, g, p, q, m, red); };
+        *launch = [=] { pcg_launch_spmv(st, g, L0, p, q, m, red); };
         return n * (4 + 2 + 4);
     }
     if (k == "pcg_update") {

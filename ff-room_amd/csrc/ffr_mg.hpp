@@ -24,6 +24,16 @@
 // coarsest level runs symmetric red-black sweeps, so the preconditioner is
 // symmetric. Levels >= 1 keep z and r in bfloat16 when mg_bf16 is set
 // (arithmetic in FP32).
+//
+// Level-0 FP32 vectors (z, r and the inner PCG's e, p, q) use a red-black
+// split layout: cell c (natural index, k fastest) of colour (i+j+k)&1 lives
+// at colour*H + (c >> 1), H = ceil(nc/2). Cells 2m and 2m+1 always differ in
+// colour (same row when nz is even; otherwise the colour is the parity of c,
+// or of c+i when ny is even and planes hold an even count), so each colour
+// array is dense and every neighbour of a cell lies in the other colour's
+// array at (c +- stride) >> 1. A colour pass of the smoother then streams
+// only its own colour's r and codes and the other colour's z. Used slots are
+// exactly [0, nc): red [0, H), black [H, nc); the slot nc (odd nc) stays 0.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <hip/hip_bf16.h>
@@ -46,6 +56,15 @@ struct MgLevel {
     // vectors: FP32 on level 0, bfloat16 or FP32 on coarser levels
     void *z, *r;
     int bf16;
+    // level 0: neighbour codes in split layout (NB_COLOR holds c & 1) and H.
+    // The parity of the natural index of slot h of colour col is
+    //   col ^ (nz even ? (h / (nz/2)) & 1 : 0) ^ (ny even ? (h / (ny*nz/2)) & 1 : 0)
+    // (the row, and the plane, of cell 2h and 2h+1 coincide), so the loads of
+    // a cell need not wait for its code.
+    uint16_t *nbs;
+    uint32_t H;
+    int par_nz, par_ny;
+    FastDiv f_hz, f_hyz;
 };
 
 struct Multigrid {
@@ -56,7 +75,27 @@ struct Multigrid {
     // level-0 vectors of the inner PCG (FP32) and the outer correction
     float *e, *ri, *pi, *qi;
     int pre_sweeps = 2, level_sweeps = 2, coarse_sweeps = 8;
+    // levels tail_first .. levels-1 run as one single-workgroup kernel with
+    // their z and r in LDS (-1: none); tail_lds = its LDS bytes
+    int tail_first = -1;
+    size_t tail_lds = 0;
     int inner_pred[16] = {0};  // inner iterations seen per outer pass (launch prediction)
+};
+
+// The levels a single workgroup runs from LDS (see k_mg_tail). Level l
+// occupies LDS bytes [off[l], off[l] + tail_level_bytes(n_l, vs)): cx, cy,
+// cz, diag, rdiag (FP32), then z and r (vs bytes each), then active.
+constexpr int kTailThreads = 1024;
+constexpr size_t kTailLdsMax = 48 * 1024;
+__host__ __device__ inline size_t tail_level_bytes(uint32_t n, size_t vs)
+{
+    return ((20 + 2 * vs + 1) * (size_t)n + 15) & ~(size_t)15;
+}
+struct TailLevels {
+    MgLevel lv[kMaxLevels];
+    uint32_t off[kMaxLevels];
+    int first, last;
+    int sweeps, coarse_sweeps;
 };
 
 // Scalars of the inner PCG and the outer FP64 defect correction. Values the

@@ -5,6 +5,7 @@
 
 namespace ffr {
 void mg_launch_build(hipStream_t s, const Geo &g, const MgLevel &F, const MgLevel &C, bool fine0);
+void mg_launch_split_codes(hipStream_t s, const Geo &g, const MgLevel &L0);
 template <typename T>
 void mg_launch_smooth(hipStream_t s, const Geo &g, const MgLevel &L, bool l0, int color,
                       int zero_start, const MgState *m);
@@ -15,10 +16,12 @@ template <typename TF, typename TC>
 void mg_launch_prolong(hipStream_t s, const Geo &g, const MgLevel &F, const MgLevel &C, bool l0,
                        const MgState *m);
 template <typename T>
+void mg_launch_tail(hipStream_t s, const TailLevels &tl, size_t lds_bytes, const MgState *m);
+template <typename T>
 void mg_launch_coarse(hipStream_t s, const Geo &g, const MgLevel &L, bool l0, int sweeps,
                       const MgState *m);
-void pcg_launch_spmv(hipStream_t s, const Geo &g, const float *p, float *q, const MgState *m,
-                     RedScratch red);
+void pcg_launch_spmv(hipStream_t s, const Geo &g, const MgLevel &L0, const float *p, float *q,
+                     const MgState *m, RedScratch red);
 void pcg_launch_update(hipStream_t s, uint32_t n, float *e, float *r, const float *p,
                        const float *q, MgState *m, RedScratch red, int it);
 void pcg_launch_check(hipStream_t s, uint32_t n, MgState *m, RedScratch red, int it);

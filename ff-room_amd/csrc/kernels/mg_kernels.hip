@@ -15,10 +15,47 @@ __device__ __forceinline__ float ldv(const __hip_bfloat16 *p, uint32_t i) { retu
 __device__ __forceinline__ void stv(float *p, uint32_t i, float v) { p[i] = v; }
 __device__ __forceinline__ void stv(__hip_bfloat16 *p, uint32_t i, float v) { p[i] = __float2bfloat16(v); }
 
+// vector element access through a buffer resource (float or bfloat16)
+__device__ __forceinline__ float bldv(__amdgpu_buffer_rsrc_t r, uint32_t i, const float *)
+{
+    return buf_ldf(r, i);
+}
+__device__ __forceinline__ float bldv(__amdgpu_buffer_rsrc_t r, uint32_t i, const __hip_bfloat16 *)
+{
+    return __bfloat162float(__builtin_bit_cast(__hip_bfloat16, buf_ldu16(r, i)));
+}
+__device__ __forceinline__ void bstv(__amdgpu_buffer_rsrc_t r, uint32_t i, float v, const float *)
+{
+    buf_stf(r, i, v);
+}
+__device__ __forceinline__ void bstv(__amdgpu_buffer_rsrc_t r, uint32_t i, float v, const __hip_bfloat16 *)
+{
+    __builtin_amdgcn_raw_buffer_store_b16(__builtin_bit_cast(uint16_t, __float2bfloat16(v)), r,
+                                          (int)(i * 2u), 0, 0);
+}
+
 __device__ __forceinline__ bool unknown0(uint8_t f) { return f == FLUID || f == INFLOW; }
 __device__ __forceinline__ double *pslot(RedScratch red, int k) { return red.partials + k * kMaxBlocksCap; }
 
 __device__ __forceinline__ bool stopped(const MgState *m) { return m->solve_done || m->inner_done; }
+
+// Deferred stop test: the flags are read at kernel entry with vector loads
+// (which complete in order, unlike scalar loads that the kernel-argument wait
+// would also cover), and the empty asm makes the value opaque where it is
+// tested, so the compiler cannot hoist the branch (or unswitch a loop on it)
+// above the data loads issued before that point: the flag's memory latency
+// overlaps the data's instead of preceding it. The value is uniform.
+__device__ __forceinline__ uint32_t stop_flags(const MgState *m)
+{
+    const auto r = buf_rsrc(m, sizeof(MgState));
+    return __builtin_amdgcn_raw_buffer_load_b32(r, (int)offsetof(MgState, solve_done), 0, 0) |
+           __builtin_amdgcn_raw_buffer_load_b32(r, (int)offsetof(MgState, inner_done), 0, 0);
+}
+__device__ __forceinline__ bool late(uint32_t v)
+{
+    asm volatile("" : "+v"(v));
+    return __builtin_amdgcn_readfirstlane(v) != 0;
+}
 
 __device__ __forceinline__ void decode(uint32_t c, const FastDiv &fnynz, const FastDiv &fnz, int ny,
                                        int nz, int &i, int &j, int &k)
@@ -41,18 +78,40 @@ __device__ __forceinline__ float diag0m(const Geo &g, uint16_t m)
     return d < 1e-30f ? 1.f : d;
 }
 
-template <typename T>
-__device__ __forceinline__ float off0m(const Geo &g, const T *__restrict__ x, uint32_t c, uint16_t m)
+// ---- level-0 split layout (see ffr_mg.hpp) -----------------------------------
+// position of natural cell c (natural code m) in its own colour's array / the
+// base of the other colour's array
+__device__ __forceinline__ uint32_t sown(uint32_t c, uint16_t m, uint32_t H)
+{
+    return ((m & NB_COLOR) ? H : 0u) + (c >> 1);
+}
+__device__ __forceinline__ uint32_t sother(uint16_t m, uint32_t H) { return (m & NB_COLOR) ? 0u : H; }
+
+// neighbour sum of natural cell c from the other colour's array xo
+__device__ __forceinline__ float off0s(const Geo &g, const float *__restrict__ xo, uint32_t c,
+                                       uint16_t m)
 {
     const uint32_t si = (uint32_t)g.ny * g.nz, sj = g.nz;
     float ox = 0.f, oy = 0.f, oz = 0.f;
-    if (m & (NB_UNK0 << 0)) ox += ldv(x, c - si);
-    if (m & (NB_UNK0 << 1)) ox += ldv(x, c + si);
-    if (m & (NB_UNK0 << 2)) oy += ldv(x, c - sj);
-    if (m & (NB_UNK0 << 3)) oy += ldv(x, c + sj);
-    if (m & (NB_UNK0 << 4)) oz += ldv(x, c - 1);
-    if (m & (NB_UNK0 << 5)) oz += ldv(x, c + 1);
+    if (m & (NB_UNK0 << 0)) ox += xo[(c - si) >> 1];
+    if (m & (NB_UNK0 << 1)) ox += xo[(c + si) >> 1];
+    if (m & (NB_UNK0 << 2)) oy += xo[(c - sj) >> 1];
+    if (m & (NB_UNK0 << 3)) oy += xo[(c + sj) >> 1];
+    if (m & (NB_UNK0 << 4)) oz += xo[(c - 1) >> 1];
+    if (m & (NB_UNK0 << 5)) oz += xo[(c + 1) >> 1];
     return g.sx * ox + g.sy * oy + g.sz * oz;
+}
+
+// split codes: nbs[colour*H + (c>>1)] = code of c with NB_COLOR replaced by
+// c & 1 (so the natural index is recovered as 2h + bit); unused slot stays 0
+__global__ void __launch_bounds__(kBlock) k_mg_split_codes(Geo g, uint16_t *nbs, uint32_t H)
+{
+    uint32_t sb, se;
+    block_span(g.nc, xcd_block(), sb, se);
+    for (uint32_t c = sb + threadIdx.x; c < se; c += kBlock) {
+        const uint16_t m = g.nbm[c];
+        nbs[sown(c, m, H)] = (uint16_t)((m & ~NB_COLOR) | ((c & 1u) ? NB_COLOR : 0u));
+    }
 }
 
 // stored operator of a coarse level: returns the neighbour sum
@@ -145,76 +204,235 @@ __global__ void __launch_bounds__(kBlock) k_mg_build(Geo g, MgLevel F, MgLevel C
 }
 
 // ---- red-black Gauss-Seidel pass of one colour ------------------------------
-// z := (r + sum coupling * z_nb) / diag on unknowns of `color`; with
-// zero_start the pass begins from z = 0 and also clears the other colour.
-template <typename T, bool L0>
-__global__ void __launch_bounds__(kBlock) k_mg_smooth(Geo g, MgLevel L, T *__restrict__ z,
+// z := (r + sum coupling * z_nb) / diag on unknowns of `color`; a pass reads
+// only the other colour, so with zero_start (z = 0 before the pass) the
+// neighbour sum is skipped.
+//
+// Level 0 (split layout): r, the code and z of the colour are streamed
+// contiguously and the six neighbours are read from the other colour's
+// array. The natural index of a slot comes from its parity formula, so every
+// load of a cell is issued before any returns; the gathers go through buffer
+// resources, so neighbours beyond either end of the other array read 0 and
+// values of faces that do not couple are dropped by the code. Each thread
+// takes kU cells a block-stride apart. Interior cells (six unknown
+// neighbours, none OUTFLOW) skip the per-face selects and the diagonal
+// count; their arithmetic is the general formula's with every face present.
+constexpr int kU = 4;
+constexpr uint16_t kInterior = 0x007F;  // NB_SELF + six NB_UNK0 bits, no NB_OUT0 bit
+constexpr uint16_t kFaceBits = 0x1FFF;
+
+__device__ __forceinline__ uint32_t split_parity(const MgLevel &L, uint32_t h, uint32_t col)
+{
+    uint32_t b = col;
+    if (L.par_nz) b ^= fdiv(h, L.f_hz) & 1u;
+    if (L.par_ny) b ^= fdiv(h, L.f_hyz) & 1u;
+    return b;
+}
+
+// the six neighbours of natural cell c from the other colour's buffer
+__device__ __forceinline__ void nb_gather(__amdgpu_buffer_rsrc_t xo, uint32_t c, uint32_t si,
+                                          uint32_t sj, float v[6])
+{
+    v[0] = buf_ldf(xo, (c - si) >> 1);
+    v[1] = buf_ldf(xo, (c + si) >> 1);
+    v[2] = buf_ldf(xo, (c - sj) >> 1);
+    v[3] = buf_ldf(xo, (c + sj) >> 1);
+    v[4] = buf_ldf(xo, (c - 1u) >> 1);
+    v[5] = buf_ldf(xo, (c + 1u) >> 1);
+}
+
+// neighbour sum from gathered values (same association as a sequential sum)
+__device__ __forceinline__ float nb_sum(const Geo &g, uint16_t nb, const float v[6])
+{
+    if ((nb & kFaceBits) == kInterior) return g.sx * (v[0] + v[1]) + g.sy * (v[2] + v[3]) + g.sz * (v[4] + v[5]);
+    const float ox = ((nb & (NB_UNK0 << 0)) ? v[0] : 0.f) + ((nb & (NB_UNK0 << 1)) ? v[1] : 0.f);
+    const float oy = ((nb & (NB_UNK0 << 2)) ? v[2] : 0.f) + ((nb & (NB_UNK0 << 3)) ? v[3] : 0.f);
+    const float oz = ((nb & (NB_UNK0 << 4)) ? v[4] : 0.f) + ((nb & (NB_UNK0 << 5)) ? v[5] : 0.f);
+    return g.sx * ox + g.sy * oy + g.sz * oz;
+}
+
+__device__ __forceinline__ float diag_of(const Geo &g, uint16_t nb, float dint)
+{
+    return (nb & kFaceBits) == kInterior ? dint : diag0m(g, nb);
+}
+
+__global__ void __launch_bounds__(kBlock) k_mg_smooth0(Geo g, MgLevel L,
+                                                       const uint16_t *__restrict__ code,
+                                                       const float *__restrict__ zo,
+                                                       float *__restrict__ zc,
+                                                       const float *__restrict__ rc, uint32_t cnt,
+                                                       uint32_t cnt_o, uint32_t col,
+                                                       int zero_start, const MgState *m)
+{
+    const uint32_t stop = stop_flags(m);  // tested after the first loads
+    const uint32_t si = (uint32_t)g.ny * g.nz, sj = g.nz;
+    const float dint = diag0m(g, kInterior);
+    const auto bcode = buf_rsrc(code, 2u * cnt), bz = buf_rsrc(zc, 4u * cnt),
+               br = buf_rsrc(rc, 4u * cnt), bo = buf_rsrc(zo, 4u * cnt_o);
+    uint32_t sb, se;
+    block_span(cnt, xcd_block(), sb, se);
+    for (uint32_t h0 = sb + threadIdx.x; h0 < se; h0 += kU * kBlock) {
+        uint16_t nb[kU];
+        float rv[kU], zv[kU][6];
+#pragma unroll
+        for (int t = 0; t < kU; t++) {
+            const uint32_t h = h0 + t * kBlock;  // reads past cnt return 0
+            nb[t] = buf_ldu16(bcode, h);
+            rv[t] = buf_ldf(br, h);
+            if (!zero_start) nb_gather(bo, 2 * h + split_parity(L, h, col), si, sj, zv[t]);
+        }
+        if (late(stop)) return;
+#pragma unroll
+        for (int t = 0; t < kU; t++) {
+            const uint32_t h = h0 + t * kBlock;
+            if (h >= se) break;
+            float v = 0.f;
+            if (nb[t] & NB_SELF) {
+                const float of = zero_start ? 0.f : nb_sum(g, nb[t], zv[t]);
+                v = (rv[t] + of) / diag_of(g, nb[t], dint);
+            }
+            buf_stf(bz, h, v);
+        }
+    }
+}
+
+// Coarse levels (natural layout, stored operator). A coupling is 0 across a
+// wall (k_mg_build stores none there, and the upper-side coupling of the last
+// cell lies past the array), so all twelve operator and neighbour values are
+// gathered unconditionally through buffer resources (out-of-range reads give
+// 0) and the stop flag is waited for only after they are issued.
+template <typename T>
+struct CoarseOp {  // buffer views of one coarse level
+    __amdgpu_buffer_rsrc_t cx, cy, cz, dg, rd, act;
+};
+template <typename T>
+__device__ __forceinline__ CoarseOp<T> coarse_op(const MgLevel &L)
+{
+    const uint32_t b = 4u * L.n;
+    return {buf_rsrc(L.cx, b), buf_rsrc(L.cy, b), buf_rsrc(L.cz, b), buf_rsrc(L.diag, b),
+            buf_rsrc(L.rdiag, b), buf_rsrc(L.active, L.n)};
+}
+// coupled neighbour sum of cell c (same order as rowc)
+template <typename T>
+__device__ __forceinline__ float coarse_row(const CoarseOp<T> &o, __amdgpu_buffer_rsrc_t x,
+                                            uint32_t c, uint32_t si, uint32_t sj)
+{
+    const T *tag = nullptr;
+    float of = 0.f;
+    of += buf_ldf(o.cx, c) * bldv(x, c - si, tag);
+    of += buf_ldf(o.cx, c + si) * bldv(x, c + si, tag);
+    of += buf_ldf(o.cy, c) * bldv(x, c - sj, tag);
+    of += buf_ldf(o.cy, c + sj) * bldv(x, c + sj, tag);
+    of += buf_ldf(o.cz, c) * bldv(x, c - 1u, tag);
+    of += buf_ldf(o.cz, c + 1u) * bldv(x, c + 1u, tag);
+    return of;
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kBlock) k_mg_smooth(MgLevel L, T *__restrict__ z,
                                                       const T *__restrict__ r, int color,
                                                       int zero_start, const MgState *m)
 {
-    if (stopped(m)) return;
+    const uint32_t stop = stop_flags(m);
+    const T *tag = nullptr;
+    const uint32_t si = (uint32_t)L.ny * L.nz, sj = L.nz;
+    const CoarseOp<T> o = coarse_op<T>(L);
+    const auto bz = buf_rsrc(z, (uint32_t)sizeof(T) * L.n), br = buf_rsrc(r, (uint32_t)sizeof(T) * L.n);
     uint32_t sb, se;
     block_span(L.n, xcd_block(), sb, se);
     for (uint32_t c = sb + threadIdx.x; c < se; c += kBlock) {
-        if (L0) {
-            const uint16_t nb = g.nbm[c];
-            const int col = (nb & NB_COLOR) ? 1 : 0;
-            if (col != color) {
-                if (zero_start) stv(z, c, 0.f);
-                continue;
-            }
-            if (!(nb & NB_SELF)) {
-                stv(z, c, 0.f);
-                continue;
-            }
-            const float of = zero_start ? 0.f : off0m(g, z, c, nb);
-            stv(z, c, (ldv(r, c) + of) / diag0m(g, nb));
-            continue;
-        }
         int i, j, k;
         decode(c, L.f_nynz, L.f_nz, L.ny, L.nz, i, j, k);
-        if (((i + j + k) & 1) != color) {
-            if (zero_start) stv(z, c, 0.f);
+        const bool mine = ((i + j + k) & 1) == color;
+        const uint8_t a = buf_ldu8(o.act, c);
+        const float rv = bldv(br, c, tag), rd = buf_ldf(o.rd, c);
+        const float of = zero_start ? 0.f : coarse_row<T>(o, bz, c, si, sj);
+        if (late(stop)) return;
+        if (!mine) {
+            if (zero_start) bstv(bz, c, 0.f, tag);
             continue;
         }
-        if (!L.active[c]) {
-            stv(z, c, 0.f);
-            continue;
-        }
-        const float of = zero_start ? 0.f : rowc<T>(L, z, i, j, k, c);
-        stv(z, c, (ldv(r, c) + of) * L.rdiag[c]);
+        bstv(bz, c, a ? (rv + of) * rd : 0.f, tag);
     }
 }
 
 // ---- fused residual + restriction: rc = P^T (r - A z) ------------------------
+// One thread per (coarse cell, child) pair: lanes 8I..8I+7 of a wave compute
+// the residuals of the up-to-8 fine cells of aggregate I (children beyond an
+// odd fine extent give 0) and sum them with an xor-shuffle tree, which every
+// lane evaluates in the same order (deterministic).
 template <typename TF, typename TC, bool L0>
 __global__ void __launch_bounds__(kBlock) k_mg_restrict(Geo g, MgLevel F, MgLevel C, const TF *z,
                                                         const TF *r, TC *rc, const MgState *m)
 {
-    if (stopped(m)) return;
+    const uint32_t stop = stop_flags(m);  // tested after the first loads
     uint32_t sb, se;
-    block_span(C.n, xcd_block(), sb, se);
-    for (uint32_t I = sb + threadIdx.x; I < se; I += kBlock) {
+    block_span(C.n * 8u, xcd_block(), sb, se);  // spans are whole multiples of 8
+    if (L0) {
+        // level 0 (split layout): the colour of a child is (a+b+d)&1, so all
+        // loads of kU children are issued before any is used
+        const uint32_t si = (uint32_t)g.ny * g.nz, sj = g.nz, H = F.H;
+        const float dint = diag0m(g, kInterior);
+        const float *zs = (const float *)z, *rs = (const float *)r;
+        const auto bred = buf_rsrc(zs, 4u * H), bblack = buf_rsrc(zs + H, 4u * (g.nc - H));
+        for (uint32_t t0 = sb + threadIdx.x; t0 < se; t0 += kU * kBlock) {
+            uint16_t nb[kU];
+            float rv[kU], zc[kU], zv[kU][6];
+            bool in[kU];
+#pragma unroll
+            for (int u = 0; u < kU; u++) {
+                const uint32_t t = t0 + u * kBlock < se ? t0 + u * kBlock : t0;
+                const uint32_t I = t >> 3, q = t & 7u;
+                int ic, jc, kc;
+                decode(I, C.f_nynz, C.f_nz, C.ny, C.nz, ic, jc, kc);
+                const int a = 2 * ic + (int)(q >> 2), b = 2 * jc + (int)((q >> 1) & 1u),
+                          d = 2 * kc + (int)(q & 1u);
+                in[u] = a < F.nx && b < F.ny && d < F.nz;
+                const uint32_t c = in[u] ? ((uint32_t)a * F.ny + b) * F.nz + d : 0u;
+                const uint32_t col = in[u] ? (uint32_t)(a + b + d) & 1u : 0u;
+                const uint32_t own = (col ? H : 0u) + (c >> 1);
+                nb[u] = g.nbm[c];
+                rv[u] = rs[own];
+                zc[u] = zs[own];
+                nb_gather(col ? bred : bblack, c, si, sj, zv[u]);
+            }
+            if (late(stop)) return;
+#pragma unroll
+            for (int u = 0; u < kU; u++) {
+                const uint32_t t = t0 + u * kBlock;
+                if (t >= se) break;
+                float v = 0.f;
+                if (in[u] && (nb[u] & NB_SELF))
+                    v = rv[u] - (diag_of(g, nb[u], dint) * zc[u] - nb_sum(g, nb[u], zv[u]));
+                v += __shfl_xor(v, 1);
+                v += __shfl_xor(v, 2);
+                v += __shfl_xor(v, 4);
+                if ((t & 7u) == 0) stv(rc, t >> 3, v);
+            }
+        }
+        return;
+    }
+    const TF *tag = nullptr;
+    const uint32_t si = (uint32_t)F.ny * F.nz, sj = F.nz;
+    const CoarseOp<TF> o = coarse_op<TF>(F);
+    const auto bz = buf_rsrc(z, (uint32_t)sizeof(TF) * F.n), br = buf_rsrc(r, (uint32_t)sizeof(TF) * F.n);
+    for (uint32_t t = sb + threadIdx.x; t < se; t += kBlock) {
+        const uint32_t I = t >> 3, q = t & 7u;
         int ic, jc, kc;
         decode(I, C.f_nynz, C.f_nz, C.ny, C.nz, ic, jc, kc);
-        const int i1 = min(2 * ic + 1, F.nx - 1), j1 = min(2 * jc + 1, F.ny - 1),
-                  k1 = min(2 * kc + 1, F.nz - 1);
-        float sum = 0.f;
-        for (int a = 2 * ic; a <= i1; a++)
-            for (int b = 2 * jc; b <= j1; b++)
-                for (int d = 2 * kc; d <= k1; d++) {
-                    const uint32_t c = ((uint32_t)a * F.ny + b) * F.nz + d;
-                    if (L0) {
-                        const uint16_t nb = g.nbm[c];
-                        if (!(nb & NB_SELF)) continue;
-                        sum += ldv(r, c) - (diag0m(g, nb) * ldv(z, c) - off0m(g, z, c, nb));
-                    } else {
-                        if (!F.active[c]) continue;
-                        const float of = rowc<TF>(F, z, a, b, d, c);
-                        sum += ldv(r, c) - (F.diag[c] * ldv(z, c) - of);
-                    }
-                }
-        stv(rc, I, sum);
+        const int a = 2 * ic + (int)(q >> 2), b = 2 * jc + (int)((q >> 1) & 1u),
+                  d = 2 * kc + (int)(q & 1u);
+        const bool in = a < F.nx && b < F.ny && d < F.nz;
+        const uint32_t c = in ? ((uint32_t)a * F.ny + b) * F.nz + d : 0u;
+        const uint8_t act = buf_ldu8(o.act, c);
+        const float rv = bldv(br, c, tag), zv = bldv(bz, c, tag), dg = buf_ldf(o.dg, c);
+        const float of = coarse_row<TF>(o, bz, c, si, sj);
+        if (late(stop)) return;
+        float v = (in && act) ? rv - (dg * zv - of) : 0.f;
+        v += __shfl_xor(v, 1);
+        v += __shfl_xor(v, 2);
+        v += __shfl_xor(v, 4);
+        if (q == 0) stv(rc, I, v);
     }
 }
 
@@ -223,16 +441,19 @@ template <typename TF, typename TC, bool L0>
 __global__ void __launch_bounds__(kBlock) k_mg_prolong(Geo g, MgLevel F, MgLevel C, TF *z,
                                                        const TC *zc, const MgState *m)
 {
-    if (stopped(m)) return;
+    const uint32_t stop = stop_flags(m);
     uint32_t sb, se;
     block_span(F.n, xcd_block(), sb, se);
     for (uint32_t c = sb + threadIdx.x; c < se; c += kBlock) {
-        const bool act = L0 ? (g.nbm[c] & NB_SELF) != 0 : F.active[c] != 0;
-        if (!act) continue;
         int i, j, k;
         decode(c, F.f_nynz, F.f_nz, F.ny, F.nz, i, j, k);
         const uint32_t P = ((uint32_t)(i >> 1) * C.ny + (j >> 1)) * C.nz + (k >> 1);
-        stv(z, c, ldv(z, c) + ldv(zc, P));
+        // level 0: split layout, colour (i+j+k)&1
+        const uint32_t zi = L0 ? (((i + j + k) & 1) ? F.H : 0u) + (c >> 1) : c;
+        const bool act = L0 ? (g.nbm[c] & NB_SELF) != 0 : F.active[c] != 0;
+        const float zv = ldv(z, zi), cv = ldv(zc, P);
+        if (late(stop)) return;
+        if (act) stv(z, zi, zv + cv);
     }
 }
 
@@ -242,6 +463,7 @@ __global__ void __launch_bounds__(1024) k_mg_coarse(Geo g, MgLevel L, T *z, cons
                                                     const MgState *m)
 {
     if (stopped(m)) return;
+    // level 0 (a single-level hierarchy) uses the split layout: slots [0, n)
     for (uint32_t c = threadIdx.x; c < L.n; c += blockDim.x) stv(z, c, 0.f);
     __syncthreads();
     // (red, black)^sweeps then (black, red)^sweeps: a symmetric operator
@@ -255,7 +477,9 @@ __global__ void __launch_bounds__(1024) k_mg_coarse(Geo g, MgLevel L, T *z, cons
             if (L0) {
                 const uint16_t nb = g.nbm[c];
                 if (!(nb & NB_SELF)) continue;
-                stv(z, c, (ldv(r, c) + off0m(g, z, c, nb)) / diag0m(g, nb));
+                const float *zs = (const float *)z;
+                const uint32_t sc = sown(c, nb, L.H);
+                stv(z, sc, (ldv(r, sc) + off0s(g, zs + sother(nb, L.H), c, nb)) / diag0m(g, nb));
             } else {
                 if (!L.active[c]) continue;
                 stv(z, c, (ldv(r, c) + rowc<T>(L, z, i, j, k, c)) * L.rdiag[c]);
@@ -265,26 +489,204 @@ __global__ void __launch_bounds__(1024) k_mg_coarse(Geo g, MgLevel L, T *z, cons
     }
 }
 
+// ---- the small levels in one workgroup (LDS-resident) --------------------------
+// The V-cycle from level tl.first down to the coarsest and back, with the same
+// passes as the per-level kernels (red-black sweeps, P^T restriction of the
+// residual, piecewise-constant prolongation, symmetric sweeps on the coarsest
+// level). Each level's operator is staged in LDS once and its z and r live in
+// LDS as T; phases are separated by workgroup barriers. Input r and output z
+// of level tl.first are exchanged with global memory.
+template <typename T>
+struct TailView {  // one level in LDS
+    uint32_t n;
+    int nx, ny, nz;
+    FastDiv f_nz, f_nynz;
+    float *cx, *cy, *cz, *diag, *rdiag;
+    T *z, *r;
+    uint8_t *act;
+};
+
+template <typename T>
+__device__ __forceinline__ TailView<T> tail_view(const TailLevels &tl, unsigned char *smem, int l)
+{
+    TailView<T> v;
+    const MgLevel &L = tl.lv[l];
+    v.n = L.n;
+    v.nx = L.nx;
+    v.ny = L.ny;
+    v.nz = L.nz;
+    v.f_nz = L.f_nz;
+    v.f_nynz = L.f_nynz;
+    float *f = reinterpret_cast<float *>(smem + tl.off[l]);
+    v.cx = f;
+    v.cy = f + v.n;
+    v.cz = f + 2 * v.n;
+    v.diag = f + 3 * v.n;
+    v.rdiag = f + 4 * v.n;
+    v.z = reinterpret_cast<T *>(f + 5 * v.n);
+    v.r = v.z + v.n;
+    v.act = reinterpret_cast<uint8_t *>(v.r + v.n);
+    return v;
+}
+
+template <typename T>
+__device__ __forceinline__ float tail_row(const TailView<T> &L, const T *x, int i, int j, int k,
+                                          uint32_t c)
+{
+    const uint32_t si = (uint32_t)L.ny * L.nz, sj = L.nz;
+    float of = 0.f;
+    if (i > 0) of += L.cx[c] * ldv(x, c - si);
+    if (i < L.nx - 1) of += L.cx[c + si] * ldv(x, c + si);
+    if (j > 0) of += L.cy[c] * ldv(x, c - sj);
+    if (j < L.ny - 1) of += L.cy[c + sj] * ldv(x, c + sj);
+    if (k > 0) of += L.cz[c] * ldv(x, c - 1);
+    if (k < L.nz - 1) of += L.cz[c + 1] * ldv(x, c + 1);
+    return of;
+}
+
+template <typename T>
+__device__ __forceinline__ void tail_smooth(const TailView<T> &L, int color, int zero_start)
+{
+    for (uint32_t c = threadIdx.x; c < L.n; c += kTailThreads) {
+        int i, j, k;
+        decode(c, L.f_nynz, L.f_nz, L.ny, L.nz, i, j, k);
+        if (((i + j + k) & 1) != color) {
+            if (zero_start) stv(L.z, c, 0.f);
+            continue;
+        }
+        if (!L.act[c]) {
+            stv(L.z, c, 0.f);
+            continue;
+        }
+        const float of = zero_start ? 0.f : tail_row(L, L.z, i, j, k, c);
+        stv(L.z, c, (ldv(L.r, c) + of) * L.rdiag[c]);
+    }
+    __syncthreads();
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kTailThreads) k_mg_tail(TailLevels tl, const MgState *m)
+{
+    extern __shared__ __align__(16) unsigned char tail_smem[];
+    if (stopped(m)) return;
+    for (int l = tl.first; l <= tl.last; l++) {  // stage the operators
+        const MgLevel &G = tl.lv[l];
+        const TailView<T> L = tail_view<T>(tl, tail_smem, l);
+        for (uint32_t c = threadIdx.x; c < L.n; c += kTailThreads) {
+            L.cx[c] = G.cx[c];
+            L.cy[c] = G.cy[c];
+            L.cz[c] = G.cz[c];
+            L.diag[c] = G.diag[c];
+            L.rdiag[c] = G.rdiag[c];
+            L.act[c] = G.active[c];
+        }
+    }
+    {
+        const TailView<T> F = tail_view<T>(tl, tail_smem, tl.first);
+        const T *rg = (const T *)tl.lv[tl.first].r;
+        for (uint32_t c = threadIdx.x; c < F.n; c += kTailThreads) F.r[c] = rg[c];
+    }
+    __syncthreads();
+    // down: pre-smoothing and restriction
+    for (int l = tl.first; l < tl.last; l++) {
+        const TailView<T> F = tail_view<T>(tl, tail_smem, l), C = tail_view<T>(tl, tail_smem, l + 1);
+        for (int sw = 0; sw < tl.sweeps; sw++) {
+            tail_smooth<T>(F, 0, sw == 0);
+            tail_smooth<T>(F, 1, 0);
+        }
+        for (uint32_t I = threadIdx.x; I < C.n; I += kTailThreads) {
+            int ic, jc, kc;
+            decode(I, C.f_nynz, C.f_nz, C.ny, C.nz, ic, jc, kc);
+            const int i1 = min(2 * ic + 1, F.nx - 1), j1 = min(2 * jc + 1, F.ny - 1),
+                      k1 = min(2 * kc + 1, F.nz - 1);
+            float sum = 0.f;
+            for (int a = 2 * ic; a <= i1; a++)
+                for (int b = 2 * jc; b <= j1; b++)
+                    for (int d = 2 * kc; d <= k1; d++) {
+                        const uint32_t c = ((uint32_t)a * F.ny + b) * F.nz + d;
+                        if (!F.act[c]) continue;
+                        const float of = tail_row(F, F.z, a, b, d, c);
+                        sum += ldv(F.r, c) - (F.diag[c] * ldv(F.z, c) - of);
+                    }
+            stv(C.r, I, sum);
+        }
+        __syncthreads();
+    }
+    // coarsest: (red, black)^s then (black, red)^s from z = 0
+    {
+        const TailView<T> L = tail_view<T>(tl, tail_smem, tl.last);
+        for (uint32_t c = threadIdx.x; c < L.n; c += kTailThreads) stv(L.z, c, 0.f);
+        __syncthreads();
+        for (int pass = 0; pass < 4 * tl.coarse_sweeps; pass++) {
+            const int half = pass / (2 * tl.coarse_sweeps);
+            tail_smooth<T>(L, ((pass & 1) ^ half) & 1, 0);
+        }
+    }
+    // up: prolongation and post-smoothing
+    for (int l = tl.last - 1; l >= tl.first; l--) {
+        const TailView<T> F = tail_view<T>(tl, tail_smem, l), C = tail_view<T>(tl, tail_smem, l + 1);
+        for (uint32_t c = threadIdx.x; c < F.n; c += kTailThreads) {
+            if (!F.act[c]) continue;
+            int i, j, k;
+            decode(c, F.f_nynz, F.f_nz, F.ny, F.nz, i, j, k);
+            const uint32_t P = ((uint32_t)(i >> 1) * C.ny + (j >> 1)) * C.nz + (k >> 1);
+            stv(F.z, c, ldv(F.z, c) + ldv(C.z, P));
+        }
+        __syncthreads();
+        for (int sw = 0; sw < tl.sweeps; sw++) {
+            tail_smooth<T>(F, 1, 0);
+            tail_smooth<T>(F, 0, 0);
+        }
+    }
+    {
+        const TailView<T> F = tail_view<T>(tl, tail_smem, tl.first);
+        T *zg = (T *)tl.lv[tl.first].z;
+        for (uint32_t c = threadIdx.x; c < F.n; c += kTailThreads) zg[c] = F.z[c];
+    }
+}
+
 // ---- inner PCG on level 0 (FP32 vectors, FP64 dots) -------------------------
-// q = A p, partials of p.q
-__global__ void __launch_bounds__(kBlock) k_pcg_spmv(Geo g, const float *p, float *q,
-                                                     const MgState *m, RedScratch red)
+// q = A p, partials of p.q (split layout: slot s of colour s >= H); loads as
+// in k_mg_smooth0
+__global__ void __launch_bounds__(kBlock) k_pcg_spmv(Geo g, MgLevel L, const float *__restrict__ p,
+                                                     float *__restrict__ q, const MgState *m,
+                                                     RedScratch red)
 {
     __shared__ double lds[kBlock / kWave];
-    if (stopped(m)) return;
-    const uint32_t blk = xcd_block();
+    const uint32_t stop = stop_flags(m);  // tested after the first loads
+    const uint32_t blk = xcd_block(), H = L.H;
+    const uint32_t si = (uint32_t)g.ny * g.nz, sj = g.nz;
+    const float dint = diag0m(g, kInterior);
+    const auto bcode = buf_rsrc(L.nbs, 2u * g.nc), bp = buf_rsrc(p, 4u * g.nc),
+               bred = buf_rsrc(p, 4u * H), bblack = buf_rsrc(p + H, 4u * (g.nc - H)),
+               bq = buf_rsrc(q, 4u * g.nc);
     double acc = 0.0;
     uint32_t sb, se;
     block_span(g.nc, blk, sb, se);
-    for (uint32_t c = sb + threadIdx.x; c < se; c += kBlock) {
-        float qv = 0.f;
-        const uint16_t nb = g.nbm[c];
-        if (nb & NB_SELF) {
-            const float pc = p[c];
-            qv = diag0m(g, nb) * pc - off0m(g, p, c, nb);
-            acc += (double)pc * (double)qv;
+    for (uint32_t s0 = sb + threadIdx.x; s0 < se; s0 += kU * kBlock) {
+        uint16_t nb[kU];
+        float pv[kU], zv[kU][6];
+#pragma unroll
+        for (int t = 0; t < kU; t++) {
+            const uint32_t s = s0 + t * kBlock;
+            const uint32_t col = s >= H ? 1u : 0u;
+            const uint32_t h = s - (col ? H : 0u);
+            nb[t] = buf_ldu16(bcode, s);
+            pv[t] = buf_ldf(bp, s);
+            nb_gather(col ? bred : bblack, 2 * h + split_parity(L, h, col), si, sj, zv[t]);
         }
-        q[c] = qv;
+        if (late(stop)) return;  // uniform: every thread of the grid returns
+#pragma unroll
+        for (int t = 0; t < kU; t++) {
+            const uint32_t s = s0 + t * kBlock;
+            if (s >= se) break;
+            float qv = 0.f;
+            if (nb[t] & NB_SELF) {
+                qv = diag_of(g, nb[t], dint) * pv[t] - nb_sum(g, nb[t], zv[t]);
+                acc += (double)pv[t] * (double)qv;
+            }
+            buf_stf(bq, s, qv);
+        }
     }
     block_partial(acc, pslot(red, SLOT_SPMV), blk, lds);
 }
@@ -450,19 +852,22 @@ __global__ void __launch_bounds__(kBlock) k_dc_residual(Geo g, const double *__r
     double acc = 0.0;
     uint32_t sb, se;
     block_span(g.nc, blk, sb, se);
+    const uint32_t H = (g.nc + 1) / 2;
     for (uint32_t c = sb + threadIdx.x; c < se; c += kBlock) {
         const uint16_t nb = g.nbm[c];
+        const uint32_t sc = sown(c, nb, H);  // e and r32: split layout
         if (!(nb & NB_SELF)) {
             xout[c] = 0.0;
-            r32[c] = 0.f;
+            r32[sc] = 0.f;
             continue;
         }
+        const float *eo = e + sother(nb, H);
         double diag, off;
-        row64(g, c, nb, [&](uint32_t q) { return xin[q] + (double)e[q]; }, diag, off);
-        const double xc = xin[c] + (double)e[c];
+        row64(g, c, nb, [&](uint32_t q) { return xin[q] + (double)eo[q >> 1]; }, diag, off);
+        const double xc = xin[c] + (double)e[sc];
         const double rv = b[c] - (diag * xc - off);
         xout[c] = xc;
-        r32[c] = (float)rv;
+        r32[sc] = (float)rv;
         acc += rv * rv;
     }
     block_partial(acc, pslot(red, SLOT_UPDATE), blk, lds);
@@ -520,18 +925,25 @@ void mg_launch_build(hipStream_t s, const Geo &g, const MgLevel &F, const MgLeve
         FFR_LAUNCH(k_mg_build<false>, dim3(grid_for(C.n)), dim3(kBlock), 0, s, g, F, C);
 }
 
+void mg_launch_split_codes(hipStream_t s, const Geo &g, const MgLevel &L0)
+{
+    FFR_LAUNCH(k_mg_split_codes, dim3(grid_for(g.nc)), dim3(kBlock), 0, s, g, L0.nbs, L0.H);
+}
+
 template <typename T>
 void mg_launch_smooth(hipStream_t s, const Geo &g, const MgLevel &L, bool l0, int color,
                       int zero_start, const MgState *m)
 {
-    T *z = (T *)L.z;
-    const T *r = (const T *)L.r;
-    if (l0)
-        FFR_LAUNCH((k_mg_smooth<T, true>), dim3(grid_for(L.n)), dim3(kBlock), 0, s, g, L, z, r,
-                           color, zero_start, m);
+    if (l0) {  // level 0 is FP32 in the split layout
+        const uint32_t H = L.H, cnt = color ? g.nc - H : H, base = color ? H : 0u;
+        float *z = (float *)L.z;
+        FFR_LAUNCH(k_mg_smooth0, dim3(grid_for(cnt)), dim3(kBlock), 0, s, g, L, L.nbs + base,
+                   z + (color ? 0u : H), z + base, (const float *)L.r + base, cnt, g.nc - cnt,
+                   (uint32_t)color, zero_start, m);
+    }
     else
-        FFR_LAUNCH((k_mg_smooth<T, false>), dim3(grid_for(L.n)), dim3(kBlock), 0, s, g, L, z,
-                           r, color, zero_start, m);
+        FFR_LAUNCH(k_mg_smooth<T>, dim3(grid_for(L.n)), dim3(kBlock), 0, s, L, (T *)L.z,
+                   (const T *)L.r, color, zero_start, m);
 }
 
 template <typename TF, typename TC>
@@ -539,10 +951,10 @@ void mg_launch_restrict(hipStream_t s, const Geo &g, const MgLevel &F, const MgL
                         const MgState *m)
 {
     if (l0)
-        FFR_LAUNCH((k_mg_restrict<TF, TC, true>), dim3(grid_for(C.n)), dim3(kBlock), 0, s, g, F,
+        FFR_LAUNCH((k_mg_restrict<TF, TC, true>), dim3(grid_for(8ull * C.n)), dim3(kBlock), 0, s, g, F,
                            C, (const TF *)F.z, (const TF *)F.r, (TC *)C.r, m);
     else
-        FFR_LAUNCH((k_mg_restrict<TF, TC, false>), dim3(grid_for(C.n)), dim3(kBlock), 0, s, g,
+        FFR_LAUNCH((k_mg_restrict<TF, TC, false>), dim3(grid_for(8ull * C.n)), dim3(kBlock), 0, s, g,
                            F, C, (const TF *)F.z, (const TF *)F.r, (TC *)C.r, m);
 }
 
@@ -556,6 +968,12 @@ void mg_launch_prolong(hipStream_t s, const Geo &g, const MgLevel &F, const MgLe
     else
         FFR_LAUNCH((k_mg_prolong<TF, TC, false>), dim3(grid_for(F.n)), dim3(kBlock), 0, s, g,
                            F, C, (TF *)F.z, (const TC *)C.z, m);
+}
+
+template <typename T>
+void mg_launch_tail(hipStream_t s, const TailLevels &tl, size_t lds_bytes, const MgState *m)
+{
+    FFR_LAUNCH(k_mg_tail<T>, dim3(1), dim3(kTailThreads), lds_bytes, s, tl, m);
 }
 
 template <typename T>
@@ -578,12 +996,15 @@ template void mg_launch_restrict<__hip_bfloat16, __hip_bfloat16>(hipStream_t, co
 template void mg_launch_prolong<float, float>(hipStream_t, const Geo &, const MgLevel &, const MgLevel &, bool, const MgState *);
 template void mg_launch_prolong<float, __hip_bfloat16>(hipStream_t, const Geo &, const MgLevel &, const MgLevel &, bool, const MgState *);
 template void mg_launch_prolong<__hip_bfloat16, __hip_bfloat16>(hipStream_t, const Geo &, const MgLevel &, const MgLevel &, bool, const MgState *);
+template void mg_launch_tail<float>(hipStream_t, const TailLevels &, size_t, const MgState *);
+template void mg_launch_tail<__hip_bfloat16>(hipStream_t, const TailLevels &, size_t, const MgState *);
 template void mg_launch_coarse<float>(hipStream_t, const Geo &, const MgLevel &, bool, int, const MgState *);
 template void mg_launch_coarse<__hip_bfloat16>(hipStream_t, const Geo &, const MgLevel &, bool, int, const MgState *);
 
-void pcg_launch_spmv(hipStream_t s, const Geo &g, const float *p, float *q, const MgState *m, RedScratch red)
+void pcg_launch_spmv(hipStream_t s, const Geo &g, const MgLevel &L0, const float *p, float *q,
+                     const MgState *m, RedScratch red)
 {
-    FFR_LAUNCH(k_pcg_spmv, dim3(grid_for(g.nc)), dim3(kBlock), 0, s, g, p, q, m, red);
+    FFR_LAUNCH(k_pcg_spmv, dim3(grid_for(g.nc)), dim3(kBlock), 0, s, g, L0, p, q, m, red);
 }
 void pcg_launch_update(hipStream_t s, uint32_t n, float *e, float *r, const float *p, const float *q,
                        MgState *m, RedScratch red, int it)
