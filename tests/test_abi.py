@@ -46,10 +46,16 @@ def test_footprint_scales_with_precision_and_grid():
                              outflows=[(0, 1, 0, 47, 0, 23)])
     L = capi.lib()
     f64 = L.ffr_scene_footprint(ctypes.byref(d), ctypes.byref(capi.default_params()))
-    fmx = L.ffr_scene_footprint(ctypes.byref(d), ctypes.byref(capi.default_params(precision=1)))
+    f32cg = L.ffr_scene_footprint(ctypes.byref(d), ctypes.byref(
+        capi.default_params(precision=1, poisson_method=capi.POISSON_CG)))
+    fmg = L.ffr_scene_footprint(ctypes.byref(d), ctypes.byref(
+        capi.default_params(precision=1, poisson_method=capi.POISSON_MGPCG)))
     n = 64 * 48 * 24
     assert 90 * n < f64 < 140 * n   # FP64 form: ~105 B/cell
-    assert 0 < fmx < f64
+    assert 0 < f32cg < f64          # FP32 fields, same solver
+    # the multigrid hierarchy adds ~22 B/cell on level 0 (five FP32 vectors,
+    # split codes) plus < 1/7 of that on the coarse levels
+    assert f32cg < fmg < f32cg + 30 * n
     # invalid descriptors are rejected (footprint 0), e.g. a patch off the wall
     d2, keep2 = capi.make_desc(64, 48, 24, h(64, 10), h(48, 8), h(24, 3), flags,
                                outflows=[(0, 1, 0, 48, 0, 23)])
