@@ -114,6 +114,51 @@ def form_params_inplace(p, mode):
         p.poisson_tol, p.poisson_max_iter = 1e-10, 50
 
 
+def pressure_errors(ctx, ps, ps64, d_mixed):
+    """Relative L2 pressure error of both forms on a realistic system: x* is
+    the pressure of the mixed scene's last step, rhs = A x* with the FP64 GPU
+    operator (A is nonsingular: OUTFLOW cells are Dirichlet neighbours), and
+    each form solves A x = rhs from x = 0. SOLID cells are excluded."""
+    from ff_room import device
+    xs = d_mixed.download()["p"]
+    rhs = d_mixed.laplacian(xs)
+    fluid = ps.flags != 1
+    out = {"reference": "x* = p after the timed mixed steps; rhs = A x* (FP64 operator)"}
+    for name, d in (("mixed", d_mixed), ("fp64_cg", None)):
+        if d is None:
+            d = device.create(ctx, ps64)
+        x, info = d.poisson(rhs)
+        err = float(np.linalg.norm((x - xs)[fluid]) / np.linalg.norm(xs[fluid]))
+        out[name] = {"rel_l2": err, "iterations": info["iterations"],
+                     "outer": info["outer_iterations"], "residual": info["residual"]}
+        if name == "fp64_cg":
+            d.close()
+    return out
+
+
+def serve_bench(ctx, n_c4: int, steps_c4: int, n_c5: int, steps_c5: int):
+    """Continuous batching samples: c4 (uneven rooms all submitted at once)
+    and c5 (a seeded Poisson stream at 50 requests/s), mixed MG-PCG form."""
+    from ff_room import capi
+    from ff_room.generator import workload_c4, workload_c5
+    from ff_room.serve import Request, Server, summary
+    res = {}
+    kw = dict(precision=capi.PREC_MIXED, poisson_method=capi.POISSON_MGPCG,
+              poisson_tol=1e-10, poisson_max_iter=500)
+    cfgs = workload_c4(n_scenes=n_c4, steps=steps_c4, precision="mixed")
+    srv = Server(ctx, slots=16, **kw)
+    done = srv.run([Request(c, steps_c4) for c in cfgs], realtime=False)
+    res["c4"] = dict(summary(done), sample=f"{n_c4} seeded rooms (grids 32x32x16..160x128x64), "
+                     f"{steps_c4} steps each, all submitted at t=0, 16 slots")
+    stream = list(workload_c5(n_requests=n_c5, steps=steps_c5, precision="mixed"))
+    srv = Server(ctx, slots=16, **kw)
+    done = srv.run([Request(c, steps_c5, t) for t, c in stream], realtime=True)
+    res["c5"] = dict(summary(done), sample=f"first {n_c5} requests of the seeded stream "
+                     f"(50 req/s, grids 24^3..256x204x80, <= {steps_c5} steps or "
+                     f"max|du|/dt < 1e-4), 16 slots")
+    return res
+
+
 def cpu_baseline(ps, steps: int = 1):
     """The reference's own FluidSolver::step (oracle/_ref, compiled from the
     reference sources with OpenMP), timed on this host."""
@@ -160,6 +205,11 @@ def main():
     ap.add_argument("--fp64-steps", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-compare", action="store_true")
+    ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--c4", type=int, default=64, help="rooms in the batched sample")
+    ap.add_argument("--c4-steps", type=int, default=20)
+    ap.add_argument("--c5", type=int, default=40, help="requests in the stream sample")
+    ap.add_argument("--c5-steps", type=int, default=200)
     ap.add_argument("--kernel-only", default=None,
                     help="time one kernel (for rocprofv3 --pmc runs) and exit")
     args = ap.parse_args()
@@ -192,7 +242,7 @@ def main():
     # ---- headline form -------------------------------------------------------
     d = device.create(ctx, ps)
     wall, res, _ = timed_steps(d, ctx, args.steps, args.warmup, prof=False)
-    d.close()
+    d_keep = d
     value = ps.cells * args.steps / wall
     iters = [r["poisson_iters"] for r in res]
     log(f"{args.mode}: {args.steps} steps in {wall:.3f} s -> {value:.4g} cell-updates/s; "
@@ -227,6 +277,9 @@ def main():
     if args.mode == "mixed" and not args.no_compare:
         ps64 = device.prepare(cfg)
         form_params_inplace(ps64.params, "fp64")
+        out["p_error"] = pressure_errors(ctx, ps, ps64, d_keep)
+        log(f"pressure error vs x*: mixed {out['p_error']['mixed']['rel_l2']:.2e}, "
+            f"fp64 CG {out['p_error']['fp64_cg']['rel_l2']:.2e}")
         d = device.create(ctx, ps64)
         w64, r64, _ = timed_steps(d, ctx, args.fp64_steps, 1, prof=False)
         d.close()
@@ -235,6 +288,13 @@ def main():
                           "iterations": [r["poisson_iters"] for r in r64]}
         out["speedup_vs_fp64_cg"] = value / v64
         log(f"fp64 CG: {v64:.4g} cell-updates/s -> mixed/fp64 = {value / v64:.1f}x")
+
+    d_keep.close()
+    if args.mode == "mixed" and not args.no_serve:
+        out["serving"] = serve_bench(ctx, args.c4, args.c4_steps, args.c5, args.c5_steps)
+        for k, v in out["serving"].items():
+            log(f"serving {k}: {v['cell_updates_per_s']:.4g} cell-updates/s, p50 "
+                f"{v['p50_latency_s']:.3f} s, p99 {v['p99_latency_s']:.3f} s over {v['requests']} requests")
 
     if not args.no_cpu:
         log("timing the reference CPU solver on a bounded sample ...")

@@ -80,6 +80,7 @@ void ffr_params_default(ffr_params *p)
     p->mg_coarse_sweeps = 8;
     p->inner_tol = 1e-3;
     p->inner_max_iter = 60;
+    p->graphs = 1;
 }
 
 int ffr_ctx_create(int device, size_t hbm_budget, ffr_ctx **out)

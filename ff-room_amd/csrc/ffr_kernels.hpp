@@ -27,7 +27,8 @@ struct StepRec {
 struct CgState {
     double rr[2];
     double bnorm, res;
-    int done, iters, max_iter, pad;
+    int done, iters, max_iter;
+    int started;  // iterations begun (bumped by the SpMV, the first kernel of one)
     double tol;
 };
 
@@ -94,9 +95,9 @@ void launch_cg_init(hipStream_t s, const Geo &g, const double *b, double *x, dou
 void launch_cg_spmv(hipStream_t s, const Geo &g, const double *d, double *Ad, CgState *cg,
                     RedScratch red);
 void launch_cg_update(hipStream_t s, uint32_t n, double *x, double *r, const double *d,
-                      const double *Ad, CgState *cg, RedScratch red, int it);
+                      const double *Ad, CgState *cg, RedScratch red);
 void launch_cg_dir(hipStream_t s, uint32_t n, double *d, const double *r, CgState *cg,
-                   RedScratch red, int it);
+                   RedScratch red);
 // Plain A*x (PoissonSolver.cpp:22-91) in FP64.
 void launch_laplacian(hipStream_t s, const Geo &g, const double *x, double *Ax);
 

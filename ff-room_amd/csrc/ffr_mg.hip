@@ -161,6 +161,10 @@ int mg_create(ffr_scene *s)
 
 void mg_destroy(ffr_scene *s)
 {
+    if (s->mg.g_iter) hipGraphExecDestroy(s->mg.g_iter);
+    if (s->mg.g_start) hipGraphExecDestroy(s->mg.g_start);
+    s->mg.g_iter = s->mg.g_start = nullptr;
+    s->mg.graphs_tried = false;
     if (s->mg.arena) {
         hipFree(s->mg.arena);
         s->ctx->in_use -= s->mg.bytes;
@@ -264,21 +268,21 @@ static void precondition(ffr_scene *s)
     else vcycle<float>(s, 0);
 }
 
-static void inner_iteration(ffr_scene *s, int it)
+static void inner_iteration(ffr_scene *s)
 {
     Multigrid &M = s->mg;
     const Geo &g = s->geo;
     hipStream_t st = s->stream;
     const double n = (double)g.nc;
     {
-        ProfScope ps(s->ctx, st, "pcg_spmv", n * (4 + 1 + 4));
+        ProfScope ps(s->ctx, st, "pcg_spmv", n * (4 + 2 + 4));
         pcg_launch_spmv(st, g, M.lv[0], M.pi, M.qi, s->mgs, s->red);
     }
     {
         ProfScope ps(s->ctx, st, "pcg_update", n * 24);
-        pcg_launch_update(st, g.nc, M.e, M.ri, M.pi, M.qi, s->mgs, s->red, it);
+        pcg_launch_update(st, g.nc, M.e, M.ri, M.pi, M.qi, s->mgs, s->red);
     }
-    pcg_launch_check(st, g.nc, s->mgs, s->red, it);
+    pcg_launch_check(st, g.nc, s->mgs, s->red);
     precondition(s);
     {
         ProfScope ps(s->ctx, st, "pcg_dot_rz", n * 8);
@@ -286,11 +290,117 @@ static void inner_iteration(ffr_scene *s, int it)
     }
     {
         ProfScope ps(s->ctx, st, "pcg_dir", n * 12);
-        pcg_launch_dir(st, g.nc, (const float *)M.lv[0].z, M.pi, s->mgs, s->red, it);
+        pcg_launch_dir(st, g.nc, (const float *)M.lv[0].z, M.pi, s->mgs, s->red);
     }
 }
 
-int mg_solve(ffr_scene *s, int *iters, int *outer, double *res)
+// ---- the solve as a task (see ffr_task.hip) ------------------------------------
+// Outer pass k: r = b - A x in FP64 and the stopping test; between passes k
+// and k+1 the inner solve k (z = M r, p = z, k_i CG steps). A solve enqueues
+// the passes and inner iterations the previous solve of the scene needed and
+// then waits once; an inner solve with no history is grown under host control
+// (a few iterations, then two at a time until the device reports it done).
+
+static void inner_start(ffr_scene *s)
+{
+    Multigrid &M = s->mg;
+    hipStream_t st = s->stream;
+    const Geo &g = s->geo;
+    precondition(s);
+    pcg_launch_dot_rz(st, g.nc, M.ri, (const float *)M.lv[0].z, s->mgs, s->red);
+    pcg_launch_start(st, g.nc, (const float *)M.lv[0].z, M.pi, M.e, s->mgs, s->red);
+}
+
+// ---- graph replay ------------------------------------------------------------
+// Every launch of an inner iteration (and of an inner solve's start) has the
+// same arguments each time: iteration state lives on the device. Each is
+// captured once per scene as a graph and replayed, which replaces ~45 kernel
+// launches by one graph launch. No memset or host work is captured. The
+// profiler needs per-launch events, so profiled runs launch eagerly.
+static hipGraphExec_t capture(ffr_scene *s, void (*body)(ffr_scene *))
+{
+    hipStream_t st = s->stream;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    if (hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal) != hipSuccess) return nullptr;
+    body(s);
+    if (hipStreamEndCapture(st, &graph) != hipSuccess || !graph) return nullptr;
+    if (hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0) != hipSuccess) exec = nullptr;
+    hipGraphDestroy(graph);
+    return exec;
+}
+
+static bool graphs_ready(ffr_scene *s)
+{
+    Multigrid &M = s->mg;
+    if (!s->prm.graphs || s->ctx->prof_on) return false;
+    if (!M.graphs_tried) {
+        M.graphs_tried = true;
+        M.g_iter = capture(s, inner_iteration);
+        M.g_start = capture(s, inner_start);
+        hipGetLastError();  // a failed capture leaves the eager path
+    }
+    return M.g_iter && M.g_start;
+}
+
+static void launch_iterations(ffr_scene *s, int k)
+{
+    if (k > 0 && graphs_ready(s)) {
+        for (int i = 0; i < k; i++) hipGraphLaunch(s->mg.g_iter, s->stream);
+        return;
+    }
+    for (int i = 0; i < k; i++) inner_iteration(s);
+}
+
+// inner solve: z = M r, p = z, then k CG steps
+static void enqueue_inner(ffr_scene *s, int k)
+{
+    if (graphs_ready(s)) hipGraphLaunch(s->mg.g_start, s->stream);
+    else inner_start(s);
+    launch_iterations(s, k);
+}
+
+// outer pass: x += e, r = b - A x (FP64), r32 = r; stopping test
+static void enqueue_pass(ffr_scene *s)
+{
+    hipStream_t st = s->stream;
+    const Geo &g = s->geo;
+    const double n = (double)g.nc;
+    {
+        ProfScope ps(s->ctx, st, "dc_residual", n * (8 + 8 + 4 + 1 + 8 + 4));
+        dc_launch_residual(st, g, s->b, s->p, s->d, s->mg.e, s->mg.ri, s->mgs, s->red);
+    }
+    dc_launch_check(st, g, s->mgs, s->red);
+    s->task.passes_enq++;
+}
+
+static int max_inner_of(const ffr_scene *s) { return std::max(1, s->prm.inner_max_iter); }
+
+// enqueues passes up to task.target_passes, stopping early at an inner solve
+// with no history; ends with a wait point
+static int mg_schedule(ffr_scene *s)
+{
+    StepTask &t = s->task;
+    const int max_inner = max_inner_of(s);
+    while (t.passes_enq < t.target_passes) {
+        const int idx = t.passes_enq - 1;  // the inner solve after residual idx
+        int k = idx < kMgHist ? s->pred.mg_inner[idx] : 0;
+        if (k <= 0) {
+            k = std::min(max_inner, 6);
+            enqueue_inner(s, k);
+            if (idx < kMgHist) t.k_used[idx] = 0;
+            t.inner_wait = idx;
+            return wait_point(s, s->h_mgs, s->mgs, sizeof(MgState));
+        }
+        k = std::min(k, max_inner);
+        enqueue_inner(s, k);
+        if (idx < kMgHist) t.k_used[idx] = k;
+        enqueue_pass(s);
+    }
+    return wait_point(s, s->h_mgs, s->mgs, sizeof(MgState));
+}
+
+int mg_begin(ffr_scene *s)
 {
     if (!s->mg.levels) {
         int rc = mg_create(s);
@@ -299,56 +409,60 @@ int mg_solve(ffr_scene *s, int *iters, int *outer, double *res)
     Multigrid &M = s->mg;
     const Geo &g = s->geo;
     hipStream_t st = s->stream;
-    MgState *h = s->h_mgs;
+    StepTask &t = s->task;
     const double n = (double)g.nc;
-    const int max_outer = 50;
-    const int max_inner = std::max(1, s->prm.inner_max_iter);
-    const int total_cap = std::max(1, s->prm.poisson_max_iter);
-    double *X[2] = {s->p, s->d};
     {
         ProfScope ps(s->ctx, st, "dc_init", n * (8 + 8 + 4 + 1));
         dc_launch_init(st, g, s->b, s->p, M.e, s->red);
     }
-    dc_launch_start(st, g, s->mgs, s->red, s->prm.poisson_tol, max_outer, max_inner, s->prm.inner_tol);
-    int passes = 0;
-    for (int o = 0; o <= max_outer; o++) {
-        {
-            ProfScope ps(s->ctx, st, "dc_residual", n * (8 + 8 + 4 + 1 + 8 + 4));
-            dc_launch_residual(st, g, s->b, X[passes & 1], M.e, X[(passes + 1) & 1], M.ri, s->mgs, s->red);
+    dc_launch_start(st, g, s->mgs, s->red, s->prm.poisson_tol, /*max_outer=*/50, max_inner_of(s),
+                    s->prm.inner_tol, std::max(1, s->prm.poisson_max_iter));
+    t.passes_enq = 0;
+    t.inner_wait = -1;
+    t.target_passes = std::max(1, s->pred.mg_passes);
+    enqueue_pass(s);
+    return mg_schedule(s);
+}
+
+int mg_continue(ffr_scene *s, bool *finished)
+{
+    StepTask &t = s->task;
+    const MgState &h = *s->h_mgs;
+    *finished = false;
+    if (t.inner_wait >= 0) {  // an inner solve grown under host control
+        if (!h.solve_done && !h.inner_done && h.inner_it < max_inner_of(s)) {
+            launch_iterations(s, 2);
+            return wait_point(s, s->h_mgs, s->mgs, sizeof(MgState));
         }
-        dc_launch_check(st, g, s->mgs, s->red);
-        HIP_CHECK_CTX(s->ctx, hipMemcpyAsync(h, s->mgs, sizeof(MgState), hipMemcpyDeviceToHost, st), FFR_E_DEVICE);
-        HIP_CHECK_CTX(s->ctx, hipStreamSynchronize(st), FFR_E_DEVICE);
-        if (h->bnorm < 1e-15) break;  // nothing ran: x is 0 in X[0]
-        passes++;
-        if (h->solve_done || h->total_iters >= total_cap) break;
-        // inner solve: z = M r, p = z, then CG steps until the inner test stops it
-        precondition(s);
-        pcg_launch_dot_rz(st, g.nc, M.ri, (const float *)M.lv[0].z, s->mgs, s->red);
-        pcg_launch_start(st, g.nc, (const float *)M.lv[0].z, M.pi, M.e, s->mgs, s->red);
-        const int slot = std::min(o, 15);
-        int ahead = M.inner_pred[slot] > 0 ? M.inner_pred[slot] : 6;
-        ahead = std::min(ahead, max_inner);
-        int it = 0;
-        for (; it < ahead; it++) inner_iteration(s, it);
-        HIP_CHECK_CTX(s->ctx, hipMemcpyAsync(h, s->mgs, sizeof(MgState), hipMemcpyDeviceToHost, st), FFR_E_DEVICE);
-        HIP_CHECK_CTX(s->ctx, hipStreamSynchronize(st), FFR_E_DEVICE);
-        while (!h->inner_done && it < max_inner) {
-            inner_iteration(s, it++);
-            HIP_CHECK_CTX(s->ctx, hipMemcpyAsync(h, s->mgs, sizeof(MgState), hipMemcpyDeviceToHost, st), FFR_E_DEVICE);
-            HIP_CHECK_CTX(s->ctx, hipStreamSynchronize(st), FFR_E_DEVICE);
-        }
-        M.inner_pred[slot] = h->inner_it;
+        t.inner_wait = -1;
+        enqueue_pass(s);
+        return mg_schedule(s);
+    }
+    if (!h.solve_done) {  // the solve needs more passes than enqueued
+        t.target_passes = t.passes_enq + 1;
+        return mg_schedule(s);
     }
     {
-        ProfScope ps(s->ctx, st, "dc_finalize", n * (8 + 8 + 1));
-        dc_launch_finalize(st, g, s->b, X[passes & 1], s->p, s->prm.use_multigrid);
+        ProfScope ps(s->ctx, s->stream, "dc_finalize", (double)s->geo.nc * (8 + 8 + 1));
+        dc_launch_finalize(s->stream, s->geo, s->b, s->p, s->d, s->p, s->prm.use_multigrid, s->mgs);
     }
-    HIP_CHECK_CTX(s->ctx, hipMemcpyAsync(h, s->mgs, sizeof(MgState), hipMemcpyDeviceToHost, st), FFR_E_DEVICE);
-    HIP_CHECK_CTX(s->ctx, hipStreamSynchronize(st), FFR_E_DEVICE);
-    *iters = h->total_iters;
-    *outer = passes;
-    *res = h->res;
+    t.iters = h.total_iters;
+    t.outer = h.passes;
+    t.res = h.res;
+    // predictions for the next solve: an inner solve that used every enqueued
+    // iteration may have wanted more, so it gets one more next time
+    SolvePred &P = s->pred;
+    P.mg_passes = h.passes;
+    for (int k = 0; k < kMgHist; k++) {
+        if (k + 1 >= h.passes) {
+            P.mg_inner[k] = 0;
+            continue;
+        }
+        const int used = h.hist[k];
+        const bool saturated = t.k_used[k] > 0 && used >= t.k_used[k];
+        P.mg_inner[k] = std::min(max_inner_of(s), used + (saturated ? 1 : 0));
+    }
+    *finished = true;
     return FFR_OK;
 }
 
@@ -382,14 +496,14 @@ double mg_prof_launch(ffr_scene *s, const char *name, std::function<void()> *lau
     if (k == "pcg_update") {
         float *e = M.e, *r = M.ri, *p = M.pi, *q = M.qi;
         RedScratch red = s->red;
-        *launch = [=] { pcg_launch_update(st, g.nc, e, r, p, q, m, red, 0); };
+        *launch = [=] { pcg_launch_update(st, g.nc, e, r, p, q, m, red); };
         return n * 24;
     }
     if (k == "dc_residual") {
         double *b = s->b, *x0 = s->p, *x1 = s->d;
         float *e = M.e, *r = M.ri;
         RedScratch red = s->red;
-        *launch = [=] { dc_launch_residual(st, g, b, x0, e, x1, r, m, red); };
+        *launch = [=] { dc_launch_residual(st, g, b, x0, x1, e, r, m, red); };
         return n * (8 + 8 + 4 + 2 + 8 + 4);
     }
     *launch = [] {};

@@ -79,7 +79,10 @@ struct Multigrid {
     // their z and r in LDS (-1: none); tail_lds = its LDS bytes
     int tail_first = -1;
     size_t tail_lds = 0;
-    int inner_pred[16] = {0};  // inner iterations seen per outer pass (launch prediction)
+    // captured graphs: one inner PCG iteration, and the start of an inner
+    // solve (V-cycle on r, r.z, p = z); built on first use when enabled
+    hipGraphExec_t g_iter = nullptr, g_start = nullptr;
+    bool graphs_tried = false;
 };
 
 // The levels a single workgroup runs from LDS (see k_mg_tail). Level l
@@ -98,13 +101,21 @@ struct TailLevels {
     int sweeps, coarse_sweeps;
 };
 
-// Scalars of the inner PCG and the outer FP64 defect correction. Values the
-// inner iteration `it` reads are double-buffered by parity (see CgState).
+// Scalars of the inner PCG and the outer FP64 defect correction, all kept on
+// the device so that launches carry no per-iteration arguments (a solve can
+// be enqueued ahead of the host, or replayed from a captured graph). rz is
+// double-buffered by the parity of inner_it, the iteration count of the
+// current inner solve, which only the one-block check kernel advances.
+// passes counts FP64 residual evaluations; x ping-pongs between the two
+// outer buffers by its parity. hist[k] = iterations of inner solve k.
+constexpr int kMgHist = 16;
 struct MgState {
     double rz[2];
     double bnorm, res, rnorm0, inner_tol, outer_tol;
     int solve_done, inner_done, outer, inner_it;
-    int total_iters, max_outer, max_inner, pad;
+    int total_iters, max_outer, max_inner, passes;
+    int total_cap, pad;
+    int hist[kMgHist];
 };
 
 }  // namespace ffr

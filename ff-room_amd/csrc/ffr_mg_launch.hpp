@@ -23,22 +23,20 @@ void mg_launch_coarse(hipStream_t s, const Geo &g, const MgLevel &L, bool l0, in
 void pcg_launch_spmv(hipStream_t s, const Geo &g, const MgLevel &L0, const float *p, float *q,
                      const MgState *m, RedScratch red);
 void pcg_launch_update(hipStream_t s, uint32_t n, float *e, float *r, const float *p,
-                       const float *q, MgState *m, RedScratch red, int it);
-void pcg_launch_check(hipStream_t s, uint32_t n, MgState *m, RedScratch red, int it);
+                       const float *q, MgState *m, RedScratch red);
+void pcg_launch_check(hipStream_t s, uint32_t n, MgState *m, RedScratch red);
 void pcg_launch_dot_rz(hipStream_t s, uint32_t n, const float *r, const float *z, const MgState *m,
                        RedScratch red);
 void pcg_launch_start(hipStream_t s, uint32_t n, const float *z, float *p, float *e, MgState *m,
                       RedScratch red);
-void pcg_launch_dir(hipStream_t s, uint32_t n, const float *z, float *p, MgState *m,
-                    RedScratch red, int it);
+void pcg_launch_dir(hipStream_t s, uint32_t n, const float *z, float *p, MgState *m, RedScratch red);
 void dc_launch_init(hipStream_t s, const Geo &g, const double *b, double *x, float *e,
                     RedScratch red);
 void dc_launch_start(hipStream_t s, const Geo &g, MgState *m, RedScratch red, double outer_tol,
-                     int max_outer, int max_inner, double inner_tol);
-void dc_launch_residual(hipStream_t s, const Geo &g, const double *b, const double *xin,
-                        const float *e, double *xout, float *r32, const MgState *m,
-                        RedScratch red);
+                     int max_outer, int max_inner, double inner_tol, int total_cap);
+void dc_launch_residual(hipStream_t s, const Geo &g, const double *b, double *xa, double *xb,
+                        const float *e, float *r32, const MgState *m, RedScratch red);
 void dc_launch_check(hipStream_t s, const Geo &g, MgState *m, RedScratch red);
-void dc_launch_finalize(hipStream_t s, const Geo &g, const double *b, const double *x, double *p,
-                        int pin_outflow);
+void dc_launch_finalize(hipStream_t s, const Geo &g, const double *b, const double *xa,
+                        const double *xb, double *p, int pin_outflow, const MgState *m);
 }  // namespace ffr

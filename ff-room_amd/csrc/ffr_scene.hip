@@ -138,6 +138,9 @@ size_t layout(char *base, const ffr_scene_desc *d, const ffr_params *p, ArenaPtr
     return off;
 }
 
+}  // namespace
+
+namespace ffr {
 int resolve_method(const ffr_params &p)
 {
     if (p.poisson_method == FFR_POISSON_CG) return FFR_POISSON_CG;
@@ -162,16 +165,13 @@ StepParams make_step_params(const ffr_params &p)
     return sp;
 }
 
-inline double bits_to_double(unsigned long long b)
+double bits_to_double(unsigned long long b)
 {
     double d;
     std::memcpy(&d, &b, sizeof d);
     return d;
 }
 
-}  // namespace
-
-namespace ffr {
 size_t scene_bytes(const ffr_scene_desc *d, const ffr_params *p, bool with_mg)
 {
     size_t b = layout(nullptr, d, p, nullptr);
@@ -215,7 +215,13 @@ int ffr_scene_create(ffr_ctx *ctx, const ffr_scene_desc *d, const ffr_params *pr
     s->fans.assign(d->fans, d->fans + d->n_fans);
     s->outflows.assign(d->outflows, d->outflows + d->n_outflows);
     s->openings.assign(d->openings, d->openings + d->n_openings);
-    s->stream = ctx->stream;
+    // a stream per scene: the serving loop overlaps the steps of many scenes
+    if (hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess) {
+        ctx->fail("hipStreamCreate failed");
+        delete s;
+        return FFR_E_DEVICE;
+    }
+    s->own_stream = true;
 
     const size_t bytes = layout(nullptr, d, prm, nullptr);
     if (ctx->in_use + bytes > ctx->budget) {
@@ -444,7 +450,9 @@ void ffr_scene_destroy(ffr_scene *s)
 {
     if (!s) return;
     hipSetDevice(s->ctx->device);
+    serve_forget(s->ctx, s);
     hipStreamSynchronize(s->stream);
+    if (s->task.ev) hipEventDestroy(s->task.ev);
     mg_destroy(s);
     if (s->arena) {
         hipFree(s->arena);
@@ -453,6 +461,7 @@ void ffr_scene_destroy(ffr_scene *s)
     if (s->h_rec) hipHostFree(s->h_rec);
     if (s->h_cg) hipHostFree(s->h_cg);
     if (s->h_mgs) hipHostFree(s->h_mgs);
+    if (s->own_stream) hipStreamDestroy(s->stream);
     delete s;
 }
 
@@ -464,8 +473,13 @@ int ffr_scene_set_params(ffr_scene *s, const ffr_params *p)
         s->ctx->fail("precision is fixed at scene creation");
         return FFR_E_STATE;
     }
+    if (task_busy(s)) {
+        s->ctx->fail("the scene has a step in flight");
+        return FFR_E_STATE;
+    }
     const bool need_mg = scene_uses_mg(*p);
     s->prm = *p;
+    s->pred = SolvePred{};  // iteration counts of other settings do not carry over
     if (need_mg && !s->mg.levels) return mg_create(s);
     return FFR_OK;
 }
@@ -526,152 +540,13 @@ int ffr_scene_download(ffr_scene *s, double *u, double *v, double *w, double *p,
 
 }  // extern "C"
 
-// ---- pressure solve dispatch ---------------------------------------------------
-namespace ffr {
-
-// Reference CG (PoissonSolver.cpp:93-158) on the device; b -> p.
-static int cg_fp64(ffr_scene *s, int *iters, double *res)
-{
-    hipStream_t st = s->stream;
-    const Geo &g = s->geo;
-    const int max_iter = s->prm.poisson_max_iter;
-    const double nc = (double)s->nc;
-    {
-        ProfScope ps(s->ctx, st, "cg_init", nc * (8 + 1 + 8 * 4));
-        launch_cg_init(st, g, s->b, s->p, s->r, s->d, s->cg, s->red, max_iter, s->prm.poisson_tol);
-    }
-    int enq = 0;
-    int chunk = s->last_iters > 0 ? s->last_iters + 2 : 16;
-    chunk = std::max(1, std::min(chunk, std::max(1, max_iter)));
-    while (enq < max_iter) {
-        const int nrun = std::min(chunk, max_iter - enq);
-        for (int m = 0; m < nrun; m++) {
-            { ProfScope ps(s->ctx, st, "cg_spmv", nc * 17); launch_cg_spmv(st, g, s->d, s->Ad, s->cg, s->red); }
-            { ProfScope ps(s->ctx, st, "cg_update", nc * 48); launch_cg_update(st, g.nc, s->p, s->r, s->d, s->Ad, s->cg, s->red, enq + m); }
-            { ProfScope ps(s->ctx, st, "cg_dir", nc * 24); launch_cg_dir(st, g.nc, s->d, s->r, s->cg, s->red, enq + m); }
-        }
-        enq += nrun;
-        HIP_CHECK_CTX(s->ctx, hipMemcpyAsync(s->h_cg, s->cg, sizeof(CgState), hipMemcpyDeviceToHost, st), FFR_E_DEVICE);
-        HIP_CHECK_CTX(s->ctx, hipStreamSynchronize(st), FFR_E_DEVICE);
-        if (s->h_cg->done) break;
-        chunk = std::max(8, chunk / 4);
-    }
-    if (max_iter <= 0 || enq == 0) {
-        HIP_CHECK_CTX(s->ctx, hipMemcpyAsync(s->h_cg, s->cg, sizeof(CgState), hipMemcpyDeviceToHost, st), FFR_E_DEVICE);
-        HIP_CHECK_CTX(s->ctx, hipStreamSynchronize(st), FFR_E_DEVICE);
-    }
-    *iters = s->h_cg->iters;
-    *res = s->h_cg->res;
-    s->last_iters = *iters;
-    return FFR_OK;
-}
-
-int poisson_device(ffr_scene *s, int *iters, int *outer, double *res)
-{
-    *outer = 0;
-    if (resolve_method(s->prm) == FFR_POISSON_CG) {
-        if (s->prm.use_multigrid) {
-            s->ctx->fail("CG with use_multigrid (pinned outflow pressure) is not provided; use MG-PCG");
-            return FFR_E_UNSUPPORTED;
-        }
-        return cg_fp64(s, iters, res);
-    }
-    return mg_solve(s, iters, outer, res);
-}
-
-template <typename R>
-static int step_impl(ffr_scene *s, ffr_step_result *out, bool sync_result)
-{
-    ffr_ctx *ctx = s->ctx;
-    hipStream_t st = s->stream;
-    const Geo &g = s->geo;
-    const ffr_params &P = s->prm;
-    const StepParams sp = make_step_params(P);
-    R *u = (R *)s->u, *v = (R *)s->v, *w = (R *)s->w, *T = (R *)s->T;
-    R *us = (R *)s->us, *vs = (R *)s->vs, *ws = (R *)s->ws, *Tn = (R *)s->Tn;
-    const double faces = (double)(s->nu + s->nv + s->nw), nc = (double)s->nc, sz = sizeof(R);
-
-    {   // K1 predictor + buoyancy + pre-projection masks
-        ProfScope ps(ctx, st, "predict", faces * 2 * sz + nc + (P.buoyancy ? nc * sz : 0));
-        launch_predict<R>(st, g, sp, u, v, w, T, us, vs, ws, s->rec);
-    }
-    {   // K4 right-hand side
-        ProfScope ps(ctx, st, "rhs", faces * sz + nc + nc * 8);
-        launch_rhs<R>(st, g, sp, us, vs, ws, s->b);
-    }
-    int iters = 0, outer = 0;
-    double pres = 0.0;
-    int rc = poisson_device(s, &iters, &outer, &pres);
-    if (rc != FFR_OK) return rc;
-    {   // K6 projection + masks + outflow + velocity change
-        ProfScope ps(ctx, st, "correct", faces * 3 * sz + nc * 8 + nc);
-        launch_correct<R>(st, g, sp, us, vs, ws, s->p, u, v, w, s->rec);
-    }
-    if (P.thermal) {
-        if (g.n_openings) {
-            ProfScope ps(ctx, st, "opening_T", 0);
-            launch_opening_T<R>(st, g, u, v, w, T);
-        }
-        {
-            ProfScope ps(ctx, st, "temperature", nc * 2 * sz + faces * sz + nc);
-            launch_temperature<R>(st, g, sp, u, v, w, T, Tn, s->rec, s->red);
-        }
-        if (g.n_openings) {
-            ProfScope ps(ctx, st, "opening_T", 0);
-            launch_opening_T<R>(st, g, u, v, w, Tn);
-        }
-        std::swap(s->T, s->Tn);
-        if (s->opening_touches_fluid) {
-            ProfScope ps(ctx, st, "tsum", nc * sz + nc);
-            launch_tsum<R>(st, g, (R *)s->T, s->rec, s->red);
-        }
-        launch_step_finalize(st, g, s->rec, s->red);
-    } else {
-        ProfScope ps(ctx, st, "divmax", faces * sz + nc);
-        launch_divmax<R>(st, g, u, v, w, s->rec);
-    }
-    s->step++;
-    if (!sync_result) return FFR_OK;
-    HIP_CHECK_CTX(ctx, hipMemcpyAsync(s->h_rec, s->rec, sizeof(StepRec), hipMemcpyDeviceToHost, st), FFR_E_DEVICE);
-    HIP_CHECK_CTX(ctx, hipStreamSynchronize(st), FFR_E_DEVICE);
-    if (out) {
-        const StepRec &r = *s->h_rec;
-        double tmean = P.T_initial;
-        if (P.thermal) tmean = s->n_fluid > 0 ? r.tsum / (double)s->n_fluid : P.T_initial;
-        const double vel_change = bits_to_double(r.vmax_bits) / P.dt;
-        bool conv;  // FluidSolver.cpp:538-547
-        if (P.thermal && P.T_target < 1e29)
-            conv = (P.T_target >= P.T_initial) ? (tmean >= P.T_target) : (tmean <= P.T_target);
-        else
-            conv = vel_change < P.convergence_tol;
-        out->step = s->step;
-        out->velocity_change = vel_change;
-        out->pressure_residual = pres;
-        out->divergence_max = bits_to_double(r.divmax_bits);
-        out->converged = conv ? 1 : 0;
-        out->T_mean = tmean;
-        out->poisson_iters = iters;
-        out->outer_iters = outer;
-        out->nonfinite = r.nonfinite;
-    }
-    return FFR_OK;
-}
-
-int scene_step(ffr_scene *s, ffr_step_result *out, bool sync_result)
-{
-    hipSetDevice(s->ctx->device);
-    return s->fp64 ? step_impl<double>(s, out, sync_result) : step_impl<float>(s, out, sync_result);
-}
-
-}  // namespace ffr
-
 extern "C" {
 
 int ffr_step(ffr_scene *s, ffr_step_result *out)
 {
     if (!s) return FFR_E_INVALID;
     ffr_step_result tmp;
-    return scene_step(s, out ? out : &tmp, true);
+    return scene_step(s, out ? out : &tmp);
 }
 
 int ffr_run(ffr_scene *s, int max_steps, ffr_step_callback cb, void *user, ffr_step_result *last)
@@ -680,7 +555,7 @@ int ffr_run(ffr_scene *s, int max_steps, ffr_step_callback cb, void *user, ffr_s
     if (!s) return FFR_E_INVALID;
     ffr_step_result r{};
     for (int k = 0; k < max_steps; k++) {
-        int rc = scene_step(s, &r, true);
+        int rc = scene_step(s, &r);
         if (rc != FFR_OK) return rc;
         if (cb) cb(&r, user);
         if (r.converged) break;
@@ -756,8 +631,8 @@ int ffr_prof_kernel(ffr_scene *s, const char *name, int reps, double *mean_ms, d
         // keep the CG state live for the timed launches
         HIP_CHECK_CTX(s->ctx, hipMemsetAsync(&s->cg->done, 0, sizeof(int), st), FFR_E_DEVICE);
         if (k == "cg_spmv") { b = nc * 17; launch = [&] { launch_cg_spmv(st, g, s->d, s->Ad, s->cg, s->red); }; }
-        else if (k == "cg_update") { b = nc * 48; launch = [&] { launch_cg_update(st, g.nc, s->p, s->r, s->d, s->Ad, s->cg, s->red, 0); }; }
-        else { b = nc * 24; launch = [&] { launch_cg_dir(st, g.nc, s->d, s->r, s->cg, s->red, 0); }; }
+        else if (k == "cg_update") { b = nc * 48; launch = [&] { launch_cg_update(st, g.nc, s->p, s->r, s->d, s->Ad, s->cg, s->red); }; }
+        else { b = nc * 24; launch = [&] { launch_cg_dir(st, g.nc, s->d, s->r, s->cg, s->red); }; }
     } else if (k == "copy") {  // plain FP64 copy b -> r (counter calibration)
         b = nc * 16;
         launch = [&] { launch_copy64(st, s->b, s->r, s->nc); };
@@ -797,17 +672,21 @@ int ffr_poisson_solve(ffr_scene *s, const double *rhs, double *p, ffr_solve_info
     if (!s || !rhs) return FFR_E_INVALID;
     hipSetDevice(s->ctx->device);
     hipStream_t st = s->stream;
+    if (task_busy(s)) {
+        s->ctx->fail("ffr_poisson_solve: the scene has a step in flight");
+        return FFR_E_STATE;
+    }
     HIP_CHECK_CTX(s->ctx, hipMemcpyAsync(s->b, rhs, s->nc * 8, hipMemcpyHostToDevice, st), FFR_E_DEVICE);
-    int iters = 0, outer = 0;
-    double res = 0;
-    int rc = poisson_device(s, &iters, &outer, &res);
+    int rc = task_begin(s, true);
+    bool done = false;
+    while (rc == FFR_OK && !done) rc = task_advance(s, true, &done);
     if (rc != FFR_OK) return rc;
     if (p) HIP_CHECK_CTX(s->ctx, hipMemcpyAsync(p, s->p, s->nc * 8, hipMemcpyDeviceToHost, st), FFR_E_DEVICE);
     HIP_CHECK_CTX(s->ctx, hipStreamSynchronize(st), FFR_E_DEVICE);
     if (info) {
-        info->residual = res;
-        info->iterations = iters;
-        info->outer_iterations = outer;
+        info->residual = s->task.res;
+        info->iterations = s->task.iters;
+        info->outer_iterations = s->task.outer;
     }
     return FFR_OK;
 }

@@ -48,6 +48,35 @@ struct ffr_ctx {
     void serve_shutdown();
 };
 
+namespace ffr {
+// An in-flight projection step (or a stand-alone pressure solve) of one
+// scene. The host enqueues as far as the scene's predictions allow, records
+// `ev`, and resumes when the device has reached it; kernels past the point
+// where the solve stopped return at once (device flags), so enqueuing ahead
+// is safe. See ffr_task.hip.
+struct StepTask {
+    enum Phase { IDLE = 0, SOLVE, RESULT } phase = IDLE;
+    bool solve_only = false;
+    hipEvent_t ev = nullptr;
+    // CG: iterations enqueued
+    int cg_enq = 0;
+    // MG-PCG: residual passes enqueued / wanted, the inner solve waiting for
+    // a host check (-1: none), iterations enqueued per inner solve
+    int passes_enq = 0, target_passes = 0, inner_wait = -1;
+    int k_used[kMgHist] = {};
+    // outcome
+    int iters = 0, outer = 0;
+    double res = 0.0;
+    ffr_step_result result{};
+};
+// Iteration counts of the previous solve, used to enqueue the next one.
+struct SolvePred {
+    int cg_iters = 0;
+    int mg_passes = 0;
+    int mg_inner[kMgHist] = {};
+};
+}  // namespace ffr
+
 // RAII-free scope marker used around each launch.
 struct ProfScope {
     ffr_ctx *c;
@@ -98,7 +127,8 @@ struct ffr_scene {
     ffr::MgState *h_mgs = nullptr;
 
     int step = 0;
-    int last_iters = 0;
+    ffr::StepTask task;
+    ffr::SolvePred pred;
 };
 
 // Returns `code` from the calling function after recording the HIP error.
@@ -113,15 +143,25 @@ struct ffr_scene {
 
 namespace ffr {
 // internal entry points shared between translation units
+int resolve_method(const ffr_params &p);
+StepParams make_step_params(const ffr_params &p);
+double bits_to_double(unsigned long long b);
 size_t scene_bytes(const ffr_scene_desc *d, const ffr_params *p, bool with_mg);
 bool scene_uses_mg(const ffr_params &p);
-int scene_step(ffr_scene *s, ffr_step_result *out, bool sync_result);
-int poisson_device(ffr_scene *s, int *iters, int *outer, double *res);
+// step / solve tasks (ffr_task.hip)
+int task_begin(ffr_scene *s, bool solve_only);
+int task_advance(ffr_scene *s, bool block, bool *done);
+bool task_busy(const ffr_scene *s);
+int scene_step(ffr_scene *s, ffr_step_result *out);
+int wait_point(ffr_scene *s, void *host, const void *dev, size_t bytes);
+// serving (ffr_serve.hip)
+void serve_forget(ffr_ctx *c, ffr_scene *s);
 // multigrid (ffr_mg.hip)
 size_t mg_bytes(int nx, int ny, int nz, const ffr_params &p);
 int mg_create(ffr_scene *s);
 void mg_destroy(ffr_scene *s);
-int mg_solve(ffr_scene *s, int *iters, int *outer, double *res);
+int mg_begin(ffr_scene *s);
+int mg_continue(ffr_scene *s, bool *finished);
 // sets *launch to one launch of a level-0 multigrid / inner-PCG / outer kernel
 // on the scene's buffers and returns its algorithmic bytes
 double mg_prof_launch(ffr_scene *s, const char *name, std::function<void()> *launch);

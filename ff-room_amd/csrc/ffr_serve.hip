@@ -1,28 +1,236 @@
-// ffr_serve.hip - batching and serving of many scenes on one device.
+// ffr_serve.hip - continuous batching of many scenes on one device.
+//
+// A ticket asks for up to max_steps projection steps of one scene (stopping
+// early on FluidSolver::run's convergence test, FluidSolver.cpp:552-560).
+// Up to `slots` tickets are active at once; the rest wait in arrival order
+// and are admitted as active ones finish. Every scene has its own stream, and
+// each pass of the loop advances every active scene's step task without
+// blocking (ffr_task.hip): a scene whose event has been reached gets its next
+// phase or next step enqueued, so the kernels of all active scenes overlap on
+// the device. Admission by HBM footprint happens where the memory is taken:
+// ffr_scene_create refuses a scene beyond the context's budget, and the
+// caller creates scenes as earlier ones are destroyed.
 #include "ffr_scene.hpp"
 
-struct ffr_ctx::Server {};
+#include <algorithm>
+#include <deque>
+#include <map>
+#include <thread>
+#include <vector>
+
+using namespace ffr;
+
+struct ffr_ctx::Server {
+    struct Ticket {
+        ffr_scene *scene = nullptr;
+        int max_steps = 0, steps = 0;
+        bool active = false, done = false;
+        int rc = FFR_OK;
+        ffr_step_result last{};
+    };
+    std::map<uint64_t, Ticket> tickets;
+    std::deque<uint64_t> waiting;
+    std::vector<uint64_t> active;
+    uint64_t next_id = 1;
+    int slots = 16;
+
+    void finish(Ticket &t, int rc)
+    {
+        t.done = true;
+        t.active = false;
+        t.rc = rc;
+    }
+
+    void admit()
+    {
+        while (!waiting.empty() && (int)active.size() < slots) {
+            const uint64_t id = waiting.front();
+            waiting.pop_front();
+            Ticket &t = tickets[id];
+            t.active = true;
+            active.push_back(id);
+        }
+    }
+
+    // one non-blocking pass over the active tickets; returns the number of
+    // steps that completed
+    int progress()
+    {
+        admit();
+        int completed = 0;
+        for (const uint64_t id : active) {
+            Ticket &t = tickets[id];
+            ffr_scene *s = t.scene;
+            if (!task_busy(s)) {
+                if (t.steps >= t.max_steps) {
+                    finish(t, FFR_OK);
+                    continue;
+                }
+                const int rc = task_begin(s, false);
+                if (rc != FFR_OK) finish(t, rc);
+                continue;
+            }
+            bool done = false;
+            const int rc = task_advance(s, false, &done);
+            if (rc != FFR_OK) {
+                finish(t, rc);
+                continue;
+            }
+            if (!done) continue;
+            completed++;
+            t.steps++;
+            t.last = s->task.result;
+            if (t.last.converged || t.steps >= t.max_steps) {
+                finish(t, FFR_OK);
+                continue;
+            }
+            const int rb = task_begin(s, false);  // the scene's next step, at once
+            if (rb != FFR_OK) finish(t, rb);
+        }
+        active.erase(std::remove_if(active.begin(), active.end(),
+                                    [&](uint64_t id) { return tickets[id].done; }),
+                     active.end());
+        admit();
+        return completed;
+    }
+
+    bool idle() const { return active.empty() && waiting.empty(); }
+};
 
 void ffr_ctx::serve_shutdown()
 {
+    if (server) {
+        // let in-flight work finish before the scenes' memory can go away
+        for (const uint64_t id : server->active) {
+            ffr_scene *s = server->tickets[id].scene;
+            if (s) hipStreamSynchronize(s->stream);
+        }
+    }
     delete server;
     server = nullptr;
 }
 
+namespace ffr {
+// A scene being destroyed leaves the serving loop: its ticket ends with
+// FFR_E_STATE unless it had finished.
+void serve_forget(ffr_ctx *c, ffr_scene *s)
+{
+    if (!c->server) return;
+    auto *sv = c->server;
+    for (auto &kv : sv->tickets) {
+        if (kv.second.scene != s) continue;
+        if (!kv.second.done) sv->finish(kv.second, FFR_E_STATE);
+        kv.second.scene = nullptr;
+    }
+    sv->active.erase(std::remove_if(sv->active.begin(), sv->active.end(),
+                                    [&](uint64_t id) { return sv->tickets[id].done; }),
+                     sv->active.end());
+    sv->waiting.erase(std::remove_if(sv->waiting.begin(), sv->waiting.end(),
+                                     [&](uint64_t id) { return sv->tickets[id].done; }),
+                      sv->waiting.end());
+}
+}  // namespace ffr
+
+static ffr_ctx::Server *server_of(ffr_ctx *c)
+{
+    if (!c->server) c->server = new (std::nothrow) ffr_ctx::Server();
+    return c->server;
+}
+
 extern "C" {
-int ffr_submit(ffr_ctx *c, ffr_scene *, int, uint64_t *)
+
+int ffr_serve_config(ffr_ctx *c, int slots)
 {
-    if (c) c->fail("serving not available yet");
-    return FFR_E_UNSUPPORTED;
+    if (!c || slots < 1) return FFR_E_INVALID;
+    auto *sv = server_of(c);
+    if (!sv) return FFR_E_NOMEM;
+    sv->slots = slots;
+    return FFR_OK;
 }
-int ffr_poll(ffr_ctx *c, uint64_t, ffr_step_result *, int *)
+
+int ffr_submit(ffr_ctx *c, ffr_scene *s, int max_steps, uint64_t *ticket)
 {
-    if (c) c->fail("serving not available yet");
-    return FFR_E_UNSUPPORTED;
+    if (!c || !s || !ticket || max_steps < 0 || s->ctx != c) return FFR_E_INVALID;
+    auto *sv = server_of(c);
+    if (!sv) return FFR_E_NOMEM;
+    for (const auto &kv : sv->tickets)
+        if (kv.second.scene == s && !kv.second.done) {
+            c->fail("ffr_submit: the scene already has an unfinished ticket");
+            return FFR_E_STATE;
+        }
+    if (task_busy(s)) {
+        c->fail("ffr_submit: the scene has a step in flight");
+        return FFR_E_STATE;
+    }
+    const uint64_t id = sv->next_id++;
+    ffr_ctx::Server::Ticket t;
+    t.scene = s;
+    t.max_steps = max_steps;
+    sv->tickets[id] = t;
+    sv->waiting.push_back(id);
+    sv->admit();
+    *ticket = id;
+    return FFR_OK;
 }
+
+int ffr_poll(ffr_ctx *c, uint64_t ticket, ffr_step_result *last, int *done)
+{
+    if (!c || !done) return FFR_E_INVALID;
+    auto *sv = c->server;
+    if (!sv || !sv->tickets.count(ticket)) {
+        c->fail("ffr_poll: unknown ticket %llu", (unsigned long long)ticket);
+        return FFR_E_NOT_FOUND;
+    }
+    sv->progress();
+    const auto &t = sv->tickets[ticket];
+    *done = t.done ? 1 : 0;
+    if (last) *last = t.last;
+    return t.done ? t.rc : FFR_OK;
+}
+
+int ffr_progress(ffr_ctx *c, int *steps_completed)
+{
+    if (!c) return FFR_E_INVALID;
+    auto *sv = c->server;
+    const int n = sv ? sv->progress() : 0;
+    if (steps_completed) *steps_completed = n;
+    return FFR_OK;
+}
+
 int ffr_drain(ffr_ctx *c)
 {
-    if (c) c->fail("serving not available yet");
-    return FFR_E_UNSUPPORTED;
+    if (!c) return FFR_E_INVALID;
+    auto *sv = c->server;
+    if (!sv) return FFR_OK;
+    while (!sv->idle()) {
+        if (sv->progress() == 0) std::this_thread::yield();
+    }
+    for (const auto &kv : sv->tickets)
+        if (kv.second.rc != FFR_OK) return kv.second.rc;
+    return FFR_OK;
+}
+
+int ffr_ticket_info(ffr_ctx *c, uint64_t ticket, int *steps, int *done)
+{
+    if (!c) return FFR_E_INVALID;
+    auto *sv = c->server;
+    if (!sv || !sv->tickets.count(ticket)) return FFR_E_NOT_FOUND;
+    const auto &t = sv->tickets[ticket];
+    if (steps) *steps = t.steps;
+    if (done) *done = t.done ? 1 : 0;
+    return t.done ? t.rc : FFR_OK;
+}
+
+int ffr_release(ffr_ctx *c, uint64_t ticket)
+{
+    if (!c) return FFR_E_INVALID;
+    auto *sv = c->server;
+    if (!sv || !sv->tickets.count(ticket)) return FFR_E_NOT_FOUND;
+    if (!sv->tickets[ticket].done) {
+        c->fail("ffr_release: ticket %llu has not finished", (unsigned long long)ticket);
+        return FFR_E_STATE;
+    }
+    sv->tickets.erase(ticket);
+    return FFR_OK;
 }
 }

@@ -114,6 +114,7 @@ __global__ void __launch_bounds__(kBlock) k_cg_start(CgState *cg, RedScratch red
         cg->iters = 0;
         cg->res = 1.0;
     }
+    cg->started = 0;
 }
 
 void launch_cg_init(hipStream_t s, const Geo &g, const double *b, double *x, double *r,
@@ -143,6 +144,8 @@ __global__ void __launch_bounds__(kBlock) k_cg_spmv(Geo g, const double *__restr
         Ad[c] = a;
         dq += dc * a;
     }
+    // no other block of this kernel reads the counter
+    if (blk == 0 && threadIdx.x == 0) cg->started += 1;
     block_partial(dq, slot(red, SLOT_SPMV), blk, lds);
 }
 
@@ -161,10 +164,11 @@ __global__ void __launch_bounds__(kBlock) k_cg_update(uint32_t n, double *__rest
                                                       double *__restrict__ r,
                                                       const double *__restrict__ d,
                                                       const double *__restrict__ Ad,
-                                                      CgState *cg, RedScratch red, int it)
+                                                      CgState *cg, RedScratch red)
 {
     __shared__ double lds[kBlock / kWave];
     if (cg->done) return;
+    const int it = cg->started - 1;  // this iteration's index
     const uint32_t blk = xcd_block();
     const double dAd = sum_partials(slot(red, SLOT_SPMV), gridDim.x, lds);
     const double rr_old = cg->rr[it & 1];
@@ -190,19 +194,19 @@ __global__ void __launch_bounds__(kBlock) k_cg_update(uint32_t n, double *__rest
 }
 
 void launch_cg_update(hipStream_t s, uint32_t n, double *x, double *r, const double *d,
-                      const double *Ad, CgState *cg, RedScratch red, int it)
+                      const double *Ad, CgState *cg, RedScratch red)
 {
-    FFR_LAUNCH(k_cg_update, dim3(grid_for(n)), dim3(kBlock), 0, s, n, x, r, d, Ad, cg,
-                       red, it);
+    FFR_LAUNCH(k_cg_update, dim3(grid_for(n)), dim3(kBlock), 0, s, n, x, r, d, Ad, cg, red);
 }
 
 // res = sqrt(rr_new)/||b|| and the stopping test (:140-145); beta, d = r + beta d (:147-152)
 __global__ void __launch_bounds__(kBlock) k_cg_dir(uint32_t n, double *__restrict__ d,
                                                    const double *__restrict__ r,
-                                                   CgState *cg, RedScratch red, int it)
+                                                   CgState *cg, RedScratch red)
 {
     __shared__ double lds[kBlock / kWave];
     if (cg->done) return;
+    const int it = cg->started - 1;
     const uint32_t blk = xcd_block();
     const double rr_new = sum_partials(slot(red, SLOT_UPDATE), gridDim.x, lds);
     const double res = sqrt(rr_new) / cg->bnorm;
@@ -230,9 +234,9 @@ __global__ void __launch_bounds__(kBlock) k_cg_dir(uint32_t n, double *__restric
 }
 
 void launch_cg_dir(hipStream_t s, uint32_t n, double *d, const double *r, CgState *cg,
-                   RedScratch red, int it)
+                   RedScratch red)
 {
-    FFR_LAUNCH(k_cg_dir, dim3(grid_for(n)), dim3(kBlock), 0, s, n, d, r, cg, red, it);
+    FFR_LAUNCH(k_cg_dir, dim3(grid_for(n)), dim3(kBlock), 0, s, n, d, r, cg, red);
 }
 
 }  // namespace ffr

@@ -694,7 +694,7 @@ __global__ void __launch_bounds__(kBlock) k_pcg_spmv(Geo g, MgLevel L, const flo
 // alpha = rz / p.q; e += alpha p; r -= alpha q; partials of r.r
 __global__ void __launch_bounds__(kBlock) k_pcg_update(uint32_t n, float *e, float *r,
                                                        const float *p, const float *q, MgState *m,
-                                                       RedScratch red, int it)
+                                                       RedScratch red)
 {
     __shared__ double lds[kBlock / kWave];
     if (stopped(m)) return;
@@ -704,7 +704,7 @@ __global__ void __launch_bounds__(kBlock) k_pcg_update(uint32_t n, float *e, flo
         if (blk == 0 && threadIdx.x == 0) m->inner_done = 1;
         return;
     }
-    const float alpha = (float)(m->rz[it & 1] / pq);
+    const float alpha = (float)(m->rz[m->inner_it & 1] / pq);
     double acc = 0.0;
     uint32_t sb, se;
     block_span(n, blk, sb, se);
@@ -718,15 +718,17 @@ __global__ void __launch_bounds__(kBlock) k_pcg_update(uint32_t n, float *e, flo
 }
 
 // stopping test of the inner solve (one block)
-__global__ void __launch_bounds__(kBlock) k_pcg_check(MgState *m, RedScratch red, uint32_t G, int it)
+__global__ void __launch_bounds__(kBlock) k_pcg_check(MgState *m, RedScratch red, uint32_t G)
 {
     __shared__ double lds[kBlock / kWave];
     if (stopped(m)) return;
     const double rr = sum_partials(pslot(red, SLOT_UPDATE), G, lds);
     if (threadIdx.x != 0) return;
-    m->inner_it = it + 1;
+    const int it = m->inner_it + 1;
+    m->inner_it = it;
     m->total_iters += 1;
-    if (sqrt(rr) <= m->inner_tol * m->rnorm0 || it + 1 >= m->max_inner || !(rr == rr))
+    if (sqrt(rr) <= m->inner_tol * m->rnorm0 || it >= m->max_inner || !(rr == rr) ||
+        m->total_iters >= m->total_cap)
         m->inner_done = 1;
 }
 
@@ -763,12 +765,13 @@ __global__ void __launch_bounds__(kBlock) k_pcg_start(uint32_t n, const float *z
 
 // beta = rz_new / rz; p = z + beta p
 __global__ void __launch_bounds__(kBlock) k_pcg_dir(uint32_t n, const float *z, float *p, MgState *m,
-                                                    RedScratch red, int it)
+                                                    RedScratch red)
 {
     __shared__ double lds[kBlock / kWave];
     if (stopped(m)) return;
     const uint32_t blk = xcd_block();
     const double rz_new = sum_partials(pslot(red, SLOT_MG_A), gridDim.x, lds);
+    const int it = m->inner_it - 1;  // the iteration the check just closed
     const float beta = (float)(rz_new / m->rz[it & 1]);
     uint32_t sb, se;
     block_span(n, blk, sb, se);
@@ -796,7 +799,7 @@ __global__ void __launch_bounds__(kBlock) k_dc_init(Geo g, const double *b, doub
 
 __global__ void __launch_bounds__(kBlock) k_dc_start(MgState *m, RedScratch red, uint32_t G,
                                                      double outer_tol, int max_outer, int max_inner,
-                                                     double inner_tol)
+                                                     double inner_tol, int total_cap)
 {
     __shared__ double lds[kBlock / kWave];
     const double bb = sum_partials(pslot(red, SLOT_INIT), G, lds);
@@ -808,6 +811,9 @@ __global__ void __launch_bounds__(kBlock) k_dc_start(MgState *m, RedScratch red,
     m->max_inner = max_inner;
     m->outer = 0;
     m->total_iters = 0;
+    m->total_cap = total_cap;
+    m->passes = 0;
+    for (int k = 0; k < kMgHist; k++) m->hist[k] = 0;
     m->inner_it = 0;
     m->inner_done = 1;
     m->solve_done = m->bnorm < 1e-15 ? 1 : 0;
@@ -840,14 +846,16 @@ __device__ __forceinline__ void row64(const Geo &g, uint32_t c, uint16_t m, F va
 // x_out = x_in + e on unknowns; r = b - A x_out (FP64); r32 = (float) r;
 // partials of ||r||^2
 __global__ void __launch_bounds__(kBlock) k_dc_residual(Geo g, const double *__restrict__ b,
-                                                        const double *__restrict__ xin,
+                                                        double *xa, double *xb,
                                                         const float *__restrict__ e,
-                                                        double *__restrict__ xout,
                                                         float *__restrict__ r32, const MgState *m,
                                                         RedScratch red)
 {
     __shared__ double lds[kBlock / kWave];
     if (m->solve_done) return;
+    // x_in is the buffer of the current pass parity, x_out the other one
+    const double *__restrict__ xin = (m->passes & 1) ? xb : xa;
+    double *__restrict__ xout = (m->passes & 1) ? xa : xb;
     const uint32_t blk = xcd_block();
     double acc = 0.0;
     uint32_t sb, se;
@@ -881,8 +889,12 @@ __global__ void __launch_bounds__(kBlock) k_dc_check(MgState *m, RedScratch red,
     const double rr = sum_partials(pslot(red, SLOT_UPDATE), G, lds);
     if (threadIdx.x != 0) return;
     const double rn = sqrt(rr);
+    const int pass = m->passes + 1;  // the residual kernel before this one ran
+    m->passes = pass;
+    if (pass >= 2 && pass - 2 < kMgHist) m->hist[pass - 2] = m->inner_it;
     m->res = rn / m->bnorm;
-    if (m->res < m->outer_tol || m->outer >= m->max_outer || !(rr == rr)) {
+    if (m->res < m->outer_tol || m->outer >= m->max_outer || !(rr == rr) ||
+        m->total_iters >= m->total_cap) {
         m->solve_done = 1;
         return;
     }
@@ -895,9 +907,11 @@ __global__ void __launch_bounds__(kBlock) k_dc_check(MgState *m, RedScratch red,
 // p: unknowns from x; OUTFLOW rows solved from their neighbours (or pinned
 // to 0 for the reference multigrid's convention); SOLID cells 0.
 __global__ void __launch_bounds__(kBlock) k_dc_finalize(Geo g, const double *__restrict__ b,
-                                                        const double *x, double *p,
-                                                        int pin_outflow)
+                                                        const double *xa, const double *xb,
+                                                        double *p, int pin_outflow,
+                                                        const MgState *m)
 {
+    const double *x = (m->passes & 1) ? xb : xa;  // the last pass's output
     uint32_t sb, se;
     block_span(g.nc, xcd_block(), sb, se);
     for (uint32_t c = sb + threadIdx.x; c < se; c += kBlock) {
@@ -1007,13 +1021,13 @@ void pcg_launch_spmv(hipStream_t s, const Geo &g, const MgLevel &L0, const float
     FFR_LAUNCH(k_pcg_spmv, dim3(grid_for(g.nc)), dim3(kBlock), 0, s, g, L0, p, q, m, red);
 }
 void pcg_launch_update(hipStream_t s, uint32_t n, float *e, float *r, const float *p, const float *q,
-                       MgState *m, RedScratch red, int it)
+                       MgState *m, RedScratch red)
 {
-    FFR_LAUNCH(k_pcg_update, dim3(grid_for(n)), dim3(kBlock), 0, s, n, e, r, p, q, m, red, it);
+    FFR_LAUNCH(k_pcg_update, dim3(grid_for(n)), dim3(kBlock), 0, s, n, e, r, p, q, m, red);
 }
-void pcg_launch_check(hipStream_t s, uint32_t n, MgState *m, RedScratch red, int it)
+void pcg_launch_check(hipStream_t s, uint32_t n, MgState *m, RedScratch red)
 {
-    FFR_LAUNCH(k_pcg_check, dim3(1), dim3(kBlock), 0, s, m, red, grid_for(n), it);
+    FFR_LAUNCH(k_pcg_check, dim3(1), dim3(kBlock), 0, s, m, red, grid_for(n));
 }
 void pcg_launch_dot_rz(hipStream_t s, uint32_t n, const float *r, const float *z, const MgState *m,
                        RedScratch red)
@@ -1025,34 +1039,35 @@ void pcg_launch_start(hipStream_t s, uint32_t n, const float *z, float *p, float
 {
     FFR_LAUNCH(k_pcg_start, dim3(grid_for(n)), dim3(kBlock), 0, s, n, z, p, e, m, red);
 }
-void pcg_launch_dir(hipStream_t s, uint32_t n, const float *z, float *p, MgState *m, RedScratch red, int it)
+void pcg_launch_dir(hipStream_t s, uint32_t n, const float *z, float *p, MgState *m, RedScratch red)
 {
-    FFR_LAUNCH(k_pcg_dir, dim3(grid_for(n)), dim3(kBlock), 0, s, n, z, p, m, red, it);
+    FFR_LAUNCH(k_pcg_dir, dim3(grid_for(n)), dim3(kBlock), 0, s, n, z, p, m, red);
 }
 void dc_launch_init(hipStream_t s, const Geo &g, const double *b, double *x, float *e, RedScratch red)
 {
     FFR_LAUNCH(k_dc_init, dim3(grid_for(g.nc)), dim3(kBlock), 0, s, g, b, x, e, red);
 }
 void dc_launch_start(hipStream_t s, const Geo &g, MgState *m, RedScratch red, double outer_tol,
-                     int max_outer, int max_inner, double inner_tol)
+                     int max_outer, int max_inner, double inner_tol, int total_cap)
 {
     FFR_LAUNCH(k_dc_start, dim3(1), dim3(kBlock), 0, s, m, red, grid_for(g.nc), outer_tol,
-                       max_outer, max_inner, inner_tol);
+               max_outer, max_inner, inner_tol, total_cap);
 }
-void dc_launch_residual(hipStream_t s, const Geo &g, const double *b, const double *xin, const float *e,
-                        double *xout, float *r32, const MgState *m, RedScratch red)
+void dc_launch_residual(hipStream_t s, const Geo &g, const double *b, double *xa, double *xb,
+                        const float *e, float *r32, const MgState *m, RedScratch red)
 {
-    FFR_LAUNCH(k_dc_residual, dim3(grid_for(g.nc)), dim3(kBlock), 0, s, g, b, xin, e, xout, r32,
-                       m, red);
+    FFR_LAUNCH(k_dc_residual, dim3(grid_for(g.nc)), dim3(kBlock), 0, s, g, b, xa, xb, e, r32, m,
+               red);
 }
 void dc_launch_check(hipStream_t s, const Geo &g, MgState *m, RedScratch red)
 {
     FFR_LAUNCH(k_dc_check, dim3(1), dim3(kBlock), 0, s, m, red, grid_for(g.nc));
 }
-void dc_launch_finalize(hipStream_t s, const Geo &g, const double *b, const double *x, double *p,
-                        int pin_outflow)
+void dc_launch_finalize(hipStream_t s, const Geo &g, const double *b, const double *xa,
+                        const double *xb, double *p, int pin_outflow, const MgState *m)
 {
-    FFR_LAUNCH(k_dc_finalize, dim3(grid_for(g.nc)), dim3(kBlock), 0, s, g, b, x, p, pin_outflow);
+    FFR_LAUNCH(k_dc_finalize, dim3(grid_for(g.nc)), dim3(kBlock), 0, s, g, b, xa, xb, p,
+               pin_outflow, m);
 }
 
 }  // namespace ffr

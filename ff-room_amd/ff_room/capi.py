@@ -66,7 +66,7 @@ class Params(C.Structure):
                 ("precision", C.c_int), ("poisson_method", C.c_int), ("mg_levels", C.c_int),
                 ("mg_bf16", C.c_int), ("mg_pre_sweeps", C.c_int), ("mg_level_sweeps", C.c_int),
                 ("mg_coarse_sweeps", C.c_int),
-                ("inner_tol", C.c_double), ("inner_max_iter", C.c_int)]
+                ("inner_tol", C.c_double), ("inner_max_iter", C.c_int), ("graphs", C.c_int)]
 
 
 class StepResult(C.Structure):
@@ -125,6 +125,10 @@ SIGNATURES = {
     "ffr_submit": (_I, [_P, _P, C.c_int, C.POINTER(C.c_uint64)]),
     "ffr_poll": (_I, [_P, C.c_uint64, C.POINTER(StepResult), C.POINTER(C.c_int)]),
     "ffr_drain": (_I, [_P]),
+    "ffr_serve_config": (_I, [_P, C.c_int]),
+    "ffr_progress": (_I, [_P, C.POINTER(C.c_int)]),
+    "ffr_ticket_info": (_I, [_P, C.c_uint64, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
+    "ffr_release": (_I, [_P, C.c_uint64]),
     "ffr_prof_enable": (_I, [_P, C.c_int]),
     "ffr_prof_reset": (_I, [_P]),
     "ffr_prof_read": (_I, [_P, C.POINTER(ProfEntry), C.c_int, C.POINTER(C.c_int)]),
@@ -199,6 +203,39 @@ class Context:
 
     def bytes_in_use(self) -> int:
         return self._l.ffr_ctx_bytes_in_use(self.h)
+
+    # ---- serving (continuous batching of many scenes) ----
+    def serve_config(self, slots: int):
+        self.check(self._l.ffr_serve_config(self.h, slots), "ffr_serve_config")
+
+    def submit(self, scene: "DeviceScene", max_steps: int) -> int:
+        t = C.c_uint64(0)
+        self.check(self._l.ffr_submit(self.h, scene.h, max_steps, C.byref(t)), "ffr_submit")
+        return t.value
+
+    def poll(self, ticket: int):
+        """Advances every active ticket once; (done, last step result)."""
+        r, done = StepResult(), C.c_int(0)
+        self.check(self._l.ffr_poll(self.h, ticket, C.byref(r), C.byref(done)), "ffr_poll")
+        return bool(done.value), r.as_dict()
+
+    def progress(self) -> int:
+        """Advances every active ticket once; the number of steps completed."""
+        n = C.c_int(0)
+        self.check(self._l.ffr_progress(self.h, C.byref(n)), "ffr_progress")
+        return n.value
+
+    def drain(self):
+        self.check(self._l.ffr_drain(self.h), "ffr_drain")
+
+    def ticket_info(self, ticket: int):
+        steps, done = C.c_int(0), C.c_int(0)
+        self.check(self._l.ffr_ticket_info(self.h, ticket, C.byref(steps), C.byref(done)),
+                   "ffr_ticket_info")
+        return steps.value, bool(done.value)
+
+    def release(self, ticket: int):
+        self.check(self._l.ffr_release(self.h, ticket), "ffr_release")
 
     def prof_enable(self, on: bool = True):
         self.check(self._l.ffr_prof_enable(self.h, int(on)), "ffr_prof_enable")

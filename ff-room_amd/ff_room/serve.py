@@ -1,0 +1,171 @@
+"""Serving: a stream of scene-solve requests, continuously batched on one GPU.
+
+A request is a scene (SceneConfig), a step budget and an arrival time.
+Requests wait in arrival order. The head of the queue is admitted when a slot
+is free and its device footprint (``ffr_scene_footprint``) fits the context's
+HBM budget beside the scenes already resident; the scene is then built (flags,
+boundary lists, initial temperature: ``device.prepare``), created on the
+device and submitted to the native serving loop (``ffr_submit``), which
+overlaps the steps of every admitted scene on per-scene streams. When a
+request finishes (its step budget, or FluidSolver::run's convergence test),
+its fields are copied to the host if asked for and its device memory is
+freed, which admits the next requests.
+
+Latency is host time from a request's arrival to its result being on the
+host; throughput is the sum of cells x completed steps over the wall time of
+the stream. There is no counterpart in the reference (one scene per process).
+"""
+from __future__ import annotations
+
+import time
+from collections import deque
+from dataclasses import dataclass, field
+from typing import Deque, Dict, List, Optional
+
+import numpy as np
+
+from . import capi, device
+from .config import SceneConfig
+
+
+@dataclass
+class Request:
+    config: SceneConfig
+    max_steps: int
+    arrival: float = 0.0  # seconds after the start of the stream
+
+
+@dataclass
+class Completion:
+    index: int
+    cells: int
+    steps: int
+    arrival: float
+    admitted: float
+    finished: float
+    last: dict
+    fields: Optional[Dict[str, np.ndarray]] = None
+
+    @property
+    def latency(self) -> float:
+        return self.finished - self.arrival
+
+
+@dataclass
+class _Active:
+    index: int
+    req: Request
+    scene: "capi.DeviceScene"
+    cells: int
+    admitted: float
+
+
+def percentile(xs: List[float], q: float) -> float:
+    """q-th percentile (0..100), linear interpolation between order statistics."""
+    if not xs:
+        return float("nan")
+    return float(np.percentile(np.asarray(xs, dtype=np.float64), q))
+
+
+class Server:
+    """Admission, submission and completion around the native serving loop.
+
+    ``overrides`` are ffr_params fields applied to every request (for example
+    ``precision=capi.PREC_MIXED``)."""
+
+    def __init__(self, ctx: capi.Context, slots: int = 16, hbm_budget: Optional[int] = None,
+                 keep_fields: bool = False, **overrides):
+        self.ctx = ctx
+        self.slots = slots
+        ctx.serve_config(slots)
+        info = ctx.device_info()
+        self.budget = hbm_budget if hbm_budget is not None else int(info["hbm_bytes"] * 0.8)
+        self.keep_fields = keep_fields
+        self.overrides = overrides
+        self.pending: Deque = deque()   # (index, Request, PreparedScene or None)
+        self.active: Dict[int, _Active] = {}
+        self.done: List[Completion] = []
+        self._next = 0
+        self.t0 = time.perf_counter()
+
+    def now(self) -> float:
+        return time.perf_counter() - self.t0
+
+    def submit(self, req: Request) -> int:
+        i = self._next
+        self._next += 1
+        self.pending.append((i, req, None))
+        return i
+
+    def _admit(self):
+        while self.pending and len(self.active) < self.slots:
+            i, req, ps = self.pending[0]
+            if ps is None:
+                ps = device.prepare(req.config, **self.overrides)
+                self.pending[0] = (i, req, ps)
+            need = ps.footprint()
+            if need == 0:
+                raise ValueError(f"request {i}: invalid scene descriptor")
+            if need > self.budget:
+                raise MemoryError(f"request {i} needs {need} bytes, above the budget {self.budget}")
+            if self.ctx.bytes_in_use() + need > self.budget:
+                break  # wait for memory to be freed by finishing requests
+            self.pending.popleft()
+            scene = device.create(self.ctx, ps)
+            ticket = self.ctx.submit(scene, req.max_steps)
+            self.active[ticket] = _Active(i, req, scene, ps.cells, self.now())
+
+    def pump(self) -> int:
+        """Admits what fits, advances every active scene once and collects
+        finished requests; returns the number of steps completed."""
+        self._admit()
+        n = self.ctx.progress() if self.active else 0
+        for ticket in list(self.active):
+            steps, fin = self.ctx.ticket_info(ticket)
+            if not fin:
+                continue
+            a = self.active.pop(ticket)
+            _, last = self.ctx.poll(ticket)
+            fields = a.scene.download() if self.keep_fields else None
+            a.scene.close()
+            self.ctx.release(ticket)
+            self.done.append(Completion(a.index, a.cells, steps, a.req.arrival, a.admitted,
+                                        self.now(), last, fields))
+        if self.done:
+            self._admit()
+        return n
+
+    def idle(self) -> bool:
+        return not self.pending and not self.active
+
+    def run(self, requests: List[Request], realtime: bool = True) -> List[Completion]:
+        """Feeds `requests` (sorted by arrival) and serves until all finished.
+        With realtime, a request enters the queue when the wall clock reaches
+        its arrival time; otherwise all enter at once."""
+        reqs = sorted(requests, key=lambda r: r.arrival)
+        self.t0 = time.perf_counter()
+        k = 0
+        while k < len(reqs) or not self.idle():
+            t = self.now()
+            while k < len(reqs) and (not realtime or reqs[k].arrival <= t):
+                r = reqs[k]
+                if not realtime:
+                    r = Request(r.config, r.max_steps, 0.0)
+                self.submit(r)
+                k += 1
+            n = self.pump()
+            if n == 0 and self.idle() and k < len(reqs):
+                time.sleep(min(max(reqs[k].arrival - self.now(), 0.0), 0.001))
+        return sorted(self.done, key=lambda c: c.index)
+
+
+def summary(done: List[Completion]) -> dict:
+    """p50/p99 latency and aggregate cell-updates/s of a served stream."""
+    lat = [c.latency for c in done]
+    start = min(c.arrival for c in done)
+    end = max(c.finished for c in done)
+    work = sum(c.cells * c.steps for c in done)
+    return {"requests": len(done), "p50_latency_s": percentile(lat, 50),
+            "p99_latency_s": percentile(lat, 99), "wall_s": end - start,
+            "cell_updates_per_s": work / max(end - start, 1e-12),
+            "steps": sum(c.steps for c in done), "cell_updates": work}

@@ -114,6 +114,8 @@ typedef struct {
     int mg_coarse_sweeps;/* symmetric sweeps on the coarsest level */
     double inner_tol;    /* mixed: relative residual target of each inner FP32 solve */
     int inner_max_iter;  /* mixed: iteration cap of each inner solve */
+    int graphs;          /* replay the inner iteration from a captured HIP graph (1) or
+                            launch its kernels one by one (0); results are identical */
 } ffr_params;
 
 /* StepResult, FluidSolver.hpp:40-47, plus solver statistics. */
@@ -196,14 +198,32 @@ int ffr_apply_laplacian(ffr_scene *scene, const double *x, double *Ax);
 int ffr_poisson_solve(ffr_scene *scene, const double *rhs, double *p, ffr_solve_info *info);
 
 /* ---- batching and serving ---------------------------------------------- */
-/* Queue `max_steps` steps (stopping early when velocity_change < conv_tol,
- * or on the thermal target) of a scene; steps of all queued scenes overlap
- * on the device. */
+/* Continuous batching of many scenes on the context's device (no counterpart
+ * in the reference, which runs one scene per process). A ticket asks for up
+ * to `max_steps` steps of a scene, ending early when a step reports
+ * `converged` (FluidSolver::run, FluidSolver.cpp:552-560). At most `slots`
+ * tickets (default 16) are active; later ones wait in submission order and
+ * are admitted as active ones finish. Each scene runs on its own stream and
+ * every ffr_poll / ffr_drain pass advances all active scenes without
+ * blocking, so their kernels overlap on the device. A scene's results are the
+ * same alone and in a batch (every reduction has a fixed order). HBM
+ * admission: ffr_scene_create returns FFR_E_NOMEM beyond the context budget;
+ * destroy finished scenes to admit new ones. */
+int ffr_serve_config(ffr_ctx *ctx, int slots);
 int ffr_submit(ffr_ctx *ctx, ffr_scene *scene, int max_steps, uint64_t *ticket);
-/* Advances queued work; *done is set when the ticket's scene finished. */
+/* Advances all active tickets once; *done is set when this ticket finished
+ * and *last receives its latest step result. Returns the ticket's error code
+ * once it has finished with one. */
 int ffr_poll(ffr_ctx *ctx, uint64_t ticket, ffr_step_result *last, int *done);
-/* Runs every queued ticket to completion. */
+/* Advances every active ticket once (non-blocking); *steps_completed = the
+ * number of steps that finished in this pass. */
+int ffr_progress(ffr_ctx *ctx, int *steps_completed);
+/* Runs every submitted ticket to completion. */
 int ffr_drain(ffr_ctx *ctx);
+/* Steps completed so far, without advancing anything. */
+int ffr_ticket_info(ffr_ctx *ctx, uint64_t ticket, int *steps, int *done);
+/* Forgets a finished ticket. */
+int ffr_release(ffr_ctx *ctx, uint64_t ticket);
 
 /* ---- profiling ---------------------------------------------------------- */
 int ffr_prof_enable(ffr_ctx *ctx, int on);
