@@ -123,6 +123,9 @@ void ffr_ctx_destroy(ffr_ctx *c)
     if (!c) return;
     hipSetDevice(c->device);
     c->serve_shutdown();
+    // scenes still alive go with their context
+    const std::set<ffr_scene *> left = c->scenes;
+    for (ffr_scene *s : left) ffr_scene_destroy(s);
     c->prof_clear();
     if (c->stream) hipStreamDestroy(c->stream);
     delete c;

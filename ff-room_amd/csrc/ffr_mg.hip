@@ -15,19 +15,45 @@ namespace {
 struct Dims {
     int levels;
     int n[kMaxLevels][3];
+    int sh[kMaxLevels][3];  // axes halved from level l-1 to l
 };
 
-Dims level_dims(int nx, int ny, int nz, const ffr_params &p)
+// Level extents: coarsen the axes (with at least 4 cells) whose spacing is
+// at most 1/sqrt(2) of the largest spacing (their coupling is at least twice
+// the weakest), or all such axes when none is (near-cubic cells); stop when
+// no axis has 4 cells, at the level cap, or once a level has at most
+// kCoarsest cells (solved by symmetric sweeps in one workgroup).
+constexpr int kCoarsest = 512;
+Dims level_dims(int nx, int ny, int nz, const double h0[3], const ffr_params &p)
 {
     Dims d{};
     const int cap = p.mg_levels > 0 ? std::min(p.mg_levels, kMaxLevels) : kMaxLevels;
     d.n[0][0] = nx; d.n[0][1] = ny; d.n[0][2] = nz;
+    double h[3] = {h0[0], h0[1], h0[2]};
     d.levels = 1;
-    // coarsen while every extent is at least 4 cells
     while (d.levels < cap) {
         const int *f = d.n[d.levels - 1];
-        if (std::min({f[0], f[1], f[2]}) < 4) break;
-        for (int a = 0; a < 3; a++) d.n[d.levels][a] = (f[a] + 1) / 2;
+        if ((long)f[0] * f[1] * f[2] <= kCoarsest) break;
+        bool can[3], any = false;
+        for (int a = 0; a < 3; a++) {
+            can[a] = f[a] >= 4;
+            any |= can[a];
+        }
+        if (!any) break;
+        const double hmax = std::max({h[0], h[1], h[2]});
+        int sh[3];
+        bool anys = false;
+        for (int a = 0; a < 3; a++) {
+            sh[a] = (can[a] && h[a] <= 0.70710678 * hmax) ? 1 : 0;
+            anys |= sh[a] != 0;
+        }
+        if (!anys)
+            for (int a = 0; a < 3; a++) sh[a] = can[a] ? 1 : 0;
+        for (int a = 0; a < 3; a++) {
+            d.sh[d.levels][a] = sh[a];
+            d.n[d.levels][a] = sh[a] ? (f[a] + 1) / 2 : f[a];
+            if (sh[a]) h[a] *= 2.0;
+        }
         d.levels++;
     }
     return d;
@@ -65,10 +91,12 @@ size_t mg_layout(char *base, const Dims &D, bool bf16, Multigrid *M, MgState **s
         uint8_t *act = (uint8_t *)take(n);
         float *cx = (float *)take(4ull * n), *cy = (float *)take(4ull * n), *cz = (float *)take(4ull * n);
         float *dg = (float *)take(4ull * n), *rd = (float *)take(4ull * n);
+        float *gx = (float *)take(4ull * n), *gy = (float *)take(4ull * n), *gz = (float *)take(4ull * n);
         void *z = take(vs * n), *r = take(vs * n);
         if (M) {
             MgLevel &L = M->lv[l];
             L.active = act; L.cx = cx; L.cy = cy; L.cz = cz; L.diag = dg; L.rdiag = rd;
+            L.dgx = gx; L.dgy = gy; L.dgz = gz;
             L.z = z; L.r = r; L.bf16 = bf16 ? 1 : 0;
         }
     }
@@ -85,9 +113,9 @@ bool uniform(const std::vector<double> &h)
 
 }  // namespace
 
-size_t mg_bytes(int nx, int ny, int nz, const ffr_params &p)
+size_t mg_bytes(int nx, int ny, int nz, const double h0[3], const ffr_params &p)
 {
-    const Dims D = level_dims(nx, ny, nz, p);
+    const Dims D = level_dims(nx, ny, nz, h0, p);
     return mg_layout(nullptr, D, p.mg_bf16 != 0, nullptr, nullptr);
 }
 
@@ -101,7 +129,8 @@ int mg_create(ffr_scene *s)
                      "use the CG solver)");
         return FFR_E_UNSUPPORTED;
     }
-    const Dims D = level_dims(s->nx, s->ny, s->nz, s->prm);
+    const double h0[3] = {s->dxv[0], s->dyv[0], s->dzv[0]};
+    const Dims D = level_dims(s->nx, s->ny, s->nz, h0, s->prm);
     Multigrid &M = s->mg;
     const bool bf16 = s->prm.mg_bf16 != 0;
     const size_t bytes = mg_layout(nullptr, D, bf16, nullptr, nullptr);
@@ -123,6 +152,7 @@ int mg_create(ffr_scene *s)
         MgLevel &L = M.lv[l];
         L.nx = D.n[l][0]; L.ny = D.n[l][1]; L.nz = D.n[l][2];
         L.n = (uint32_t)L.nx * L.ny * L.nz;
+        L.shx = D.sh[l][0]; L.shy = D.sh[l][1]; L.shz = D.sh[l][2];
         L.f_nz = make_fastdiv(L.nz);
         L.f_nynz = make_fastdiv(L.ny * L.nz);
     }
@@ -175,15 +205,18 @@ void mg_destroy(ffr_scene *s)
 
 // ---- V-cycle enqueue (z_0 = M r_0 on level 0) --------------------------------
 // kernel names per level for the profiler (bytes differ per level)
-static const char *kSmoothName[kMaxLevels] = {"mg_smooth_l0", "mg_smooth_l1", "mg_smooth_l2",
-                                              "mg_smooth_l3", "mg_smooth_l4", "mg_smooth_l5",
-                                              "mg_smooth_l6", "mg_smooth_l7"};
-static const char *kRestrictName[kMaxLevels] = {"mg_restrict_l0", "mg_restrict_l1", "mg_restrict_l2",
-                                                "mg_restrict_l3", "mg_restrict_l4", "mg_restrict_l5",
-                                                "mg_restrict_l6", "mg_restrict_l7"};
-static const char *kProlongName[kMaxLevels] = {"mg_prolong_l0", "mg_prolong_l1", "mg_prolong_l2",
-                                               "mg_prolong_l3", "mg_prolong_l4", "mg_prolong_l5",
-                                               "mg_prolong_l6", "mg_prolong_l7"};
+static const char *kSmoothName[kMaxLevels] = {
+    "mg_smooth_l0", "mg_smooth_l1", "mg_smooth_l2", "mg_smooth_l3", "mg_smooth_l4",
+    "mg_smooth_l5", "mg_smooth_l6", "mg_smooth_l7", "mg_smooth_l8", "mg_smooth_l9",
+    "mg_smooth_l10", "mg_smooth_l11"};
+static const char *kRestrictName[kMaxLevels] = {
+    "mg_restrict_l0", "mg_restrict_l1", "mg_restrict_l2", "mg_restrict_l3", "mg_restrict_l4",
+    "mg_restrict_l5", "mg_restrict_l6", "mg_restrict_l7", "mg_restrict_l8", "mg_restrict_l9",
+    "mg_restrict_l10", "mg_restrict_l11"};
+static const char *kProlongName[kMaxLevels] = {
+    "mg_prolong_l0", "mg_prolong_l1", "mg_prolong_l2", "mg_prolong_l3", "mg_prolong_l4",
+    "mg_prolong_l5", "mg_prolong_l6", "mg_prolong_l7", "mg_prolong_l8", "mg_prolong_l9",
+    "mg_prolong_l10", "mg_prolong_l11"};
 
 template <typename TC>
 static void vcycle(ffr_scene *s, int l)

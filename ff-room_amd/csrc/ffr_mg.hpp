@@ -13,13 +13,18 @@
 // Inner solve (FP32 vectors, FP64 dot accumulation): CG preconditioned by one
 // V-cycle, to inner_tol relative to its start.
 //
-// Hierarchy: 2x2x2 aggregation (an odd extent keeps a one-cell-thick last
-// aggregate). Level 0 evaluates its stencil from the cell flags and the
-// per-axis coefficients; every coarser level stores the scaled Galerkin
-// operator A_{l+1} = (1/2) P^T A_l P of piecewise-constant interpolation P as
-// one coupling per cell and axis (to the lower neighbour) plus a diagonal.
-// The factor 1/2 makes aggregation agree with re-discretisation on the 2h
-// grid. Smoother: red-black Gauss-Seidel, red then black before the coarse
+// Hierarchy: aggregation of 2 cells along each coarsened axis (an odd extent
+// keeps a one-cell-thick last aggregate). Each level coarsens the axes whose
+// spacing is at most 1/sqrt(2) of the largest (semi-coarsening, so that strongly
+// coupled directions of anisotropic cells are coarsened first), or every axis
+// when the cells are near-cubic; an axis with fewer than 4 cells is kept.
+// Level 0 evaluates its stencil from the cell flags and the per-axis
+// coefficients; every coarser level stores the Galerkin operator of
+// piecewise-constant interpolation P, P^T A_l P, with the couplings and the
+// diagonal part along each axis scaled by 1/2 when that axis was coarsened
+// (1 otherwise): this makes aggregation agree with re-discretisation on the
+// coarser grid. One coupling per cell and axis (to the lower neighbour), the
+// per-axis diagonal parts and their sum are stored. Smoother: red-black Gauss-Seidel, red then black before the coarse
 // correction and black then red after it; restriction is P^T and the
 // coarsest level runs symmetric red-black sweeps, so the preconditioner is
 // symmetric. Levels >= 1 keep z and r in bfloat16 when mg_bf16 is set
@@ -43,7 +48,7 @@
 
 namespace ffr {
 
-constexpr int kMaxLevels = 8;
+constexpr int kMaxLevels = 12;
 
 struct MgLevel {
     int nx, ny, nz;
@@ -53,6 +58,8 @@ struct MgLevel {
     uint8_t *active;       // 1 for unknowns
     float *cx, *cy, *cz;   // coupling to the -x/-y/-z neighbour (>= 0), 0 if none
     float *diag, *rdiag;   // 1 for cells with no unknown child
+    float *dgx, *dgy, *dgz;  // diagonal parts along each axis (sum = diag)
+    int shx, shy, shz;     // 1 where this level halved that axis of the finer one
     // vectors: FP32 on level 0, bfloat16 or FP32 on coarser levels
     void *z, *r;
     int bf16;
@@ -100,6 +107,7 @@ struct TailLevels {
     int first, last;
     int sweeps, coarse_sweeps;
 };
+static_assert(sizeof(TailLevels) <= 4096, "kernel arguments are limited to 4 KiB");
 
 // Scalars of the inner PCG and the outer FP64 defect correction, all kept on
 // the device so that launches carry no per-iteration arguments (a solve can

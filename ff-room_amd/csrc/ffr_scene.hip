@@ -175,7 +175,10 @@ double bits_to_double(unsigned long long b)
 size_t scene_bytes(const ffr_scene_desc *d, const ffr_params *p, bool with_mg)
 {
     size_t b = layout(nullptr, d, p, nullptr);
-    if (with_mg) b += mg_bytes(d->nx, d->ny, d->nz, *p);
+    if (with_mg) {
+        const double h0[3] = {d->dxv[0], d->dyv[0], d->dzv[0]};
+        b += mg_bytes(d->nx, d->ny, d->nz, h0, *p);
+    }
     return b;
 }
 bool scene_uses_mg(const ffr_params &p) { return resolve_method(p) == FFR_POISSON_MGPCG; }
@@ -442,6 +445,7 @@ int ffr_scene_create(ffr_ctx *ctx, const ffr_scene_desc *d, const ffr_params *pr
             return rc;
         }
     }
+    ctx->scenes.insert(s);
     *out = s;
     return FFR_OK;
 }
@@ -450,6 +454,7 @@ void ffr_scene_destroy(ffr_scene *s)
 {
     if (!s) return;
     hipSetDevice(s->ctx->device);
+    s->ctx->scenes.erase(s);
     serve_forget(s->ctx, s);
     hipStreamSynchronize(s->stream);
     if (s->task.ev) hipEventDestroy(s->task.ev);

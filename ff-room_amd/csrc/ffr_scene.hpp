@@ -4,6 +4,7 @@
 
 #include <functional>
 #include <map>
+#include <set>
 #include <memory>
 #include <string>
 #include <vector>
@@ -41,6 +42,9 @@ struct ffr_ctx {
     void prof_end(hipStream_t s);
     void prof_resolve();
     void prof_clear();
+
+    // scenes created on this context (destroyed with it)
+    std::set<ffr_scene *> scenes;
 
     // serving (ffr_serve.hip)
     struct Server;
@@ -157,7 +161,7 @@ int wait_point(ffr_scene *s, void *host, const void *dev, size_t bytes);
 // serving (ffr_serve.hip)
 void serve_forget(ffr_ctx *c, ffr_scene *s);
 // multigrid (ffr_mg.hip)
-size_t mg_bytes(int nx, int ny, int nz, const ffr_params &p);
+size_t mg_bytes(int nx, int ny, int nz, const double h0[3], const ffr_params &p);
 int mg_create(ffr_scene *s);
 void mg_destroy(ffr_scene *s);
 int mg_begin(ffr_scene *s);

@@ -129,7 +129,7 @@ __device__ __forceinline__ float rowc(const MgLevel &L, const T *x, int i, int j
     return of;
 }
 
-// ---- hierarchy construction: A_{l+1} = 1/2 P^T A_l P ----------------------
+// ---- hierarchy construction: A_{l+1} = P^T A_l P, per-axis scaled -----------
 // coupling of a fine cell to its lower neighbour along `axis` (0 unless both unknown)
 __device__ __forceinline__ float coupling0(const Geo &g, int axis, int i, int j, int k)
 {
@@ -142,7 +142,20 @@ __device__ __forceinline__ float couplingc(const MgLevel &F, int axis, int i, in
     const uint32_t c = ((uint32_t)i * F.ny + j) * F.nz + k;
     return axis == 0 ? F.cx[c] : (axis == 1 ? F.cy[c] : F.cz[c]);
 }
+// diagonal part along `axis` of an unknown fine cell: its couplings plus
+// 2 * scale per OUTFLOW neighbour (level 0, PoissonSolver.cpp:52-87), or the
+// stored part (coarser levels)
+__device__ __forceinline__ float dpart0(const Geo &g, uint16_t m, int axis)
+{
+    const uint32_t two = 3u << (2 * axis);
+    const float cnt = (float)(__popc(m & (two << 1)) + 2 * __popc(m & (two << 7)));
+    return (axis == 0 ? g.sx : (axis == 1 ? g.sy : g.sz)) * cnt;
+}
 
+// One thread per coarse cell: sums over the aggregate's children of the
+// diagonal parts (minus twice the couplings inside the aggregate) and of the
+// couplings across its lower faces, per axis, then scales each axis by 1/2
+// when the coarse level halved it.
 template <bool FINE0>
 __global__ void __launch_bounds__(kBlock) k_mg_build(Geo g, MgLevel F, MgLevel C)
 {
@@ -151,53 +164,62 @@ __global__ void __launch_bounds__(kBlock) k_mg_build(Geo g, MgLevel F, MgLevel C
     for (uint32_t I = sb + threadIdx.x; I < se; I += kBlock) {
         int ic, jc, kc;
         decode(I, C.f_nynz, C.f_nz, C.ny, C.nz, ic, jc, kc);
-        const int i0 = 2 * ic, i1 = min(2 * ic + 1, F.nx - 1);
-        const int j0 = 2 * jc, j1 = min(2 * jc + 1, F.ny - 1);
-        const int k0 = 2 * kc, k1 = min(2 * kc + 1, F.nz - 1);
-        float dsum = 0.f, inner = 0.f, cxs = 0.f, cys = 0.f, czs = 0.f;
+        const int i0 = ic << C.shx, i1 = min(i0 + (1 << C.shx) - 1, F.nx - 1);
+        const int j0 = jc << C.shy, j1 = min(j0 + (1 << C.shy) - 1, F.ny - 1);
+        const int k0 = kc << C.shz, k1 = min(k0 + (1 << C.shz) - 1, F.nz - 1);
+        float D[3] = {0.f, 0.f, 0.f}, inner[3] = {0.f, 0.f, 0.f}, low[3] = {0.f, 0.f, 0.f};
         bool any = false;
         for (int a = i0; a <= i1; a++)
             for (int b = j0; b <= j1; b++)
                 for (int d = k0; d <= k1; d++) {
-                    float dg;
                     bool act;
+                    float dp[3];
                     if (FINE0) {
                         const uint16_t m = g.nbm[cidx(g, a, b, d)];
                         act = (m & NB_SELF) != 0;
-                        dg = act ? diag0m(g, m) : 0.f;
+                        for (int ax = 0; ax < 3; ax++) dp[ax] = act ? dpart0(g, m, ax) : 0.f;
                     } else {
                         const uint32_t c = ((uint32_t)a * F.ny + b) * F.nz + d;
                         act = F.active[c] != 0;
-                        dg = act ? F.diag[c] : 0.f;
+                        dp[0] = act ? F.dgx[c] : 0.f;
+                        dp[1] = act ? F.dgy[c] : 0.f;
+                        dp[2] = act ? F.dgz[c] : 0.f;
                     }
                     if (!act) continue;
                     any = true;
-                    dsum += dg;
                     auto cp = [&](int axis, int x, int y, int z) {
                         return FINE0 ? coupling0(g, axis, x, y, z) : couplingc(F, axis, x, y, z);
                     };
-                    // couplings inside the aggregate (counted from the upper cell)
-                    if (a == i1 && i1 > i0) inner += cp(0, a, b, d);
-                    if (b == j1 && j1 > j0) inner += cp(1, a, b, d);
-                    if (d == k1 && k1 > k0) inner += cp(2, a, b, d);
-                    // couplings across the aggregate's lower faces
-                    if (a == i0 && ic > 0) cxs += cp(0, a, b, d);
-                    if (b == j0 && jc > 0) cys += cp(1, a, b, d);
-                    if (d == k0 && kc > 0) czs += cp(2, a, b, d);
+                    for (int ax = 0; ax < 3; ax++) D[ax] += dp[ax];
+                    // couplings to the lower neighbour: inside the aggregate, or
+                    // across its lower face
+                    if (a > i0) inner[0] += cp(0, a, b, d);
+                    else if (ic > 0) low[0] += cp(0, a, b, d);
+                    if (b > j0) inner[1] += cp(1, a, b, d);
+                    else if (jc > 0) low[1] += cp(1, a, b, d);
+                    if (d > k0) inner[2] += cp(2, a, b, d);
+                    else if (kc > 0) low[2] += cp(2, a, b, d);
                 }
+        const float sc[3] = {C.shx ? 0.5f : 1.f, C.shy ? 0.5f : 1.f, C.shz ? 0.5f : 1.f};
         if (any) {
-            float dg = 0.5f * (dsum - 2.f * inner);
+            float dg3[3];
+            for (int ax = 0; ax < 3; ax++) dg3[ax] = sc[ax] * (D[ax] - 2.f * inner[ax]);
+            float dg = dg3[0] + dg3[1] + dg3[2];
             if (!(dg > 1e-30f)) dg = 1.f;  // a pocket with no Dirichlet cell: keep finite
             C.active[I] = 1;
             C.diag[I] = dg;
             C.rdiag[I] = 1.f / dg;
-            C.cx[I] = 0.5f * cxs;
-            C.cy[I] = 0.5f * cys;
-            C.cz[I] = 0.5f * czs;
+            C.dgx[I] = dg3[0];
+            C.dgy[I] = dg3[1];
+            C.dgz[I] = dg3[2];
+            C.cx[I] = sc[0] * low[0];
+            C.cy[I] = sc[1] * low[1];
+            C.cz[I] = sc[2] * low[2];
         } else {
             C.active[I] = 0;
             C.diag[I] = 1.f;
             C.rdiag[I] = 1.f;
+            C.dgx[I] = C.dgy[I] = C.dgz[I] = 0.f;
             C.cx[I] = C.cy[I] = C.cz[I] = 0.f;
         }
     }
@@ -385,9 +407,10 @@ __global__ void __launch_bounds__(kBlock) k_mg_restrict(Geo g, MgLevel F, MgLeve
                 const uint32_t I = t >> 3, q = t & 7u;
                 int ic, jc, kc;
                 decode(I, C.f_nynz, C.f_nz, C.ny, C.nz, ic, jc, kc);
-                const int a = 2 * ic + (int)(q >> 2), b = 2 * jc + (int)((q >> 1) & 1u),
-                          d = 2 * kc + (int)(q & 1u);
-                in[u] = a < F.nx && b < F.ny && d < F.nz;
+                const int qx = (int)(q >> 2), qy = (int)((q >> 1) & 1u), qz = (int)(q & 1u);
+                const int a = (ic << C.shx) + qx, b = (jc << C.shy) + qy, d = (kc << C.shz) + qz;
+                in[u] = qx <= C.shx && qy <= C.shy && qz <= C.shz && a < F.nx && b < F.ny &&
+                        d < F.nz;
                 const uint32_t c = in[u] ? ((uint32_t)a * F.ny + b) * F.nz + d : 0u;
                 const uint32_t col = in[u] ? (uint32_t)(a + b + d) & 1u : 0u;
                 const uint32_t own = (col ? H : 0u) + (c >> 1);
@@ -420,9 +443,10 @@ __global__ void __launch_bounds__(kBlock) k_mg_restrict(Geo g, MgLevel F, MgLeve
         const uint32_t I = t >> 3, q = t & 7u;
         int ic, jc, kc;
         decode(I, C.f_nynz, C.f_nz, C.ny, C.nz, ic, jc, kc);
-        const int a = 2 * ic + (int)(q >> 2), b = 2 * jc + (int)((q >> 1) & 1u),
-                  d = 2 * kc + (int)(q & 1u);
-        const bool in = a < F.nx && b < F.ny && d < F.nz;
+        const int qx = (int)(q >> 2), qy = (int)((q >> 1) & 1u), qz = (int)(q & 1u);
+        const int a = (ic << C.shx) + qx, b = (jc << C.shy) + qy, d = (kc << C.shz) + qz;
+        const bool in = qx <= C.shx && qy <= C.shy && qz <= C.shz && a < F.nx && b < F.ny &&
+                        d < F.nz;
         const uint32_t c = in ? ((uint32_t)a * F.ny + b) * F.nz + d : 0u;
         const uint8_t act = buf_ldu8(o.act, c);
         const float rv = bldv(br, c, tag), zv = bldv(bz, c, tag), dg = buf_ldf(o.dg, c);
@@ -447,7 +471,7 @@ __global__ void __launch_bounds__(kBlock) k_mg_prolong(Geo g, MgLevel F, MgLevel
     for (uint32_t c = sb + threadIdx.x; c < se; c += kBlock) {
         int i, j, k;
         decode(c, F.f_nynz, F.f_nz, F.ny, F.nz, i, j, k);
-        const uint32_t P = ((uint32_t)(i >> 1) * C.ny + (j >> 1)) * C.nz + (k >> 1);
+        const uint32_t P = ((uint32_t)(i >> C.shx) * C.ny + (j >> C.shy)) * C.nz + (k >> C.shz);
         // level 0: split layout, colour (i+j+k)&1
         const uint32_t zi = L0 ? (((i + j + k) & 1) ? F.H : 0u) + (c >> 1) : c;
         const bool act = L0 ? (g.nbm[c] & NB_SELF) != 0 : F.active[c] != 0;
@@ -500,6 +524,7 @@ template <typename T>
 struct TailView {  // one level in LDS
     uint32_t n;
     int nx, ny, nz;
+    int shx, shy, shz;
     FastDiv f_nz, f_nynz;
     float *cx, *cy, *cz, *diag, *rdiag;
     T *z, *r;
@@ -515,6 +540,9 @@ __device__ __forceinline__ TailView<T> tail_view(const TailLevels &tl, unsigned 
     v.nx = L.nx;
     v.ny = L.ny;
     v.nz = L.nz;
+    v.shx = L.shx;
+    v.shy = L.shy;
+    v.shz = L.shz;
     v.f_nz = L.f_nz;
     v.f_nynz = L.f_nynz;
     float *f = reinterpret_cast<float *>(smem + tl.off[l]);
@@ -597,12 +625,14 @@ __global__ void __launch_bounds__(kTailThreads) k_mg_tail(TailLevels tl, const M
         for (uint32_t I = threadIdx.x; I < C.n; I += kTailThreads) {
             int ic, jc, kc;
             decode(I, C.f_nynz, C.f_nz, C.ny, C.nz, ic, jc, kc);
-            const int i1 = min(2 * ic + 1, F.nx - 1), j1 = min(2 * jc + 1, F.ny - 1),
-                      k1 = min(2 * kc + 1, F.nz - 1);
+            const int i0 = ic << C.shx, j0 = jc << C.shy, k0 = kc << C.shz;
+            const int i1 = min(i0 + (1 << C.shx) - 1, F.nx - 1),
+                      j1 = min(j0 + (1 << C.shy) - 1, F.ny - 1),
+                      k1 = min(k0 + (1 << C.shz) - 1, F.nz - 1);
             float sum = 0.f;
-            for (int a = 2 * ic; a <= i1; a++)
-                for (int b = 2 * jc; b <= j1; b++)
-                    for (int d = 2 * kc; d <= k1; d++) {
+            for (int a = i0; a <= i1; a++)
+                for (int b = j0; b <= j1; b++)
+                    for (int d = k0; d <= k1; d++) {
                         const uint32_t c = ((uint32_t)a * F.ny + b) * F.nz + d;
                         if (!F.act[c]) continue;
                         const float of = tail_row(F, F.z, a, b, d, c);
@@ -629,7 +659,8 @@ __global__ void __launch_bounds__(kTailThreads) k_mg_tail(TailLevels tl, const M
             if (!F.act[c]) continue;
             int i, j, k;
             decode(c, F.f_nynz, F.f_nz, F.ny, F.nz, i, j, k);
-            const uint32_t P = ((uint32_t)(i >> 1) * C.ny + (j >> 1)) * C.nz + (k >> 1);
+            const uint32_t P =
+                ((uint32_t)(i >> C.shx) * C.ny + (j >> C.shy)) * C.nz + (k >> C.shz);
             stv(F.z, c, ldv(F.z, c) + ldv(C.z, P));
         }
         __syncthreads();

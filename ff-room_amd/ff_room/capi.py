@@ -302,7 +302,8 @@ class DeviceScene:
 
     def close(self):
         if getattr(self, "h", None):
-            self.ctx._l.ffr_scene_destroy(self.h)
+            if getattr(self.ctx, "h", None):  # a destroyed context took its scenes along
+                self.ctx._l.ffr_scene_destroy(self.h)
             self.h = None
 
     def __del__(self):

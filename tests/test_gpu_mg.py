@@ -54,6 +54,28 @@ def test_mgpcg_solves_the_reference_system(ctx, bf16, precision):
     assert rel_l2(p, x) < 1e-9
 
 
+@pytest.mark.parametrize("dims,size", [((16, 12, 14), (4.0, 3.0, 0.7)),   # z-cells 3.4x thinner
+                                       ((6, 20, 9), (3.0, 1.0, 2.4)),    # y strongly coupled
+                                       ((11, 9, 13), (1.0, 1.0, 1.0))])  # odd extents, near-cubic
+def test_mgpcg_semicoarsened_hierarchy_solves_the_system(ctx, dims, size):
+    """Anisotropic cells coarsen the strongly coupled axes first: the solve
+    still equals the dense solution, within few iterations."""
+    rng = np.random.default_rng(11)
+    nx, ny, nz = dims
+    f = rng.choice([0] * 9 + [SOLID], size=(nx, ny, nz)).astype(np.uint8)
+    f[:, :, -1] = OUTFLOW
+    sc = SceneArrays(nx, ny, nz, *size, f.reshape(-1))
+    A = _dense(sc)
+    b = _rhs(sc, 5)
+    x = np.linalg.solve(A, b)
+    d = device_scene(ctx, sc, capi_params({}, poisson_method=capi.POISSON_MGPCG, poisson_tol=1e-11,
+                                          precision=capi.PREC_MIXED, poisson_max_iter=500))
+    p, info = d.poisson(b)
+    assert info["residual"] < 1e-11
+    assert rel_l2(p, x) < 1e-9
+    assert info["iterations"] <= 60, info
+
+
 def test_mgpcg_pinned_outflow_is_the_reference_multigrid_system(ctx):
     """use_multigrid: OUTFLOW pressure pinned to 0 (PoissonSolver.cpp:201)."""
     rng = np.random.default_rng(2)
