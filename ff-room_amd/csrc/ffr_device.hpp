@@ -11,6 +11,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <algorithm>
+
 namespace ffr {
 
 constexpr int kBlock = 256;  // threads per block (4 wave64)
@@ -71,6 +73,18 @@ enum : uint16_t {
     NB_SELF_SOLID = 1u << 15
 };
 
+// Per-face mask code (built once per scene, k_face_codes), concatenated
+// over the u, v and w face arrays: what the no-slip, fan and outflow
+// conditions (BoundaryManager.cpp:60-130, 180-220) do to that face.
+//   FC_BOUND   the face lies on the domain boundary (not updated by the stencils)
+//   FC_ZERO    no-slip: zeroed (wall face whose cell is not OUTFLOW, or a face of a SOLID cell)
+//   FC_FAN     set to the velocity of fan (code >> 8), the last covering fan with a
+//              non-zero component along the face's axis
+//   FC_OUT_LO / FC_OUT_HI  boundary face of an outflow patch on the low / high
+//              wall: after the correction it copies the face one step inside
+enum : uint16_t { FC_BOUND = 1, FC_ZERO = 2, FC_FAN = 4, FC_OUT_LO = 8, FC_OUT_HI = 16 };
+constexpr int kMaxFans = 255;
+
 // Geometry of one scene as the kernels see it (passed by value).
 struct Geo {
     int nx, ny, nz;
@@ -81,11 +95,14 @@ struct Geo {
     const double *hx, *hy, *hz;        // cell widths dxv, dyv, dzv
     const double *i2x, *i2y, *i2z;     // 1/(h*h), FluidSolver.cpp:101-103
     const double *cx, *cy, *cz;        // centre distance 0.5*(d[i-1]+d[i]) at index i (i>=1)
+    const double *ihx, *ihy, *ihz;     // 1/h and 1/centre distance (the FP32 form
+    const double *icx, *icy, *icz;     // multiplies where the reference divides)
     const double *lap;                 // slx srx | sly sry | slz srz (PoissonSolver.cpp:34-49)
     // FP32 copies
     const float *hxf, *hyf, *hzf, *ihxf, *ihyf, *ihzf, *i2xf, *i2yf, *i2zf;
-    const float *cxf, *cyf, *czf, *lapf;
+    const float *cxf, *cyf, *czf, *icxf, *icyf, *iczf, *lapf;
     const uint16_t *nbm;               // neighbour codes (see NB_*)
+    const uint16_t *fcode;             // face mask codes (see FC_*), u | v | w
     double sxd, syd, szd;              // uniform-grid coupling 1/h^2 per axis (0 if not uniform)
     float sx, sy, sz;
     int uniform;
@@ -148,6 +165,43 @@ __device__ __forceinline__ void block_span(uint32_t n, uint32_t blk, uint32_t &b
     b = (uint32_t)(b64 < n ? b64 : n);
     const uint64_t e64 = b64 + (uint64_t)per * kBlock;
     e = (uint32_t)(e64 < n ? e64 : n);
+}
+
+// The same split of n elements over G of the grid's blocks (a section of
+// the grid handling one array).
+__device__ __forceinline__ void block_span_of(uint32_t n, uint32_t blk, uint32_t G, uint32_t &b,
+                                              uint32_t &e)
+{
+    const uint32_t chunks = (n + kBlock - 1) / kBlock;
+    const uint32_t per = (chunks + G - 1) / G;
+    const uint64_t b64 = (uint64_t)blk * per * kBlock;
+    b = (uint32_t)(b64 < n ? b64 : n);
+    const uint64_t e64 = b64 + (uint64_t)per * kBlock;
+    e = (uint32_t)(e64 < n ? e64 : n);
+}
+
+// Blocks of a grid of G split over the u, v and w face arrays in proportion
+// to their sizes (each at least one), so a block works on one array only.
+struct FaceSections {
+    uint32_t gu, gv;  // blocks [0, gu) on u, [gu, gu+gv) on v, the rest on w
+};
+inline FaceSections face_sections(uint64_t nu, uint64_t nv, uint64_t nw, uint32_t G)
+{
+    const uint64_t n = nu + nv + nw;
+    uint32_t gu = (uint32_t)std::max<uint64_t>(1, (G * nu + n / 2) / n);
+    uint32_t gv = (uint32_t)std::max<uint64_t>(1, (G * nv + n / 2) / n);
+    if (gu + gv >= G) gu = gv = G / 3 > 0 ? G / 3 : 1;
+    return {gu, gv};
+}
+// section of logical block blk: axis, block index within it, its block count
+__device__ __forceinline__ int face_section(const FaceSections &fs, uint32_t blk, uint32_t &lb,
+                                            uint32_t &G)
+{
+    if (blk < fs.gu) { lb = blk; G = fs.gu; return 0; }
+    if (blk < fs.gu + fs.gv) { lb = blk - fs.gu; G = fs.gv; return 1; }
+    lb = blk - fs.gu - fs.gv;
+    G = gridDim.x - fs.gu - fs.gv;
+    return 2;
 }
 
 // ---- reductions -----------------------------------------------------------

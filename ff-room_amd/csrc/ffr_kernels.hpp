@@ -83,6 +83,8 @@ void launch_convert(hipStream_t s, const double *src, R *dst, uint64_t n);
 template <typename R>
 void launch_convert_back(hipStream_t s, const R *src, double *dst, uint64_t n);
 
+// face mask codes of a scene (once, at creation)
+void launch_face_codes(hipStream_t s, const Geo &g, uint16_t *fcode);
 // plain FP64 copy (calibration of memory counters)
 void launch_copy64(hipStream_t s, const double *src, double *dst, uint64_t n);
 

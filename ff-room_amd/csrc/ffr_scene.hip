@@ -24,6 +24,7 @@ bool valid_desc(const ffr_scene_desc *d, char *err, size_t errn)
                            (uint64_t)d->nx * (d->ny + 1) * d->nz +
                            (uint64_t)d->nx * d->ny * (d->nz + 1);
     if (faces >= (1ull << 31)) { snprintf(err, errn, "grid too large for 32-bit indexing"); return false; }
+    if (d->n_fans > kMaxFans) { snprintf(err, errn, "at most %d fans per scene", kMaxFans); return false; }
     if (!d->dxv || !d->dyv || !d->dzv || !d->cell_type) {
         snprintf(err, errn, "spacings and cell_type are required");
         return false;
@@ -87,6 +88,7 @@ bool valid_params(const ffr_params *p, char *err, size_t errn)
 struct ArenaPtrs {
     uint8_t *flags;
     uint16_t *nbm;
+    uint16_t *fcode;
     double *ax64;
     float *ax32;
     FanDev *fans;
@@ -116,6 +118,7 @@ size_t layout(char *base, const ffr_scene_desc *d, const ffr_params *p, ArenaPtr
     ArenaPtrs a{};
     a.flags = (uint8_t *)take(nc);
     a.nbm = (uint16_t *)take(2 * nc);
+    a.fcode = (uint16_t *)take(2 * (nu + nv + nw));
     a.ax64 = (double *)take(sizeof(double) * 8 * na);
     a.ax32 = (float *)take(sizeof(float) * 10 * na);
     a.fans = (FanDev *)take(sizeof(FanDev) * std::max(1, d->n_fans));
@@ -246,9 +249,9 @@ int ffr_scene_create(ffr_ctx *ctx, const ffr_scene_desc *d, const ffr_params *pr
     const int nx = d->nx, ny = d->ny, nz = d->nz, na = nx + ny + nz;
     std::vector<double> ax64(8 * (size_t)na);
     std::vector<float> ax32(10 * (size_t)na);
-    // per-axis blocks: [h | i2 | c] then lap (sl | sr) per axis
-    double *h64[3], *i2_64[3], *c64[3];
-    float *h32[3], *ih32[3], *i2_32[3], *c32[3];
+    // per-axis blocks: [h | i2 | c | 1/h | 1/c] then lap (sl | sr) per axis
+    double *h64[3], *i2_64[3], *c64[3], *ih64[3], *ic64[3];
+    float *h32[3], *ih32[3], *i2_32[3], *c32[3], *ic32[3];
     const int n[3] = {nx, ny, nz};
     const std::vector<double> *hv[3] = {&s->dxv, &s->dyv, &s->dzv};
     {
@@ -256,10 +259,13 @@ int ffr_scene_create(ffr_ctx *ctx, const ffr_scene_desc *d, const ffr_params *pr
         for (int q = 0; q < 3; q++) { h64[q] = &ax64[o]; o += n[q]; }
         for (int q = 0; q < 3; q++) { i2_64[q] = &ax64[o]; o += n[q]; }
         for (int q = 0; q < 3; q++) { c64[q] = &ax64[o]; o += n[q]; }
+        for (int q = 0; q < 3; q++) { ih64[q] = &ax64[o]; o += n[q]; }
+        for (int q = 0; q < 3; q++) { ic64[q] = &ax64[o]; o += n[q]; }
         for (int q = 0; q < 3; q++) { h32[q] = &ax32[o32]; o32 += n[q]; }
         for (int q = 0; q < 3; q++) { ih32[q] = &ax32[o32]; o32 += n[q]; }
         for (int q = 0; q < 3; q++) { i2_32[q] = &ax32[o32]; o32 += n[q]; }
         for (int q = 0; q < 3; q++) { c32[q] = &ax32[o32]; o32 += n[q]; }
+        for (int q = 0; q < 3; q++) { ic32[q] = &ax32[o32]; o32 += n[q]; }
         // lap blocks follow at o (2*na doubles) and o32 (2*na floats)
         double *lap = &ax64[o];
         float *lapf = &ax32[o32];
@@ -271,10 +277,13 @@ int ffr_scene_create(ffr_ctx *ctx, const ffr_scene_desc *d, const ffr_params *pr
                 h64[q][i] = hi;
                 i2_64[q][i] = 1.0 / (hi * hi);                           // FluidSolver.cpp:101-103
                 c64[q][i] = i > 0 ? 0.5 * (hh[i - 1] + hh[i]) : hi;      // :343, 355, 367
+                ih64[q][i] = 1.0 / hi;           // reciprocals: the FP32 form multiplies
+                ic64[q][i] = 1.0 / c64[q][i];
                 h32[q][i] = (float)hi;
                 ih32[q][i] = (float)(1.0 / hi);
                 i2_32[q][i] = (float)(1.0 / (hi * hi));
                 c32[q][i] = (float)c64[q][i];
+                ic32[q][i] = (float)ic64[q][i];
                 const double hl = (i > 0) ? 0.5 * (hh[i - 1] + hh[i]) : hh[i];         // PoissonSolver.cpp:34-49
                 const double hr = (i < n[q] - 1) ? 0.5 * (hh[i] + hh[i + 1]) : hh[i];
                 lap[lo + i] = 1.0 / (hl * hh[i]);
@@ -391,7 +400,7 @@ int ffr_scene_create(ffr_ctx *ctx, const ffr_scene_desc *d, const ffr_params *pr
     {
         auto uni = [](const std::vector<double> &h) {
             for (double x : h)
-                if (std::fabs(x - h[0]) > 1e-12 * std::fabs(h[0])) return false;
+                if (x != h[0]) return false;  // exactly: the uniform kernels read h[0] for every cell
             return true;
         };
         g.uniform = uni(s->dxv) && uni(s->dyv) && uni(s->dzv);
@@ -408,12 +417,15 @@ int ffr_scene_create(ffr_ctx *ctx, const ffr_scene_desc *d, const ffr_params *pr
         g.hx = b64; g.hy = b64 + nx; g.hz = b64 + nx + ny; b64 += na;
         g.i2x = b64; g.i2y = b64 + nx; g.i2z = b64 + nx + ny; b64 += na;
         g.cx = b64; g.cy = b64 + nx; g.cz = b64 + nx + ny; b64 += na;
+        g.ihx = b64; g.ihy = b64 + nx; g.ihz = b64 + nx + ny; b64 += na;
+        g.icx = b64; g.icy = b64 + nx; g.icz = b64 + nx + ny; b64 += na;
         g.lap = b64;
         const float *b32 = a.ax32;
         g.hxf = b32; g.hyf = b32 + nx; g.hzf = b32 + nx + ny; b32 += na;
         g.ihxf = b32; g.ihyf = b32 + nx; g.ihzf = b32 + nx + ny; b32 += na;
         g.i2xf = b32; g.i2yf = b32 + nx; g.i2zf = b32 + nx + ny; b32 += na;
         g.cxf = b32; g.cyf = b32 + nx; g.czf = b32 + nx + ny; b32 += na;
+        g.icxf = b32; g.icyf = b32 + nx; g.iczf = b32 + nx + ny; b32 += na;
         g.lapf = b32;
     }
     g.fans = a.fans;
@@ -422,6 +434,13 @@ int ffr_scene_create(ffr_ctx *ctx, const ffr_scene_desc *d, const ffr_params *pr
     g.openings = a.opens;
     g.n_openings = d->n_openings;
     g.open_threads = open_threads;
+    g.fcode = a.fcode;
+    launch_face_codes(st, g, a.fcode);
+    if (hipStreamSynchronize(st) != hipSuccess) {
+        ctx->fail("face code construction failed");
+        ffr_scene_destroy(s);
+        return FFR_E_DEVICE;
+    }
 
     s->u = a.u; s->v = a.v; s->w = a.w; s->us = a.us; s->vs = a.vs; s->ws = a.ws;
     s->T = a.T; s->Tn = a.Tn;

@@ -2,7 +2,8 @@
 //
 // K1  predict:    u*, v*, w* (advection + diffusion, FluidSolver.cpp:83-309)
 //                 + buoyancy on w* (:373-386) + no-slip and fan masks
-//                 (BoundaryManager.cpp:60-130), one face per thread.
+//                 (BoundaryManager.cpp:60-130) from the scene's per-face mask
+//                 codes, one face per thread, each block on one face array.
 // K4  rhs:        b = -(rho/dt) div(u*) (:311-330).
 // K6  correct:    u = u* - (dt/rho) grad p (:332-371) + no-slip, fan and
 //                 outflow masks (BoundaryManager.cpp:60-130, 180-220)
@@ -22,67 +23,83 @@ namespace ffr {
 
 template <typename R>
 struct Axes {
-    const R *hx, *hy, *hz, *i2x, *i2y, *i2z;
+    const R *hx, *hy, *hz, *i2x, *i2y, *i2z, *ihx, *ihy, *ihz;
 };
 __device__ __forceinline__ Axes<double> axes_of(const Geo &g, double)
 {
-    return {g.hx, g.hy, g.hz, g.i2x, g.i2y, g.i2z};
+    return {g.hx, g.hy, g.hz, g.i2x, g.i2y, g.i2z, g.ihx, g.ihy, g.ihz};
 }
 __device__ __forceinline__ Axes<float> axes_of(const Geo &g, float)
 {
-    return {g.hxf, g.hyf, g.hzf, g.i2xf, g.i2yf, g.i2zf};
+    return {g.hxf, g.hyf, g.hzf, g.i2xf, g.i2yf, g.i2zf, g.ihxf, g.ihyf, g.ihzf};
+}
+
+// Divisions by a spacing h: the FP64 form divides exactly as the reference
+// does; the FP32 form multiplies by the reciprocal (rounded once on the host),
+// which spares the IEEE division sequence. by_h2 divides by h*h.
+__device__ __forceinline__ double by_h(double x, double h, double) { return x / h; }
+__device__ __forceinline__ float by_h(float x, float, float ih) { return x * ih; }
+__device__ __forceinline__ double by_h2(double x, double h, double) { return x / (h * h); }
+__device__ __forceinline__ float by_h2(float x, float, float i2) { return x * i2; }
+// an FP64 quantity of the step of field type R
+template <typename R>
+__device__ __forceinline__ double by_h64(double x, double h, double ih)
+{
+    return sizeof(R) == 8 ? x / h : x * ih;
 }
 
 // ---- one-direction advection terms, FluidSolver.cpp:47-81 --------------
+// x / 8 is computed as x * 0.125: the same correctly rounded value (a power
+// of two), so the FP64 form keeps the reference's bits.
 template <typename R>
-__device__ __forceinline__ R adv_upwind(R m, R c, R p, R vel, R h)
+__device__ __forceinline__ R adv_upwind(R m, R c, R p, R vel, R h, R ih)
 {
-    if (vel > R(0)) return vel * (c - m) / h;
-    return vel * (p - c) / h;
+    if (vel > R(0)) return by_h(vel * (c - m), h, ih);
+    return by_h(vel * (p - c), h, ih);
 }
 
 template <typename R>
-__device__ __forceinline__ R adv_quick(R mm, R m, R c, R p, R vel, R h)
+__device__ __forceinline__ R adv_quick(R mm, R m, R c, R p, R vel, R h, R ih)
 {
     R right, left;
     if (vel >= R(0)) {
-        right = (R(6) * c + R(3) * p - m) / R(8);
-        left = (R(6) * m + R(3) * c - mm) / R(8);
+        right = (R(6) * c + R(3) * p - m) * R(0.125);
+        left = (R(6) * m + R(3) * c - mm) * R(0.125);
     } else {  // the reference's u<0 branch (:64-69)
-        right = (R(6) * p + R(3) * c - mm) / R(8);
-        left = (R(6) * c + R(3) * p - m) / R(8);
+        right = (R(6) * p + R(3) * c - mm) * R(0.125);
+        left = (R(6) * c + R(3) * p - m) * R(0.125);
     }
-    return vel * (right - left) / h;
+    return by_h(vel * (right - left), h, ih);
 }
 
 template <typename R>
-__device__ __forceinline__ R adv_lw(R m, R c, R p, R vel, R dt, R h)
+__device__ __forceinline__ R adv_lw(R m, R c, R p, R vel, R dt, R h, R ih)
 {
-    R cr = vel * dt / h;
+    R cr = by_h(vel * dt, h, ih);
     R fr = vel * (R(0.5) * (c + p) - R(0.5) * cr * (p - c));
     R fl = vel * (R(0.5) * (m + c) - R(0.5) * cr * (c - m));
-    return (fr - fl) / h;
+    return by_h(fr - fl, h, ih);
 }
 
 // Scheme switch and diffusion for one face (:132-162). nb = {-x,+x,-y,+y,-z,+z}.
 template <typename R>
 __device__ __forceinline__ R face_update(int scheme, bool qok, const R nb[6], const R q[3], R c,
-                                         const R vel[3], const R h[3], const R i2[3], R dt,
-                                         R nu)
+                                         const R vel[3], const R h[3], const R ih[3],
+                                         const R i2[3], R dt, R nu)
 {
     R adv;
     if (scheme == QUICK && qok) {
-        adv = adv_quick(q[0], nb[0], c, nb[1], vel[0], h[0]) +
-              adv_quick(q[1], nb[2], c, nb[3], vel[1], h[1]) +
-              adv_quick(q[2], nb[4], c, nb[5], vel[2], h[2]);
+        adv = adv_quick(q[0], nb[0], c, nb[1], vel[0], h[0], ih[0]) +
+              adv_quick(q[1], nb[2], c, nb[3], vel[1], h[1], ih[1]) +
+              adv_quick(q[2], nb[4], c, nb[5], vel[2], h[2], ih[2]);
     } else if (scheme == LAX_WENDROFF) {
-        adv = adv_lw(nb[0], c, nb[1], vel[0], dt, h[0]) +
-              adv_lw(nb[2], c, nb[3], vel[1], dt, h[1]) +
-              adv_lw(nb[4], c, nb[5], vel[2], dt, h[2]);
+        adv = adv_lw(nb[0], c, nb[1], vel[0], dt, h[0], ih[0]) +
+              adv_lw(nb[2], c, nb[3], vel[1], dt, h[1], ih[1]) +
+              adv_lw(nb[4], c, nb[5], vel[2], dt, h[2], ih[2]);
     } else {
-        adv = adv_upwind(nb[0], c, nb[1], vel[0], h[0]) +
-              adv_upwind(nb[2], c, nb[3], vel[1], h[1]) +
-              adv_upwind(nb[4], c, nb[5], vel[2], h[2]);
+        adv = adv_upwind(nb[0], c, nb[1], vel[0], h[0], ih[0]) +
+              adv_upwind(nb[2], c, nb[3], vel[1], h[1], ih[1]) +
+              adv_upwind(nb[4], c, nb[5], vel[2], h[2], ih[2]);
     }
     R dif = nu * (i2[0] * (nb[0] - R(2) * c + nb[1]) + i2[1] * (nb[2] - R(2) * c + nb[3]) +
                   i2[2] * (nb[4] - R(2) * c + nb[5]));
@@ -92,24 +109,27 @@ __device__ __forceinline__ R face_update(int scheme, bool qok, const R nb[6], co
 __device__ __forceinline__ double fan_v(const FanDev &f, int a, double) { return f.vel[a]; }
 __device__ __forceinline__ float fan_v(const FanDev &f, int a, float) { return f.velf[a]; }
 
-// No-slip + fan masks of one face along `axis` at (i,j,k) (BoundaryManager.cpp:60-130).
-// The face lies between cells lo = (i,j,k) - e_axis and hi = (i,j,k).
-template <typename R>
-__device__ __forceinline__ R mask_face(const Geo &g, int axis, int i, int j, int k, R val)
+// Mask code of one face (run once per scene): no-slip zeroes wall faces
+// unless the adjacent boundary cell is OUTFLOW (BoundaryManager.cpp:62-94)
+// and all faces of SOLID cells (:96-107); fans then set their faces, in
+// registration order, the last covering fan with a non-zero component along
+// the face's axis winning (:111-130); outflow patches mark their boundary
+// faces for the copy after the correction (:180-220). The face lies between
+// cells lo = (i,j,k) - e_axis and hi = (i,j,k).
+__device__ __forceinline__ uint16_t face_code(const Geo &g, int axis, int i, int j, int k)
 {
     const uint8_t *f = g.flags;
     const int n_ax = axis == 0 ? g.nx : (axis == 1 ? g.ny : g.nz);
     const int pos = axis == 0 ? i : (axis == 1 ? j : k);
-    const int stride = axis == 0 ? g.ny * g.nz : (axis == 1 ? g.nz : 1);
     const int lo_i = i - (axis == 0), lo_j = j - (axis == 1), lo_k = k - (axis == 2);
-    // walls: zeroed unless the adjacent boundary cell is OUTFLOW (:62-94)
-    if (pos == 0 && f[cidx(g, i, j, k)] != OUTFLOW) val = R(0);
-    if (pos == n_ax && f[cidx(g, lo_i, lo_j, lo_k)] != OUTFLOW) val = R(0);
-    // faces of SOLID cells (:96-107)
-    if (pos > 0 && f[cidx(g, lo_i, lo_j, lo_k)] == SOLID) val = R(0);
-    if (pos < n_ax && f[cidx(g, i, j, k)] == SOLID) val = R(0);
-    (void)stride;
-    // fans in registration order, the last covering fan wins (:111-130)
+    uint16_t code = 0;
+    if (pos == 0 || pos == n_ax) code |= FC_BOUND;
+    bool zero = false;
+    if (pos == 0 && f[cidx(g, i, j, k)] != OUTFLOW) zero = true;
+    if (pos == n_ax && f[cidx(g, lo_i, lo_j, lo_k)] != OUTFLOW) zero = true;
+    if (pos > 0 && f[cidx(g, lo_i, lo_j, lo_k)] == SOLID) zero = true;
+    if (pos < n_ax && f[cidx(g, i, j, k)] == SOLID) zero = true;
+    int fan = -1;
     for (int n = 0; n < g.n_fans; n++) {
         const FanDev &fn = g.fans[n];
         if (fn.vel[axis] == 0.0) continue;
@@ -120,13 +140,71 @@ __device__ __forceinline__ R mask_face(const Geo &g, int axis, int i, int j, int
             in = i >= fn.i0 && i <= fn.i1 && j >= fn.j0 && j - 1 <= fn.j1 && k >= fn.k0 && k <= fn.k1;
         else
             in = i >= fn.i0 && i <= fn.i1 && j >= fn.j0 && j <= fn.j1 && k >= fn.k0 && k - 1 <= fn.k1;
-        if (in) val = fan_v(fn, axis, R(0));
+        if (in) fan = n;
     }
+    if (fan >= 0) code |= FC_FAN | (uint16_t)(fan << 8);
+    else if (zero) code |= FC_ZERO;
+    // outflow copy (BoundaryManager.cpp:180-220)
+    if (axis == 0) {
+        if (i == 0 && g.wall_of[wall_x(g, 0, j, k)]) code |= FC_OUT_LO;
+        if (i == g.nx && g.wall_of[wall_x(g, 1, j, k)]) code |= FC_OUT_HI;
+    } else if (axis == 1) {
+        if (j == 0 && g.wall_of[wall_y(g, 0, i, k)]) code |= FC_OUT_LO;
+        if (j == g.ny && g.wall_of[wall_y(g, 1, i, k)]) code |= FC_OUT_HI;
+    } else {
+        if (k == 0 && g.wall_of[wall_z(g, 0, i, j)]) code |= FC_OUT_LO;
+        if (k == g.nz && g.wall_of[wall_z(g, 1, i, j)]) code |= FC_OUT_HI;
+    }
+    return code;
+}
+
+// (i, j, k) of face e of the array along `axis`
+__device__ __forceinline__ void face_ijk(const Geo &g, int axis, uint32_t e, int &i, int &j, int &k)
+{
+    if (axis == 0) {
+        const uint32_t ii = fdiv(e, g.f_nynz), rem = e - ii * (g.ny * g.nz);
+        const uint32_t jj = fdiv(rem, g.f_nz);
+        i = ii; j = jj; k = rem - jj * g.nz;
+    } else if (axis == 1) {
+        const uint32_t ii = fdiv(e, g.f_ny1nz), rem = e - ii * ((g.ny + 1) * g.nz);
+        const uint32_t jj = fdiv(rem, g.f_nz);
+        i = ii; j = jj; k = rem - jj * g.nz;
+    } else {
+        const uint32_t ii = fdiv(e, g.f_nynz1), rem = e - ii * (g.ny * (g.nz + 1));
+        const uint32_t jj = fdiv(rem, g.f_nz1);
+        i = ii; j = jj; k = rem - jj * (g.nz + 1);
+    }
+}
+
+__global__ void __launch_bounds__(kBlock) k_face_codes(Geo g, uint16_t *fcode)
+{
+    const uint32_t n = g.nu + g.nv + g.nw;
+    for (uint32_t t = blockIdx.x * kBlock + threadIdx.x; t < n; t += gridDim.x * kBlock) {
+        const int axis = t < g.nu ? 0 : (t < g.nu + g.nv ? 1 : 2);
+        const uint32_t e = t - (axis >= 1 ? g.nu : 0u) - (axis == 2 ? g.nv : 0u);
+        int i, j, k;
+        face_ijk(g, axis, e, i, j, k);
+        fcode[t] = face_code(g, axis, i, j, k);
+    }
+}
+
+void launch_face_codes(hipStream_t s, const Geo &g, uint16_t *fcode)
+{
+    const uint64_t n = (uint64_t)g.nu + g.nv + g.nw;
+    FFR_LAUNCH(k_face_codes, dim3(grid_for(n)), dim3(kBlock), 0, s, g, fcode);
+}
+
+// the masks of a face from its code
+template <typename R>
+__device__ __forceinline__ R apply_mask(const Geo &g, uint16_t code, int axis, R val)
+{
+    if (code & FC_FAN) return fan_v(g.fans[code >> 8], axis, R(0));
+    if (code & FC_ZERO) return R(0);
     return val;
 }
 
 // ---- K1: predictor -------------------------------------------------------
-template <typename R>
+template <typename R, bool UNI>
 __device__ __forceinline__ R predict_u(const Geo &g, const StepParams &sp, const Axes<R> &A,
                                        const R *u, const R *v, const R *w, int i, int j, int k,
                                        uint32_t e)
@@ -136,8 +214,10 @@ __device__ __forceinline__ R predict_u(const Geo &g, const StepParams &sp, const
     const R c = u[e];
     if (i == 0 || i == nx) return c;  // boundary faces copied (:164-169)
     R nb[6], q[3] = {R(0), R(0), R(0)}, vel[3];
-    const R h[3] = {A.hx[i], A.hy[j], A.hz[k]};
-    const R i2[3] = {A.i2x[i], A.i2y[j], A.i2z[k]};
+    const int ia = UNI ? 0 : i, ja = UNI ? 0 : j, ka = UNI ? 0 : k;  // uniform: one value per axis
+    const R h[3] = {A.hx[ia], A.hy[ja], A.hz[ka]};
+    const R ih[3] = {A.ihx[ia], A.ihy[ja], A.ihz[ka]};
+    const R i2[3] = {A.i2x[ia], A.i2y[ja], A.i2z[ka]};
     nb[0] = i > 1 ? u[e - si] : R(0);
     nb[1] = i < nx - 1 ? u[e + si] : R(0);
     nb[2] = j > 0 ? u[e - sj] : R(0);
@@ -153,10 +233,10 @@ __device__ __forceinline__ R predict_u(const Geo &g, const StepParams &sp, const
         q[1] = u[e - 2 * sj];  // flat index, wraps into row j-1 at j=1 (:139)
         q[2] = u[e - 2];       // wraps at k=1 (:141)
     }
-    return face_update<R>(sp.scheme, qok, nb, q, c, vel, h, i2, (R)sp.dt, (R)sp.nu);
+    return face_update<R>(sp.scheme, qok, nb, q, c, vel, h, ih, i2, (R)sp.dt, (R)sp.nu);
 }
 
-template <typename R>
+template <typename R, bool UNI>
 __device__ __forceinline__ R predict_v(const Geo &g, const StepParams &sp, const Axes<R> &A,
                                        const R *u, const R *v, const R *w, int i, int j, int k,
                                        uint32_t e)
@@ -166,8 +246,10 @@ __device__ __forceinline__ R predict_v(const Geo &g, const StepParams &sp, const
     const R c = v[e];
     if (j == 0 || j == ny) return c;  // (:232-236)
     R nb[6], q[3] = {R(0), R(0), R(0)}, vel[3];
-    const R h[3] = {A.hx[i], A.hy[j], A.hz[k]};
-    const R i2[3] = {A.i2x[i], A.i2y[j], A.i2z[k]};
+    const int ia = UNI ? 0 : i, ja = UNI ? 0 : j, ka = UNI ? 0 : k;  // uniform: one value per axis
+    const R h[3] = {A.hx[ia], A.hy[ja], A.hz[ka]};
+    const R ih[3] = {A.ihx[ia], A.ihy[ja], A.ihz[ka]};
+    const R i2[3] = {A.i2x[ia], A.i2y[ja], A.i2z[ka]};
     vel[0] = i > 0 ? R(0.5) * (u[uidx(g, i, j - 1, k)] + u[uidx(g, i, j, k)]) : R(0);
     nb[0] = i > 0 ? v[e - si] : R(0);
     nb[1] = i < nx - 1 ? v[e + si] : R(0);
@@ -183,10 +265,10 @@ __device__ __forceinline__ R predict_v(const Geo &g, const StepParams &sp, const
         q[1] = v[e - 2 * sj];
         q[2] = v[e - 1];       // adjacent (:211)
     }
-    return face_update<R>(sp.scheme, qok, nb, q, c, vel, h, i2, (R)sp.dt, (R)sp.nu);
+    return face_update<R>(sp.scheme, qok, nb, q, c, vel, h, ih, i2, (R)sp.dt, (R)sp.nu);
 }
 
-template <typename R>
+template <typename R, bool UNI>
 __device__ __forceinline__ R predict_w(const Geo &g, const StepParams &sp, const Axes<R> &A,
                                        const R *u, const R *v, const R *w, const R *T, int i,
                                        int j, int k, uint32_t e)
@@ -196,8 +278,10 @@ __device__ __forceinline__ R predict_w(const Geo &g, const StepParams &sp, const
     const R c = w[e];
     if (k == 0 || k == nz) return c;  // (:299-303)
     R nb[6], q[3] = {R(0), R(0), R(0)}, vel[3];
-    const R h[3] = {A.hx[i], A.hy[j], A.hz[k]};
-    const R i2[3] = {A.i2x[i], A.i2y[j], A.i2z[k]};
+    const int ia = UNI ? 0 : i, ja = UNI ? 0 : j, ka = UNI ? 0 : k;  // uniform: one value per axis
+    const R h[3] = {A.hx[ia], A.hy[ja], A.hz[ka]};
+    const R ih[3] = {A.ihx[ia], A.ihy[ja], A.ihz[ka]};
+    const R i2[3] = {A.i2x[ia], A.i2y[ja], A.i2z[ka]};
     vel[0] = i > 0 ? R(0.5) * (u[uidx(g, i, j, k - 1)] + u[uidx(g, i, j, k)]) : R(0);
     nb[0] = i > 0 ? w[e - si] : R(0);
     nb[1] = i < nx - 1 ? w[e + si] : R(0);
@@ -213,7 +297,7 @@ __device__ __forceinline__ R predict_w(const Geo &g, const StepParams &sp, const
         q[1] = w[e - sj];  // adjacent (:276)
         q[2] = w[e - 2];
     }
-    R val = face_update<R>(sp.scheme, qok, nb, q, c, vel, h, i2, (R)sp.dt, (R)sp.nu);
+    R val = face_update<R>(sp.scheme, qok, nb, q, c, vel, h, ih, i2, (R)sp.dt, (R)sp.nu);
     if (sp.buoyancy) {  // (:373-386), interior z-faces, before the BC masks
         const uint32_t cc = cidx(g, i, j, k);
         const R lo = T[cc - 1], hi = T[cc];
@@ -222,10 +306,14 @@ __device__ __forceinline__ R predict_w(const Geo &g, const StepParams &sp, const
     return val;
 }
 
-template <typename R>
-__global__ void __launch_bounds__(kBlock) k_predict(Geo g, StepParams sp, const R *u, const R *v,
-                                                    const R *w, const R *T, R *us, R *vs, R *ws,
-                                                    StepRec *rec)
+// One thread per cell updates the u, v and w faces of that cell's low side
+// (index (i,j,k) in each face array), so the three arrays' neighbourhoods of
+// one region are fetched once into one cache; threads past the cells take the
+// remaining boundary faces (u at i = nx, v at j = ny, w at k = nz).
+template <typename R, bool UNI>
+__global__ void __launch_bounds__(kBlock) k_predict(Geo g, StepParams sp, const R *u,
+                                                    const R *v, const R *w, const R *T, R *us,
+                                                    R *vs, R *ws, StepRec *rec)
 {
     if (blockIdx.x == 0 && threadIdx.x == 0) {  // start-of-step diagnostics reset
         rec->vmax_bits = 0ull;
@@ -234,27 +322,33 @@ __global__ void __launch_bounds__(kBlock) k_predict(Geo g, StepParams sp, const 
         rec->nonfinite = 0;
     }
     const Axes<R> A = axes_of(g, R(0));
+    const uint16_t *cu = g.fcode, *cv = g.fcode + g.nu, *cw = g.fcode + g.nu + g.nv;
+    const uint32_t nxu = (uint32_t)g.ny * g.nz, nyv = (uint32_t)g.nx * g.nz;
+    const uint32_t total = g.nc + nxu + nyv + (uint32_t)g.nx * g.ny;
     uint32_t b, end;
-    block_span(g.nu + g.nv + g.nw, xcd_block(), b, end);
-    for (uint32_t e0 = b + threadIdx.x; e0 < end; e0 += kBlock) {
-        if (e0 < g.nu) {
-            const uint32_t e = e0;
-            const uint32_t i = fdiv(e, g.f_nynz), rem = e - i * (g.ny * g.nz);
+    block_span(total, xcd_block(), b, end);
+    for (uint32_t t = b + threadIdx.x; t < end; t += kBlock) {
+        if (t < g.nc) {
+            const uint32_t i = fdiv(t, g.f_nynz), rem = t - i * nxu;
             const uint32_t j = fdiv(rem, g.f_nz), k = rem - j * g.nz;
-            R val = predict_u<R>(g, sp, A, u, v, w, i, j, k, e);
-            us[e] = mask_face<R>(g, 0, i, j, k, val);
-        } else if (e0 < g.nu + g.nv) {
-            const uint32_t e = e0 - g.nu;
-            const uint32_t i = fdiv(e, g.f_ny1nz), rem = e - i * ((g.ny + 1) * g.nz);
-            const uint32_t j = fdiv(rem, g.f_nz), k = rem - j * g.nz;
-            R val = predict_v<R>(g, sp, A, u, v, w, i, j, k, e);
-            vs[e] = mask_face<R>(g, 1, i, j, k, val);
-        } else {
-            const uint32_t e = e0 - g.nu - g.nv;
-            const uint32_t i = fdiv(e, g.f_nynz1), rem = e - i * (g.ny * (g.nz + 1));
-            const uint32_t j = fdiv(rem, g.f_nz1), k = rem - j * (g.nz + 1);
-            R val = predict_w<R>(g, sp, A, u, v, w, T, i, j, k, e);
-            ws[e] = mask_face<R>(g, 2, i, j, k, val);
+            const uint32_t eu = t, ev = vidx(g, i, j, k), ew = widx(g, i, j, k);
+            const R vu = predict_u<R, UNI>(g, sp, A, u, v, w, i, j, k, eu);
+            const R vv = predict_v<R, UNI>(g, sp, A, u, v, w, i, j, k, ev);
+            const R vw = predict_w<R, UNI>(g, sp, A, u, v, w, T, i, j, k, ew);
+            us[eu] = apply_mask<R>(g, cu[eu], 0, vu);
+            vs[ev] = apply_mask<R>(g, cv[ev], 1, vv);
+            ws[ew] = apply_mask<R>(g, cw[ew], 2, vw);
+        } else if (t < g.nc + nxu) {  // u at i = nx: copied, then masked
+            const uint32_t e = g.nc + (t - g.nc);
+            us[e] = apply_mask<R>(g, cu[e], 0, u[e]);
+        } else if (t < g.nc + nxu + nyv) {  // v at j = ny
+            const uint32_t q = t - g.nc - nxu, i = fdiv(q, g.f_nz), k = q - i * g.nz;
+            const uint32_t e = vidx(g, i, g.ny, k);
+            vs[e] = apply_mask<R>(g, cv[e], 1, v[e]);
+        } else {  // w at k = nz
+            const uint32_t q = t - g.nc - nxu - nyv, i = q / (uint32_t)g.ny, j = q - i * g.ny;
+            const uint32_t e = widx(g, i, j, g.nz);
+            ws[e] = apply_mask<R>(g, cw[e], 2, w[e]);
         }
     }
 }
@@ -263,9 +357,14 @@ template <typename R>
 void launch_predict(hipStream_t s, const Geo &g, const StepParams &sp, const R *u, const R *v,
                     const R *w, const R *T, R *us, R *vs, R *ws, StepRec *rec)
 {
-    const uint64_t n = (uint64_t)g.nu + g.nv + g.nw;
-    FFR_LAUNCH(k_predict<R>, dim3(grid_for(n)), dim3(kBlock), 0, s, g, sp, u, v, w, T,
-                       us, vs, ws, rec);
+    const uint64_t n = (uint64_t)g.nc + (uint64_t)g.ny * g.nz + (uint64_t)g.nx * g.nz +
+                       (uint64_t)g.nx * g.ny;
+    if (g.uniform)
+        FFR_LAUNCH((k_predict<R, true>), dim3(grid_for(n)), dim3(kBlock), 0, s, g, sp, u, v, w, T,
+                   us, vs, ws, rec);
+    else
+        FFR_LAUNCH((k_predict<R, false>), dim3(grid_for(n)), dim3(kBlock), 0, s, g, sp, u, v, w,
+                   T, us, vs, ws, rec);
 }
 
 // ---- K4: Poisson right-hand side ----------------------------------------
@@ -286,7 +385,8 @@ __global__ void __launch_bounds__(kBlock) k_rhs(Geo g, StepParams sp, const R *u
         const uint32_t ev = vidx(g, i, j, k), ew = widx(g, i, j, k);
         const double dv = (double)vs[ev + g.nz] - (double)vs[ev];
         const double dw = (double)ws[ew + 1] - (double)ws[ew];
-        const double div = du / g.hx[i] + dv / g.hy[j] + dw / g.hz[k];
+        const double div = by_h64<R>(du, g.hx[i], g.ihx[i]) + by_h64<R>(dv, g.hy[j], g.ihy[j]) +
+                           by_h64<R>(dw, g.hz[k], g.ihz[k]);
         b[c] = sp.rhs_factor * div;
     }
 }
@@ -299,82 +399,67 @@ void launch_rhs(hipStream_t s, const Geo &g, const StepParams &sp, const R *us, 
 }
 
 // ---- K6: projection + post-projection masks + velocity change ----------
+template <typename R>
 __device__ __forceinline__ double pgrad(const Geo &g, const double *p, int axis, int i, int j,
                                         int k)
 {
     const uint32_t c = cidx(g, i, j, k);
-    if (axis == 0) return (p[c] - p[c - g.ny * g.nz]) / g.cx[i];
-    if (axis == 1) return (p[c] - p[c - g.nz]) / g.cy[j];
-    return (p[c] - p[c - 1]) / g.cz[k];
+    if (axis == 0) return by_h64<R>(p[c] - p[c - g.ny * g.nz], g.cx[i], g.icx[i]);
+    if (axis == 1) return by_h64<R>(p[c] - p[c - g.nz], g.cy[j], g.icy[j]);
+    return by_h64<R>(p[c] - p[c - 1], g.cz[k], g.icz[k]);
 }
 
 // Value of a face after correction, no-slip and fans (before outflow copy).
 template <typename R>
 __device__ __forceinline__ R corrected(const Geo &g, const StepParams &sp, const R *fs,
-                                       const double *p, int axis, int i, int j, int k, uint32_t e)
+                                       const double *p, int axis, int i, int j, int k, uint32_t e,
+                                       uint16_t code)
 {
     R x = fs[e];
-    const int n_ax = axis == 0 ? g.nx : (axis == 1 ? g.ny : g.nz);
-    const int pos = axis == 0 ? i : (axis == 1 ? j : k);
-    if (pos >= 1 && pos <= n_ax - 1) {
-        const double gp = pgrad(g, p, axis, i, j, k);
+    if (!(code & FC_BOUND)) {
+        const double gp = pgrad<R>(g, p, axis, i, j, k);
         x = x - (R)(sp.dt_rho * gp);
     }
-    return mask_face<R>(g, axis, i, j, k, x);
+    return apply_mask<R>(g, code, axis, x);
 }
 
 // fp64: x -= dt_rho*gp exactly as the reference (:345); the generic version
 // rounds dt_rho*gp to R first, which for R = double is the same operation.
 
+// One block section per face array (u, v, w), one face per thread.
 template <typename R>
-__global__ void __launch_bounds__(kBlock) k_correct(Geo g, StepParams sp, const R *us,
-                                                    const R *vs, const R *ws, const double *p,
-                                                    R *u, R *v, R *w, StepRec *rec)
+__global__ void __launch_bounds__(kBlock) k_correct(Geo g, StepParams sp, FaceSections fsec,
+                                                    const R *us, const R *vs, const R *ws,
+                                                    const double *p, R *u, R *v, R *w,
+                                                    StepRec *rec)
 {
     __shared__ double lds[kBlock / kWave];
     double dv = 0.0;
     bool bad = false;
-    uint32_t sb, se;
-    block_span(g.nu + g.nv + g.nw, xcd_block(), sb, se);
-    for (uint32_t e0 = sb + threadIdx.x; e0 < se; e0 += kBlock) {
-        R val, old;
-        if (e0 < g.nu) {
-            const uint32_t e = e0;
-            const uint32_t i = fdiv(e, g.f_nynz), rem = e - i * (g.ny * g.nz);
-            const uint32_t j = fdiv(rem, g.f_nz), k = rem - j * g.nz;
-            if (i == 0 && g.wall_of[wall_x(g, 0, j, k)])
-                val = corrected<R>(g, sp, us, p, 0, 1, j, k, uidx(g, 1, j, k));
-            else if (i == (uint32_t)g.nx && g.wall_of[wall_x(g, 1, j, k)])
-                val = corrected<R>(g, sp, us, p, 0, g.nx - 1, j, k, uidx(g, g.nx - 1, j, k));
-            else
-                val = corrected<R>(g, sp, us, p, 0, i, j, k, e);
-            old = u[e];
-            u[e] = val;
-        } else if (e0 < g.nu + g.nv) {
-            const uint32_t e = e0 - g.nu;
-            const uint32_t i = fdiv(e, g.f_ny1nz), rem = e - i * ((g.ny + 1) * g.nz);
-            const uint32_t j = fdiv(rem, g.f_nz), k = rem - j * g.nz;
-            if (j == 0 && g.wall_of[wall_y(g, 0, i, k)])
-                val = corrected<R>(g, sp, vs, p, 1, i, 1, k, vidx(g, i, 1, k));
-            else if (j == (uint32_t)g.ny && g.wall_of[wall_y(g, 1, i, k)])
-                val = corrected<R>(g, sp, vs, p, 1, i, g.ny - 1, k, vidx(g, i, g.ny - 1, k));
-            else
-                val = corrected<R>(g, sp, vs, p, 1, i, j, k, e);
-            old = v[e];
-            v[e] = val;
+    uint32_t lb, G, sb, se;
+    const int axis = face_section(fsec, xcd_block(), lb, G);  // uniform per block
+    const uint32_t n = axis == 0 ? g.nu : (axis == 1 ? g.nv : g.nw);
+    const uint16_t *code = g.fcode + (axis >= 1 ? g.nu : 0u) + (axis == 2 ? g.nv : 0u);
+    const R *fs = axis == 0 ? us : (axis == 1 ? vs : ws);
+    R *out = axis == 0 ? u : (axis == 1 ? v : w);
+    // index step to the next face along the axis
+    const uint32_t step = axis == 0 ? (uint32_t)g.ny * g.nz : (axis == 1 ? (uint32_t)g.nz : 1u);
+    block_span_of(n, lb, G, sb, se);
+    for (uint32_t e = sb + threadIdx.x; e < se; e += kBlock) {
+        const uint16_t c = code[e];
+        int i, j, k;
+        face_ijk(g, axis, e, i, j, k);
+        R val;
+        if (c & (FC_OUT_LO | FC_OUT_HI)) {  // outflow patch: the face one step inside
+            const int d = (c & FC_OUT_LO) ? 1 : -1;
+            const uint32_t e2 = d > 0 ? e + step : e - step;
+            val = corrected<R>(g, sp, fs, p, axis, i + d * (axis == 0), j + d * (axis == 1),
+                               k + d * (axis == 2), e2, code[e2]);
         } else {
-            const uint32_t e = e0 - g.nu - g.nv;
-            const uint32_t i = fdiv(e, g.f_nynz1), rem = e - i * (g.ny * (g.nz + 1));
-            const uint32_t j = fdiv(rem, g.f_nz1), k = rem - j * (g.nz + 1);
-            if (k == 0 && g.wall_of[wall_z(g, 0, i, j)])
-                val = corrected<R>(g, sp, ws, p, 2, i, j, 1, widx(g, i, j, 1));
-            else if (k == (uint32_t)g.nz && g.wall_of[wall_z(g, 1, i, j)])
-                val = corrected<R>(g, sp, ws, p, 2, i, j, g.nz - 1, widx(g, i, j, g.nz - 1));
-            else
-                val = corrected<R>(g, sp, ws, p, 2, i, j, k, e);
-            old = w[e];
-            w[e] = val;
+            val = corrected<R>(g, sp, fs, p, axis, i, j, k, e, c);
         }
+        const R old = out[e];
+        out[e] = val;
         dv = fmax(dv, fabs((double)val - (double)old));  // fmax drops NaN as std::max does
         bad |= !isfinite((double)val);
     }
@@ -388,8 +473,9 @@ void launch_correct(hipStream_t s, const Geo &g, const StepParams &sp, const R *
                     const R *ws, const double *p, R *u, R *v, R *w, StepRec *rec)
 {
     const uint64_t n = (uint64_t)g.nu + g.nv + g.nw;
-    FFR_LAUNCH(k_correct<R>, dim3(grid_for(n)), dim3(kBlock), 0, s, g, sp, us, vs, ws, p,
-                       u, v, w, rec);
+    const uint32_t G = std::max(3u, grid_for(n));
+    const FaceSections fs = face_sections(g.nu, g.nv, g.nw, G);
+    FFR_LAUNCH(k_correct<R>, dim3(G), dim3(kBlock), 0, s, g, sp, fs, us, vs, ws, p, u, v, w, rec);
 }
 
 // ---- divergence (FluidSolver.cpp:470-484) --------------------------------
@@ -398,9 +484,9 @@ __device__ __forceinline__ double cell_div(const Geo &g, const R *u, const R *v,
                                            int i, int j, int k, uint32_t c)
 {
     const uint32_t ev = vidx(g, i, j, k), ew = widx(g, i, j, k);
-    return fabs(((double)u[c + g.ny * g.nz] - (double)u[c]) / g.hx[i] +
-                ((double)v[ev + g.nz] - (double)v[ev]) / g.hy[j] +
-                ((double)w[ew + 1] - (double)w[ew]) / g.hz[k]);
+    return fabs(by_h64<R>((double)u[c + g.ny * g.nz] - (double)u[c], g.hx[i], g.ihx[i]) +
+                by_h64<R>((double)v[ev + g.nz] - (double)v[ev], g.hy[j], g.ihy[j]) +
+                by_h64<R>((double)w[ew + 1] - (double)w[ew], g.hz[k], g.ihz[k]));
 }
 
 template <typename R>
@@ -428,7 +514,7 @@ void launch_divmax(hipStream_t s, const Geo &g, const R *u, const R *v, const R 
 }
 
 // ---- K8: temperature transport (FluidSolver.cpp:388-456) ---------------
-template <typename R>
+template <typename R, bool UNI>
 __global__ void __launch_bounds__(kBlock) k_temperature(Geo g, StepParams sp, const R *u,
                                                         const R *v, const R *w, const R *T,
                                                         R *Tn, StepRec *rec, RedScratch red)
@@ -448,7 +534,10 @@ __global__ void __launch_bounds__(kBlock) k_temperature(Geo g, StepParams sp, co
         }
         const uint32_t i = fdiv(c, g.f_nynz), rem = c - i * (ny * nz);
         const uint32_t j = fdiv(rem, g.f_nz), k = rem - j * nz;
-        const R hx = A.hx[i], hy = A.hy[j], hz = A.hz[k];
+        const int ia = UNI ? 0 : i, ja = UNI ? 0 : j, ka = UNI ? 0 : k;
+        const R hx = A.hx[ia], hy = A.hy[ja], hz = A.hz[ka];
+        const R ihx = A.ihx[ia], ihy = A.ihy[ja], ihz = A.ihz[ka];
+        const R i2x = A.i2x[ia], i2y = A.i2y[ja], i2z = A.i2z[ka];
         const R Tc = T[c];
         const uint32_t ev = vidx(g, i, j, k), ew = widx(g, i, j, k);
         const R u0 = u[c], u1 = u[c + ny * nz], v0 = v[ev], v1 = v[ev + nz], w0 = w[ew],
@@ -469,25 +558,25 @@ __global__ void __launch_bounds__(kBlock) k_temperature(Geo g, StepParams sp, co
         if (sp.scheme == QUICK && ii >= 1 && ii <= nx - 2 && jj >= 1 && jj <= ny - 2 && kk >= 1 &&
             kk <= nz - 2) {
             const R xmm = tn(ii - 2, jj, kk), ymm = tn(ii, jj - 2, kk), zmm = tn(ii, jj, kk - 2);
-            adv = adv_quick(xmm, xm, Tc, xp, uc, hx) + adv_quick(ymm, ym, Tc, yp, vc, hy) +
-                  adv_quick(zmm, zm, Tc, zp, wc, hz);
+            adv = adv_quick(xmm, xm, Tc, xp, uc, hx, ihx) + adv_quick(ymm, ym, Tc, yp, vc, hy, ihy) +
+                  adv_quick(zmm, zm, Tc, zp, wc, hz, ihz);
         } else if (sp.scheme == LAX_WENDROFF) {
-            adv = adv_lw(xm, Tc, xp, uc, dt, hx) + adv_lw(ym, Tc, yp, vc, dt, hy) +
-                  adv_lw(zm, Tc, zp, wc, dt, hz);
+            adv = adv_lw(xm, Tc, xp, uc, dt, hx, ihx) + adv_lw(ym, Tc, yp, vc, dt, hy, ihy) +
+                  adv_lw(zm, Tc, zp, wc, dt, hz, ihz);
         } else {
-            adv = adv_upwind(xm, Tc, xp, uc, hx) + adv_upwind(ym, Tc, yp, vc, hy) +
-                  adv_upwind(zm, Tc, zp, wc, hz);
+            adv = adv_upwind(xm, Tc, xp, uc, hx, ihx) + adv_upwind(ym, Tc, yp, vc, hy, ihy) +
+                  adv_upwind(zm, Tc, zp, wc, hz, ihz);
         }
         const R al = (R)sp.alpha_T;
-        const R dif = al * ((xm - R(2) * Tc + xp) / (hx * hx) + (ym - R(2) * Tc + yp) / (hy * hy) +
-                            (zm - R(2) * Tc + zp) / (hz * hz));
+        const R dif = al * (by_h2(xm - R(2) * Tc + xp, hx, i2x) + by_h2(ym - R(2) * Tc + yp, hy, i2y) +
+                            by_h2(zm - R(2) * Tc + zp, hz, i2z));
         const R tnew = Tc + dt * (-adv + dif);
         Tn[c] = tnew;
         tsum += (double)tnew;
         bad |= !isfinite((double)tnew);
-        dmax = fmax(dmax, fabs(((double)u1 - (double)u0) / g.hx[i] +
-                               ((double)v1 - (double)v0) / g.hy[j] +
-                               ((double)w1 - (double)w0) / g.hz[k]));
+        dmax = fmax(dmax, fabs(by_h64<R>((double)u1 - (double)u0, g.hx[ia], g.ihx[ia]) +
+                               by_h64<R>((double)v1 - (double)v0, g.hy[ja], g.ihy[ja]) +
+                               by_h64<R>((double)w1 - (double)w0, g.hz[ka], g.ihz[ka])));
     }
     if (bad) rec->nonfinite = 1;
     const double m = block_max(dmax, lds);
@@ -499,8 +588,12 @@ template <typename R>
 void launch_temperature(hipStream_t s, const Geo &g, const StepParams &sp, const R *u,
                         const R *v, const R *w, const R *T, R *Tn, StepRec *rec, RedScratch red)
 {
-    FFR_LAUNCH(k_temperature<R>, dim3(grid_for(g.nc)), dim3(kBlock), 0, s, g, sp, u, v,
-                       w, T, Tn, rec, red);
+    if (g.uniform)
+        FFR_LAUNCH((k_temperature<R, true>), dim3(grid_for(g.nc)), dim3(kBlock), 0, s, g, sp, u,
+                   v, w, T, Tn, rec, red);
+    else
+        FFR_LAUNCH((k_temperature<R, false>), dim3(grid_for(g.nc)), dim3(kBlock), 0, s, g, sp, u,
+                   v, w, T, Tn, rec, red);
 }
 
 // Sum of T over FLUID cells (used when an opening overwrites FLUID cells).
