@@ -101,14 +101,21 @@ def roofline(table: dict, d, workload: str):
                   "launches_per_step": None}
 
 
-def form_params_inplace(p, mode):
+# residual tolerance of the plain FP64 CG form per workload, chosen so that
+# its pressure error (measured by pressure_errors) meets 1e-6: the error is
+# about the condition number times the residual, which grows with the grid
+FP64_TOL = {"c2": 1e-9, "c3": 1e-12}
+
+
+def form_params_inplace(p, mode, workload="c2"):
     """The two forms compared: the reference algorithm (unpreconditioned FP64
-    CG, run to 1e-9 so its pressure error meets 1e-6) and the mixed form (FP32
-    stencils, FP32/BF16 MG-PCG inner solves, FP64 outer defect correction)."""
+    CG, run tightly enough that its pressure error meets 1e-6) and the mixed
+    form (FP32 stencils, FP32/BF16 MG-PCG inner solves, FP64 outer defect
+    correction)."""
     from ff_room import capi
     if mode == "fp64":
         p.precision, p.poisson_method = capi.PREC_FP64, capi.POISSON_CG
-        p.poisson_tol, p.poisson_max_iter = 1e-9, 50000
+        p.poisson_tol, p.poisson_max_iter = FP64_TOL.get(workload, 1e-9), 200000
     else:
         p.precision, p.poisson_method = capi.PREC_MIXED, capi.POISSON_MGPCG
         p.poisson_tol, p.poisson_max_iter = 1e-10, 50
@@ -206,6 +213,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-compare", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--graphs", type=int, default=1,
+                    help="replay inner iterations from HIP graphs (0 for tracing profilers)")
     ap.add_argument("--c4", type=int, default=64, help="rooms in the batched sample")
     ap.add_argument("--c4-steps", type=int, default=20)
     ap.add_argument("--c5", type=int, default=40, help="requests in the stream sample")
@@ -225,7 +234,8 @@ def main():
     ps = device.prepare(cfg)
     log(f"device {info['name']} {info['arch']} CUs={info['compute_units']}; workload {desc}")
 
-    form_params_inplace(ps.params, args.mode)
+    form_params_inplace(ps.params, args.mode, args.workload)
+    ps.params.graphs = args.graphs
     if args.kernel_only:
         # the named kernel, then the library's plain FP64 copy of nx*ny*nz
         # doubles, whose memory counters calibrate the kernel's
@@ -276,7 +286,7 @@ def main():
     # ---- the plain FP64 CG form on the same workload (comparison) -----------
     if args.mode == "mixed" and not args.no_compare:
         ps64 = device.prepare(cfg)
-        form_params_inplace(ps64.params, "fp64")
+        form_params_inplace(ps64.params, "fp64", args.workload)
         out["p_error"] = pressure_errors(ctx, ps, ps64, d_keep)
         log(f"pressure error vs x*: mixed {out['p_error']['mixed']['rel_l2']:.2e}, "
             f"fp64 CG {out['p_error']['fp64_cg']['rel_l2']:.2e}")

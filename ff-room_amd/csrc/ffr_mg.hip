@@ -213,13 +213,16 @@ static const char *kRestrictName[kMaxLevels] = {
     "mg_restrict_l0", "mg_restrict_l1", "mg_restrict_l2", "mg_restrict_l3", "mg_restrict_l4",
     "mg_restrict_l5", "mg_restrict_l6", "mg_restrict_l7", "mg_restrict_l8", "mg_restrict_l9",
     "mg_restrict_l10", "mg_restrict_l11"};
-static const char *kProlongName[kMaxLevels] = {
-    "mg_prolong_l0", "mg_prolong_l1", "mg_prolong_l2", "mg_prolong_l3", "mg_prolong_l4",
-    "mg_prolong_l5", "mg_prolong_l6", "mg_prolong_l7", "mg_prolong_l8", "mg_prolong_l9",
-    "mg_prolong_l10", "mg_prolong_l11"};
+static const char *kPostName[kMaxLevels] = {
+    "mg_post_l0", "mg_post_l1", "mg_post_l2", "mg_post_l3", "mg_post_l4", "mg_post_l5",
+    "mg_post_l6", "mg_post_l7", "mg_post_l8", "mg_post_l9", "mg_post_l10", "mg_post_l11"};
 
+// V-cycle from level l. first_red_done: the level's first pass (the red
+// sweep from z = 0) was done by the kernel that produced its r (the PCG
+// update or the FP64 residual on level 0, the restriction on the coarse
+// levels). The prolongation is fused into the first post-smoothing sweep.
 template <typename TC>
-static void vcycle(ffr_scene *s, int l)
+static void vcycle(ffr_scene *s, int l, bool first_red_done)
 {
     Multigrid &M = s->mg;
     const Geo &g = s->geo;
@@ -250,55 +253,50 @@ static void vcycle(ffr_scene *s, int l)
         else mg_launch_coarse<TC>(st, g, L, false, M.coarse_sweeps, m);
         return;
     }
+    const MgLevel &C = M.lv[l + 1];
     // algorithmic bytes of one colour pass: z and r read, the code (level 0,
     // 2 B) or operator (coarse: active + 3 couplings + rdiag), z written
     // (level 0, split layout: r, code and z of the colour, z of the other colour)
     const double bsm = l0 ? n * (4 + 4 + 2 + 4) / 2 : n * (2 * sizeof(TC) + 1 + 16 + sizeof(TC));
-    const double bres = l0 ? n * (4 + 4 + 2) + M.lv[l + 1].n * (double)sizeof(TC)
-                           : n * (2 * sizeof(TC) + 1 + 16) + M.lv[l + 1].n * (double)sizeof(TC);
-    const double bpro = n * (2.0 * (l0 ? 4 : sizeof(TC)) + (l0 ? 2 : 1)) + M.lv[l + 1].n * (double)sizeof(TC);
+    const double bres = l0 ? n * (4 + 4 + 2) + C.n * (double)sizeof(TC)
+                           : n * (2 * sizeof(TC) + 1 + 16) + C.n * (double)sizeof(TC);
+    const double bpost = bsm + C.n * (double)sizeof(TC);
     const int sweeps = l0 ? M.pre_sweeps : M.level_sweeps;
+    auto smooth = [&](int color, int zero_start) {
+        ProfScope ps(s->ctx, st, kSmoothName[l], bsm);
+        if (l0) mg_launch_smooth<float>(st, g, L, true, color, zero_start, m);
+        else mg_launch_smooth<TC>(st, g, L, false, color, zero_start, m);
+    };
     for (int sw = 0; sw < sweeps; sw++) {
-        {
-            ProfScope ps(s->ctx, st, kSmoothName[l], bsm);
-            if (l0) mg_launch_smooth<float>(st, g, L, true, 0, sw == 0, m);
-            else mg_launch_smooth<TC>(st, g, L, false, 0, sw == 0, m);
-        }
-        {
-            ProfScope ps(s->ctx, st, kSmoothName[l], bsm);
-            if (l0) mg_launch_smooth<float>(st, g, L, true, 1, 0, m);
-            else mg_launch_smooth<TC>(st, g, L, false, 1, 0, m);
-        }
+        if (!(sw == 0 && first_red_done)) smooth(0, sw == 0);
+        smooth(1, 0);
     }
+    // the next level's first sweep rides on the restriction unless that level
+    // is the single-workgroup tail or the coarsest (both start from z = 0)
+    const bool red0 = l + 1 != M.tail_first && l + 1 != M.levels - 1;
     {
         ProfScope ps(s->ctx, st, kRestrictName[l], bres);
-        if (l0) mg_launch_restrict<float, TC>(st, g, L, M.lv[l + 1], true, m);
-        else mg_launch_restrict<TC, TC>(st, g, L, M.lv[l + 1], false, m);
+        if (l0) mg_launch_restrict<float, TC>(st, g, L, C, true, red0, m);
+        else mg_launch_restrict<TC, TC>(st, g, L, C, false, red0, m);
     }
-    vcycle<TC>(s, l + 1);
+    vcycle<TC>(s, l + 1, red0);
     {
-        ProfScope ps(s->ctx, st, kProlongName[l], bpro);
-        if (l0) mg_launch_prolong<float, TC>(st, g, L, M.lv[l + 1], true, m);
-        else mg_launch_prolong<TC, TC>(st, g, L, M.lv[l + 1], false, m);
+        ProfScope ps(s->ctx, st, kPostName[l], bpost);
+        if (l0) mg_launch_post_black<float, TC>(st, g, L, C, true, m);
+        else mg_launch_post_black<TC, TC>(st, g, L, C, false, m);
     }
-    for (int sw = 0; sw < sweeps; sw++) {
-        {
-            ProfScope ps(s->ctx, st, kSmoothName[l], bsm);
-            if (l0) mg_launch_smooth<float>(st, g, L, true, 1, 0, m);
-            else mg_launch_smooth<TC>(st, g, L, false, 1, 0, m);
-        }
-        {
-            ProfScope ps(s->ctx, st, kSmoothName[l], bsm);
-            if (l0) mg_launch_smooth<float>(st, g, L, true, 0, 0, m);
-            else mg_launch_smooth<TC>(st, g, L, false, 0, 0, m);
-        }
+    smooth(0, 0);
+    for (int sw = 1; sw < sweeps; sw++) {
+        smooth(1, 0);
+        smooth(0, 0);
     }
 }
 
 static void precondition(ffr_scene *s)
 {
-    if (s->mg.levels > 1 && s->mg.lv[1].bf16) vcycle<__hip_bfloat16>(s, 0);
-    else vcycle<float>(s, 0);
+    // level 0's first red sweep is done by the PCG update / FP64 residual
+    if (s->mg.levels > 1 && s->mg.lv[1].bf16) vcycle<__hip_bfloat16>(s, 0, true);
+    else vcycle<float>(s, 0, true);
 }
 
 static void inner_iteration(ffr_scene *s)
@@ -313,7 +311,7 @@ static void inner_iteration(ffr_scene *s)
     }
     {
         ProfScope ps(s->ctx, st, "pcg_update", n * 24);
-        pcg_launch_update(st, g.nc, M.e, M.ri, M.pi, M.qi, s->mgs, s->red);
+        pcg_launch_update(st, g, M.lv[0], M.e, M.ri, M.pi, M.qi, s->mgs, s->red);
     }
     pcg_launch_check(st, g.nc, s->mgs, s->red);
     precondition(s);
@@ -401,7 +399,8 @@ static void enqueue_pass(ffr_scene *s)
     const double n = (double)g.nc;
     {
         ProfScope ps(s->ctx, st, "dc_residual", n * (8 + 8 + 4 + 1 + 8 + 4));
-        dc_launch_residual(st, g, s->b, s->p, s->d, s->mg.e, s->mg.ri, s->mgs, s->red);
+        dc_launch_residual(st, g, s->b, s->p, s->d, s->mg.e, s->mg.ri, (float *)s->mg.lv[0].z,
+                           s->mgs, s->red);
     }
     dc_launch_check(st, g, s->mgs, s->red);
     s->task.passes_enq++;
@@ -516,8 +515,8 @@ double mg_prof_launch(ffr_scene *s, const char *name, std::function<void()> *lau
         const MgLevel L1 = M.lv[1];
         const bool bf = L1.bf16;
         const double nc1 = (double)L1.n * (bf ? 2 : 4);
-        if (bf) *launch = [=] { mg_launch_restrict<float, __hip_bfloat16>(st, g, L0, L1, true, m); };
-        else *launch = [=] { mg_launch_restrict<float, float>(st, g, L0, L1, true, m); };
+        if (bf) *launch = [=] { mg_launch_restrict<float, __hip_bfloat16>(st, g, L0, L1, true, false, m); };
+        else *launch = [=] { mg_launch_restrict<float, float>(st, g, L0, L1, true, false, m); };
         return n * (4 + 4 + 2) + nc1;
     }
     if (k == "pcg_spmv") {
@@ -529,14 +528,15 @@ double mg_prof_launch(ffr_scene *s, const char *name, std::function<void()> *lau
     if (k == "pcg_update") {
         float *e = M.e, *r = M.ri, *p = M.pi, *q = M.qi;
         RedScratch red = s->red;
-        *launch = [=] { pcg_launch_update(st, g.nc, e, r, p, q, m, red); };
+        *launch = [=] { pcg_launch_update(st, g, L0, e, r, p, q, m, red); };
         return n * 24;
     }
     if (k == "dc_residual") {
         double *b = s->b, *x0 = s->p, *x1 = s->d;
         float *e = M.e, *r = M.ri;
         RedScratch red = s->red;
-        *launch = [=] { dc_launch_residual(st, g, b, x0, x1, e, r, m, red); };
+        float *z0 = (float *)L0.z;
+        *launch = [=] { dc_launch_residual(st, g, b, x0, x1, e, r, z0, m, red); };
         return n * (8 + 8 + 4 + 2 + 8 + 4);
     }
     *launch = [] {};

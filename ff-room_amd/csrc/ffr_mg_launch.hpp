@@ -11,10 +11,11 @@ void mg_launch_smooth(hipStream_t s, const Geo &g, const MgLevel &L, bool l0, in
                       int zero_start, const MgState *m);
 template <typename TF, typename TC>
 void mg_launch_restrict(hipStream_t s, const Geo &g, const MgLevel &F, const MgLevel &C, bool l0,
-                        const MgState *m);
+                        bool red0, const MgState *m);
+// prolongation fused with the first post-smoothing (black) sweep on F
 template <typename TF, typename TC>
-void mg_launch_prolong(hipStream_t s, const Geo &g, const MgLevel &F, const MgLevel &C, bool l0,
-                       const MgState *m);
+void mg_launch_post_black(hipStream_t s, const Geo &g, const MgLevel &F, const MgLevel &C, bool l0,
+                          const MgState *m);
 template <typename T>
 void mg_launch_tail(hipStream_t s, const TailLevels &tl, size_t lds_bytes, const MgState *m);
 template <typename T>
@@ -22,8 +23,8 @@ void mg_launch_coarse(hipStream_t s, const Geo &g, const MgLevel &L, bool l0, in
                       const MgState *m);
 void pcg_launch_spmv(hipStream_t s, const Geo &g, const MgLevel &L0, const float *p, float *q,
                      const MgState *m, RedScratch red);
-void pcg_launch_update(hipStream_t s, uint32_t n, float *e, float *r, const float *p,
-                       const float *q, MgState *m, RedScratch red);
+void pcg_launch_update(hipStream_t s, const Geo &g, const MgLevel &L0, float *e, float *r,
+                       const float *p, const float *q, MgState *m, RedScratch red);
 void pcg_launch_check(hipStream_t s, uint32_t n, MgState *m, RedScratch red);
 void pcg_launch_dot_rz(hipStream_t s, uint32_t n, const float *r, const float *z, const MgState *m,
                        RedScratch red);
@@ -35,7 +36,7 @@ void dc_launch_init(hipStream_t s, const Geo &g, const double *b, double *x, flo
 void dc_launch_start(hipStream_t s, const Geo &g, MgState *m, RedScratch red, double outer_tol,
                      int max_outer, int max_inner, double inner_tol, int total_cap);
 void dc_launch_residual(hipStream_t s, const Geo &g, const double *b, double *xa, double *xb,
-                        const float *e, float *r32, const MgState *m, RedScratch red);
+                        const float *e, float *r32, float *z0, const MgState *m, RedScratch red);
 void dc_launch_check(hipStream_t s, const Geo &g, MgState *m, RedScratch red);
 void dc_launch_finalize(hipStream_t s, const Geo &g, const double *b, const double *xa,
                         const double *xb, double *p, int pin_outflow, const MgState *m);

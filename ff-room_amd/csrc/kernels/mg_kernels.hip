@@ -34,6 +34,13 @@ __device__ __forceinline__ void bstv(__amdgpu_buffer_rsrc_t r, uint32_t i, float
                                           (int)(i * 2u), 0, 0);
 }
 
+// x rounded to the storage type T and back (what a store and a load give)
+__device__ __forceinline__ float rnd(float x, const float *) { return x; }
+__device__ __forceinline__ float rnd(float x, const __hip_bfloat16 *)
+{
+    return __bfloat162float(__float2bfloat16(x));
+}
+
 __device__ __forceinline__ bool unknown0(uint8_t f) { return f == FLUID || f == INFLOW; }
 __device__ __forceinline__ double *pslot(RedScratch red, int k) { return red.partials + k * kMaxBlocksCap; }
 
@@ -383,9 +390,21 @@ __global__ void __launch_bounds__(kBlock) k_mg_smooth(MgLevel L, T *__restrict__
 // the residuals of the up-to-8 fine cells of aggregate I (children beyond an
 // odd fine extent give 0) and sum them with an xor-shuffle tree, which every
 // lane evaluates in the same order (deterministic).
+// With red0, the coarse level's first pass (the red sweep from z = 0) is done
+// here as well: zc = rc / diag on red unknowns, 0 elsewhere (rc rounded to its
+// storage type first, as that pass would read it).
+template <typename TC>
+__device__ __forceinline__ void coarse_first_red(const MgLevel &C, TC *zc, uint32_t I, int ic,
+                                                 int jc, int kc, float v)
+{
+    const bool red = ((ic + jc + kc) & 1) == 0;
+    stv(zc, I, (red && C.active[I]) ? rnd(v, zc) * C.rdiag[I] : 0.f);
+}
+
 template <typename TF, typename TC, bool L0>
 __global__ void __launch_bounds__(kBlock) k_mg_restrict(Geo g, MgLevel F, MgLevel C, const TF *z,
-                                                        const TF *r, TC *rc, const MgState *m)
+                                                        const TF *r, TC *rc, TC *zc, int red0,
+                                                        const MgState *m)
 {
     const uint32_t stop = stop_flags(m);  // tested after the first loads
     uint32_t sb, se;
@@ -399,7 +418,7 @@ __global__ void __launch_bounds__(kBlock) k_mg_restrict(Geo g, MgLevel F, MgLeve
         const auto bred = buf_rsrc(zs, 4u * H), bblack = buf_rsrc(zs + H, 4u * (g.nc - H));
         for (uint32_t t0 = sb + threadIdx.x; t0 < se; t0 += kU * kBlock) {
             uint16_t nb[kU];
-            float rv[kU], zc[kU], zv[kU][6];
+            float rv[kU], zown[kU], zv[kU][6];
             bool in[kU];
 #pragma unroll
             for (int u = 0; u < kU; u++) {
@@ -416,7 +435,7 @@ __global__ void __launch_bounds__(kBlock) k_mg_restrict(Geo g, MgLevel F, MgLeve
                 const uint32_t own = (col ? H : 0u) + (c >> 1);
                 nb[u] = g.nbm[c];
                 rv[u] = rs[own];
-                zc[u] = zs[own];
+                zown[u] = zs[own];
                 nb_gather(col ? bred : bblack, c, si, sj, zv[u]);
             }
             if (late(stop)) return;
@@ -426,11 +445,19 @@ __global__ void __launch_bounds__(kBlock) k_mg_restrict(Geo g, MgLevel F, MgLeve
                 if (t >= se) break;
                 float v = 0.f;
                 if (in[u] && (nb[u] & NB_SELF))
-                    v = rv[u] - (diag_of(g, nb[u], dint) * zc[u] - nb_sum(g, nb[u], zv[u]));
+                    v = rv[u] - (diag_of(g, nb[u], dint) * zown[u] - nb_sum(g, nb[u], zv[u]));
                 v += __shfl_xor(v, 1);
                 v += __shfl_xor(v, 2);
                 v += __shfl_xor(v, 4);
-                if ((t & 7u) == 0) stv(rc, t >> 3, v);
+                if ((t & 7u) == 0) {
+                    const uint32_t I = t >> 3;
+                    stv(rc, I, v);
+                    if (red0) {
+                        int ic, jc, kc;
+                        decode(I, C.f_nynz, C.f_nz, C.ny, C.nz, ic, jc, kc);
+                        coarse_first_red<TC>(C, zc, I, ic, jc, kc, v);
+                    }
+                }
             }
         }
         return;
@@ -456,28 +483,110 @@ __global__ void __launch_bounds__(kBlock) k_mg_restrict(Geo g, MgLevel F, MgLeve
         v += __shfl_xor(v, 1);
         v += __shfl_xor(v, 2);
         v += __shfl_xor(v, 4);
-        if (q == 0) stv(rc, I, v);
+        if (q == 0) {
+            stv(rc, I, v);
+            if (red0) coarse_first_red<TC>(C, zc, I, ic, jc, kc, v);
+        }
     }
 }
 
-// ---- prolongation + correction: z += P zc ------------------------------------
-template <typename TF, typename TC, bool L0>
-__global__ void __launch_bounds__(kBlock) k_mg_prolong(Geo g, MgLevel F, MgLevel C, TF *z,
-                                                       const TC *zc, const MgState *m)
+// ---- prolongation fused into the first post-smoothing sweep ----------------
+// After the coarse correction the first post pass is the black sweep; it reads
+// only red neighbours, so the prolonged red values z_nb + zc[parent(nb)] are
+// formed where they are read (rounded to the level's storage type, as a stored
+// prolongation would give) and never stored: the red sweep that follows
+// overwrites every red value. Black unknowns get (r + sum coupling *
+// (z_nb + zc)) / diag, black non-unknowns 0, red slots are not touched.
+
+// parent of fine cell (i, j, k) on the coarse level C (an out-of-range fine
+// index gives an index whose read returns 0)
+__device__ __forceinline__ uint32_t parent_of(const MgLevel &C, int i, int j, int k)
+{
+    return ((uint32_t)(i >> C.shx) * C.ny + (uint32_t)(j >> C.shy)) * C.nz + (uint32_t)(k >> C.shz);
+}
+
+// level 0 (split layout): one thread per black slot, kU slots per thread
+template <typename TC>
+__global__ void __launch_bounds__(kBlock) k_mg_post_black0(Geo g, MgLevel L, MgLevel C,
+                                                           const TC *__restrict__ zc,
+                                                           const MgState *m)
 {
     const uint32_t stop = stop_flags(m);
+    const uint32_t H = L.H, cnt = g.nc - H;
+    const uint32_t si = (uint32_t)g.ny * g.nz, sj = g.nz;
+    const float dint = diag0m(g, kInterior);
+    const TC *tag = nullptr;
+    float *zb = (float *)L.z + H;
+    const auto bcode = buf_rsrc(L.nbs + H, 2u * cnt), br = buf_rsrc((const float *)L.r + H, 4u * cnt),
+               bred = buf_rsrc(L.z, 4u * H), bzc = buf_rsrc(zc, (uint32_t)sizeof(TC) * C.n),
+               bz = buf_rsrc(zb, 4u * cnt);
     uint32_t sb, se;
-    block_span(F.n, xcd_block(), sb, se);
+    block_span(cnt, xcd_block(), sb, se);
+    for (uint32_t h0 = sb + threadIdx.x; h0 < se; h0 += kU * kBlock) {
+        uint16_t nb[kU];
+        float rv[kU], zv[kU][6];
+#pragma unroll
+        for (int t = 0; t < kU; t++) {
+            const uint32_t h = h0 + t * kBlock;  // reads past cnt return 0
+            nb[t] = buf_ldu16(bcode, h);
+            rv[t] = buf_ldf(br, h);
+            const uint32_t c = 2 * h + split_parity(L, h, 1u);
+            const int i = (int)fdiv(c, g.f_nynz), rem = (int)(c - (uint32_t)i * si);
+            const int j = (int)fdiv((uint32_t)rem, g.f_nz), k = rem - j * (int)sj;
+            nb_gather(bred, c, si, sj, zv[t]);
+            zv[t][0] += bldv(bzc, parent_of(C, i - 1, j, k), tag);
+            zv[t][1] += bldv(bzc, parent_of(C, i + 1, j, k), tag);
+            zv[t][2] += bldv(bzc, parent_of(C, i, j - 1, k), tag);
+            zv[t][3] += bldv(bzc, parent_of(C, i, j + 1, k), tag);
+            zv[t][4] += bldv(bzc, parent_of(C, i, j, k - 1), tag);
+            zv[t][5] += bldv(bzc, parent_of(C, i, j, k + 1), tag);
+        }
+        if (late(stop)) return;
+#pragma unroll
+        for (int t = 0; t < kU; t++) {
+            const uint32_t h = h0 + t * kBlock;
+            if (h >= se) break;
+            float v = 0.f;
+            if (nb[t] & NB_SELF) v = (rv[t] + nb_sum(g, nb[t], zv[t])) / diag_of(g, nb[t], dint);
+            buf_stf(bz, h, v);
+        }
+    }
+}
+
+// coarse levels (natural layout)
+template <typename T, typename TC>
+__global__ void __launch_bounds__(kBlock) k_mg_post_black(MgLevel L, MgLevel C, T *__restrict__ z,
+                                                          const T *__restrict__ r,
+                                                          const TC *__restrict__ zc,
+                                                          const MgState *m)
+{
+    const uint32_t stop = stop_flags(m);
+    const T *tag = nullptr;
+    const TC *ctag = nullptr;
+    const uint32_t si = (uint32_t)L.ny * L.nz, sj = L.nz;
+    const CoarseOp<T> o = coarse_op<T>(L);
+    const auto bz = buf_rsrc(z, (uint32_t)sizeof(T) * L.n), br = buf_rsrc(r, (uint32_t)sizeof(T) * L.n),
+               bzc = buf_rsrc(zc, (uint32_t)sizeof(TC) * C.n);
+    uint32_t sb, se;
+    block_span(L.n, xcd_block(), sb, se);
     for (uint32_t c = sb + threadIdx.x; c < se; c += kBlock) {
         int i, j, k;
-        decode(c, F.f_nynz, F.f_nz, F.ny, F.nz, i, j, k);
-        const uint32_t P = ((uint32_t)(i >> C.shx) * C.ny + (j >> C.shy)) * C.nz + (k >> C.shz);
-        // level 0: split layout, colour (i+j+k)&1
-        const uint32_t zi = L0 ? (((i + j + k) & 1) ? F.H : 0u) + (c >> 1) : c;
-        const bool act = L0 ? (g.nbm[c] & NB_SELF) != 0 : F.active[c] != 0;
-        const float zv = ldv(z, zi), cv = ldv(zc, P);
+        decode(c, L.f_nynz, L.f_nz, L.ny, L.nz, i, j, k);
+        const bool black = ((i + j + k) & 1) == 1;
+        const uint8_t a = buf_ldu8(o.act, c);
+        const float rv = bldv(br, c, tag), rd = buf_ldf(o.rd, c);
+        auto nbv = [&](uint32_t q, int x, int y, int w) {  // prolonged neighbour value
+            return rnd(bldv(bz, q, tag) + bldv(bzc, parent_of(C, x, y, w), ctag), tag);
+        };
+        float of = 0.f;
+        of += buf_ldf(o.cx, c) * nbv(c - si, i - 1, j, k);
+        of += buf_ldf(o.cx, c + si) * nbv(c + si, i + 1, j, k);
+        of += buf_ldf(o.cy, c) * nbv(c - sj, i, j - 1, k);
+        of += buf_ldf(o.cy, c + sj) * nbv(c + sj, i, j + 1, k);
+        of += buf_ldf(o.cz, c) * nbv(c - 1u, i, j, k - 1);
+        of += buf_ldf(o.cz, c + 1u) * nbv(c + 1u, i, j, k + 1);
         if (late(stop)) return;
-        if (act) stv(z, zi, zv + cv);
+        if (black) bstv(bz, c, a ? (rv + of) * rd : 0.f, tag);
     }
 }
 
@@ -723,9 +832,13 @@ __global__ void __launch_bounds__(kBlock) k_pcg_spmv(Geo g, MgLevel L, const flo
 }
 
 // alpha = rz / p.q; e += alpha p; r -= alpha q; partials of r.r
-__global__ void __launch_bounds__(kBlock) k_pcg_update(uint32_t n, float *e, float *r,
-                                                       const float *p, const float *q, MgState *m,
-                                                       RedScratch red)
+//
+// The V-cycle's first pass, the red sweep from z = 0 on level 0, is done here:
+// z = r / diag on red unknowns (red slots [0, H) of the split layout), 0 on
+// other red slots; the V-cycle then starts with the black sweep.
+__global__ void __launch_bounds__(kBlock) k_pcg_update(Geo g, MgLevel L, float *e, float *r,
+                                                       const float *p, const float *q, float *z,
+                                                       MgState *m, RedScratch red)
 {
     __shared__ double lds[kBlock / kWave];
     if (stopped(m)) return;
@@ -736,14 +849,19 @@ __global__ void __launch_bounds__(kBlock) k_pcg_update(uint32_t n, float *e, flo
         return;
     }
     const float alpha = (float)(m->rz[m->inner_it & 1] / pq);
+    const float dint = diag0m(g, kInterior);
     double acc = 0.0;
     uint32_t sb, se;
-    block_span(n, blk, sb, se);
+    block_span(g.nc, blk, sb, se);
     for (uint32_t c = sb + threadIdx.x; c < se; c += kBlock) {
         e[c] += alpha * p[c];
         const float rv = r[c] - alpha * q[c];
         r[c] = rv;
         acc += (double)rv * (double)rv;
+        if (c < L.H) {  // red slot
+            const uint16_t nb = L.nbs[c];
+            z[c] = (nb & NB_SELF) ? rv / diag_of(g, nb, dint) : 0.f;
+        }
     }
     block_partial(acc, pslot(red, SLOT_UPDATE), blk, lds);
 }
@@ -879,7 +997,8 @@ __device__ __forceinline__ void row64(const Geo &g, uint32_t c, uint16_t m, F va
 __global__ void __launch_bounds__(kBlock) k_dc_residual(Geo g, const double *__restrict__ b,
                                                         double *xa, double *xb,
                                                         const float *__restrict__ e,
-                                                        float *__restrict__ r32, const MgState *m,
+                                                        float *__restrict__ r32,
+                                                        float *__restrict__ z0, const MgState *m,
                                                         RedScratch red)
 {
     __shared__ double lds[kBlock / kWave];
@@ -895,9 +1014,11 @@ __global__ void __launch_bounds__(kBlock) k_dc_residual(Geo g, const double *__r
     for (uint32_t c = sb + threadIdx.x; c < se; c += kBlock) {
         const uint16_t nb = g.nbm[c];
         const uint32_t sc = sown(c, nb, H);  // e and r32: split layout
+        const bool red_slot = !(nb & NB_COLOR);
         if (!(nb & NB_SELF)) {
             xout[c] = 0.0;
             r32[sc] = 0.f;
+            if (red_slot) z0[sc] = 0.f;
             continue;
         }
         const float *eo = e + sother(nb, H);
@@ -906,7 +1027,9 @@ __global__ void __launch_bounds__(kBlock) k_dc_residual(Geo g, const double *__r
         const double xc = xin[c] + (double)e[sc];
         const double rv = b[c] - (diag * xc - off);
         xout[c] = xc;
-        r32[sc] = (float)rv;
+        const float rf = (float)rv;
+        r32[sc] = rf;
+        if (red_slot) z0[sc] = rf / diag0m(g, nb);  // the V-cycle's first (red, z = 0) sweep
         acc += rv * rv;
     }
     block_partial(acc, pslot(red, SLOT_UPDATE), blk, lds);
@@ -993,26 +1116,26 @@ void mg_launch_smooth(hipStream_t s, const Geo &g, const MgLevel &L, bool l0, in
 
 template <typename TF, typename TC>
 void mg_launch_restrict(hipStream_t s, const Geo &g, const MgLevel &F, const MgLevel &C, bool l0,
-                        const MgState *m)
+                        bool red0, const MgState *m)
 {
     if (l0)
-        FFR_LAUNCH((k_mg_restrict<TF, TC, true>), dim3(grid_for(8ull * C.n)), dim3(kBlock), 0, s, g, F,
-                           C, (const TF *)F.z, (const TF *)F.r, (TC *)C.r, m);
+        FFR_LAUNCH((k_mg_restrict<TF, TC, true>), dim3(grid_for(8ull * C.n)), dim3(kBlock), 0, s, g,
+                   F, C, (const TF *)F.z, (const TF *)F.r, (TC *)C.r, (TC *)C.z, (int)red0, m);
     else
-        FFR_LAUNCH((k_mg_restrict<TF, TC, false>), dim3(grid_for(8ull * C.n)), dim3(kBlock), 0, s, g,
-                           F, C, (const TF *)F.z, (const TF *)F.r, (TC *)C.r, m);
+        FFR_LAUNCH((k_mg_restrict<TF, TC, false>), dim3(grid_for(8ull * C.n)), dim3(kBlock), 0, s,
+                   g, F, C, (const TF *)F.z, (const TF *)F.r, (TC *)C.r, (TC *)C.z, (int)red0, m);
 }
 
 template <typename TF, typename TC>
-void mg_launch_prolong(hipStream_t s, const Geo &g, const MgLevel &F, const MgLevel &C, bool l0,
-                       const MgState *m)
+void mg_launch_post_black(hipStream_t s, const Geo &g, const MgLevel &F, const MgLevel &C, bool l0,
+                          const MgState *m)
 {
     if (l0)
-        FFR_LAUNCH((k_mg_prolong<TF, TC, true>), dim3(grid_for(F.n)), dim3(kBlock), 0, s, g, F,
-                           C, (TF *)F.z, (const TC *)C.z, m);
+        FFR_LAUNCH((k_mg_post_black0<TC>), dim3(grid_for(g.nc - F.H)), dim3(kBlock), 0, s, g, F, C,
+                   (const TC *)C.z, m);
     else
-        FFR_LAUNCH((k_mg_prolong<TF, TC, false>), dim3(grid_for(F.n)), dim3(kBlock), 0, s, g,
-                           F, C, (TF *)F.z, (const TC *)C.z, m);
+        FFR_LAUNCH((k_mg_post_black<TF, TC>), dim3(grid_for(F.n)), dim3(kBlock), 0, s, F, C,
+                   (TF *)F.z, (const TF *)F.r, (const TC *)C.z, m);
 }
 
 template <typename T>
@@ -1035,12 +1158,12 @@ void mg_launch_coarse(hipStream_t s, const Geo &g, const MgLevel &L, bool l0, in
 
 template void mg_launch_smooth<float>(hipStream_t, const Geo &, const MgLevel &, bool, int, int, const MgState *);
 template void mg_launch_smooth<__hip_bfloat16>(hipStream_t, const Geo &, const MgLevel &, bool, int, int, const MgState *);
-template void mg_launch_restrict<float, float>(hipStream_t, const Geo &, const MgLevel &, const MgLevel &, bool, const MgState *);
-template void mg_launch_restrict<float, __hip_bfloat16>(hipStream_t, const Geo &, const MgLevel &, const MgLevel &, bool, const MgState *);
-template void mg_launch_restrict<__hip_bfloat16, __hip_bfloat16>(hipStream_t, const Geo &, const MgLevel &, const MgLevel &, bool, const MgState *);
-template void mg_launch_prolong<float, float>(hipStream_t, const Geo &, const MgLevel &, const MgLevel &, bool, const MgState *);
-template void mg_launch_prolong<float, __hip_bfloat16>(hipStream_t, const Geo &, const MgLevel &, const MgLevel &, bool, const MgState *);
-template void mg_launch_prolong<__hip_bfloat16, __hip_bfloat16>(hipStream_t, const Geo &, const MgLevel &, const MgLevel &, bool, const MgState *);
+template void mg_launch_restrict<float, float>(hipStream_t, const Geo &, const MgLevel &, const MgLevel &, bool, bool, const MgState *);
+template void mg_launch_restrict<float, __hip_bfloat16>(hipStream_t, const Geo &, const MgLevel &, const MgLevel &, bool, bool, const MgState *);
+template void mg_launch_restrict<__hip_bfloat16, __hip_bfloat16>(hipStream_t, const Geo &, const MgLevel &, const MgLevel &, bool, bool, const MgState *);
+template void mg_launch_post_black<float, float>(hipStream_t, const Geo &, const MgLevel &, const MgLevel &, bool, const MgState *);
+template void mg_launch_post_black<float, __hip_bfloat16>(hipStream_t, const Geo &, const MgLevel &, const MgLevel &, bool, const MgState *);
+template void mg_launch_post_black<__hip_bfloat16, __hip_bfloat16>(hipStream_t, const Geo &, const MgLevel &, const MgLevel &, bool, const MgState *);
 template void mg_launch_tail<float>(hipStream_t, const TailLevels &, size_t, const MgState *);
 template void mg_launch_tail<__hip_bfloat16>(hipStream_t, const TailLevels &, size_t, const MgState *);
 template void mg_launch_coarse<float>(hipStream_t, const Geo &, const MgLevel &, bool, int, const MgState *);
@@ -1051,10 +1174,11 @@ void pcg_launch_spmv(hipStream_t s, const Geo &g, const MgLevel &L0, const float
 {
     FFR_LAUNCH(k_pcg_spmv, dim3(grid_for(g.nc)), dim3(kBlock), 0, s, g, L0, p, q, m, red);
 }
-void pcg_launch_update(hipStream_t s, uint32_t n, float *e, float *r, const float *p, const float *q,
-                       MgState *m, RedScratch red)
+void pcg_launch_update(hipStream_t s, const Geo &g, const MgLevel &L0, float *e, float *r,
+                       const float *p, const float *q, MgState *m, RedScratch red)
 {
-    FFR_LAUNCH(k_pcg_update, dim3(grid_for(n)), dim3(kBlock), 0, s, n, e, r, p, q, m, red);
+    FFR_LAUNCH(k_pcg_update, dim3(grid_for(g.nc)), dim3(kBlock), 0, s, g, L0, e, r, p, q,
+               (float *)L0.z, m, red);
 }
 void pcg_launch_check(hipStream_t s, uint32_t n, MgState *m, RedScratch red)
 {
@@ -1085,10 +1209,10 @@ void dc_launch_start(hipStream_t s, const Geo &g, MgState *m, RedScratch red, do
                max_outer, max_inner, inner_tol, total_cap);
 }
 void dc_launch_residual(hipStream_t s, const Geo &g, const double *b, double *xa, double *xb,
-                        const float *e, float *r32, const MgState *m, RedScratch red)
+                        const float *e, float *r32, float *z0, const MgState *m, RedScratch red)
 {
-    FFR_LAUNCH(k_dc_residual, dim3(grid_for(g.nc)), dim3(kBlock), 0, s, g, b, xa, xb, e, r32, m,
-               red);
+    FFR_LAUNCH(k_dc_residual, dim3(grid_for(g.nc)), dim3(kBlock), 0, s, g, b, xa, xb, e, r32, z0,
+               m, red);
 }
 void dc_launch_check(hipStream_t s, const Geo &g, MgState *m, RedScratch red)
 {
