@@ -213,6 +213,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-compare", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--no-tuning", action="store_true", help="library defaults, not the tuning table")
     ap.add_argument("--graphs", type=int, default=1,
                     help="replay inner iterations from HIP graphs (0 for tracing profilers)")
     ap.add_argument("--c4", type=int, default=64, help="rooms in the batched sample")
@@ -236,6 +237,13 @@ def main():
 
     form_params_inplace(ps.params, args.mode, args.workload)
     ps.params.graphs = args.graphs
+    tuned = None
+    if args.mode == "mixed" and not args.no_tuning:  # the autotuner's entry for this grid class
+        from ff_room import autotune
+        tuned = autotune.lookup(ps.nx, ps.ny, ps.nz, ps.params.precision)
+        if tuned:
+            autotune.apply(ps.params, tuned)
+            log(f"tuning: {tuned}")
     if args.kernel_only:
         # the named kernel, then the library's plain FP64 copy of nx*ny*nz
         # doubles, whose memory counters calibrate the kernel's
@@ -274,7 +282,7 @@ def main():
         "dtype": "f32+bf16 stencils/multigrid, f64 outer residual" if args.mode == "mixed" else "f64",
         "data": "synthetic (seeded scene generator, zero initial velocity)",
         "config": {"workload": desc, "cells": ps.cells, "mode": args.mode,
-                   "poisson_tol": ps.params.poisson_tol},
+                   "poisson_tol": ps.params.poisson_tol, "tuning": tuned},
         "roofline": roof,
         "kernels": {k: {"launches": v["launches"], "mean_us": 1e3 * v["total_ms"] / v["launches"],
                         "GBps": (v["bytes_per_launch"] / (v["total_ms"] / v["launches"] / 1e3) / 1e9

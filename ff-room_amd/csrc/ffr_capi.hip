@@ -81,6 +81,7 @@ void ffr_params_default(ffr_params *p)
     p->inner_tol = 1e-3;
     p->inner_max_iter = 60;
     p->graphs = 1;
+    p->mg_tail_kb = 0;
 }
 
 int ffr_ctx_create(int device, size_t hbm_budget, ffr_ctx **out)
@@ -245,4 +246,13 @@ int ffr_prof_read(ffr_ctx *c, ffr_prof_entry *out, int cap, int *n_out)
     *n_out = n;
     return FFR_OK;
 }
+
+int ffr_set_max_blocks(int max_blocks)
+{
+    if (max_blocks < 256 || max_blocks > (int)ffr::kMaxBlocksCap) return FFR_E_INVALID;
+    ffr::g_max_blocks = (unsigned)max_blocks;
+    return FFR_OK;
+}
+
+int ffr_get_max_blocks(void) { return (int)ffr::g_max_blocks; }
 }  // extern "C"

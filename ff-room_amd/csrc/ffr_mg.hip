@@ -167,12 +167,15 @@ int mg_create(ffr_scene *s)
     // a single kernel; level 0 never does (it is FP32 in the split layout)
     {
         const size_t vs = bf16 ? 2 : 4;
+        const size_t budget = s->prm.mg_tail_kb > 0
+                                  ? std::min<size_t>((size_t)s->prm.mg_tail_kb * 1024, kTailLdsMax)
+                                  : kTailLdsDefault;
         M.tail_first = -1;
         M.tail_lds = 0;
         size_t acc = 0;
         for (int l = D.levels - 1; l >= 1; l--) {
             acc += tail_level_bytes(M.lv[l].n, vs);
-            if (acc > kTailLdsMax) break;
+            if (acc > budget) break;
             M.tail_first = l;
             M.tail_lds = acc;
         }
@@ -276,7 +279,7 @@ static void vcycle(ffr_scene *s, int l, bool first_red_done)
     const bool red0 = l + 1 != M.tail_first && l + 1 != M.levels - 1;
     {
         ProfScope ps(s->ctx, st, kRestrictName[l], bres);
-        if (l0) mg_launch_restrict<float, TC>(st, g, L, C, true, red0, m);
+        if (l0) mg_launch_restrict<float, TC>(st, g, L, C, true, red0, m, M.qi);
         else mg_launch_restrict<TC, TC>(st, g, L, C, false, red0, m);
     }
     vcycle<TC>(s, l + 1, red0);

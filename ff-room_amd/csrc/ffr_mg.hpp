@@ -96,7 +96,7 @@ struct Multigrid {
 // occupies LDS bytes [off[l], off[l] + tail_level_bytes(n_l, vs)): cx, cy,
 // cz, diag, rdiag (FP32), then z and r (vs bytes each), then active.
 constexpr int kTailThreads = 1024;
-constexpr size_t kTailLdsMax = 48 * 1024;
+constexpr size_t kTailLdsDefault = 48 * 1024, kTailLdsMax = 150 * 1024;
 __host__ __device__ inline size_t tail_level_bytes(uint32_t n, size_t vs)
 {
     return ((20 + 2 * vs + 1) * (size_t)n + 15) & ~(size_t)15;

@@ -9,9 +9,10 @@ void mg_launch_split_codes(hipStream_t s, const Geo &g, const MgLevel &L0);
 template <typename T>
 void mg_launch_smooth(hipStream_t s, const Geo &g, const MgLevel &L, bool l0, int color,
                       int zero_start, const MgState *m);
+// level 0 with `scratch` (nc floats): residual pass + child-sum pass
 template <typename TF, typename TC>
 void mg_launch_restrict(hipStream_t s, const Geo &g, const MgLevel &F, const MgLevel &C, bool l0,
-                        bool red0, const MgState *m);
+                        bool red0, const MgState *m, float *scratch = nullptr);
 // prolongation fused with the first post-smoothing (black) sweep on F
 template <typename TF, typename TC>
 void mg_launch_post_black(hipStream_t s, const Geo &g, const MgLevel &F, const MgLevel &C, bool l0,

@@ -490,6 +490,87 @@ __global__ void __launch_bounds__(kBlock) k_mg_restrict(Geo g, MgLevel F, MgLeve
     }
 }
 
+// ---- level-0 restriction in two passes --------------------------------------
+// (1) res = r - A z on every slot of the split layout (one thread per slot,
+//     the smoother's gather pattern); (2) rc = P^T res, one thread per coarse
+//     cell summing its up-to-8 children, whose split slots are contiguous
+//     across consecutive coarse cells. res lives in the inner PCG's q, which
+//     is not in use while the preconditioner runs.
+__global__ void __launch_bounds__(kBlock) k_mg_resid0(Geo g, MgLevel L, const float *__restrict__ z,
+                                                      const float *__restrict__ r,
+                                                      float *__restrict__ res, const MgState *m)
+{
+    const uint32_t stop = stop_flags(m);
+    const uint32_t H = L.H, si = (uint32_t)g.ny * g.nz, sj = g.nz;
+    const float dint = diag0m(g, kInterior);
+    const auto bcode = buf_rsrc(L.nbs, 2u * g.nc), bz = buf_rsrc(z, 4u * g.nc),
+               br = buf_rsrc(r, 4u * g.nc), bred = buf_rsrc(z, 4u * H),
+               bblack = buf_rsrc(z + H, 4u * (g.nc - H)), bres = buf_rsrc(res, 4u * g.nc);
+    uint32_t sb, se;
+    block_span(g.nc, xcd_block(), sb, se);
+    for (uint32_t s0 = sb + threadIdx.x; s0 < se; s0 += kU * kBlock) {
+        uint16_t nb[kU];
+        float rv[kU], zs[kU], zv[kU][6];
+#pragma unroll
+        for (int t = 0; t < kU; t++) {
+            const uint32_t s = s0 + t * kBlock;
+            const uint32_t col = s >= H ? 1u : 0u;
+            const uint32_t h = s - (col ? H : 0u);
+            nb[t] = buf_ldu16(bcode, s);
+            rv[t] = buf_ldf(br, s);
+            zs[t] = buf_ldf(bz, s);
+            nb_gather(col ? bred : bblack, 2 * h + split_parity(L, h, col), si, sj, zv[t]);
+        }
+        if (late(stop)) return;
+#pragma unroll
+        for (int t = 0; t < kU; t++) {
+            const uint32_t s = s0 + t * kBlock;
+            if (s >= se) break;
+            float v = 0.f;
+            if (nb[t] & NB_SELF)
+                v = rv[t] - (diag_of(g, nb[t], dint) * zs[t] - nb_sum(g, nb[t], zv[t]));
+            buf_stf(bres, s, v);
+        }
+    }
+}
+
+template <typename TC>
+__global__ void __launch_bounds__(kBlock) k_mg_restrict_sum0(Geo g, MgLevel F, MgLevel C,
+                                                             const float *__restrict__ res,
+                                                             TC *__restrict__ rc,
+                                                             TC *__restrict__ zc, int red0,
+                                                             const MgState *m)
+{
+    const uint32_t stop = stop_flags(m);
+    const uint32_t H = F.H;
+    const auto bred = buf_rsrc(res, 4u * H), bblack = buf_rsrc(res + H, 4u * (g.nc - H));
+    uint32_t sb, se;
+    block_span(C.n, xcd_block(), sb, se);
+    for (uint32_t I = sb + threadIdx.x; I < se; I += kBlock) {
+        int ic, jc, kc;
+        decode(I, C.f_nynz, C.f_nz, C.ny, C.nz, ic, jc, kc);
+        float part[8];
+#pragma unroll
+        for (int q = 0; q < 8; q++) {  // child (qx, qy, qz), in the same order as the 8-lane sum
+            const int qx = q >> 2, qy = (q >> 1) & 1, qz = q & 1;
+            const int a = (ic << C.shx) + qx, b = (jc << C.shy) + qy, d = (kc << C.shz) + qz;
+            const bool in = qx <= C.shx && qy <= C.shy && qz <= C.shz && a < F.nx && b < F.ny &&
+                            d < F.nz;
+            const uint32_t c = ((uint32_t)a * F.ny + b) * F.nz + d;
+            // out-of-range children read past the buffer (0)
+            const uint32_t h = in ? (c >> 1) : 0xFFFFFFFFu >> 3;
+            part[q] = ((a + b + d) & 1) ? buf_ldf(bblack, h) : buf_ldf(bred, h);
+        }
+        if (late(stop)) return;
+        // the xor-tree order of k_mg_restrict's lane sum
+        const float s01 = part[0] + part[1], s23 = part[2] + part[3];
+        const float s45 = part[4] + part[5], s67 = part[6] + part[7];
+        const float v = (s01 + s23) + (s45 + s67);
+        stv(rc, I, v);
+        if (red0) coarse_first_red<TC>(C, zc, I, ic, jc, kc, v);
+    }
+}
+
 // ---- prolongation fused into the first post-smoothing sweep ----------------
 // After the coarse correction the first post pass is the black sweep; it reads
 // only red neighbours, so the prolonged red values z_nb + zc[parent(nb)] are
@@ -1116,9 +1197,14 @@ void mg_launch_smooth(hipStream_t s, const Geo &g, const MgLevel &L, bool l0, in
 
 template <typename TF, typename TC>
 void mg_launch_restrict(hipStream_t s, const Geo &g, const MgLevel &F, const MgLevel &C, bool l0,
-                        bool red0, const MgState *m)
+                        bool red0, const MgState *m, float *scratch)
 {
-    if (l0)
+    if (l0 && scratch) {  // two passes through the scratch residual
+        FFR_LAUNCH(k_mg_resid0, dim3(grid_for(g.nc)), dim3(kBlock), 0, s, g, F, (const float *)F.z,
+                   (const float *)F.r, scratch, m);
+        FFR_LAUNCH((k_mg_restrict_sum0<TC>), dim3(grid_for(C.n)), dim3(kBlock), 0, s, g, F, C,
+                   (const float *)scratch, (TC *)C.r, (TC *)C.z, (int)red0, m);
+    } else if (l0)
         FFR_LAUNCH((k_mg_restrict<TF, TC, true>), dim3(grid_for(8ull * C.n)), dim3(kBlock), 0, s, g,
                    F, C, (const TF *)F.z, (const TF *)F.r, (TC *)C.r, (TC *)C.z, (int)red0, m);
     else
@@ -1141,6 +1227,12 @@ void mg_launch_post_black(hipStream_t s, const Geo &g, const MgLevel &F, const M
 template <typename T>
 void mg_launch_tail(hipStream_t s, const TailLevels &tl, size_t lds_bytes, const MgState *m)
 {
+    static bool raised = false;  // dynamic LDS beyond 64 KiB must be allowed per kernel
+    if (!raised) {
+        hipFuncSetAttribute((const void *)k_mg_tail<T>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                            (int)kTailLdsMax);
+        raised = true;
+    }
     FFR_LAUNCH(k_mg_tail<T>, dim3(1), dim3(kTailThreads), lds_bytes, s, tl, m);
 }
 
@@ -1158,9 +1250,9 @@ void mg_launch_coarse(hipStream_t s, const Geo &g, const MgLevel &L, bool l0, in
 
 template void mg_launch_smooth<float>(hipStream_t, const Geo &, const MgLevel &, bool, int, int, const MgState *);
 template void mg_launch_smooth<__hip_bfloat16>(hipStream_t, const Geo &, const MgLevel &, bool, int, int, const MgState *);
-template void mg_launch_restrict<float, float>(hipStream_t, const Geo &, const MgLevel &, const MgLevel &, bool, bool, const MgState *);
-template void mg_launch_restrict<float, __hip_bfloat16>(hipStream_t, const Geo &, const MgLevel &, const MgLevel &, bool, bool, const MgState *);
-template void mg_launch_restrict<__hip_bfloat16, __hip_bfloat16>(hipStream_t, const Geo &, const MgLevel &, const MgLevel &, bool, bool, const MgState *);
+template void mg_launch_restrict<float, float>(hipStream_t, const Geo &, const MgLevel &, const MgLevel &, bool, bool, const MgState *, float *);
+template void mg_launch_restrict<float, __hip_bfloat16>(hipStream_t, const Geo &, const MgLevel &, const MgLevel &, bool, bool, const MgState *, float *);
+template void mg_launch_restrict<__hip_bfloat16, __hip_bfloat16>(hipStream_t, const Geo &, const MgLevel &, const MgLevel &, bool, bool, const MgState *, float *);
 template void mg_launch_post_black<float, float>(hipStream_t, const Geo &, const MgLevel &, const MgLevel &, bool, const MgState *);
 template void mg_launch_post_black<float, __hip_bfloat16>(hipStream_t, const Geo &, const MgLevel &, const MgLevel &, bool, const MgState *);
 template void mg_launch_post_black<__hip_bfloat16, __hip_bfloat16>(hipStream_t, const Geo &, const MgLevel &, const MgLevel &, bool, const MgState *);

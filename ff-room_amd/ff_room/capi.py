@@ -66,7 +66,8 @@ class Params(C.Structure):
                 ("precision", C.c_int), ("poisson_method", C.c_int), ("mg_levels", C.c_int),
                 ("mg_bf16", C.c_int), ("mg_pre_sweeps", C.c_int), ("mg_level_sweeps", C.c_int),
                 ("mg_coarse_sweeps", C.c_int),
-                ("inner_tol", C.c_double), ("inner_max_iter", C.c_int), ("graphs", C.c_int)]
+                ("inner_tol", C.c_double), ("inner_max_iter", C.c_int), ("graphs", C.c_int),
+                ("mg_tail_kb", C.c_int)]
 
 
 class StepResult(C.Structure):
@@ -129,6 +130,8 @@ SIGNATURES = {
     "ffr_progress": (_I, [_P, C.POINTER(C.c_int)]),
     "ffr_ticket_info": (_I, [_P, C.c_uint64, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
     "ffr_release": (_I, [_P, C.c_uint64]),
+    "ffr_set_max_blocks": (_I, [C.c_int]),
+    "ffr_get_max_blocks": (_I, []),
     "ffr_prof_enable": (_I, [_P, C.c_int]),
     "ffr_prof_reset": (_I, [_P]),
     "ffr_prof_read": (_I, [_P, C.POINTER(ProfEntry), C.c_int, C.POINTER(C.c_int)]),

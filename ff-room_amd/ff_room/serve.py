@@ -24,7 +24,7 @@ from typing import Deque, Dict, List, Optional
 
 import numpy as np
 
-from . import capi, device
+from . import autotune, capi, device
 from .config import SceneConfig
 
 
@@ -74,13 +74,14 @@ class Server:
     ``precision=capi.PREC_MIXED``)."""
 
     def __init__(self, ctx: capi.Context, slots: int = 16, hbm_budget: Optional[int] = None,
-                 keep_fields: bool = False, **overrides):
+                 keep_fields: bool = False, tuned: bool = True, **overrides):
         self.ctx = ctx
         self.slots = slots
         ctx.serve_config(slots)
         info = ctx.device_info()
         self.budget = hbm_budget if hbm_budget is not None else int(info["hbm_bytes"] * 0.8)
         self.keep_fields = keep_fields
+        self.tuned = tuned
         self.overrides = overrides
         self.pending: Deque = deque()   # (index, Request, PreparedScene or None)
         self.active: Dict[int, _Active] = {}
@@ -102,6 +103,11 @@ class Server:
             i, req, ps = self.pending[0]
             if ps is None:
                 ps = device.prepare(req.config, **self.overrides)
+                if self.tuned:  # per-scene settings of the autotuner's table for this size class
+                    entry = autotune.lookup(ps.nx, ps.ny, ps.nz, ps.params.precision)
+                    if entry:
+                        autotune.apply(ps.params, {k: v for k, v in entry.items()
+                                                   if k != "max_blocks"})
                 self.pending[0] = (i, req, ps)
             need = ps.footprint()
             if need == 0:

@@ -116,6 +116,8 @@ typedef struct {
     int inner_max_iter;  /* mixed: iteration cap of each inner solve */
     int graphs;          /* replay the inner iteration from a captured HIP graph (1) or
                             launch its kernels one by one (0); results are identical */
+    int mg_tail_kb;      /* LDS budget (KiB, <= 150) of the single-workgroup kernel that runs
+                            the smallest multigrid levels; 0 = 48 */
 } ffr_params;
 
 /* StepResult, FluidSolver.hpp:40-47, plus solver statistics. */
@@ -224,6 +226,13 @@ int ffr_drain(ffr_ctx *ctx);
 int ffr_ticket_info(ffr_ctx *ctx, uint64_t ticket, int *steps, int *done);
 /* Forgets a finished ticket. */
 int ffr_release(ffr_ctx *ctx, uint64_t ticket);
+
+/* ---- launch tuning (no counterpart in the reference) ----------------------- */
+/* Upper bound on the blocks of one grid-strided launch (process-wide; 256 to
+ * 16384, default 2048). Results do not depend on it: every reduction keeps a
+ * fixed order for a given value, and the value is part of the tuning. */
+int ffr_set_max_blocks(int max_blocks);
+int ffr_get_max_blocks(void);
 
 /* ---- profiling ---------------------------------------------------------- */
 int ffr_prof_enable(ffr_ctx *ctx, int on);
