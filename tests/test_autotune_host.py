@@ -8,7 +8,7 @@ from ff_room import autotune, capi
 def test_shape_key_groups_by_cell_count_and_precision():
     assert autotune.shape_key(256, 204, 80, capi.PREC_MIXED) == "mixed:2^21"
     assert autotune.shape_key(256, 200, 80, capi.PREC_MIXED) == "mixed:2^21"
-    assert autotune.shape_key(512, 408, 160, capi.PREC_MIXED) == "mixed:2^25"
+    assert autotune.shape_key(512, 408, 160, capi.PREC_MIXED) == "mixed:2^24"
     assert autotune.shape_key(256, 204, 80, capi.PREC_FP64).startswith("fp64:")
 
 

@@ -83,7 +83,7 @@ def fake(monkeypatch):
 
 
 def test_admission_respects_slots_and_budget(fake):
-    srv = serve.Server(fake, slots=2, hbm_budget=1000, keep_fields=True)
+    srv = serve.Server(fake, slots=2, hbm_budget=1000, keep_fields=True, tuned=False)
     reqs = [serve.Request((400, 10), 5, 0.0) for _ in range(5)]
     done = srv.run(reqs, realtime=False)
     assert [c.index for c in done] == list(range(5))
@@ -95,7 +95,7 @@ def test_admission_respects_slots_and_budget(fake):
 
 
 def test_head_of_line_waits_for_memory(fake):
-    srv = serve.Server(fake, slots=4, hbm_budget=1000)
+    srv = serve.Server(fake, slots=4, hbm_budget=1000, tuned=False)
     done = srv.run([serve.Request((700, 1), 1, 0.0), serve.Request((600, 1), 1, 0.0),
                     serve.Request((100, 1), 1, 0.0)], realtime=False)
     a = {c.index: c for c in done}
@@ -105,7 +105,7 @@ def test_head_of_line_waits_for_memory(fake):
 
 
 def test_oversized_request_is_rejected(fake):
-    srv = serve.Server(fake, slots=1, hbm_budget=100)
+    srv = serve.Server(fake, slots=1, hbm_budget=100, tuned=False)
     with pytest.raises(MemoryError):
         srv.run([serve.Request((500, 1), 1, 0.0)], realtime=False)
 
