@@ -73,10 +73,11 @@ size_t mg_layout(char *base, const Dims &D, bool bf16, Multigrid *M, MgState **s
     MgState *st = (MgState *)take(sizeof(MgState));
     float *e = (float *)take(8ull * H), *ri = (float *)take(8ull * H);
     float *pi = (float *)take(8ull * H), *qi = (float *)take(8ull * H);
+    float *sc = (float *)take(8ull * H);
     float *z0 = (float *)take(8ull * H);
     uint16_t *nbs = (uint16_t *)take(4ull * H);
     if (M) {
-        M->e = e; M->ri = ri; M->pi = pi; M->qi = qi;
+        M->e = e; M->ri = ri; M->pi = pi; M->qi = qi; M->sc = sc;
         M->lv[0].z = z0;
         M->lv[0].r = ri;
         M->lv[0].nbs = nbs;
@@ -225,7 +226,9 @@ static const char *kPostName[kMaxLevels] = {
 // update or the FP64 residual on level 0, the restriction on the coarse
 // levels). The prolongation is fused into the first post-smoothing sweep.
 template <typename TC>
-static void vcycle(ffr_scene *s, int l, bool first_red_done)
+// Returns whether the last pass wrote the r.z partials (level 0 with a
+// smoothed level 0: the last red pass does, when `dot`).
+static bool vcycle(ffr_scene *s, int l, bool first_red_done, bool dot = false)
 {
     Multigrid &M = s->mg;
     const Geo &g = s->geo;
@@ -248,13 +251,13 @@ static void vcycle(ffr_scene *s, int l, bool first_red_done)
         tl.coarse_sweeps = M.coarse_sweeps;
         ProfScope ps(s->ctx, st, "mg_tail", 0);
         mg_launch_tail<TC>(st, tl, M.tail_lds, m);
-        return;
+        return false;
     }
     if (l == M.levels - 1) {
         ProfScope ps(s->ctx, st, l0 ? "mg_coarse_l0" : "mg_coarse", 0);
         if (l0) mg_launch_coarse<float>(st, g, L, true, M.coarse_sweeps, m);
         else mg_launch_coarse<TC>(st, g, L, false, M.coarse_sweeps, m);
-        return;
+        return false;
     }
     const MgLevel &C = M.lv[l + 1];
     // algorithmic bytes of one colour pass: z and r read, the code (level 0,
@@ -265,21 +268,22 @@ static void vcycle(ffr_scene *s, int l, bool first_red_done)
                            : n * (2 * sizeof(TC) + 1 + 16) + C.n * (double)sizeof(TC);
     const double bpost = bsm + C.n * (double)sizeof(TC);
     const int sweeps = l0 ? M.pre_sweeps : M.level_sweeps;
-    auto smooth = [&](int color, int zero_start) {
-        ProfScope ps(s->ctx, st, kSmoothName[l], bsm);
-        if (l0) mg_launch_smooth<float>(st, g, L, true, color, zero_start, m);
+    const RedScratch red = s->red;
+    auto smooth = [&](int color, int zero_start, bool with_dot) {
+        ProfScope ps(s->ctx, st, kSmoothName[l], bsm + (with_dot ? n * 4 : 0.0));
+        if (l0) mg_launch_smooth<float>(st, g, L, true, color, zero_start, m, with_dot ? &red : nullptr);
         else mg_launch_smooth<TC>(st, g, L, false, color, zero_start, m);
     };
     for (int sw = 0; sw < sweeps; sw++) {
-        if (!(sw == 0 && first_red_done)) smooth(0, sw == 0);
-        smooth(1, 0);
+        if (!(sw == 0 && first_red_done)) smooth(0, sw == 0, false);
+        smooth(1, 0, false);
     }
     // the next level's first sweep rides on the restriction unless that level
     // is the single-workgroup tail or the coarsest (both start from z = 0)
     const bool red0 = l + 1 != M.tail_first && l + 1 != M.levels - 1;
     {
         ProfScope ps(s->ctx, st, kRestrictName[l], bres);
-        if (l0) mg_launch_restrict<float, TC>(st, g, L, C, true, red0, m, M.qi);
+        if (l0) mg_launch_restrict<float, TC>(st, g, L, C, true, red0, m, M.sc);
         else mg_launch_restrict<TC, TC>(st, g, L, C, false, red0, m);
     }
     vcycle<TC>(s, l + 1, red0);
@@ -288,18 +292,40 @@ static void vcycle(ffr_scene *s, int l, bool first_red_done)
         if (l0) mg_launch_post_black<float, TC>(st, g, L, C, true, m);
         else mg_launch_post_black<TC, TC>(st, g, L, C, false, m);
     }
-    smooth(0, 0);
-    for (int sw = 1; sw < sweeps; sw++) {
-        smooth(1, 0);
-        smooth(0, 0);
+    // post-smoothing: the red half of the first sweep, then whole sweeps
+    // (at least one sweep, so the V-cycle ends on a red pass)
+    const int post = std::max(sweeps, 1);
+    for (int sw = 0; sw < post; sw++) {
+        if (sw > 0) smooth(1, 0, false);
+        smooth(0, 0, l0 && dot && sw == post - 1);
+    }
+    return l0 && dot;
+}
+
+// Whether the V-cycle ends with a red pass on level 0 (level 0 smoothed by
+// launches: neither the coarsest level nor in the tail). That pass then
+// writes the partials of r.z, and the PCG takes its main form (k_pcg_dirq).
+static bool main_form(const Multigrid &M) { return M.levels > 1 && M.tail_first != 0; }
+
+// z = M r (level 0's first red sweep is done by the PCG update / FP64
+// residual), then the partials of r.z
+static void precondition(ffr_scene *s)
+{
+    const bool fused = s->mg.levels > 1 && s->mg.lv[1].bf16
+                           ? vcycle<__hip_bfloat16>(s, 0, true, true)
+                           : vcycle<float>(s, 0, true, true);
+    if (!fused) {
+        ProfScope ps(s->ctx, s->stream, "pcg_dot_rz", (double)s->geo.nc * 8);
+        pcg_launch_dot_rz(s->stream, s->geo.nc, s->mg.ri, (const float *)s->mg.lv[0].z, s->mgs,
+                          s->red);
     }
 }
 
-static void precondition(ffr_scene *s)
+// the number of r.z partials precondition() leaves: one per block of the
+// last red pass (over the H red slots) or of k_pcg_dot_rz (all slots)
+static uint32_t gdot_of(const ffr_scene *s)
 {
-    // level 0's first red sweep is done by the PCG update / FP64 residual
-    if (s->mg.levels > 1 && s->mg.lv[1].bf16) vcycle<__hip_bfloat16>(s, 0, true);
-    else vcycle<float>(s, 0, true);
+    return main_form(s->mg) ? grid_for(s->mg.lv[0].H) : grid_for(s->geo.nc);
 }
 
 static void inner_iteration(ffr_scene *s)
@@ -308,24 +334,23 @@ static void inner_iteration(ffr_scene *s)
     const Geo &g = s->geo;
     hipStream_t st = s->stream;
     const double n = (double)g.nc;
-    {
-        ProfScope ps(s->ctx, st, "pcg_spmv", n * (4 + 2 + 4));
+    uint32_t gq;
+    if (main_form(M)) {
+        // z, r (red half), code, p and q read; p and q written
+        ProfScope ps(s->ctx, st, "pcg_dirq", n * (4 + 2 + 2 + 4 + 4 + 4 + 4));
+        pcg_launch_dirq(st, g, M.lv[0], M.ri, M.pi, M.qi, s->mgs, s->red, gdot_of(s));
+        gq = grid_for(M.lv[0].H);
+    } else {
+        pcg_launch_dir(st, g.nc, (const float *)M.lv[0].z, M.pi, s->mgs, s->red, gdot_of(s));
         pcg_launch_spmv(st, g, M.lv[0], M.pi, M.qi, s->mgs, s->red);
+        gq = grid_for(g.nc);
     }
     {
         ProfScope ps(s->ctx, st, "pcg_update", n * 24);
-        pcg_launch_update(st, g, M.lv[0], M.e, M.ri, M.pi, M.qi, s->mgs, s->red);
+        pcg_launch_update(st, g, M.lv[0], M.e, M.ri, M.pi, M.qi, s->mgs, s->red, gq);
     }
     pcg_launch_check(st, g.nc, s->mgs, s->red);
     precondition(s);
-    {
-        ProfScope ps(s->ctx, st, "pcg_dot_rz", n * 8);
-        pcg_launch_dot_rz(st, g.nc, M.ri, (const float *)M.lv[0].z, s->mgs, s->red);
-    }
-    {
-        ProfScope ps(s->ctx, st, "pcg_dir", n * 12);
-        pcg_launch_dir(st, g.nc, (const float *)M.lv[0].z, M.pi, s->mgs, s->red);
-    }
 }
 
 // ---- the solve as a task (see ffr_task.hip) ------------------------------------
@@ -335,15 +360,9 @@ static void inner_iteration(ffr_scene *s)
 // then waits once; an inner solve with no history is grown under host control
 // (a few iterations, then two at a time until the device reports it done).
 
-static void inner_start(ffr_scene *s)
-{
-    Multigrid &M = s->mg;
-    hipStream_t st = s->stream;
-    const Geo &g = s->geo;
-    precondition(s);
-    pcg_launch_dot_rz(st, g.nc, M.ri, (const float *)M.lv[0].z, s->mgs, s->red);
-    pcg_launch_start(st, g.nc, (const float *)M.lv[0].z, M.pi, M.e, s->mgs, s->red);
-}
+// an inner solve starts with z = M r and r.z; its first iteration takes
+// p = z and e = 0 (k_pcg_dirq or k_pcg_dir, and k_pcg_update)
+static void inner_start(ffr_scene *s) { precondition(s); }
 
 // ---- graph replay ------------------------------------------------------------
 // Every launch of an inner iteration (and of an inner solve's start) has the
@@ -514,24 +533,23 @@ double mg_prof_launch(ffr_scene *s, const char *name, std::function<void()> *lau
         *launch = [=] { mg_launch_smooth<float>(st, g, L0, true, 1, 0, m); };
         return (double)(g.nc - L0.H) * (4 + 4 + 2 + 4);
     }
-    if (k == "mg_restrict_l0" && M.levels > 1) {
-        const MgLevel L1 = M.lv[1];
-        const bool bf = L1.bf16;
-        const double nc1 = (double)L1.n * (bf ? 2 : 4);
-        if (bf) *launch = [=] { mg_launch_restrict<float, __hip_bfloat16>(st, g, L0, L1, true, false, m); };
-        else *launch = [=] { mg_launch_restrict<float, float>(st, g, L0, L1, true, false, m); };
-        return n * (4 + 4 + 2) + nc1;
+    if (k == "mg_resid0" && M.levels > 1) {  // the first pass of the level-0 restriction
+        float *res = M.sc;
+        *launch = [=] { mg_launch_resid0(st, g, L0, res, m); };
+        return n * (4 + 4 + 2 + 4);
     }
-    if (k == "pcg_spmv") {
-        float *p = M.pi, *q = M.qi;
+    if (k == "pcg_dirq" && main_form(M)) {
+        float *r = M.ri, *p = M.pi, *q = M.qi;
         RedScratch red = s->red;
-        *launch = [=] { pcg_launch_spmv(st, g, L0, p, q, m, red); };
-        return n * (4 + 2 + 4);
+        const uint32_t gd = gdot_of(s);
+        *launch = [=] { pcg_launch_dirq(st, g, L0, r, p, q, m, red, gd); };
+        return n * (4 + 2 + 2 + 4 + 4 + 4 + 4);
     }
     if (k == "pcg_update") {
         float *e = M.e, *r = M.ri, *p = M.pi, *q = M.qi;
         RedScratch red = s->red;
-        *launch = [=] { pcg_launch_update(st, g, L0, e, r, p, q, m, red); };
+        const uint32_t gq = main_form(M) ? grid_for(L0.H) : grid_for(g.nc);
+        *launch = [=] { pcg_launch_update(st, g, L0, e, r, p, q, m, red, gq); };
         return n * 24;
     }
     if (k == "dc_residual") {

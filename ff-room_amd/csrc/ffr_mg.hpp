@@ -79,15 +79,16 @@ struct Multigrid {
     MgLevel lv[kMaxLevels];
     void *arena = nullptr;
     size_t bytes = 0;
-    // level-0 vectors of the inner PCG (FP32) and the outer correction
-    float *e, *ri, *pi, *qi;
+    // level-0 vectors of the inner PCG (FP32) and the outer correction; sc
+    // holds the level-0 residual of the V-cycle's restriction
+    float *e, *ri, *pi, *qi, *sc;
     int pre_sweeps = 2, level_sweeps = 2, coarse_sweeps = 8;
     // levels tail_first .. levels-1 run as one single-workgroup kernel with
     // their z and r in LDS (-1: none); tail_lds = its LDS bytes
     int tail_first = -1;
     size_t tail_lds = 0;
     // captured graphs: one inner PCG iteration, and the start of an inner
-    // solve (V-cycle on r, r.z, p = z); built on first use when enabled
+    // solve (V-cycle on r and r.z); built on first use when enabled
     hipGraphExec_t g_iter = nullptr, g_start = nullptr;
     bool graphs_tried = false;
 };

@@ -6,13 +6,16 @@
 namespace ffr {
 void mg_launch_build(hipStream_t s, const Geo &g, const MgLevel &F, const MgLevel &C, bool fine0);
 void mg_launch_split_codes(hipStream_t s, const Geo &g, const MgLevel &L0);
+// `dot` (level 0, red pass only): also write the r.z partials of the PCG
 template <typename T>
 void mg_launch_smooth(hipStream_t s, const Geo &g, const MgLevel &L, bool l0, int color,
-                      int zero_start, const MgState *m);
+                      int zero_start, const MgState *m, const RedScratch *dot = nullptr);
 // level 0 with `scratch` (nc floats): residual pass + child-sum pass
 template <typename TF, typename TC>
 void mg_launch_restrict(hipStream_t s, const Geo &g, const MgLevel &F, const MgLevel &C, bool l0,
                         bool red0, const MgState *m, float *scratch = nullptr);
+// the level-0 residual pass of the restriction alone (profiling)
+void mg_launch_resid0(hipStream_t s, const Geo &g, const MgLevel &L0, float *res, const MgState *m);
 // prolongation fused with the first post-smoothing (black) sweep on F
 template <typename TF, typename TC>
 void mg_launch_post_black(hipStream_t s, const Geo &g, const MgLevel &F, const MgLevel &C, bool l0,
@@ -22,16 +25,21 @@ void mg_launch_tail(hipStream_t s, const TailLevels &tl, size_t lds_bytes, const
 template <typename T>
 void mg_launch_coarse(hipStream_t s, const Geo &g, const MgLevel &L, bool l0, int sweeps,
                       const MgState *m);
+// main form (the V-cycle's last pass is a red level-0 pass): p = z + beta p
+// and q = A z + beta q in place; gdot = number of r.z partials
+void pcg_launch_dirq(hipStream_t s, const Geo &g, const MgLevel &L0, const float *r, float *p,
+                     float *q, MgState *m, RedScratch red, uint32_t gdot);
+// general form: p = z + beta p, then q = A p
+void pcg_launch_dir(hipStream_t s, uint32_t n, const float *z, float *p, MgState *m, RedScratch red,
+                    uint32_t gdot);
 void pcg_launch_spmv(hipStream_t s, const Geo &g, const MgLevel &L0, const float *p, float *q,
                      const MgState *m, RedScratch red);
+// gq = number of p.q partials (the grid of dirq or spmv)
 void pcg_launch_update(hipStream_t s, const Geo &g, const MgLevel &L0, float *e, float *r,
-                       const float *p, const float *q, MgState *m, RedScratch red);
+                       const float *p, const float *q, MgState *m, RedScratch red, uint32_t gq);
 void pcg_launch_check(hipStream_t s, uint32_t n, MgState *m, RedScratch red);
 void pcg_launch_dot_rz(hipStream_t s, uint32_t n, const float *r, const float *z, const MgState *m,
                        RedScratch red);
-void pcg_launch_start(hipStream_t s, uint32_t n, const float *z, float *p, float *e, MgState *m,
-                      RedScratch red);
-void pcg_launch_dir(hipStream_t s, uint32_t n, const float *z, float *p, MgState *m, RedScratch red);
 void dc_launch_init(hipStream_t s, const Geo &g, const double *b, double *x, float *e,
                     RedScratch red);
 void dc_launch_start(hipStream_t s, const Geo &g, MgState *m, RedScratch red, double outer_tol,

@@ -660,7 +660,7 @@ int ffr_prof_kernel(ffr_scene *s, const char *name, int reps, double *mean_ms, d
     } else if (k == "copy") {  // plain FP64 copy b -> r (counter calibration)
         b = nc * 16;
         launch = [&] { launch_copy64(st, s->b, s->r, s->nc); };
-    } else if (mg && (k == "mg_smooth_l0" || k == "mg_restrict_l0" || k == "pcg_spmv" ||
+    } else if (mg && (k == "mg_smooth_l0" || k == "mg_resid0" || k == "pcg_dirq" ||
                       k == "pcg_update" || k == "dc_residual")) {
         HIP_CHECK_CTX(s->ctx, hipMemsetAsync(&s->mgs->solve_done, 0, 2 * sizeof(int), st), FFR_E_DEVICE);
         b = mg_prof_launch(s, k.c_str(), &launch);

@@ -247,6 +247,14 @@ __global__ void __launch_bounds__(kBlock) k_mg_build(Geo g, MgLevel F, MgLevel C
 // neighbours, none OUTFLOW) skip the per-face selects and the diagonal
 // count; their arithmetic is the general formula's with every face present.
 constexpr int kU = 4;
+// Level-0 gather kernels are launched with up to 2048 blocks of 4 waves, 8
+// waves per SIMD on 256 CUs: asking for 8 waves per EU (<= 64 VGPRs) keeps
+// every block resident, so no block waits for a second round.
+#define FFR_RESIDENT __attribute__((amdgpu_waves_per_eu(8)))
+// cells per thread of the gather kernels with the most values in flight
+// (the visiting order of a thread's cells, and so its partial sums, is the
+// same for any unroll: s, s + kBlock, s + 2 kBlock, ...)
+constexpr int kU2 = 2;
 constexpr uint16_t kInterior = 0x007F;  // NB_SELF + six NB_UNK0 bits, no NB_OUT0 bit
 constexpr uint16_t kFaceBits = 0x1FFF;
 
@@ -285,30 +293,45 @@ __device__ __forceinline__ float diag_of(const Geo &g, uint16_t nb, float dint)
     return (nb & kFaceBits) == kInterior ? dint : diag0m(g, nb);
 }
 
-__global__ void __launch_bounds__(kBlock) k_mg_smooth0(Geo g, MgLevel L,
+// With DOT (the V-cycle's last pass on level 0, a red pass), the kernel also
+// writes the partials of r.z over both colours for the PCG: the thread of
+// own slot h adds r z of that slot after its update and r z of the other
+// colour's slot h (the black array is never longer than the red one).
+template <bool DOT>
+__global__ void __launch_bounds__(kBlock) FFR_RESIDENT k_mg_smooth0(Geo g, MgLevel L,
                                                        const uint16_t *__restrict__ code,
                                                        const float *__restrict__ zo,
                                                        float *__restrict__ zc,
-                                                       const float *__restrict__ rc, uint32_t cnt,
+                                                       const float *__restrict__ rc,
+                                                       const float *__restrict__ ro, uint32_t cnt,
                                                        uint32_t cnt_o, uint32_t col,
-                                                       int zero_start, const MgState *m)
+                                                       int zero_start, const MgState *m,
+                                                       RedScratch red)
 {
+    __shared__ double lds[kBlock / kWave];
     const uint32_t stop = stop_flags(m);  // tested after the first loads
     const uint32_t si = (uint32_t)g.ny * g.nz, sj = g.nz;
     const float dint = diag0m(g, kInterior);
     const auto bcode = buf_rsrc(code, 2u * cnt), bz = buf_rsrc(zc, 4u * cnt),
-               br = buf_rsrc(rc, 4u * cnt), bo = buf_rsrc(zo, 4u * cnt_o);
+               br = buf_rsrc(rc, 4u * cnt), bo = buf_rsrc(zo, 4u * cnt_o),
+               bro = buf_rsrc(ro, 4u * cnt_o);
+    const uint32_t blk = xcd_block();
+    double acc = 0.0;
     uint32_t sb, se;
-    block_span(cnt, xcd_block(), sb, se);
+    block_span(cnt, blk, sb, se);
     for (uint32_t h0 = sb + threadIdx.x; h0 < se; h0 += kU * kBlock) {
         uint16_t nb[kU];
-        float rv[kU], zv[kU][6];
+        float rv[kU], zv[kU][6], rov[kU], zov[kU];
 #pragma unroll
         for (int t = 0; t < kU; t++) {
             const uint32_t h = h0 + t * kBlock;  // reads past cnt return 0
             nb[t] = buf_ldu16(bcode, h);
             rv[t] = buf_ldf(br, h);
             if (!zero_start) nb_gather(bo, 2 * h + split_parity(L, h, col), si, sj, zv[t]);
+            if (DOT) {  // past cnt_o both read 0
+                rov[t] = buf_ldf(bro, h);
+                zov[t] = buf_ldf(bo, h);
+            }
         }
         if (late(stop)) return;
 #pragma unroll
@@ -321,8 +344,10 @@ __global__ void __launch_bounds__(kBlock) k_mg_smooth0(Geo g, MgLevel L,
                 v = (rv[t] + of) / diag_of(g, nb[t], dint);
             }
             buf_stf(bz, h, v);
+            if (DOT) acc += (double)rv[t] * (double)v + (double)rov[t] * (double)zov[t];
         }
     }
+    if (DOT) block_partial(acc, pslot(red, SLOT_MG_A), blk, lds);
 }
 
 // Coarse levels (natural layout, stored operator). A coupling is 0 across a
@@ -496,7 +521,7 @@ __global__ void __launch_bounds__(kBlock) k_mg_restrict(Geo g, MgLevel F, MgLeve
 //     cell summing its up-to-8 children, whose split slots are contiguous
 //     across consecutive coarse cells. res lives in the inner PCG's q, which
 //     is not in use while the preconditioner runs.
-__global__ void __launch_bounds__(kBlock) k_mg_resid0(Geo g, MgLevel L, const float *__restrict__ z,
+__global__ void __launch_bounds__(kBlock) FFR_RESIDENT k_mg_resid0(Geo g, MgLevel L, const float *__restrict__ z,
                                                       const float *__restrict__ r,
                                                       float *__restrict__ res, const MgState *m)
 {
@@ -508,11 +533,11 @@ __global__ void __launch_bounds__(kBlock) k_mg_resid0(Geo g, MgLevel L, const fl
                bblack = buf_rsrc(z + H, 4u * (g.nc - H)), bres = buf_rsrc(res, 4u * g.nc);
     uint32_t sb, se;
     block_span(g.nc, xcd_block(), sb, se);
-    for (uint32_t s0 = sb + threadIdx.x; s0 < se; s0 += kU * kBlock) {
-        uint16_t nb[kU];
-        float rv[kU], zs[kU], zv[kU][6];
+    for (uint32_t s0 = sb + threadIdx.x; s0 < se; s0 += kU2 * kBlock) {
+        uint16_t nb[kU2];
+        float rv[kU2], zs[kU2], zv[kU2][6];
 #pragma unroll
-        for (int t = 0; t < kU; t++) {
+        for (int t = 0; t < kU2; t++) {
             const uint32_t s = s0 + t * kBlock;
             const uint32_t col = s >= H ? 1u : 0u;
             const uint32_t h = s - (col ? H : 0u);
@@ -523,7 +548,7 @@ __global__ void __launch_bounds__(kBlock) k_mg_resid0(Geo g, MgLevel L, const fl
         }
         if (late(stop)) return;
 #pragma unroll
-        for (int t = 0; t < kU; t++) {
+        for (int t = 0; t < kU2; t++) {
             const uint32_t s = s0 + t * kBlock;
             if (s >= se) break;
             float v = 0.f;
@@ -588,7 +613,7 @@ __device__ __forceinline__ uint32_t parent_of(const MgLevel &C, int i, int j, in
 
 // level 0 (split layout): one thread per black slot, kU slots per thread
 template <typename TC>
-__global__ void __launch_bounds__(kBlock) k_mg_post_black0(Geo g, MgLevel L, MgLevel C,
+__global__ void __launch_bounds__(kBlock) FFR_RESIDENT k_mg_post_black0(Geo g, MgLevel L, MgLevel C,
                                                            const TC *__restrict__ zc,
                                                            const MgState *m)
 {
@@ -603,11 +628,11 @@ __global__ void __launch_bounds__(kBlock) k_mg_post_black0(Geo g, MgLevel L, MgL
                bz = buf_rsrc(zb, 4u * cnt);
     uint32_t sb, se;
     block_span(cnt, xcd_block(), sb, se);
-    for (uint32_t h0 = sb + threadIdx.x; h0 < se; h0 += kU * kBlock) {
-        uint16_t nb[kU];
-        float rv[kU], zv[kU][6];
+    for (uint32_t h0 = sb + threadIdx.x; h0 < se; h0 += kU2 * kBlock) {
+        uint16_t nb[kU2];
+        float rv[kU2], zv[kU2][6];
 #pragma unroll
-        for (int t = 0; t < kU; t++) {
+        for (int t = 0; t < kU2; t++) {
             const uint32_t h = h0 + t * kBlock;  // reads past cnt return 0
             nb[t] = buf_ldu16(bcode, h);
             rv[t] = buf_ldf(br, h);
@@ -624,7 +649,7 @@ __global__ void __launch_bounds__(kBlock) k_mg_post_black0(Geo g, MgLevel L, MgL
         }
         if (late(stop)) return;
 #pragma unroll
-        for (int t = 0; t < kU; t++) {
+        for (int t = 0; t < kU2; t++) {
             const uint32_t h = h0 + t * kBlock;
             if (h >= se) break;
             float v = 0.f;
@@ -867,6 +892,110 @@ __global__ void __launch_bounds__(kTailThreads) k_mg_tail(TailLevels tl, const M
 }
 
 // ---- inner PCG on level 0 (FP32 vectors, FP64 dots) -------------------------
+// Iteration it (m->inner_it) of an inner solve starts after the V-cycle of
+// the previous one left z = M r and the partials of r.z (gdot of them):
+//   beta = r.z / (r.z before); p = z + beta p (p = z on the first iteration,
+//   whose r.z is recorded as rz[0]); q = A p; partials of p.q.
+//
+// Main form: the V-cycle ended with a red pass on level 0, which set
+// z_red = (r_red + offdiag(z_black)) / diag. So (A z) on red cells is r_red
+// up to that pass's rounding, and q comes from the recurrence
+// q = A z + beta q: red slots gather nothing, black slots gather the six red
+// z; p and q are updated in place. The recurrence drifts from A p only at
+// FP32 rounding level over an inner solve's few iterations, and the outer
+// FP64 residual is recomputed from scratch every pass. Thread work pairs red
+// slot h with black slot H + h, so every block streams both colours.
+__global__ void __launch_bounds__(kBlock) k_pcg_dirq(Geo g, MgLevel L, const float *__restrict__ z,
+                                                     const float *__restrict__ r, float *p,
+                                                     float *q, MgState *m, RedScratch red,
+                                                     uint32_t gdot)
+{
+    __shared__ double lds[kBlock / kWave];
+    const uint32_t stop = stop_flags(m);  // tested after the reduction's loads
+    const int it = m->inner_it;
+    const double rz_prev = m->rz[(it + 1) & 1];  // rz of iteration it - 1 (unused at 0)
+    const double rz_new = sum_partials(pslot(red, SLOT_MG_A), gdot, lds);
+    if (late(stop)) return;  // uniform: every thread of the grid returns
+    const bool first = it == 0;
+    const float beta = first ? 0.f : (float)(rz_new / rz_prev);
+    const uint32_t blk = xcd_block(), H = L.H, nb_black = g.nc - H;
+    const uint32_t si = (uint32_t)g.ny * g.nz, sj = g.nz;
+    const float dint = diag0m(g, kInterior);
+    // red views [0, H), black views [H, nc) (reads past an end return 0 and
+    // stores there are dropped)
+    const auto cr = buf_rsrc(L.nbs, 2u * H), cb = buf_rsrc(L.nbs + H, 2u * nb_black),
+               zr = buf_rsrc(z, 4u * H), zb = buf_rsrc(z + H, 4u * nb_black),
+               rr = buf_rsrc(r, 4u * H), pr = buf_rsrc(p, 4u * H),
+               pb = buf_rsrc(p + H, 4u * nb_black), qr = buf_rsrc(q, 4u * H),
+               qb = buf_rsrc(q + H, 4u * nb_black);
+    double acc = 0.0;
+    uint32_t hb, he;
+    block_span(H, blk, hb, he);
+    for (uint32_t h0 = hb + threadIdx.x; h0 < he; h0 += kU2 * kBlock) {
+        uint16_t nr[kU2], nk[kU2];
+        float zrv[kU2], rv[kU2], prv[kU2], qrv[kU2], zbv[kU2], pbv[kU2], qbv[kU2], zn[kU2][6];
+#pragma unroll
+        for (int t = 0; t < kU2; t++) {
+            const uint32_t h = h0 + t * kBlock;
+            nr[t] = buf_ldu16(cr, h);
+            zrv[t] = buf_ldf(zr, h);
+            rv[t] = buf_ldf(rr, h);
+            nk[t] = buf_ldu16(cb, h);
+            zbv[t] = buf_ldf(zb, h);
+            nb_gather(zr, 2 * h + split_parity(L, h, 1u), si, sj, zn[t]);
+            if (!first) {
+                prv[t] = buf_ldf(pr, h);
+                qrv[t] = buf_ldf(qr, h);
+                pbv[t] = buf_ldf(pb, h);
+                qbv[t] = buf_ldf(qb, h);
+            }
+        }
+#pragma unroll
+        for (int t = 0; t < kU2; t++) {
+            const uint32_t h = h0 + t * kBlock;
+            if (h >= he) break;
+            const float azr = (nr[t] & NB_SELF) ? rv[t] : 0.f;
+            const float azb = (nk[t] & NB_SELF)
+                                  ? diag_of(g, nk[t], dint) * zbv[t] - nb_sum(g, nk[t], zn[t])
+                                  : 0.f;
+            float p0 = zrv[t], q0 = azr, p1 = zbv[t], q1 = azb;
+            if (!first) {
+                p0 = __builtin_fmaf(beta, prv[t], p0);
+                q0 = __builtin_fmaf(beta, qrv[t], q0);
+                p1 = __builtin_fmaf(beta, pbv[t], p1);
+                q1 = __builtin_fmaf(beta, qbv[t], q1);
+            }
+            if (nr[t] & NB_SELF) acc += (double)p0 * (double)q0;
+            if (nk[t] & NB_SELF) acc += (double)p1 * (double)q1;
+            buf_stf(pr, h, p0);
+            buf_stf(qr, h, q0);
+            buf_stf(pb, h, p1);
+            buf_stf(qb, h, q1);
+        }
+    }
+    block_partial(acc, pslot(red, SLOT_SPMV), blk, lds);
+    if (blk == 0 && threadIdx.x == 0) m->rz[it & 1] = rz_new;
+}
+
+// General form (level 0 is the tail or the coarsest level, small grids):
+// p = z + beta p in place, then q = A p with gathers (k_pcg_spmv).
+__global__ void __launch_bounds__(kBlock) k_pcg_dir(uint32_t n, const float *z, float *p, MgState *m,
+                                                    RedScratch red, uint32_t gdot)
+{
+    __shared__ double lds[kBlock / kWave];
+    if (stopped(m)) return;
+    const int it = m->inner_it;
+    const bool first = it == 0;
+    const double rz_new = sum_partials(pslot(red, SLOT_MG_A), gdot, lds);
+    const float beta = first ? 0.f : (float)(rz_new / m->rz[(it + 1) & 1]);
+    const uint32_t blk = xcd_block();
+    uint32_t sb, se;
+    block_span(n, blk, sb, se);
+    for (uint32_t c = sb + threadIdx.x; c < se; c += kBlock)
+        p[c] = first ? z[c] : __builtin_fmaf(beta, p[c], z[c]);
+    if (blk == 0 && threadIdx.x == 0) m->rz[it & 1] = rz_new;
+}
+
 // q = A p, partials of p.q (split layout: slot s of colour s >= H); loads as
 // in k_mg_smooth0
 __global__ void __launch_bounds__(kBlock) k_pcg_spmv(Geo g, MgLevel L, const float *__restrict__ p,
@@ -912,30 +1041,35 @@ __global__ void __launch_bounds__(kBlock) k_pcg_spmv(Geo g, MgLevel L, const flo
     block_partial(acc, pslot(red, SLOT_SPMV), blk, lds);
 }
 
-// alpha = rz / p.q; e += alpha p; r -= alpha q; partials of r.r
+// alpha = rz / p.q; e += alpha p (e = alpha p on an inner solve's first
+// iteration); r -= alpha q; partials of r.r
 //
 // The V-cycle's first pass, the red sweep from z = 0 on level 0, is done here:
 // z = r / diag on red unknowns (red slots [0, H) of the split layout), 0 on
 // other red slots; the V-cycle then starts with the black sweep.
 __global__ void __launch_bounds__(kBlock) k_pcg_update(Geo g, MgLevel L, float *e, float *r,
                                                        const float *p, const float *q, float *z,
-                                                       MgState *m, RedScratch red)
+                                                       MgState *m, RedScratch red, uint32_t gq)
 {
     __shared__ double lds[kBlock / kWave];
     if (stopped(m)) return;
     const uint32_t blk = xcd_block();
-    const double pq = sum_partials(pslot(red, SLOT_SPMV), gridDim.x, lds);
+    const int it = m->inner_it;
+    const bool first = it == 0;
+    uint32_t sb, se;
+    block_span(g.nc, blk, sb, se);
+    const double pq = sum_partials(pslot(red, SLOT_SPMV), gq, lds);  // gq p.q partials
     if (!(pq > 0.0)) {  // breakdown or a non-positive operator: end the inner solve
+        if (first)      // the correction of an inner solve that made no step is 0
+            for (uint32_t c = sb + threadIdx.x; c < se; c += kBlock) e[c] = 0.f;
         if (blk == 0 && threadIdx.x == 0) m->inner_done = 1;
         return;
     }
-    const float alpha = (float)(m->rz[m->inner_it & 1] / pq);
+    const float alpha = (float)(m->rz[it & 1] / pq);
     const float dint = diag0m(g, kInterior);
     double acc = 0.0;
-    uint32_t sb, se;
-    block_span(g.nc, blk, sb, se);
     for (uint32_t c = sb + threadIdx.x; c < se; c += kBlock) {
-        e[c] += alpha * p[c];
+        e[c] = (first ? 0.f : e[c]) + alpha * p[c];
         const float rv = r[c] - alpha * q[c];
         r[c] = rv;
         acc += (double)rv * (double)rv;
@@ -962,7 +1096,8 @@ __global__ void __launch_bounds__(kBlock) k_pcg_check(MgState *m, RedScratch red
         m->inner_done = 1;
 }
 
-// partials of r.z (after the V-cycle)
+// partials of r.z (after a V-cycle whose level 0 is the tail or the coarsest
+// level; otherwise its last red pass writes them)
 __global__ void __launch_bounds__(kBlock) k_pcg_dot_rz(uint32_t n, const float *r, const float *z,
                                                        const MgState *m, RedScratch red)
 {
@@ -974,39 +1109,6 @@ __global__ void __launch_bounds__(kBlock) k_pcg_dot_rz(uint32_t n, const float *
     block_span(n, blk, sb, se);
     for (uint32_t c = sb + threadIdx.x; c < se; c += kBlock) acc += (double)r[c] * (double)z[c];
     block_partial(acc, pslot(red, SLOT_MG_A), blk, lds);
-}
-
-// first direction: p = z, rz[0] = r.z, e = 0
-__global__ void __launch_bounds__(kBlock) k_pcg_start(uint32_t n, const float *z, float *p, float *e,
-                                                      MgState *m, RedScratch red)
-{
-    __shared__ double lds[kBlock / kWave];
-    if (stopped(m)) return;
-    const uint32_t blk = xcd_block();
-    const double rz = sum_partials(pslot(red, SLOT_MG_A), gridDim.x, lds);
-    uint32_t sb, se;
-    block_span(n, blk, sb, se);
-    for (uint32_t c = sb + threadIdx.x; c < se; c += kBlock) {
-        p[c] = z[c];
-        e[c] = 0.f;
-    }
-    if (blk == 0 && threadIdx.x == 0) m->rz[0] = rz;
-}
-
-// beta = rz_new / rz; p = z + beta p
-__global__ void __launch_bounds__(kBlock) k_pcg_dir(uint32_t n, const float *z, float *p, MgState *m,
-                                                    RedScratch red)
-{
-    __shared__ double lds[kBlock / kWave];
-    if (stopped(m)) return;
-    const uint32_t blk = xcd_block();
-    const double rz_new = sum_partials(pslot(red, SLOT_MG_A), gridDim.x, lds);
-    const int it = m->inner_it - 1;  // the iteration the check just closed
-    const float beta = (float)(rz_new / m->rz[it & 1]);
-    uint32_t sb, se;
-    block_span(n, blk, sb, se);
-    for (uint32_t c = sb + threadIdx.x; c < se; c += kBlock) p[c] = z[c] + beta * p[c];
-    if (blk == 0 && threadIdx.x == 0) m->rz[(it + 1) & 1] = rz_new;
 }
 
 // ---- outer FP64 defect correction -------------------------------------------
@@ -1181,14 +1283,21 @@ void mg_launch_split_codes(hipStream_t s, const Geo &g, const MgLevel &L0)
 
 template <typename T>
 void mg_launch_smooth(hipStream_t s, const Geo &g, const MgLevel &L, bool l0, int color,
-                      int zero_start, const MgState *m)
+                      int zero_start, const MgState *m, const RedScratch *dot)
 {
     if (l0) {  // level 0 is FP32 in the split layout
         const uint32_t H = L.H, cnt = color ? g.nc - H : H, base = color ? H : 0u;
+        const uint32_t obase = color ? 0u : H;
         float *z = (float *)L.z;
-        FFR_LAUNCH(k_mg_smooth0, dim3(grid_for(cnt)), dim3(kBlock), 0, s, g, L, L.nbs + base,
-                   z + (color ? 0u : H), z + base, (const float *)L.r + base, cnt, g.nc - cnt,
-                   (uint32_t)color, zero_start, m);
+        const float *r = (const float *)L.r;
+        if (dot && color == 0)
+            FFR_LAUNCH(k_mg_smooth0<true>, dim3(grid_for(cnt)), dim3(kBlock), 0, s, g, L,
+                       L.nbs + base, z + obase, z + base, r + base, r + obase, cnt, g.nc - cnt,
+                       (uint32_t)color, zero_start, m, *dot);
+        else
+            FFR_LAUNCH(k_mg_smooth0<false>, dim3(grid_for(cnt)), dim3(kBlock), 0, s, g, L,
+                       L.nbs + base, z + obase, z + base, r + base, r + obase, cnt, g.nc - cnt,
+                       (uint32_t)color, zero_start, m, RedScratch{});
     }
     else
         FFR_LAUNCH(k_mg_smooth<T>, dim3(grid_for(L.n)), dim3(kBlock), 0, s, L, (T *)L.z,
@@ -1210,6 +1319,12 @@ void mg_launch_restrict(hipStream_t s, const Geo &g, const MgLevel &F, const MgL
     else
         FFR_LAUNCH((k_mg_restrict<TF, TC, false>), dim3(grid_for(8ull * C.n)), dim3(kBlock), 0, s,
                    g, F, C, (const TF *)F.z, (const TF *)F.r, (TC *)C.r, (TC *)C.z, (int)red0, m);
+}
+
+void mg_launch_resid0(hipStream_t s, const Geo &g, const MgLevel &L0, float *res, const MgState *m)
+{
+    FFR_LAUNCH(k_mg_resid0, dim3(grid_for(g.nc)), dim3(kBlock), 0, s, g, L0, (const float *)L0.z,
+               (const float *)L0.r, res, m);
 }
 
 template <typename TF, typename TC>
@@ -1248,8 +1363,8 @@ void mg_launch_coarse(hipStream_t s, const Geo &g, const MgLevel &L, bool l0, in
                            (const T *)L.r, sweeps, m);
 }
 
-template void mg_launch_smooth<float>(hipStream_t, const Geo &, const MgLevel &, bool, int, int, const MgState *);
-template void mg_launch_smooth<__hip_bfloat16>(hipStream_t, const Geo &, const MgLevel &, bool, int, int, const MgState *);
+template void mg_launch_smooth<float>(hipStream_t, const Geo &, const MgLevel &, bool, int, int, const MgState *, const RedScratch *);
+template void mg_launch_smooth<__hip_bfloat16>(hipStream_t, const Geo &, const MgLevel &, bool, int, int, const MgState *, const RedScratch *);
 template void mg_launch_restrict<float, float>(hipStream_t, const Geo &, const MgLevel &, const MgLevel &, bool, bool, const MgState *, float *);
 template void mg_launch_restrict<float, __hip_bfloat16>(hipStream_t, const Geo &, const MgLevel &, const MgLevel &, bool, bool, const MgState *, float *);
 template void mg_launch_restrict<__hip_bfloat16, __hip_bfloat16>(hipStream_t, const Geo &, const MgLevel &, const MgLevel &, bool, bool, const MgState *, float *);
@@ -1261,16 +1376,27 @@ template void mg_launch_tail<__hip_bfloat16>(hipStream_t, const TailLevels &, si
 template void mg_launch_coarse<float>(hipStream_t, const Geo &, const MgLevel &, bool, int, const MgState *);
 template void mg_launch_coarse<__hip_bfloat16>(hipStream_t, const Geo &, const MgLevel &, bool, int, const MgState *);
 
+void pcg_launch_dirq(hipStream_t s, const Geo &g, const MgLevel &L0, const float *r, float *p,
+                     float *q, MgState *m, RedScratch red, uint32_t gdot)
+{
+    FFR_LAUNCH(k_pcg_dirq, dim3(grid_for(L0.H)), dim3(kBlock), 0, s, g, L0, (const float *)L0.z, r,
+               p, q, m, red, gdot);
+}
+void pcg_launch_dir(hipStream_t s, uint32_t n, const float *z, float *p, MgState *m, RedScratch red,
+                    uint32_t gdot)
+{
+    FFR_LAUNCH(k_pcg_dir, dim3(grid_for(n)), dim3(kBlock), 0, s, n, z, p, m, red, gdot);
+}
 void pcg_launch_spmv(hipStream_t s, const Geo &g, const MgLevel &L0, const float *p, float *q,
                      const MgState *m, RedScratch red)
 {
     FFR_LAUNCH(k_pcg_spmv, dim3(grid_for(g.nc)), dim3(kBlock), 0, s, g, L0, p, q, m, red);
 }
 void pcg_launch_update(hipStream_t s, const Geo &g, const MgLevel &L0, float *e, float *r,
-                       const float *p, const float *q, MgState *m, RedScratch red)
+                       const float *p, const float *q, MgState *m, RedScratch red, uint32_t gq)
 {
     FFR_LAUNCH(k_pcg_update, dim3(grid_for(g.nc)), dim3(kBlock), 0, s, g, L0, e, r, p, q,
-               (float *)L0.z, m, red);
+               (float *)L0.z, m, red, gq);
 }
 void pcg_launch_check(hipStream_t s, uint32_t n, MgState *m, RedScratch red)
 {
@@ -1280,15 +1406,6 @@ void pcg_launch_dot_rz(hipStream_t s, uint32_t n, const float *r, const float *z
                        RedScratch red)
 {
     FFR_LAUNCH(k_pcg_dot_rz, dim3(grid_for(n)), dim3(kBlock), 0, s, n, r, z, m, red);
-}
-void pcg_launch_start(hipStream_t s, uint32_t n, const float *z, float *p, float *e, MgState *m,
-                      RedScratch red)
-{
-    FFR_LAUNCH(k_pcg_start, dim3(grid_for(n)), dim3(kBlock), 0, s, n, z, p, e, m, red);
-}
-void pcg_launch_dir(hipStream_t s, uint32_t n, const float *z, float *p, MgState *m, RedScratch red)
-{
-    FFR_LAUNCH(k_pcg_dir, dim3(grid_for(n)), dim3(kBlock), 0, s, n, z, p, m, red);
 }
 void dc_launch_init(hipStream_t s, const Geo &g, const double *b, double *x, float *e, RedScratch red)
 {

@@ -242,8 +242,8 @@ int ffr_prof_read(ffr_ctx *ctx, ffr_prof_entry *out, int cap, int *n_out);
  * buffers, each between the begin and end timestamps of its own dispatch
  * (the figure a kernel trace reports): the mean kernel duration, and the
  * kernel's algorithmic bytes per launch. Kernels: "predict", "rhs",
- * "cg_spmv", "cg_update", "cg_dir", "mg_smooth_l0", "mg_restrict_l0",
- * "pcg_spmv", "pcg_update", "dc_residual", and "copy" (a plain FP64 copy
+ * "cg_spmv", "cg_update", "cg_dir", "mg_smooth_l0", "mg_resid0",
+ * "pcg_dirq", "pcg_update", "dc_residual", and "copy" (a plain FP64 copy
  * of nx*ny*nz doubles, to calibrate memory counters). The timed kernels overwrite
  * solver scratch (not u, v, w, T): call between steps, not inside one. */
 int ffr_prof_kernel(ffr_scene *scene, const char *name, int reps, double *mean_ms,
