@@ -22,8 +22,7 @@ struct Dims {
 // at most 1/sqrt(2) of the largest spacing (their coupling is at least twice
 // the weakest), or all such axes when none is (near-cubic cells); stop when
 // no axis has 4 cells, at the level cap, or once a level has at most
-// kCoarsest cells (solved by symmetric sweeps in one workgroup).
-constexpr int kCoarsest = 512;
+// kCoarsest cells (solved with a dense inverse in one workgroup).
 Dims level_dims(int nx, int ny, int nz, const double h0[3], const ffr_params &p)
 {
     Dims d{};
@@ -60,7 +59,7 @@ Dims level_dims(int nx, int ny, int nz, const double h0[3], const ffr_params &p)
 }
 
 // arena layout (sizing pass when base == nullptr)
-size_t mg_layout(char *base, const Dims &D, bool bf16, Multigrid *M, MgState **state)
+size_t mg_layout(char *base, const Dims &D, bool bf16, bool dense, Multigrid *M, MgState **state)
 {
     size_t off = 0;
     auto take = [&](size_t bytes) -> char * {
@@ -85,6 +84,13 @@ size_t mg_layout(char *base, const Dims &D, bool bf16, Multigrid *M, MgState **s
         M->lv[0].active = nullptr;
         M->lv[0].cx = M->lv[0].cy = M->lv[0].cz = M->lv[0].diag = M->lv[0].rdiag = nullptr;
         M->lv[0].bf16 = 0;
+    }
+    {
+        const int L = D.levels - 1;
+        const uint32_t n = (uint32_t)D.n[L][0] * D.n[L][1] * D.n[L][2];
+        const bool has = dense && L >= 1 && n <= (uint32_t)kCoarsest;
+        float *gi = has ? (float *)take(4ull * n * n) : nullptr;
+        if (M) M->ginv = gi;
     }
     for (int l = 1; l < D.levels; l++) {
         const uint32_t n = (uint32_t)D.n[l][0] * D.n[l][1] * D.n[l][2];
@@ -117,7 +123,105 @@ bool uniform(const std::vector<double> &h)
 size_t mg_bytes(int nx, int ny, int nz, const double h0[3], const ffr_params &p)
 {
     const Dims D = level_dims(nx, ny, nz, h0, p);
-    return mg_layout(nullptr, D, p.mg_bf16 != 0, nullptr, nullptr);
+    return mg_layout(nullptr, D, p.mg_bf16 != 0, p.mg_coarse_sweeps == 0, nullptr, nullptr);
+}
+
+// Dense generalised inverse of the coarsest level's operator A (n <= kCoarsest),
+// computed once per scene in FP64 and stored in FP32: A = L D L^T without
+// pivoting (A is symmetric positive semidefinite); a pivot at or below
+// 1e-12 of the largest diagonal marks a direction A does not see (a pocket
+// with no Dirichlet neighbour; for a semidefinite A its column below is 0
+// too) and gets D^+ = 0. G = L^-T D^+ L^-1 is symmetric and semidefinite, so
+// the V-cycle stays a symmetric preconditioner. Rows of cells with no
+// unknown child are the identity.
+static int coarse_inverse(ffr_scene *s)
+{
+    Multigrid &M = s->mg;
+    const MgLevel &C = M.lv[M.levels - 1];
+    const int n = (int)C.n, si = C.ny * C.nz, sj = C.nz;
+    std::vector<float> cx(n), cy(n), cz(n), dg(n);
+    std::vector<uint8_t> act(n);
+    hipStream_t st = s->stream;
+    HIP_CHECK_CTX(s->ctx, hipMemcpyAsync(cx.data(), C.cx, 4ull * n, hipMemcpyDeviceToHost, st), FFR_E_DEVICE);
+    HIP_CHECK_CTX(s->ctx, hipMemcpyAsync(cy.data(), C.cy, 4ull * n, hipMemcpyDeviceToHost, st), FFR_E_DEVICE);
+    HIP_CHECK_CTX(s->ctx, hipMemcpyAsync(cz.data(), C.cz, 4ull * n, hipMemcpyDeviceToHost, st), FFR_E_DEVICE);
+    HIP_CHECK_CTX(s->ctx, hipMemcpyAsync(dg.data(), C.diag, 4ull * n, hipMemcpyDeviceToHost, st), FFR_E_DEVICE);
+    HIP_CHECK_CTX(s->ctx, hipMemcpyAsync(act.data(), C.active, n, hipMemcpyDeviceToHost, st), FFR_E_DEVICE);
+    HIP_CHECK_CTX(s->ctx, hipStreamSynchronize(st), FFR_E_DEVICE);
+    // A (FP64): diag, and -coupling to each neighbour (couplings stored on
+    // the upper cell of each pair)
+    std::vector<double> A((size_t)n * n, 0.0);
+    double dmax = 0.0;
+    for (int c = 0; c < n; c++) {
+        if (!act[c]) {
+            A[(size_t)c * n + c] = 1.0;
+            continue;
+        }
+        A[(size_t)c * n + c] = dg[c];
+        dmax = std::max(dmax, (double)dg[c]);
+        const int i = c / si, j = (c / sj) % C.ny, k = c % sj;
+        const int lo[3] = {i > 0 ? c - si : -1, j > 0 ? c - sj : -1, k > 0 ? c - 1 : -1};
+        const float cp[3] = {cx[c], cy[c], cz[c]};
+        for (int a = 0; a < 3; a++)
+            if (lo[a] >= 0 && cp[a] != 0.f) {
+                A[(size_t)c * n + lo[a]] = -(double)cp[a];
+                A[(size_t)lo[a] * n + c] = -(double)cp[a];
+            }
+    }
+    // L D L^T in place: strict lower triangle of A holds L, d the pivots
+    std::vector<double> d(n, 0.0), w(n);
+    const double tol = 1e-12 * std::max(dmax, 1e-300);
+    for (int k = 0; k < n; k++) {
+        double dk = A[(size_t)k * n + k];
+        for (int p = 0; p < k; p++) dk -= A[(size_t)k * n + p] * A[(size_t)k * n + p] * d[p];
+        for (int p = 0; p < k; p++) w[p] = A[(size_t)k * n + p] * d[p];
+        if (dk <= tol) {
+            d[k] = 0.0;
+            for (int i = k + 1; i < n; i++) A[(size_t)i * n + k] = 0.0;
+            continue;
+        }
+        d[k] = dk;
+        for (int i = k + 1; i < n; i++) {
+            double v = A[(size_t)i * n + k];
+            const double *Li = &A[(size_t)i * n];
+            for (int p = 0; p < k; p++) v -= Li[p] * w[p];
+            A[(size_t)i * n + k] = v / dk;
+        }
+    }
+    // X = L^-1 (unit lower triangular) by rows: X_i = e_i - sum_{p<i} L_ip X_p;
+    // then G = X^T D^+ X as rank-one updates by the rows of X. Every inner
+    // loop runs along a row (n = 512: ~35 ms with the factorisation).
+    std::vector<double> X((size_t)n * n, 0.0);
+    for (int i = 0; i < n; i++) {
+        double *Xi = &X[(size_t)i * n];
+        Xi[i] = 1.0;
+        const double *Li = &A[(size_t)i * n];
+        for (int p = 0; p < i; p++) {
+            const double l = Li[p];
+            if (l == 0.0) continue;
+            const double *Xp = &X[(size_t)p * n];
+            for (int c = 0; c <= p; c++) Xi[c] -= l * Xp[c];
+        }
+    }
+    std::vector<double> Gd((size_t)n * n, 0.0);
+    for (int k = 0; k < n; k++) {
+        if (!(d[k] > 0.0)) continue;
+        const double *Xk = &X[(size_t)k * n];
+        const double sk = 1.0 / d[k];
+        for (int i = 0; i <= k; i++) {
+            const double a = Xk[i] * sk;
+            if (a == 0.0) continue;
+            double *Gi = &Gd[(size_t)i * n];
+            for (int j = 0; j <= i; j++) Gi[j] += a * Xk[j];
+        }
+    }
+    std::vector<float> G((size_t)n * n);
+    for (int i = 0; i < n; i++)
+        for (int j = 0; j <= i; j++)
+            G[(size_t)i * n + j] = G[(size_t)j * n + i] = (float)Gd[(size_t)i * n + j];
+    HIP_CHECK_CTX(s->ctx, hipMemcpyAsync(M.ginv, G.data(), 4ull * n * n, hipMemcpyHostToDevice, st), FFR_E_DEVICE);
+    HIP_CHECK_CTX(s->ctx, hipStreamSynchronize(st), FFR_E_DEVICE);
+    return FFR_OK;
 }
 
 static MgState *g_unused_state;  // placeholder for sizing calls
@@ -130,11 +234,17 @@ int mg_create(ffr_scene *s)
                      "use the CG solver)");
         return FFR_E_UNSUPPORTED;
     }
+    if (mg_tail_allow_lds() != 0) {
+        s->ctx->fail("hipFuncSetAttribute: %zu bytes of LDS for the multigrid tail refused",
+                     (size_t)kTailLdsMax);
+        return FFR_E_DEVICE;
+    }
     const double h0[3] = {s->dxv[0], s->dyv[0], s->dzv[0]};
     const Dims D = level_dims(s->nx, s->ny, s->nz, h0, s->prm);
     Multigrid &M = s->mg;
     const bool bf16 = s->prm.mg_bf16 != 0;
-    const size_t bytes = mg_layout(nullptr, D, bf16, nullptr, nullptr);
+    const bool dense = s->prm.mg_coarse_sweeps == 0;
+    const size_t bytes = mg_layout(nullptr, D, bf16, dense, nullptr, nullptr);
     if (s->ctx->in_use + bytes > s->ctx->budget) {
         s->ctx->fail("multigrid hierarchy needs %zu bytes beyond the context budget", bytes);
         return FFR_E_NOMEM;
@@ -146,7 +256,7 @@ int mg_create(ffr_scene *s)
     M.bytes = bytes;
     s->ctx->in_use += bytes;
     MgState *st = nullptr;
-    mg_layout((char *)M.arena, D, bf16, &M, &st);
+    mg_layout((char *)M.arena, D, bf16, dense, &M, &st);
     s->mgs = st;
     M.levels = D.levels;
     for (int l = 0; l < D.levels; l++) {
@@ -157,38 +267,54 @@ int mg_create(ffr_scene *s)
         L.f_nz = make_fastdiv(L.nz);
         L.f_nynz = make_fastdiv(L.ny * L.nz);
     }
-    {
-        MgLevel &L = M.lv[0];
+    for (int l = 0; l < D.levels; l++) {  // colour-slot parity (level 0: split layout; tail)
+        MgLevel &L = M.lv[l];
         L.par_nz = (L.nz % 2) == 0;
         L.par_ny = (L.ny % 2) == 0;
         L.f_hz = make_fastdiv(L.par_nz ? L.nz / 2 : 1);
         L.f_hyz = make_fastdiv(L.par_ny ? L.ny * L.nz / 2 : 1);
     }
-    // the small levels that fit one workgroup's LDS (z and r of each) run as
-    // a single kernel; level 0 never does (it is FP32 in the split layout)
+    // the small levels that fit one workgroup's LDS (operator, z and r of
+    // each) run as a single kernel, and so does the next finer level when its
+    // z and r fit too and it is small (its operator is then read from global
+    // memory); level 0 never does (it is FP32 in the split layout)
     {
         const size_t vs = bf16 ? 2 : 4;
         const size_t budget = s->prm.mg_tail_kb > 0
                                   ? std::min<size_t>((size_t)s->prm.mg_tail_kb * 1024, kTailLdsMax)
                                   : kTailLdsDefault;
-        M.tail_first = -1;
+        M.tail_first = M.tail_staged = -1;
         M.tail_lds = 0;
+        // with more than one level, the restriction adds an FP32 scratch of
+        // the first level's size
+        auto scratch = [&](int l) { return l < D.levels - 1 ? tail_scratch_bytes(M.lv[l].n) : 0; };
         size_t acc = 0;
         for (int l = D.levels - 1; l >= 1; l--) {
-            acc += tail_level_bytes(M.lv[l].n, vs);
-            if (acc > budget) break;
-            M.tail_first = l;
-            M.tail_lds = acc;
+            const size_t b = tail_level_bytes(M.lv[l], vs, true);
+            if (acc + b + scratch(l) > budget) break;
+            acc += b;
+            M.tail_first = M.tail_staged = l;
         }
+        const int u = M.tail_first - 1;
+        if (M.tail_first > 1 && M.lv[u].n <= kTailUnstagedMax &&
+            acc + tail_level_bytes(M.lv[u], vs, false) + scratch(u) <= budget) {
+            acc += tail_level_bytes(M.lv[u], vs, false);
+            M.tail_first = u;
+        }
+        M.tail_lds = M.tail_first >= 0 ? acc + scratch(M.tail_first) : 0;
     }
     M.pre_sweeps = std::max(1, s->prm.mg_pre_sweeps);
     M.level_sweeps = std::max(1, s->prm.mg_level_sweeps);
-    M.coarse_sweeps = std::max(1, s->prm.mg_coarse_sweeps);
+    M.coarse_sweeps = s->prm.mg_coarse_sweeps > 0 ? s->prm.mg_coarse_sweeps : 8;
     hipStream_t str = s->stream;
     HIP_CHECK_CTX(s->ctx, hipMemsetAsync(M.arena, 0, bytes, str), FFR_E_DEVICE);
     mg_launch_split_codes(str, s->geo, M.lv[0]);
     for (int l = 0; l + 1 < D.levels; l++) mg_launch_build(str, s->geo, M.lv[l], M.lv[l + 1], l == 0);
     HIP_CHECK_CTX(s->ctx, hipStreamSynchronize(str), FFR_E_DEVICE);
+    if (M.ginv) {
+        const int rc = coarse_inverse(s);
+        if (rc != FFR_OK) return rc;
+    }
     (void)g_unused_state;
     return FFR_OK;
 }
@@ -243,12 +369,15 @@ static bool vcycle(ffr_scene *s, int l, bool first_red_done, bool dot = false)
         for (int k = l; k < M.levels; k++) {
             tl.lv[k] = M.lv[k];
             tl.off[k] = off;
-            off += (uint32_t)tail_level_bytes(M.lv[k].n, sizeof(TC));
+            off += (uint32_t)tail_level_bytes(M.lv[k], sizeof(TC), k >= M.tail_staged);
         }
+        tl.scratch_off = off;  // FP32 restriction scratch of the first level's size
         tl.first = l;
         tl.last = M.levels - 1;
+        tl.staged_from = M.tail_staged;
         tl.sweeps = M.level_sweeps;
         tl.coarse_sweeps = M.coarse_sweeps;
+        tl.ginv = M.ginv;
         ProfScope ps(s->ctx, st, "mg_tail", 0);
         mg_launch_tail<TC>(st, tl, M.tail_lds, m);
         return false;

@@ -84,9 +84,14 @@ struct Multigrid {
     float *e, *ri, *pi, *qi, *sc;
     int pre_sweeps = 2, level_sweeps = 2, coarse_sweeps = 8;
     // levels tail_first .. levels-1 run as one single-workgroup kernel with
-    // their z and r in LDS (-1: none); tail_lds = its LDS bytes
-    int tail_first = -1;
+    // their z and r in LDS (-1: none); tail_lds = its LDS bytes; the
+    // operators of levels tail_staged .. levels-1 are staged in LDS too
+    int tail_first = -1, tail_staged = -1;
     size_t tail_lds = 0;
+    // dense (generalised) inverse of the coarsest level's operator, FP32,
+    // row-major n x n, n <= kCoarsest: the tail's coarse solve is one
+    // matrix-vector product (null when levels == 1)
+    float *ginv = nullptr;
     // captured graphs: one inner PCG iteration, and the start of an inner
     // solve (V-cycle on r and r.z); built on first use when enabled
     hipGraphExec_t g_iter = nullptr, g_start = nullptr;
@@ -94,19 +99,57 @@ struct Multigrid {
 };
 
 // The levels a single workgroup runs from LDS (see k_mg_tail). Level l
-// occupies LDS bytes [off[l], off[l] + tail_level_bytes(n_l, vs)): cx, cy,
-// cz, diag, rdiag (FP32), then z and r (vs bytes each), then active.
+// occupies LDS bytes [off[l], off[l] + tail_carve(...).end): a staged level
+// holds its operator (cx, cy, cz padded at the top by si, sj and 1 zeros,
+// diag, rdiag, active), every level its z (padded by si zeros at both ends,
+// so a cell reads its six neighbours without bounds tests: the coupling
+// across a wall is 0) and its r (vs bytes each). The first tail level may be unstaged: its
+// operator is then read from global memory (it stays in L2) and only z and
+// r take LDS.
 constexpr int kTailThreads = 1024;
-constexpr size_t kTailLdsDefault = 48 * 1024, kTailLdsMax = 150 * 1024;
-__host__ __device__ inline size_t tail_level_bytes(uint32_t n, size_t vs)
+// the hierarchy stops at a level of at most this many cells; such a coarsest
+// level is solved with a dense inverse
+constexpr int kCoarsest = 512;
+constexpr size_t kTailLdsDefault = 96 * 1024, kTailLdsMax = 150 * 1024;
+// an unstaged first tail level is taken only up to this many cells: beyond
+// it, one workgroup's passes cost more than the kernel launches they replace
+constexpr uint32_t kTailUnstagedMax = 16384;
+struct TailCarve {
+    uint32_t cx, cy, cz, diag, rdiag, act, z, r, end;  // byte offsets in the level's region
+};
+__host__ __device__ inline TailCarve tail_carve(uint32_t n, uint32_t si, uint32_t sj, uint32_t vs,
+                                                bool staged)
 {
-    return ((20 + 2 * vs + 1) * (size_t)n + 15) & ~(size_t)15;
+    auto al = [](uint32_t b) { return (b + 15u) & ~15u; };
+    TailCarve t{};
+    uint32_t o = 0;
+    if (staged) {
+        t.cx = o; o += al(4u * (n + si));
+        t.cy = o; o += al(4u * (n + sj));
+        t.cz = o; o += al(4u * (n + 1));
+        t.diag = o; o += al(4u * n);
+        t.rdiag = o; o += al(4u * n);
+        t.act = o; o += al(n);
+    }
+    t.z = o + vs * si;  // element 0; the pads are z[-si .. 0) and z[n .. n + si)
+    o += al(vs * (n + 2 * si));
+    t.r = o; o += al(vs * n);
+    t.end = o;
+    return t;
+}
+inline size_t tail_scratch_bytes(uint32_t n) { return ((size_t)4 * n + 15) & ~(size_t)15; }
+inline size_t tail_level_bytes(const MgLevel &L, size_t vs, bool staged)
+{
+    return tail_carve(L.n, (uint32_t)(L.ny * L.nz), (uint32_t)L.nz, (uint32_t)vs, staged).end;
 }
 struct TailLevels {
     MgLevel lv[kMaxLevels];
     uint32_t off[kMaxLevels];
+    uint32_t scratch_off; // FP32 residuals of a level being restricted (first level's size)
     int first, last;
+    int staged_from;      // first level whose operator is in LDS (first or first + 1)
     int sweeps, coarse_sweeps;
+    const float *ginv;    // dense inverse of the coarsest operator (n_last^2), or null
 };
 static_assert(sizeof(TailLevels) <= 4096, "kernel arguments are limited to 4 KiB");
 

@@ -20,6 +20,8 @@ void mg_launch_resid0(hipStream_t s, const Geo &g, const MgLevel &L0, float *res
 template <typename TF, typename TC>
 void mg_launch_post_black(hipStream_t s, const Geo &g, const MgLevel &F, const MgLevel &C, bool l0,
                           const MgState *m);
+// allows the tail kernel kTailLdsMax bytes of dynamic LDS (0 on success)
+int mg_tail_allow_lds();
 template <typename T>
 void mg_launch_tail(hipStream_t s, const TailLevels &tl, size_t lds_bytes, const MgState *m);
 template <typename T>

@@ -736,14 +736,17 @@ __global__ void __launch_bounds__(1024) k_mg_coarse(Geo g, MgLevel L, T *z, cons
 // LDS as T; phases are separated by workgroup barriers. Input r and output z
 // of level tl.first are exchanged with global memory.
 template <typename T>
-struct TailView {  // one level in LDS
-    uint32_t n;
+struct TailView {  // one level of the tail
+    uint32_t n, H, si, sj;
     int nx, ny, nz;
     int shx, shy, shz;
-    FastDiv f_nz, f_nynz;
-    float *cx, *cy, *cz, *diag, *rdiag;
-    T *z, *r;
-    uint8_t *act;
+    int par_nz, par_ny;
+    FastDiv f_nz, f_nynz, f_hz, f_hyz;
+    bool staged;             // operator in LDS at base + tc, else in global memory (G)
+    unsigned char *base;     // the level's LDS region
+    TailCarve tc;
+    const MgLevel *G;
+    T *z, *r;  // LDS; z[-si .. n + si) readable, the pads 0
 };
 
 template <typename T>
@@ -752,59 +755,197 @@ __device__ __forceinline__ TailView<T> tail_view(const TailLevels &tl, unsigned 
     TailView<T> v;
     const MgLevel &L = tl.lv[l];
     v.n = L.n;
+    v.H = (L.n + 1) / 2;
     v.nx = L.nx;
     v.ny = L.ny;
     v.nz = L.nz;
+    v.si = (uint32_t)L.ny * L.nz;
+    v.sj = (uint32_t)L.nz;
     v.shx = L.shx;
     v.shy = L.shy;
     v.shz = L.shz;
+    v.par_nz = L.par_nz;
+    v.par_ny = L.par_ny;
     v.f_nz = L.f_nz;
     v.f_nynz = L.f_nynz;
-    float *f = reinterpret_cast<float *>(smem + tl.off[l]);
-    v.cx = f;
-    v.cy = f + v.n;
-    v.cz = f + 2 * v.n;
-    v.diag = f + 3 * v.n;
-    v.rdiag = f + 4 * v.n;
-    v.z = reinterpret_cast<T *>(f + 5 * v.n);
-    v.r = v.z + v.n;
-    v.act = reinterpret_cast<uint8_t *>(v.r + v.n);
+    v.f_hz = L.f_hz;
+    v.f_hyz = L.f_hyz;
+    v.staged = l >= tl.staged_from;
+    v.tc = tail_carve(v.n, v.si, v.sj, (uint32_t)sizeof(T), v.staged);
+    v.base = smem + tl.off[l];
+    v.G = &tl.lv[l];
+    v.z = reinterpret_cast<T *>(v.base + v.tc.z);
+    v.r = reinterpret_cast<T *>(v.base + v.tc.r);
     return v;
 }
 
+// The operator of a tail level, from LDS (ST: pointers derived from the LDS
+// region, so the compiler emits LDS loads) or from global memory
+struct TailOp {
+    const float *cx, *cy, *cz, *diag, *rdiag;
+    const uint8_t *act;
+};
+template <bool ST, typename T>
+__device__ __forceinline__ TailOp tail_op(const TailView<T> &L)
+{
+    if (ST)
+        return {reinterpret_cast<const float *>(L.base + L.tc.cx),
+                reinterpret_cast<const float *>(L.base + L.tc.cy),
+                reinterpret_cast<const float *>(L.base + L.tc.cz),
+                reinterpret_cast<const float *>(L.base + L.tc.diag),
+                reinterpret_cast<const float *>(L.base + L.tc.rdiag), L.base + L.tc.act};
+    return {L.G->cx, L.G->cy, L.G->cz, L.G->diag, L.G->rdiag, L.G->active};
+}
+
+// natural index of slot m of colour col (cells 2m and 2m + 1 always differ
+// in colour; the parity formula is level 0's split-layout one)
 template <typename T>
-__device__ __forceinline__ float tail_row(const TailView<T> &L, const T *x, int i, int j, int k,
+__device__ __forceinline__ uint32_t tail_cell(const TailView<T> &L, uint32_t m, uint32_t col)
+{
+    uint32_t b = col;
+    if (L.par_nz) b ^= fdiv(m, L.f_hz) & 1u;
+    if (L.par_ny) b ^= fdiv(m, L.f_hyz) & 1u;
+    return 2 * m + b;
+}
+
+// coupled neighbour sum of cell c from x (padded): the couplings of the
+// faces on walls are 0, and the upper neighbours' couplings past the end
+// read staged zero pads or, for an operator in global memory, the next
+// array of the arena (finite) times a zero pad of x
+template <typename T>
+__device__ __forceinline__ float tail_row(const TailView<T> &L, const TailOp &o, const T *x,
                                           uint32_t c)
 {
-    const uint32_t si = (uint32_t)L.ny * L.nz, sj = L.nz;
+    const T *xc = x + c;
     float of = 0.f;
-    if (i > 0) of += L.cx[c] * ldv(x, c - si);
-    if (i < L.nx - 1) of += L.cx[c + si] * ldv(x, c + si);
-    if (j > 0) of += L.cy[c] * ldv(x, c - sj);
-    if (j < L.ny - 1) of += L.cy[c + sj] * ldv(x, c + sj);
-    if (k > 0) of += L.cz[c] * ldv(x, c - 1);
-    if (k < L.nz - 1) of += L.cz[c + 1] * ldv(x, c + 1);
+    of += o.cx[c] * ldv(xc - L.si, 0u);
+    of += o.cx[c + L.si] * ldv(xc + L.si, 0u);
+    of += o.cy[c] * ldv(xc - L.sj, 0u);
+    of += o.cy[c + L.sj] * ldv(xc + L.sj, 0u);
+    of += o.cz[c] * ldv(xc - 1, 0u);
+    of += o.cz[c + 1] * ldv(xc + 1, 0u);
     return of;
 }
 
-template <typename T>
-__device__ __forceinline__ void tail_smooth(const TailView<T> &L, int color, int zero_start)
+// one colour pass: z = (r + offdiag z) / diag on cells of colour `color`
+// (from z = 0 with zero_start), by threads [0, nthr) in steps of nthr. Cells
+// with no unknown child have r = 0 and no couplings, so their z stays 0
+// without a test. A zero_start pass also writes 0 to the other cell of each
+// pair: a row's last cell and the next row's first cell can share a colour
+// (even nz), and the zero coupling between them must multiply a finite
+// value, not whatever LDS held. Every load is issued for all U cells of a
+// thread before any is used (out-of-range slots read cell 0 and store
+// nothing).
+template <int U, bool ST, typename T>
+__device__ __forceinline__ void tail_pass_u(const TailView<T> &L, uint32_t color, int zero_start,
+                                            uint32_t tid, uint32_t nthr)
 {
-    for (uint32_t c = threadIdx.x; c < L.n; c += kTailThreads) {
-        int i, j, k;
-        decode(c, L.f_nynz, L.f_nz, L.ny, L.nz, i, j, k);
-        if (((i + j + k) & 1) != color) {
-            if (zero_start) stv(L.z, c, 0.f);
-            continue;
+    const TailOp o = tail_op<ST>(L);
+    for (uint32_t m0 = tid; m0 < L.H; m0 += U * nthr) {
+        uint32_t c[U];
+        bool ok[U];
+        float rv[U], rd[U], of[U];
+#pragma unroll
+        for (int t = 0; t < U; t++) {
+            const uint32_t m = m0 + (uint32_t)t * nthr;
+            c[t] = tail_cell(L, m < L.H ? m : 0u, color);
+            ok[t] = m < L.H && c[t] < L.n;
+            const uint32_t cc = ok[t] ? c[t] : 0u;
+            rv[t] = ldv(L.r, cc);
+            rd[t] = o.rdiag[cc];
+            of[t] = zero_start ? 0.f : tail_row(L, o, L.z, cc);
         }
-        if (!L.act[c]) {
-            stv(L.z, c, 0.f);
-            continue;
+#pragma unroll
+        for (int t = 0; t < U; t++) {
+            const uint32_t m = m0 + (uint32_t)t * nthr;
+            if (zero_start && m < L.H && (c[t] ^ 1u) < L.n) stv(L.z, c[t] ^ 1u, 0.f);
+            if (ok[t]) stv(L.z, c[t], (rv[t] + of[t]) * rd[t]);
         }
-        const float of = zero_start ? 0.f : tail_row(L, L.z, i, j, k, c);
-        stv(L.z, c, (ldv(L.r, c) + of) * L.rdiag[c]);
     }
+}
+
+// The per-cell phases are instruction-bound: a thread takes 1, 2 or 4 cells
+// per round, as few as cover the level, so no slot is computed for nothing
+// while the loads of a thread's cells are still issued together.
+template <bool ST, typename T>
+__device__ __forceinline__ void tail_pass(const TailView<T> &L, uint32_t color, int zero_start,
+                                          uint32_t tid, uint32_t nthr)
+{
+    if (L.H <= nthr) tail_pass_u<1, ST>(L, color, zero_start, tid, nthr);
+    else if (L.H <= 2 * nthr) tail_pass_u<2, ST>(L, color, zero_start, tid, nthr);
+    else tail_pass_u<4, ST>(L, color, zero_start, tid, nthr);
+}
+
+template <typename T>
+__device__ __forceinline__ void tail_smooth(const TailView<T> &L, uint32_t color, int zero_start)
+{
+    if (L.staged) tail_pass<true>(L, color, zero_start, threadIdx.x, kTailThreads);
+    else tail_pass<false>(L, color, zero_start, threadIdx.x, kTailThreads);
     __syncthreads();
+}
+
+// residuals r - A z of level F into res (FP32), all cells
+template <int U, bool ST, typename T>
+__device__ __forceinline__ void tail_residual_u(const TailView<T> &F, float *res)
+{
+    const TailOp o = tail_op<ST>(F);
+    for (uint32_t c0 = threadIdx.x; c0 < F.n; c0 += U * kTailThreads) {
+        float v[U];
+        uint8_t a[U];
+#pragma unroll
+        for (int t = 0; t < U; t++) {
+            const uint32_t c = c0 + (uint32_t)t * kTailThreads;
+            const uint32_t cc = c < F.n ? c : 0u;
+            a[t] = o.act[cc];
+            v[t] = ldv(F.r, cc) - (o.diag[cc] * ldv(F.z, cc) - tail_row(F, o, F.z, cc));
+        }
+#pragma unroll
+        for (int t = 0; t < U; t++) {
+            const uint32_t c = c0 + (uint32_t)t * kTailThreads;
+            if (c < F.n) res[c] = a[t] ? v[t] : 0.f;
+        }
+    }
+}
+template <bool ST, typename T>
+__device__ __forceinline__ void tail_residual(const TailView<T> &F, float *res)
+{
+    if (F.n <= kTailThreads) tail_residual_u<1, ST>(F, res);
+    else if (F.n <= 2 * kTailThreads) tail_residual_u<2, ST>(F, res);
+    else tail_residual_u<4, ST>(F, res);
+}
+
+// F.z += prolonged C.z on unknown cells
+template <int U, bool ST, typename T>
+__device__ __forceinline__ void tail_prolong_u(const TailView<T> &F, const TailView<T> &C)
+{
+    const TailOp o = tail_op<ST>(F);
+    for (uint32_t c0 = threadIdx.x; c0 < F.n; c0 += U * kTailThreads) {
+        float v[U];
+        uint8_t a[U];
+#pragma unroll
+        for (int t = 0; t < U; t++) {
+            const uint32_t c = c0 + (uint32_t)t * kTailThreads;
+            const uint32_t cc = c < F.n ? c : 0u;
+            int i, j, k;
+            decode(cc, F.f_nynz, F.f_nz, F.ny, F.nz, i, j, k);
+            const uint32_t P =
+                ((uint32_t)(i >> C.shx) * C.ny + (j >> C.shy)) * C.nz + (k >> C.shz);
+            a[t] = o.act[cc];
+            v[t] = ldv(F.z, cc) + ldv(C.z, P);
+        }
+#pragma unroll
+        for (int t = 0; t < U; t++) {
+            const uint32_t c = c0 + (uint32_t)t * kTailThreads;
+            if (c < F.n && a[t]) stv(F.z, c, v[t]);
+        }
+    }
+}
+template <bool ST, typename T>
+__device__ __forceinline__ void tail_prolong(const TailView<T> &F, const TailView<T> &C)
+{
+    if (F.n <= kTailThreads) tail_prolong_u<1, ST>(F, C);
+    else if (F.n <= 2 * kTailThreads) tail_prolong_u<2, ST>(F, C);
+    else tail_prolong_u<4, ST>(F, C);
 }
 
 template <typename T>
@@ -812,16 +953,31 @@ __global__ void __launch_bounds__(kTailThreads) k_mg_tail(TailLevels tl, const M
 {
     extern __shared__ __align__(16) unsigned char tail_smem[];
     if (stopped(m)) return;
-    for (int l = tl.first; l <= tl.last; l++) {  // stage the operators
+    for (int l = tl.first; l <= tl.last; l++) {  // stage the operators, clear the pads
         const MgLevel &G = tl.lv[l];
         const TailView<T> L = tail_view<T>(tl, tail_smem, l);
-        for (uint32_t c = threadIdx.x; c < L.n; c += kTailThreads) {
-            L.cx[c] = G.cx[c];
-            L.cy[c] = G.cy[c];
-            L.cz[c] = G.cz[c];
-            L.diag[c] = G.diag[c];
-            L.rdiag[c] = G.rdiag[c];
-            L.act[c] = G.active[c];
+        if (L.staged) {
+            float *cx = reinterpret_cast<float *>(L.base + L.tc.cx),
+                  *cy = reinterpret_cast<float *>(L.base + L.tc.cy),
+                  *cz = reinterpret_cast<float *>(L.base + L.tc.cz),
+                  *dg = reinterpret_cast<float *>(L.base + L.tc.diag),
+                  *rd = reinterpret_cast<float *>(L.base + L.tc.rdiag);
+            uint8_t *act = L.base + L.tc.act;
+            for (uint32_t c = threadIdx.x; c < L.n; c += kTailThreads) {
+                cx[c] = G.cx[c];
+                cy[c] = G.cy[c];
+                cz[c] = G.cz[c];
+                dg[c] = G.diag[c];
+                rd[c] = G.rdiag[c];
+                act[c] = G.active[c];
+            }
+            for (uint32_t c = L.n + threadIdx.x; c < L.n + L.si; c += kTailThreads) cx[c] = 0.f;
+            for (uint32_t c = L.n + threadIdx.x; c < L.n + L.sj; c += kTailThreads) cy[c] = 0.f;
+            if (threadIdx.x == 0) cz[L.n] = 0.f;
+        }
+        for (uint32_t c = threadIdx.x; c < L.si; c += kTailThreads) {
+            stv(L.z - L.si, c, 0.f);  // the pads below and above the level
+            stv(L.z + L.n, c, 0.f);
         }
     }
     {
@@ -837,6 +993,12 @@ __global__ void __launch_bounds__(kTailThreads) k_mg_tail(TailLevels tl, const M
             tail_smooth<T>(F, 0, sw == 0);
             tail_smooth<T>(F, 1, 0);
         }
+        // residuals of the fine level into the scratch (loads issued
+        // together), then each coarse cell sums its children
+        float *res = reinterpret_cast<float *>(tail_smem + tl.scratch_off);
+        if (F.staged) tail_residual<true>(F, res);
+        else tail_residual<false>(F, res);
+        __syncthreads();
         for (uint32_t I = threadIdx.x; I < C.n; I += kTailThreads) {
             int ic, jc, kc;
             decode(I, C.f_nynz, C.f_nz, C.ny, C.nz, ic, jc, kc);
@@ -847,37 +1009,52 @@ __global__ void __launch_bounds__(kTailThreads) k_mg_tail(TailLevels tl, const M
             float sum = 0.f;
             for (int a = i0; a <= i1; a++)
                 for (int b = j0; b <= j1; b++)
-                    for (int d = k0; d <= k1; d++) {
-                        const uint32_t c = ((uint32_t)a * F.ny + b) * F.nz + d;
-                        if (!F.act[c]) continue;
-                        const float of = tail_row(F, F.z, a, b, d, c);
-                        sum += ldv(F.r, c) - (F.diag[c] * ldv(F.z, c) - of);
-                    }
+                    for (int d = k0; d <= k1; d++) sum += res[((uint32_t)a * F.ny + b) * F.nz + d];
             stv(C.r, I, sum);
         }
         __syncthreads();
     }
-    // coarsest: (red, black)^s then (black, red)^s from z = 0
+    // coarsest: z = G r with the dense inverse, or (red, black)^s then
+    // (black, red)^s from z = 0
     {
         const TailView<T> L = tail_view<T>(tl, tail_smem, tl.last);
-        for (uint32_t c = threadIdx.x; c < L.n; c += kTailThreads) stv(L.z, c, 0.f);
-        __syncthreads();
-        for (int pass = 0; pass < 4 * tl.coarse_sweeps; pass++) {
-            const int half = pass / (2 * tl.coarse_sweeps);
-            tail_smooth<T>(L, ((pass & 1) ^ half) & 1, 0);
+        if (tl.ginv) {
+            // thread i forms row i; G is symmetric, so reading G[j][i] for
+            // consecutive i is a coalesced read of row j; r_j is broadcast
+            const uint32_t i = threadIdx.x;
+            float acc = 0.f;
+            if (i < L.n) {
+                const float *gcol = tl.ginv + i;
+                uint32_t j = 0;
+                for (; j + 8 <= L.n; j += 8) {
+                    float gv[8];
+#pragma unroll
+                    for (int t = 0; t < 8; t++) gv[t] = gcol[(size_t)(j + t) * L.n];
+#pragma unroll
+                    for (int t = 0; t < 8; t++) acc += gv[t] * ldv(L.r, j + t);
+                }
+                for (; j < L.n; j++) acc += gcol[(size_t)j * L.n] * ldv(L.r, j);
+                stv(L.z, i, acc);
+            }
+            __syncthreads();
+        } else if (threadIdx.x < kWave) {
+            // one wave: its LDS accesses are ordered, so passes need no
+            // workgroup barrier (a wavefront fence orders them for the compiler)
+            for (int pass = 0; pass < 4 * tl.coarse_sweeps; pass++) {
+                const int half = pass / (2 * tl.coarse_sweeps);
+                const uint32_t col = (uint32_t)(((pass & 1) ^ half) & 1);
+                if (L.staged) tail_pass<true>(L, col, pass == 0, threadIdx.x, kWave);
+                else tail_pass<false>(L, col, pass == 0, threadIdx.x, kWave);
+                __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+            }
         }
+        if (!tl.ginv) __syncthreads();
     }
     // up: prolongation and post-smoothing
     for (int l = tl.last - 1; l >= tl.first; l--) {
         const TailView<T> F = tail_view<T>(tl, tail_smem, l), C = tail_view<T>(tl, tail_smem, l + 1);
-        for (uint32_t c = threadIdx.x; c < F.n; c += kTailThreads) {
-            if (!F.act[c]) continue;
-            int i, j, k;
-            decode(c, F.f_nynz, F.f_nz, F.ny, F.nz, i, j, k);
-            const uint32_t P =
-                ((uint32_t)(i >> C.shx) * C.ny + (j >> C.shy)) * C.nz + (k >> C.shz);
-            stv(F.z, c, ldv(F.z, c) + ldv(C.z, P));
-        }
+        if (F.staged) tail_prolong<true>(F, C);
+        else tail_prolong<false>(F, C);
         __syncthreads();
         for (int sw = 0; sw < tl.sweeps; sw++) {
             tail_smooth<T>(F, 1, 0);
@@ -1339,15 +1516,22 @@ void mg_launch_post_black(hipStream_t s, const Geo &g, const MgLevel &F, const M
                    (TF *)F.z, (const TF *)F.r, (const TC *)C.z, m);
 }
 
+// Dynamic LDS beyond 64 KiB must be allowed per kernel before a launch; done
+// at hierarchy creation, outside any stream capture, with its status checked.
+int mg_tail_allow_lds()
+{
+    if (hipFuncSetAttribute((const void *)k_mg_tail<float>,
+                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)kTailLdsMax) != hipSuccess)
+        return -1;
+    if (hipFuncSetAttribute((const void *)k_mg_tail<__hip_bfloat16>,
+                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)kTailLdsMax) != hipSuccess)
+        return -1;
+    return 0;
+}
+
 template <typename T>
 void mg_launch_tail(hipStream_t s, const TailLevels &tl, size_t lds_bytes, const MgState *m)
 {
-    static bool raised = false;  // dynamic LDS beyond 64 KiB must be allowed per kernel
-    if (!raised) {
-        hipFuncSetAttribute((const void *)k_mg_tail<T>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                            (int)kTailLdsMax);
-        raised = true;
-    }
     FFR_LAUNCH(k_mg_tail<T>, dim3(1), dim3(kTailThreads), lds_bytes, s, tl, m);
 }
 

@@ -32,12 +32,12 @@ SPACE = {
     "max_blocks": [1024, 2048, 4096, 8192],
     "mg_pre_sweeps": [1, 2, 3],
     "mg_level_sweeps": [1, 2, 3],
-    "mg_coarse_sweeps": [4, 8, 16],
+    "mg_coarse_sweeps": [0, 4, 8, 16],  # 0: dense inverse of the coarsest level
     "inner_tol": [3e-4, 1e-3, 3e-3, 1e-2],
     "mg_tail_kb": [48, 96, 150],
 }
 DEFAULTS = {"max_blocks": 2048, "mg_pre_sweeps": 2, "mg_level_sweeps": 2, "mg_coarse_sweeps": 8,
-            "inner_tol": 1e-3, "mg_tail_kb": 48}
+            "inner_tol": 1e-3, "mg_tail_kb": 96}
 DEFAULT_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tuning.json")
 
 

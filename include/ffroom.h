@@ -111,7 +111,9 @@ typedef struct {
     int mg_bf16;         /* store multigrid levels >= 1 in bfloat16 */
     int mg_pre_sweeps;   /* red-black Gauss-Seidel sweeps per side of the V-cycle, level 0 */
     int mg_level_sweeps; /* the same on levels 1 .. coarsest-1 */
-    int mg_coarse_sweeps;/* symmetric sweeps on the coarsest level */
+    int mg_coarse_sweeps;/* symmetric sweeps on the coarsest level; 0: solve it with a dense
+                            inverse when it has at most 512 cells and runs in the
+                            one-workgroup tail (8 sweeps otherwise) */
     double inner_tol;    /* mixed: relative residual target of each inner FP32 solve */
     int inner_max_iter;  /* mixed: iteration cap of each inner solve */
     int graphs;          /* replay the inner iteration from a captured HIP graph (1) or

@@ -54,12 +54,14 @@ def test_mgpcg_solves_the_reference_system(ctx, bf16, precision):
     assert rel_l2(p, x) < 1e-9
 
 
+@pytest.mark.parametrize("coarse", [8, 0])  # sweeps on the coarsest level, or its dense inverse
 @pytest.mark.parametrize("dims,size", [((16, 12, 14), (4.0, 3.0, 0.7)),   # z-cells 3.4x thinner
                                        ((6, 20, 9), (3.0, 1.0, 2.4)),    # y strongly coupled
                                        ((11, 9, 13), (1.0, 1.0, 1.0))])  # odd extents, near-cubic
-def test_mgpcg_semicoarsened_hierarchy_solves_the_system(ctx, dims, size):
+def test_mgpcg_semicoarsened_hierarchy_solves_the_system(ctx, dims, size, coarse):
     """Anisotropic cells coarsen the strongly coupled axes first: the solve
-    still equals the dense solution, within few iterations."""
+    still equals the dense solution, within few iterations, with the coarsest
+    level swept or solved by its dense inverse."""
     rng = np.random.default_rng(11)
     nx, ny, nz = dims
     f = rng.choice([0] * 9 + [SOLID], size=(nx, ny, nz)).astype(np.uint8)
@@ -69,7 +71,8 @@ def test_mgpcg_semicoarsened_hierarchy_solves_the_system(ctx, dims, size):
     b = _rhs(sc, 5)
     x = np.linalg.solve(A, b)
     d = device_scene(ctx, sc, capi_params({}, poisson_method=capi.POISSON_MGPCG, poisson_tol=1e-11,
-                                          precision=capi.PREC_MIXED, poisson_max_iter=500))
+                                          precision=capi.PREC_MIXED, poisson_max_iter=500,
+                                          mg_coarse_sweeps=coarse))
     p, info = d.poisson(b)
     assert info["residual"] < 1e-11
     assert rel_l2(p, x) < 1e-9
