@@ -18,6 +18,7 @@
 
 namespace ffr {
 unsigned g_max_blocks = 2048;
+unsigned g_resident_blocks = 2048;
 thread_local ProfEvents g_prof_events;
 }
 
@@ -110,6 +111,10 @@ int ffr_ctx_create(int device, size_t hbm_budget, ffr_ctx **out)
         return FFR_E_DEVICE;
     }
     c->prop = prop;
+    // blocks of kBlock threads resident at once at 8 waves per SIMD (4 SIMDs
+    // per CU): the grid cap of the level-0 gather kernels
+    ffr::g_resident_blocks = (unsigned)std::max(1, prop.multiProcessorCount) * 8u * 4u /
+                             (unsigned)(kBlock / kWave);
     c->budget = hbm_budget ? hbm_budget : (size_t)prop.totalGlobalMem * 9 / 10;
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
         delete c;

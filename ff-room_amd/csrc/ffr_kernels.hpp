@@ -55,6 +55,16 @@ inline unsigned grid_for(uint64_t n)
     return (unsigned)(g ? g : 1);
 }
 
+// Kernels that gather neighbours run best as one resident round of blocks
+// (8 waves per SIMD): their grid stays at that capacity even when the cap
+// above is raised for the streaming kernels.
+extern unsigned g_resident_blocks;
+inline unsigned grid_resident(uint64_t n)
+{
+    const unsigned g = grid_for(n);
+    return g < g_resident_blocks ? g : g_resident_blocks;
+}
+
 // ---- step kernels (step_kernels.hip), R = double or float --------------
 template <typename R>
 void launch_predict(hipStream_t s, const Geo &g, const StepParams &sp, const R *u, const R *v,

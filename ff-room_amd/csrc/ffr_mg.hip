@@ -451,10 +451,11 @@ static void precondition(ffr_scene *s)
 }
 
 // the number of r.z partials precondition() leaves: one per block of the
-// last red pass (over the H red slots) or of k_pcg_dot_rz (all slots)
+// last red pass (over the H red slots, a resident grid) or of k_pcg_dot_rz
+// (all slots)
 static uint32_t gdot_of(const ffr_scene *s)
 {
-    return main_form(s->mg) ? grid_for(s->mg.lv[0].H) : grid_for(s->geo.nc);
+    return main_form(s->mg) ? grid_resident(s->mg.lv[0].H) : grid_for(s->geo.nc);
 }
 
 static void inner_iteration(ffr_scene *s)
@@ -468,7 +469,7 @@ static void inner_iteration(ffr_scene *s)
         // z, r (red half), code, p and q read; p and q written
         ProfScope ps(s->ctx, st, "pcg_dirq", n * (4 + 2 + 2 + 4 + 4 + 4 + 4));
         pcg_launch_dirq(st, g, M.lv[0], M.ri, M.pi, M.qi, s->mgs, s->red, gdot_of(s));
-        gq = grid_for(M.lv[0].H);
+        gq = grid_resident(M.lv[0].H);
     } else {
         pcg_launch_dir(st, g.nc, (const float *)M.lv[0].z, M.pi, s->mgs, s->red, gdot_of(s));
         pcg_launch_spmv(st, g, M.lv[0], M.pi, M.qi, s->mgs, s->red);
@@ -677,7 +678,7 @@ double mg_prof_launch(ffr_scene *s, const char *name, std::function<void()> *lau
     if (k == "pcg_update") {
         float *e = M.e, *r = M.ri, *p = M.pi, *q = M.qi;
         RedScratch red = s->red;
-        const uint32_t gq = main_form(M) ? grid_for(L0.H) : grid_for(g.nc);
+        const uint32_t gq = main_form(M) ? grid_resident(L0.H) : grid_for(g.nc);
         *launch = [=] { pcg_launch_update(st, g, L0, e, r, p, q, m, red, gq); };
         return n * 24;
     }

@@ -1468,11 +1468,11 @@ void mg_launch_smooth(hipStream_t s, const Geo &g, const MgLevel &L, bool l0, in
         float *z = (float *)L.z;
         const float *r = (const float *)L.r;
         if (dot && color == 0)
-            FFR_LAUNCH(k_mg_smooth0<true>, dim3(grid_for(cnt)), dim3(kBlock), 0, s, g, L,
+            FFR_LAUNCH(k_mg_smooth0<true>, dim3(grid_resident(cnt)), dim3(kBlock), 0, s, g, L,
                        L.nbs + base, z + obase, z + base, r + base, r + obase, cnt, g.nc - cnt,
                        (uint32_t)color, zero_start, m, *dot);
         else
-            FFR_LAUNCH(k_mg_smooth0<false>, dim3(grid_for(cnt)), dim3(kBlock), 0, s, g, L,
+            FFR_LAUNCH(k_mg_smooth0<false>, dim3(grid_resident(cnt)), dim3(kBlock), 0, s, g, L,
                        L.nbs + base, z + obase, z + base, r + base, r + obase, cnt, g.nc - cnt,
                        (uint32_t)color, zero_start, m, RedScratch{});
     }
@@ -1486,7 +1486,7 @@ void mg_launch_restrict(hipStream_t s, const Geo &g, const MgLevel &F, const MgL
                         bool red0, const MgState *m, float *scratch)
 {
     if (l0 && scratch) {  // two passes through the scratch residual
-        FFR_LAUNCH(k_mg_resid0, dim3(grid_for(g.nc)), dim3(kBlock), 0, s, g, F, (const float *)F.z,
+        FFR_LAUNCH(k_mg_resid0, dim3(grid_resident(g.nc)), dim3(kBlock), 0, s, g, F, (const float *)F.z,
                    (const float *)F.r, scratch, m);
         FFR_LAUNCH((k_mg_restrict_sum0<TC>), dim3(grid_for(C.n)), dim3(kBlock), 0, s, g, F, C,
                    (const float *)scratch, (TC *)C.r, (TC *)C.z, (int)red0, m);
@@ -1500,7 +1500,7 @@ void mg_launch_restrict(hipStream_t s, const Geo &g, const MgLevel &F, const MgL
 
 void mg_launch_resid0(hipStream_t s, const Geo &g, const MgLevel &L0, float *res, const MgState *m)
 {
-    FFR_LAUNCH(k_mg_resid0, dim3(grid_for(g.nc)), dim3(kBlock), 0, s, g, L0, (const float *)L0.z,
+    FFR_LAUNCH(k_mg_resid0, dim3(grid_resident(g.nc)), dim3(kBlock), 0, s, g, L0, (const float *)L0.z,
                (const float *)L0.r, res, m);
 }
 
@@ -1509,7 +1509,7 @@ void mg_launch_post_black(hipStream_t s, const Geo &g, const MgLevel &F, const M
                           const MgState *m)
 {
     if (l0)
-        FFR_LAUNCH((k_mg_post_black0<TC>), dim3(grid_for(g.nc - F.H)), dim3(kBlock), 0, s, g, F, C,
+        FFR_LAUNCH((k_mg_post_black0<TC>), dim3(grid_resident(g.nc - F.H)), dim3(kBlock), 0, s, g, F, C,
                    (const TC *)C.z, m);
     else
         FFR_LAUNCH((k_mg_post_black<TF, TC>), dim3(grid_for(F.n)), dim3(kBlock), 0, s, F, C,
@@ -1563,7 +1563,7 @@ template void mg_launch_coarse<__hip_bfloat16>(hipStream_t, const Geo &, const M
 void pcg_launch_dirq(hipStream_t s, const Geo &g, const MgLevel &L0, const float *r, float *p,
                      float *q, MgState *m, RedScratch red, uint32_t gdot)
 {
-    FFR_LAUNCH(k_pcg_dirq, dim3(grid_for(L0.H)), dim3(kBlock), 0, s, g, L0, (const float *)L0.z, r,
+    FFR_LAUNCH(k_pcg_dirq, dim3(grid_resident(L0.H)), dim3(kBlock), 0, s, g, L0, (const float *)L0.z, r,
                p, q, m, red, gdot);
 }
 void pcg_launch_dir(hipStream_t s, uint32_t n, const float *z, float *p, MgState *m, RedScratch red,

@@ -134,6 +134,37 @@ def _workload_scene(grid, seed=2):
     return device.prepare(cfg)
 
 
+def test_block_cap_beyond_the_resident_grid(ctx):
+    """A block cap above one resident round (8192 > 2048 blocks) keeps the
+    level-0 gather kernels at the resident grid, and the reductions they
+    feed count their partials accordingly: the solve of a 1.3M-cell system
+    converges as with the default cap, to the same solution within the
+    tolerance of two independently rounded solves."""
+    from ff_room import device
+    ps = _workload_scene((128, 128, 80))
+    rng = np.random.default_rng(7)
+    b = rng.standard_normal(ps.cells)
+    b[ps.flags == SOLID] = 0.0
+    pm = capi.default_params(**{k: getattr(ps.params, k) for k, _ in capi.Params._fields_})
+    pm.precision, pm.poisson_method, pm.poisson_tol, pm.poisson_max_iter = 1, 2, 1e-11, 500
+    L = capi.lib()
+    before = L.ffr_get_max_blocks()
+    out = []
+    try:
+        for cap in (8192, 2048):
+            assert L.ffr_set_max_blocks(cap) == 0
+            d = device.create(ctx, ps, pm)
+            x, info = d.poisson(b)
+            d.close()
+            assert info["residual"] < 1e-11, (cap, info)
+            out.append((x, info["iterations"]))
+    finally:
+        L.ffr_set_max_blocks(before)
+    m = ps.flags != SOLID
+    assert rel_l2(out[0][0][m], out[1][0][m]) < 1e-8
+    assert abs(out[0][1] - out[1][1]) <= 2
+
+
 def test_mixed_pressure_error_meets_1e6(ctx):
     """The headline criterion at a reduced c2 grid: the mixed solve's pressure
     on a step's right-hand side is within relative L2 1e-6 (non-SOLID cells)
