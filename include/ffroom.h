@@ -119,7 +119,7 @@ typedef struct {
     int graphs;          /* replay the inner iteration from a captured HIP graph (1) or
                             launch its kernels one by one (0); results are identical */
     int mg_tail_kb;      /* LDS budget (KiB, <= 150) of the single-workgroup kernel that runs
-                            the smallest multigrid levels; 0 = 48 */
+                            the smallest multigrid levels; 0 = 96 */
 } ffr_params;
 
 /* StepResult, FluidSolver.hpp:40-47, plus solver statistics. */
