@@ -66,8 +66,8 @@ def timed_steps(d, ctx, steps: int, warmup: int, prof: bool):
     return t1 - t0, res, table
 
 
-TIMED_KERNELS = ("predict", "rhs", "cg_spmv", "cg_update", "cg_dir", "mg_smooth_l0", "mg_resid0",
-                 "pcg_dirq", "pcg_update", "dc_residual")
+TIMED_KERNELS = ("predict", "correct", "temperature", "rhs", "cg_spmv", "cg_update", "cg_dir",
+                 "mg_smooth_l0", "mg_resid0", "pcg_dirq", "pcg_update", "dc_residual")
 
 
 def traffic_from_profiles(kernel: str, workload: str):

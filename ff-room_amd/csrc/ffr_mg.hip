@@ -596,8 +596,11 @@ int mg_begin(ffr_scene *s)
     StepTask &t = s->task;
     const double n = (double)g.nc;
     {
+        // a step starts from the pressure of the previous step (warm_start);
+        // a bare solve, like PoissonSolver::solve, from 0
+        const bool warm = s->prm.warm_start != 0 && !t.solve_only;
         ProfScope ps(s->ctx, st, "dc_init", n * (8 + 8 + 4 + 1));
-        dc_launch_init(st, g, s->b, s->p, M.e, s->red);
+        dc_launch_init(st, g, s->b, s->p, M.e, warm, s->red);
     }
     dc_launch_start(st, g, s->mgs, s->red, s->prm.poisson_tol, /*max_outer=*/50, max_inner_of(s),
                     s->prm.inner_tol, std::max(1, s->prm.poisson_max_iter));

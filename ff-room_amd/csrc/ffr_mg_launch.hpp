@@ -42,7 +42,8 @@ void pcg_launch_update(hipStream_t s, const Geo &g, const MgLevel &L0, float *e,
 void pcg_launch_check(hipStream_t s, uint32_t n, MgState *m, RedScratch red);
 void pcg_launch_dot_rz(hipStream_t s, uint32_t n, const float *r, const float *z, const MgState *m,
                        RedScratch red);
-void dc_launch_init(hipStream_t s, const Geo &g, const double *b, double *x, float *e,
+// warm: x keeps its values (the previous pressure) instead of starting at 0
+void dc_launch_init(hipStream_t s, const Geo &g, const double *b, double *x, float *e, bool warm,
                     RedScratch red);
 void dc_launch_start(hipStream_t s, const Geo &g, MgState *m, RedScratch red, double outer_tol,
                      int max_outer, int max_inner, double inner_tol, int total_cap);

@@ -647,6 +647,18 @@ int ffr_prof_kernel(ffr_scene *s, const char *name, int reps, double *mean_ms, d
             launch = [&] { launch_predict<double>(st, g, sp, (double *)s->u, (double *)s->v, (double *)s->w, (double *)s->T, (double *)s->us, (double *)s->vs, (double *)s->ws, s->rec); };
         else
             launch = [&] { launch_predict<float>(st, g, sp, (float *)s->u, (float *)s->v, (float *)s->w, (float *)s->T, (float *)s->us, (float *)s->vs, (float *)s->ws, s->rec); };
+    } else if (k == "correct") {  // u = u* - dt/rho grad p from the last step's u* and p
+        b = faces * 3 * sz + nc * 8 + nc;
+        if (s->fp64)
+            launch = [&] { launch_correct<double>(st, g, sp, (double *)s->us, (double *)s->vs, (double *)s->ws, s->p, (double *)s->u, (double *)s->v, (double *)s->w, s->rec); };
+        else
+            launch = [&] { launch_correct<float>(st, g, sp, (float *)s->us, (float *)s->vs, (float *)s->ws, s->p, (float *)s->u, (float *)s->v, (float *)s->w, s->rec); };
+    } else if (k == "temperature" && s->prm.thermal) {  // T -> Tn (the state keeps T)
+        b = nc * 2 * sz + faces * sz + nc;
+        if (s->fp64)
+            launch = [&] { launch_temperature<double>(st, g, sp, (double *)s->u, (double *)s->v, (double *)s->w, (double *)s->T, (double *)s->Tn, s->rec, s->red); };
+        else
+            launch = [&] { launch_temperature<float>(st, g, sp, (float *)s->u, (float *)s->v, (float *)s->w, (float *)s->T, (float *)s->Tn, s->rec, s->red); };
     } else if (k == "rhs") {
         b = faces * sz + nc + nc * 8;
         if (s->fp64) launch = [&] { launch_rhs<double>(st, g, sp, (double *)s->us, (double *)s->vs, (double *)s->ws, s->b); };

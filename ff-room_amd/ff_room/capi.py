@@ -67,7 +67,7 @@ class Params(C.Structure):
                 ("mg_bf16", C.c_int), ("mg_pre_sweeps", C.c_int), ("mg_level_sweeps", C.c_int),
                 ("mg_coarse_sweeps", C.c_int),
                 ("inner_tol", C.c_double), ("inner_max_iter", C.c_int), ("graphs", C.c_int),
-                ("mg_tail_kb", C.c_int)]
+                ("mg_tail_kb", C.c_int), ("warm_start", C.c_int)]
 
 
 class StepResult(C.Structure):

@@ -120,6 +120,9 @@ typedef struct {
                             launch its kernels one by one (0); results are identical */
     int mg_tail_kb;      /* LDS budget (KiB, <= 150) of the single-workgroup kernel that runs
                             the smallest multigrid levels; 0 = 96 */
+    int warm_start;      /* mixed MG-PCG: a step's pressure solve starts from the previous
+                            step's pressure (1) or from 0 as the reference CG does (0);
+                            ffr_poisson_solve always starts from 0 */
 } ffr_params;
 
 /* StepResult, FluidSolver.hpp:40-47, plus solver statistics. */
@@ -243,11 +246,13 @@ int ffr_prof_read(ffr_ctx *ctx, ffr_prof_entry *out, int cap, int *n_out);
 /* Times `reps` launches of one kernel of the step on the scene's current
  * buffers, each between the begin and end timestamps of its own dispatch
  * (the figure a kernel trace reports): the mean kernel duration, and the
- * kernel's algorithmic bytes per launch. Kernels: "predict", "rhs",
+ * kernel's algorithmic bytes per launch. Kernels: "predict", "correct",
+ * "temperature" (thermal scenes), "rhs",
  * "cg_spmv", "cg_update", "cg_dir", "mg_smooth_l0", "mg_resid0",
  * "pcg_dirq", "pcg_update", "dc_residual", and "copy" (a plain FP64 copy
  * of nx*ny*nz doubles, to calibrate memory counters). The timed kernels overwrite
- * solver scratch (not u, v, w, T): call between steps, not inside one. */
+ * solver scratch; "correct" rewrites u, v, w with the values the last step gave
+ * them: call between steps, not inside one. */
 int ffr_prof_kernel(ffr_scene *scene, const char *name, int reps, double *mean_ms,
                     double *bytes_per_launch);
 
