@@ -75,8 +75,11 @@ def _copy(p: capi.Params) -> capi.Params:
 
 
 def measure(ctx: capi.Context, ps: device.PreparedScene, cand: Dict, warmup: int = 2,
-            steps: int = 5) -> Optional[float]:
-    """Mean seconds per step of `cand`, or None if a solve misses its tolerance."""
+            steps: int = 15) -> Optional[float]:
+    """Mean seconds per step of `cand`, or None if a solve misses its tolerance.
+    The window matches the bench's (a few steps into the run): a solve's cost
+    changes as the flow develops and the previous pressure becomes a better
+    start, and 5 steps ranked c2's coarse solves the wrong way round."""
     prm = apply(_copy(ps.params), cand)
     d = device.create(ctx, ps, prm)
     try:
