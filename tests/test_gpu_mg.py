@@ -134,6 +134,33 @@ def _workload_scene(grid, seed=2):
     return device.prepare(cfg)
 
 
+def test_warm_start_meets_the_same_tolerance(ctx):
+    """A step's mixed pressure solve started from the previous pressure meets
+    the same relative residual as one started from 0 (the reference CG's
+    start, PoissonSolver.cpp:96), and the trajectories agree to the solve's
+    accuracy; a bare solve starts from 0 either way."""
+    from ff_room import device
+    ps = _workload_scene((64, 52, 20))
+    out = []
+    for warm in (0, 1):
+        pm = capi.default_params(**{k: getattr(ps.params, k) for k, _ in capi.Params._fields_})
+        pm.precision, pm.poisson_method, pm.poisson_tol, pm.poisson_max_iter = 1, 2, 1e-10, 500
+        pm.warm_start = warm
+        d = device.create(ctx, ps, pm)
+        res = [d.step() for _ in range(6)]
+        assert all(r["pressure_residual"] < 1e-10 for r in res), (warm, res)
+        f = d.download()
+        b = d.laplacian(f["p"])
+        x, info = d.poisson(b)  # from 0: the previous pressure is this solution
+        assert info["iterations"] > 0 and info["residual"] < 1e-10
+        out.append(f)
+        d.close()
+    m = ps.flags != SOLID
+    for k in ("u", "v", "w"):
+        assert rel_l2(out[1][k], out[0][k]) < 1e-6, k
+    assert rel_l2(out[1]["p"][m], out[0]["p"][m]) < 1e-6
+
+
 def test_block_cap_beyond_the_resident_grid(ctx):
     """A block cap above one resident round (8192 > 2048 blocks) keeps the
     level-0 gather kernels at the resident grid, and the reductions they
