@@ -79,7 +79,7 @@ void ffr_params_default(ffr_params *p)
     p->mg_pre_sweeps = 2;
     p->mg_level_sweeps = 2;
     p->mg_coarse_sweeps = 8;
-    p->inner_tol = 1e-3;
+    p->inner_tol = 1e-4;
     p->inner_max_iter = 60;
     p->graphs = 1;
     p->mg_tail_kb = 0;

@@ -9,14 +9,15 @@ outer correction stops every solve at the scene's ``poisson_tol`` whatever
 these are, so a candidate is judged by the wall time of a few steps alone;
 a candidate whose solve does not reach the tolerance is rejected.
 
-Search: coordinate descent from the library defaults, one parameter at a
-time over its candidate values, keeping the fastest, for ``rounds`` rounds.
+Search: coordinate descent from the library defaults (or, with ``--refine``,
+from the stored entry), one parameter at a time over its candidate values,
+keeping the fastest, for ``rounds`` rounds.
 Each evaluation builds a fresh scene, takes ``warmup`` steps and times
 ``steps`` steps. Results are keyed by grid shape class and precision and
 stored as JSON (``tuning.json`` beside this module by default); ``lookup``
 returns the entry for a scene, ``apply`` writes it into ffr_params.
 
-    python -m ff_room.autotune --workload c2 [--out tuning.json]
+    python -m ff_room.autotune --workload c2 [--out tuning.json] [--refine]
 """
 from __future__ import annotations
 
@@ -33,11 +34,12 @@ SPACE = {
     "mg_pre_sweeps": [1, 2, 3],
     "mg_level_sweeps": [1, 2, 3],
     "mg_coarse_sweeps": [0, 4, 8, 16],  # 0: dense inverse of the coarsest level
-    "inner_tol": [3e-4, 1e-3, 3e-3, 1e-2],
+    # tighter inner solves trade a few inner iterations for fewer FP64 outer passes
+    "inner_tol": [1e-6, 1e-5, 1e-4, 3e-4, 1e-3, 3e-3],
     "mg_tail_kb": [48, 96, 150],
 }
 DEFAULTS = {"max_blocks": 2048, "mg_pre_sweeps": 2, "mg_level_sweeps": 2, "mg_coarse_sweeps": 8,
-            "inner_tol": 1e-3, "mg_tail_kb": 96}
+            "inner_tol": 1e-4, "mg_tail_kb": 96}
 DEFAULT_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tuning.json")
 
 
@@ -95,10 +97,14 @@ def measure(ctx: capi.Context, ps: device.PreparedScene, cand: Dict, warmup: int
     return dt
 
 
-def tune(ctx: capi.Context, ps: device.PreparedScene, rounds: int = 2, log=print) -> Dict:
-    best = dict(DEFAULTS)
+def tune(ctx: capi.Context, ps: device.PreparedScene, rounds: int = 2, log=print,
+         start: Optional[Dict] = None) -> Dict:
+    best = dict(DEFAULTS, **(start or {}))
     best_t = measure(ctx, ps, best)
-    log(f"  defaults: {1e3 * best_t:.3f} ms/step")
+    if best_t is None:  # a stored start that no longer converges: begin from the defaults
+        best = dict(DEFAULTS)
+        best_t = measure(ctx, ps, best)
+    log(f"  {'start' if start else 'defaults'}: {1e3 * best_t:.3f} ms/step")
     for r in range(rounds):
         changed = False
         for key, values in SPACE.items():
@@ -122,6 +128,8 @@ def main():
     ap.add_argument("--workload", default="c2", choices=["c2", "c3", "c4"])
     ap.add_argument("--out", default=DEFAULT_PATH)
     ap.add_argument("--rounds", type=int, default=2)
+    ap.add_argument("--refine", action="store_true",
+                    help="start from the stored entry and re-tune classes already in the table")
     args = ap.parse_args()
     ctx = capi.Context(0)
     if args.workload == "c4":  # one representative room per size class
@@ -129,14 +137,17 @@ def main():
     else:
         cfgs = [(workload_c2 if args.workload == "c2" else workload_c3)(precision="mixed")]
     table = load(args.out)
+    tuned = set()  # each shape class once per run, by the first of its rooms
     for cfg in cfgs:
         ps = device.prepare(cfg, precision=capi.PREC_MIXED, poisson_method=capi.POISSON_MGPCG,
                             poisson_tol=1e-10, poisson_max_iter=500)
         key = shape_key(ps.nx, ps.ny, ps.nz, capi.PREC_MIXED)
-        if key in table and args.workload == "c4":
+        if key in tuned or (key in table and args.workload == "c4" and not args.refine):
             continue
+        tuned.add(key)
+        start = table[key]["params"] if args.refine and key in table else None
         print(f"tuning {key} ({ps.nx}x{ps.ny}x{ps.nz})", flush=True)
-        table[key] = dict(tune(ctx, ps, args.rounds), grid=[ps.nx, ps.ny, ps.nz])
+        table[key] = dict(tune(ctx, ps, args.rounds, start=start), grid=[ps.nx, ps.ny, ps.nz])
         with open(args.out, "w") as f:
             json.dump(table, f, indent=1, sort_keys=True)
     ctx.close()

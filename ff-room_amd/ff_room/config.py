@@ -112,7 +112,7 @@ class SolverConfig:
     precision: str = "fp64"
     mg_levels: int = 0
     mg_bf16: bool = True
-    inner_tol: float = 1e-3
+    inner_tol: float = 1e-4
     inner_max_iter: int = 60
 
 

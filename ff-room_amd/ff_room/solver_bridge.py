@@ -97,7 +97,7 @@ def make_params(config: SceneConfig, c):
     p.precision = c.Precision.MIXED if prec == "mixed" else c.Precision.FP64
     p.mg_levels = int(getattr(sc, "mg_levels", 0) or 0)
     p.mg_bf16 = bool(getattr(sc, "mg_bf16", True))
-    p.inner_tol = float(getattr(sc, "inner_tol", 1e-3))
+    p.inner_tol = float(getattr(sc, "inner_tol", 1e-4))
     p.inner_max_iter = int(getattr(sc, "inner_max_iter", 60))
     return p
 
