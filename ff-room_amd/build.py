@@ -24,7 +24,7 @@ INCLUDE = os.path.join(os.path.dirname(HERE), "include")
 ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
 
 HIP_SOURCES = [
-    "ffr_capi.hip", "ffr_scene.hip", "ffr_task.hip", "ffr_mg.hip", "ffr_serve.hip",
+    "ffr_capi.hip", "ffr_scene.hip", "ffr_task.hip", "ffr_mg.hip", "ffr_serve.hip", "ffr_gang.hip",
     "kernels/step_kernels.hip", "kernels/cg_kernels.hip", "kernels/mg_kernels.hip",
 ]
 HOST_SOURCES = [

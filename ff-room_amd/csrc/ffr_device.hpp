@@ -145,15 +145,29 @@ __device__ __forceinline__ uint32_t widx(const Geo &g, int i, int j, int k)
     return ((uint32_t)i * g.ny + j) * (g.nz + 1) + k;
 }
 
-// Block index remapped so that blocks dispatched to the same XCD
-// (b mod 8) own one contiguous range of logical blocks.
-__device__ __forceinline__ uint32_t xcd_block()
+// Level-0 gather kernels are launched with up to 2048 blocks of 4 waves, 8
+// waves per SIMD on 256 CUs: asking for 8 waves per EU (<= 64 VGPRs) keeps
+// every block resident, so no block waits for a second round.
+#define FFR_RESIDENT __attribute__((amdgpu_waves_per_eu(8)))
+
+// Block b of G remapped so that blocks dispatched to the same XCD (b mod 8)
+// own one contiguous range of logical blocks.
+__device__ __forceinline__ uint32_t xcd_remap(uint32_t b, uint32_t G)
 {
-    const uint32_t b = blockIdx.x, G = gridDim.x;
     const uint32_t q = G / kXcds, r = G % kXcds;
     const uint32_t x = b % kXcds, slot = b / kXcds;
     return (x < r) ? x * (q + 1) + slot : r * (q + 1) + (x - r) * q + slot;
 }
+__device__ __forceinline__ uint32_t xcd_block() { return xcd_remap(blockIdx.x, gridDim.x); }
+
+// A block's place in the launch of one scene: its logical (XCD-remapped)
+// block, the number of blocks of that launch and its raw index. A plain
+// launch takes them from the hardware; a gang launch, which runs the same
+// launch of several scenes as one grid, from its table (ffr_gang.hpp).
+struct VB {
+    uint32_t blk, G, raw;
+};
+__device__ __forceinline__ VB vb_here() { return VB{xcd_block(), gridDim.x, blockIdx.x}; }
 
 // Contiguous element range [b, e) of logical block `blk` when n elements are
 // split over gridDim.x blocks in whole 256-element chunks.
@@ -193,14 +207,15 @@ inline FaceSections face_sections(uint64_t nu, uint64_t nv, uint64_t nw, uint32_
     if (gu + gv >= G) gu = gv = G / 3 > 0 ? G / 3 : 1;
     return {gu, gv};
 }
-// section of logical block blk: axis, block index within it, its block count
-__device__ __forceinline__ int face_section(const FaceSections &fs, uint32_t blk, uint32_t &lb,
-                                            uint32_t &G)
+// section of logical block blk of a grid of Gall: axis, block index within
+// it, its block count
+__device__ __forceinline__ int face_section_of(const FaceSections &fs, uint32_t blk, uint32_t Gall,
+                                               uint32_t &lb, uint32_t &G)
 {
     if (blk < fs.gu) { lb = blk; G = fs.gu; return 0; }
     if (blk < fs.gu + fs.gv) { lb = blk - fs.gu; G = fs.gv; return 1; }
     lb = blk - fs.gu - fs.gv;
-    G = gridDim.x - fs.gu - fs.gv;
+    G = Gall - fs.gu - fs.gv;
     return 2;
 }
 
@@ -319,11 +334,19 @@ struct ProfEvents {
 };
 extern thread_local ProfEvents g_prof_events;
 
+// While a gang records (ffr_gang.hpp), launches are recorded, not made; a
+// kernel without a gang form marks the record unusable.
+struct Recorder;
+extern thread_local Recorder *t_rec;
+void rec_unsupported();
+
 }  // namespace ffr
 
 #define FFR_LAUNCH(kernel, grid, block, shm, stream, ...)                                   \
     do {                                                                                    \
-        if (ffr::g_prof_events.start) {                                                     \
+        if (ffr::t_rec) {                                                                   \
+            ffr::rec_unsupported();                                                         \
+        } else if (ffr::g_prof_events.start) {                                              \
             hipEvent_t a_ = ffr::g_prof_events.start, b_ = ffr::g_prof_events.stop;        \
             ffr::g_prof_events = ffr::ProfEvents{};                                         \
             hipExtLaunchKernelGGL(kernel, grid, block, shm, stream, a_, b_, 0, __VA_ARGS__); \

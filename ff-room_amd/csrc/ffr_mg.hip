@@ -1,6 +1,7 @@
 // ffr_mg.hip - host side of the multigrid-preconditioned pressure solve
 // (method in ffr_mg.hpp): hierarchy set-up on the device and the enqueue
 // logic of the outer FP64 / inner FP32 iteration.
+#include "ffr_gang.hpp"
 #include "ffr_scene.hpp"
 #include "ffr_mg_launch.hpp"
 
@@ -516,7 +517,7 @@ static hipGraphExec_t capture(ffr_scene *s, void (*body)(ffr_scene *))
 static bool graphs_ready(ffr_scene *s)
 {
     Multigrid &M = s->mg;
-    if (!s->prm.graphs || s->ctx->prof_on) return false;
+    if (!s->prm.graphs || s->ctx->prof_on || t_rec) return false;
     if (!M.graphs_tried) {
         M.graphs_tried = true;
         M.g_iter = capture(s, inner_iteration);
@@ -530,6 +531,12 @@ static void launch_iterations(ffr_scene *s, int k)
 {
     if (k > 0 && graphs_ready(s)) {
         for (int i = 0; i < k; i++) hipGraphLaunch(s->mg.g_iter, s->stream);
+        return;
+    }
+    if (k > 0 && t_rec) {  // recorded once, repeated with the same arguments
+        const size_t b = t_rec->ops.size();
+        inner_iteration(s);
+        t_rec->repeat(b, t_rec->ops.size(), k - 1);
         return;
     }
     for (int i = 0; i < k; i++) inner_iteration(s);

@@ -158,8 +158,40 @@ int task_advance(ffr_scene *s, bool block, bool *done);
 bool task_busy(const ffr_scene *s);
 int scene_step(ffr_scene *s, ffr_step_result *out);
 int wait_point(ffr_scene *s, void *host, const void *dev, size_t bytes);
+// one iteration of a task whose last event the device has reached: handles
+// it and enqueues (or records) the next part; *done when the step finished
+int task_resume(ffr_scene *s, bool *done);
+// what must exist before a scene's step can be recorded for a gang (the
+// multigrid hierarchy); false when the scene cannot take part in a gang
+int task_prepare_gang(ffr_scene *s, bool *eligible);
 // serving (ffr_serve.hip)
 void serve_forget(ffr_ctx *c, ffr_scene *s);
+
+struct Recorder;
+// Several scenes' step tasks advanced in rounds of merged launches on one
+// stream (ffr_gang.hip).
+struct Gang {
+    ffr_ctx *ctx = nullptr;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev = nullptr;
+    uint8_t *d_tab = nullptr, *h_stage = nullptr;  // tables: device / pinned staging
+    size_t cap = 0;
+    std::vector<uint8_t> stage_buf;
+    bool in_flight = false;
+    // rounds, launches issued, operations recorded by the members, the
+    // longest member's operations (summed over rounds), host time spent
+    // recording and issuing rounds
+    unsigned long long rounds = 0, merged_launches = 0, recorded_ops = 0, longest_ops = 0;
+    double host_us = 0.0;
+    Gang() = default;
+    Gang(const Gang &) = delete;
+    Gang &operator=(const Gang &) = delete;
+    ~Gang();
+    int init(ffr_ctx *c);
+    void release();
+    int reserve(size_t bytes);
+    int launch_round(const std::vector<Recorder *> &recs);
+};
 // multigrid (ffr_mg.hip)
 size_t mg_bytes(int nx, int ny, int nz, const double h0[3], const ffr_params &p);
 int mg_create(ffr_scene *s);

@@ -10,11 +10,14 @@
 // the device. Admission by HBM footprint happens where the memory is taken:
 // ffr_scene_create refuses a scene beyond the context's budget, and the
 // caller creates scenes as earlier ones are destroyed.
+#include "ffr_gang.hpp"
 #include "ffr_scene.hpp"
 
 #include <algorithm>
+#include <chrono>
 #include <deque>
 #include <map>
+#include <memory>
 #include <thread>
 #include <vector>
 
@@ -27,18 +30,173 @@ struct ffr_ctx::Server {
         bool active = false, done = false;
         int rc = FFR_OK;
         ffr_step_result last{};
+        int gang = -2;  // -2: not yet placed, -1: on the scene's own stream
+        std::unique_ptr<Recorder> rec;
     };
     std::map<uint64_t, Ticket> tickets;
     std::deque<uint64_t> waiting;
     std::vector<uint64_t> active;
     uint64_t next_id = 1;
     int slots = 16;
+    // gangs (ffr_gang.hip): 0 = every scene on its own stream
+    int ngangs = 0;
+    std::vector<std::unique_ptr<Gang>> gangs;
+    std::vector<std::vector<uint64_t>> members;
+    ffr_ctx *ctx = nullptr;
 
     void finish(Ticket &t, int rc)
     {
         t.done = true;
         t.active = false;
         t.rc = rc;
+    }
+
+    // a step of ticket t completed: bookkeeping; whether it goes on
+    bool step_done(Ticket &t, int &completed)
+    {
+        completed++;
+        t.steps++;
+        t.last = t.scene->task.result;
+        if (t.last.converged || t.steps >= t.max_steps) {
+            finish(t, FFR_OK);
+            return false;
+        }
+        return true;
+    }
+
+    int set_gangs(int n)
+    {
+        for (const uint64_t id : active)
+            if (tickets[id].gang != -2) {
+                ctx->fail("ffr_serve_gangs: tickets are in flight");
+                return FFR_E_STATE;
+            }
+        gangs.clear();
+        members.assign(n, {});
+        for (int g = 0; g < n; g++) {
+            gangs.emplace_back(new Gang());
+            const int rc = gangs.back()->init(ctx);
+            if (rc != FFR_OK) {
+                gangs.clear();
+                members.clear();
+                ngangs = 0;
+                return rc;
+            }
+        }
+        ngangs = n;
+        return FFR_OK;
+    }
+
+    // places newly admitted tickets: a gang when the scene can take part
+    void place()
+    {
+        for (const uint64_t id : active) {
+            Ticket &t = tickets[id];
+            if (t.gang != -2 || t.done) continue;
+            t.gang = -1;
+            if (ngangs == 0) continue;
+            bool eligible = false;
+            const int rc = task_prepare_gang(t.scene, &eligible);
+            if (rc != FFR_OK) {
+                finish(t, rc);
+                continue;
+            }
+            if (!eligible) continue;
+            int g = 0;
+            for (int k = 1; k < ngangs; k++)
+                if (members[k].size() < members[g].size()) g = k;
+            t.gang = g;
+            if (!t.rec) t.rec.reset(new Recorder());
+            members[g].push_back(id);
+        }
+    }
+
+    // one round of gang g, whose previous round (if any) has completed: every
+    // member resumes or begins its step with its recorder active, then the
+    // records go out as merged launches
+    int gang_round(int g, int &completed)
+    {
+        Gang &G = *gangs[g];
+        const auto t0 = std::chrono::steady_clock::now();
+        struct HostTime {  // host time of the round, however it ends
+            Gang &G;
+            std::chrono::steady_clock::time_point t0;
+            ~HostTime()
+            {
+                G.host_us += std::chrono::duration<double, std::micro>(
+                                 std::chrono::steady_clock::now() - t0).count();
+            }
+        } ht{G, t0};
+        std::vector<Recorder *> recs;
+        for (const uint64_t id : members[g]) {
+            Ticket &t = tickets[id];
+            ffr_scene *s = t.scene;
+            Recorder &r = *t.rec;
+            r.clear();
+            t_rec = &r;
+            int rc = FFR_OK;
+            bool go = true;
+            if (task_busy(s)) {
+                bool done = false;
+                rc = task_resume(s, &done);
+                go = rc == FFR_OK && done && step_done(t, completed);
+            } else if (t.steps >= t.max_steps) {
+                finish(t, FFR_OK);
+                go = false;
+            }
+            if (rc == FFR_OK && go) rc = task_begin(s, false);
+            t_rec = nullptr;
+            if (rc == FFR_OK && r.unsupported) {
+                ctx->fail("gang: a kernel without a gang form was reached");
+                rc = FFR_E_UNSUPPORTED;
+            }
+            if (rc != FFR_OK) {
+                s->task.phase = StepTask::IDLE;
+                finish(t, rc);
+                continue;
+            }
+            if (!t.done && !r.ops.empty()) recs.push_back(&r);
+        }
+        members[g].erase(std::remove_if(members[g].begin(), members[g].end(),
+                                        [&](uint64_t id) { return tickets[id].done; }),
+                         members[g].end());
+        if (recs.empty()) return FFR_OK;
+        const int rc = G.launch_round(recs);
+        if (rc != FFR_OK) {
+            hipStreamSynchronize(G.stream);
+            for (const uint64_t id : members[g]) {
+                tickets[id].scene->task.phase = StepTask::IDLE;
+                finish(tickets[id], rc);
+            }
+            members[g].clear();
+            return rc;
+        }
+        G.in_flight = true;
+        return FFR_OK;
+    }
+
+    int progress_gangs()
+    {
+        int completed = 0;
+        for (int g = 0; g < ngangs; g++) {
+            Gang &G = *gangs[g];
+            if (G.in_flight) {
+                const hipError_t q = hipEventQuery(G.ev);
+                if (q == hipErrorNotReady) continue;
+                G.in_flight = false;
+                if (q != hipSuccess) {
+                    ctx->fail("hipEventQuery failed: %s", hipGetErrorString(q));
+                    for (const uint64_t id : members[g]) {
+                        tickets[id].scene->task.phase = StepTask::IDLE;
+                        finish(tickets[id], FFR_E_DEVICE);
+                    }
+                    members[g].clear();
+                    continue;
+                }
+            }
+            if (!members[g].empty()) gang_round(g, completed);
+        }
+        return completed;
     }
 
     void admit()
@@ -57,9 +215,11 @@ struct ffr_ctx::Server {
     int progress()
     {
         admit();
-        int completed = 0;
+        place();
+        int completed = ngangs > 0 ? progress_gangs() : 0;
         for (const uint64_t id : active) {
             Ticket &t = tickets[id];
+            if (t.done || t.gang >= 0) continue;
             ffr_scene *s = t.scene;
             if (!task_busy(s)) {
                 if (t.steps >= t.max_steps) {
@@ -77,13 +237,7 @@ struct ffr_ctx::Server {
                 continue;
             }
             if (!done) continue;
-            completed++;
-            t.steps++;
-            t.last = s->task.result;
-            if (t.last.converged || t.steps >= t.max_steps) {
-                finish(t, FFR_OK);
-                continue;
-            }
+            if (!step_done(t, completed)) continue;
             const int rb = task_begin(s, false);  // the scene's next step, at once
             if (rb != FFR_OK) finish(t, rb);
         }
@@ -91,6 +245,7 @@ struct ffr_ctx::Server {
                                     [&](uint64_t id) { return tickets[id].done; }),
                      active.end());
         admit();
+        place();
         return completed;
     }
 
@@ -105,6 +260,7 @@ void ffr_ctx::serve_shutdown()
             ffr_scene *s = server->tickets[id].scene;
             if (s) hipStreamSynchronize(s->stream);
         }
+        for (auto &g : server->gangs) hipStreamSynchronize(g->stream);
     }
     delete server;
     server = nullptr;
@@ -119,6 +275,13 @@ void serve_forget(ffr_ctx *c, ffr_scene *s)
     auto *sv = c->server;
     for (auto &kv : sv->tickets) {
         if (kv.second.scene != s) continue;
+        const int g = kv.second.gang;
+        if (g >= 0 && g < sv->ngangs) {  // its gang's launches may still use the scene
+            hipStreamSynchronize(sv->gangs[g]->stream);
+            auto &m = sv->members[g];
+            m.erase(std::remove(m.begin(), m.end(), kv.first), m.end());
+            s->task.phase = StepTask::IDLE;
+        }
         if (!kv.second.done) sv->finish(kv.second, FFR_E_STATE);
         kv.second.scene = nullptr;
     }
@@ -133,7 +296,10 @@ void serve_forget(ffr_ctx *c, ffr_scene *s)
 
 static ffr_ctx::Server *server_of(ffr_ctx *c)
 {
-    if (!c->server) c->server = new (std::nothrow) ffr_ctx::Server();
+    if (!c->server) {
+        c->server = new (std::nothrow) ffr_ctx::Server();
+        if (c->server) c->server->ctx = c;
+    }
     return c->server;
 }
 
@@ -146,6 +312,29 @@ int ffr_serve_config(ffr_ctx *c, int slots)
     if (!sv) return FFR_E_NOMEM;
     sv->slots = slots;
     return FFR_OK;
+}
+
+int ffr_serve_gang_stats(ffr_ctx *c, ffr_gang_stats *out)
+{
+    if (!c || !out) return FFR_E_INVALID;
+    *out = ffr_gang_stats{};
+    if (c->server)
+        for (const auto &g : c->server->gangs) {
+            out->rounds += g->rounds;
+            out->launches += g->merged_launches;
+            out->ops += g->recorded_ops;
+            out->longest_ops += g->longest_ops;
+            out->host_us += g->host_us;
+        }
+    return FFR_OK;
+}
+
+int ffr_serve_gangs(ffr_ctx *c, int gangs)
+{
+    if (!c || gangs < 0 || gangs > 64) return FFR_E_INVALID;
+    auto *sv = server_of(c);
+    if (!sv) return FFR_E_NOMEM;
+    return sv->set_gangs(gangs);
 }
 
 int ffr_submit(ffr_ctx *c, ffr_scene *s, int max_steps, uint64_t *ticket)
@@ -163,10 +352,9 @@ int ffr_submit(ffr_ctx *c, ffr_scene *s, int max_steps, uint64_t *ticket)
         return FFR_E_STATE;
     }
     const uint64_t id = sv->next_id++;
-    ffr_ctx::Server::Ticket t;
+    ffr_ctx::Server::Ticket &t = sv->tickets[id];
     t.scene = s;
     t.max_steps = max_steps;
-    sv->tickets[id] = t;
     sv->waiting.push_back(id);
     sv->admit();
     *ticket = id;

@@ -11,6 +11,7 @@
 // device-side stopping test ended the solve return at once. A blocking
 // caller (ffr_step) waits on each event; the serving loop polls them, so the
 // steps of many scenes overlap on the device.
+#include "ffr_gang.hpp"
 #include "ffr_scene.hpp"
 
 #include <algorithm>
@@ -21,6 +22,10 @@ namespace ffr {
 // task's event after it.
 int wait_point(ffr_scene *s, void *host, const void *dev, size_t bytes)
 {
+    if (t_rec) {  // a gang member: its round's gather copies, its event ends the round
+        t_rec->push_copy(host, dev, (uint32_t)bytes);
+        return FFR_OK;
+    }
     hipStream_t st = s->stream;
     HIP_CHECK_CTX(s->ctx, hipMemcpyAsync(host, dev, bytes, hipMemcpyDeviceToHost, st), FFR_E_DEVICE);
     HIP_CHECK_CTX(s->ctx, hipEventRecord(s->task.ev, st), FFR_E_DEVICE);
@@ -209,6 +214,34 @@ int task_begin(ffr_scene *s, bool solve_only)
     return FFR_OK;
 }
 
+int task_resume(ffr_scene *s, bool *done)
+{
+    StepTask &t = s->task;
+    *done = false;
+    if (t.phase == StepTask::SOLVE) {
+        bool finished = false;
+        int rc = solve_continue(s, &finished);
+        if (rc == FFR_OK && finished) {
+            if (t.solve_only) rc = wait_point(s, s->h_rec, s->rec, sizeof(StepRec));
+            else rc = s->fp64 ? enqueue_finish<double>(s) : enqueue_finish<float>(s);
+            t.phase = StepTask::RESULT;
+        }
+        if (rc != FFR_OK) {
+            if (!t_rec) hipStreamSynchronize(s->stream);
+            t.phase = StepTask::IDLE;
+        }
+        return rc;
+    }
+    // RESULT: the step's record is on the host
+    if (!t.solve_only) {
+        s->step++;
+        build_result(s);
+    }
+    t.phase = StepTask::IDLE;
+    *done = true;
+    return FFR_OK;
+}
+
 int task_advance(ffr_scene *s, bool block, bool *done)
 {
     StepTask &t = s->task;
@@ -230,30 +263,24 @@ int task_advance(ffr_scene *s, bool block, bool *done)
                 return FFR_E_DEVICE;
             }
         }
-        if (t.phase == StepTask::SOLVE) {
-            bool finished = false;
-            int rc = solve_continue(s, &finished);
-            if (rc == FFR_OK && finished) {
-                if (t.solve_only) rc = wait_point(s, s->h_rec, s->rec, sizeof(StepRec));
-                else rc = s->fp64 ? enqueue_finish<double>(s) : enqueue_finish<float>(s);
-                t.phase = StepTask::RESULT;
-            }
-            if (rc != FFR_OK) {
-                hipStreamSynchronize(s->stream);
-                t.phase = StepTask::IDLE;
-                return rc;
-            }
-            continue;  // wait for (or poll) the event just recorded
-        }
-        // RESULT: the step's record is on the host
-        if (!t.solve_only) {
-            s->step++;
-            build_result(s);
-        }
-        t.phase = StepTask::IDLE;
-        *done = true;
-        return FFR_OK;
+        const int rc = task_resume(s, done);
+        if (rc != FFR_OK || *done) return rc;
+        // wait for (or poll) the event just recorded
     }
+}
+
+int task_prepare_gang(ffr_scene *s, bool *eligible)
+{
+    // mixed precision with the multigrid-preconditioned solve: every kernel
+    // of its step has a gang form; the FP64 form and plain CG run alone
+    *eligible = false;
+    if (s->fp64 || use_cg(s) || s->ctx->prof_on) return FFR_OK;
+    if (!s->mg.levels) {
+        const int rc = mg_create(s);
+        if (rc != FFR_OK) return rc;
+    }
+    *eligible = true;
+    return FFR_OK;
 }
 
 int scene_step(ffr_scene *s, ffr_step_result *out)

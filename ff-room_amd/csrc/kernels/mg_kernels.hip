@@ -5,6 +5,7 @@
 // order). Every kernel of the inner iteration returns at once when the
 // solve or the inner iteration has stopped, so the host may enqueue ahead.
 #include "../ffr_kernels.hpp"
+#include "../ffr_gang.hpp"
 #include "../ffr_mg.hpp"
 #include "../ffr_mg_launch.hpp"
 
@@ -247,10 +248,6 @@ __global__ void __launch_bounds__(kBlock) k_mg_build(Geo g, MgLevel F, MgLevel C
 // neighbours, none OUTFLOW) skip the per-face selects and the diagonal
 // count; their arithmetic is the general formula's with every face present.
 constexpr int kU = 4;
-// Level-0 gather kernels are launched with up to 2048 blocks of 4 waves, 8
-// waves per SIMD on 256 CUs: asking for 8 waves per EU (<= 64 VGPRs) keeps
-// every block resident, so no block waits for a second round.
-#define FFR_RESIDENT __attribute__((amdgpu_waves_per_eu(8)))
 // cells per thread of the gather kernels with the most values in flight
 // (the visiting order of a thread's cells, and so its partial sums, is the
 // same for any unroll: s, s + kBlock, s + 2 kBlock, ...)
@@ -298,15 +295,14 @@ __device__ __forceinline__ float diag_of(const Geo &g, uint16_t nb, float dint)
 // own slot h adds r z of that slot after its update and r z of the other
 // colour's slot h (the black array is never longer than the red one).
 template <bool DOT>
-__global__ void __launch_bounds__(kBlock) FFR_RESIDENT k_mg_smooth0(Geo g, MgLevel L,
-                                                       const uint16_t *__restrict__ code,
-                                                       const float *__restrict__ zo,
-                                                       float *__restrict__ zc,
-                                                       const float *__restrict__ rc,
-                                                       const float *__restrict__ ro, uint32_t cnt,
-                                                       uint32_t cnt_o, uint32_t col,
-                                                       int zero_start, const MgState *m,
-                                                       RedScratch red)
+struct KB_mg_smooth0 {
+    static __device__ __forceinline__ void run(const VB &vb, const Geo &g, const MgLevel &L,
+                                               const uint16_t *__restrict__ code,
+                                               const float *__restrict__ zo, float *__restrict__ zc,
+                                               const float *__restrict__ rc,
+                                               const float *__restrict__ ro, uint32_t cnt,
+                                               uint32_t cnt_o, uint32_t col, int zero_start,
+                                               const MgState *m, const RedScratch &red)
 {
     __shared__ double lds[kBlock / kWave];
     const uint32_t stop = stop_flags(m);  // tested after the first loads
@@ -315,10 +311,10 @@ __global__ void __launch_bounds__(kBlock) FFR_RESIDENT k_mg_smooth0(Geo g, MgLev
     const auto bcode = buf_rsrc(code, 2u * cnt), bz = buf_rsrc(zc, 4u * cnt),
                br = buf_rsrc(rc, 4u * cnt), bo = buf_rsrc(zo, 4u * cnt_o),
                bro = buf_rsrc(ro, 4u * cnt_o);
-    const uint32_t blk = xcd_block();
+    const uint32_t blk = vb.blk;
     double acc = 0.0;
     uint32_t sb, se;
-    block_span(cnt, blk, sb, se);
+    block_span_of(cnt, blk, vb.G, sb, se);
     for (uint32_t h0 = sb + threadIdx.x; h0 < se; h0 += kU * kBlock) {
         uint16_t nb[kU];
         float rv[kU], zv[kU][6], rov[kU], zov[kU];
@@ -348,6 +344,20 @@ __global__ void __launch_bounds__(kBlock) FFR_RESIDENT k_mg_smooth0(Geo g, MgLev
         }
     }
     if (DOT) block_partial(acc, pslot(red, SLOT_MG_A), blk, lds);
+}
+};
+template <bool DOT>
+__global__ void __launch_bounds__(kBlock) FFR_RESIDENT k_mg_smooth0(Geo g, MgLevel L,
+                                                       const uint16_t *__restrict__ code,
+                                                       const float *__restrict__ zo,
+                                                       float *__restrict__ zc,
+                                                       const float *__restrict__ rc,
+                                                       const float *__restrict__ ro, uint32_t cnt,
+                                                       uint32_t cnt_o, uint32_t col,
+                                                       int zero_start, const MgState *m,
+                                                       RedScratch red)
+{
+    KB_mg_smooth0<DOT>::run(vb_here(), g, L, code, zo, zc, rc, ro, cnt, cnt_o, col, zero_start, m, red);
 }
 
 // Coarse levels (natural layout, stored operator). A coupling is 0 across a
@@ -383,9 +393,10 @@ __device__ __forceinline__ float coarse_row(const CoarseOp<T> &o, __amdgpu_buffe
 }
 
 template <typename T>
-__global__ void __launch_bounds__(kBlock) k_mg_smooth(MgLevel L, T *__restrict__ z,
-                                                      const T *__restrict__ r, int color,
-                                                      int zero_start, const MgState *m)
+struct KB_mg_smooth {
+    static __device__ __forceinline__ void run(const VB &vb, const MgLevel &L, T *__restrict__ z,
+                                               const T *__restrict__ r, int color, int zero_start,
+                                               const MgState *m)
 {
     const uint32_t stop = stop_flags(m);
     const T *tag = nullptr;
@@ -393,7 +404,7 @@ __global__ void __launch_bounds__(kBlock) k_mg_smooth(MgLevel L, T *__restrict__
     const CoarseOp<T> o = coarse_op<T>(L);
     const auto bz = buf_rsrc(z, (uint32_t)sizeof(T) * L.n), br = buf_rsrc(r, (uint32_t)sizeof(T) * L.n);
     uint32_t sb, se;
-    block_span(L.n, xcd_block(), sb, se);
+    block_span_of(L.n, vb.blk, vb.G, sb, se);
     for (uint32_t c = sb + threadIdx.x; c < se; c += kBlock) {
         int i, j, k;
         decode(c, L.f_nynz, L.f_nz, L.ny, L.nz, i, j, k);
@@ -408,6 +419,14 @@ __global__ void __launch_bounds__(kBlock) k_mg_smooth(MgLevel L, T *__restrict__
         }
         bstv(bz, c, a ? (rv + of) * rd : 0.f, tag);
     }
+}
+};
+template <typename T>
+__global__ void __launch_bounds__(kBlock) k_mg_smooth(MgLevel L, T *__restrict__ z,
+                                                      const T *__restrict__ r, int color,
+                                                      int zero_start, const MgState *m)
+{
+    KB_mg_smooth<T>::run(vb_here(), L, z, r, color, zero_start, m);
 }
 
 // ---- fused residual + restriction: rc = P^T (r - A z) ------------------------
@@ -427,13 +446,14 @@ __device__ __forceinline__ void coarse_first_red(const MgLevel &C, TC *zc, uint3
 }
 
 template <typename TF, typename TC, bool L0>
-__global__ void __launch_bounds__(kBlock) k_mg_restrict(Geo g, MgLevel F, MgLevel C, const TF *z,
-                                                        const TF *r, TC *rc, TC *zc, int red0,
-                                                        const MgState *m)
+struct KB_mg_restrict {
+    static __device__ __forceinline__ void run(const VB &vb, const Geo &g, const MgLevel &F,
+                                               const MgLevel &C, const TF *z, const TF *r, TC *rc,
+                                               TC *zc, int red0, const MgState *m)
 {
     const uint32_t stop = stop_flags(m);  // tested after the first loads
     uint32_t sb, se;
-    block_span(C.n * 8u, xcd_block(), sb, se);  // spans are whole multiples of 8
+    block_span_of(C.n * 8u, vb.blk, vb.G, sb, se);  // spans are whole multiples of 8
     if (L0) {
         // level 0 (split layout): the colour of a child is (a+b+d)&1, so all
         // loads of kU children are issued before any is used
@@ -514,6 +534,14 @@ __global__ void __launch_bounds__(kBlock) k_mg_restrict(Geo g, MgLevel F, MgLeve
         }
     }
 }
+};
+template <typename TF, typename TC, bool L0>
+__global__ void __launch_bounds__(kBlock) k_mg_restrict(Geo g, MgLevel F, MgLevel C, const TF *z,
+                                                        const TF *r, TC *rc, TC *zc, int red0,
+                                                        const MgState *m)
+{
+    KB_mg_restrict<TF, TC, L0>::run(vb_here(), g, F, C, z, r, rc, zc, red0, m);
+}
 
 // ---- level-0 restriction in two passes --------------------------------------
 // (1) res = r - A z on every slot of the split layout (one thread per slot,
@@ -521,9 +549,11 @@ __global__ void __launch_bounds__(kBlock) k_mg_restrict(Geo g, MgLevel F, MgLeve
 //     cell summing its up-to-8 children, whose split slots are contiguous
 //     across consecutive coarse cells. res lives in the inner PCG's q, which
 //     is not in use while the preconditioner runs.
-__global__ void __launch_bounds__(kBlock) FFR_RESIDENT k_mg_resid0(Geo g, MgLevel L, const float *__restrict__ z,
-                                                      const float *__restrict__ r,
-                                                      float *__restrict__ res, const MgState *m)
+struct KB_mg_resid0 {
+    static __device__ __forceinline__ void run(const VB &vb, const Geo &g, const MgLevel &L,
+                                               const float *__restrict__ z,
+                                               const float *__restrict__ r, float *__restrict__ res,
+                                               const MgState *m)
 {
     const uint32_t stop = stop_flags(m);
     const uint32_t H = L.H, si = (uint32_t)g.ny * g.nz, sj = g.nz;
@@ -532,7 +562,7 @@ __global__ void __launch_bounds__(kBlock) FFR_RESIDENT k_mg_resid0(Geo g, MgLeve
                br = buf_rsrc(r, 4u * g.nc), bred = buf_rsrc(z, 4u * H),
                bblack = buf_rsrc(z + H, 4u * (g.nc - H)), bres = buf_rsrc(res, 4u * g.nc);
     uint32_t sb, se;
-    block_span(g.nc, xcd_block(), sb, se);
+    block_span_of(g.nc, vb.blk, vb.G, sb, se);
     for (uint32_t s0 = sb + threadIdx.x; s0 < se; s0 += kU2 * kBlock) {
         uint16_t nb[kU2];
         float rv[kU2], zs[kU2], zv[kU2][6];
@@ -558,19 +588,26 @@ __global__ void __launch_bounds__(kBlock) FFR_RESIDENT k_mg_resid0(Geo g, MgLeve
         }
     }
 }
+};
+__global__ void __launch_bounds__(kBlock) FFR_RESIDENT k_mg_resid0(Geo g, MgLevel L, const float *__restrict__ z,
+                                                      const float *__restrict__ r,
+                                                      float *__restrict__ res, const MgState *m)
+{
+    KB_mg_resid0::run(vb_here(), g, L, z, r, res, m);
+}
 
 template <typename TC>
-__global__ void __launch_bounds__(kBlock) k_mg_restrict_sum0(Geo g, MgLevel F, MgLevel C,
-                                                             const float *__restrict__ res,
-                                                             TC *__restrict__ rc,
-                                                             TC *__restrict__ zc, int red0,
-                                                             const MgState *m)
+struct KB_mg_restrict_sum0 {
+    static __device__ __forceinline__ void run(const VB &vb, const Geo &g, const MgLevel &F,
+                                               const MgLevel &C, const float *__restrict__ res,
+                                               TC *__restrict__ rc, TC *__restrict__ zc, int red0,
+                                               const MgState *m)
 {
     const uint32_t stop = stop_flags(m);
     const uint32_t H = F.H;
     const auto bred = buf_rsrc(res, 4u * H), bblack = buf_rsrc(res + H, 4u * (g.nc - H));
     uint32_t sb, se;
-    block_span(C.n, xcd_block(), sb, se);
+    block_span_of(C.n, vb.blk, vb.G, sb, se);
     for (uint32_t I = sb + threadIdx.x; I < se; I += kBlock) {
         int ic, jc, kc;
         decode(I, C.f_nynz, C.f_nz, C.ny, C.nz, ic, jc, kc);
@@ -595,6 +632,16 @@ __global__ void __launch_bounds__(kBlock) k_mg_restrict_sum0(Geo g, MgLevel F, M
         if (red0) coarse_first_red<TC>(C, zc, I, ic, jc, kc, v);
     }
 }
+};
+template <typename TC>
+__global__ void __launch_bounds__(kBlock) k_mg_restrict_sum0(Geo g, MgLevel F, MgLevel C,
+                                                             const float *__restrict__ res,
+                                                             TC *__restrict__ rc,
+                                                             TC *__restrict__ zc, int red0,
+                                                             const MgState *m)
+{
+    KB_mg_restrict_sum0<TC>::run(vb_here(), g, F, C, res, rc, zc, red0, m);
+}
 
 // ---- prolongation fused into the first post-smoothing sweep ----------------
 // After the coarse correction the first post pass is the black sweep; it reads
@@ -613,9 +660,10 @@ __device__ __forceinline__ uint32_t parent_of(const MgLevel &C, int i, int j, in
 
 // level 0 (split layout): one thread per black slot, kU slots per thread
 template <typename TC>
-__global__ void __launch_bounds__(kBlock) FFR_RESIDENT k_mg_post_black0(Geo g, MgLevel L, MgLevel C,
-                                                           const TC *__restrict__ zc,
-                                                           const MgState *m)
+struct KB_mg_post_black0 {
+    static __device__ __forceinline__ void run(const VB &vb, const Geo &g, const MgLevel &L,
+                                               const MgLevel &C, const TC *__restrict__ zc,
+                                               const MgState *m)
 {
     const uint32_t stop = stop_flags(m);
     const uint32_t H = L.H, cnt = g.nc - H;
@@ -627,7 +675,7 @@ __global__ void __launch_bounds__(kBlock) FFR_RESIDENT k_mg_post_black0(Geo g, M
                bred = buf_rsrc(L.z, 4u * H), bzc = buf_rsrc(zc, (uint32_t)sizeof(TC) * C.n),
                bz = buf_rsrc(zb, 4u * cnt);
     uint32_t sb, se;
-    block_span(cnt, xcd_block(), sb, se);
+    block_span_of(cnt, vb.blk, vb.G, sb, se);
     for (uint32_t h0 = sb + threadIdx.x; h0 < se; h0 += kU2 * kBlock) {
         uint16_t nb[kU2];
         float rv[kU2], zv[kU2][6];
@@ -658,13 +706,21 @@ __global__ void __launch_bounds__(kBlock) FFR_RESIDENT k_mg_post_black0(Geo g, M
         }
     }
 }
+};
+template <typename TC>
+__global__ void __launch_bounds__(kBlock) FFR_RESIDENT k_mg_post_black0(Geo g, MgLevel L, MgLevel C,
+                                                           const TC *__restrict__ zc,
+                                                           const MgState *m)
+{
+    KB_mg_post_black0<TC>::run(vb_here(), g, L, C, zc, m);
+}
 
 // coarse levels (natural layout)
 template <typename T, typename TC>
-__global__ void __launch_bounds__(kBlock) k_mg_post_black(MgLevel L, MgLevel C, T *__restrict__ z,
-                                                          const T *__restrict__ r,
-                                                          const TC *__restrict__ zc,
-                                                          const MgState *m)
+struct KB_mg_post_black {
+    static __device__ __forceinline__ void run(const VB &vb, const MgLevel &L, const MgLevel &C,
+                                               T *__restrict__ z, const T *__restrict__ r,
+                                               const TC *__restrict__ zc, const MgState *m)
 {
     const uint32_t stop = stop_flags(m);
     const T *tag = nullptr;
@@ -674,7 +730,7 @@ __global__ void __launch_bounds__(kBlock) k_mg_post_black(MgLevel L, MgLevel C, 
     const auto bz = buf_rsrc(z, (uint32_t)sizeof(T) * L.n), br = buf_rsrc(r, (uint32_t)sizeof(T) * L.n),
                bzc = buf_rsrc(zc, (uint32_t)sizeof(TC) * C.n);
     uint32_t sb, se;
-    block_span(L.n, xcd_block(), sb, se);
+    block_span_of(L.n, vb.blk, vb.G, sb, se);
     for (uint32_t c = sb + threadIdx.x; c < se; c += kBlock) {
         int i, j, k;
         decode(c, L.f_nynz, L.f_nz, L.ny, L.nz, i, j, k);
@@ -695,11 +751,21 @@ __global__ void __launch_bounds__(kBlock) k_mg_post_black(MgLevel L, MgLevel C, 
         if (black) bstv(bz, c, a ? (rv + of) * rd : 0.f, tag);
     }
 }
+};
+template <typename T, typename TC>
+__global__ void __launch_bounds__(kBlock) k_mg_post_black(MgLevel L, MgLevel C, T *__restrict__ z,
+                                                          const T *__restrict__ r,
+                                                          const TC *__restrict__ zc,
+                                                          const MgState *m)
+{
+    KB_mg_post_black<T, TC>::run(vb_here(), L, C, z, r, zc, m);
+}
 
 // ---- coarsest level: symmetric red-black sweeps from z = 0 (one block) -----
 template <typename T, bool L0>
-__global__ void __launch_bounds__(1024) k_mg_coarse(Geo g, MgLevel L, T *z, const T *r, int sweeps,
-                                                    const MgState *m)
+struct KB_mg_coarse {
+    static __device__ __forceinline__ void run(const VB &vb, const Geo &g, const MgLevel &L, T *z,
+                                               const T *r, int sweeps, const MgState *m)
 {
     if (stopped(m)) return;
     // level 0 (a single-level hierarchy) uses the split layout: slots [0, n)
@@ -726,6 +792,13 @@ __global__ void __launch_bounds__(1024) k_mg_coarse(Geo g, MgLevel L, T *z, cons
         }
         __syncthreads();
     }
+}
+};
+template <typename T, bool L0>
+__global__ void __launch_bounds__(1024) k_mg_coarse(Geo g, MgLevel L, T *z, const T *r, int sweeps,
+                                                    const MgState *m)
+{
+    KB_mg_coarse<T, L0>::run(vb_here(), g, L, z, r, sweeps, m);
 }
 
 // ---- the small levels in one workgroup (LDS-resident) --------------------------
@@ -949,7 +1022,8 @@ __device__ __forceinline__ void tail_prolong(const TailView<T> &F, const TailVie
 }
 
 template <typename T>
-__global__ void __launch_bounds__(kTailThreads) k_mg_tail(TailLevels tl, const MgState *m)
+struct KB_mg_tail {
+    static __device__ __forceinline__ void run(const VB &vb, const TailLevels &tl, const MgState *m)
 {
     extern __shared__ __align__(16) unsigned char tail_smem[];
     if (stopped(m)) return;
@@ -1067,6 +1141,12 @@ __global__ void __launch_bounds__(kTailThreads) k_mg_tail(TailLevels tl, const M
         for (uint32_t c = threadIdx.x; c < F.n; c += kTailThreads) zg[c] = F.z[c];
     }
 }
+};
+template <typename T>
+__global__ void __launch_bounds__(kTailThreads) k_mg_tail(TailLevels tl, const MgState *m)
+{
+    KB_mg_tail<T>::run(vb_here(), tl, m);
+}
 
 // ---- inner PCG on level 0 (FP32 vectors, FP64 dots) -------------------------
 // Iteration it (m->inner_it) of an inner solve starts after the V-cycle of
@@ -1082,10 +1162,11 @@ __global__ void __launch_bounds__(kTailThreads) k_mg_tail(TailLevels tl, const M
 // FP32 rounding level over an inner solve's few iterations, and the outer
 // FP64 residual is recomputed from scratch every pass. Thread work pairs red
 // slot h with black slot H + h, so every block streams both colours.
-__global__ void __launch_bounds__(kBlock) k_pcg_dirq(Geo g, MgLevel L, const float *__restrict__ z,
-                                                     const float *__restrict__ r, float *p,
-                                                     float *q, MgState *m, RedScratch red,
-                                                     uint32_t gdot)
+struct KB_pcg_dirq {
+    static __device__ __forceinline__ void run(const VB &vb, const Geo &g, const MgLevel &L,
+                                               const float *__restrict__ z,
+                                               const float *__restrict__ r, float *p, float *q,
+                                               MgState *m, const RedScratch &red, uint32_t gdot)
 {
     __shared__ double lds[kBlock / kWave];
     const uint32_t stop = stop_flags(m);  // tested after the reduction's loads
@@ -1095,7 +1176,7 @@ __global__ void __launch_bounds__(kBlock) k_pcg_dirq(Geo g, MgLevel L, const flo
     if (late(stop)) return;  // uniform: every thread of the grid returns
     const bool first = it == 0;
     const float beta = first ? 0.f : (float)(rz_new / rz_prev);
-    const uint32_t blk = xcd_block(), H = L.H, nb_black = g.nc - H;
+    const uint32_t blk = vb.blk, H = L.H, nb_black = g.nc - H;
     const uint32_t si = (uint32_t)g.ny * g.nz, sj = g.nz;
     const float dint = diag0m(g, kInterior);
     // red views [0, H), black views [H, nc) (reads past an end return 0 and
@@ -1107,7 +1188,7 @@ __global__ void __launch_bounds__(kBlock) k_pcg_dirq(Geo g, MgLevel L, const flo
                qb = buf_rsrc(q + H, 4u * nb_black);
     double acc = 0.0;
     uint32_t hb, he;
-    block_span(H, blk, hb, he);
+    block_span_of(H, blk, vb.G, hb, he);
     for (uint32_t h0 = hb + threadIdx.x; h0 < he; h0 += kU2 * kBlock) {
         uint16_t nr[kU2], nk[kU2];
         float zrv[kU2], rv[kU2], prv[kU2], qrv[kU2], zbv[kU2], pbv[kU2], qbv[kU2], zn[kU2][6];
@@ -1153,11 +1234,20 @@ __global__ void __launch_bounds__(kBlock) k_pcg_dirq(Geo g, MgLevel L, const flo
     block_partial(acc, pslot(red, SLOT_SPMV), blk, lds);
     if (blk == 0 && threadIdx.x == 0) m->rz[it & 1] = rz_new;
 }
+};
+__global__ void __launch_bounds__(kBlock) k_pcg_dirq(Geo g, MgLevel L, const float *__restrict__ z,
+                                                     const float *__restrict__ r, float *p,
+                                                     float *q, MgState *m, RedScratch red,
+                                                     uint32_t gdot)
+{
+    KB_pcg_dirq::run(vb_here(), g, L, z, r, p, q, m, red, gdot);
+}
 
 // General form (level 0 is the tail or the coarsest level, small grids):
 // p = z + beta p in place, then q = A p with gathers (k_pcg_spmv).
-__global__ void __launch_bounds__(kBlock) k_pcg_dir(uint32_t n, const float *z, float *p, MgState *m,
-                                                    RedScratch red, uint32_t gdot)
+struct KB_pcg_dir {
+    static __device__ __forceinline__ void run(const VB &vb, uint32_t n, const float *z, float *p,
+                                               MgState *m, const RedScratch &red, uint32_t gdot)
 {
     __shared__ double lds[kBlock / kWave];
     if (stopped(m)) return;
@@ -1165,23 +1255,30 @@ __global__ void __launch_bounds__(kBlock) k_pcg_dir(uint32_t n, const float *z, 
     const bool first = it == 0;
     const double rz_new = sum_partials(pslot(red, SLOT_MG_A), gdot, lds);
     const float beta = first ? 0.f : (float)(rz_new / m->rz[(it + 1) & 1]);
-    const uint32_t blk = xcd_block();
+    const uint32_t blk = vb.blk;
     uint32_t sb, se;
-    block_span(n, blk, sb, se);
+    block_span_of(n, blk, vb.G, sb, se);
     for (uint32_t c = sb + threadIdx.x; c < se; c += kBlock)
         p[c] = first ? z[c] : __builtin_fmaf(beta, p[c], z[c]);
     if (blk == 0 && threadIdx.x == 0) m->rz[it & 1] = rz_new;
 }
+};
+__global__ void __launch_bounds__(kBlock) k_pcg_dir(uint32_t n, const float *z, float *p, MgState *m,
+                                                    RedScratch red, uint32_t gdot)
+{
+    KB_pcg_dir::run(vb_here(), n, z, p, m, red, gdot);
+}
 
 // q = A p, partials of p.q (split layout: slot s of colour s >= H); loads as
 // in k_mg_smooth0
-__global__ void __launch_bounds__(kBlock) k_pcg_spmv(Geo g, MgLevel L, const float *__restrict__ p,
-                                                     float *__restrict__ q, const MgState *m,
-                                                     RedScratch red)
+struct KB_pcg_spmv {
+    static __device__ __forceinline__ void run(const VB &vb, const Geo &g, const MgLevel &L,
+                                               const float *__restrict__ p, float *__restrict__ q,
+                                               const MgState *m, const RedScratch &red)
 {
     __shared__ double lds[kBlock / kWave];
     const uint32_t stop = stop_flags(m);  // tested after the first loads
-    const uint32_t blk = xcd_block(), H = L.H;
+    const uint32_t blk = vb.blk, H = L.H;
     const uint32_t si = (uint32_t)g.ny * g.nz, sj = g.nz;
     const float dint = diag0m(g, kInterior);
     const auto bcode = buf_rsrc(L.nbs, 2u * g.nc), bp = buf_rsrc(p, 4u * g.nc),
@@ -1189,7 +1286,7 @@ __global__ void __launch_bounds__(kBlock) k_pcg_spmv(Geo g, MgLevel L, const flo
                bq = buf_rsrc(q, 4u * g.nc);
     double acc = 0.0;
     uint32_t sb, se;
-    block_span(g.nc, blk, sb, se);
+    block_span_of(g.nc, blk, vb.G, sb, se);
     for (uint32_t s0 = sb + threadIdx.x; s0 < se; s0 += kU * kBlock) {
         uint16_t nb[kU];
         float pv[kU], zv[kU][6];
@@ -1217,6 +1314,13 @@ __global__ void __launch_bounds__(kBlock) k_pcg_spmv(Geo g, MgLevel L, const flo
     }
     block_partial(acc, pslot(red, SLOT_SPMV), blk, lds);
 }
+};
+__global__ void __launch_bounds__(kBlock) k_pcg_spmv(Geo g, MgLevel L, const float *__restrict__ p,
+                                                     float *__restrict__ q, const MgState *m,
+                                                     RedScratch red)
+{
+    KB_pcg_spmv::run(vb_here(), g, L, p, q, m, red);
+}
 
 // alpha = rz / p.q; e += alpha p (e = alpha p on an inner solve's first
 // iteration); r -= alpha q; partials of r.r
@@ -1224,17 +1328,19 @@ __global__ void __launch_bounds__(kBlock) k_pcg_spmv(Geo g, MgLevel L, const flo
 // The V-cycle's first pass, the red sweep from z = 0 on level 0, is done here:
 // z = r / diag on red unknowns (red slots [0, H) of the split layout), 0 on
 // other red slots; the V-cycle then starts with the black sweep.
-__global__ void __launch_bounds__(kBlock) k_pcg_update(Geo g, MgLevel L, float *e, float *r,
-                                                       const float *p, const float *q, float *z,
-                                                       MgState *m, RedScratch red, uint32_t gq)
+struct KB_pcg_update {
+    static __device__ __forceinline__ void run(const VB &vb, const Geo &g, const MgLevel &L,
+                                               float *e, float *r, const float *p, const float *q,
+                                               float *z, MgState *m, const RedScratch &red,
+                                               uint32_t gq)
 {
     __shared__ double lds[kBlock / kWave];
     if (stopped(m)) return;
-    const uint32_t blk = xcd_block();
+    const uint32_t blk = vb.blk;
     const int it = m->inner_it;
     const bool first = it == 0;
     uint32_t sb, se;
-    block_span(g.nc, blk, sb, se);
+    block_span_of(g.nc, blk, vb.G, sb, se);
     const double pq = sum_partials(pslot(red, SLOT_SPMV), gq, lds);  // gq p.q partials
     if (!(pq > 0.0)) {  // breakdown or a non-positive operator: end the inner solve
         if (first)      // the correction of an inner solve that made no step is 0
@@ -1257,10 +1363,19 @@ __global__ void __launch_bounds__(kBlock) k_pcg_update(Geo g, MgLevel L, float *
     }
     block_partial(acc, pslot(red, SLOT_UPDATE), blk, lds);
 }
+};
+__global__ void __launch_bounds__(kBlock) k_pcg_update(Geo g, MgLevel L, float *e, float *r,
+                                                       const float *p, const float *q, float *z,
+                                                       MgState *m, RedScratch red, uint32_t gq)
+{
+    KB_pcg_update::run(vb_here(), g, L, e, r, p, q, z, m, red, gq);
+}
 
 // stopping test of the inner solve (one block): its relative target, or the
 // outer target reached already
-__global__ void __launch_bounds__(kBlock) k_pcg_check(MgState *m, RedScratch red, uint32_t G)
+struct KB_pcg_check {
+    static __device__ __forceinline__ void run(const VB &vb, MgState *m, const RedScratch &red,
+                                               uint32_t G)
 {
     __shared__ double lds[kBlock / kWave];
     if (stopped(m)) return;
@@ -1278,34 +1393,48 @@ __global__ void __launch_bounds__(kBlock) k_pcg_check(MgState *m, RedScratch red
         it >= m->max_inner || !(rr == rr) || m->total_iters >= m->total_cap)
         m->inner_done = 1;
 }
+};
+__global__ void __launch_bounds__(kBlock) k_pcg_check(MgState *m, RedScratch red, uint32_t G)
+{
+    KB_pcg_check::run(vb_here(), m, red, G);
+}
 
 // partials of r.z (after a V-cycle whose level 0 is the tail or the coarsest
 // level; otherwise its last red pass writes them)
-__global__ void __launch_bounds__(kBlock) k_pcg_dot_rz(uint32_t n, const float *r, const float *z,
-                                                       const MgState *m, RedScratch red)
+struct KB_pcg_dot_rz {
+    static __device__ __forceinline__ void run(const VB &vb, uint32_t n, const float *r,
+                                               const float *z, const MgState *m,
+                                               const RedScratch &red)
 {
     __shared__ double lds[kBlock / kWave];
     if (stopped(m)) return;
-    const uint32_t blk = xcd_block();
+    const uint32_t blk = vb.blk;
     double acc = 0.0;
     uint32_t sb, se;
-    block_span(n, blk, sb, se);
+    block_span_of(n, blk, vb.G, sb, se);
     for (uint32_t c = sb + threadIdx.x; c < se; c += kBlock) acc += (double)r[c] * (double)z[c];
     block_partial(acc, pslot(red, SLOT_MG_A), blk, lds);
+}
+};
+__global__ void __launch_bounds__(kBlock) k_pcg_dot_rz(uint32_t n, const float *r, const float *z,
+                                                       const MgState *m, RedScratch red)
+{
+    KB_pcg_dot_rz::run(vb_here(), n, r, z, m, red);
 }
 
 // ---- outer FP64 defect correction -------------------------------------------
 // x = 0 (PoissonSolver.cpp:96) or, warm, the pressure x already holds (the
 // previous step's: SOLID cells 0, OUTFLOW values never read as unknowns);
 // e = 0; partials of ||b||^2 over non-SOLID cells (PoissonSolver.cpp:101-113)
-__global__ void __launch_bounds__(kBlock) k_dc_init(Geo g, const double *b, double *x, float *e,
-                                                    int warm, RedScratch red)
+struct KB_dc_init {
+    static __device__ __forceinline__ void run(const VB &vb, const Geo &g, const double *b,
+                                               double *x, float *e, int warm, const RedScratch &red)
 {
     __shared__ double lds[kBlock / kWave];
-    const uint32_t blk = xcd_block();
+    const uint32_t blk = vb.blk;
     double acc = 0.0;
     uint32_t sb, se;
-    block_span(g.nc, blk, sb, se);
+    block_span_of(g.nc, blk, vb.G, sb, se);
     for (uint32_t c = sb + threadIdx.x; c < se; c += kBlock) {
         if (!warm) x[c] = 0.0;  // warm: x keeps the previous step's pressure
         e[c] = 0.f;
@@ -1313,10 +1442,17 @@ __global__ void __launch_bounds__(kBlock) k_dc_init(Geo g, const double *b, doub
     }
     block_partial(acc, pslot(red, SLOT_INIT), blk, lds);
 }
+};
+__global__ void __launch_bounds__(kBlock) k_dc_init(Geo g, const double *b, double *x, float *e,
+                                                    int warm, RedScratch red)
+{
+    KB_dc_init::run(vb_here(), g, b, x, e, warm, red);
+}
 
-__global__ void __launch_bounds__(kBlock) k_dc_start(MgState *m, RedScratch red, uint32_t G,
-                                                     double outer_tol, int max_outer, int max_inner,
-                                                     double inner_tol, int total_cap)
+struct KB_dc_start {
+    static __device__ __forceinline__ void run(const VB &vb, MgState *m, const RedScratch &red,
+                                               uint32_t G, double outer_tol, int max_outer,
+                                               int max_inner, double inner_tol, int total_cap)
 {
     __shared__ double lds[kBlock / kWave];
     const double bb = sum_partials(pslot(red, SLOT_INIT), G, lds);
@@ -1335,6 +1471,13 @@ __global__ void __launch_bounds__(kBlock) k_dc_start(MgState *m, RedScratch red,
     m->inner_done = 1;
     m->solve_done = m->bnorm < 1e-15 ? 1 : 0;
     m->res = 0.0;
+}
+};
+__global__ void __launch_bounds__(kBlock) k_dc_start(MgState *m, RedScratch red, uint32_t G,
+                                                     double outer_tol, int max_outer, int max_inner,
+                                                     double inner_tol, int total_cap)
+{
+    KB_dc_start::run(vb_here(), m, red, G, outer_tol, max_outer, max_inner, inner_tol, total_cap);
 }
 
 // FP64 reference row (PoissonSolver.cpp:52-87) on a uniform grid from the
@@ -1362,22 +1505,22 @@ __device__ __forceinline__ void row64(const Geo &g, uint32_t c, uint16_t m, F va
 
 // x_out = x_in + e on unknowns; r = b - A x_out (FP64); r32 = (float) r;
 // partials of ||r||^2
-__global__ void __launch_bounds__(kBlock) k_dc_residual(Geo g, const double *__restrict__ b,
-                                                        double *xa, double *xb,
-                                                        const float *__restrict__ e,
-                                                        float *__restrict__ r32,
-                                                        float *__restrict__ z0, const MgState *m,
-                                                        RedScratch red)
+struct KB_dc_residual {
+    static __device__ __forceinline__ void run(const VB &vb, const Geo &g,
+                                               const double *__restrict__ b, double *xa, double *xb,
+                                               const float *__restrict__ e, float *__restrict__ r32,
+                                               float *__restrict__ z0, const MgState *m,
+                                               const RedScratch &red)
 {
     __shared__ double lds[kBlock / kWave];
     if (m->solve_done) return;
     // x_in is the buffer of the current pass parity, x_out the other one
     const double *__restrict__ xin = (m->passes & 1) ? xb : xa;
     double *__restrict__ xout = (m->passes & 1) ? xa : xb;
-    const uint32_t blk = xcd_block();
+    const uint32_t blk = vb.blk;
     double acc = 0.0;
     uint32_t sb, se;
-    block_span(g.nc, blk, sb, se);
+    block_span_of(g.nc, blk, vb.G, sb, se);
     const uint32_t H = (g.nc + 1) / 2;
     for (uint32_t c = sb + threadIdx.x; c < se; c += kBlock) {
         const uint16_t nb = g.nbm[c];
@@ -1402,9 +1545,21 @@ __global__ void __launch_bounds__(kBlock) k_dc_residual(Geo g, const double *__r
     }
     block_partial(acc, pslot(red, SLOT_UPDATE), blk, lds);
 }
+};
+__global__ void __launch_bounds__(kBlock) k_dc_residual(Geo g, const double *__restrict__ b,
+                                                        double *xa, double *xb,
+                                                        const float *__restrict__ e,
+                                                        float *__restrict__ r32,
+                                                        float *__restrict__ z0, const MgState *m,
+                                                        RedScratch red)
+{
+    KB_dc_residual::run(vb_here(), g, b, xa, xb, e, r32, z0, m, red);
+}
 
 // outer stopping test: ||r|| / ||b|| < tol (one block)
-__global__ void __launch_bounds__(kBlock) k_dc_check(MgState *m, RedScratch red, uint32_t G)
+struct KB_dc_check {
+    static __device__ __forceinline__ void run(const VB &vb, MgState *m, const RedScratch &red,
+                                               uint32_t G)
 {
     __shared__ double lds[kBlock / kWave];
     if (m->solve_done) return;
@@ -1425,17 +1580,23 @@ __global__ void __launch_bounds__(kBlock) k_dc_check(MgState *m, RedScratch red,
     m->inner_it = 0;
     m->inner_done = 0;
 }
+};
+__global__ void __launch_bounds__(kBlock) k_dc_check(MgState *m, RedScratch red, uint32_t G)
+{
+    KB_dc_check::run(vb_here(), m, red, G);
+}
 
 // p: unknowns from x; OUTFLOW rows solved from their neighbours (or pinned
 // to 0 for the reference multigrid's convention); SOLID cells 0.
-__global__ void __launch_bounds__(kBlock) k_dc_finalize(Geo g, const double *__restrict__ b,
-                                                        const double *xa, const double *xb,
-                                                        double *p, int pin_outflow,
-                                                        const MgState *m)
+struct KB_dc_finalize {
+    static __device__ __forceinline__ void run(const VB &vb, const Geo &g,
+                                               const double *__restrict__ b, const double *xa,
+                                               const double *xb, double *p, int pin_outflow,
+                                               const MgState *m)
 {
     const double *x = (m->passes & 1) ? xb : xa;  // the last pass's output
     uint32_t sb, se;
-    block_span(g.nc, xcd_block(), sb, se);
+    block_span_of(g.nc, vb.blk, vb.G, sb, se);
     for (uint32_t c = sb + threadIdx.x; c < se; c += kBlock) {
         const uint16_t nb = g.nbm[c];
         if (nb & NB_SELF) {
@@ -1450,6 +1611,14 @@ __global__ void __launch_bounds__(kBlock) k_dc_finalize(Geo g, const double *__r
         row64(g, c, nb, [&](uint32_t q) { return x[q]; }, diag, off);
         p[c] = (b[c] + off) / diag;
     }
+}
+};
+__global__ void __launch_bounds__(kBlock) k_dc_finalize(Geo g, const double *__restrict__ b,
+                                                        const double *xa, const double *xb,
+                                                        double *p, int pin_outflow,
+                                                        const MgState *m)
+{
+    KB_dc_finalize::run(vb_here(), g, b, xa, xb, p, pin_outflow, m);
 }
 
 // ---- launchers ------------------------------------------------------------------
@@ -1476,17 +1645,17 @@ void mg_launch_smooth(hipStream_t s, const Geo &g, const MgLevel &L, bool l0, in
         float *z = (float *)L.z;
         const float *r = (const float *)L.r;
         if (dot && color == 0)
-            FFR_LAUNCH(k_mg_smooth0<true>, dim3(grid_resident(cnt)), dim3(kBlock), 0, s, g, L,
-                       L.nbs + base, z + obase, z + base, r + base, r + obase, cnt, g.nc - cnt,
-                       (uint32_t)color, zero_start, m, *dot);
+            FFR_GANG((KB_mg_smooth0<true>), GB_RES, k_mg_smooth0<true>, dim3(grid_resident(cnt)),
+                     dim3(kBlock), 0, s, g, L, L.nbs + base, z + obase, z + base, r + base,
+                     r + obase, cnt, g.nc - cnt, (uint32_t)color, zero_start, m, *dot);
         else
-            FFR_LAUNCH(k_mg_smooth0<false>, dim3(grid_resident(cnt)), dim3(kBlock), 0, s, g, L,
-                       L.nbs + base, z + obase, z + base, r + base, r + obase, cnt, g.nc - cnt,
-                       (uint32_t)color, zero_start, m, RedScratch{});
+            FFR_GANG((KB_mg_smooth0<false>), GB_RES, k_mg_smooth0<false>, dim3(grid_resident(cnt)),
+                     dim3(kBlock), 0, s, g, L, L.nbs + base, z + obase, z + base, r + base,
+                     r + obase, cnt, g.nc - cnt, (uint32_t)color, zero_start, m, RedScratch{});
     }
     else
-        FFR_LAUNCH(k_mg_smooth<T>, dim3(grid_for(L.n)), dim3(kBlock), 0, s, L, (T *)L.z,
-                   (const T *)L.r, color, zero_start, m);
+        FFR_GANG((KB_mg_smooth<T>), GB_PLAIN, k_mg_smooth<T>, dim3(grid_for(L.n)), dim3(kBlock), 0,
+                 s, L, (T *)L.z, (const T *)L.r, color, zero_start, m);
 }
 
 template <typename TF, typename TC>
@@ -1494,22 +1663,25 @@ void mg_launch_restrict(hipStream_t s, const Geo &g, const MgLevel &F, const MgL
                         bool red0, const MgState *m, float *scratch)
 {
     if (l0 && scratch) {  // two passes through the scratch residual
-        FFR_LAUNCH(k_mg_resid0, dim3(grid_resident(g.nc)), dim3(kBlock), 0, s, g, F, (const float *)F.z,
-                   (const float *)F.r, scratch, m);
-        FFR_LAUNCH((k_mg_restrict_sum0<TC>), dim3(grid_for(C.n)), dim3(kBlock), 0, s, g, F, C,
-                   (const float *)scratch, (TC *)C.r, (TC *)C.z, (int)red0, m);
+        FFR_GANG((KB_mg_resid0), GB_RES, k_mg_resid0, dim3(grid_resident(g.nc)), dim3(kBlock), 0, s,
+                 g, F, (const float *)F.z, (const float *)F.r, scratch, m);
+        FFR_GANG((KB_mg_restrict_sum0<TC>), GB_PLAIN, (k_mg_restrict_sum0<TC>), dim3(grid_for(C.n)),
+                 dim3(kBlock), 0, s, g, F, C, (const float *)scratch, (TC *)C.r, (TC *)C.z,
+                 (int)red0, m);
     } else if (l0)
-        FFR_LAUNCH((k_mg_restrict<TF, TC, true>), dim3(grid_for(8ull * C.n)), dim3(kBlock), 0, s, g,
-                   F, C, (const TF *)F.z, (const TF *)F.r, (TC *)C.r, (TC *)C.z, (int)red0, m);
+        FFR_GANG((KB_mg_restrict<TF, TC, true>), GB_PLAIN, (k_mg_restrict<TF, TC, true>),
+                 dim3(grid_for(8ull * C.n)), dim3(kBlock), 0, s, g, F, C, (const TF *)F.z,
+                 (const TF *)F.r, (TC *)C.r, (TC *)C.z, (int)red0, m);
     else
-        FFR_LAUNCH((k_mg_restrict<TF, TC, false>), dim3(grid_for(8ull * C.n)), dim3(kBlock), 0, s,
-                   g, F, C, (const TF *)F.z, (const TF *)F.r, (TC *)C.r, (TC *)C.z, (int)red0, m);
+        FFR_GANG((KB_mg_restrict<TF, TC, false>), GB_PLAIN, (k_mg_restrict<TF, TC, false>),
+                 dim3(grid_for(8ull * C.n)), dim3(kBlock), 0, s, g, F, C, (const TF *)F.z,
+                 (const TF *)F.r, (TC *)C.r, (TC *)C.z, (int)red0, m);
 }
 
 void mg_launch_resid0(hipStream_t s, const Geo &g, const MgLevel &L0, float *res, const MgState *m)
 {
-    FFR_LAUNCH(k_mg_resid0, dim3(grid_resident(g.nc)), dim3(kBlock), 0, s, g, L0, (const float *)L0.z,
-               (const float *)L0.r, res, m);
+    FFR_GANG((KB_mg_resid0), GB_RES, k_mg_resid0, dim3(grid_resident(g.nc)), dim3(kBlock), 0, s, g,
+             L0, (const float *)L0.z, (const float *)L0.r, res, m);
 }
 
 template <typename TF, typename TC>
@@ -1517,11 +1689,12 @@ void mg_launch_post_black(hipStream_t s, const Geo &g, const MgLevel &F, const M
                           const MgState *m)
 {
     if (l0)
-        FFR_LAUNCH((k_mg_post_black0<TC>), dim3(grid_resident(g.nc - F.H)), dim3(kBlock), 0, s, g, F, C,
-                   (const TC *)C.z, m);
+        FFR_GANG((KB_mg_post_black0<TC>), GB_RES, (k_mg_post_black0<TC>),
+                 dim3(grid_resident(g.nc - F.H)), dim3(kBlock), 0, s, g, F, C, (const TC *)C.z, m);
     else
-        FFR_LAUNCH((k_mg_post_black<TF, TC>), dim3(grid_for(F.n)), dim3(kBlock), 0, s, F, C,
-                   (TF *)F.z, (const TF *)F.r, (const TC *)C.z, m);
+        FFR_GANG((KB_mg_post_black<TF, TC>), GB_PLAIN, (k_mg_post_black<TF, TC>),
+                 dim3(grid_for(F.n)), dim3(kBlock), 0, s, F, C, (TF *)F.z, (const TF *)F.r,
+                 (const TC *)C.z, m);
 }
 
 // Dynamic LDS beyond 64 KiB must be allowed per kernel before a launch; done
@@ -1534,13 +1707,21 @@ int mg_tail_allow_lds()
     if (hipFuncSetAttribute((const void *)k_mg_tail<__hip_bfloat16>,
                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)kTailLdsMax) != hipSuccess)
         return -1;
+    // and the gang launches of the same kernels
+    if (hipFuncSetAttribute(gang_kernel<KB_mg_tail<float>, GB_WIDE, TailLevels, const MgState *>(),
+                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)kTailLdsMax) != hipSuccess)
+        return -1;
+    if (hipFuncSetAttribute(gang_kernel<KB_mg_tail<__hip_bfloat16>, GB_WIDE, TailLevels, const MgState *>(),
+                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)kTailLdsMax) != hipSuccess)
+        return -1;
     return 0;
 }
 
 template <typename T>
 void mg_launch_tail(hipStream_t s, const TailLevels &tl, size_t lds_bytes, const MgState *m)
 {
-    FFR_LAUNCH(k_mg_tail<T>, dim3(1), dim3(kTailThreads), lds_bytes, s, tl, m);
+    FFR_GANG((KB_mg_tail<T>), GB_WIDE, k_mg_tail<T>, dim3(1), dim3(kTailThreads), lds_bytes, s, tl,
+             m);
 }
 
 template <typename T>
@@ -1548,11 +1729,11 @@ void mg_launch_coarse(hipStream_t s, const Geo &g, const MgLevel &L, bool l0, in
                       const MgState *m)
 {
     if (l0)
-        FFR_LAUNCH((k_mg_coarse<T, true>), dim3(1), dim3(1024), 0, s, g, L, (T *)L.z,
-                           (const T *)L.r, sweeps, m);
+        FFR_GANG((KB_mg_coarse<T, true>), GB_WIDE, (k_mg_coarse<T, true>), dim3(1), dim3(1024), 0,
+                 s, g, L, (T *)L.z, (const T *)L.r, sweeps, m);
     else
-        FFR_LAUNCH((k_mg_coarse<T, false>), dim3(1), dim3(1024), 0, s, g, L, (T *)L.z,
-                           (const T *)L.r, sweeps, m);
+        FFR_GANG((KB_mg_coarse<T, false>), GB_WIDE, (k_mg_coarse<T, false>), dim3(1), dim3(1024), 0,
+                 s, g, L, (T *)L.z, (const T *)L.r, sweeps, m);
 }
 
 template void mg_launch_smooth<float>(hipStream_t, const Geo &, const MgLevel &, bool, int, int, const MgState *, const RedScratch *);
@@ -1571,60 +1752,65 @@ template void mg_launch_coarse<__hip_bfloat16>(hipStream_t, const Geo &, const M
 void pcg_launch_dirq(hipStream_t s, const Geo &g, const MgLevel &L0, const float *r, float *p,
                      float *q, MgState *m, RedScratch red, uint32_t gdot)
 {
-    FFR_LAUNCH(k_pcg_dirq, dim3(grid_resident(L0.H)), dim3(kBlock), 0, s, g, L0, (const float *)L0.z, r,
-               p, q, m, red, gdot);
+    FFR_GANG((KB_pcg_dirq), GB_PLAIN, k_pcg_dirq, dim3(grid_resident(L0.H)), dim3(kBlock), 0, s, g,
+             L0, (const float *)L0.z, r, p, q, m, red, gdot);
 }
 void pcg_launch_dir(hipStream_t s, uint32_t n, const float *z, float *p, MgState *m, RedScratch red,
                     uint32_t gdot)
 {
-    FFR_LAUNCH(k_pcg_dir, dim3(grid_for(n)), dim3(kBlock), 0, s, n, z, p, m, red, gdot);
+    FFR_GANG((KB_pcg_dir), GB_PLAIN, k_pcg_dir, dim3(grid_for(n)), dim3(kBlock), 0, s, n, z, p, m,
+             red, gdot);
 }
 void pcg_launch_spmv(hipStream_t s, const Geo &g, const MgLevel &L0, const float *p, float *q,
                      const MgState *m, RedScratch red)
 {
-    FFR_LAUNCH(k_pcg_spmv, dim3(grid_for(g.nc)), dim3(kBlock), 0, s, g, L0, p, q, m, red);
+    FFR_GANG((KB_pcg_spmv), GB_PLAIN, k_pcg_spmv, dim3(grid_for(g.nc)), dim3(kBlock), 0, s, g, L0,
+             p, q, m, red);
 }
 void pcg_launch_update(hipStream_t s, const Geo &g, const MgLevel &L0, float *e, float *r,
                        const float *p, const float *q, MgState *m, RedScratch red, uint32_t gq)
 {
-    FFR_LAUNCH(k_pcg_update, dim3(grid_for(g.nc)), dim3(kBlock), 0, s, g, L0, e, r, p, q,
-               (float *)L0.z, m, red, gq);
+    FFR_GANG((KB_pcg_update), GB_PLAIN, k_pcg_update, dim3(grid_for(g.nc)), dim3(kBlock), 0, s, g,
+             L0, e, r, p, q, (float *)L0.z, m, red, gq);
 }
 void pcg_launch_check(hipStream_t s, uint32_t n, MgState *m, RedScratch red)
 {
-    FFR_LAUNCH(k_pcg_check, dim3(1), dim3(kBlock), 0, s, m, red, grid_for(n));
+    FFR_GANG((KB_pcg_check), GB_PLAIN, k_pcg_check, dim3(1), dim3(kBlock), 0, s, m, red, grid_for(n));
 }
 void pcg_launch_dot_rz(hipStream_t s, uint32_t n, const float *r, const float *z, const MgState *m,
                        RedScratch red)
 {
-    FFR_LAUNCH(k_pcg_dot_rz, dim3(grid_for(n)), dim3(kBlock), 0, s, n, r, z, m, red);
+    FFR_GANG((KB_pcg_dot_rz), GB_PLAIN, k_pcg_dot_rz, dim3(grid_for(n)), dim3(kBlock), 0, s, n, r,
+             z, m, red);
 }
 void dc_launch_init(hipStream_t s, const Geo &g, const double *b, double *x, float *e, bool warm,
                     RedScratch red)
 {
-    FFR_LAUNCH(k_dc_init, dim3(grid_for(g.nc)), dim3(kBlock), 0, s, g, b, x, e, warm ? 1 : 0, red);
+    FFR_GANG((KB_dc_init), GB_PLAIN, k_dc_init, dim3(grid_for(g.nc)), dim3(kBlock), 0, s, g, b, x,
+             e, warm ? 1 : 0, red);
 }
 void dc_launch_start(hipStream_t s, const Geo &g, MgState *m, RedScratch red, double outer_tol,
                      int max_outer, int max_inner, double inner_tol, int total_cap)
 {
-    FFR_LAUNCH(k_dc_start, dim3(1), dim3(kBlock), 0, s, m, red, grid_for(g.nc), outer_tol,
-               max_outer, max_inner, inner_tol, total_cap);
+    FFR_GANG((KB_dc_start), GB_PLAIN, k_dc_start, dim3(1), dim3(kBlock), 0, s, m, red,
+             grid_for(g.nc), outer_tol, max_outer, max_inner, inner_tol, total_cap);
 }
 void dc_launch_residual(hipStream_t s, const Geo &g, const double *b, double *xa, double *xb,
                         const float *e, float *r32, float *z0, const MgState *m, RedScratch red)
 {
-    FFR_LAUNCH(k_dc_residual, dim3(grid_for(g.nc)), dim3(kBlock), 0, s, g, b, xa, xb, e, r32, z0,
-               m, red);
+    FFR_GANG((KB_dc_residual), GB_PLAIN, k_dc_residual, dim3(grid_for(g.nc)), dim3(kBlock), 0, s, g,
+             b, xa, xb, e, r32, z0, m, red);
 }
 void dc_launch_check(hipStream_t s, const Geo &g, MgState *m, RedScratch red)
 {
-    FFR_LAUNCH(k_dc_check, dim3(1), dim3(kBlock), 0, s, m, red, grid_for(g.nc));
+    FFR_GANG((KB_dc_check), GB_PLAIN, k_dc_check, dim3(1), dim3(kBlock), 0, s, m, red,
+             grid_for(g.nc));
 }
 void dc_launch_finalize(hipStream_t s, const Geo &g, const double *b, const double *xa,
                         const double *xb, double *p, int pin_outflow, const MgState *m)
 {
-    FFR_LAUNCH(k_dc_finalize, dim3(grid_for(g.nc)), dim3(kBlock), 0, s, g, b, xa, xb, p,
-               pin_outflow, m);
+    FFR_GANG((KB_dc_finalize), GB_PLAIN, k_dc_finalize, dim3(grid_for(g.nc)), dim3(kBlock), 0, s, g,
+             b, xa, xb, p, pin_outflow, m);
 }
 
 }  // namespace ffr

@@ -15,6 +15,7 @@
 // All kernels are templated on the field type R: double reproduces the
 // reference's arithmetic operation by operation (contraction off, same
 // operand order, true divisions), float is the mixed-precision form.
+#include "../ffr_gang.hpp"
 #include "../ffr_kernels.hpp"
 
 #pragma clang fp contract(off)
@@ -311,11 +312,12 @@ __device__ __forceinline__ R predict_w(const Geo &g, const StepParams &sp, const
 // one region are fetched once into one cache; threads past the cells take the
 // remaining boundary faces (u at i = nx, v at j = ny, w at k = nz).
 template <typename R, bool UNI>
-__global__ void __launch_bounds__(kBlock) k_predict(Geo g, StepParams sp, const R *u,
-                                                    const R *v, const R *w, const R *T, R *us,
-                                                    R *vs, R *ws, StepRec *rec)
+struct KB_predict {
+    static __device__ __forceinline__ void run(const VB &vb, const Geo &g, const StepParams &sp,
+                                               const R *u, const R *v, const R *w, const R *T,
+                                               R *us, R *vs, R *ws, StepRec *rec)
 {
-    if (blockIdx.x == 0 && threadIdx.x == 0) {  // start-of-step diagnostics reset
+    if (vb.raw == 0 && threadIdx.x == 0) {  // start-of-step diagnostics reset
         rec->vmax_bits = 0ull;
         rec->divmax_bits = 0ull;
         rec->tsum = 0.0;
@@ -326,7 +328,7 @@ __global__ void __launch_bounds__(kBlock) k_predict(Geo g, StepParams sp, const 
     const uint32_t nxu = (uint32_t)g.ny * g.nz, nyv = (uint32_t)g.nx * g.nz;
     const uint32_t total = g.nc + nxu + nyv + (uint32_t)g.nx * g.ny;
     uint32_t b, end;
-    block_span(total, xcd_block(), b, end);
+    block_span_of(total, vb.blk, vb.G, b, end);
     for (uint32_t t = b + threadIdx.x; t < end; t += kBlock) {
         if (t < g.nc) {
             const uint32_t i = fdiv(t, g.f_nynz), rem = t - i * nxu;
@@ -352,6 +354,14 @@ __global__ void __launch_bounds__(kBlock) k_predict(Geo g, StepParams sp, const 
         }
     }
 }
+};
+template <typename R, bool UNI>
+__global__ void __launch_bounds__(kBlock) k_predict(Geo g, StepParams sp, const R *u,
+                                                    const R *v, const R *w, const R *T, R *us,
+                                                    R *vs, R *ws, StepRec *rec)
+{
+    KB_predict<R, UNI>::run(vb_here(), g, sp, u, v, w, T, us, vs, ws, rec);
+}
 
 template <typename R>
 void launch_predict(hipStream_t s, const Geo &g, const StepParams &sp, const R *u, const R *v,
@@ -360,20 +370,21 @@ void launch_predict(hipStream_t s, const Geo &g, const StepParams &sp, const R *
     const uint64_t n = (uint64_t)g.nc + (uint64_t)g.ny * g.nz + (uint64_t)g.nx * g.nz +
                        (uint64_t)g.nx * g.ny;
     if (g.uniform)
-        FFR_LAUNCH((k_predict<R, true>), dim3(grid_for(n)), dim3(kBlock), 0, s, g, sp, u, v, w, T,
-                   us, vs, ws, rec);
+        FFR_GANG((KB_predict<R, true>), GB_PLAIN, (k_predict<R, true>), dim3(grid_for(n)),
+                 dim3(kBlock), 0, s, g, sp, u, v, w, T, us, vs, ws, rec);
     else
-        FFR_LAUNCH((k_predict<R, false>), dim3(grid_for(n)), dim3(kBlock), 0, s, g, sp, u, v, w,
-                   T, us, vs, ws, rec);
+        FFR_GANG((KB_predict<R, false>), GB_PLAIN, (k_predict<R, false>), dim3(grid_for(n)),
+                 dim3(kBlock), 0, s, g, sp, u, v, w, T, us, vs, ws, rec);
 }
 
 // ---- K4: Poisson right-hand side ----------------------------------------
 template <typename R>
-__global__ void __launch_bounds__(kBlock) k_rhs(Geo g, StepParams sp, const R *us, const R *vs,
-                                                const R *ws, double *b)
+struct KB_rhs {
+    static __device__ __forceinline__ void run(const VB &vb, const Geo &g, const StepParams &sp,
+                                               const R *us, const R *vs, const R *ws, double *b)
 {
     uint32_t sb, se;
-    block_span(g.nc, xcd_block(), sb, se);
+    block_span_of(g.nc, vb.blk, vb.G, sb, se);
     for (uint32_t c = sb + threadIdx.x; c < se; c += kBlock) {
         if (g.flags[c] == SOLID) {
             b[c] = 0.0;
@@ -390,12 +401,20 @@ __global__ void __launch_bounds__(kBlock) k_rhs(Geo g, StepParams sp, const R *u
         b[c] = sp.rhs_factor * div;
     }
 }
+};
+template <typename R>
+__global__ void __launch_bounds__(kBlock) k_rhs(Geo g, StepParams sp, const R *us, const R *vs,
+                                                const R *ws, double *b)
+{
+    KB_rhs<R>::run(vb_here(), g, sp, us, vs, ws, b);
+}
 
 template <typename R>
 void launch_rhs(hipStream_t s, const Geo &g, const StepParams &sp, const R *us, const R *vs,
                 const R *ws, double *b)
 {
-    FFR_LAUNCH(k_rhs<R>, dim3(grid_for(g.nc)), dim3(kBlock), 0, s, g, sp, us, vs, ws, b);
+    FFR_GANG((KB_rhs<R>), GB_PLAIN, k_rhs<R>, dim3(grid_for(g.nc)), dim3(kBlock), 0, s, g, sp, us,
+             vs, ws, b);
 }
 
 // ---- K6: projection + post-projection masks + velocity change ----------
@@ -428,17 +447,22 @@ __device__ __forceinline__ R corrected(const Geo &g, const StepParams &sp, const
 
 // One block section per face array (u, v, w), one face per thread.
 template <typename R>
-__global__ void __launch_bounds__(kBlock) k_correct(Geo g, StepParams sp, FaceSections fsec,
-                                                    const R *us, const R *vs, const R *ws,
-                                                    const double *p, R *u, R *v, R *w,
-                                                    StepRec *rec)
+struct KB_correct {
+    static __device__ __forceinline__ void run(const VB &vb, const Geo &g, const StepParams &sp,
+                                               const FaceSections &fsec, const R *us, const R *vs,
+                                               const R *ws, const double *p, R *u, R *v, R *w,
+                                               StepRec *rec)
 {
     __shared__ double lds[kBlock / kWave];
     double dv = 0.0;
     bool bad = false;
     uint32_t lb, G, sb, se;
-    const int axis = face_section(fsec, xcd_block(), lb, G);  // uniform per block
-    const uint32_t n = axis == 0 ? g.nu : (axis == 1 ? g.nv : g.nw);
+    const int axis = face_section_of(fsec, vb.blk, vb.G, lb, G);  // uniform per block
+    // all three counts loaded, then combined: a select between the loads
+    // would become one load at a selected offset into g, which keeps a copy
+    // of g in scratch memory
+    const uint32_t nu = g.nu, nv = g.nv, nw = g.nw;
+    const uint32_t n = nu + (axis >= 1 ? nv - nu : 0u) + (axis == 2 ? nw - nv : 0u);
     const uint16_t *code = g.fcode + (axis >= 1 ? g.nu : 0u) + (axis == 2 ? g.nv : 0u);
     const R *fs = axis == 0 ? us : (axis == 1 ? vs : ws);
     R *out = axis == 0 ? u : (axis == 1 ? v : w);
@@ -467,6 +491,15 @@ __global__ void __launch_bounds__(kBlock) k_correct(Geo g, StepParams sp, FaceSe
     const double m = block_max(dv, lds);
     if (threadIdx.x == 0 && m > 0.0) atomic_max_nonneg(&rec->vmax_bits, m);
 }
+};
+template <typename R>
+__global__ void __launch_bounds__(kBlock) k_correct(Geo g, StepParams sp, FaceSections fsec,
+                                                    const R *us, const R *vs, const R *ws,
+                                                    const double *p, R *u, R *v, R *w,
+                                                    StepRec *rec)
+{
+    KB_correct<R>::run(vb_here(), g, sp, fsec, us, vs, ws, p, u, v, w, rec);
+}
 
 template <typename R>
 void launch_correct(hipStream_t s, const Geo &g, const StepParams &sp, const R *us, const R *vs,
@@ -475,7 +508,8 @@ void launch_correct(hipStream_t s, const Geo &g, const StepParams &sp, const R *
     const uint64_t n = (uint64_t)g.nu + g.nv + g.nw;
     const uint32_t G = std::max(3u, grid_for(n));
     const FaceSections fs = face_sections(g.nu, g.nv, g.nw, G);
-    FFR_LAUNCH(k_correct<R>, dim3(G), dim3(kBlock), 0, s, g, sp, fs, us, vs, ws, p, u, v, w, rec);
+    FFR_GANG((KB_correct<R>), GB_PLAIN, k_correct<R>, dim3(G), dim3(kBlock), 0, s, g, sp, fs, us,
+             vs, ws, p, u, v, w, rec);
 }
 
 // ---- divergence (FluidSolver.cpp:470-484) --------------------------------
@@ -490,13 +524,14 @@ __device__ __forceinline__ double cell_div(const Geo &g, const R *u, const R *v,
 }
 
 template <typename R>
-__global__ void __launch_bounds__(kBlock) k_divmax(Geo g, const R *u, const R *v, const R *w,
-                                                   StepRec *rec)
+struct KB_divmax {
+    static __device__ __forceinline__ void run(const VB &vb, const Geo &g, const R *u, const R *v,
+                                               const R *w, StepRec *rec)
 {
     __shared__ double lds[kBlock / kWave];
     double d = 0.0;
     uint32_t sb, se;
-    block_span(g.nc, xcd_block(), sb, se);
+    block_span_of(g.nc, vb.blk, vb.G, sb, se);
     for (uint32_t c = sb + threadIdx.x; c < se; c += kBlock) {
         if (g.flags[c] != FLUID) continue;
         const uint32_t i = fdiv(c, g.f_nynz), rem = c - i * (g.ny * g.nz);
@@ -506,27 +541,36 @@ __global__ void __launch_bounds__(kBlock) k_divmax(Geo g, const R *u, const R *v
     const double m = block_max(d, lds);
     if (threadIdx.x == 0 && m > 0.0) atomic_max_nonneg(&rec->divmax_bits, m);
 }
+};
+template <typename R>
+__global__ void __launch_bounds__(kBlock) k_divmax(Geo g, const R *u, const R *v, const R *w,
+                                                   StepRec *rec)
+{
+    KB_divmax<R>::run(vb_here(), g, u, v, w, rec);
+}
 
 template <typename R>
 void launch_divmax(hipStream_t s, const Geo &g, const R *u, const R *v, const R *w, StepRec *rec)
 {
-    FFR_LAUNCH(k_divmax<R>, dim3(grid_for(g.nc)), dim3(kBlock), 0, s, g, u, v, w, rec);
+    FFR_GANG((KB_divmax<R>), GB_PLAIN, k_divmax<R>, dim3(grid_for(g.nc)), dim3(kBlock), 0, s, g, u,
+             v, w, rec);
 }
 
 // ---- K8: temperature transport (FluidSolver.cpp:388-456) ---------------
 template <typename R, bool UNI>
-__global__ void __launch_bounds__(kBlock) k_temperature(Geo g, StepParams sp, const R *u,
-                                                        const R *v, const R *w, const R *T,
-                                                        R *Tn, StepRec *rec, RedScratch red)
+struct KB_temperature {
+    static __device__ __forceinline__ void run(const VB &vb, const Geo &g, const StepParams &sp,
+                                               const R *u, const R *v, const R *w, const R *T,
+                                               R *Tn, StepRec *rec, const RedScratch &red)
 {
     __shared__ double lds[kBlock / kWave];
-    const uint32_t blk = xcd_block();
+    const uint32_t blk = vb.blk;
     double dmax = 0.0, tsum = 0.0;
     bool bad = false;
     const int nx = g.nx, ny = g.ny, nz = g.nz;
     const Axes<R> A = axes_of(g, R(0));
     uint32_t sb, se;
-    block_span(g.nc, blk, sb, se);
+    block_span_of(g.nc, blk, vb.G, sb, se);
     for (uint32_t c = sb + threadIdx.x; c < se; c += kBlock) {
         if (g.flags[c] != FLUID) {  // non-FLUID cells are frozen (:402-405)
             Tn[c] = T[c];
@@ -583,58 +627,84 @@ __global__ void __launch_bounds__(kBlock) k_temperature(Geo g, StepParams sp, co
     if (threadIdx.x == 0 && m > 0.0) atomic_max_nonneg(&rec->divmax_bits, m);
     block_partial(tsum, red.partials + SLOT_TSUM * kMaxBlocksCap, blk, lds);
 }
+};
+template <typename R, bool UNI>
+__global__ void __launch_bounds__(kBlock) k_temperature(Geo g, StepParams sp, const R *u,
+                                                        const R *v, const R *w, const R *T,
+                                                        R *Tn, StepRec *rec, RedScratch red)
+{
+    KB_temperature<R, UNI>::run(vb_here(), g, sp, u, v, w, T, Tn, rec, red);
+}
 
 template <typename R>
 void launch_temperature(hipStream_t s, const Geo &g, const StepParams &sp, const R *u,
                         const R *v, const R *w, const R *T, R *Tn, StepRec *rec, RedScratch red)
 {
     if (g.uniform)
-        FFR_LAUNCH((k_temperature<R, true>), dim3(grid_for(g.nc)), dim3(kBlock), 0, s, g, sp, u,
-                   v, w, T, Tn, rec, red);
+        FFR_GANG((KB_temperature<R, true>), GB_PLAIN, (k_temperature<R, true>),
+                 dim3(grid_for(g.nc)), dim3(kBlock), 0, s, g, sp, u, v, w, T, Tn, rec, red);
     else
-        FFR_LAUNCH((k_temperature<R, false>), dim3(grid_for(g.nc)), dim3(kBlock), 0, s, g, sp, u,
-                   v, w, T, Tn, rec, red);
+        FFR_GANG((KB_temperature<R, false>), GB_PLAIN, (k_temperature<R, false>),
+                 dim3(grid_for(g.nc)), dim3(kBlock), 0, s, g, sp, u, v, w, T, Tn, rec, red);
 }
 
 // Sum of T over FLUID cells (used when an opening overwrites FLUID cells).
 template <typename R>
-__global__ void __launch_bounds__(kBlock) k_tsum(Geo g, const R *T, StepRec *rec, RedScratch red)
+struct KB_tsum {
+    static __device__ __forceinline__ void run(const VB &vb, const Geo &g, const R *T, StepRec *rec,
+                                               const RedScratch &red)
 {
     __shared__ double lds[kBlock / kWave];
-    const uint32_t blk = xcd_block();
+    const uint32_t blk = vb.blk;
     double t = 0.0;
     uint32_t sb, se;
-    block_span(g.nc, blk, sb, se);
+    block_span_of(g.nc, blk, vb.G, sb, se);
     for (uint32_t c = sb + threadIdx.x; c < se; c += kBlock)
         if (g.flags[c] == FLUID) t += (double)T[c];
     block_partial(t, red.partials + SLOT_TSUM * kMaxBlocksCap, blk, lds);
+}
+};
+template <typename R>
+__global__ void __launch_bounds__(kBlock) k_tsum(Geo g, const R *T, StepRec *rec, RedScratch red)
+{
+    KB_tsum<R>::run(vb_here(), g, T, rec, red);
 }
 
 template <typename R>
 void launch_tsum(hipStream_t s, const Geo &g, const R *T, StepRec *rec, RedScratch red)
 {
-    FFR_LAUNCH(k_tsum<R>, dim3(grid_for(g.nc)), dim3(kBlock), 0, s, g, T, rec, red);
+    FFR_GANG((KB_tsum<R>), GB_PLAIN, k_tsum<R>, dim3(grid_for(g.nc)), dim3(kBlock), 0, s, g, T, rec,
+             red);
 }
 
 // rec->tsum from the per-block partials of temperature / tsum (one block).
-__global__ void __launch_bounds__(kBlock) k_step_finalize(StepRec *rec, RedScratch red, uint32_t G)
+struct KB_step_finalize {
+    static __device__ __forceinline__ void run(const VB &vb, StepRec *rec, const RedScratch &red,
+                                               uint32_t G)
 {
     __shared__ double lds[kBlock / kWave];
     const double t = sum_partials(red.partials + SLOT_TSUM * kMaxBlocksCap, G, lds);
     if (threadIdx.x == 0) rec->tsum = t;
 }
+};
+__global__ void __launch_bounds__(kBlock) k_step_finalize(StepRec *rec, RedScratch red, uint32_t G)
+{
+    KB_step_finalize::run(vb_here(), rec, red, G);
+}
 
 void launch_step_finalize(hipStream_t s, const Geo &g, StepRec *rec, RedScratch red)
 {
-    FFR_LAUNCH(k_step_finalize, dim3(1), dim3(kBlock), 0, s, rec, red, grid_for(g.nc));
+    FFR_GANG((KB_step_finalize), GB_PLAIN, k_step_finalize, dim3(1), dim3(kBlock), 0, s, rec, red,
+             grid_for(g.nc));
 }
 
 // ---- opening temperatures (BoundaryManager.cpp:137-178) -----------------
 // One block walks the openings in registration order, so overlapping
 // patches resolve exactly as the reference's sequential loop does.
 template <typename R>
-__global__ void __launch_bounds__(kBlock) k_opening_T(Geo g, const R *u, const R *v, const R *w,
-                                                      R *T)
+struct KB_opening_T {
+    static __device__ __forceinline__ void run(const VB &vb, const Geo &g, const R *u, const R *v,
+                                               const R *w, R *T)
 {
     const int nx = g.nx, ny = g.ny, nz = g.nz;
     for (int n = 0; n < g.n_openings; n++) {
@@ -665,12 +735,19 @@ __global__ void __launch_bounds__(kBlock) k_opening_T(Geo g, const R *u, const R
         __syncthreads();
     }
 }
+};
+template <typename R>
+__global__ void __launch_bounds__(kBlock) k_opening_T(Geo g, const R *u, const R *v, const R *w,
+                                                      R *T)
+{
+    KB_opening_T<R>::run(vb_here(), g, u, v, w, T);
+}
 
 template <typename R>
 void launch_opening_T(hipStream_t s, const Geo &g, const R *u, const R *v, const R *w, R *T)
 {
     if (g.n_openings == 0) return;
-    FFR_LAUNCH(k_opening_T<R>, dim3(1), dim3(kBlock), 0, s, g, u, v, w, T);
+    FFR_GANG((KB_opening_T<R>), GB_PLAIN, k_opening_T<R>, dim3(1), dim3(kBlock), 0, s, g, u, v, w, T);
 }
 
 // ---- precision conversion ------------------------------------------------

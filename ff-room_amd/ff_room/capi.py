@@ -90,6 +90,11 @@ class DeviceInfo(C.Structure):
                 ("lds_bytes_per_cu", C.c_size_t), ("clock_khz", C.c_int)]
 
 
+class GangStats(C.Structure):
+    _fields_ = [("rounds", C.c_uint64), ("launches", C.c_uint64), ("ops", C.c_uint64),
+                ("longest_ops", C.c_uint64), ("host_us", C.c_double)]
+
+
 class ProfEntry(C.Structure):
     _fields_ = [("name", C.c_char * 48), ("launches", C.c_long), ("total_ms", C.c_double),
                 ("bytes_per_launch", C.c_double)]
@@ -127,6 +132,8 @@ SIGNATURES = {
     "ffr_poll": (_I, [_P, C.c_uint64, C.POINTER(StepResult), C.POINTER(C.c_int)]),
     "ffr_drain": (_I, [_P]),
     "ffr_serve_config": (_I, [_P, C.c_int]),
+    "ffr_serve_gangs": (_I, [_P, C.c_int]),
+    "ffr_serve_gang_stats": (_I, [_P, C.c_void_p]),
     "ffr_progress": (_I, [_P, C.POINTER(C.c_int)]),
     "ffr_ticket_info": (_I, [_P, C.c_uint64, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
     "ffr_release": (_I, [_P, C.c_uint64]),
@@ -210,6 +217,16 @@ class Context:
     # ---- serving (continuous batching of many scenes) ----
     def serve_config(self, slots: int):
         self.check(self._l.ffr_serve_config(self.h, slots), "ffr_serve_config")
+
+    def serve_gangs(self, gangs: int):
+        """Merged launches over groups of scenes (0: every scene on its own stream)."""
+        self.check(self._l.ffr_serve_gangs(self.h, gangs), "ffr_serve_gangs")
+
+    def gang_stats(self) -> dict:
+        """Totals over all gangs (ffr_gang_stats)."""
+        st = GangStats()
+        self.check(self._l.ffr_serve_gang_stats(self.h, C.byref(st)), "ffr_serve_gang_stats")
+        return {k: getattr(st, k) for k, _ in GangStats._fields_}
 
     def submit(self, scene: "DeviceScene", max_steps: int) -> int:
         t = C.c_uint64(0)

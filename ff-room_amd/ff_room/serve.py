@@ -71,13 +71,16 @@ class Server:
     """Admission, submission and completion around the native serving loop.
 
     ``overrides`` are ffr_params fields applied to every request (for example
-    ``precision=capi.PREC_MIXED``)."""
+    ``precision=capi.PREC_MIXED``). ``gangs`` >= 1 advances the active scenes
+    in that many groups of merged launches (ffr_serve_gangs); 0 runs every
+    scene on its own stream."""
 
     def __init__(self, ctx: capi.Context, slots: int = 16, hbm_budget: Optional[int] = None,
-                 keep_fields: bool = False, tuned: bool = True, **overrides):
+                 keep_fields: bool = False, tuned: bool = True, gangs: int = 0, **overrides):
         self.ctx = ctx
         self.slots = slots
         ctx.serve_config(slots)
+        ctx.serve_gangs(gangs)
         info = ctx.device_info()
         self.budget = hbm_budget if hbm_budget is not None else int(info["hbm_bytes"] * 0.8)
         self.keep_fields = keep_fields

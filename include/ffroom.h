@@ -217,6 +217,25 @@ int ffr_poisson_solve(ffr_scene *scene, const double *rhs, double *p, ffr_solve_
  * admission: ffr_scene_create returns FFR_E_NOMEM beyond the context budget;
  * destroy finished scenes to admit new ones. */
 int ffr_serve_config(ffr_ctx *ctx, int slots);
+/* Gangs (no counterpart in the reference): with gangs >= 1, the active
+ * mixed-precision MG-PCG scenes are spread over `gangs` groups, and each
+ * group advances its scenes in rounds of merged launches: one launch runs the
+ * same kernel of every member whose step is at that kernel, each member with
+ * its own grid and arguments, so a round costs the launches of its longest
+ * member instead of the sum. Results are bit-identical to gangs = 0 (every
+ * scene on its own stream, the default); other scenes (FP64 form, plain CG)
+ * keep their own streams. Set it before submitting. */
+int ffr_serve_gangs(ffr_ctx *ctx, int gangs);
+/* Totals over the context's gangs since ffr_serve_gangs. */
+typedef struct ffr_gang_stats {
+    uint64_t rounds;      /* rounds issued */
+    uint64_t launches;    /* merged launches issued */
+    uint64_t ops;         /* operations the members recorded (launches and scalar
+                             copies): what their own streams would have issued */
+    uint64_t longest_ops; /* the longest member's operations, summed over rounds */
+    double host_us;       /* host time recording, merging and issuing rounds */
+} ffr_gang_stats;
+int ffr_serve_gang_stats(ffr_ctx *ctx, ffr_gang_stats *out);
 int ffr_submit(ffr_ctx *ctx, ffr_scene *scene, int max_steps, uint64_t *ticket);
 /* Advances all active tickets once; *done is set when this ticket finished
  * and *last receives its latest step result. Returns the ticket's error code

@@ -10,7 +10,7 @@ of FluidSolver::run's convergence test; and the ticket API's errors.
 import numpy as np
 import pytest
 
-from ff_room import capi, device
+from ff_room import autotune, capi, device
 from ff_room.generator import random_scene
 from ff_room.serve import Request, Server, summary
 
@@ -32,8 +32,12 @@ def _scenes():
     ]
 
 
-def _alone(ctx, cfg, steps):
+def _alone(ctx, cfg, steps, tuned=False):
     ps = device.prepare(cfg)
+    if tuned:  # the per-scene entries of the tuning table, as the Server applies them
+        entry = autotune.lookup(ps.nx, ps.ny, ps.nz, ps.params.precision)
+        if entry:
+            autotune.apply(ps.params, {k: v for k, v in entry.items() if k != "max_blocks"})
     d = device.create(ctx, ps)
     last = None
     for _ in range(steps):
@@ -103,3 +107,33 @@ def test_ticket_api(ctx):
     # the scene is usable alone afterwards
     assert d.step()["step"] == 3
     d.close()
+
+
+@pytest.mark.parametrize("gangs", [1, 3])
+def test_gangs_equal_alone_bitwise(ctx, gangs):
+    """Merged launches over groups of scenes (ffr_serve_gangs): every mixed
+    scene's fields and results equal its own-stream run bit for bit, with
+    scenes of different sizes (different hierarchies and launch sequences)
+    sharing launches; the FP64 scene keeps its own stream."""
+    cfgs = _scenes()
+    base = dict(dt=0.005, max_steps=K, convergence_tol=0.0, advection="quick", buoyancy=True,
+                beta=3.4e-3, alpha=2.1e-5, T_initial=26.0)
+    cfgs += [random_scene(105, grid=(64, 48, 30), n_fans=2, n_obstacles=4, precision="mixed", **base),
+             random_scene(106, grid=(17, 45, 21), n_fans=1, n_obstacles=1, precision="mixed", **base)]
+    alone = [_alone(ctx, c, K, tuned=True) for c in cfgs]
+    srv = Server(ctx, slots=len(cfgs), keep_fields=True, gangs=gangs)  # counts start at 0
+    done = srv.run([Request(c, K) for c in cfgs], realtime=False)
+    st = ctx.gang_stats()
+    ctx.serve_gangs(0)
+    for (fa, la), c in zip(alone, sorted(done, key=lambda c: c.index)):
+        assert c.steps == K
+        for k in ("u", "v", "w", "p", "T"):
+            assert np.array_equal(fa[k], c.fields[k]), (c.index, k)
+        for k in ("velocity_change", "pressure_residual", "divergence_max", "T_mean",
+                  "poisson_iters", "outer_iters"):
+            assert la[k] == c.last[k], (c.index, k)
+    rounds, launches, ops = st["rounds"], st["launches"], st["ops"]
+    assert rounds > 0
+    # five mixed scenes: fewer launches than their own streams would issue
+    # (all five share launches with one gang; 2 + 2 + 1 with three)
+    assert launches < (0.6 if gangs == 1 else 0.9) * ops, (launches, ops)

@@ -42,6 +42,9 @@ class _FakeCtx:
     def serve_config(self, slots):
         self.slots = slots
 
+    def serve_gangs(self, gangs):
+        self.gangs = gangs
+
     def device_info(self):
         return {"hbm_bytes": 10_000}
 
