@@ -194,7 +194,24 @@ PYBIND11_MODULE(_ffroom_core, m)
         .def_property_readonly("current_step", &FluidSolver::current_step)
         // extensions
         .def("compute_T_mean", &FluidSolver::compute_T_mean)
-        .def("sync_to_host", &FluidSolver::sync_to_host);
+        .def("sync_to_host", &FluidSolver::sync_to_host)
+        .def("snapshot_begin", &FluidSolver::snapshot_begin)
+        .def(
+            "snapshot_end",
+            [](FluidSolver &s, int slot) {
+                const Grid &g = s.grid();
+                py::array_t<double> u({g.Nx + 1, g.Ny, g.Nz}), v({g.Nx, g.Ny + 1, g.Nz}),
+                    w({g.Nx, g.Ny, g.Nz + 1}), p({g.Nx, g.Ny, g.Nz}), T({g.Nx, g.Ny, g.Nz});
+                double *pu = u.mutable_data(), *pv = v.mutable_data(), *pw = w.mutable_data(),
+                       *pp = p.mutable_data(), *pT = T.mutable_data();
+                int step;
+                {
+                    py::gil_scoped_release nogil;
+                    step = s.snapshot_end(slot, pu, pv, pw, pp, pT);
+                }
+                return py::make_tuple(step, u, v, w, p, T);
+            },
+            py::arg("slot"));
 
     // extensions: the pressure solvers on their own
     py::class_<PoissonSolver>(m, "PoissonSolver")

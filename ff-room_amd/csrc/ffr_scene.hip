@@ -476,6 +476,7 @@ void ffr_scene_destroy(ffr_scene *s)
     s->ctx->scenes.erase(s);
     serve_forget(s->ctx, s);
     hipStreamSynchronize(s->stream);
+    snapshot_release(s);
     if (s->task.ev) hipEventDestroy(s->task.ev);
     mg_destroy(s);
     if (s->arena) {

@@ -133,6 +133,17 @@ struct ffr_scene {
     int step = 0;
     ffr::StepTask task;
     ffr::SolvePred pred;
+
+    // snapshot egress (ffr_snapshot.hip): device slot, pinned mirror, events
+    struct Snap {
+        bool busy = false;
+        int step = 0;
+        void *dev = nullptr, *host = nullptr;
+        hipEvent_t copied = nullptr, landed = nullptr;
+    };
+    Snap snaps[FFR_SNAPSHOT_SLOTS];
+    size_t snap_bytes = 0;  // bytes of one slot
+    hipStream_t copy_stream = nullptr;
 };
 
 // Returns `code` from the calling function after recording the HIP error.
@@ -166,6 +177,8 @@ int task_resume(ffr_scene *s, bool *done);
 int task_prepare_gang(ffr_scene *s, bool *eligible);
 // serving (ffr_serve.hip)
 void serve_forget(ffr_ctx *c, ffr_scene *s);
+// snapshot egress (ffr_snapshot.hip)
+void snapshot_release(ffr_scene *s);
 
 struct Recorder;
 // Several scenes' step tasks advanced in rounds of merged launches on one

@@ -92,6 +92,21 @@ void FluidSolver::sync_to_host()
 
 void FluidSolver::mark_host_modified() { host_dirty_ = true; }
 
+int FluidSolver::snapshot_begin()
+{
+    push_if_dirty();
+    int slot = -1;
+    ffr_check(ffr_snapshot_begin(dev_->get(), &slot), "ffr_snapshot_begin");
+    return slot;
+}
+
+int FluidSolver::snapshot_end(int slot, double *u, double *v, double *w, double *p, double *T)
+{
+    int step = 0;
+    ffr_check(ffr_snapshot_end(dev_->get(), slot, &step, u, v, w, p, T), "ffr_snapshot_end");
+    return step;
+}
+
 void FluidSolver::push_if_dirty()
 {
     if (grid_.geometry_version + bm_.version() != geom_seen_) {

@@ -81,6 +81,11 @@ public:
     // device mirror control (no counterpart in the reference)
     void sync_to_host();       // download u, v, w, p, T into the Grid if stale
     void mark_host_modified(); // upload the Grid's fields before the next step
+    // snapshot egress (ffr_snapshot_begin/end): queue a copy of the current
+    // fields without waiting for it; collect it later (returns its step)
+    int snapshot_begin();
+    int snapshot_end(int slot, double *u, double *v, double *w, double *p, double *T);
+    const Grid &grid() const { return grid_; }
 
 private:
     Grid &grid_;

@@ -120,6 +120,8 @@ SIGNATURES = {
     "ffr_scene_set_params": (_I, [_P, C.POINTER(Params)]),
     "ffr_scene_upload": (_I, [_P, _dp, _dp, _dp, _dp, _dp]),
     "ffr_scene_download": (_I, [_P, _dp, _dp, _dp, _dp, _dp]),
+    "ffr_snapshot_begin": (_I, [_P, C.POINTER(C.c_int)]),
+    "ffr_snapshot_end": (_I, [_P, C.c_int, C.POINTER(C.c_int), _dp, _dp, _dp, _dp, _dp]),
     "ffr_scene_step_count": (_I, [_P]),
     "ffr_scene_set_step_count": (_I, [_P, C.c_int]),
     "ffr_step": (_I, [_P, C.POINTER(StepResult)]),
@@ -348,6 +350,21 @@ class DeviceScene:
         self.ctx.check(self.ctx._l.ffr_scene_download(
             self.h, *[_ptr(out[k]) for k in ("u", "v", "w", "p", "T")]), "ffr_scene_download")
         return out
+
+    def snapshot_begin(self) -> int:
+        """Queues a copy of the current fields (ffr_snapshot_begin); returns its slot."""
+        slot = C.c_int(-1)
+        self.ctx.check(self.ctx._l.ffr_snapshot_begin(self.h, C.byref(slot)), "ffr_snapshot_begin")
+        return slot.value
+
+    def snapshot_end(self, slot: int):
+        """(step at capture, fields) of a queued snapshot (ffr_snapshot_end)."""
+        out = {k: np.empty(n) for k, n in self.sizes.items()}
+        step = C.c_int(0)
+        self.ctx.check(self.ctx._l.ffr_snapshot_end(
+            self.h, slot, C.byref(step), *[_ptr(out[k]) for k in ("u", "v", "w", "p", "T")]),
+            "ffr_snapshot_end")
+        return step.value, out
 
     def step(self) -> dict:
         r = StepResult()

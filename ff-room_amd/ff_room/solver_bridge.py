@@ -155,23 +155,35 @@ class SolverBridge:
     def run_animated(self, save_every: int = 50, progress_callback: Optional[Callable] = None,
                      print_interval: int = 50):
         """Like run(), also capturing a FieldSnapshot every ``save_every``
-        steps and at the end. Returns (SimResult, list[FieldSnapshot])."""
+        steps and at the end. Returns (SimResult, list[FieldSnapshot]).
+
+        A capture is queued on the device (FluidSolver.snapshot_begin) and the
+        run goes on at once; its copy to the host overlaps the next steps and
+        it is turned into a FieldSnapshot later, keeping at most
+        ``_IN_FLIGHT`` captures pending."""
         scene, params, solver = self._solver()
         g = scene.grid
         dt = self.config.solver.dt
         snaps: List[FieldSnapshot] = []
+        pending: List[tuple] = []  # (slot, T_mean) in capture order
 
-        def snapshot(step: int, T_mean: float) -> FieldSnapshot:
-            u, v, w = np.array(g.u_shaped()), np.array(g.v_shaped()), np.array(g.w_shaped())
+        def make(step: int, T_mean: float, u, v, w, T) -> FieldSnapshot:
             uc, vc, wc = _cell_centred(u, v, w)
-            T = np.array(g.T_shaped()) if params.thermal else np.zeros(uc.shape)
+            T = np.asarray(T) if params.thermal else np.zeros(uc.shape)
             return FieldSnapshot(step=step, time_s=step * dt, T_mean=T_mean,
                                  speed=np.sqrt(uc ** 2 + vc ** 2 + wc ** 2), T_field=T,
                                  vel=np.stack([uc, vc, wc], axis=-1))
 
+        def collect():
+            slot, T_mean = pending.pop(0)
+            step, u, v, w, _p, T = solver.snapshot_end(slot)
+            snaps.append(make(step, T_mean, u, v, w, T))
+
         def _cb(res):
             if res.step % save_every == 0:
-                snaps.append(snapshot(res.step, res.T_mean))
+                if len(pending) >= self._IN_FLIGHT:
+                    collect()
+                pending.append((solver.snapshot_begin(), res.T_mean))
             if progress_callback:
                 progress_callback(res.step, res.velocity_change, res.divergence_max, res.T_mean)
             if print_interval > 0 and res.step % print_interval == 0:
@@ -180,6 +192,12 @@ class SolverBridge:
         t0 = time.perf_counter()
         final = solver.run(_cb)
         wall = time.perf_counter() - t0
+        while pending:
+            collect()
         if not snaps or snaps[-1].step != final.step:
-            snaps.append(snapshot(final.step, final.T_mean))
+            snaps.append(make(final.step, final.T_mean, np.array(g.u_shaped()),
+                              np.array(g.v_shaped()), np.array(g.w_shaped()),
+                              np.array(g.T_shaped())))
         return self._result(g, final, wall), snaps
+
+    _IN_FLIGHT = 3  # pending captures (the library holds up to FFR_SNAPSHOT_SLOTS)

@@ -183,6 +183,20 @@ int ffr_scene_create(ffr_ctx *ctx, const ffr_scene_desc *desc, const ffr_params 
                      ffr_scene **out);
 void ffr_scene_destroy(ffr_scene *scene);
 int ffr_scene_set_params(ffr_scene *scene, const ffr_params *params);
+/* Snapshot egress (the captures of SolverBridge.run_animated,
+ * solver_bridge.py:172-281, and the fields of SimResult): ffr_snapshot_begin
+ * queues a copy of the scene's fields as they are after its last step and
+ * returns a slot without waiting for them to reach the host; the next steps
+ * may be issued at once, and the copy to the host overlaps them. At most
+ * FFR_SNAPSHOT_SLOTS snapshots of a scene are pending (FFR_E_STATE beyond);
+ * each slot holds device memory counted against the context budget until the
+ * scene is destroyed. ffr_snapshot_end waits for slot `slot`, writes its
+ * fields in the reference layout as FP64 (NULL skips a field), the step count
+ * at capture into *step, and frees the slot. */
+#define FFR_SNAPSHOT_SLOTS 4
+int ffr_snapshot_begin(ffr_scene *scene, int *slot);
+int ffr_snapshot_end(ffr_scene *scene, int slot, int *step, double *u, double *v, double *w,
+                     double *p, double *T);
 /* Host <-> device state copies; a NULL pointer leaves that field alone. */
 int ffr_scene_upload(ffr_scene *scene, const double *u, const double *v, const double *w,
                      const double *p, const double *T);
