@@ -95,6 +95,15 @@ void launch_convert_back(hipStream_t s, const R *src, double *dst, uint64_t n);
 
 // face mask codes of a scene (once, at creation)
 void launch_face_codes(hipStream_t s, const Geo &g, uint16_t *fcode);
+// Cell flags from marks (type, i0, i1, j0, j1, k0, k1) applied in order, and
+// the neighbour codes of the pressure operator from the flags (at creation).
+constexpr int kMaxMarks = 64;
+struct MarkList {
+    int n;
+    int m[kMaxMarks][7];
+};
+void launch_mark_cells(hipStream_t s, int nx, int ny, int nz, const MarkList &ml, uint8_t *flags);
+void launch_nb_codes(hipStream_t s, int nx, int ny, int nz, const uint8_t *flags, uint16_t *nbm);
 // plain FP64 copy (calibration of memory counters)
 void launch_copy64(hipStream_t s, const double *src, double *dst, uint64_t n);
 

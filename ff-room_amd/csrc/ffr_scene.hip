@@ -25,9 +25,20 @@ bool valid_desc(const ffr_scene_desc *d, char *err, size_t errn)
                            (uint64_t)d->nx * d->ny * (d->nz + 1);
     if (faces >= (1ull << 31)) { snprintf(err, errn, "grid too large for 32-bit indexing"); return false; }
     if (d->n_fans > kMaxFans) { snprintf(err, errn, "at most %d fans per scene", kMaxFans); return false; }
-    if (!d->dxv || !d->dyv || !d->dzv || !d->cell_type) {
-        snprintf(err, errn, "spacings and cell_type are required");
+    if (!d->dxv || !d->dyv || !d->dzv) {
+        snprintf(err, errn, "spacings are required");
         return false;
+    }
+    if (!d->cell_type) {  // flags from marks, built on the device
+        if (d->n_marks < 0 || d->n_marks > FFR_MAX_MARKS || (d->n_marks && !d->marks)) {
+            snprintf(err, errn, "cell_type or 0..%d marks are required", FFR_MAX_MARKS);
+            return false;
+        }
+        for (int q = 0; q < d->n_marks; q++)
+            if (d->marks[q].type < 0 || d->marks[q].type > 3) {
+                snprintf(err, errn, "mark %d: type %d out of range", q, d->marks[q].type);
+                return false;
+            }
     }
     const int n[3] = {d->nx, d->ny, d->nz};
     const double *h[3] = {d->dxv, d->dyv, d->dzv};
@@ -63,9 +74,11 @@ bool valid_desc(const ffr_scene_desc *d, char *err, size_t errn)
         const ffr_opening &o = d->openings[q];
         if (!check_patch(o.axis, o.side, o.a_min, o.a_max, o.b_min, o.b_max, "opening")) return false;
     }
-    const long nc = (long)d->nx * d->ny * d->nz;
-    for (long c = 0; c < nc; c++)
-        if (d->cell_type[c] > 3) { snprintf(err, errn, "cell_type[%ld] out of range", c); return false; }
+    if (d->cell_type) {
+        const long nc = (long)d->nx * d->ny * d->nz;
+        for (long c = 0; c < nc; c++)
+            if (d->cell_type[c] > 3) { snprintf(err, errn, "cell_type[%ld] out of range", c); return false; }
+    }
     return true;
 }
 
@@ -217,7 +230,7 @@ int ffr_scene_create(ffr_ctx *ctx, const ffr_scene_desc *d, const ffr_params *pr
     s->dxv.assign(d->dxv, d->dxv + d->nx);
     s->dyv.assign(d->dyv, d->dyv + d->ny);
     s->dzv.assign(d->dzv, d->dzv + d->nz);
-    s->flags.assign(d->cell_type, d->cell_type + s->nc);
+    if (d->cell_type) s->flags.assign(d->cell_type, d->cell_type + s->nc);
     s->fans.assign(d->fans, d->fans + d->n_fans);
     s->outflows.assign(d->outflows, d->outflows + d->n_outflows);
     s->openings.assign(d->openings, d->openings + d->n_openings);
@@ -244,6 +257,25 @@ int ffr_scene_create(ffr_ctx *ctx, const ffr_scene_desc *d, const ffr_params *pr
     ctx->in_use += bytes;
     ArenaPtrs a;
     layout((char *)s->arena, d, prm, &a);
+    if (!d->cell_type) {  // the flags are marked on the device; the host keeps a copy
+        MarkList ml{};
+        ml.n = d->n_marks;
+        for (int q = 0; q < d->n_marks; q++) {
+            const ffr_mark &m = d->marks[q];
+            const int v[7] = {m.type, m.i_min, m.i_max, m.j_min, m.j_max, m.k_min, m.k_max};
+            for (int e = 0; e < 7; e++) ml.m[q][e] = v[e];
+        }
+        s->flags.resize(s->nc);
+        launch_mark_cells(s->stream, d->nx, d->ny, d->nz, ml, a.flags);
+        if (hipMemcpyAsync(s->flags.data(), a.flags, s->nc, hipMemcpyDeviceToHost, s->stream) != hipSuccess ||
+            hipStreamSynchronize(s->stream) != hipSuccess) {
+            ctx->fail("marking the cell flags failed");
+            hipFree(s->arena);
+            ctx->in_use -= bytes;
+            delete s;
+            return FFR_E_DEVICE;
+        }
+    }
 
     // ---- geometry, computed on the host exactly as the reference does -----
     const int nx = d->nx, ny = d->ny, nz = d->nz, na = nx + ny + nz;
@@ -339,36 +371,13 @@ int ffr_scene_create(ffr_ctx *ctx, const ffr_scene_desc *d, const ffr_params *pr
                 if (s->flags[c] == FLUID) s->opening_touches_fluid = true;
             }
 
-    // neighbour codes of the pressure operator
-    std::vector<uint16_t> nbm(s->nc);
-    {
-        const uint8_t *f = s->flags.data();
-        auto unk = [](uint8_t t) { return t == FLUID || t == INFLOW; };
-        for (int i = 0; i < nx; i++)
-            for (int j = 0; j < ny; j++)
-                for (int k = 0; k < nz; k++) {
-                    const long c = ((long)i * ny + j) * nz + k;
-                    uint16_t m = 0;
-                    if (unk(f[c])) m |= NB_SELF;
-                    if (f[c] == OUTFLOW) m |= NB_SELF_OUT;
-                    if (f[c] == SOLID) m |= NB_SELF_SOLID;
-                    if ((i + j + k) & 1) m |= NB_COLOR;
-                    const long nb[6] = {i > 0 ? c - (long)ny * nz : -1, i < nx - 1 ? c + (long)ny * nz : -1,
-                                        j > 0 ? c - nz : -1, j < ny - 1 ? c + nz : -1,
-                                        k > 0 ? c - 1 : -1, k < nz - 1 ? c + 1 : -1};
-                    for (int q = 0; q < 6; q++) {
-                        if (nb[q] < 0) continue;
-                        if (unk(f[nb[q]])) m |= (uint16_t)(NB_UNK0 << q);
-                        else if (f[nb[q]] == OUTFLOW) m |= (uint16_t)(NB_OUT0 << q);
-                    }
-                    nbm[c] = m;
-                }
-    }
-
     hipStream_t st = s->stream;
     bool ok = true;
-    ok &= hipMemcpyAsync(a.flags, s->flags.data(), s->nc, hipMemcpyHostToDevice, st) == hipSuccess;
-    ok &= hipMemcpyAsync(a.nbm, nbm.data(), s->nc * 2, hipMemcpyHostToDevice, st) == hipSuccess;
+    if (d->cell_type)
+        ok &= hipMemcpyAsync(a.flags, s->flags.data(), s->nc, hipMemcpyHostToDevice, st) == hipSuccess;
+    // neighbour codes of the pressure operator, from the flags on the device
+    launch_nb_codes(st, nx, ny, nz, a.flags, a.nbm);
+    ok &= hipGetLastError() == hipSuccess;
     ok &= hipMemcpyAsync(a.ax64, ax64.data(), ax64.size() * 8, hipMemcpyHostToDevice, st) == hipSuccess;
     ok &= hipMemcpyAsync(a.ax32, ax32.data(), ax32.size() * 4, hipMemcpyHostToDevice, st) == hipSuccess;
     ok &= hipMemcpyAsync(a.fans, fans.data(), fans.size() * sizeof(FanDev), hipMemcpyHostToDevice, st) == hipSuccess;
@@ -488,6 +497,13 @@ void ffr_scene_destroy(ffr_scene *s)
     if (s->h_mgs) hipHostFree(s->h_mgs);
     if (s->own_stream) hipStreamDestroy(s->stream);
     delete s;
+}
+
+int ffr_scene_cell_type(const ffr_scene *s, uint8_t *cell_type)
+{
+    if (!s || !cell_type) return FFR_E_INVALID;
+    std::copy(s->flags.begin(), s->flags.end(), cell_type);
+    return FFR_OK;
 }
 
 int ffr_scene_set_params(ffr_scene *s, const ffr_params *p)

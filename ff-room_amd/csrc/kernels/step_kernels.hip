@@ -195,6 +195,69 @@ void launch_face_codes(hipStream_t s, const Geo &g, uint16_t *fcode)
     FFR_LAUNCH(k_face_codes, dim3(grid_for(n)), dim3(kBlock), 0, s, g, fcode);
 }
 
+// ---- scene construction on the device ------------------------------------
+// Cell flags from marks applied in order, FLUID where none applies: a cell
+// takes the type of the last mark whose box holds it, which is what marking
+// the boxes one after the other gives (obstacles, fans, openings, automatic
+// outflow: scene.py:89-106; boxes of BoundaryManager.cpp:6-44, 132-135,
+// clamped to the grid - a box reaching past the grid holds only its cells).
+__global__ void __launch_bounds__(kBlock) k_mark_cells(int nx, int ny, int nz, MarkList ml,
+                                                        uint8_t *flags)
+{
+    const uint32_t nc = (uint32_t)nx * ny * nz, nynz = (uint32_t)ny * nz;
+    for (uint32_t c = blockIdx.x * kBlock + threadIdx.x; c < nc; c += gridDim.x * kBlock) {
+        const int i = (int)(c / nynz), rem = (int)(c - (uint32_t)i * nynz);
+        const int j = rem / nz, k = rem - j * nz;
+        uint8_t t = FLUID;
+        for (int q = 0; q < ml.n; q++) {
+            const int *m = ml.m[q];
+            if (i >= m[1] && i <= m[2] && j >= m[3] && j <= m[4] && k >= m[5] && k <= m[6])
+                t = (uint8_t)m[0];
+        }
+        flags[c] = t;
+    }
+}
+
+// Neighbour codes of the pressure operator (NB_* bits) from the flags: the
+// row structure of PoissonSolver.cpp:52-87 per cell.
+__global__ void __launch_bounds__(kBlock) k_nb_codes(int nx, int ny, int nz, const uint8_t *flags,
+                                                      uint16_t *nbm)
+{
+    const uint32_t nc = (uint32_t)nx * ny * nz, nynz = (uint32_t)ny * nz;
+    for (uint32_t c = blockIdx.x * kBlock + threadIdx.x; c < nc; c += gridDim.x * kBlock) {
+        const int i = (int)(c / nynz), rem = (int)(c - (uint32_t)i * nynz);
+        const int j = rem / nz, k = rem - j * nz;
+        const uint8_t f = flags[c];
+        uint16_t m = 0;
+        if (f == FLUID || f == INFLOW) m |= NB_SELF;
+        if (f == OUTFLOW) m |= NB_SELF_OUT;
+        if (f == SOLID) m |= NB_SELF_SOLID;
+        if ((i + j + k) & 1) m |= NB_COLOR;
+        const bool has[6] = {i > 0, i < nx - 1, j > 0, j < ny - 1, k > 0, k < nz - 1};
+        const uint32_t nb[6] = {c - nynz, c + nynz, c - (uint32_t)nz, c + (uint32_t)nz, c - 1, c + 1};
+#pragma unroll
+        for (int q = 0; q < 6; q++) {
+            if (!has[q]) continue;
+            const uint8_t fn = flags[nb[q]];
+            if (fn == FLUID || fn == INFLOW) m |= (uint16_t)(NB_UNK0 << q);
+            else if (fn == OUTFLOW) m |= (uint16_t)(NB_OUT0 << q);
+        }
+        nbm[c] = m;
+    }
+}
+
+void launch_mark_cells(hipStream_t s, int nx, int ny, int nz, const MarkList &ml, uint8_t *flags)
+{
+    FFR_LAUNCH(k_mark_cells, dim3(grid_for((uint64_t)nx * ny * nz)), dim3(kBlock), 0, s, nx, ny,
+               nz, ml, flags);
+}
+
+void launch_nb_codes(hipStream_t s, int nx, int ny, int nz, const uint8_t *flags, uint16_t *nbm)
+{
+    FFR_LAUNCH(k_nb_codes, dim3(grid_for((uint64_t)nx * ny * nz)), dim3(kBlock), 0, s, nx, ny, nz,
+               flags, nbm);
+}
+
 // the masks of a face from its code
 template <typename R>
 __device__ __forceinline__ R apply_mask(const Geo &g, uint16_t code, int axis, R val)

@@ -47,12 +47,18 @@ class Opening(C.Structure):
 _dp = C.POINTER(C.c_double)
 
 
+class Mark(C.Structure):
+    _fields_ = [("type", C.c_int), ("i_min", C.c_int), ("i_max", C.c_int), ("j_min", C.c_int),
+                ("j_max", C.c_int), ("k_min", C.c_int), ("k_max", C.c_int)]
+
+
 class SceneDesc(C.Structure):
     _fields_ = [("nx", C.c_int), ("ny", C.c_int), ("nz", C.c_int),
                 ("dxv", _dp), ("dyv", _dp), ("dzv", _dp), ("cell_type", C.POINTER(C.c_uint8)),
                 ("fans", C.POINTER(Fan)), ("n_fans", C.c_int),
                 ("outflows", C.POINTER(Outflow)), ("n_outflows", C.c_int),
-                ("openings", C.POINTER(Opening)), ("n_openings", C.c_int)]
+                ("openings", C.POINTER(Opening)), ("n_openings", C.c_int),
+                ("marks", C.POINTER(Mark)), ("n_marks", C.c_int)]
 
 
 class Params(C.Structure):
@@ -120,6 +126,7 @@ SIGNATURES = {
     "ffr_scene_set_params": (_I, [_P, C.POINTER(Params)]),
     "ffr_scene_upload": (_I, [_P, _dp, _dp, _dp, _dp, _dp]),
     "ffr_scene_download": (_I, [_P, _dp, _dp, _dp, _dp, _dp]),
+    "ffr_scene_cell_type": (_I, [_P, C.POINTER(C.c_uint8)]),
     "ffr_snapshot_begin": (_I, [_P, C.POINTER(C.c_int)]),
     "ffr_snapshot_end": (_I, [_P, C.c_int, C.POINTER(C.c_int), _dp, _dp, _dp, _dp, _dp]),
     "ffr_scene_step_count": (_I, [_P]),
@@ -275,12 +282,19 @@ class Context:
                 for i in range(n.value)}
 
 
-def make_desc(nx, ny, nz, dxv, dyv, dzv, flags, fans=(), outflows=(), openings=()):
-    """Returns (SceneDesc, keepalive): the arrays must outlive the descriptor."""
+def make_desc(nx, ny, nz, dxv, dyv, dzv, flags, fans=(), outflows=(), openings=(), marks=()):
+    """Returns (SceneDesc, keepalive): the arrays must outlive the descriptor.
+    With flags None the library marks the cells on the device from `marks`
+    [(type, i0, i1, j0, j1, k0, k1)] in order."""
     dxv = np.ascontiguousarray(dxv, dtype=np.float64)
     dyv = np.ascontiguousarray(dyv, dtype=np.float64)
     dzv = np.ascontiguousarray(dzv, dtype=np.float64)
-    flags = np.ascontiguousarray(flags, dtype=np.uint8).reshape(-1)
+    if flags is not None:
+        flags = np.ascontiguousarray(flags, dtype=np.uint8).reshape(-1)
+    ma = (Mark * max(1, len(marks)))()
+    for n, m in enumerate(marks):
+        (ma[n].type, ma[n].i_min, ma[n].i_max, ma[n].j_min, ma[n].j_max, ma[n].k_min,
+         ma[n].k_max) = [int(x) for x in m]
     fa = (Fan * max(1, len(fans)))()
     for n, f in enumerate(fans):
         fa[n].i_min, fa[n].i_max, fa[n].j_min, fa[n].j_max, fa[n].k_min, fa[n].k_max = [int(x) for x in f[:6]]
@@ -295,22 +309,24 @@ def make_desc(nx, ny, nz, dxv, dyv, dzv, flags, fans=(), outflows=(), openings=(
     d = SceneDesc()
     d.nx, d.ny, d.nz = nx, ny, nz
     d.dxv, d.dyv, d.dzv = _ptr(dxv), _ptr(dyv), _ptr(dzv)
-    d.cell_type = flags.ctypes.data_as(C.POINTER(C.c_uint8))
+    d.cell_type = flags.ctypes.data_as(C.POINTER(C.c_uint8)) if flags is not None else None
     d.fans, d.n_fans = fa, len(fans)
     d.outflows, d.n_outflows = oa, len(outflows)
     d.openings, d.n_openings = pa, len(openings)
-    return d, (dxv, dyv, dzv, flags, fa, oa, pa)
+    d.marks, d.n_marks = ma, (0 if flags is not None else len(marks))
+    return d, (dxv, dyv, dzv, flags, fa, oa, pa, ma)
 
 
 class DeviceScene:
     """A scene resident on the GPU (ffr_scene)."""
 
     def __init__(self, ctx: Context, nx, ny, nz, dxv, dyv, dzv, flags, fans=(), outflows=(),
-                 openings=(), params: Optional[Params] = None):
+                 openings=(), params: Optional[Params] = None, marks=()):
         self.ctx = ctx
         self.nx, self.ny, self.nz = nx, ny, nz
         self.params = params if params is not None else default_params()
-        self.desc, self._keep = make_desc(nx, ny, nz, dxv, dyv, dzv, flags, fans, outflows, openings)
+        self.desc, self._keep = make_desc(nx, ny, nz, dxv, dyv, dzv, flags, fans, outflows, openings,
+                                          marks)
         h = C.c_void_p()
         ctx.check(ctx._l.ffr_scene_create(ctx.h, C.byref(self.desc), C.byref(self.params),
                                           C.byref(h)), "ffr_scene_create")
@@ -349,6 +365,13 @@ class DeviceScene:
         out = {k: np.empty(n) for k, n in self.sizes.items()}
         self.ctx.check(self.ctx._l.ffr_scene_download(
             self.h, *[_ptr(out[k]) for k in ("u", "v", "w", "p", "T")]), "ffr_scene_download")
+        return out
+
+    def cell_type(self) -> np.ndarray:
+        """The scene's cell flags (ffr_scene_cell_type), nx*ny*nz bytes."""
+        out = np.empty(self.nx * self.ny * self.nz, dtype=np.uint8)
+        self.ctx.check(self.ctx._l.ffr_scene_cell_type(
+            self.h, out.ctypes.data_as(C.POINTER(C.c_uint8))), "ffr_scene_cell_type")
         return out
 
     def snapshot_begin(self) -> int:

@@ -55,6 +55,110 @@ def _stretched_faces(N: int, L: float, r: float) -> List[float]:
     return [f * scale for f in faces]
 
 
+# ---- the boxes the scene marks, as pure functions of the configuration ----
+# Scene (below) hands them to the host BoundaryManager; cell_marks and
+# boundary_lists give the same boxes to a scene whose flags are marked on the
+# device (ffr_scene_desc.marks), so both paths share this arithmetic.
+FLUID, SOLID, INFLOW, OUTFLOW = 0, 1, 2, 3
+
+
+def _cells(config: SceneConfig, axis: int):
+    r = config.room
+    return float(r.size[axis]), int(r.grid[axis])
+
+
+def obstacle_box(config: SceneConfig, ob: ObstacleConfig):
+    """(i0, i1, j0, j1, k0, k1) of an obstacle, unclamped (add_solid_box clamps)."""
+    lo, hi = [], []
+    for a in range(3):
+        L, N = _cells(config, a)
+        lo.append(int(ob.bbox_min[a] / L * N))
+        hi.append(int(ob.bbox_max[a] / L * N) - 1)
+    return lo[0], hi[0], lo[1], hi[1], lo[2], hi[2]
+
+
+def fan_box(config: SceneConfig, fan: FanConfig):
+    """Cell box of a fan: one cell thick along its snapped axis, its radius
+    across, clamped to the grid."""
+    ax = fan.snapped_axis()
+    box = []
+    for a in range(3):
+        L, N = _cells(config, a)
+        if a == ax:  # one cell thick along the blowing axis
+            c = max(0, min(N - 1, int(fan.position[a] / L * N)))
+            box += [c, c]
+        else:
+            box += [max(0, int((fan.position[a] - fan.radius) / L * N)),
+                    min(N - 1, int((fan.position[a] + fan.radius) / L * N))]
+    return tuple(box)
+
+
+def fan_velocity(fan: FanConfig):
+    vec = fan.direction_vector()
+    return (vec[0] * fan.velocity, vec[1] * fan.velocity, vec[2] * fan.velocity)
+
+
+def opening_patch(config: SceneConfig, op: OpeningConfig):
+    """(axis, side, a0, a1, b0, b1) of an opening on its wall."""
+    ax, sd = _WALLS[op.wall.lower()]  # KeyError for an unknown wall, as in the reference
+    ta, tb = [a for a in range(3) if a != ax]
+
+    def span(c, w, a):
+        L, N = _cells(config, a)
+        return max(0, int((c - w / 2) / L * N)), min(N - 1, int((c + w / 2) / L * N))
+
+    a_lo, a_hi = span(op.center[0], op.size[0], ta)
+    b_lo, b_hi = span(op.center[1], op.size[1], tb)
+    return ax, sd, a_lo, a_hi, b_lo, b_hi
+
+
+def auto_outflow_patch(config: SceneConfig, fan: FanConfig):
+    """The whole wall the first fan blows towards."""
+    ax = fan.snapped_axis()
+    ta, tb = [a for a in range(3) if a != ax]
+    return ax, 1 if fan.snapped_sign() > 0 else 0, 0, _cells(config, ta)[1] - 1, 0, \
+        _cells(config, tb)[1] - 1
+
+
+def patch_box(config: SceneConfig, axis: int, side: int, a0: int, a1: int, b0: int, b1: int):
+    """Cell box of a wall patch: the wall's layer of cells, a and b the two
+    other axes in order (BoundaryManager.cpp:24-44)."""
+    w = (_cells(config, axis)[1] - 1) if side else 0
+    box = [None] * 3
+    box[axis] = (w, w)
+    ta, tb = [a for a in range(3) if a != axis]
+    box[ta], box[tb] = (a0, a1), (b0, b1)
+    return box[0] + box[1] + box[2]
+
+
+def opening_T_outside(config: SceneConfig, op: OpeningConfig) -> float:
+    return op.T_outside if op.T_outside is not None else config.solver.T_outside
+
+
+def cell_marks(config: SceneConfig):
+    """[(type, i0, i1, j0, j1, k0, k1)] in the marking order of Scene._build:
+    obstacles, fans, openings, then the automatic outflow (scene.py:89-106)."""
+    marks = [(SOLID,) + tuple(obstacle_box(config, ob)) for ob in config.obstacles]
+    marks += [(INFLOW,) + fan_box(config, f) for f in config.fans]
+    marks += [(OUTFLOW,) + tuple(patch_box(config, *opening_patch(config, op)))
+              for op in config.openings]
+    if config.fans and not config.openings:
+        marks.append((OUTFLOW,) + tuple(patch_box(config, *auto_outflow_patch(config, config.fans[0]))))
+    return marks
+
+
+def boundary_lists(config: SceneConfig):
+    """(fans, outflows, openings) as the BoundaryManager registers them:
+    fans as (box..., vx, vy, vz); outflows hold every opening's patch, then
+    the automatic outflow; openings add their outside temperature."""
+    fans = [fan_box(config, f) + fan_velocity(f) for f in config.fans]
+    opens = [opening_patch(config, op) + (opening_T_outside(config, op),) for op in config.openings]
+    outs = [o[:6] for o in opens]
+    if config.fans and not config.openings:
+        outs.append(auto_outflow_patch(config, config.fans[0]))
+    return fans, outs, opens
+
+
 class Scene:
     """A built scene: ``grid`` and ``bm`` objects of ``_ffroom_core``."""
 
@@ -90,54 +194,23 @@ class Scene:
 
     def _add_auto_outflow(self, fan: FanConfig) -> None:
         """Whole wall the first fan blows towards becomes OUTFLOW (p = 0)."""
-        ax = fan.snapped_axis()
         o = self._c.OutflowBC()
-        o.axis = ax
-        o.side = 1 if fan.snapped_sign() > 0 else 0
-        ta, tb = [a for a in range(3) if a != ax]
-        o.a_min, o.a_max = 0, self._cells(ta)[1] - 1
-        o.b_min, o.b_max = 0, self._cells(tb)[1] - 1
+        o.axis, o.side, o.a_min, o.a_max, o.b_min, o.b_max = auto_outflow_patch(self.config, fan)
         self.bm.add_outflow(self.grid, o)
 
     def _add_obstacle(self, ob: ObstacleConfig) -> None:
-        lo, hi = [], []
-        for a in range(3):
-            L, N = self._cells(a)
-            lo.append(int(ob.bbox_min[a] / L * N))
-            hi.append(int(ob.bbox_max[a] / L * N) - 1)
-        self.bm.add_solid_box(self.grid, lo[0], hi[0], lo[1], hi[1], lo[2], hi[2])
+        self.bm.add_solid_box(self.grid, *obstacle_box(self.config, ob))
 
     def _add_fan(self, fan: FanConfig) -> None:
-        ax = fan.snapped_axis()
-        vec = fan.direction_vector()
         bc = self._c.FanBC()
-        bc.vel = [vec[0] * fan.velocity, vec[1] * fan.velocity, vec[2] * fan.velocity]
-        box = []
-        for a in range(3):
-            L, N = self._cells(a)
-            if a == ax:  # one cell thick along the blowing axis
-                c = max(0, min(N - 1, int(fan.position[a] / L * N)))
-                box += [c, c]
-            else:
-                box += [max(0, int((fan.position[a] - fan.radius) / L * N)),
-                        min(N - 1, int((fan.position[a] + fan.radius) / L * N))]
-        bc.i_min, bc.i_max, bc.j_min, bc.j_max, bc.k_min, bc.k_max = box
+        bc.vel = list(fan_velocity(fan))
+        bc.i_min, bc.i_max, bc.j_min, bc.j_max, bc.k_min, bc.k_max = fan_box(self.config, fan)
         self.bm.add_fan(self.grid, bc)
 
     def _add_opening(self, op: OpeningConfig) -> None:
-        ax, sd = _WALLS[op.wall.lower()]  # KeyError for an unknown wall, as in the reference
-        ta, tb = [a for a in range(3) if a != ax]
-
-        def span(c, w, a):
-            L, N = self._cells(a)
-            return max(0, int((c - w / 2) / L * N)), min(N - 1, int((c + w / 2) / L * N))
-
-        a_lo, a_hi = span(op.center[0], op.size[0], ta)
-        b_lo, b_hi = span(op.center[1], op.size[1], tb)
         o = self._c.OpeningBC()
-        o.axis, o.side = ax, sd
-        o.a_min, o.a_max, o.b_min, o.b_max = a_lo, a_hi, b_lo, b_hi
-        o.T_outside = op.T_outside if op.T_outside is not None else self.config.solver.T_outside
+        o.axis, o.side, o.a_min, o.a_max, o.b_min, o.b_max = opening_patch(self.config, op)
+        o.T_outside = opening_T_outside(self.config, op)
         self.bm.add_opening(self.grid, o)
 
     def rebuild(self) -> None:

@@ -105,7 +105,8 @@ class Server:
         while self.pending and len(self.active) < self.slots:
             i, req, ps = self.pending[0]
             if ps is None:
-                ps = device.prepare(req.config, **self.overrides)
+                # the cells are marked on the device: no host Grid per arrival
+                ps = device.prepare(req.config, on_device=True, **self.overrides)
                 if self.tuned:  # per-scene settings of the autotuner's table for this size class
                     entry = autotune.lookup(ps.nx, ps.ny, ps.nz, ps.params.precision)
                     if entry:

@@ -73,19 +73,33 @@ typedef struct {
     double T_outside;
 } ffr_opening;
 
+/* A box of cells given one type (FFR_SOLID, FFR_INFLOW, FFR_OUTFLOW or
+ * FFR_FLUID), inclusive ranges; parts outside the grid are ignored, as the
+ * BoundaryManager's markers clamp them (BoundaryManager.cpp:6-44). */
+typedef struct {
+    int type;
+    int i_min, i_max, j_min, j_max, k_min, k_max;
+} ffr_mark;
+#define FFR_MAX_MARKS 64
+
 /* Geometry and boundary lists of one scene (what Grid + BoundaryManager
  * hold). `outflows` lists every registered outflow patch, openings included,
- * as BoundaryManager::outflows() does (BoundaryManager.cpp:24-44, 132-135). */
+ * as BoundaryManager::outflows() does (BoundaryManager.cpp:24-44, 132-135).
+ * The cell flags come either from `cell_type` or, when it is NULL, from
+ * `marks` applied in order on the device (FLUID where no mark applies; the
+ * last mark holding a cell wins - the marking order of scene.py:89-106). */
 typedef struct {
     int nx, ny, nz;
     const double *dxv, *dyv, *dzv;  /* cell widths, sizes nx, ny, nz */
-    const uint8_t *cell_type;       /* nx*ny*nz */
+    const uint8_t *cell_type;       /* nx*ny*nz, or NULL to use marks */
     const ffr_fan *fans;
     int n_fans;
     const ffr_outflow *outflows;
     int n_outflows;
     const ffr_opening *openings;
     int n_openings;
+    const ffr_mark *marks;          /* used when cell_type is NULL */
+    int n_marks;                    /* 0..FFR_MAX_MARKS */
 } ffr_scene_desc;
 
 /* FluidSolverParams (FluidSolver.hpp:11-38) + PoissonSolverParams
@@ -183,6 +197,8 @@ int ffr_scene_create(ffr_ctx *ctx, const ffr_scene_desc *desc, const ffr_params 
                      ffr_scene **out);
 void ffr_scene_destroy(ffr_scene *scene);
 int ffr_scene_set_params(ffr_scene *scene, const ffr_params *params);
+/* The scene's cell flags (nx*ny*nz bytes), however they were given. */
+int ffr_scene_cell_type(const ffr_scene *scene, uint8_t *cell_type);
 /* Snapshot egress (the captures of SolverBridge.run_animated,
  * solver_bridge.py:172-281, and the fields of SimResult): ffr_snapshot_begin
  * queues a copy of the scene's fields as they are after its last step and
