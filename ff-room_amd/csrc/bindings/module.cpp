@@ -195,16 +195,30 @@ PYBIND11_MODULE(_ffroom_core, m)
         // extensions
         .def("compute_T_mean", &FluidSolver::compute_T_mean)
         .def("sync_to_host", &FluidSolver::sync_to_host)
-        .def("snapshot_begin", &FluidSolver::snapshot_begin)
+        .def(
+            "snapshot_begin",
+            [](FluidSolver &s, bool cells) { return s.snapshot_begin(cells ? FFR_SNAP_CELLS : FFR_SNAP_FIELDS); },
+            py::arg("cells") = false)
         .def(
             "snapshot_end",
-            [](FluidSolver &s, int slot) {
+            // fields: (step, u, v, w, p, T); cells: (step, vel, speed, T)
+            [](FluidSolver &s, int slot) -> py::tuple {
                 const Grid &g = s.grid();
+                int step;
+                if (s.snapshot_kind(slot) == FFR_SNAP_CELLS) {
+                    py::array_t<double> vel({g.Nx, g.Ny, g.Nz, 3}), speed({g.Nx, g.Ny, g.Nz}),
+                        T({g.Nx, g.Ny, g.Nz});
+                    double *pv = vel.mutable_data(), *ps = speed.mutable_data(), *pT = T.mutable_data();
+                    {
+                        py::gil_scoped_release nogil;
+                        step = s.snapshot_end(slot, pv, ps, pT, nullptr, nullptr);
+                    }
+                    return py::make_tuple(step, vel, speed, T);
+                }
                 py::array_t<double> u({g.Nx + 1, g.Ny, g.Nz}), v({g.Nx, g.Ny + 1, g.Nz}),
                     w({g.Nx, g.Ny, g.Nz + 1}), p({g.Nx, g.Ny, g.Nz}), T({g.Nx, g.Ny, g.Nz});
                 double *pu = u.mutable_data(), *pv = v.mutable_data(), *pw = w.mutable_data(),
                        *pp = p.mutable_data(), *pT = T.mutable_data();
-                int step;
                 {
                     py::gil_scoped_release nogil;
                     step = s.snapshot_end(slot, pu, pv, pw, pp, pT);

@@ -137,7 +137,7 @@ struct ffr_scene {
     // snapshot egress (ffr_snapshot.hip): device slot, pinned mirror, events
     struct Snap {
         bool busy = false;
-        int step = 0;
+        int step = 0, kind = 0;
         void *dev = nullptr, *host = nullptr;
         hipEvent_t copied = nullptr, landed = nullptr;
     };

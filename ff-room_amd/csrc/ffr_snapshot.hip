@@ -2,15 +2,18 @@
 // while later steps run (the captures of SolverBridge.run_animated,
 // solver_bridge.py:172-281, which copy the fields between steps).
 //
-// ffr_snapshot_begin enqueues, on the scene's stream, a device-to-device copy
-// of u, v, w, T (in the state's precision) and p into a slot of the scene;
-// steps enqueued afterwards may overwrite the live fields at once. A second
-// stream then copies the slot to pinned host memory, overlapping the next
-// steps' kernels. ffr_snapshot_end waits for that copy, writes FP64 fields in
-// the reference layout and frees the slot. Nothing here runs inside a graph.
+// ffr_snapshot_begin enqueues, on the scene's stream, a copy of the fields
+// into a slot of the scene: u, v, w, T (in the state's precision) and p
+// (FFR_SNAP_FIELDS), or the quantities of a FieldSnapshot computed from them
+// in FP64 (FFR_SNAP_CELLS: cell-centred velocity, speed, T); steps enqueued
+// afterwards may overwrite the live fields at once. A second stream then
+// copies the slot to pinned host memory, overlapping the next steps' kernels.
+// ffr_snapshot_end waits for that copy, writes FP64 arrays and frees the
+// slot. Nothing here runs inside a graph.
 #include "ffr_scene.hpp"
 
 #include <algorithm>
+#include <cstring>
 
 using namespace ffr;
 
@@ -35,6 +38,37 @@ void snapshot_release(ffr_scene *s)
 }  // namespace ffr
 
 namespace {
+// Cell-centred velocity, speed and T of one cell in FP64, with the
+// operations of the reference's capture (solver_bridge.py:224-236: 0.5 *
+// (a + b) per axis, sqrt(uc**2 + vc**2 + wc**2)) in that order and without
+// fused multiply-adds, so the values are numpy's bit for bit. vel is
+// (nx, ny, nz, 3) interleaved, as np.stack(..., axis=-1) lays it out.
+template <typename R>
+__global__ void __launch_bounds__(kBlock) k_snap_cells(Geo g, const R *u, const R *v, const R *w,
+                                                        const R *T, double *vel, double *speed,
+                                                        double *Tc)
+{
+#pragma clang fp contract(off)
+    const uint32_t nc = g.nc;
+    for (uint32_t c = blockIdx.x * kBlock + threadIdx.x; c < nc; c += gridDim.x * kBlock) {
+        int i, j, k;
+        const uint32_t ii = fdiv(c, g.f_nynz), rem = c - ii * (uint32_t)(g.ny * g.nz);
+        const uint32_t jj = fdiv(rem, g.f_nz);
+        i = ii;
+        j = jj;
+        k = rem - jj * g.nz;
+        const double uc = 0.5 * ((double)u[uidx(g, i, j, k)] + (double)u[uidx(g, i + 1, j, k)]);
+        const double vc = 0.5 * ((double)v[vidx(g, i, j, k)] + (double)v[vidx(g, i, j + 1, k)]);
+        const double wc = 0.5 * ((double)w[widx(g, i, j, k)] + (double)w[widx(g, i, j, k + 1)]);
+        vel[3ull * c] = uc;
+        vel[3ull * c + 1] = vc;
+        vel[3ull * c + 2] = wc;
+        const double uu = uc * uc, vv = vc * vc, ww = wc * wc;
+        speed[c] = sqrt((uu + vv) + ww);
+        Tc[c] = (double)T[c];
+    }
+}
+
 // field q (u, v, w, T, p): element count, bytes per element, offset in a slot
 struct SnapLayout {
     uint64_t cnt[5];
@@ -56,13 +90,15 @@ SnapLayout snap_layout(const ffr_scene *s)
     L.total = off;
     return L;
 }
+// cell kind: vel (3 nc), speed (nc), T (nc), FP64
+size_t cells_bytes(const ffr_scene *s) { return 5ull * 8ull * s->nc; }
 }  // namespace
 
 extern "C" {
 
-int ffr_snapshot_begin(ffr_scene *s, int *slot)
+int ffr_snapshot_begin(ffr_scene *s, int kind, int *slot)
 {
-    if (!s || !slot) return FFR_E_INVALID;
+    if (!s || !slot || (kind != FFR_SNAP_FIELDS && kind != FFR_SNAP_CELLS)) return FFR_E_INVALID;
     ffr_ctx *c = s->ctx;
     if (task_busy(s)) {
         c->fail("ffr_snapshot_begin: the scene has a step in flight");
@@ -80,39 +116,56 @@ int ffr_snapshot_begin(ffr_scene *s, int *slot)
     }
     HIP_CHECK_CTX(c, hipSetDevice(c->device), FFR_E_DEVICE);
     const SnapLayout L = snap_layout(s);
+    const size_t need = kind == FFR_SNAP_CELLS ? cells_bytes(s) : L.total;
     ffr_scene::Snap &q = s->snaps[k];
     if (!s->copy_stream)
         HIP_CHECK_CTX(c, hipStreamCreateWithFlags(&s->copy_stream, hipStreamNonBlocking), FFR_E_DEVICE);
-    if (!q.dev) {  // slots are allocated on first use and count against the budget
-        if (c->in_use + L.total > c->budget) {
+    if (!q.dev) {  // slots are allocated on first use (for either kind) and count against the budget
+        const size_t bytes = std::max(L.total, cells_bytes(s));
+        if (c->in_use + bytes > c->budget) {
             c->fail("ffr_snapshot_begin: a slot needs %zu bytes, the context budget has %zu free",
-                    L.total, c->budget - c->in_use);
+                    bytes, c->budget - c->in_use);
             return FFR_E_NOMEM;
         }
-        HIP_CHECK_CTX(c, hipMalloc(&q.dev, L.total), FFR_E_NOMEM);
-        c->in_use += L.total;
-        s->snap_bytes = L.total;
-        HIP_CHECK_CTX(c, hipHostMalloc(&q.host, L.total), FFR_E_NOMEM);
+        HIP_CHECK_CTX(c, hipMalloc(&q.dev, bytes), FFR_E_NOMEM);
+        c->in_use += bytes;
+        s->snap_bytes = bytes;
+        HIP_CHECK_CTX(c, hipHostMalloc(&q.host, bytes), FFR_E_NOMEM);
         HIP_CHECK_CTX(c, hipEventCreateWithFlags(&q.copied, hipEventDisableTiming), FFR_E_DEVICE);
         HIP_CHECK_CTX(c, hipEventCreateWithFlags(&q.landed, hipEventDisableTiming), FFR_E_DEVICE);
     }
-    const void *src[5] = {s->u, s->v, s->w, s->T, s->p};
-    for (int f = 0; f < 5; f++)
-        HIP_CHECK_CTX(c, hipMemcpyAsync((char *)q.dev + L.off[f], src[f], L.cnt[f] * L.esz[f],
-                                        hipMemcpyDeviceToDevice, s->stream), FFR_E_DEVICE);
+    if (kind == FFR_SNAP_CELLS) {
+        double *vel = (double *)q.dev, *speed = vel + 3 * s->nc, *Tc = speed + s->nc;
+        const dim3 grid(grid_for(s->nc)), block(kBlock);
+        if (s->fp64)
+            hipLaunchKernelGGL(k_snap_cells<double>, grid, block, 0, s->stream, s->geo,
+                               (const double *)s->u, (const double *)s->v, (const double *)s->w,
+                               (const double *)s->T, vel, speed, Tc);
+        else
+            hipLaunchKernelGGL(k_snap_cells<float>, grid, block, 0, s->stream, s->geo,
+                               (const float *)s->u, (const float *)s->v, (const float *)s->w,
+                               (const float *)s->T, vel, speed, Tc);
+        HIP_CHECK_CTX(c, hipGetLastError(), FFR_E_DEVICE);
+    } else {
+        const void *src[5] = {s->u, s->v, s->w, s->T, s->p};
+        for (int f = 0; f < 5; f++)
+            HIP_CHECK_CTX(c, hipMemcpyAsync((char *)q.dev + L.off[f], src[f], L.cnt[f] * L.esz[f],
+                                            hipMemcpyDeviceToDevice, s->stream), FFR_E_DEVICE);
+    }
     HIP_CHECK_CTX(c, hipEventRecord(q.copied, s->stream), FFR_E_DEVICE);
     HIP_CHECK_CTX(c, hipStreamWaitEvent(s->copy_stream, q.copied, 0), FFR_E_DEVICE);
-    HIP_CHECK_CTX(c, hipMemcpyAsync(q.host, q.dev, L.total, hipMemcpyDeviceToHost, s->copy_stream),
+    HIP_CHECK_CTX(c, hipMemcpyAsync(q.host, q.dev, need, hipMemcpyDeviceToHost, s->copy_stream),
                   FFR_E_DEVICE);
     HIP_CHECK_CTX(c, hipEventRecord(q.landed, s->copy_stream), FFR_E_DEVICE);
     q.busy = true;
+    q.kind = kind;
     q.step = s->step;
     *slot = k;
     return FFR_OK;
 }
 
-int ffr_snapshot_end(ffr_scene *s, int slot, int *step, double *u, double *v, double *w,
-                     double *p, double *T)
+int ffr_snapshot_end(ffr_scene *s, int slot, int *step, double *a, double *b, double *c3,
+                     double *d, double *e)
 {
     if (!s || slot < 0 || slot >= FFR_SNAPSHOT_SLOTS) return FFR_E_INVALID;
     ffr_ctx *c = s->ctx;
@@ -123,8 +176,18 @@ int ffr_snapshot_end(ffr_scene *s, int slot, int *step, double *u, double *v, do
     }
     HIP_CHECK_CTX(c, hipSetDevice(c->device), FFR_E_DEVICE);
     HIP_CHECK_CTX(c, hipEventSynchronize(q.landed), FFR_E_DEVICE);
+    if (q.kind == FFR_SNAP_CELLS) {  // a: vel (3 nc), b: speed, c3: T
+        const double *h = (const double *)q.host;
+        double *dst[3] = {a, b, c3};
+        const size_t off[3] = {0, 3 * s->nc, 4 * s->nc}, cnt[3] = {3 * s->nc, s->nc, s->nc};
+        for (int f = 0; f < 3; f++)
+            if (dst[f]) std::memcpy(dst[f], h + off[f], cnt[f] * sizeof(double));
+        if (step) *step = q.step;
+        q.busy = false;
+        return FFR_OK;
+    }
     const SnapLayout L = snap_layout(s);
-    double *dst[5] = {u, v, w, T, p};
+    double *dst[5] = {a, b, c3, e, d};  // u, v, w, T, p
     for (int f = 0; f < 5; f++) {
         if (!dst[f]) continue;
         const char *h = (const char *)q.host + L.off[f];

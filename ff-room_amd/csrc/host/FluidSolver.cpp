@@ -92,18 +92,19 @@ void FluidSolver::sync_to_host()
 
 void FluidSolver::mark_host_modified() { host_dirty_ = true; }
 
-int FluidSolver::snapshot_begin()
+int FluidSolver::snapshot_begin(int kind)
 {
     push_if_dirty();
     int slot = -1;
-    ffr_check(ffr_snapshot_begin(dev_->get(), &slot), "ffr_snapshot_begin");
+    ffr_check(ffr_snapshot_begin(dev_->get(), kind, &slot), "ffr_snapshot_begin");
+    snap_kind_[slot & 3] = kind;
     return slot;
 }
 
-int FluidSolver::snapshot_end(int slot, double *u, double *v, double *w, double *p, double *T)
+int FluidSolver::snapshot_end(int slot, double *a, double *b, double *c, double *d, double *e)
 {
     int step = 0;
-    ffr_check(ffr_snapshot_end(dev_->get(), slot, &step, u, v, w, p, T), "ffr_snapshot_end");
+    ffr_check(ffr_snapshot_end(dev_->get(), slot, &step, a, b, c, d, e), "ffr_snapshot_end");
     return step;
 }
 

@@ -82,9 +82,11 @@ public:
     void sync_to_host();       // download u, v, w, p, T into the Grid if stale
     void mark_host_modified(); // upload the Grid's fields before the next step
     // snapshot egress (ffr_snapshot_begin/end): queue a copy of the current
-    // fields without waiting for it; collect it later (returns its step)
-    int snapshot_begin();
-    int snapshot_end(int slot, double *u, double *v, double *w, double *p, double *T);
+    // state (FFR_SNAP_FIELDS or FFR_SNAP_CELLS) without waiting for it;
+    // collect it later into the kind's arrays (returns its step)
+    int snapshot_begin(int kind);
+    int snapshot_kind(int slot) const { return snap_kind_[slot & 3]; }
+    int snapshot_end(int slot, double *a, double *b, double *c, double *d, double *e);
     const Grid &grid() const { return grid_; }
 
 private:
@@ -96,6 +98,7 @@ private:
     bool host_stale_ = false;  // device ahead of the Grid vectors
     bool host_dirty_ = false;  // Grid vectors ahead of the device
     uint64_t geom_seen_ = 0;
+    int snap_kind_[4] = {0, 0, 0, 0};
 
     void push_if_dirty();
     void rebuild_device();

@@ -19,6 +19,7 @@ LIB_PATH = os.path.join(_HERE, "libffroom.so")
 
 FFR_OK = 0
 FLUID, SOLID, INFLOW, OUTFLOW = 0, 1, 2, 3
+SNAP_FIELDS, SNAP_CELLS = 0, 1
 UPWIND1, QUICK, LAX_WENDROFF = 0, 1, 2
 PREC_FP64, PREC_MIXED = 0, 1
 POISSON_AUTO, POISSON_CG, POISSON_MGPCG = 0, 1, 2
@@ -127,7 +128,7 @@ SIGNATURES = {
     "ffr_scene_upload": (_I, [_P, _dp, _dp, _dp, _dp, _dp]),
     "ffr_scene_download": (_I, [_P, _dp, _dp, _dp, _dp, _dp]),
     "ffr_scene_cell_type": (_I, [_P, C.POINTER(C.c_uint8)]),
-    "ffr_snapshot_begin": (_I, [_P, C.POINTER(C.c_int)]),
+    "ffr_snapshot_begin": (_I, [_P, C.c_int, C.POINTER(C.c_int)]),
     "ffr_snapshot_end": (_I, [_P, C.c_int, C.POINTER(C.c_int), _dp, _dp, _dp, _dp, _dp]),
     "ffr_scene_step_count": (_I, [_P]),
     "ffr_scene_set_step_count": (_I, [_P, C.c_int]),
@@ -374,19 +375,31 @@ class DeviceScene:
             self.h, out.ctypes.data_as(C.POINTER(C.c_uint8))), "ffr_scene_cell_type")
         return out
 
-    def snapshot_begin(self) -> int:
-        """Queues a copy of the current fields (ffr_snapshot_begin); returns its slot."""
+    def snapshot_begin(self, cells: bool = False) -> int:
+        """Queues a copy of the current fields, or with ``cells`` of the
+        FieldSnapshot arrays computed on the device (ffr_snapshot_begin);
+        returns its slot."""
         slot = C.c_int(-1)
-        self.ctx.check(self.ctx._l.ffr_snapshot_begin(self.h, C.byref(slot)), "ffr_snapshot_begin")
+        kind = SNAP_CELLS if cells else SNAP_FIELDS
+        self.ctx.check(self.ctx._l.ffr_snapshot_begin(self.h, kind, C.byref(slot)),
+                       "ffr_snapshot_begin")
+        self._snap_kind = getattr(self, "_snap_kind", {})
+        self._snap_kind[slot.value] = kind
         return slot.value
 
     def snapshot_end(self, slot: int):
-        """(step at capture, fields) of a queued snapshot (ffr_snapshot_end)."""
-        out = {k: np.empty(n) for k, n in self.sizes.items()}
+        """(step at capture, arrays) of a queued snapshot (ffr_snapshot_end):
+        u, v, w, p, T for fields; vel (n, 3), speed, T for cells (flat)."""
         step = C.c_int(0)
-        self.ctx.check(self.ctx._l.ffr_snapshot_end(
-            self.h, slot, C.byref(step), *[_ptr(out[k]) for k in ("u", "v", "w", "p", "T")]),
-            "ffr_snapshot_end")
+        if getattr(self, "_snap_kind", {}).get(slot) == SNAP_CELLS:
+            n = self.nx * self.ny * self.nz
+            out = {"vel": np.empty((n, 3)), "speed": np.empty(n), "T": np.empty(n)}
+            ptrs = [_ptr(out["vel"]), _ptr(out["speed"]), _ptr(out["T"]), None, None]
+        else:
+            out = {k: np.empty(n) for k, n in self.sizes.items()}
+            ptrs = [_ptr(out[k]) for k in ("u", "v", "w", "p", "T")]
+        self.ctx.check(self.ctx._l.ffr_snapshot_end(self.h, slot, C.byref(step), *ptrs),
+                       "ffr_snapshot_end")
         return step.value, out
 
     def step(self) -> dict:

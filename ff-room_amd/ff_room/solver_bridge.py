@@ -157,10 +157,11 @@ class SolverBridge:
         """Like run(), also capturing a FieldSnapshot every ``save_every``
         steps and at the end. Returns (SimResult, list[FieldSnapshot]).
 
-        A capture is queued on the device (FluidSolver.snapshot_begin) and the
-        run goes on at once; its copy to the host overlaps the next steps and
-        it is turned into a FieldSnapshot later, keeping at most
-        ``_IN_FLIGHT`` captures pending."""
+        A capture is queued on the device (FluidSolver.snapshot_begin): the
+        device computes its cell-centred velocity and speed in FP64 as the
+        reference does on the host (solver_bridge.py:224-236), the copy to the
+        host overlaps the next steps, and the arrays are collected later,
+        keeping at most ``_IN_FLIGHT`` captures pending."""
         scene, params, solver = self._solver()
         g = scene.grid
         dt = self.config.solver.dt
@@ -176,14 +177,16 @@ class SolverBridge:
 
         def collect():
             slot, T_mean = pending.pop(0)
-            step, u, v, w, _p, T = solver.snapshot_end(slot)
-            snaps.append(make(step, T_mean, u, v, w, T))
+            step, vel, speed, T = solver.snapshot_end(slot)
+            snaps.append(FieldSnapshot(step=step, time_s=step * dt, T_mean=T_mean, speed=speed,
+                                       T_field=T if params.thermal else np.zeros(speed.shape),
+                                       vel=vel))
 
         def _cb(res):
             if res.step % save_every == 0:
                 if len(pending) >= self._IN_FLIGHT:
                     collect()
-                pending.append((solver.snapshot_begin(), res.T_mean))
+                pending.append((solver.snapshot_begin(cells=True), res.T_mean))
             if progress_callback:
                 progress_callback(res.step, res.velocity_change, res.divergence_max, res.T_mean)
             if print_interval > 0 and res.step % print_interval == 0:

@@ -201,18 +201,25 @@ int ffr_scene_set_params(ffr_scene *scene, const ffr_params *params);
 int ffr_scene_cell_type(const ffr_scene *scene, uint8_t *cell_type);
 /* Snapshot egress (the captures of SolverBridge.run_animated,
  * solver_bridge.py:172-281, and the fields of SimResult): ffr_snapshot_begin
- * queues a copy of the scene's fields as they are after its last step and
- * returns a slot without waiting for them to reach the host; the next steps
- * may be issued at once, and the copy to the host overlaps them. At most
- * FFR_SNAPSHOT_SLOTS snapshots of a scene are pending (FFR_E_STATE beyond);
- * each slot holds device memory counted against the context budget until the
- * scene is destroyed. ffr_snapshot_end waits for slot `slot`, writes its
- * fields in the reference layout as FP64 (NULL skips a field), the step count
- * at capture into *step, and frees the slot. */
+ * queues a copy of the scene's state as it is after its last step and
+ * returns a slot without waiting for it to reach the host; the next steps
+ * may be issued at once, and the copy to the host overlaps them.
+ *   FFR_SNAP_FIELDS: u, v, w, p, T in the reference layout;
+ *   FFR_SNAP_CELLS:  a FieldSnapshot's arrays, computed on the device in FP64
+ *                    as the reference computes them on the host
+ *                    (solver_bridge.py:224-236): cell-centred velocity
+ *                    (nx, ny, nz, 3 interleaved), speed, T.
+ * At most FFR_SNAPSHOT_SLOTS snapshots of a scene are pending (FFR_E_STATE
+ * beyond); each slot holds device and pinned memory, the device part counted
+ * against the context budget, until the scene is destroyed. ffr_snapshot_end
+ * waits for slot `slot`, writes its arrays as FP64 - (u, v, w, p, T) for
+ * fields, (vel, speed, T, -, -) for cells; NULL skips one - and the step
+ * count at capture into *step, and frees the slot. */
 #define FFR_SNAPSHOT_SLOTS 4
-int ffr_snapshot_begin(ffr_scene *scene, int *slot);
-int ffr_snapshot_end(ffr_scene *scene, int slot, int *step, double *u, double *v, double *w,
-                     double *p, double *T);
+enum { FFR_SNAP_FIELDS = 0, FFR_SNAP_CELLS = 1 };
+int ffr_snapshot_begin(ffr_scene *scene, int kind, int *slot);
+int ffr_snapshot_end(ffr_scene *scene, int slot, int *step, double *a, double *b, double *c,
+                     double *d, double *e);
 /* Host <-> device state copies; a NULL pointer leaves that field alone. */
 int ffr_scene_upload(ffr_scene *scene, const double *u, const double *v, const double *w,
                      const double *p, const double *T);

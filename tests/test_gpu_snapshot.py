@@ -39,6 +39,29 @@ def test_snapshot_holds_the_fields_at_capture(ctx, precision):
     d.close()
 
 
+def test_cell_snapshot_matches_numpy(ctx):
+    """FFR_SNAP_CELLS against numpy on the fields of a FFR_SNAP_FIELDS capture
+    of the same state (mixed form: FP32 state, FP64 arithmetic)."""
+    cfg = random_scene(203, grid=(30, 22, 14), n_fans=2, n_obstacles=2, precision="mixed",
+                       dt=0.005, max_steps=10, convergence_tol=0.0, buoyancy=True)
+    d = device.create(ctx, device.prepare(cfg))
+    for _ in range(3):
+        d.step()
+    sc, sf = d.snapshot_begin(cells=True), d.snapshot_begin()
+    d.step()
+    _, c = d.snapshot_end(sc)
+    _, f = d.snapshot_end(sf)
+    nx, ny, nz = d.nx, d.ny, d.nz
+    u = f["u"].reshape(nx + 1, ny, nz)
+    v = f["v"].reshape(nx, ny + 1, nz)
+    w = f["w"].reshape(nx, ny, nz + 1)
+    uc, vc, wc = 0.5 * (u[:-1] + u[1:]), 0.5 * (v[:, :-1] + v[:, 1:]), 0.5 * (w[:, :, :-1] + w[:, :, 1:])
+    assert np.array_equal(c["vel"].reshape(nx, ny, nz, 3), np.stack([uc, vc, wc], axis=-1))
+    assert np.array_equal(c["speed"].reshape(nx, ny, nz), np.sqrt(uc ** 2 + vc ** 2 + wc ** 2))
+    assert np.array_equal(c["T"], f["T"])
+    d.close()
+
+
 def test_snapshot_slots_run_out(ctx):
     cfg = random_scene(202, grid=(20, 16, 12), n_fans=1, n_obstacles=1, precision="mixed",
                        dt=0.005, max_steps=10, convergence_tol=0.0)
