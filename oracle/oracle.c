@@ -844,3 +844,228 @@ void orc_step(orc_state *s, const orc_params *p, int *step_counter, orc_result *
     out->T_mean = tmean;
     out->poisson_iters = iters;
 }
+
+
+/* ---- D3Q19 lattice Boltzmann, LBMSolver.cpp -----------------------------
+ * A restatement, not a transcription: streaming is written as a pull. The
+ * reference pushes into a zeroed buffer (:151-190): a population moves to
+ * the neighbour in its direction, or, when that neighbour is outside the
+ * domain or SOLID, is added to the same cell in the direction OPP[q]. Its
+ * OPP table (:28) reverses the axis directions but, for the twelve diagonal
+ * ones, only one component (7 (+1,+1,0) <-> 8 (-1,+1,0), ...), so a target
+ * population of a cell may receive the streamed value from upstream, the
+ * bounced value from the cell itself, both (summed) or neither (zero); the
+ * pull below sums exactly those. A SOLID cell receives nothing. */
+static const int LBM_E[19][3] = { /* directions, LBMSolver.cpp:14-16 */
+    {0, 0, 0}, {1, 0, 0}, {-1, 0, 0}, {0, 1, 0}, {0, -1, 0}, {0, 0, 1}, {0, 0, -1},
+    {1, 1, 0}, {-1, 1, 0}, {1, -1, 0}, {-1, -1, 0}, {1, 0, 1}, {-1, 0, 1},
+    {1, 0, -1}, {-1, 0, -1}, {0, 1, 1}, {0, -1, 1}, {0, 1, -1}, {0, -1, -1}};
+static const int LBM_OPP[19] = {0, 2, 1, 4, 3, 6, 5, 8, 7, 10, 9, 12, 11, 14, 13, 16, 15, 18, 17};
+
+static double lbm_w(int q) { return q == 0 ? 1.0 / 3.0 : (q < 7 ? 1.0 / 18.0 : 1.0 / 36.0); }
+
+/* equilibrium, LBMSolver.hpp:85-90 (cs2 = 1/3 as a double) */
+static double lbm_feq(int q, double rho, double ux, double uy, double uz)
+{
+    const double cs2 = 1.0 / 3.0;
+    const double eu = LBM_E[q][0] * ux + LBM_E[q][1] * uy + LBM_E[q][2] * uz;
+    const double usq = ux * ux + uy * uy + uz * uz;
+    return lbm_w(q) * rho * (1.0 + eu / cs2 + eu * eu / (2.0 * cs2 * cs2) - usq / (2.0 * cs2));
+}
+
+/* density and velocity of one cell's populations (:114-125, :306-317) */
+static void lbm_moments(const double *f, long nc, long c, double *rho, double *ux, double *uy,
+                        double *uz)
+{
+    double r = 0.0, x = 0.0, y = 0.0, z = 0.0;
+    for (int q = 0; q < 19; q++) {
+        const double fq = f[q * nc + c];
+        r += fq;
+        x += fq * LBM_E[q][0];
+        y += fq * LBM_E[q][1];
+        z += fq * LBM_E[q][2];
+    }
+    if (r > 1e-14) {
+        x /= r;
+        y /= r;
+        z /= r;
+    }
+    *rho = r; *ux = x; *uy = y; *uz = z;
+}
+
+static int lbm_in(const orc_state *s, int i, int j, int k)
+{
+    return i >= 0 && i < s->nx && j >= 0 && j < s->ny && k >= 0 && k < s->nz;
+}
+
+/* inflow cells at equilibrium with their fan's velocity, fans in order (:81-100) */
+static void lbm_inflow(const orc_state *s, const orc_lbm *L)
+{
+    const long nc = n_c(s);
+    const double s2l = L->dt / L->dx_ref;
+    for (int n = 0; n < s->n_fans; n++) {
+        const orc_fan *fn = &s->fans[n];
+        const double vx = fn->vel[0] * s2l, vy = fn->vel[1] * s2l, vz = fn->vel[2] * s2l;
+        for (int i = fn->i_min; i <= fn->i_max; i++)
+            for (int j = fn->j_min; j <= fn->j_max; j++)
+                for (int k = fn->k_min; k <= fn->k_max; k++) {
+                    if (!lbm_in(s, i, j, k)) continue;
+                    const long c = IC(s, i, j, k);
+                    if (s->flags[c] != ORC_INFLOW) continue;
+                    for (int q = 0; q < 19; q++) L->f[q * nc + c] = lbm_feq(q, 1.0, vx, vy, vz);
+                }
+    }
+}
+
+/* BGK on FLUID and OUTFLOW cells (:106-144) */
+static void lbm_collide(const orc_state *s, orc_lbm *L)
+{
+    const long nc = n_c(s);
+    const double inv_tau = 1.0 / L->tau;
+    for (long c = 0; c < nc; c++) {
+        if (s->flags[c] == ORC_SOLID || s->flags[c] == ORC_INFLOW) continue;
+        double rho, ux, uy, uz;
+        lbm_moments(L->f, nc, c, &rho, &ux, &uy, &uz);
+        for (int q = 0; q < 19; q++) {
+            double *fq = &L->f[q * nc + c];
+            *fq += inv_tau * (lbm_feq(q, rho, ux, uy, uz) - *fq);
+        }
+    }
+}
+
+/* streaming as a pull into f2, then the buffers swap (:151-190) */
+static void lbm_stream(const orc_state *s, orc_lbm *L)
+{
+    const long nc = n_c(s);
+    for (int i = 0; i < s->nx; i++)
+        for (int j = 0; j < s->ny; j++)
+            for (int k = 0; k < s->nz; k++) {
+                const long c = IC(s, i, j, k);
+                for (int q = 0; q < 19; q++) {
+                    double v = 0.0;
+                    if (s->flags[c] != ORC_SOLID) {
+                        /* streamed in from upstream */
+                        const int ui = i - LBM_E[q][0], uj = j - LBM_E[q][1], uk = k - LBM_E[q][2];
+                        if (lbm_in(s, ui, uj, uk) && s->flags[IC(s, ui, uj, uk)] != ORC_SOLID)
+                            v += L->f[q * nc + IC(s, ui, uj, uk)];
+                        /* this cell's population OPP[q], bounced at a wall or SOLID */
+                        const int b = LBM_OPP[q];
+                        const int bi = i + LBM_E[b][0], bj = j + LBM_E[b][1], bk = k + LBM_E[b][2];
+                        if (!lbm_in(s, bi, bj, bk) || s->flags[IC(s, bi, bj, bk)] == ORC_SOLID)
+                            v += L->f[b * nc + c];
+                    }
+                    L->f2[q * nc + c] = v;
+                }
+            }
+    double *t = L->f;
+    L->f = L->f2;
+    L->f2 = t;
+}
+
+/* zero-gradient copy of one patch's OUTFLOW cells from one cell inside (:196-282) */
+static void lbm_outflow_patch(const orc_state *s, orc_lbm *L, int axis, int side, int a0, int a1,
+                              int b0, int b1)
+{
+    const long nc = n_c(s);
+    const int n_ax = axis == 0 ? s->nx : (axis == 1 ? s->ny : s->nz);
+    const int wall = side ? n_ax - 1 : 0, src = side ? n_ax - 2 : 1;
+    if (src < 0 || src >= n_ax) return;
+    for (int a = a0; a <= a1; a++)
+        for (int b = b0; b <= b1; b++) {
+            int cw[3], cs[3];
+            if (axis == 0) { cw[0] = wall; cw[1] = a; cw[2] = b; }
+            else if (axis == 1) { cw[0] = a; cw[1] = wall; cw[2] = b; }
+            else { cw[0] = a; cw[1] = b; cw[2] = wall; }
+            if (!lbm_in(s, cw[0], cw[1], cw[2])) continue;
+            cs[0] = cw[0]; cs[1] = cw[1]; cs[2] = cw[2];
+            cs[axis] = src;
+            const long co = IC(s, cw[0], cw[1], cw[2]);
+            if (s->flags[co] != ORC_OUTFLOW) continue;
+            const long ci = IC(s, cs[0], cs[1], cs[2]);
+            for (int q = 0; q < 19; q++) L->f[q * nc + co] = L->f[q * nc + ci];
+        }
+}
+
+/* cell velocities, p = rho/3 on non-SOLID cells, MAC faces (:288-376) */
+static void lbm_macroscopic(orc_state *s, orc_lbm *L)
+{
+    const long nc = n_c(s);
+    const double l2s = L->dx_ref / L->dt;
+    for (long c = 0; c < nc; c++) {
+        if (s->flags[c] == ORC_SOLID) {
+            L->ux[c] = L->uy[c] = L->uz[c] = 0.0;
+            continue;
+        }
+        double rho, ux, uy, uz;
+        lbm_moments(L->f, nc, c, &rho, &ux, &uy, &uz);
+        L->ux[c] = ux * l2s;
+        L->uy[c] = uy * l2s;
+        L->uz[c] = uz * l2s;
+        s->p[c] = rho * (1.0 / 3.0);
+    }
+    const int nx = s->nx, ny = s->ny, nz = s->nz;
+    for (int i = 0; i <= nx; i++)
+        for (int j = 0; j < ny; j++)
+            for (int k = 0; k < nz; k++) {
+                const double lo = L->ux[IC(s, i > 0 ? i - 1 : 0, j, k)];
+                const double hi = L->ux[IC(s, i < nx ? i : nx - 1, j, k)];
+                s->u[IU(s, i, j, k)] = (i == 0) ? hi : (i == nx ? lo : 0.5 * (lo + hi));
+            }
+    for (int i = 0; i < nx; i++)
+        for (int j = 0; j <= ny; j++)
+            for (int k = 0; k < nz; k++) {
+                const double lo = L->uy[IC(s, i, j > 0 ? j - 1 : 0, k)];
+                const double hi = L->uy[IC(s, i, j < ny ? j : ny - 1, k)];
+                s->v[IV(s, i, j, k)] = (j == 0) ? hi : (j == ny ? lo : 0.5 * (lo + hi));
+            }
+    for (int i = 0; i < nx; i++)
+        for (int j = 0; j < ny; j++)
+            for (int k = 0; k <= nz; k++) {
+                const double lo = L->uz[IC(s, i, j, k > 0 ? k - 1 : 0)];
+                const double hi = L->uz[IC(s, i, j, k < nz ? k : nz - 1)];
+                s->w[IW(s, i, j, k)] = (k == 0) ? hi : (k == nz ? lo : 0.5 * (lo + hi));
+            }
+}
+
+void orc_lbm_init(orc_state *s, orc_lbm *L)
+{ /* constructor and initialize_f (:39-75): rest equilibrium, then inflow */
+    const long nc = n_c(s);
+    for (int q = 0; q < 19; q++)
+        for (long c = 0; c < nc; c++) L->f[q * nc + c] = lbm_w(q);
+    lbm_inflow(s, L);
+}
+
+void orc_lbm_step(orc_state *s, orc_lbm *L, int *step_counter, orc_result *out)
+{ /* LBMSolver::step (:403-444) */
+    const long nu = n_u(s), nv = n_v(s), nw = n_w(s);
+    double *prev = (double *)malloc(sizeof(double) * (size_t)(nu + nv + nw));
+    memcpy(prev, s->u, sizeof(double) * nu);
+    memcpy(prev + nu, s->v, sizeof(double) * nv);
+    memcpy(prev + nu + nv, s->w, sizeof(double) * nw);
+    lbm_inflow(s, L);
+    lbm_collide(s, L);
+    lbm_stream(s, L);
+    for (int n = 0; n < s->n_outflows; n++) {
+        const orc_outflow *o = &s->outflows[n];
+        lbm_outflow_patch(s, L, o->axis, o->side, o->a_min, o->a_max, o->b_min, o->b_max);
+    }
+    for (int n = 0; n < s->n_openings; n++) {
+        const orc_opening *o = &s->openings[n];
+        lbm_outflow_patch(s, L, o->axis, o->side, o->a_min, o->a_max, o->b_min, o->b_max);
+    }
+    lbm_macroscopic(s, L);
+    double dv = 0.0;
+    for (long e = 0; e < nu; e++) { const double d = fabs(s->u[e] - prev[e]); if (d > dv) dv = d; }
+    for (long e = 0; e < nv; e++) { const double d = fabs(s->v[e] - prev[nu + e]); if (d > dv) dv = d; }
+    for (long e = 0; e < nw; e++) { const double d = fabs(s->w[e] - prev[nu + nv + e]); if (d > dv) dv = d; }
+    free(prev);
+    dv /= L->dt;
+    (*step_counter)++;
+    memset(out, 0, sizeof *out);
+    out->step = *step_counter;
+    out->velocity_change = dv;
+    out->pressure_residual = 0.0;
+    out->divergence_max = orc_divergence_max(s);
+    out->converged = dv < L->convergence_tol;
+    out->T_mean = 0.0;
+}

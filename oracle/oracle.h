@@ -98,6 +98,20 @@ double orc_poisson_cg(orc_state *s, const double *rhs, int max_iter, double tol,
 /* MultigridPoissonSolver::solve, PoissonSolver.cpp:446-505; writes s->p. */
 double orc_poisson_mg(orc_state *s, const double *rhs, int max_iter, double tol, int *iters);
 
+/* ---- D3Q19 lattice Boltzmann (LBMSolver.cpp/.hpp) ------------------------
+ * BGK collision with one relaxation time, inflow cells held at equilibrium
+ * with their fan's velocity, streaming with bounce-back at the domain walls
+ * and SOLID cells, zero-gradient copies on outflow and opening patches, and
+ * macroscopic fields written to the MAC grid. Populations f[q*nc + cell]. */
+typedef struct {
+    double tau, dt, dx_ref;     /* relaxation time, physical step, Grid::dx = Lx/Nx */
+    double convergence_tol;
+    double *f, *f2;             /* 19*nc each (caller-owned) */
+    double *ux, *uy, *uz;       /* nc each: cell velocities [m/s] (caller-owned) */
+} orc_lbm;
+void orc_lbm_init(orc_state *s, orc_lbm *L);                       /* :39-75 */
+void orc_lbm_step(orc_state *s, orc_lbm *L, int *step_counter, orc_result *out); /* :403-444 */
+
 #ifdef __cplusplus
 }
 #endif

@@ -407,3 +407,86 @@ def ref_flags_from_calls(nx, ny, nz, lx, ly, lz, solid_boxes=(), fans=(), outflo
         return out
     finally:
         lib.ref_destroy(h)
+
+
+# ---- D3Q19 lattice Boltzmann (LBMSolver) ------------------------------------
+class _OrcLbm(C.Structure):
+    _fields_ = [("tau", C.c_double), ("dt", C.c_double), ("dx_ref", C.c_double),
+                ("convergence_tol", C.c_double), ("f", _dp), ("f2", _dp), ("ux", _dp),
+                ("uy", _dp), ("uz", _dp)]
+
+
+_lbm_sigs_done = False
+
+
+def _lbm_sigs(lib):
+    global _lbm_sigs_done
+    if not _lbm_sigs_done:
+        lib.orc_lbm_init.argtypes = [C.POINTER(_OrcState), C.POINTER(_OrcLbm)]
+        lib.orc_lbm_step.argtypes = [C.POINTER(_OrcState), C.POINTER(_OrcLbm),
+                                     C.POINTER(C.c_int), C.POINTER(_OrcResult)]
+        _lbm_sigs_done = True
+
+
+class OracleLBM:
+    """LBMSolver semantics over the C restatement (oracle.c); the boundary
+    lists as the scene registers them (openings, then plain outflows)."""
+
+    def __init__(self, scene: SceneArrays, tau: float, dt: float, convergence_tol: float = 1e-6):
+        self.base = OracleSolver(scene, {"dt": dt}, init=False)
+        self.lib = self.base.lib
+        _lbm_sigs(self.lib)
+        nc = scene.nx * scene.ny * scene.nz
+        self.f, self.f2 = np.zeros(19 * nc), np.zeros(19 * nc)
+        self.ux, self.uy, self.uz = np.zeros(nc), np.zeros(nc), np.zeros(nc)
+        self.L = _OrcLbm(tau, dt, scene.lx / scene.nx, convergence_tol, _ptr(self.f),
+                         _ptr(self.f2), _ptr(self.ux), _ptr(self.uy), _ptr(self.uz))
+        self._st = self.base.state()
+        self.lib.orc_lbm_init(C.byref(self._st), C.byref(self.L))
+
+    def step(self) -> dict:
+        r = _OrcResult()
+        self.lib.orc_lbm_step(C.byref(self._st), C.byref(self.L), C.byref(self.base.step_count),
+                              C.byref(r))
+        return {k: getattr(r, k) for k, _ in _OrcResult._fields_ if k != "poisson_iters"}
+
+    def fields(self) -> dict:
+        return self.base.fields()
+
+
+class RefLBM:
+    """The reference's own LBMSolver (compiled from its sources)."""
+
+    def __init__(self, scene: SceneArrays, tau: float, dt: float, convergence_tol: float = 1e-6,
+                 max_steps: int = 1000):
+        self.lib = ref_lib()
+        self.lib.ref_make_lbm.argtypes = [C.c_void_p, C.c_double, C.c_int, C.c_double, C.c_double]
+        self.lib.ref_lbm_step.argtypes = [C.c_void_p, C.POINTER(_RefResult)]
+        self.scene = scene
+        self.h = self.lib.ref_create(scene.nx, scene.ny, scene.nz, scene.lx, scene.ly, scene.lz)
+        for f in scene.fans:
+            self.lib.ref_add_fan(self.h, *[int(x) for x in f[:6]], *[float(x) for x in f[6:9]])
+        for o in scene.openings:
+            self.lib.ref_add_opening(self.h, *[int(x) for x in o[:6]], float(o[6]))
+        for o in scene.outflows:
+            self.lib.ref_add_outflow(self.h, *[int(x) for x in o])
+        f = np.ascontiguousarray(scene.flags, dtype=np.uint8)
+        self.lib.ref_set_flags(self.h, _ptr(f, _u8p))
+        self.lib.ref_make_lbm(self.h, tau, max_steps, convergence_tol, dt)
+
+    def __del__(self):
+        try:
+            self.lib.ref_destroy(self.h)
+        except Exception:
+            pass
+
+    def step(self) -> dict:
+        r = _RefResult()
+        self.lib.ref_lbm_step(self.h, C.byref(r))
+        return {k: getattr(r, k) for k, _ in _RefResult._fields_}
+
+    def fields(self) -> dict:
+        sh = self.scene.shapes
+        out = {k: np.zeros(n) for k, n in sh.items()}
+        self.lib.ref_get_fields(self.h, *[_ptr(out[k]) for k in ("u", "v", "w", "p", "T")])
+        return out

@@ -2,7 +2,7 @@
 //
 // TEST INFRASTRUCTURE ONLY. oracle/Makefile compiles this file together
 // with the reference's Grid.cpp, BoundaryManager.cpp, PoissonSolver.cpp and
-// FluidSolver.cpp, read in place from the reference tree, into
+// FluidSolver.cpp (and LBMSolver.cpp), read in place from the reference tree, into
 // oracle/_ref/libffref.so. Tests use it to pin the C restatement (oracle.c)
 // and to write golden fixtures; the benchmark may time it as the CPU
 // baseline. Nothing in the product links it.
@@ -10,6 +10,7 @@
 #include "core/BoundaryManager.hpp"
 #include "core/PoissonSolver.hpp"
 #include "core/FluidSolver.hpp"
+#include "core/LBMSolver.hpp"
 
 #include <cstring>
 #include <memory>
@@ -22,6 +23,7 @@ struct RefHandle {
     std::unique_ptr<Grid> grid;
     BoundaryManager bm;
     std::unique_ptr<FluidSolver> solver;
+    std::unique_ptr<LBMSolver> lbm;
 };
 
 void copy_out(const std::vector<double> &src, double *dst)
@@ -192,6 +194,25 @@ double ref_poisson_solve(void *hp, const double *rhs, int use_mg, int max_iter, 
         if (iters) *iters = s.last_iterations();
     }
     return res;
+}
+
+// LBMSolver (LBMSolver.hpp:24-39) on the handle's grid and boundary lists.
+void ref_make_lbm(void *hp, double tau, int max_steps, double convergence_tol, double dt)
+{
+    auto *h = static_cast<RefHandle *>(hp);
+    h->lbm.reset(new LBMSolver(*h->grid, h->bm, tau, max_steps, convergence_tol, dt));
+}
+
+void ref_lbm_step(void *hp, ref_result *out)
+{
+    auto *h = static_cast<RefHandle *>(hp);
+    StepResult r = h->lbm->step();
+    out->step = r.step;
+    out->velocity_change = r.velocity_change;
+    out->pressure_residual = r.pressure_residual;
+    out->divergence_max = r.divergence_max;
+    out->converged = r.converged ? 1 : 0;
+    out->T_mean = r.T_mean;
 }
 
 void ref_apply_noslip(void *hp) { auto *h = static_cast<RefHandle *>(hp); h->bm.apply_noslip(*h->grid); }

@@ -10,6 +10,10 @@ Fixtures (numpy .npz, FP64):
   small_lw.npz         12x10x8 Lax-Wendroff, 10 steps
   poisson_tiny.npz     6x5x4 mixed-flag grid: rhs, CG p, multigrid p
   couette.npz          the reference's Couette test case run to convergence
+  lbm_small.npz        LBMSolver (D3Q19) on small_thermal's room and on
+                       basic_room: fields after 25 steps, every step's result
+
+    python tests/golden/make_golden.py lbm     # only lbm_small.npz
 """
 from __future__ import annotations
 
@@ -22,7 +26,7 @@ ROOT = os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__)
 sys.path.insert(0, ROOT)
 sys.path.insert(0, os.path.join(ROOT, "tests"))
 
-from oracle.pyoracle import RefSolver, SceneArrays, ref_available  # noqa: E402
+from oracle.pyoracle import RefLBM, RefSolver, SceneArrays, ref_available  # noqa: E402
 from oracle.scene_flags import build_scene  # noqa: E402
 from scenes import BASIC_ROOM, SMALL_LW, SMALL_THERMAL, solver_params, variant  # noqa: E402
 
@@ -87,14 +91,34 @@ def couette():
     np.savez_compressed(os.path.join(OUT, "couette.npz"), result=_res(r), **f)
 
 
+LBM_CASES = (("thermal", SMALL_THERMAL, 0.8, 0.01), ("basic", BASIC_ROOM, 0.6, 0.01))
+LBM_STEPS = 25
+
+
+def lbm_small():
+    out = {}
+    for tag, cfg, tau, dt in LBM_CASES:
+        sc = build_scene(cfg)
+        ref = RefLBM(sc, tau, dt, convergence_tol=0.0)
+        out[f"{tag}_results"] = np.array([_res(ref.step()) for _ in range(LBM_STEPS)])
+        for k, a in ref.fields().items():
+            if k != "T":
+                out[f"{tag}_{k}"] = a
+    np.savez_compressed(os.path.join(OUT, "lbm_small.npz"), **out)
+
+
 def main():
     if not ref_available():
         sys.exit("oracle/_ref/libffref.so is not built (make -C oracle ref)")
+    if sys.argv[1:] == ["lbm"]:
+        lbm_small()
+        return
     trajectory(BASIC_ROOM, (1, 10, 100), "c1_upwind_cg.npz")
     trajectory(SMALL_THERMAL, (20,), "small_thermal.npz")
     trajectory(SMALL_LW, (10,), "small_lw.npz")
     poisson_tiny()
     couette()
+    lbm_small()
     for f in sorted(os.listdir(OUT)):
         if f.endswith(".npz"):
             print(f, os.path.getsize(os.path.join(OUT, f)))
