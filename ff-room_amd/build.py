@@ -25,12 +25,13 @@ ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
 
 HIP_SOURCES = [
     "ffr_capi.hip", "ffr_scene.hip", "ffr_task.hip", "ffr_mg.hip", "ffr_serve.hip", "ffr_gang.hip",
-    "ffr_snapshot.hip",
+    "ffr_snapshot.hip", "ffr_lbm.hip",
     "kernels/step_kernels.hip", "kernels/cg_kernels.hip", "kernels/mg_kernels.hip",
+    "kernels/lbm_kernels.hip",
 ]
 HOST_SOURCES = [
     "host/Grid.cpp", "host/BoundaryManager.cpp", "host/Device.cpp",
-    "host/PoissonSolver.cpp", "host/FluidSolver.cpp", "bindings/module.cpp",
+    "host/PoissonSolver.cpp", "host/FluidSolver.cpp", "host/LBMSolver.cpp", "bindings/module.cpp",
 ]
 
 HIPCC_FLAGS = [

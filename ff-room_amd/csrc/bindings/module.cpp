@@ -15,6 +15,7 @@
 #include "../host/Device.hpp"
 #include "../host/FluidSolver.hpp"
 #include "../host/Grid.hpp"
+#include "../host/LBMSolver.hpp"
 #include "../host/PoissonSolver.hpp"
 
 namespace py = pybind11;
@@ -227,6 +228,19 @@ PYBIND11_MODULE(_ffroom_core, m)
             },
             py::arg("slot"));
 
+    // LBMSolver (module.cpp:202-209 of the reference): the D3Q19 lattice on the GPU
+    py::class_<LBMSolver>(m, "LBMSolver")
+        .def(py::init<Grid &, BoundaryManager &, double, int, double, double>(), py::arg("grid"),
+             py::arg("bm"), py::arg("tau"), py::arg("max_steps"), py::arg("convergence_tol"),
+             py::arg("dt") = 0.01)
+        .def("step", &LBMSolver::step, py::call_guard<py::gil_scoped_release>())
+        .def("run", &LBMSolver::run, py::arg("callback") = nullptr,
+             py::call_guard<py::gil_scoped_release>())
+        .def("compute_divergence_max", &LBMSolver::compute_divergence_max)
+        .def_property_readonly("current_step", &LBMSolver::current_step)
+        .def("compute_T_mean", &LBMSolver::compute_T_mean)
+        .def("sync_to_host", &LBMSolver::sync_to_host);
+
     // extensions: the pressure solvers on their own
     py::class_<PoissonSolver>(m, "PoissonSolver")
         .def(py::init<PoissonSolverParams>(), py::arg("params") = PoissonSolverParams{})
@@ -237,5 +251,5 @@ PYBIND11_MODULE(_ffroom_core, m)
         .def(py::init<PoissonSolverParams>(), py::arg("params") = PoissonSolverParams{});
 
     m.attr("ABI_VERSION") = ffr_abi_version();
-    m.attr("HAS_LBM") = false;
+    m.attr("HAS_LBM") = true;
 }

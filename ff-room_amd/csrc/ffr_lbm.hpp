@@ -1,0 +1,35 @@
+// ffr_lbm.hpp - D3Q19 lattice Boltzmann on a scene (LBMSolver.hpp/.cpp).
+#pragma once
+#include <hip/hip_runtime.h>
+#include "ffr_kernels.hpp"
+
+namespace ffr {
+
+// one outflow or opening patch: the wall (axis, side) and the inclusive
+// ranges of its two in-wall axes in order
+struct LbmPatch {
+    int axis, side, a0, a1, b0, b1;
+};
+
+// the lattice state a scene carries once ffr_lbm_attach has run
+struct LbmState {
+    bool on = false;
+    double tau = 0.0, dt = 0.0, tol = 0.0, dx_ref = 0.0;
+    double *f = nullptr, *f2 = nullptr;  // 19 * nc each
+    double *cu = nullptr;                // cell velocities ux | uy | uz, nc each
+    void *arena = nullptr;
+    size_t bytes = 0;
+    int steps = 0;
+};
+
+void lbm_launch_rest(hipStream_t s, uint32_t nc, double *f);
+void lbm_launch_relax(hipStream_t s, const Geo &g, double *f, double s2l, double inv_tau,
+                      bool collide, StepRec *rec);
+void lbm_launch_stream(hipStream_t s, const Geo &g, const double *f, double *f2);
+void lbm_launch_patch(hipStream_t s, const Geo &g, double *f, const LbmPatch &pt);
+void lbm_launch_macro(hipStream_t s, const Geo &g, const double *f, double *cu, double *p,
+                      double l2s);
+void lbm_launch_faces(hipStream_t s, const Geo &g, const double *cu, double *u, double *v,
+                      double *w, StepRec *rec);
+
+}  // namespace ffr

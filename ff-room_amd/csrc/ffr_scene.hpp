@@ -11,6 +11,7 @@
 
 #include "../../include/ffroom.h"
 #include "ffr_kernels.hpp"
+#include "ffr_lbm.hpp"
 #include "ffr_mg.hpp"
 
 struct ffr_ctx {
@@ -142,6 +143,7 @@ struct ffr_scene {
         hipEvent_t copied = nullptr, landed = nullptr;
     };
     Snap snaps[FFR_SNAPSHOT_SLOTS];
+    ffr::LbmState lbm;  // the D3Q19 lattice, once attached (ffr_lbm.hip)
     size_t snap_bytes = 0;  // bytes of one slot
     hipStream_t copy_stream = nullptr;
 };
@@ -179,6 +181,8 @@ int task_prepare_gang(ffr_scene *s, bool *eligible);
 void serve_forget(ffr_ctx *c, ffr_scene *s);
 // snapshot egress (ffr_snapshot.hip)
 void snapshot_release(ffr_scene *s);
+// the lattice of a scene (ffr_lbm.hip)
+void lbm_release(ffr_scene *s);
 
 struct Recorder;
 // Several scenes' step tasks advanced in rounds of merged launches on one

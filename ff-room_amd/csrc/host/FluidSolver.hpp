@@ -62,7 +62,7 @@ struct StepResult {
 
 class DeviceScene;
 
-class FluidSolver {
+class FluidSolver : public DeviceMirror {
 public:
     FluidSolver(Grid &grid, BoundaryManager &bm, FluidSolverParams params = {});
     ~FluidSolver();
@@ -79,8 +79,8 @@ public:
     int current_step() const { return step_; }
 
     // device mirror control (no counterpart in the reference)
-    void sync_to_host();       // download u, v, w, p, T into the Grid if stale
-    void mark_host_modified(); // upload the Grid's fields before the next step
+    void sync_to_host() override;        // download u, v, w, p, T into the Grid if stale
+    void mark_host_modified() override;  // upload the Grid's fields before the next step
     // snapshot egress (ffr_snapshot_begin/end): queue a copy of the current
     // state (FFR_SNAP_FIELDS or FFR_SNAP_CELLS) without waiting for it;
     // collect it later into the kind's arrays (returns its step)

@@ -1,9 +1,9 @@
 // Grid.hpp - host-side MAC grid, same public interface as the reference's
 // cpp/src/core/Grid.hpp:26-69 (public std::vector fields, k-fastest flat
-// indexing). The simulation state itself lives on the GPU while a
-// FluidSolver is attached; the vectors here are its host mirror, refreshed
-// by FluidSolver::sync_to_host() (called at the end of run() and by the
-// Python views) and pushed back by mark_host_modified().
+// indexing). The simulation state itself lives on the GPU while a solver
+// (FluidSolver or LBMSolver) is attached; the vectors here are its host
+// mirror, refreshed by the solver's sync_to_host() (called at the end of
+// run() and by the Python views) and pushed back by mark_host_modified().
 #pragma once
 #include <array>
 #include <cstdint>
@@ -13,7 +13,14 @@ namespace ffroom {
 
 enum class CellType : uint8_t { FLUID = 0, SOLID = 1, INFLOW = 2, OUTFLOW = 3 };
 
-class FluidSolver;
+// A solver that keeps a Grid's state on the GPU.
+struct DeviceMirror {
+    virtual void sync_to_host() = 0;        // pull the state if the device is ahead
+    virtual void mark_host_modified() = 0;  // push host edits before the next step
+
+protected:
+    ~DeviceMirror() = default;
+};
 
 struct Grid {
     int Nx, Ny, Nz;
@@ -43,7 +50,7 @@ struct Grid {
     // Bumped whenever flags, spacings or boundary lists change, so that an
     // attached solver rebuilds its device geometry before the next step.
     uint64_t geometry_version = 0;
-    FluidSolver *attached = nullptr;
+    DeviceMirror *attached = nullptr;
     // Makes the host vectors current (pulls from the GPU if needed).
     void sync_to_host();
     // Declares that host vectors were (or may be) written: the next step

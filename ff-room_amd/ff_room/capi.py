@@ -129,6 +129,8 @@ SIGNATURES = {
     "ffr_scene_download": (_I, [_P, _dp, _dp, _dp, _dp, _dp]),
     "ffr_scene_cell_type": (_I, [_P, C.POINTER(C.c_uint8)]),
     "ffr_snapshot_begin": (_I, [_P, C.c_int, C.POINTER(C.c_int)]),
+    "ffr_lbm_attach": (_I, [_P, C.c_double, C.c_double, C.c_double, C.c_double]),
+    "ffr_lbm_step": (_I, [_P, C.c_void_p]),
     "ffr_snapshot_end": (_I, [_P, C.c_int, C.POINTER(C.c_int), _dp, _dp, _dp, _dp, _dp]),
     "ffr_scene_step_count": (_I, [_P]),
     "ffr_scene_set_step_count": (_I, [_P, C.c_int]),
@@ -367,6 +369,16 @@ class DeviceScene:
         self.ctx.check(self.ctx._l.ffr_scene_download(
             self.h, *[_ptr(out[k]) for k in ("u", "v", "w", "p", "T")]), "ffr_scene_download")
         return out
+
+    def lbm_attach(self, tau: float, dt: float, convergence_tol: float, dx_ref: float):
+        """A D3Q19 lattice at rest on this FP64 scene (ffr_lbm_attach)."""
+        self.ctx.check(self.ctx._l.ffr_lbm_attach(self.h, tau, dt, convergence_tol, dx_ref),
+                       "ffr_lbm_attach")
+
+    def lbm_step(self) -> dict:
+        r = StepResult()
+        self.ctx.check(self.ctx._l.ffr_lbm_step(self.h, C.byref(r)), "ffr_lbm_step")
+        return r.as_dict()
 
     def cell_type(self) -> np.ndarray:
         """The scene's cell flags (ffr_scene_cell_type), nx*ny*nz bytes."""

@@ -110,11 +110,13 @@ class SolverBridge:
         self._c = _load_core()
 
     def _solver(self):
-        if self.config.solver.method == "lbm":
-            raise NotImplementedError(
-                "method 'lbm' (D3Q19 lattice Boltzmann) is not part of this GPU build")
         scene = Scene(self.config)
         params = make_params(self.config, self._c)
+        sc = self.config.solver
+        if sc.method == "lbm":  # solver_bridge.py:129-130: the D3Q19 lattice, isothermal
+            params.thermal = False
+            return scene, params, self._c.LBMSolver(scene.grid, scene.bm, sc.tau, sc.max_steps,
+                                                    sc.convergence_tol, sc.dt)
         return scene, params, self._c.FluidSolver(scene.grid, scene.bm, params)
 
     @staticmethod
@@ -182,11 +184,18 @@ class SolverBridge:
                                        T_field=T if params.thermal else np.zeros(speed.shape),
                                        vel=vel))
 
+        queued = hasattr(solver, "snapshot_begin")  # the lattice captures from the Grid
+
         def _cb(res):
             if res.step % save_every == 0:
-                if len(pending) >= self._IN_FLIGHT:
-                    collect()
-                pending.append((solver.snapshot_begin(cells=True), res.T_mean))
+                if not queued:
+                    snaps.append(make(res.step, res.T_mean, np.array(g.u_shaped()),
+                                      np.array(g.v_shaped()), np.array(g.w_shaped()),
+                                      np.array(g.T_shaped())))
+                else:
+                    if len(pending) >= self._IN_FLIGHT:
+                        collect()
+                    pending.append((solver.snapshot_begin(cells=True), res.T_mean))
             if progress_callback:
                 progress_callback(res.step, res.velocity_change, res.divergence_max, res.T_mean)
             if print_interval > 0 and res.step % print_interval == 0:

@@ -241,6 +241,21 @@ int ffr_apply_laplacian(ffr_scene *scene, const double *x, double *Ax);
 /* Solves A p = rhs with the scene's poisson settings; p is written back. */
 int ffr_poisson_solve(ffr_scene *scene, const double *rhs, double *p, ffr_solve_info *info);
 
+/* ---- lattice Boltzmann (LBMSolver, LBMSolver.hpp:17-39) ------------------
+ * The reference's second solver method: D3Q19, BGK with relaxation time tau,
+ * bounce-back at walls and SOLID cells with the reference's direction
+ * pairing, fans as equilibrium inflow, zero-gradient copies on outflow and
+ * opening patches. ffr_lbm_attach gives an FP64 scene a lattice at rest
+ * (LBMSolver's constructor, LBMSolver.cpp:39-75; dx_ref is Grid::dx = Lx/Nx,
+ * the unit between lattice and physical velocity); ffr_lbm_step advances it
+ * one step (LBMSolver::step, :403-444) and leaves u, v, w (faces) and p =
+ * rho/3 (cells that are not SOLID) in the scene, results equal to the
+ * reference's bits. ffr_step is the projection step and does not touch the
+ * lattice. */
+int ffr_lbm_attach(ffr_scene *scene, double tau, double dt, double convergence_tol,
+                   double dx_ref);
+int ffr_lbm_step(ffr_scene *scene, ffr_step_result *out);
+
 /* ---- batching and serving ---------------------------------------------- */
 /* Continuous batching of many scenes on the context's device (no counterpart
  * in the reference, which runs one scene per process). A ticket asks for up
@@ -305,7 +320,9 @@ int ffr_prof_read(ffr_ctx *ctx, ffr_prof_entry *out, int cap, int *n_out);
  * kernel's algorithmic bytes per launch. Kernels: "predict", "correct",
  * "temperature" (thermal scenes), "rhs",
  * "cg_spmv", "cg_update", "cg_dir", "mg_smooth_l0", "mg_resid0",
- * "pcg_dirq", "pcg_update", "dc_residual", and "copy" (a plain FP64 copy
+ * "pcg_dirq", "pcg_update", "dc_residual", the lattice's "lbm_relax",
+ * "lbm_stream", "lbm_macro", "lbm_faces" (after ffr_lbm_attach; timing
+ * advances the lattice), and "copy" (a plain FP64 copy
  * of nx*ny*nz doubles, to calibrate memory counters). The timed kernels overwrite
  * solver scratch; "correct" rewrites u, v, w with the values the last step gave
  * them: call between steps, not inside one. */
