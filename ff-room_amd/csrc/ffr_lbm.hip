@@ -41,7 +41,7 @@ int ffr_lbm_attach(ffr_scene *s, double tau, double dt, double convergence_tol, 
     HIP_CHECK_CTX(c, hipSetDevice(c->device), FFR_E_DEVICE);
     hipStreamSynchronize(s->stream);
     lbm_release(s);
-    const size_t nc = s->nc, bytes = (2 * 19 * nc + 3 * nc) * sizeof(double);
+    const size_t nc = s->nc, bytes = (2 * 19 * nc + 3 * nc) * sizeof(double) + nc * sizeof(uint32_t);
     if (c->in_use + bytes > c->budget) {
         c->fail("ffr_lbm_attach: the lattice needs %zu bytes, the context budget has %zu free", bytes,
                 c->budget - c->in_use);
@@ -54,12 +54,14 @@ int ffr_lbm_attach(ffr_scene *s, double tau, double dt, double convergence_tol, 
     L.f = (double *)L.arena;
     L.f2 = L.f + 19 * nc;
     L.cu = L.f2 + 19 * nc;
+    L.mask = (uint32_t *)(L.cu + 3 * nc);
     L.tau = tau;
     L.dt = dt;
     L.tol = convergence_tol;
     L.dx_ref = dx_ref;
     L.steps = 0;
     L.on = true;
+    lbm_launch_mask(s->stream, s->geo, L.mask);
     lbm_launch_rest(s->stream, (uint32_t)nc, L.f);
     lbm_launch_relax(s->stream, s->geo, L.f, dt / dx_ref, 1.0 / tau, false, s->rec);
     HIP_CHECK_CTX(c, hipStreamSynchronize(s->stream), FFR_E_DEVICE);
@@ -87,9 +89,10 @@ int ffr_lbm_step(ffr_scene *s, ffr_step_result *out)
         ProfScope ps(c, st, "lbm_relax", n * 19 * 16 + n);
         lbm_launch_relax(st, g, L.f, L.dt / L.dx_ref, 1.0 / L.tau, true, s->rec);
     }
+    const double l2s = L.dx_ref / L.dt;
     {
-        ProfScope ps(c, st, "lbm_stream", n * 19 * 16 + n);
-        lbm_launch_stream(st, g, L.f, L.f2);
+        ProfScope ps(c, st, "lbm_stream", n * (19 * 16 + 4 + 32));
+        lbm_launch_stream(st, g, L.mask, L.f, L.f2, L.cu, s->p, l2s);
     }
     std::swap(L.f, L.f2);
     // the reference copies every outflow patch (openings included), then
@@ -98,14 +101,10 @@ int ffr_lbm_step(ffr_scene *s, ffr_step_result *out)
         const int n_ax = axis == 0 ? s->nx : (axis == 1 ? s->ny : s->nz);
         const int src = side ? n_ax - 2 : 1;
         if (src < 0 || src >= n_ax || a1 < a0 || b1 < b0) return;
-        lbm_launch_patch(st, g, L.f, LbmPatch{axis, side, a0, a1, b0, b1});
+        lbm_launch_patch(st, g, L.f, LbmPatch{axis, side, a0, a1, b0, b1}, L.cu, s->p, l2s);
     };
     for (const auto &o : s->outflows) patch(o.axis, o.side, o.a_min, o.a_max, o.b_min, o.b_max);
     for (const auto &o : s->openings) patch(o.axis, o.side, o.a_min, o.a_max, o.b_min, o.b_max);
-    {
-        ProfScope ps(c, st, "lbm_macro", n * 19 * 8 + n * 32);
-        lbm_launch_macro(st, g, L.f, L.cu, s->p, L.dx_ref / L.dt);
-    }
     {
         const double faces = (double)(s->nu + s->nv + s->nw);
         ProfScope ps(c, st, "lbm_faces", faces * 16 + n * 24);

@@ -17,6 +17,7 @@ struct LbmState {
     double tau = 0.0, dt = 0.0, tol = 0.0, dx_ref = 0.0;
     double *f = nullptr, *f2 = nullptr;  // 19 * nc each
     double *cu = nullptr;                // cell velocities ux | uy | uz, nc each
+    uint32_t *mask = nullptr;            // open neighbours per cell (lbm_kernels.hip)
     void *arena = nullptr;
     size_t bytes = 0;
     int steps = 0;
@@ -25,10 +26,13 @@ struct LbmState {
 void lbm_launch_rest(hipStream_t s, uint32_t nc, double *f);
 void lbm_launch_relax(hipStream_t s, const Geo &g, double *f, double s2l, double inv_tau,
                       bool collide, StepRec *rec);
-void lbm_launch_stream(hipStream_t s, const Geo &g, const double *f, double *f2);
-void lbm_launch_patch(hipStream_t s, const Geo &g, double *f, const LbmPatch &pt);
-void lbm_launch_macro(hipStream_t s, const Geo &g, const double *f, double *cu, double *p,
-                      double l2s);
+void lbm_launch_mask(hipStream_t s, const Geo &g, uint32_t *mask);
+// streaming, and the moments of the result (cell velocity [m/s] = lattice
+// velocity * l2s, p = rho / 3)
+void lbm_launch_stream(hipStream_t s, const Geo &g, const uint32_t *mask, const double *f,
+                       double *f2, double *cu, double *p, double l2s);
+void lbm_launch_patch(hipStream_t s, const Geo &g, double *f, const LbmPatch &pt, double *cu,
+                      double *p, double l2s);
 void lbm_launch_faces(hipStream_t s, const Geo &g, const double *cu, double *u, double *v,
                       double *w, StepRec *rec);
 

@@ -671,20 +671,16 @@ int ffr_prof_kernel(ffr_scene *s, const char *name, int reps, double *mean_ms, d
             launch = [&] { launch_correct<double>(st, g, sp, (double *)s->us, (double *)s->vs, (double *)s->ws, s->p, (double *)s->u, (double *)s->v, (double *)s->w, s->rec); };
         else
             launch = [&] { launch_correct<float>(st, g, sp, (float *)s->us, (float *)s->vs, (float *)s->ws, s->p, (float *)s->u, (float *)s->v, (float *)s->w, s->rec); };
-    } else if (s->lbm.on && (k == "lbm_relax" || k == "lbm_stream" || k == "lbm_macro" ||
-                             k == "lbm_faces")) {
+    } else if (s->lbm.on && (k == "lbm_relax" || k == "lbm_stream" || k == "lbm_faces")) {
         // the lattice's kernels on its current state (algorithmic bytes:
         // populations read and written, flags, cell velocities and faces)
         LbmState &L = s->lbm;
         if (k == "lbm_relax") {
             b = nc * (19 * 16 + 1);
             launch = [&] { lbm_launch_relax(st, g, L.f, L.dt / L.dx_ref, 1.0 / L.tau, true, s->rec); };
-        } else if (k == "lbm_stream") {
-            b = nc * (19 * 16 + 1);
-            launch = [&] { lbm_launch_stream(st, g, L.f, L.f2); };
-        } else if (k == "lbm_macro") {
-            b = nc * (19 * 8 + 1 + 32);
-            launch = [&] { lbm_launch_macro(st, g, L.f, L.cu, s->p, L.dx_ref / L.dt); };
+        } else if (k == "lbm_stream") {  // populations, mask, cell velocities and p
+            b = nc * (19 * 16 + 4 + 32);
+            launch = [&] { lbm_launch_stream(st, g, L.mask, L.f, L.f2, L.cu, s->p, L.dx_ref / L.dt); };
         } else {
             b = faces * 16 + nc * 24;
             launch = [&] { lbm_launch_faces(st, g, L.cu, (double *)s->u, (double *)s->v, (double *)s->w, s->rec); };

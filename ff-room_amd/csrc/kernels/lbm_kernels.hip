@@ -7,7 +7,11 @@
 // other buffer; the zero-gradient copies of the outflow and opening patches,
 // one launch per patch in the reference's order; cell moments; MAC faces with
 // the velocity change. All arithmetic is FP64 without contraction, in the
-// reference's operation order, so results equal its bits.
+// reference's operation order, so results equal its bits. Streaming reads a
+// per-cell mask of open neighbours (built once: bit d set when the cell in
+// direction d is inside the domain and not SOLID, bit 31 when the cell is
+// SOLID) instead of 38 flag bytes, and also forms the cell's moments; a
+// patch copy recomputes the moments of the cells it overwrites.
 //
 // Streaming: the reference pushes into a zeroed buffer (LBMSolver.cpp:151-
 // 190) and bounces a population that would leave the domain or enter a
@@ -45,6 +49,13 @@ __device__ __forceinline__ int lbm_opp(int q)
     constexpr int o[19] = {0, 2, 1, 4, 3, 6, 5, 8, 7, 10, 9, 12, 11, 14, 13, 16, 15, 18, 17};
     return o[q];
 }
+// the true reverse of direction q (for the upstream neighbour)
+__device__ __forceinline__ int lbm_rev(int q)
+{
+    constexpr int r[19] = {0, 2, 1, 4, 3, 6, 5, 10, 9, 8, 7, 14, 13, 12, 11, 18, 17, 16, 15};
+    return r[q];
+}
+constexpr uint32_t kSolidBit = 1u << 31;
 __device__ __forceinline__ double lbm_w(int q)
 {
     return q == 0 ? 1.0 / 3.0 : (q < 7 ? 1.0 / 18.0 : 1.0 / 36.0);
@@ -138,37 +149,85 @@ __global__ void __launch_bounds__(kBlock) k_lbm_relax(Geo g, double *f, double s
     }
 }
 
-// streaming with bounce-back, pulled into f2 (see the file comment)
-__global__ void __launch_bounds__(kBlock) k_lbm_stream(Geo g, const double *__restrict__ f,
-                                                        double *__restrict__ f2)
+// open-neighbour masks (see the file comment)
+__global__ void __launch_bounds__(kBlock) k_lbm_mask(Geo g, uint32_t *mask)
 {
     const uint32_t nc = g.nc;
     for (uint32_t c = blockIdx.x * kBlock + threadIdx.x; c < nc; c += gridDim.x * kBlock) {
-        const bool solid = g.flags[c] == SOLID;
         int i, j, k;
         cell_ijk(g, c, i, j, k);
-        auto open = [&](int a, int b, int d) {
-            return a >= 0 && a < g.nx && b >= 0 && b < g.ny && d >= 0 && d < g.nz &&
-                   g.flags[((uint32_t)a * g.ny + b) * g.nz + d] != SOLID;
-        };
+        uint32_t m = g.flags[c] == SOLID ? kSolidBit : 0u;
+#pragma unroll
+        for (int d = 0; d < 19; d++) {
+            const int a = i + lbm_ex(d), b = j + lbm_ey(d), e = k + lbm_ez(d);
+            if (a >= 0 && a < g.nx && b >= 0 && b < g.ny && e >= 0 && e < g.nz &&
+                g.flags[((uint32_t)a * g.ny + b) * g.nz + e] != SOLID)
+                m |= 1u << d;
+        }
+        mask[c] = m;
+    }
+}
+
+// moments of a cell's new populations -> cell velocity [m/s] and p = rho/3
+// (LBMSolver.cpp:288-326)
+__device__ __forceinline__ void store_moments(uint32_t nc, uint32_t c, double r, double x, double y,
+                                              double z, double l2s, double *cu, double *p)
+{
+    if (r > 1e-14) {
+        x /= r;
+        y /= r;
+        z /= r;
+    }
+    cu[c] = x * l2s;
+    cu[nc + c] = y * l2s;
+    cu[2ull * nc + c] = z * l2s;
+    p[c] = r * (1.0 / 3.0);
+}
+
+// streaming with bounce-back, pulled into f2 (see the file comment), and
+// the moments of the result; SOLID cells get zero populations and velocity
+// and keep their p
+__global__ void __launch_bounds__(kBlock) k_lbm_stream(Geo g, const uint32_t *__restrict__ mask,
+                                                        const double *__restrict__ f,
+                                                        double *__restrict__ f2, double *cu, double *p,
+                                                        double l2s)
+{
+    const uint32_t nc = g.nc, nynz = (uint32_t)g.ny * g.nz, nz = g.nz;
+    for (uint32_t c = blockIdx.x * kBlock + threadIdx.x; c < nc; c += gridDim.x * kBlock) {
+        const uint32_t m = mask[c];
+        if (m & kSolidBit) {
+#pragma unroll
+            for (int q = 0; q < 19; q++) f2[(size_t)q * nc + c] = 0.0;
+            cu[c] = 0.0;
+            cu[nc + c] = 0.0;
+            cu[2ull * nc + c] = 0.0;
+            continue;
+        }
+        double r = 0.0, x = 0.0, y = 0.0, z = 0.0;
 #pragma unroll
         for (int q = 0; q < 19; q++) {
             double v = 0.0;
-            if (!solid) {
-                const int ui = i - lbm_ex(q), uj = j - lbm_ey(q), uk = k - lbm_ez(q);
-                if (open(ui, uj, uk)) v += f[(size_t)q * nc + ((uint32_t)ui * g.ny + uj) * g.nz + uk];
-                const int b = lbm_opp(q);
-                if (!open(i + lbm_ex(b), j + lbm_ey(b), k + lbm_ez(b))) v += f[(size_t)b * nc + c];
+            if (m & (1u << lbm_rev(q))) {  // upstream neighbour c - e_q is open
+                const int off = lbm_ex(q) * (int)nynz + lbm_ey(q) * (int)nz + lbm_ez(q);
+                v += f[(size_t)q * nc + (uint32_t)((int)c - off)];
             }
+            const int b = lbm_opp(q);
+            if (!(m & (1u << b))) v += f[(size_t)b * nc + c];  // bounced at a wall or SOLID
             f2[(size_t)q * nc + c] = v;
+            r += v;
+            x += v * lbm_ex(q);
+            y += v * lbm_ey(q);
+            z += v * lbm_ez(q);
         }
+        store_moments(nc, c, r, x, y, z, l2s, cu, p);
     }
 }
 
 // Zero-gradient copy of one patch's OUTFLOW cells from the cell one step
 // inside along the patch axis (LBMSolver.cpp:196-282); a, b the patch's two
 // in-wall axes in order.
-__global__ void __launch_bounds__(kBlock) k_lbm_patch(Geo g, double *f, LbmPatch pt)
+__global__ void __launch_bounds__(kBlock) k_lbm_patch(Geo g, double *f, LbmPatch pt, double *cu,
+                                                       double *p, double l2s)
 {
     const uint32_t na = (uint32_t)(pt.a1 - pt.a0 + 1), nb = (uint32_t)(pt.b1 - pt.b0 + 1);
     const uint32_t n = na * nb, nc = g.nc;
@@ -189,29 +248,17 @@ __global__ void __launch_bounds__(kBlock) k_lbm_patch(Geo g, double *f, LbmPatch
         const uint32_t co = ((uint32_t)cw[0] * g.ny + cw[1]) * g.nz + cw[2];
         if (g.flags[co] != OUTFLOW) continue;
         const uint32_t ci = ((uint32_t)cs[0] * g.ny + cs[1]) * g.nz + cs[2];
+        double r = 0.0, x = 0.0, y = 0.0, z = 0.0;
 #pragma unroll
-        for (int q = 0; q < 19; q++) f[(size_t)q * nc + co] = f[(size_t)q * nc + ci];
-    }
-}
-
-// cell velocities [m/s] and p = rho/3 on non-SOLID cells (:288-326)
-__global__ void __launch_bounds__(kBlock) k_lbm_macro(Geo g, const double *f, double *cu, double *p,
-                                                       double l2s)
-{
-    const uint32_t nc = g.nc;
-    for (uint32_t c = blockIdx.x * kBlock + threadIdx.x; c < nc; c += gridDim.x * kBlock) {
-        if (g.flags[c] == SOLID) {
-            cu[c] = 0.0;
-            cu[nc + c] = 0.0;
-            cu[2ull * nc + c] = 0.0;
-            continue;
+        for (int q = 0; q < 19; q++) {
+            const double v = f[(size_t)q * nc + ci];
+            f[(size_t)q * nc + co] = v;
+            r += v;
+            x += v * lbm_ex(q);
+            y += v * lbm_ey(q);
+            z += v * lbm_ez(q);
         }
-        double rho, ux, uy, uz;
-        moments(f, nc, c, rho, ux, uy, uz);
-        cu[c] = ux * l2s;
-        cu[nc + c] = uy * l2s;
-        cu[2ull * nc + c] = uz * l2s;
-        p[c] = rho * (1.0 / 3.0);
+        store_moments(nc, co, r, x, y, z, l2s, cu, p);
     }
 }
 
@@ -279,19 +326,20 @@ void lbm_launch_relax(hipStream_t s, const Geo &g, double *f, double s2l, double
     FFR_LAUNCH(k_lbm_relax, dim3(grid_for(g.nc)), dim3(kBlock), 0, s, g, f, s2l, inv_tau,
                collide ? 1 : 0, rec);
 }
-void lbm_launch_stream(hipStream_t s, const Geo &g, const double *f, double *f2)
+void lbm_launch_mask(hipStream_t s, const Geo &g, uint32_t *mask)
 {
-    FFR_LAUNCH(k_lbm_stream, dim3(grid_for(g.nc)), dim3(kBlock), 0, s, g, f, f2);
+    FFR_LAUNCH(k_lbm_mask, dim3(grid_for(g.nc)), dim3(kBlock), 0, s, g, mask);
 }
-void lbm_launch_patch(hipStream_t s, const Geo &g, double *f, const LbmPatch &pt)
+void lbm_launch_stream(hipStream_t s, const Geo &g, const uint32_t *mask, const double *f,
+                       double *f2, double *cu, double *p, double l2s)
+{
+    FFR_LAUNCH(k_lbm_stream, dim3(grid_for(g.nc)), dim3(kBlock), 0, s, g, mask, f, f2, cu, p, l2s);
+}
+void lbm_launch_patch(hipStream_t s, const Geo &g, double *f, const LbmPatch &pt, double *cu,
+                      double *p, double l2s)
 {
     const uint64_t n = (uint64_t)(pt.a1 - pt.a0 + 1) * (pt.b1 - pt.b0 + 1);
-    FFR_LAUNCH(k_lbm_patch, dim3(grid_for(n)), dim3(kBlock), 0, s, g, f, pt);
-}
-void lbm_launch_macro(hipStream_t s, const Geo &g, const double *f, double *cu, double *p,
-                      double l2s)
-{
-    FFR_LAUNCH(k_lbm_macro, dim3(grid_for(g.nc)), dim3(kBlock), 0, s, g, f, cu, p, l2s);
+    FFR_LAUNCH(k_lbm_patch, dim3(grid_for(n)), dim3(kBlock), 0, s, g, f, pt, cu, p, l2s);
 }
 void lbm_launch_faces(hipStream_t s, const Geo &g, const double *cu, double *u, double *v,
                       double *w, StepRec *rec)

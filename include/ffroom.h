@@ -321,7 +321,7 @@ int ffr_prof_read(ffr_ctx *ctx, ffr_prof_entry *out, int cap, int *n_out);
  * "temperature" (thermal scenes), "rhs",
  * "cg_spmv", "cg_update", "cg_dir", "mg_smooth_l0", "mg_resid0",
  * "pcg_dirq", "pcg_update", "dc_residual", the lattice's "lbm_relax",
- * "lbm_stream", "lbm_macro", "lbm_faces" (after ffr_lbm_attach; timing
+ * "lbm_stream", "lbm_faces" (after ffr_lbm_attach; timing
  * advances the lattice), and "copy" (a plain FP64 copy
  * of nx*ny*nz doubles, to calibrate memory counters). The timed kernels overwrite
  * solver scratch; "correct" rewrites u, v, w with the values the last step gave
