@@ -102,6 +102,9 @@ struct Geo {
     const float *hxf, *hyf, *hzf, *ihxf, *ihyf, *ihzf, *i2xf, *i2yf, *i2zf;
     const float *cxf, *cyf, *czf, *icxf, *icyf, *iczf, *lapf;
     const uint16_t *nbm;               // neighbour codes (see NB_*)
+    // FP32 per-axis factors of the multigrid's symmetric level-0 operator on a
+    // stretched grid (relative widths w | face factors t, mg_kernels.hip)
+    const float *mgw, *mgt;
     const uint16_t *fcode;             // face mask codes (see FC_*), u | v | w
     double sxd, syd, szd;              // uniform-grid coupling 1/h^2 per axis (0 if not uniform)
     float sx, sy, sz;

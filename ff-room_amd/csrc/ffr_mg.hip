@@ -76,8 +76,11 @@ size_t mg_layout(char *base, const Dims &D, bool bf16, bool dense, Multigrid *M,
     float *sc = (float *)take(8ull * H);
     float *z0 = (float *)take(8ull * H);
     uint16_t *nbs = (uint16_t *)take(4ull * H);
+    const size_t na = (size_t)D.n[0][0] + D.n[0][1] + D.n[0][2];
+    float *mw = (float *)take(4 * na), *mt = (float *)take(4 * (na + 3));
     if (M) {
         M->e = e; M->ri = ri; M->pi = pi; M->qi = qi; M->sc = sc;
+        M->mgw = mw; M->mgt = mt;
         M->lv[0].z = z0;
         M->lv[0].r = ri;
         M->lv[0].nbs = nbs;
@@ -112,17 +115,32 @@ size_t mg_layout(char *base, const Dims &D, bool bf16, bool dense, Multigrid *M,
     return off;
 }
 
-bool uniform(const std::vector<double> &h)
+double mean(const double *h, int n)
 {
-    for (double x : h)
-        if (x != h[0]) return false;  // exactly: the uniform kernels read h[0] for every cell
-    return true;
+    double s = 0.0;
+    for (int i = 0; i < n; i++) s += h[i];
+    return s / n;
+}
+
+// Level-0 factors of a stretched grid (see Multigrid::mgw / mgt): per axis,
+// w[i] = d[i] / m and t[i] = 1 / (0.5 (d[i-1] + d[i]) m), the centre distance
+// of PoissonSolver.cpp:34-49, for 1 <= i < n; t[0] = t[n] = 0.
+void stretch_factors(const double *d, int n, std::vector<float> &w, std::vector<float> &t)
+{
+    const double m = mean(d, n);
+    for (int i = 0; i < n; i++) w.push_back((float)(d[i] / m));
+    t.push_back(0.f);
+    for (int i = 1; i < n; i++) t.push_back((float)(1.0 / (0.5 * (d[i - 1] + d[i]) * m)));
+    t.push_back(0.f);
 }
 
 }  // namespace
 
-size_t mg_bytes(int nx, int ny, int nz, const double h0[3], const ffr_params &p)
+size_t mg_bytes(int nx, int ny, int nz, const double *dxv, const double *dyv, const double *dzv,
+                const ffr_params &p)
 {
+    // level extents follow the mean spacing (the uniform spacing when uniform)
+    const double h0[3] = {mean(dxv, nx), mean(dyv, ny), mean(dzv, nz)};
     const Dims D = level_dims(nx, ny, nz, h0, p);
     return mg_layout(nullptr, D, p.mg_bf16 != 0, p.mg_coarse_sweeps == 0, nullptr, nullptr);
 }
@@ -230,17 +248,13 @@ static MgState *g_unused_state;  // placeholder for sizing calls
 int mg_create(ffr_scene *s)
 {
     if (s->mg.levels) return FFR_OK;
-    if (!uniform(s->dxv) || !uniform(s->dyv) || !uniform(s->dzv)) {
-        s->ctx->fail("multigrid-preconditioned CG needs uniform spacing per axis (stretched grids "
-                     "use the CG solver)");
-        return FFR_E_UNSUPPORTED;
-    }
     if (mg_tail_allow_lds() != 0) {
         s->ctx->fail("hipFuncSetAttribute: %zu bytes of LDS for the multigrid tail refused",
                      (size_t)kTailLdsMax);
         return FFR_E_DEVICE;
     }
-    const double h0[3] = {s->dxv[0], s->dyv[0], s->dzv[0]};
+    const double h0[3] = {mean(s->dxv.data(), s->nx), mean(s->dyv.data(), s->ny),
+                          mean(s->dzv.data(), s->nz)};
     const Dims D = level_dims(s->nx, s->ny, s->nz, h0, s->prm);
     Multigrid &M = s->mg;
     const bool bf16 = s->prm.mg_bf16 != 0;
@@ -309,6 +323,18 @@ int mg_create(ffr_scene *s)
     M.coarse_sweeps = s->prm.mg_coarse_sweeps > 0 ? s->prm.mg_coarse_sweeps : 8;
     hipStream_t str = s->stream;
     HIP_CHECK_CTX(s->ctx, hipMemsetAsync(M.arena, 0, bytes, str), FFR_E_DEVICE);
+    s->geo.mgw = M.mgw;
+    s->geo.mgt = M.mgt;
+    std::vector<float> fw, ft;  // (stay zero on a uniform grid, where no kernel reads them)
+    if (!s->geo.uniform) {
+        stretch_factors(s->dxv.data(), s->nx, fw, ft);
+        stretch_factors(s->dyv.data(), s->ny, fw, ft);
+        stretch_factors(s->dzv.data(), s->nz, fw, ft);
+        HIP_CHECK_CTX(s->ctx, hipMemcpyAsync(M.mgw, fw.data(), 4 * fw.size(), hipMemcpyHostToDevice, str),
+                      FFR_E_DEVICE);
+        HIP_CHECK_CTX(s->ctx, hipMemcpyAsync(M.mgt, ft.data(), 4 * ft.size(), hipMemcpyHostToDevice, str),
+                      FFR_E_DEVICE);
+    }
     mg_launch_split_codes(str, s->geo, M.lv[0]);
     for (int l = 0; l + 1 < D.levels; l++) mg_launch_build(str, s->geo, M.lv[l], M.lv[l + 1], l == 0);
     HIP_CHECK_CTX(s->ctx, hipStreamSynchronize(str), FFR_E_DEVICE);

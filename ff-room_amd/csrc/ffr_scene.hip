@@ -192,8 +192,7 @@ size_t scene_bytes(const ffr_scene_desc *d, const ffr_params *p, bool with_mg)
 {
     size_t b = layout(nullptr, d, p, nullptr);
     if (with_mg) {
-        const double h0[3] = {d->dxv[0], d->dyv[0], d->dzv[0]};
-        b += mg_bytes(d->nx, d->ny, d->nz, h0, *p);
+        b += mg_bytes(d->nx, d->ny, d->nz, d->dxv, d->dyv, d->dzv, *p);
     }
     return b;
 }

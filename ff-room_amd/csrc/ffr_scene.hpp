@@ -210,7 +210,8 @@ struct Gang {
     int launch_round(const std::vector<Recorder *> &recs);
 };
 // multigrid (ffr_mg.hip)
-size_t mg_bytes(int nx, int ny, int nz, const double h0[3], const ffr_params &p);
+size_t mg_bytes(int nx, int ny, int nz, const double *dxv, const double *dyv, const double *dzv,
+                const ffr_params &p);
 int mg_create(ffr_scene *s);
 void mg_destroy(ffr_scene *s);
 int mg_begin(ffr_scene *s);

@@ -86,6 +86,52 @@ __device__ __forceinline__ float diag0m(const Geo &g, uint16_t m)
     return d < 1e-30f ? 1.f : d;
 }
 
+// ---- level-0 operator of a stretched grid (ST) ------------------------------
+// The reference row (PoissonSolver.cpp:34-49) scales face f of cell c by
+// 1/(h_f d_c), h_f the centre distance across the face and d_c the cell's
+// width along its axis, so the two cells of a face see different scales and
+// the operator is not symmetric. Multiplied by the cell's volume relative to
+// the mean cell's (w_x w_y w_z, w = d / mean width of the axis), a row's face
+// coefficient becomes w_b w_c / (h_f m_a) (the other two axes' relative
+// widths, m_a the face axis' mean width): the same from both sides. The inner
+// solve runs on that symmetric, volume-weighted form with the outer residual
+// weighted alike, which has the same solution; per axis, Geo.mgw holds w
+// (n values) and Geo.mgt holds t = 1/(h_f m_a) of the face below index i
+// (n + 1 values, 0 at both walls), so the couplings of a cell are t w w.
+template <bool ST>
+struct Cof {
+    float f[6];  // -x +x -y +y -z +z
+};
+template <>
+struct Cof<false> {};  // uniform spacing: the per-axis constants of Geo
+
+template <bool ST>
+__device__ __forceinline__ Cof<ST> cof_at(const Geo &g, uint32_t c)
+{
+    Cof<ST> r;
+    if constexpr (ST) {
+        c = c < g.nc ? c : g.nc - 1;  // slots past the end: any in-range cell
+        const uint32_t i = fdiv(c, g.f_nynz), rem = c - i * (uint32_t)(g.ny * g.nz);
+        const uint32_t j = fdiv(rem, g.f_nz), k = rem - j * (uint32_t)g.nz;
+        const float *W = g.mgw, *tx = g.mgt, *ty = g.mgt + g.nx + 1, *tz = g.mgt + g.nx + g.ny + 2;
+        const float wx = W[i], wy = W[g.nx + j], wz = W[g.nx + g.ny + k];
+        const float yz = wy * wz, xz = wx * wz, xy = wx * wy;
+        r.f[0] = tx[i] * yz;
+        r.f[1] = tx[i + 1] * yz;
+        r.f[2] = ty[j] * xz;
+        r.f[3] = ty[j + 1] * xz;
+        r.f[4] = tz[k] * xy;
+        r.f[5] = tz[k + 1] * xy;
+    }
+    return r;
+}
+// weight of face q in the diagonal: 1 to an unknown neighbour, 2 to an
+// OUTFLOW one (PoissonSolver.cpp:52-87), 0 across a wall or to SOLID
+__device__ __forceinline__ float face_cnt(uint16_t m, int q)
+{
+    return (m & (NB_UNK0 << q)) ? 1.f : ((m & (NB_OUT0 << q)) ? 2.f : 0.f);
+}
+
 // ---- level-0 split layout (see ffr_mg.hpp) -----------------------------------
 // position of natural cell c (natural code m) in its own colour's array / the
 // base of the other colour's array
@@ -164,7 +210,7 @@ __device__ __forceinline__ float dpart0(const Geo &g, uint16_t m, int axis)
 // diagonal parts (minus twice the couplings inside the aggregate) and of the
 // couplings across its lower faces, per axis, then scales each axis by 1/2
 // when the coarse level halved it.
-template <bool FINE0>
+template <bool FINE0, bool ST>
 __global__ void __launch_bounds__(kBlock) k_mg_build(Geo g, MgLevel F, MgLevel C)
 {
     uint32_t sb, se;
@@ -182,7 +228,17 @@ __global__ void __launch_bounds__(kBlock) k_mg_build(Geo g, MgLevel F, MgLevel C
                 for (int d = k0; d <= k1; d++) {
                     bool act;
                     float dp[3];
-                    if (FINE0) {
+                    if constexpr (FINE0 && ST) {
+                        const uint32_t c = cidx(g, a, b, d);
+                        const uint16_t m = g.nbm[c];
+                        act = (m & NB_SELF) != 0;
+                        const Cof<ST> cf = cof_at<ST>(g, c);
+#pragma unroll
+                        for (int ax = 0; ax < 3; ax++)
+                            dp[ax] = act ? cf.f[2 * ax] * face_cnt(m, 2 * ax) +
+                                               cf.f[2 * ax + 1] * face_cnt(m, 2 * ax + 1)
+                                         : 0.f;
+                    } else if (FINE0) {
                         const uint16_t m = g.nbm[cidx(g, a, b, d)];
                         act = (m & NB_SELF) != 0;
                         for (int ax = 0; ax < 3; ax++) dp[ax] = act ? dpart0(g, m, ax) : 0.f;
@@ -196,6 +252,13 @@ __global__ void __launch_bounds__(kBlock) k_mg_build(Geo g, MgLevel F, MgLevel C
                     if (!act) continue;
                     any = true;
                     auto cp = [&](int axis, int x, int y, int z) {
+                        if constexpr (FINE0 && ST) {  // the lower face's coupling, both cells unknown
+                            const uint32_t c = cidx(g, x, y, z);
+                            const uint16_t m = g.nbm[c];
+                            if (!(m & NB_SELF) || !(m & (NB_UNK0 << (2 * axis)))) return 0.f;
+                            const Cof<ST> cf = cof_at<ST>(g, c);
+                            return axis == 0 ? cf.f[0] : (axis == 1 ? cf.f[2] : cf.f[4]);
+                        }
                         return FINE0 ? coupling0(g, axis, x, y, z) : couplingc(F, axis, x, y, z);
                     };
                     for (int ax = 0; ax < 3; ax++) D[ax] += dp[ax];
@@ -290,11 +353,40 @@ __device__ __forceinline__ float diag_of(const Geo &g, uint16_t nb, float dint)
     return (nb & kFaceBits) == kInterior ? dint : diag0m(g, nb);
 }
 
+template <bool ST>
+__device__ __forceinline__ float nb_sum_t(const Geo &g, uint16_t nb, const float v[6],
+                                          const Cof<ST> &cf)
+{
+    if constexpr (!ST) {
+        return nb_sum(g, nb, v);
+    } else {
+        const float ox = ((nb & (NB_UNK0 << 0)) ? cf.f[0] * v[0] : 0.f) +
+                         ((nb & (NB_UNK0 << 1)) ? cf.f[1] * v[1] : 0.f);
+        const float oy = ((nb & (NB_UNK0 << 2)) ? cf.f[2] * v[2] : 0.f) +
+                         ((nb & (NB_UNK0 << 3)) ? cf.f[3] * v[3] : 0.f);
+        const float oz = ((nb & (NB_UNK0 << 4)) ? cf.f[4] * v[4] : 0.f) +
+                         ((nb & (NB_UNK0 << 5)) ? cf.f[5] * v[5] : 0.f);
+        return ox + oy + oz;
+    }
+}
+template <bool ST>
+__device__ __forceinline__ float diag_t(const Geo &g, uint16_t nb, float dint, const Cof<ST> &cf)
+{
+    if constexpr (!ST) {
+        return diag_of(g, nb, dint);
+    } else {
+        float d = 0.f;
+#pragma unroll
+        for (int q = 0; q < 6; q++) d += cf.f[q] * face_cnt(nb, q);
+        return d < 1e-30f ? 1.f : d;
+    }
+}
+
 // With DOT (the V-cycle's last pass on level 0, a red pass), the kernel also
 // writes the partials of r.z over both colours for the PCG: the thread of
 // own slot h adds r z of that slot after its update and r z of the other
 // colour's slot h (the black array is never longer than the red one).
-template <bool DOT>
+template <bool DOT, bool ST>
 struct KB_mg_smooth0 {
     static __device__ __forceinline__ void run(const VB &vb, const Geo &g, const MgLevel &L,
                                                const uint16_t *__restrict__ code,
@@ -318,12 +410,15 @@ struct KB_mg_smooth0 {
     for (uint32_t h0 = sb + threadIdx.x; h0 < se; h0 += kU * kBlock) {
         uint16_t nb[kU];
         float rv[kU], zv[kU][6], rov[kU], zov[kU];
+        Cof<ST> cf[kU];
 #pragma unroll
         for (int t = 0; t < kU; t++) {
             const uint32_t h = h0 + t * kBlock;  // reads past cnt return 0
+            const uint32_t cn = 2 * h + split_parity(L, h, col);
             nb[t] = buf_ldu16(bcode, h);
             rv[t] = buf_ldf(br, h);
-            if (!zero_start) nb_gather(bo, 2 * h + split_parity(L, h, col), si, sj, zv[t]);
+            cf[t] = cof_at<ST>(g, cn);
+            if (!zero_start) nb_gather(bo, cn, si, sj, zv[t]);
             if (DOT) {  // past cnt_o both read 0
                 rov[t] = buf_ldf(bro, h);
                 zov[t] = buf_ldf(bo, h);
@@ -336,8 +431,8 @@ struct KB_mg_smooth0 {
             if (h >= se) break;
             float v = 0.f;
             if (nb[t] & NB_SELF) {
-                const float of = zero_start ? 0.f : nb_sum(g, nb[t], zv[t]);
-                v = (rv[t] + of) / diag_of(g, nb[t], dint);
+                const float of = zero_start ? 0.f : nb_sum_t<ST>(g, nb[t], zv[t], cf[t]);
+                v = (rv[t] + of) / diag_t<ST>(g, nb[t], dint, cf[t]);
             }
             buf_stf(bz, h, v);
             if (DOT) acc += (double)rv[t] * (double)v + (double)rov[t] * (double)zov[t];
@@ -346,7 +441,7 @@ struct KB_mg_smooth0 {
     if (DOT) block_partial(acc, pslot(red, SLOT_MG_A), blk, lds);
 }
 };
-template <bool DOT>
+template <bool DOT, bool ST>
 __global__ void __launch_bounds__(kBlock) FFR_RESIDENT k_mg_smooth0(Geo g, MgLevel L,
                                                        const uint16_t *__restrict__ code,
                                                        const float *__restrict__ zo,
@@ -357,7 +452,7 @@ __global__ void __launch_bounds__(kBlock) FFR_RESIDENT k_mg_smooth0(Geo g, MgLev
                                                        int zero_start, const MgState *m,
                                                        RedScratch red)
 {
-    KB_mg_smooth0<DOT>::run(vb_here(), g, L, code, zo, zc, rc, ro, cnt, cnt_o, col, zero_start, m, red);
+    KB_mg_smooth0<DOT, ST>::run(vb_here(), g, L, code, zo, zc, rc, ro, cnt, cnt_o, col, zero_start, m, red);
 }
 
 // Coarse levels (natural layout, stored operator). A coupling is 0 across a
@@ -445,7 +540,7 @@ __device__ __forceinline__ void coarse_first_red(const MgLevel &C, TC *zc, uint3
     stv(zc, I, (red && C.active[I]) ? rnd(v, zc) * C.rdiag[I] : 0.f);
 }
 
-template <typename TF, typename TC, bool L0>
+template <typename TF, typename TC, bool L0, bool ST>
 struct KB_mg_restrict {
     static __device__ __forceinline__ void run(const VB &vb, const Geo &g, const MgLevel &F,
                                                const MgLevel &C, const TF *z, const TF *r, TC *rc,
@@ -465,6 +560,7 @@ struct KB_mg_restrict {
             uint16_t nb[kU];
             float rv[kU], zown[kU], zv[kU][6];
             bool in[kU];
+            Cof<ST> cf[kU];
 #pragma unroll
             for (int u = 0; u < kU; u++) {
                 const uint32_t t = t0 + u * kBlock < se ? t0 + u * kBlock : t0;
@@ -481,6 +577,7 @@ struct KB_mg_restrict {
                 nb[u] = g.nbm[c];
                 rv[u] = rs[own];
                 zown[u] = zs[own];
+                cf[u] = cof_at<ST>(g, c);
                 nb_gather(col ? bred : bblack, c, si, sj, zv[u]);
             }
             if (late(stop)) return;
@@ -490,7 +587,8 @@ struct KB_mg_restrict {
                 if (t >= se) break;
                 float v = 0.f;
                 if (in[u] && (nb[u] & NB_SELF))
-                    v = rv[u] - (diag_of(g, nb[u], dint) * zown[u] - nb_sum(g, nb[u], zv[u]));
+                    v = rv[u] - (diag_t<ST>(g, nb[u], dint, cf[u]) * zown[u] -
+                                 nb_sum_t<ST>(g, nb[u], zv[u], cf[u]));
                 v += __shfl_xor(v, 1);
                 v += __shfl_xor(v, 2);
                 v += __shfl_xor(v, 4);
@@ -535,12 +633,12 @@ struct KB_mg_restrict {
     }
 }
 };
-template <typename TF, typename TC, bool L0>
+template <typename TF, typename TC, bool L0, bool ST>
 __global__ void __launch_bounds__(kBlock) k_mg_restrict(Geo g, MgLevel F, MgLevel C, const TF *z,
                                                         const TF *r, TC *rc, TC *zc, int red0,
                                                         const MgState *m)
 {
-    KB_mg_restrict<TF, TC, L0>::run(vb_here(), g, F, C, z, r, rc, zc, red0, m);
+    KB_mg_restrict<TF, TC, L0, ST>::run(vb_here(), g, F, C, z, r, rc, zc, red0, m);
 }
 
 // ---- level-0 restriction in two passes --------------------------------------
@@ -549,6 +647,7 @@ __global__ void __launch_bounds__(kBlock) k_mg_restrict(Geo g, MgLevel F, MgLeve
 //     cell summing its up-to-8 children, whose split slots are contiguous
 //     across consecutive coarse cells. res lives in the inner PCG's q, which
 //     is not in use while the preconditioner runs.
+template <bool ST>
 struct KB_mg_resid0 {
     static __device__ __forceinline__ void run(const VB &vb, const Geo &g, const MgLevel &L,
                                                const float *__restrict__ z,
@@ -566,15 +665,18 @@ struct KB_mg_resid0 {
     for (uint32_t s0 = sb + threadIdx.x; s0 < se; s0 += kU2 * kBlock) {
         uint16_t nb[kU2];
         float rv[kU2], zs[kU2], zv[kU2][6];
+        Cof<ST> cf[kU2];
 #pragma unroll
         for (int t = 0; t < kU2; t++) {
             const uint32_t s = s0 + t * kBlock;
             const uint32_t col = s >= H ? 1u : 0u;
             const uint32_t h = s - (col ? H : 0u);
+            const uint32_t cn = 2 * h + split_parity(L, h, col);
             nb[t] = buf_ldu16(bcode, s);
             rv[t] = buf_ldf(br, s);
             zs[t] = buf_ldf(bz, s);
-            nb_gather(col ? bred : bblack, 2 * h + split_parity(L, h, col), si, sj, zv[t]);
+            cf[t] = cof_at<ST>(g, cn);
+            nb_gather(col ? bred : bblack, cn, si, sj, zv[t]);
         }
         if (late(stop)) return;
 #pragma unroll
@@ -583,17 +685,19 @@ struct KB_mg_resid0 {
             if (s >= se) break;
             float v = 0.f;
             if (nb[t] & NB_SELF)
-                v = rv[t] - (diag_of(g, nb[t], dint) * zs[t] - nb_sum(g, nb[t], zv[t]));
+                v = rv[t] - (diag_t<ST>(g, nb[t], dint, cf[t]) * zs[t] -
+                             nb_sum_t<ST>(g, nb[t], zv[t], cf[t]));
             buf_stf(bres, s, v);
         }
     }
 }
 };
+template <bool ST>
 __global__ void __launch_bounds__(kBlock) FFR_RESIDENT k_mg_resid0(Geo g, MgLevel L, const float *__restrict__ z,
                                                       const float *__restrict__ r,
                                                       float *__restrict__ res, const MgState *m)
 {
-    KB_mg_resid0::run(vb_here(), g, L, z, r, res, m);
+    KB_mg_resid0<ST>::run(vb_here(), g, L, z, r, res, m);
 }
 
 template <typename TC>
@@ -659,7 +763,7 @@ __device__ __forceinline__ uint32_t parent_of(const MgLevel &C, int i, int j, in
 }
 
 // level 0 (split layout): one thread per black slot, kU slots per thread
-template <typename TC>
+template <typename TC, bool ST>
 struct KB_mg_post_black0 {
     static __device__ __forceinline__ void run(const VB &vb, const Geo &g, const MgLevel &L,
                                                const MgLevel &C, const TC *__restrict__ zc,
@@ -679,12 +783,14 @@ struct KB_mg_post_black0 {
     for (uint32_t h0 = sb + threadIdx.x; h0 < se; h0 += kU2 * kBlock) {
         uint16_t nb[kU2];
         float rv[kU2], zv[kU2][6];
+        Cof<ST> cf[kU2];
 #pragma unroll
         for (int t = 0; t < kU2; t++) {
             const uint32_t h = h0 + t * kBlock;  // reads past cnt return 0
             nb[t] = buf_ldu16(bcode, h);
             rv[t] = buf_ldf(br, h);
             const uint32_t c = 2 * h + split_parity(L, h, 1u);
+            cf[t] = cof_at<ST>(g, c);
             const int i = (int)fdiv(c, g.f_nynz), rem = (int)(c - (uint32_t)i * si);
             const int j = (int)fdiv((uint32_t)rem, g.f_nz), k = rem - j * (int)sj;
             nb_gather(bred, c, si, sj, zv[t]);
@@ -701,18 +807,19 @@ struct KB_mg_post_black0 {
             const uint32_t h = h0 + t * kBlock;
             if (h >= se) break;
             float v = 0.f;
-            if (nb[t] & NB_SELF) v = (rv[t] + nb_sum(g, nb[t], zv[t])) / diag_of(g, nb[t], dint);
+            if (nb[t] & NB_SELF)
+                v = (rv[t] + nb_sum_t<ST>(g, nb[t], zv[t], cf[t])) / diag_t<ST>(g, nb[t], dint, cf[t]);
             buf_stf(bz, h, v);
         }
     }
 }
 };
-template <typename TC>
+template <typename TC, bool ST>
 __global__ void __launch_bounds__(kBlock) FFR_RESIDENT k_mg_post_black0(Geo g, MgLevel L, MgLevel C,
                                                            const TC *__restrict__ zc,
                                                            const MgState *m)
 {
-    KB_mg_post_black0<TC>::run(vb_here(), g, L, C, zc, m);
+    KB_mg_post_black0<TC, ST>::run(vb_here(), g, L, C, zc, m);
 }
 
 // coarse levels (natural layout)
@@ -762,7 +869,7 @@ __global__ void __launch_bounds__(kBlock) k_mg_post_black(MgLevel L, MgLevel C, 
 }
 
 // ---- coarsest level: symmetric red-black sweeps from z = 0 (one block) -----
-template <typename T, bool L0>
+template <typename T, bool L0, bool ST>
 struct KB_mg_coarse {
     static __device__ __forceinline__ void run(const VB &vb, const Geo &g, const MgLevel &L, T *z,
                                                const T *r, int sweeps, const MgState *m)
@@ -784,7 +891,21 @@ struct KB_mg_coarse {
                 if (!(nb & NB_SELF)) continue;
                 const float *zs = (const float *)z;
                 const uint32_t sc = sown(c, nb, L.H);
-                stv(z, sc, (ldv(r, sc) + off0s(g, zs + sother(nb, L.H), c, nb)) / diag0m(g, nb));
+                if constexpr (ST) {
+                    const Cof<ST> cf = cof_at<ST>(g, c);
+                    const float *xo = zs + sother(nb, L.H);
+                    const uint32_t si = (uint32_t)g.ny * g.nz, sj = g.nz;
+                    float v[6];  // values of faces that do not couple are dropped
+                    v[0] = (nb & (NB_UNK0 << 0)) ? xo[(c - si) >> 1] : 0.f;
+                    v[1] = (nb & (NB_UNK0 << 1)) ? xo[(c + si) >> 1] : 0.f;
+                    v[2] = (nb & (NB_UNK0 << 2)) ? xo[(c - sj) >> 1] : 0.f;
+                    v[3] = (nb & (NB_UNK0 << 3)) ? xo[(c + sj) >> 1] : 0.f;
+                    v[4] = (nb & (NB_UNK0 << 4)) ? xo[(c - 1) >> 1] : 0.f;
+                    v[5] = (nb & (NB_UNK0 << 5)) ? xo[(c + 1) >> 1] : 0.f;
+                    stv(z, sc, (ldv(r, sc) + nb_sum_t<ST>(g, nb, v, cf)) / diag_t<ST>(g, nb, 0.f, cf));
+                } else {
+                    stv(z, sc, (ldv(r, sc) + off0s(g, zs + sother(nb, L.H), c, nb)) / diag0m(g, nb));
+                }
             } else {
                 if (!L.active[c]) continue;
                 stv(z, c, (ldv(r, c) + rowc<T>(L, z, i, j, k, c)) * L.rdiag[c]);
@@ -794,11 +915,11 @@ struct KB_mg_coarse {
     }
 }
 };
-template <typename T, bool L0>
+template <typename T, bool L0, bool ST>
 __global__ void __launch_bounds__(1024) k_mg_coarse(Geo g, MgLevel L, T *z, const T *r, int sweeps,
                                                     const MgState *m)
 {
-    KB_mg_coarse<T, L0>::run(vb_here(), g, L, z, r, sweeps, m);
+    KB_mg_coarse<T, L0, ST>::run(vb_here(), g, L, z, r, sweeps, m);
 }
 
 // ---- the small levels in one workgroup (LDS-resident) --------------------------
@@ -1162,6 +1283,7 @@ __global__ void __launch_bounds__(kTailThreads) k_mg_tail(TailLevels tl, const M
 // FP32 rounding level over an inner solve's few iterations, and the outer
 // FP64 residual is recomputed from scratch every pass. Thread work pairs red
 // slot h with black slot H + h, so every block streams both colours.
+template <bool ST>
 struct KB_pcg_dirq {
     static __device__ __forceinline__ void run(const VB &vb, const Geo &g, const MgLevel &L,
                                                const float *__restrict__ z,
@@ -1192,15 +1314,18 @@ struct KB_pcg_dirq {
     for (uint32_t h0 = hb + threadIdx.x; h0 < he; h0 += kU2 * kBlock) {
         uint16_t nr[kU2], nk[kU2];
         float zrv[kU2], rv[kU2], prv[kU2], qrv[kU2], zbv[kU2], pbv[kU2], qbv[kU2], zn[kU2][6];
+        Cof<ST> cf[kU2];
 #pragma unroll
         for (int t = 0; t < kU2; t++) {
             const uint32_t h = h0 + t * kBlock;
+            const uint32_t cn = 2 * h + split_parity(L, h, 1u);
             nr[t] = buf_ldu16(cr, h);
             zrv[t] = buf_ldf(zr, h);
             rv[t] = buf_ldf(rr, h);
             nk[t] = buf_ldu16(cb, h);
             zbv[t] = buf_ldf(zb, h);
-            nb_gather(zr, 2 * h + split_parity(L, h, 1u), si, sj, zn[t]);
+            cf[t] = cof_at<ST>(g, cn);
+            nb_gather(zr, cn, si, sj, zn[t]);
             if (!first) {
                 prv[t] = buf_ldf(pr, h);
                 qrv[t] = buf_ldf(qr, h);
@@ -1213,9 +1338,9 @@ struct KB_pcg_dirq {
             const uint32_t h = h0 + t * kBlock;
             if (h >= he) break;
             const float azr = (nr[t] & NB_SELF) ? rv[t] : 0.f;
-            const float azb = (nk[t] & NB_SELF)
-                                  ? diag_of(g, nk[t], dint) * zbv[t] - nb_sum(g, nk[t], zn[t])
-                                  : 0.f;
+            const float azb = (nk[t] & NB_SELF) ? diag_t<ST>(g, nk[t], dint, cf[t]) * zbv[t] -
+                                                      nb_sum_t<ST>(g, nk[t], zn[t], cf[t])
+                                                : 0.f;
             float p0 = zrv[t], q0 = azr, p1 = zbv[t], q1 = azb;
             if (!first) {
                 p0 = __builtin_fmaf(beta, prv[t], p0);
@@ -1235,12 +1360,13 @@ struct KB_pcg_dirq {
     if (blk == 0 && threadIdx.x == 0) m->rz[it & 1] = rz_new;
 }
 };
+template <bool ST>
 __global__ void __launch_bounds__(kBlock) k_pcg_dirq(Geo g, MgLevel L, const float *__restrict__ z,
                                                      const float *__restrict__ r, float *p,
                                                      float *q, MgState *m, RedScratch red,
                                                      uint32_t gdot)
 {
-    KB_pcg_dirq::run(vb_here(), g, L, z, r, p, q, m, red, gdot);
+    KB_pcg_dirq<ST>::run(vb_here(), g, L, z, r, p, q, m, red, gdot);
 }
 
 // General form (level 0 is the tail or the coarsest level, small grids):
@@ -1271,6 +1397,7 @@ __global__ void __launch_bounds__(kBlock) k_pcg_dir(uint32_t n, const float *z, 
 
 // q = A p, partials of p.q (split layout: slot s of colour s >= H); loads as
 // in k_mg_smooth0
+template <bool ST>
 struct KB_pcg_spmv {
     static __device__ __forceinline__ void run(const VB &vb, const Geo &g, const MgLevel &L,
                                                const float *__restrict__ p, float *__restrict__ q,
@@ -1290,14 +1417,17 @@ struct KB_pcg_spmv {
     for (uint32_t s0 = sb + threadIdx.x; s0 < se; s0 += kU * kBlock) {
         uint16_t nb[kU];
         float pv[kU], zv[kU][6];
+        Cof<ST> cf[kU];
 #pragma unroll
         for (int t = 0; t < kU; t++) {
             const uint32_t s = s0 + t * kBlock;
             const uint32_t col = s >= H ? 1u : 0u;
             const uint32_t h = s - (col ? H : 0u);
+            const uint32_t cn = 2 * h + split_parity(L, h, col);
             nb[t] = buf_ldu16(bcode, s);
             pv[t] = buf_ldf(bp, s);
-            nb_gather(col ? bred : bblack, 2 * h + split_parity(L, h, col), si, sj, zv[t]);
+            cf[t] = cof_at<ST>(g, cn);
+            nb_gather(col ? bred : bblack, cn, si, sj, zv[t]);
         }
         if (late(stop)) return;  // uniform: every thread of the grid returns
 #pragma unroll
@@ -1306,7 +1436,7 @@ struct KB_pcg_spmv {
             if (s >= se) break;
             float qv = 0.f;
             if (nb[t] & NB_SELF) {
-                qv = diag_of(g, nb[t], dint) * pv[t] - nb_sum(g, nb[t], zv[t]);
+                qv = diag_t<ST>(g, nb[t], dint, cf[t]) * pv[t] - nb_sum_t<ST>(g, nb[t], zv[t], cf[t]);
                 acc += (double)pv[t] * (double)qv;
             }
             buf_stf(bq, s, qv);
@@ -1315,11 +1445,12 @@ struct KB_pcg_spmv {
     block_partial(acc, pslot(red, SLOT_SPMV), blk, lds);
 }
 };
+template <bool ST>
 __global__ void __launch_bounds__(kBlock) k_pcg_spmv(Geo g, MgLevel L, const float *__restrict__ p,
                                                      float *__restrict__ q, const MgState *m,
                                                      RedScratch red)
 {
-    KB_pcg_spmv::run(vb_here(), g, L, p, q, m, red);
+    KB_pcg_spmv<ST>::run(vb_here(), g, L, p, q, m, red);
 }
 
 // alpha = rz / p.q; e += alpha p (e = alpha p on an inner solve's first
@@ -1328,6 +1459,7 @@ __global__ void __launch_bounds__(kBlock) k_pcg_spmv(Geo g, MgLevel L, const flo
 // The V-cycle's first pass, the red sweep from z = 0 on level 0, is done here:
 // z = r / diag on red unknowns (red slots [0, H) of the split layout), 0 on
 // other red slots; the V-cycle then starts with the black sweep.
+template <bool ST>
 struct KB_pcg_update {
     static __device__ __forceinline__ void run(const VB &vb, const Geo &g, const MgLevel &L,
                                                float *e, float *r, const float *p, const float *q,
@@ -1358,17 +1490,19 @@ struct KB_pcg_update {
         acc += (double)rv * (double)rv;
         if (c < L.H) {  // red slot
             const uint16_t nb = L.nbs[c];
-            z[c] = (nb & NB_SELF) ? rv / diag_of(g, nb, dint) : 0.f;
+            const Cof<ST> cf = cof_at<ST>(g, 2 * c + split_parity(L, c, 0u));
+            z[c] = (nb & NB_SELF) ? rv / diag_t<ST>(g, nb, dint, cf) : 0.f;
         }
     }
     block_partial(acc, pslot(red, SLOT_UPDATE), blk, lds);
 }
 };
+template <bool ST>
 __global__ void __launch_bounds__(kBlock) k_pcg_update(Geo g, MgLevel L, float *e, float *r,
                                                        const float *p, const float *q, float *z,
                                                        MgState *m, RedScratch red, uint32_t gq)
 {
-    KB_pcg_update::run(vb_here(), g, L, e, r, p, q, z, m, red, gq);
+    KB_pcg_update<ST>::run(vb_here(), g, L, e, r, p, q, z, m, red, gq);
 }
 
 // stopping test of the inner solve (one block): its relative target, or the
@@ -1480,31 +1614,48 @@ __global__ void __launch_bounds__(kBlock) k_dc_start(MgState *m, RedScratch red,
     KB_dc_start::run(vb_here(), m, red, G, outer_tol, max_outer, max_inner, inner_tol, total_cap);
 }
 
-// FP64 reference row (PoissonSolver.cpp:52-87) on a uniform grid from the
-// neighbour code; neighbour values come from `val` (x + e for the outer
-// residual, x for the OUTFLOW back substitution).
-template <typename F>
+// FP64 reference row (PoissonSolver.cpp:52-87) from the neighbour code, with
+// one scale per axis on a uniform grid or the reference's per-face scales
+// (PoissonSolver.cpp:34-49, Geo.lap) on a stretched one (ST); neighbour
+// values come from `val` (x + e for the outer residual, x for the OUTFLOW
+// back substitution).
+template <bool ST, typename F>
 __device__ __forceinline__ void row64(const Geo &g, uint32_t c, uint16_t m, F val, double &diag,
                                       double &off)
 {
     const uint32_t si = (uint32_t)g.ny * g.nz, sj = g.nz;
+    double sxl, sxr, syl, syr, szl, szr;
+    if constexpr (ST) {
+        const uint32_t i = fdiv(c, g.f_nynz), rem = c - i * si;
+        const uint32_t j = fdiv(rem, g.f_nz), k = rem - j * sj;
+        const double *L = g.lap;
+        sxl = L[i]; sxr = L[g.nx + i];
+        syl = L[2 * g.nx + j]; syr = L[2 * g.nx + g.ny + j];
+        szl = L[2 * g.nx + 2 * g.ny + k]; szr = L[2 * g.nx + 2 * g.ny + g.nz + k];
+    } else {
+        sxl = sxr = g.sxd;
+        syl = syr = g.syd;
+        szl = szr = g.szd;
+    }
     double dg = 0.0, of = 0.0;
 #define FFR_FACE(q, nc, sc)                                                   \
     if (m & (NB_UNK0 << q)) { dg += (sc); of += (sc) * val(nc); }            \
     else if (m & (NB_OUT0 << q)) dg += 2.0 * (sc);
-    FFR_FACE(0, c - si, g.sxd)
-    FFR_FACE(1, c + si, g.sxd)
-    FFR_FACE(2, c - sj, g.syd)
-    FFR_FACE(3, c + sj, g.syd)
-    FFR_FACE(4, c - 1, g.szd)
-    FFR_FACE(5, c + 1, g.szd)
+    FFR_FACE(0, c - si, sxl)
+    FFR_FACE(1, c + si, sxr)
+    FFR_FACE(2, c - sj, syl)
+    FFR_FACE(3, c + sj, syr)
+    FFR_FACE(4, c - 1, szl)
+    FFR_FACE(5, c + 1, szr)
 #undef FFR_FACE
     diag = dg < 1e-30 ? 1.0 : dg;
     off = of;
 }
 
-// x_out = x_in + e on unknowns; r = b - A x_out (FP64); r32 = (float) r;
-// partials of ||r||^2
+// x_out = x_in + e on unknowns; r = b - A x_out (FP64); r32 = (float) r,
+// weighted by the cell's relative volume on a stretched grid (the inner
+// solve's symmetric form); partials of ||r||^2
+template <bool ST>
 struct KB_dc_residual {
     static __device__ __forceinline__ void run(const VB &vb, const Geo &g,
                                                const double *__restrict__ b, double *xa, double *xb,
@@ -1534,18 +1685,28 @@ struct KB_dc_residual {
         }
         const float *eo = e + sother(nb, H);
         double diag, off;
-        row64(g, c, nb, [&](uint32_t q) { return xin[q] + (double)eo[q >> 1]; }, diag, off);
+        row64<ST>(g, c, nb, [&](uint32_t q) { return xin[q] + (double)eo[q >> 1]; }, diag, off);
         const double xc = xin[c] + (double)e[sc];
         const double rv = b[c] - (diag * xc - off);
         xout[c] = xc;
-        const float rf = (float)rv;
-        r32[sc] = rf;
-        if (red_slot) z0[sc] = rf / diag0m(g, nb);  // the V-cycle's first (red, z = 0) sweep
+        if constexpr (ST) {
+            const uint32_t i = fdiv(c, g.f_nynz), rem = c - i * (uint32_t)(g.ny * g.nz);
+            const uint32_t j = fdiv(rem, g.f_nz), k = rem - j * (uint32_t)g.nz;
+            const double vol = (double)g.mgw[i] * (double)g.mgw[g.nx + j] * (double)g.mgw[g.nx + g.ny + k];
+            const float rf = (float)(rv * vol);
+            r32[sc] = rf;
+            if (red_slot) z0[sc] = rf / diag_t<ST>(g, nb, 0.f, cof_at<ST>(g, c));
+        } else {
+            const float rf = (float)rv;
+            r32[sc] = rf;
+            if (red_slot) z0[sc] = rf / diag0m(g, nb);  // the V-cycle's first (red, z = 0) sweep
+        }
         acc += rv * rv;
     }
     block_partial(acc, pslot(red, SLOT_UPDATE), blk, lds);
 }
 };
+template <bool ST>
 __global__ void __launch_bounds__(kBlock) k_dc_residual(Geo g, const double *__restrict__ b,
                                                         double *xa, double *xb,
                                                         const float *__restrict__ e,
@@ -1553,7 +1714,7 @@ __global__ void __launch_bounds__(kBlock) k_dc_residual(Geo g, const double *__r
                                                         float *__restrict__ z0, const MgState *m,
                                                         RedScratch red)
 {
-    KB_dc_residual::run(vb_here(), g, b, xa, xb, e, r32, z0, m, red);
+    KB_dc_residual<ST>::run(vb_here(), g, b, xa, xb, e, r32, z0, m, red);
 }
 
 // outer stopping test: ||r|| / ||b|| < tol (one block)
@@ -1588,6 +1749,7 @@ __global__ void __launch_bounds__(kBlock) k_dc_check(MgState *m, RedScratch red,
 
 // p: unknowns from x; OUTFLOW rows solved from their neighbours (or pinned
 // to 0 for the reference multigrid's convention); SOLID cells 0.
+template <bool ST>
 struct KB_dc_finalize {
     static __device__ __forceinline__ void run(const VB &vb, const Geo &g,
                                                const double *__restrict__ b, const double *xa,
@@ -1608,26 +1770,29 @@ struct KB_dc_finalize {
             continue;
         }
         double diag, off;
-        row64(g, c, nb, [&](uint32_t q) { return x[q]; }, diag, off);
+        row64<ST>(g, c, nb, [&](uint32_t q) { return x[q]; }, diag, off);
         p[c] = (b[c] + off) / diag;
     }
 }
 };
+template <bool ST>
 __global__ void __launch_bounds__(kBlock) k_dc_finalize(Geo g, const double *__restrict__ b,
                                                         const double *xa, const double *xb,
                                                         double *p, int pin_outflow,
                                                         const MgState *m)
 {
-    KB_dc_finalize::run(vb_here(), g, b, xa, xb, p, pin_outflow, m);
+    KB_dc_finalize<ST>::run(vb_here(), g, b, xa, xb, p, pin_outflow, m);
 }
 
 // ---- launchers ------------------------------------------------------------------
 void mg_launch_build(hipStream_t s, const Geo &g, const MgLevel &F, const MgLevel &C, bool fine0)
 {
-    if (fine0)
-        FFR_LAUNCH(k_mg_build<true>, dim3(grid_for(C.n)), dim3(kBlock), 0, s, g, F, C);
+    if (fine0 && !g.uniform)
+        FFR_LAUNCH((k_mg_build<true, true>), dim3(grid_for(C.n)), dim3(kBlock), 0, s, g, F, C);
+    else if (fine0)
+        FFR_LAUNCH((k_mg_build<true, false>), dim3(grid_for(C.n)), dim3(kBlock), 0, s, g, F, C);
     else
-        FFR_LAUNCH(k_mg_build<false>, dim3(grid_for(C.n)), dim3(kBlock), 0, s, g, F, C);
+        FFR_LAUNCH((k_mg_build<false, false>), dim3(grid_for(C.n)), dim3(kBlock), 0, s, g, F, C);
 }
 
 void mg_launch_split_codes(hipStream_t s, const Geo &g, const MgLevel &L0)
@@ -1635,27 +1800,54 @@ void mg_launch_split_codes(hipStream_t s, const Geo &g, const MgLevel &L0)
     FFR_LAUNCH(k_mg_split_codes, dim3(grid_for(g.nc)), dim3(kBlock), 0, s, g, L0.nbs, L0.H);
 }
 
+// level-0 launches on uniform (ST false) or stretched (ST true) spacing
+template <bool ST>
+void smooth0_launch(hipStream_t s, const Geo &g, const MgLevel &L, int color, int zero_start,
+                    const MgState *m, const RedScratch *dot)
+{
+    // level 0 is FP32 in the split layout
+    const uint32_t H = L.H, cnt = color ? g.nc - H : H, base = color ? H : 0u;
+    const uint32_t obase = color ? 0u : H;
+    float *z = (float *)L.z;
+    const float *r = (const float *)L.r;
+    if (dot && color == 0)
+        FFR_GANG((KB_mg_smooth0<true, ST>), GB_RES, (k_mg_smooth0<true, ST>), dim3(grid_resident(cnt)),
+                 dim3(kBlock), 0, s, g, L, L.nbs + base, z + obase, z + base, r + base, r + obase,
+                 cnt, g.nc - cnt, (uint32_t)color, zero_start, m, *dot);
+    else
+        FFR_GANG((KB_mg_smooth0<false, ST>), GB_RES, (k_mg_smooth0<false, ST>),
+                 dim3(grid_resident(cnt)), dim3(kBlock), 0, s, g, L, L.nbs + base, z + obase,
+                 z + base, r + base, r + obase, cnt, g.nc - cnt, (uint32_t)color, zero_start, m,
+                 RedScratch{});
+}
+
 template <typename T>
 void mg_launch_smooth(hipStream_t s, const Geo &g, const MgLevel &L, bool l0, int color,
                       int zero_start, const MgState *m, const RedScratch *dot)
 {
-    if (l0) {  // level 0 is FP32 in the split layout
-        const uint32_t H = L.H, cnt = color ? g.nc - H : H, base = color ? H : 0u;
-        const uint32_t obase = color ? 0u : H;
-        float *z = (float *)L.z;
-        const float *r = (const float *)L.r;
-        if (dot && color == 0)
-            FFR_GANG((KB_mg_smooth0<true>), GB_RES, k_mg_smooth0<true>, dim3(grid_resident(cnt)),
-                     dim3(kBlock), 0, s, g, L, L.nbs + base, z + obase, z + base, r + base,
-                     r + obase, cnt, g.nc - cnt, (uint32_t)color, zero_start, m, *dot);
-        else
-            FFR_GANG((KB_mg_smooth0<false>), GB_RES, k_mg_smooth0<false>, dim3(grid_resident(cnt)),
-                     dim3(kBlock), 0, s, g, L, L.nbs + base, z + obase, z + base, r + base,
-                     r + obase, cnt, g.nc - cnt, (uint32_t)color, zero_start, m, RedScratch{});
-    }
+    if (l0 && g.uniform)
+        smooth0_launch<false>(s, g, L, color, zero_start, m, dot);
+    else if (l0)
+        smooth0_launch<true>(s, g, L, color, zero_start, m, dot);
     else
         FFR_GANG((KB_mg_smooth<T>), GB_PLAIN, k_mg_smooth<T>, dim3(grid_for(L.n)), dim3(kBlock), 0,
                  s, L, (T *)L.z, (const T *)L.r, color, zero_start, m);
+}
+
+template <bool ST>
+void resid0_launch(hipStream_t s, const Geo &g, const MgLevel &L0, float *res, const MgState *m)
+{
+    FFR_GANG((KB_mg_resid0<ST>), GB_RES, (k_mg_resid0<ST>), dim3(grid_resident(g.nc)), dim3(kBlock),
+             0, s, g, L0, (const float *)L0.z, (const float *)L0.r, res, m);
+}
+
+template <typename TF, typename TC, bool ST>
+void restrict0_launch(hipStream_t s, const Geo &g, const MgLevel &F, const MgLevel &C, bool red0,
+                      const MgState *m)
+{
+    FFR_GANG((KB_mg_restrict<TF, TC, true, ST>), GB_PLAIN, (k_mg_restrict<TF, TC, true, ST>),
+             dim3(grid_for(8ull * C.n)), dim3(kBlock), 0, s, g, F, C, (const TF *)F.z,
+             (const TF *)F.r, (TC *)C.r, (TC *)C.z, (int)red0, m);
 }
 
 template <typename TF, typename TC>
@@ -1663,33 +1855,37 @@ void mg_launch_restrict(hipStream_t s, const Geo &g, const MgLevel &F, const MgL
                         bool red0, const MgState *m, float *scratch)
 {
     if (l0 && scratch) {  // two passes through the scratch residual
-        FFR_GANG((KB_mg_resid0), GB_RES, k_mg_resid0, dim3(grid_resident(g.nc)), dim3(kBlock), 0, s,
-                 g, F, (const float *)F.z, (const float *)F.r, scratch, m);
+        mg_launch_resid0(s, g, F, scratch, m);
         FFR_GANG((KB_mg_restrict_sum0<TC>), GB_PLAIN, (k_mg_restrict_sum0<TC>), dim3(grid_for(C.n)),
                  dim3(kBlock), 0, s, g, F, C, (const float *)scratch, (TC *)C.r, (TC *)C.z,
                  (int)red0, m);
-    } else if (l0)
-        FFR_GANG((KB_mg_restrict<TF, TC, true>), GB_PLAIN, (k_mg_restrict<TF, TC, true>),
-                 dim3(grid_for(8ull * C.n)), dim3(kBlock), 0, s, g, F, C, (const TF *)F.z,
-                 (const TF *)F.r, (TC *)C.r, (TC *)C.z, (int)red0, m);
+    } else if (l0 && g.uniform)
+        restrict0_launch<TF, TC, false>(s, g, F, C, red0, m);
+    else if (l0)
+        restrict0_launch<TF, TC, true>(s, g, F, C, red0, m);
     else
-        FFR_GANG((KB_mg_restrict<TF, TC, false>), GB_PLAIN, (k_mg_restrict<TF, TC, false>),
+        FFR_GANG((KB_mg_restrict<TF, TC, false, false>), GB_PLAIN, (k_mg_restrict<TF, TC, false, false>),
                  dim3(grid_for(8ull * C.n)), dim3(kBlock), 0, s, g, F, C, (const TF *)F.z,
                  (const TF *)F.r, (TC *)C.r, (TC *)C.z, (int)red0, m);
 }
 
 void mg_launch_resid0(hipStream_t s, const Geo &g, const MgLevel &L0, float *res, const MgState *m)
 {
-    FFR_GANG((KB_mg_resid0), GB_RES, k_mg_resid0, dim3(grid_resident(g.nc)), dim3(kBlock), 0, s, g,
-             L0, (const float *)L0.z, (const float *)L0.r, res, m);
+    if (g.uniform)
+        resid0_launch<false>(s, g, L0, res, m);
+    else
+        resid0_launch<true>(s, g, L0, res, m);
 }
 
 template <typename TF, typename TC>
 void mg_launch_post_black(hipStream_t s, const Geo &g, const MgLevel &F, const MgLevel &C, bool l0,
                           const MgState *m)
 {
-    if (l0)
-        FFR_GANG((KB_mg_post_black0<TC>), GB_RES, (k_mg_post_black0<TC>),
+    if (l0 && g.uniform)
+        FFR_GANG((KB_mg_post_black0<TC, false>), GB_RES, (k_mg_post_black0<TC, false>),
+                 dim3(grid_resident(g.nc - F.H)), dim3(kBlock), 0, s, g, F, C, (const TC *)C.z, m);
+    else if (l0)
+        FFR_GANG((KB_mg_post_black0<TC, true>), GB_RES, (k_mg_post_black0<TC, true>),
                  dim3(grid_resident(g.nc - F.H)), dim3(kBlock), 0, s, g, F, C, (const TC *)C.z, m);
     else
         FFR_GANG((KB_mg_post_black<TF, TC>), GB_PLAIN, (k_mg_post_black<TF, TC>),
@@ -1728,12 +1924,15 @@ template <typename T>
 void mg_launch_coarse(hipStream_t s, const Geo &g, const MgLevel &L, bool l0, int sweeps,
                       const MgState *m)
 {
-    if (l0)
-        FFR_GANG((KB_mg_coarse<T, true>), GB_WIDE, (k_mg_coarse<T, true>), dim3(1), dim3(1024), 0,
-                 s, g, L, (T *)L.z, (const T *)L.r, sweeps, m);
+    if (l0 && g.uniform)
+        FFR_GANG((KB_mg_coarse<T, true, false>), GB_WIDE, (k_mg_coarse<T, true, false>), dim3(1),
+                 dim3(1024), 0, s, g, L, (T *)L.z, (const T *)L.r, sweeps, m);
+    else if (l0)
+        FFR_GANG((KB_mg_coarse<T, true, true>), GB_WIDE, (k_mg_coarse<T, true, true>), dim3(1),
+                 dim3(1024), 0, s, g, L, (T *)L.z, (const T *)L.r, sweeps, m);
     else
-        FFR_GANG((KB_mg_coarse<T, false>), GB_WIDE, (k_mg_coarse<T, false>), dim3(1), dim3(1024), 0,
-                 s, g, L, (T *)L.z, (const T *)L.r, sweeps, m);
+        FFR_GANG((KB_mg_coarse<T, false, false>), GB_WIDE, (k_mg_coarse<T, false, false>), dim3(1),
+                 dim3(1024), 0, s, g, L, (T *)L.z, (const T *)L.r, sweeps, m);
 }
 
 template void mg_launch_smooth<float>(hipStream_t, const Geo &, const MgLevel &, bool, int, int, const MgState *, const RedScratch *);
@@ -1752,8 +1951,12 @@ template void mg_launch_coarse<__hip_bfloat16>(hipStream_t, const Geo &, const M
 void pcg_launch_dirq(hipStream_t s, const Geo &g, const MgLevel &L0, const float *r, float *p,
                      float *q, MgState *m, RedScratch red, uint32_t gdot)
 {
-    FFR_GANG((KB_pcg_dirq), GB_PLAIN, k_pcg_dirq, dim3(grid_resident(L0.H)), dim3(kBlock), 0, s, g,
-             L0, (const float *)L0.z, r, p, q, m, red, gdot);
+    if (g.uniform)
+        FFR_GANG((KB_pcg_dirq<false>), GB_PLAIN, (k_pcg_dirq<false>), dim3(grid_resident(L0.H)),
+                 dim3(kBlock), 0, s, g, L0, (const float *)L0.z, r, p, q, m, red, gdot);
+    else
+        FFR_GANG((KB_pcg_dirq<true>), GB_PLAIN, (k_pcg_dirq<true>), dim3(grid_resident(L0.H)),
+                 dim3(kBlock), 0, s, g, L0, (const float *)L0.z, r, p, q, m, red, gdot);
 }
 void pcg_launch_dir(hipStream_t s, uint32_t n, const float *z, float *p, MgState *m, RedScratch red,
                     uint32_t gdot)
@@ -1764,14 +1967,22 @@ void pcg_launch_dir(hipStream_t s, uint32_t n, const float *z, float *p, MgState
 void pcg_launch_spmv(hipStream_t s, const Geo &g, const MgLevel &L0, const float *p, float *q,
                      const MgState *m, RedScratch red)
 {
-    FFR_GANG((KB_pcg_spmv), GB_PLAIN, k_pcg_spmv, dim3(grid_for(g.nc)), dim3(kBlock), 0, s, g, L0,
-             p, q, m, red);
+    if (g.uniform)
+        FFR_GANG((KB_pcg_spmv<false>), GB_PLAIN, (k_pcg_spmv<false>), dim3(grid_for(g.nc)),
+                 dim3(kBlock), 0, s, g, L0, p, q, m, red);
+    else
+        FFR_GANG((KB_pcg_spmv<true>), GB_PLAIN, (k_pcg_spmv<true>), dim3(grid_for(g.nc)),
+                 dim3(kBlock), 0, s, g, L0, p, q, m, red);
 }
 void pcg_launch_update(hipStream_t s, const Geo &g, const MgLevel &L0, float *e, float *r,
                        const float *p, const float *q, MgState *m, RedScratch red, uint32_t gq)
 {
-    FFR_GANG((KB_pcg_update), GB_PLAIN, k_pcg_update, dim3(grid_for(g.nc)), dim3(kBlock), 0, s, g,
-             L0, e, r, p, q, (float *)L0.z, m, red, gq);
+    if (g.uniform)
+        FFR_GANG((KB_pcg_update<false>), GB_PLAIN, (k_pcg_update<false>), dim3(grid_for(g.nc)),
+                 dim3(kBlock), 0, s, g, L0, e, r, p, q, (float *)L0.z, m, red, gq);
+    else
+        FFR_GANG((KB_pcg_update<true>), GB_PLAIN, (k_pcg_update<true>), dim3(grid_for(g.nc)),
+                 dim3(kBlock), 0, s, g, L0, e, r, p, q, (float *)L0.z, m, red, gq);
 }
 void pcg_launch_check(hipStream_t s, uint32_t n, MgState *m, RedScratch red)
 {
@@ -1798,8 +2009,12 @@ void dc_launch_start(hipStream_t s, const Geo &g, MgState *m, RedScratch red, do
 void dc_launch_residual(hipStream_t s, const Geo &g, const double *b, double *xa, double *xb,
                         const float *e, float *r32, float *z0, const MgState *m, RedScratch red)
 {
-    FFR_GANG((KB_dc_residual), GB_PLAIN, k_dc_residual, dim3(grid_for(g.nc)), dim3(kBlock), 0, s, g,
-             b, xa, xb, e, r32, z0, m, red);
+    if (g.uniform)
+        FFR_GANG((KB_dc_residual<false>), GB_PLAIN, (k_dc_residual<false>), dim3(grid_for(g.nc)),
+                 dim3(kBlock), 0, s, g, b, xa, xb, e, r32, z0, m, red);
+    else
+        FFR_GANG((KB_dc_residual<true>), GB_PLAIN, (k_dc_residual<true>), dim3(grid_for(g.nc)),
+                 dim3(kBlock), 0, s, g, b, xa, xb, e, r32, z0, m, red);
 }
 void dc_launch_check(hipStream_t s, const Geo &g, MgState *m, RedScratch red)
 {
@@ -1809,8 +2024,12 @@ void dc_launch_check(hipStream_t s, const Geo &g, MgState *m, RedScratch red)
 void dc_launch_finalize(hipStream_t s, const Geo &g, const double *b, const double *xa,
                         const double *xb, double *p, int pin_outflow, const MgState *m)
 {
-    FFR_GANG((KB_dc_finalize), GB_PLAIN, k_dc_finalize, dim3(grid_for(g.nc)), dim3(kBlock), 0, s, g,
-             b, xa, xb, p, pin_outflow, m);
+    if (g.uniform)
+        FFR_GANG((KB_dc_finalize<false>), GB_PLAIN, (k_dc_finalize<false>), dim3(grid_for(g.nc)),
+                 dim3(kBlock), 0, s, g, b, xa, xb, p, pin_outflow, m);
+    else
+        FFR_GANG((KB_dc_finalize<true>), GB_PLAIN, (k_dc_finalize<true>), dim3(grid_for(g.nc)),
+                 dim3(kBlock), 0, s, g, b, xa, xb, p, pin_outflow, m);
 }
 
 }  // namespace ffr

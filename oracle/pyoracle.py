@@ -42,8 +42,14 @@ class SceneArrays:
     fans: List[Tuple] = field(default_factory=list)      # (i0,i1,j0,j1,k0,k1,vx,vy,vz)
     outflows: List[Tuple] = field(default_factory=list)  # plain patches (axis,side,a0,a1,b0,b1)
     openings: List[Tuple] = field(default_factory=list)  # (axis,side,a0,a1,b0,b1,T_out)
+    # face coordinates (xs, ys, zs) of a stretched grid, sizes n+1 per axis
+    # (Grid::set_face_coords); None: uniform
+    faces: Optional[Tuple[np.ndarray, np.ndarray, np.ndarray]] = None
 
     def spacings(self):
+        if self.faces is not None:  # Grid.cpp:44-46: d[i] = face[i+1] - face[i]
+            return tuple(np.asarray(f, dtype=np.float64)[1:] - np.asarray(f, dtype=np.float64)[:-1]
+                         for f in self.faces)
         # Grid.cpp:6-37: uniform spacings lx/nx etc.
         dx, dy, dz = self.lx / self.nx, self.ly / self.ny, self.lz / self.nz
         return (np.full(self.nx, dx), np.full(self.ny, dy), np.full(self.nz, dz))
@@ -303,6 +309,7 @@ def ref_lib():
         lib.ref_get_flags.argtypes = [C.c_void_p, _u8p]
         lib.ref_set_flags.argtypes = [C.c_void_p, _u8p]
         lib.ref_make_solver.argtypes = [C.c_void_p, C.POINTER(_RefParams)]
+        lib.ref_set_face_coords.argtypes = [C.c_void_p, _dp, _dp, _dp]
         lib.ref_step.argtypes = [C.c_void_p, C.POINTER(_RefResult)]
         lib.ref_run.argtypes = [C.c_void_p, C.POINTER(_RefResult)]
         lib.ref_get_fields.argtypes = [C.c_void_p] + [_dp] * 5
@@ -327,6 +334,9 @@ class RefSolver:
         self.lib = ref_lib()
         self.scene = scene
         self.h = self.lib.ref_create(scene.nx, scene.ny, scene.nz, scene.lx, scene.ly, scene.lz)
+        if scene.faces is not None:
+            self._faces = [np.ascontiguousarray(f, dtype=np.float64) for f in scene.faces]
+            self.lib.ref_set_face_coords(self.h, *[_ptr(f) for f in self._faces])
         self._bc(scene)
         flags = scene.flags if flags_override is None else flags_override
         f = np.ascontiguousarray(flags, dtype=np.uint8)

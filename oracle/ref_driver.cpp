@@ -114,6 +114,16 @@ void ref_set_flags(void *hp, const unsigned char *in)
         h->grid->cell_type[i] = static_cast<CellType>(in[i]);
 }
 
+// Grid::set_face_coords (Grid.cpp:39-47): xs, ys, zs of sizes nx+1, ny+1, nz+1
+void ref_set_face_coords(void *hp, const double *xs, const double *ys, const double *zs)
+{
+    auto *h = static_cast<RefHandle *>(hp);
+    const Grid &g = *h->grid;
+    h->grid->set_face_coords(std::vector<double>(xs, xs + g.Nx + 1),
+                             std::vector<double>(ys, ys + g.Ny + 1),
+                             std::vector<double>(zs, zs + g.Nz + 1));
+}
+
 void ref_make_solver(void *hp, const ref_params *p)
 {
     auto *h = static_cast<RefHandle *>(hp);

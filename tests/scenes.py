@@ -61,3 +61,15 @@ def variant(cfg: dict, **solver) -> dict:
     c = copy.deepcopy(cfg)
     c.setdefault("solver", {}).update(solver)
     return c
+
+
+def stretched(sc, ratio: float):
+    """The scene on a stretched grid: Grid::set_face_coords faces of the given
+    geometric ratio on every axis (reference scene.py:34-59, 82-87). The flags
+    stay as they are: the reference maps positions to cells with
+    int(x / L * N) whatever the faces."""
+    import numpy as np
+    from ff_room.scene import _stretched_faces
+    sc.faces = tuple(np.array(_stretched_faces(n, L, ratio), dtype=np.float64)
+                     for n, L in ((sc.nx, sc.lx), (sc.ny, sc.ly), (sc.nz, sc.lz)))
+    return sc

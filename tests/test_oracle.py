@@ -20,7 +20,7 @@ from conftest import GOLDEN
 from oracle.pyoracle import (INFLOW, OUTFLOW, SOLID, OracleSolver, RefSolver, SceneArrays,
                              ref_available)
 from oracle.scene_flags import build_scene
-from scenes import BASIC_ROOM, SMALL_LW, SMALL_THERMAL, solver_params, variant
+from scenes import BASIC_ROOM, SMALL_LW, SMALL_THERMAL, solver_params, stretched, variant
 
 
 def _golden(name):
@@ -206,6 +206,71 @@ def test_oracle_against_compiled_reference(cfg, extra, steps):
     fo, fr = o.fields(), r.fields()
     for k in fo:  # the reference multigrid diverges to NaN on basic_room; so must the oracle
         assert np.array_equal(fo[k], fr[k], equal_nan=True), k
+
+
+@pytest.mark.skipif(not ref_available(), reason="oracle/_ref not built (reference tree absent)")
+@pytest.mark.parametrize("cfg,ratio,extra,steps", [
+    (BASIC_ROOM, 1.08, dict(), 10),
+    (BASIC_ROOM, 1.15, dict(advection=1), 6),
+    (SMALL_THERMAL, 1.2, dict(advection=2), 10),
+    (SMALL_LW, 1.1, dict(advection=1, thermal=1, buoyancy=1), 8),
+])
+def test_oracle_against_compiled_reference_stretched(cfg, ratio, extra, steps):
+    """Stretched grids (Grid::set_face_coords): every variable-spacing term of
+    the step and the non-symmetric pressure operator, bit for bit."""
+    sc = stretched(build_scene(cfg), ratio)
+    dxv = sc.spacings()[0]
+    assert dxv.max() / dxv.min() > 1.2  # the spacing really varies
+    params = solver_params(cfg)
+    params.update(extra, convergence_tol=0.0)
+    o, r = OracleSolver(sc, params), RefSolver(sc, params)
+    for _ in range(steps):
+        a, b = o.step(), r.step()
+        assert all(a[k] == b[k] for k in b), (a, b)
+    fo, fr = o.fields(), r.fields()
+    for k in fo:
+        assert np.array_equal(fo[k], fr[k]), k
+
+
+def test_volume_weighted_operator_is_symmetric():
+    """On a stretched grid the reference operator is not symmetric (its face
+    scale 1/(h_f d_c) depends on the cell's own width); scaled row by row by
+    the cell volume relative to the mean cell's, its unknown block is, and a
+    face's coupling is t w w: t = 1/(centre distance x mean width) of the face
+    axis, w = width / mean width of the two other axes. The multigrid's inner
+    solve runs on exactly this form."""
+    rng = np.random.default_rng(7)
+    nx, ny, nz = 9, 7, 6
+    f = rng.choice([0] * 8 + [SOLID, INFLOW], size=(nx, ny, nz)).astype(np.uint8)
+    f[:, -1, :] = OUTFLOW
+    sc = stretched(SceneArrays(nx, ny, nz, 1.8, 1.4, 1.2, f.reshape(-1)), 1.25)
+    o = OracleSolver(sc, {}, init=False)
+    n = nx * ny * nz
+    A = np.stack([o.laplacian(np.eye(n)[c]) for c in range(n)], axis=1)
+    d = sc.spacings()
+    w = [h / h.mean() for h in d]
+    t = []
+    for h in d:
+        ta = np.zeros(len(h) + 1)
+        ta[1:-1] = 1.0 / (0.5 * (h[:-1] + h[1:]) * h.mean())
+        t.append(ta)
+    vol = (w[0][:, None, None] * w[1][None, :, None] * w[2][None, None, :]).reshape(-1)
+    unk = np.isin(f.reshape(-1), [0, INFLOW])
+    Au = A[np.ix_(unk, unk)]
+    assert np.abs(Au - Au.T).max() > 1.0  # the reference operator itself is not symmetric
+    M = (vol[:, None] * A)[np.ix_(unk, unk)]
+    assert np.abs(M - M.T).max() < 1e-14 * np.abs(M).max()
+    I, J, K = np.unravel_index(np.arange(n), (nx, ny, nz))
+    for c in np.flatnonzero(unk):
+        i, j, k = I[c], J[c], K[c]
+        for a, (nb, ok) in enumerate(((c + ny * nz, i + 1 < nx), (c + nz, j + 1 < ny),
+                                      (c + 1, k + 1 < nz))):
+            if not ok or not unk[nb]:
+                continue
+            idx = (i, j, k)
+            others = [w[b][idx[b]] for b in range(3) if b != a]
+            expect = t[a][idx[a] + 1] * others[0] * others[1]
+            assert -vol[c] * A[c, nb] == pytest.approx(expect, rel=1e-14)
 
 
 def test_planted_mistake_is_caught():
