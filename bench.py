@@ -37,7 +37,15 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def workload(name: str, precision: str):
+def workload(name: str, precision: str, stretch: float = 1.0):
+    cfg, desc = _workload(name, precision)
+    if stretch != 1.0:  # Grid::set_face_coords faces, finer towards every wall
+        cfg.room.stretch = stretch
+        desc += f", stretched grid (ratio {stretch} per axis)"
+    return cfg, desc
+
+
+def _workload(name: str, precision: str):
     from ff_room.generator import workload_c2, workload_c3
     if name == "c2":
         cfg = workload_c2(precision=precision)
@@ -263,6 +271,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", default="c2")
     ap.add_argument("--mode", default="mixed", choices=["mixed", "fp64"])
+    ap.add_argument("--stretch", type=float, default=1.0,
+                    help="geometric stretch ratio of the grid towards the walls (1: uniform)")
     ap.add_argument("--fp64-steps", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-compare", action="store_true")
@@ -287,7 +297,7 @@ def main():
     from ff_room import capi, device
     ctx = capi.Context(0)
     info = ctx.device_info()
-    cfg, desc = workload(args.workload, args.mode)
+    cfg, desc = workload(args.workload, args.mode, args.stretch)
     ps = device.prepare(cfg)
     log(f"device {info['name']} {info['arch']} CUs={info['compute_units']}; workload {desc}")
 

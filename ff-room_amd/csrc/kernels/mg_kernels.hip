@@ -410,14 +410,14 @@ struct KB_mg_smooth0 {
     for (uint32_t h0 = sb + threadIdx.x; h0 < se; h0 += kU * kBlock) {
         uint16_t nb[kU];
         float rv[kU], zv[kU][6], rov[kU], zov[kU];
-        Cof<ST> cf[kU];
+        uint32_t cnat[kU];  // natural cell index (stretched grids)
 #pragma unroll
         for (int t = 0; t < kU; t++) {
             const uint32_t h = h0 + t * kBlock;  // reads past cnt return 0
             const uint32_t cn = 2 * h + split_parity(L, h, col);
             nb[t] = buf_ldu16(bcode, h);
             rv[t] = buf_ldf(br, h);
-            cf[t] = cof_at<ST>(g, cn);
+            cnat[t] = cn;
             if (!zero_start) nb_gather(bo, cn, si, sj, zv[t]);
             if (DOT) {  // past cnt_o both read 0
                 rov[t] = buf_ldf(bro, h);
@@ -431,8 +431,9 @@ struct KB_mg_smooth0 {
             if (h >= se) break;
             float v = 0.f;
             if (nb[t] & NB_SELF) {
-                const float of = zero_start ? 0.f : nb_sum_t<ST>(g, nb[t], zv[t], cf[t]);
-                v = (rv[t] + of) / diag_t<ST>(g, nb[t], dint, cf[t]);
+                const Cof<ST> cf = cof_at<ST>(g, cnat[t]);
+                const float of = zero_start ? 0.f : nb_sum_t<ST>(g, nb[t], zv[t], cf);
+                v = (rv[t] + of) / diag_t<ST>(g, nb[t], dint, cf);
             }
             buf_stf(bz, h, v);
             if (DOT) acc += (double)rv[t] * (double)v + (double)rov[t] * (double)zov[t];
@@ -560,7 +561,7 @@ struct KB_mg_restrict {
             uint16_t nb[kU];
             float rv[kU], zown[kU], zv[kU][6];
             bool in[kU];
-            Cof<ST> cf[kU];
+            uint32_t cnat[kU];  // natural cell index (stretched grids)
 #pragma unroll
             for (int u = 0; u < kU; u++) {
                 const uint32_t t = t0 + u * kBlock < se ? t0 + u * kBlock : t0;
@@ -577,7 +578,7 @@ struct KB_mg_restrict {
                 nb[u] = g.nbm[c];
                 rv[u] = rs[own];
                 zown[u] = zs[own];
-                cf[u] = cof_at<ST>(g, c);
+                cnat[u] = c;
                 nb_gather(col ? bred : bblack, c, si, sj, zv[u]);
             }
             if (late(stop)) return;
@@ -586,9 +587,11 @@ struct KB_mg_restrict {
                 const uint32_t t = t0 + u * kBlock;
                 if (t >= se) break;
                 float v = 0.f;
-                if (in[u] && (nb[u] & NB_SELF))
-                    v = rv[u] - (diag_t<ST>(g, nb[u], dint, cf[u]) * zown[u] -
-                                 nb_sum_t<ST>(g, nb[u], zv[u], cf[u]));
+                if (in[u] && (nb[u] & NB_SELF)) {
+                    const Cof<ST> cf = cof_at<ST>(g, cnat[u]);
+                    v = rv[u] - (diag_t<ST>(g, nb[u], dint, cf) * zown[u] -
+                                 nb_sum_t<ST>(g, nb[u], zv[u], cf));
+                }
                 v += __shfl_xor(v, 1);
                 v += __shfl_xor(v, 2);
                 v += __shfl_xor(v, 4);
@@ -665,7 +668,7 @@ struct KB_mg_resid0 {
     for (uint32_t s0 = sb + threadIdx.x; s0 < se; s0 += kU2 * kBlock) {
         uint16_t nb[kU2];
         float rv[kU2], zs[kU2], zv[kU2][6];
-        Cof<ST> cf[kU2];
+        uint32_t cnat[kU2];  // natural cell index (stretched grids)
 #pragma unroll
         for (int t = 0; t < kU2; t++) {
             const uint32_t s = s0 + t * kBlock;
@@ -675,7 +678,7 @@ struct KB_mg_resid0 {
             nb[t] = buf_ldu16(bcode, s);
             rv[t] = buf_ldf(br, s);
             zs[t] = buf_ldf(bz, s);
-            cf[t] = cof_at<ST>(g, cn);
+            cnat[t] = cn;
             nb_gather(col ? bred : bblack, cn, si, sj, zv[t]);
         }
         if (late(stop)) return;
@@ -684,9 +687,10 @@ struct KB_mg_resid0 {
             const uint32_t s = s0 + t * kBlock;
             if (s >= se) break;
             float v = 0.f;
-            if (nb[t] & NB_SELF)
-                v = rv[t] - (diag_t<ST>(g, nb[t], dint, cf[t]) * zs[t] -
-                             nb_sum_t<ST>(g, nb[t], zv[t], cf[t]));
+            if (nb[t] & NB_SELF) {
+                const Cof<ST> cf = cof_at<ST>(g, cnat[t]);
+                v = rv[t] - (diag_t<ST>(g, nb[t], dint, cf) * zs[t] - nb_sum_t<ST>(g, nb[t], zv[t], cf));
+            }
             buf_stf(bres, s, v);
         }
     }
@@ -783,14 +787,14 @@ struct KB_mg_post_black0 {
     for (uint32_t h0 = sb + threadIdx.x; h0 < se; h0 += kU2 * kBlock) {
         uint16_t nb[kU2];
         float rv[kU2], zv[kU2][6];
-        Cof<ST> cf[kU2];
+        uint32_t cnat[kU2];  // natural cell index (stretched grids)
 #pragma unroll
         for (int t = 0; t < kU2; t++) {
             const uint32_t h = h0 + t * kBlock;  // reads past cnt return 0
             nb[t] = buf_ldu16(bcode, h);
             rv[t] = buf_ldf(br, h);
             const uint32_t c = 2 * h + split_parity(L, h, 1u);
-            cf[t] = cof_at<ST>(g, c);
+            cnat[t] = c;
             const int i = (int)fdiv(c, g.f_nynz), rem = (int)(c - (uint32_t)i * si);
             const int j = (int)fdiv((uint32_t)rem, g.f_nz), k = rem - j * (int)sj;
             nb_gather(bred, c, si, sj, zv[t]);
@@ -807,8 +811,10 @@ struct KB_mg_post_black0 {
             const uint32_t h = h0 + t * kBlock;
             if (h >= se) break;
             float v = 0.f;
-            if (nb[t] & NB_SELF)
-                v = (rv[t] + nb_sum_t<ST>(g, nb[t], zv[t], cf[t])) / diag_t<ST>(g, nb[t], dint, cf[t]);
+            if (nb[t] & NB_SELF) {
+                const Cof<ST> cf = cof_at<ST>(g, cnat[t]);
+                v = (rv[t] + nb_sum_t<ST>(g, nb[t], zv[t], cf)) / diag_t<ST>(g, nb[t], dint, cf);
+            }
             buf_stf(bz, h, v);
         }
     }
@@ -1314,7 +1320,7 @@ struct KB_pcg_dirq {
     for (uint32_t h0 = hb + threadIdx.x; h0 < he; h0 += kU2 * kBlock) {
         uint16_t nr[kU2], nk[kU2];
         float zrv[kU2], rv[kU2], prv[kU2], qrv[kU2], zbv[kU2], pbv[kU2], qbv[kU2], zn[kU2][6];
-        Cof<ST> cf[kU2];
+        uint32_t cnat[kU2];  // natural cell index (stretched grids)
 #pragma unroll
         for (int t = 0; t < kU2; t++) {
             const uint32_t h = h0 + t * kBlock;
@@ -1324,7 +1330,7 @@ struct KB_pcg_dirq {
             rv[t] = buf_ldf(rr, h);
             nk[t] = buf_ldu16(cb, h);
             zbv[t] = buf_ldf(zb, h);
-            cf[t] = cof_at<ST>(g, cn);
+            cnat[t] = cn;
             nb_gather(zr, cn, si, sj, zn[t]);
             if (!first) {
                 prv[t] = buf_ldf(pr, h);
@@ -1338,9 +1344,11 @@ struct KB_pcg_dirq {
             const uint32_t h = h0 + t * kBlock;
             if (h >= he) break;
             const float azr = (nr[t] & NB_SELF) ? rv[t] : 0.f;
-            const float azb = (nk[t] & NB_SELF) ? diag_t<ST>(g, nk[t], dint, cf[t]) * zbv[t] -
-                                                      nb_sum_t<ST>(g, nk[t], zn[t], cf[t])
-                                                : 0.f;
+            float azb = 0.f;
+            if (nk[t] & NB_SELF) {
+                const Cof<ST> cf = cof_at<ST>(g, cnat[t]);
+                azb = diag_t<ST>(g, nk[t], dint, cf) * zbv[t] - nb_sum_t<ST>(g, nk[t], zn[t], cf);
+            }
             float p0 = zrv[t], q0 = azr, p1 = zbv[t], q1 = azb;
             if (!first) {
                 p0 = __builtin_fmaf(beta, prv[t], p0);
@@ -1417,7 +1425,7 @@ struct KB_pcg_spmv {
     for (uint32_t s0 = sb + threadIdx.x; s0 < se; s0 += kU * kBlock) {
         uint16_t nb[kU];
         float pv[kU], zv[kU][6];
-        Cof<ST> cf[kU];
+        uint32_t cnat[kU];  // natural cell index (stretched grids)
 #pragma unroll
         for (int t = 0; t < kU; t++) {
             const uint32_t s = s0 + t * kBlock;
@@ -1426,7 +1434,7 @@ struct KB_pcg_spmv {
             const uint32_t cn = 2 * h + split_parity(L, h, col);
             nb[t] = buf_ldu16(bcode, s);
             pv[t] = buf_ldf(bp, s);
-            cf[t] = cof_at<ST>(g, cn);
+            cnat[t] = cn;
             nb_gather(col ? bred : bblack, cn, si, sj, zv[t]);
         }
         if (late(stop)) return;  // uniform: every thread of the grid returns
@@ -1436,7 +1444,8 @@ struct KB_pcg_spmv {
             if (s >= se) break;
             float qv = 0.f;
             if (nb[t] & NB_SELF) {
-                qv = diag_t<ST>(g, nb[t], dint, cf[t]) * pv[t] - nb_sum_t<ST>(g, nb[t], zv[t], cf[t]);
+                const Cof<ST> cf = cof_at<ST>(g, cnat[t]);
+                qv = diag_t<ST>(g, nb[t], dint, cf) * pv[t] - nb_sum_t<ST>(g, nb[t], zv[t], cf);
                 acc += (double)pv[t] * (double)qv;
             }
             buf_stf(bq, s, qv);
