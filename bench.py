@@ -228,6 +228,87 @@ def serve_bench(ctx, n_c4: int, steps_c4: int, n_c5: int, steps_c5: int, gangs: 
     return res
 
 
+def _scene_arrays(ps):
+    """The prepared scene in the oracle's layout (stretched faces included)."""
+    from oracle.pyoracle import SceneArrays
+    sc = SceneArrays(ps.nx, ps.ny, ps.nz, float(ps.dxv.sum()), float(ps.dyv.sum()),
+                     float(ps.dzv.sum()), ps.flags, ps.fans,
+                     [o for o in ps.outflows if o not in [tuple(x[:6]) for x in ps.openings]],
+                     ps.openings)
+    if any(np.ptp(h) > 0 for h in (ps.dxv, ps.dyv, ps.dzv)):
+        sc.faces = tuple(np.concatenate([[0.0], np.cumsum(h)]) for h in (ps.dxv, ps.dyv, ps.dzv))
+    return sc
+
+
+def lattice_bench(ctx, cfg, desc, ps, args):
+    """The D3Q19 lattice (LBMSolver, FP64) on the workload's scene: cell-updates/s
+    of the timed lattice steps, the dominant lattice kernel's roofline, and the
+    reference's own serial LBMSolver timed on one step of the same scene."""
+    from ff_room import capi
+    tau, dt = 0.8, ps.params.dt
+    prm = capi.default_params(dt=dt, precision=capi.PREC_FP64, poisson_method=capi.POISSON_CG)
+    d = capi.DeviceScene(ctx, ps.nx, ps.ny, ps.nz, ps.dxv, ps.dyv, ps.dzv, ps.flags, ps.fans,
+                         ps.outflows, ps.openings, prm)
+    d.lbm_attach(tau, dt, 0.0, float(cfg.room.size[0]) / ps.nx)
+    for _ in range(args.warmup):
+        d.lbm_step()
+    if args.kernel_only:  # as for the projection step: the kernel, then the calibration copy
+        for name in (args.kernel_only, "copy"):
+            ms, nbytes = d.prof_kernel(name, 20)
+            print(json.dumps({"kernel": name, "mean_us": ms * 1e3, "bytes": nbytes}), flush=True)
+        d.close()
+        return None
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        r = d.lbm_step()
+    wall = time.perf_counter() - t0
+    value = ps.cells * args.steps / wall
+    ctx.prof_reset()
+    ctx.prof_enable(True)
+    for _ in range(args.steps):
+        d.lbm_step()
+    table = ctx.prof_read()
+    ctx.prof_enable(False)
+    name, e = max(table.items(), key=lambda kv: kv[1]["total_ms"])
+    ms, nbytes = d.prof_kernel(name, 200)
+    ach = nbytes / (ms / 1e3)
+    roof = {"bound": "hbm", "achieved": ach / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+            "frac": ach / HBM_PEAK, "traffic": traffic_from_profiles(name, args.workload),
+            "kernel": name, "bytes_per_launch": nbytes, "mean_launch_us": ms * 1e3,
+            "timing": "200 launches, dispatch begin/end events",
+            "launches_per_step": e["launches"] / args.steps}
+    out = {
+        "metric": "lattice cell-updates/s (D3Q19 LBMSolver step)", "value": value,
+        "unit": "cell-updates/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": 1e3 * wall / args.steps, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic (seeded scene generator, lattice at rest)",
+        "config": {"workload": desc + f"; lattice tau {tau}, dt {dt}", "cells": ps.cells},
+        "roofline": roof,
+        "kernels": {k: {"launches": v["launches"], "mean_us": 1e3 * v["total_ms"] / v["launches"],
+                        "GBps": (v["bytes_per_launch"] / (v["total_ms"] / v["launches"] / 1e3) / 1e9
+                                 if v["bytes_per_launch"] else None)}
+                    for k, v in table.items()},
+        "last_step": {"velocity_change": r["velocity_change"], "divergence_max": r["divergence_max"]},
+    }
+    d.close()
+    log(f"lattice: {value:.4g} cell-updates/s ({out['ms_per_step']:.3f} ms/step); dominant {name} "
+        f"{roof['achieved']:.0f} GB/s ({100 * roof['frac']:.1f}% of peak)")
+    if not args.no_cpu:
+        from oracle.pyoracle import RefLBM, ref_available
+        if ref_available():
+            ref = RefLBM(_scene_arrays(ps), tau, dt, 0.0)
+            t0 = time.perf_counter()
+            ref.step()
+            tr = time.perf_counter() - t0
+            out["cpu_baseline"] = {"value": ps.cells / tr, "unit": "cell-updates/s", "cores": 1,
+                                   "kind": "reference",
+                                   "sample": f"one lattice step of the same scene, the reference's "
+                                             f"serial LBMSolver; {tr:.2f} s"}
+            log(f"cpu: {out['cpu_baseline']['value']:.4g} cell-updates/s")
+    return out
+
+
 def cpu_baseline(ps, steps: int = 1):
     """The reference's own FluidSolver::step (oracle/_ref, compiled from the
     reference sources with OpenMP), timed on this host."""
@@ -241,10 +322,7 @@ def cpu_baseline(ps, steps: int = 1):
                   use_multigrid=0, thermal=p.thermal, T_initial=p.T_initial,
                   T_target=p.T_target, buoyancy=p.buoyancy, g_accel=p.g_accel, beta=p.beta,
                   k_thermal=p.k_thermal, cp=p.cp)
-    sc = SceneArrays(ps.nx, ps.ny, ps.nz, float(ps.dxv.sum()), float(ps.dyv.sum()),
-                     float(ps.dzv.sum()), ps.flags, ps.fans,
-                     [o for o in ps.outflows if o not in [tuple(x[:6]) for x in ps.openings]],
-                     ps.openings)
+    sc = _scene_arrays(ps)
     if ref_available():
         solver, kind = RefSolver(sc, params, use_openmp=True), "reference"
         sh = sc.shapes
@@ -271,6 +349,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", default="c2")
     ap.add_argument("--mode", default="mixed", choices=["mixed", "fp64"])
+    ap.add_argument("--method", default="projection", choices=["projection", "lbm"],
+                    help="lbm: time the D3Q19 lattice solver on the workload's scene instead")
     ap.add_argument("--stretch", type=float, default=1.0,
                     help="geometric stretch ratio of the grid towards the walls (1: uniform)")
     ap.add_argument("--fp64-steps", type=int, default=3)
@@ -300,6 +380,13 @@ def main():
     cfg, desc = workload(args.workload, args.mode, args.stretch)
     ps = device.prepare(cfg)
     log(f"device {info['name']} {info['arch']} CUs={info['compute_units']}; workload {desc}")
+
+    if args.method == "lbm":
+        out = lattice_bench(ctx, cfg, desc, ps, args)
+        ctx.close()
+        if out is not None:
+            print(json.dumps(out), flush=True)
+        return
 
     form_params_inplace(ps.params, args.mode, args.workload)
     ps.params.graphs = args.graphs
