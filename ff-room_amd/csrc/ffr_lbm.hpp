@@ -18,6 +18,10 @@ struct LbmState {
     double *f = nullptr, *f2 = nullptr;  // 19 * nc each
     double *cu = nullptr;                // cell velocities ux | uy | uz, nc each
     uint32_t *mask = nullptr;            // open neighbours per cell (lbm_kernels.hip)
+    // fused form: f holds the next step's populations after inflow and
+    // collision (k_lbm_stream_collide); off when a patch copies from an
+    // INFLOW cell, where f holds the streamed populations between steps
+    bool fused = false;
     void *arena = nullptr;
     size_t bytes = 0;
     int steps = 0;
@@ -33,6 +37,13 @@ void lbm_launch_stream(hipStream_t s, const Geo &g, const uint32_t *mask, const 
                        double *f2, double *cu, double *p, double l2s);
 void lbm_launch_patch(hipStream_t s, const Geo &g, double *f, const LbmPatch &pt, double *cu,
                       double *p, double l2s);
+// fused form: streaming, moments, then the next step's inflow and collision
+void lbm_launch_stream_collide(hipStream_t s, const Geo &g, const uint32_t *mask, const double *f,
+                               double *f2, double *cu, double *p, double l2s, double s2l,
+                               double inv_tau, StepRec *rec);
+// fused form's patch copy (post-collision populations and moments)
+void lbm_launch_patch_post(hipStream_t s, const Geo &g, double *f, const LbmPatch &pt, double *cu,
+                           double *p);
 void lbm_launch_faces(hipStream_t s, const Geo &g, const double *cu, double *u, double *v,
                       double *w, StepRec *rec);
 

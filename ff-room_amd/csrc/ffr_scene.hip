@@ -674,9 +674,15 @@ int ffr_prof_kernel(ffr_scene *s, const char *name, int reps, double *mean_ms, d
         // the lattice's kernels on its current state (algorithmic bytes:
         // populations read and written, flags, cell velocities and faces)
         LbmState &L = s->lbm;
-        if (k == "lbm_relax") {
+        if (k == "lbm_relax") {  // on the spare buffer: the state stays as it is
             b = nc * (19 * 16 + 1);
-            launch = [&] { lbm_launch_relax(st, g, L.f, L.dt / L.dx_ref, 1.0 / L.tau, true, s->rec); };
+            launch = [&] { lbm_launch_relax(st, g, L.f2, L.dt / L.dx_ref, 1.0 / L.tau, true, s->rec); };
+        } else if (k == "lbm_stream" && L.fused) {  // the step's streaming pass (writes the spare buffer)
+            b = nc * (19 * 16 + 4 + 32);
+            launch = [&] {
+                lbm_launch_stream_collide(st, g, L.mask, L.f, L.f2, L.cu, s->p, L.dx_ref / L.dt,
+                                          L.dt / L.dx_ref, 1.0 / L.tau, s->rec);
+            };
         } else if (k == "lbm_stream") {  // populations, mask, cell velocities and p
             b = nc * (19 * 16 + 4 + 32);
             launch = [&] { lbm_launch_stream(st, g, L.mask, L.f, L.f2, L.cu, s->p, L.dx_ref / L.dt); };

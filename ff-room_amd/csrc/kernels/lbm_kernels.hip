@@ -10,8 +10,18 @@
 // reference's operation order, so results equal its bits. Streaming reads a
 // per-cell mask of open neighbours (built once: bit d set when the cell in
 // direction d is inside the domain and not SOLID, bit 31 when the cell is
-// SOLID) instead of 38 flag bytes, and also forms the cell's moments; a
-// patch copy recomputes the moments of the cells it overwrites.
+// SOLID, bit 30 when it is INFLOW) instead of 38 flag bytes, and also forms
+// the cell's moments; a patch copy recomputes the moments of the cells it
+// overwrites.
+//
+// Fused form (k_lbm_stream_collide): the next step's inflow and collision
+// act on each cell's streamed values alone, so the streaming pass applies
+// them at once and stores the next step's pre-streaming populations, and the
+// separate relaxation pass disappears. A patch copy then copies the source
+// cell's post-collision values and moments, which equal the collision of the
+// copied populations when the source cell collides (FLUID, OUTFLOW) or is
+// SOLID (zero populations stay zero); scenes where a patch copies from an
+// INFLOW cell keep the separate passes.
 //
 // Streaming: the reference pushes into a zeroed buffer (LBMSolver.cpp:151-
 // 190) and bounces a population that would leave the domain or enter a
@@ -55,7 +65,7 @@ __device__ __forceinline__ int lbm_rev(int q)
     constexpr int r[19] = {0, 2, 1, 4, 3, 6, 5, 10, 9, 8, 7, 14, 13, 12, 11, 18, 17, 16, 15};
     return r[q];
 }
-constexpr uint32_t kSolidBit = 1u << 31;
+constexpr uint32_t kSolidBit = 1u << 31, kInflowBit = 1u << 30;
 __device__ __forceinline__ double lbm_w(int q)
 {
     return q == 0 ? 1.0 / 3.0 : (q < 7 ? 1.0 / 18.0 : 1.0 / 36.0);
@@ -111,6 +121,24 @@ __global__ void __launch_bounds__(kBlock) k_lbm_rest(uint32_t nc, double *f)
         for (int q = 0; q < 19; q++) f[(size_t)q * nc + c] = lbm_w(q);
 }
 
+// the velocity (lattice units) of the last fan whose box holds cell c (the
+// fans applied in order, LBMSolver.cpp:81-100); false when none does
+__device__ __forceinline__ bool inflow_of(const Geo &g, uint32_t c, double s2l, double &vx,
+                                          double &vy, double &vz)
+{
+    int i, j, k;
+    cell_ijk(g, c, i, j, k);
+    for (int n = g.n_fans - 1; n >= 0; n--) {
+        const FanDev &fn = g.fans[n];
+        if (i < fn.i0 || i > fn.i1 || j < fn.j0 || j > fn.j1 || k < fn.k0 || k > fn.k1) continue;
+        vx = fn.vel[0] * s2l;
+        vy = fn.vel[1] * s2l;
+        vz = fn.vel[2] * s2l;
+        return true;
+    }
+    return false;
+}
+
 // Inflow cells at the equilibrium of the last fan whose box holds them (the
 // fans applied in order, LBMSolver.cpp:81-100); with `collide`, BGK on FLUID
 // and OUTFLOW cells (:106-144). Block 0 clears the step's diagnostics.
@@ -126,16 +154,10 @@ __global__ void __launch_bounds__(kBlock) k_lbm_relax(Geo g, double *f, double s
     for (uint32_t c = blockIdx.x * kBlock + threadIdx.x; c < nc; c += gridDim.x * kBlock) {
         const uint8_t t = g.flags[c];
         if (t == INFLOW) {
-            int i, j, k;
-            cell_ijk(g, c, i, j, k);
-            for (int n = g.n_fans - 1; n >= 0; n--) {
-                const FanDev &fn = g.fans[n];
-                if (i < fn.i0 || i > fn.i1 || j < fn.j0 || j > fn.j1 || k < fn.k0 || k > fn.k1) continue;
-                const double vx = fn.vel[0] * s2l, vy = fn.vel[1] * s2l, vz = fn.vel[2] * s2l;
+            double vx, vy, vz;
+            if (inflow_of(g, c, s2l, vx, vy, vz))
 #pragma unroll
                 for (int q = 0; q < 19; q++) f[(size_t)q * nc + c] = feq(q, 1.0, vx, vy, vz);
-                break;
-            }
             continue;
         }
         if (!collide || t == SOLID) continue;
@@ -156,7 +178,7 @@ __global__ void __launch_bounds__(kBlock) k_lbm_mask(Geo g, uint32_t *mask)
     for (uint32_t c = blockIdx.x * kBlock + threadIdx.x; c < nc; c += gridDim.x * kBlock) {
         int i, j, k;
         cell_ijk(g, c, i, j, k);
-        uint32_t m = g.flags[c] == SOLID ? kSolidBit : 0u;
+        uint32_t m = g.flags[c] == SOLID ? kSolidBit : (g.flags[c] == INFLOW ? kInflowBit : 0u);
 #pragma unroll
         for (int d = 0; d < 19; d++) {
             const int a = i + lbm_ex(d), b = j + lbm_ey(d), e = k + lbm_ez(d);
@@ -223,6 +245,73 @@ __global__ void __launch_bounds__(kBlock) k_lbm_stream(Geo g, const uint32_t *__
     }
 }
 
+// Fused form: streaming and moments as k_lbm_stream, then the next step's
+// inflow equilibrium (INFLOW cells) or BGK collision (FLUID, OUTFLOW) on the
+// streamed values, stored in f2; block 0 clears the step's diagnostics.
+__global__ void __launch_bounds__(kBlock) k_lbm_stream_collide(Geo g, const uint32_t *__restrict__ mask,
+                                                                const double *__restrict__ f,
+                                                                double *__restrict__ f2, double *cu,
+                                                                double *p, double l2s, double s2l,
+                                                                double inv_tau, StepRec *rec)
+{
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        rec->vmax_bits = 0ull;
+        rec->divmax_bits = 0ull;
+        rec->nonfinite = 0;
+    }
+    const uint32_t nc = g.nc, nynz = (uint32_t)g.ny * g.nz, nz = g.nz;
+    for (uint32_t c = blockIdx.x * kBlock + threadIdx.x; c < nc; c += gridDim.x * kBlock) {
+        const uint32_t m = mask[c];
+        if (m & kSolidBit) {  // streams nothing in, collides nothing
+#pragma unroll
+            for (int q = 0; q < 19; q++) f2[(size_t)q * nc + c] = 0.0;
+            cu[c] = 0.0;
+            cu[nc + c] = 0.0;
+            cu[2ull * nc + c] = 0.0;
+            continue;
+        }
+        double v[19];
+        double r = 0.0, x = 0.0, y = 0.0, z = 0.0;
+#pragma unroll
+        for (int q = 0; q < 19; q++) {
+            double a = 0.0;
+            if (m & (1u << lbm_rev(q))) {
+                const int off = lbm_ex(q) * (int)nynz + lbm_ey(q) * (int)nz + lbm_ez(q);
+                a += f[(size_t)q * nc + (uint32_t)((int)c - off)];
+            }
+            const int b = lbm_opp(q);
+            if (!(m & (1u << b))) a += f[(size_t)b * nc + c];
+            v[q] = a;
+            r += a;
+            x += a * lbm_ex(q);
+            y += a * lbm_ey(q);
+            z += a * lbm_ez(q);
+        }
+        if (r > 1e-14) {  // LBMSolver.cpp:114-125 and :288-326 divide alike
+            x /= r;
+            y /= r;
+            z /= r;
+        }
+        cu[c] = x * l2s;
+        cu[nc + c] = y * l2s;
+        cu[2ull * nc + c] = z * l2s;
+        p[c] = r * (1.0 / 3.0);
+        if (m & kInflowBit) {
+            double vx, vy, vz;
+            if (inflow_of(g, c, s2l, vx, vy, vz)) {
+#pragma unroll
+                for (int q = 0; q < 19; q++) f2[(size_t)q * nc + c] = feq(q, 1.0, vx, vy, vz);
+            } else {
+#pragma unroll
+                for (int q = 0; q < 19; q++) f2[(size_t)q * nc + c] = v[q];
+            }
+            continue;
+        }
+#pragma unroll
+        for (int q = 0; q < 19; q++) f2[(size_t)q * nc + c] = v[q] + inv_tau * (feq(q, r, x, y, z) - v[q]);
+    }
+}
+
 // Zero-gradient copy of one patch's OUTFLOW cells from the cell one step
 // inside along the patch axis (LBMSolver.cpp:196-282); a, b the patch's two
 // in-wall axes in order.
@@ -259,6 +348,40 @@ __global__ void __launch_bounds__(kBlock) k_lbm_patch(Geo g, double *f, LbmPatch
             z += v * lbm_ez(q);
         }
         store_moments(nc, co, r, x, y, z, l2s, cu, p);
+    }
+}
+
+// Fused form's patch copy: the source cell's post-collision populations and
+// its moments (zero for a SOLID source, whose populations are zero).
+__global__ void __launch_bounds__(kBlock) k_lbm_patch_post(Geo g, double *f, LbmPatch pt, double *cu,
+                                                            double *p)
+{
+    const uint32_t na = (uint32_t)(pt.a1 - pt.a0 + 1), nb = (uint32_t)(pt.b1 - pt.b0 + 1);
+    const uint32_t n = na * nb, nc = g.nc;
+    const int n_ax = pt.axis == 0 ? g.nx : (pt.axis == 1 ? g.ny : g.nz);
+    const int wall = pt.side ? n_ax - 1 : 0, src = pt.side ? n_ax - 2 : 1;
+    for (uint32_t t = blockIdx.x * kBlock + threadIdx.x; t < n; t += gridDim.x * kBlock) {
+        const int a = pt.a0 + (int)(t / nb), b = pt.b0 + (int)(t % nb);
+        int cw[3], cs[3];
+        if (pt.axis == 0) { cw[0] = wall; cw[1] = a; cw[2] = b; }
+        else if (pt.axis == 1) { cw[0] = a; cw[1] = wall; cw[2] = b; }
+        else { cw[0] = a; cw[1] = b; cw[2] = wall; }
+        if (cw[0] < 0 || cw[0] >= g.nx || cw[1] < 0 || cw[1] >= g.ny || cw[2] < 0 || cw[2] >= g.nz)
+            continue;
+        cs[0] = cw[0];
+        cs[1] = cw[1];
+        cs[2] = cw[2];
+        cs[pt.axis] = src;
+        const uint32_t co = ((uint32_t)cw[0] * g.ny + cw[1]) * g.nz + cw[2];
+        if (g.flags[co] != OUTFLOW) continue;
+        const uint32_t ci = ((uint32_t)cs[0] * g.ny + cs[1]) * g.nz + cs[2];
+#pragma unroll
+        for (int q = 0; q < 19; q++) f[(size_t)q * nc + co] = f[(size_t)q * nc + ci];
+        const bool solid = g.flags[ci] == SOLID;
+        cu[co] = solid ? 0.0 : cu[ci];
+        cu[nc + co] = solid ? 0.0 : cu[nc + ci];
+        cu[2ull * nc + co] = solid ? 0.0 : cu[2ull * nc + ci];
+        p[co] = solid ? 0.0 : p[ci];
     }
 }
 
@@ -334,6 +457,19 @@ void lbm_launch_stream(hipStream_t s, const Geo &g, const uint32_t *mask, const 
                        double *f2, double *cu, double *p, double l2s)
 {
     FFR_LAUNCH(k_lbm_stream, dim3(grid_for(g.nc)), dim3(kBlock), 0, s, g, mask, f, f2, cu, p, l2s);
+}
+void lbm_launch_stream_collide(hipStream_t s, const Geo &g, const uint32_t *mask, const double *f,
+                               double *f2, double *cu, double *p, double l2s, double s2l,
+                               double inv_tau, StepRec *rec)
+{
+    FFR_LAUNCH(k_lbm_stream_collide, dim3(grid_for(g.nc)), dim3(kBlock), 0, s, g, mask, f, f2, cu, p,
+               l2s, s2l, inv_tau, rec);
+}
+void lbm_launch_patch_post(hipStream_t s, const Geo &g, double *f, const LbmPatch &pt, double *cu,
+                           double *p)
+{
+    const uint64_t n = (uint64_t)(pt.a1 - pt.a0 + 1) * (pt.b1 - pt.b0 + 1);
+    FFR_LAUNCH(k_lbm_patch_post, dim3(grid_for(n)), dim3(kBlock), 0, s, g, f, pt, cu, p);
 }
 void lbm_launch_patch(hipStream_t s, const Geo &g, double *f, const LbmPatch &pt, double *cu,
                       double *p, double l2s)

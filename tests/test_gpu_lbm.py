@@ -94,3 +94,45 @@ def test_lattice_needs_fp64(ctx):
     with pytest.raises(capi.FfrError):
         d.lbm_attach(tau, dt, 0.0, sc.lx / sc.nx)
     d.close()
+
+
+def _edge_scene(fan_at_outflow: bool):
+    """12x10x8 room: an outflow wall at x = max, a SOLID box, an INFLOW cell
+    that no fan box holds (it streams but neither collides nor takes an
+    equilibrium), and a fan either inside the room or on the cells the
+    outflow copies from."""
+    from oracle.pyoracle import INFLOW, OUTFLOW, SOLID, SceneArrays
+    nx, ny, nz = 12, 10, 8
+    f = np.zeros((nx, ny, nz), dtype=np.uint8)
+    f[nx - 1] = OUTFLOW
+    f[4:6, 2:4, 0:3] = SOLID
+    f[2, 7, 5] = INFLOW
+    i = nx - 2 if fan_at_outflow else 3
+    f[i, 3:6, 2:5] = INFLOW
+    fans = [(i, i, 3, 5, 2, 4, 1.5, 0.2, 0.0)]
+    return SceneArrays(nx, ny, nz, 2.4, 2.0, 1.6, f.reshape(-1), fans, [(0, 1, 0, ny - 1, 0, nz - 1)])
+
+
+@pytest.mark.parametrize("fan_at_outflow", [False, True])
+def test_lattice_fused_and_separate_forms(ctx, fan_at_outflow):
+    """The streaming pass applies the next step's inflow and collision unless
+    a patch copies from an INFLOW cell; either form equals the oracle bit for
+    bit, and the profile shows which ran (a separate relaxation pass only in
+    the second)."""
+    sc = _edge_scene(fan_at_outflow)
+    tau, dt = 0.7, 0.01
+    d = _device(ctx, sc, tau, dt)
+    d.lbm_attach(tau, dt, 0.0, sc.lx / sc.nx)
+    o = OracleLBM(sc, tau, dt, 0.0)
+    ctx.prof_reset()
+    ctx.prof_enable(True)
+    for _ in range(15):
+        rd, ro = d.lbm_step(), o.step()
+        assert np.array_equal(_res(rd), _res(ro)), (rd, ro)
+    table = ctx.prof_read()
+    ctx.prof_enable(False)
+    assert ("lbm_relax" in table) == fan_at_outflow, sorted(table)
+    fd, fo = d.download(), o.fields()
+    for k in ("u", "v", "w", "p"):
+        assert np.array_equal(fd[k], fo[k]), k
+    d.close()

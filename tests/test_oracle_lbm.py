@@ -17,7 +17,7 @@ import numpy as np
 import pytest
 
 from conftest import GOLDEN
-from oracle.pyoracle import OracleLBM, RefLBM, SceneArrays, ref_available
+from oracle.pyoracle import INFLOW, OUTFLOW, SOLID, OracleLBM, RefLBM, SceneArrays, ref_available
 from oracle.scene_flags import build_scene
 from scenes import BASIC_ROOM, SMALL_LW, SMALL_THERMAL
 
@@ -52,10 +52,25 @@ def _random_room():
     return SceneArrays(18, 14, 10, 4.0, 3.0, 2.5, flags, fans, plain, opens)
 
 
+def _edge_room(fan_at_outflow: bool):
+    """An outflow wall, a SOLID box, an INFLOW cell no fan holds, and a fan
+    inside the room or on the cells the outflow copies from."""
+    nx, ny, nz = 12, 10, 8
+    f = np.zeros((nx, ny, nz), dtype=np.uint8)
+    f[nx - 1] = OUTFLOW
+    f[4:6, 2:4, 0:3] = SOLID
+    f[2, 7, 5] = INFLOW
+    i = nx - 2 if fan_at_outflow else 3
+    f[i, 3:6, 2:5] = INFLOW
+    fans = [(i, i, 3, 5, 2, 4, 1.5, 0.2, 0.0)]
+    return SceneArrays(nx, ny, nz, 2.4, 2.0, 1.6, f.reshape(-1), fans, [(0, 1, 0, ny - 1, 0, nz - 1)])
+
+
 @pytest.mark.skipif(not ref_available(), reason="oracle/_ref not built (reference tree absent)")
-@pytest.mark.parametrize("which", ["lw", "random"])
+@pytest.mark.parametrize("which", ["lw", "random", "edge", "edge_fan_at_outflow"])
 def test_oracle_lbm_matches_reference(which):
-    sc = build_scene(SMALL_LW) if which == "lw" else _random_room()
+    sc = {"lw": lambda: build_scene(SMALL_LW), "random": _random_room,
+          "edge": lambda: _edge_room(False), "edge_fan_at_outflow": lambda: _edge_room(True)}[which]()
     o, r = OracleLBM(sc, 0.7, 0.005, 0.0), RefLBM(sc, 0.7, 0.005, 0.0)
     for _ in range(15):
         assert o.step() == {k: v for k, v in r.step().items()}
