@@ -248,7 +248,10 @@ __global__ void __launch_bounds__(kBlock) k_lbm_stream(Geo g, const uint32_t *__
 // Fused form: streaming and moments as k_lbm_stream, then the next step's
 // inflow equilibrium (INFLOW cells) or BGK collision (FLUID, OUTFLOW) on the
 // streamed values, stored in f2; block 0 clears the step's diagnostics.
-__global__ void __launch_bounds__(kBlock) k_lbm_stream_collide(Geo g, const uint32_t *__restrict__ mask,
+// Eight waves per SIMD (64 VGPRs, the 19 streamed values partly spilled to
+// scratch) beat the unconstrained 111 VGPRs at four waves: 419 -> 366 us on
+// c2, measured; five and six waves fall in between.
+__global__ void __launch_bounds__(kBlock, 8) k_lbm_stream_collide(Geo g, const uint32_t *__restrict__ mask,
                                                                 const double *__restrict__ f,
                                                                 double *__restrict__ f2, double *cu,
                                                                 double *p, double l2s, double s2l,
