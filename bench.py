@@ -298,22 +298,39 @@ def lattice_bench(ctx, cfg, desc, ps, args):
         from oracle.pyoracle import RefLBM, ref_available
         if ref_available():
             ref = RefLBM(_scene_arrays(ps), tau, dt, 0.0)
+            ref.step()  # warm-up
             t0 = time.perf_counter()
-            ref.step()
+            for _ in range(5):
+                ref.step()
             tr = time.perf_counter() - t0
-            out["cpu_baseline"] = {"value": ps.cells / tr, "unit": "cell-updates/s", "cores": 1,
+            out["cpu_baseline"] = {"value": ps.cells * 5 / tr, "unit": "cell-updates/s", "cores": 1,
                                    "kind": "reference",
-                                   "sample": f"one lattice step of the same scene, the reference's "
-                                             f"serial LBMSolver; {tr:.2f} s"}
+                                   "sample": f"5 lattice steps after one warm-up step of the same "
+                                             f"scene, the reference's LBMSolver (serial: it has no "
+                                             f"threading); {tr:.2f} s",
+                                   "nproc": os.cpu_count(), "cpu_model": _cpu_model()}
             log(f"cpu: {out['cpu_baseline']['value']:.4g} cell-updates/s")
     return out
 
 
-def cpu_baseline(ps, steps: int = 1):
+def _cpu_model() -> str:
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(ps, steps: int = 2, single: bool = False):
     """The reference's own FluidSolver::step (oracle/_ref, compiled from the
-    reference sources with OpenMP), timed on this host."""
-    from oracle.pyoracle import RefSolver, SceneArrays, ref_available, OracleSolver
-    from ff_room import capi
+    reference sources), timed on this host: with OpenMP on every thread the
+    box grants, one warm-up step then `steps` timed (steady state), and with
+    `single` the reference's default single thread for one step (tens of
+    seconds on c2). Without the reference build, the oracle's restatement
+    stands in (kind "port")."""
+    from oracle.pyoracle import RefSolver, ref_available, OracleSolver
     threads = int(os.environ.get("OMP_NUM_THREADS", "16"))
     os.environ["OMP_NUM_THREADS"] = str(threads)
     p = ps.params
@@ -323,23 +340,42 @@ def cpu_baseline(ps, steps: int = 1):
                   T_target=p.T_target, buoyancy=p.buoyancy, g_accel=p.g_accel, beta=p.beta,
                   k_thermal=p.k_thermal, cp=p.cp)
     sc = _scene_arrays(ps)
-    if ref_available():
-        solver, kind = RefSolver(sc, params, use_openmp=True), "reference"
-        sh = sc.shapes
-        solver.set_fields(np.zeros(sh["u"]), np.zeros(sh["v"]), np.zeros(sh["w"]),
-                          np.zeros(sh["p"]), ps.T0)
-    else:
-        solver, kind = OracleSolver(sc, params, nthreads=threads), "port"
+
+    def make(parallel: bool):
+        if ref_available():
+            solver = RefSolver(sc, params, use_openmp=parallel)
+            sh = sc.shapes
+            solver.set_fields(np.zeros(sh["u"]), np.zeros(sh["v"]), np.zeros(sh["w"]),
+                              np.zeros(sh["p"]), ps.T0)
+            return solver, "reference"
+        solver = OracleSolver(sc, params, nthreads=threads if parallel else 1)
         solver.set_fields(T=ps.T0)
+        return solver, "port"
+
+    solver, kind = make(True)
+    solver.step()  # warm-up
     t0 = time.perf_counter()
     for _ in range(steps):
-        solver.step()
+        r = solver.step()
     dt = time.perf_counter() - t0
-    return {"value": ps.cells * steps / dt, "unit": "cell-updates/s", "cores": threads,
-            "kind": kind,
-            "sample": f"{steps} step(s) of the same workload from its initial state, the "
-                      f"reference CG at its default cap (1000 iterations, tol 1e-6), "
-                      f"OpenMP {threads} threads; {dt:.1f} s"}
+    out = {"value": ps.cells * steps / dt, "unit": "cell-updates/s", "cores": threads,
+           "kind": kind,
+           "sample": f"{steps} steps after one warm-up step of the same workload from its initial "
+                     f"state, the reference CG at its default cap (1000 iterations, tol 1e-6), "
+                     f"OpenMP {threads} threads; {dt:.1f} s",
+           "cg_residual_at_cap": r.get("pressure_residual"),
+           "nproc": os.cpu_count(), "cpu_model": _cpu_model()}
+    del solver
+    if not single:
+        return out
+    solver, _ = make(False)
+    t0 = time.perf_counter()
+    solver.step()
+    d1 = time.perf_counter() - t0
+    out["single_thread"] = {"value": ps.cells / d1, "unit": "cell-updates/s", "cores": 1,
+                            "sample": f"the first step, the reference default (parallel off); "
+                                      f"{d1:.1f} s"}
+    return out
 
 
 def main():
@@ -355,6 +391,8 @@ def main():
                     help="geometric stretch ratio of the grid towards the walls (1: uniform)")
     ap.add_argument("--fp64-steps", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-single", action="store_true",
+                    help="also time the reference's single-threaded step (tens of seconds on c2)")
     ap.add_argument("--no-compare", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--no-tuning", action="store_true", help="library defaults, not the tuning table")
@@ -474,7 +512,7 @@ def main():
 
     if not args.no_cpu:
         log("timing the reference CPU solver on a bounded sample ...")
-        out["cpu_baseline"] = cpu_baseline(device.prepare(cfg), steps=1)
+        out["cpu_baseline"] = cpu_baseline(device.prepare(cfg), single=args.cpu_single)
         log(f"cpu: {out['cpu_baseline']['value']:.4g} cell-updates/s")
     ctx.close()
     print(json.dumps(out), flush=True)
