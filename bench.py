@@ -323,6 +323,45 @@ def _cpu_model() -> str:
     return "unknown"
 
 
+def next_rows(ctx, args):
+    """Short measurements of the solver's other paths on the same room: the
+    D3Q19 lattice (as --method lbm) and the stretched grid (as --stretch
+    1.01: the mixed form against the plain FP64 CG form)."""
+    from ff_room import autotune, device
+    rows = {}
+    cfg, desc = workload(args.workload, "mixed")
+    lat = lattice_bench(ctx, cfg, desc, device.prepare(cfg), args)
+    rows["lattice"] = {k: lat[k] for k in ("value", "unit", "ms_per_step", "dtype")}
+    rows["lattice"]["roofline"] = {k: lat["roofline"][k] for k in ("kernel", "achieved", "frac", "traffic")}
+    if "cpu_baseline" in lat:
+        rows["lattice"]["cpu_baseline"] = lat["cpu_baseline"]
+    cfg, desc = workload(args.workload, "mixed", 1.01)
+    ps = device.prepare(cfg)
+    form_params_inplace(ps.params, "mixed", args.workload)
+    ps.params.graphs = args.graphs
+    tuned = autotune.lookup(ps.nx, ps.ny, ps.nz, ps.params.precision) if not args.no_tuning else None
+    if tuned:
+        autotune.apply(ps.params, tuned)
+    d = device.create(ctx, ps)
+    wall, res, _ = timed_steps(d, ctx, args.steps, args.warmup, prof=False)
+    d.close()
+    ps64 = device.prepare(cfg)
+    form_params_inplace(ps64.params, "fp64", args.workload)
+    d = device.create(ctx, ps64)
+    w64, r64, _ = timed_steps(d, ctx, 1, 1, prof=False)
+    d.close()
+    v, v64 = ps.cells * args.steps / wall, ps64.cells / w64
+    rows["stretched"] = {"workload": desc, "value": v, "unit": "cell-updates/s",
+                         "ms_per_step": 1e3 * wall / args.steps,
+                         "inner_iterations": [min(r["poisson_iters"] for r in res),
+                                              max(r["poisson_iters"] for r in res)],
+                         "fp64_cg": {"value": v64, "iterations": r64[0]["poisson_iters"],
+                                     "residual": r64[0]["pressure_residual"]},
+                         "speedup_vs_fp64_cg": v / v64}
+    log(f"stretched: {v:.4g} cell-updates/s, fp64 CG {v64:.4g} -> {v / v64:.1f}x")
+    return rows
+
+
 def cpu_baseline(ps, steps: int = 2, single: bool = False):
     """The reference's own FluidSolver::step (oracle/_ref, compiled from the
     reference sources), timed on this host: with OpenMP on every thread the
@@ -395,6 +434,8 @@ def main():
                     help="also time the reference's single-threaded step (tens of seconds on c2)")
     ap.add_argument("--no-compare", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--no-next", action="store_true",
+                    help="skip the lattice and stretched-grid measurements")
     ap.add_argument("--no-tuning", action="store_true", help="library defaults, not the tuning table")
     ap.add_argument("--graphs", type=int, default=1,
                     help="replay inner iterations from HIP graphs (0 for tracing profilers)")
@@ -504,6 +545,8 @@ def main():
         log(f"fp64 CG: {v64:.4g} cell-updates/s -> mixed/fp64 = {value / v64:.1f}x")
 
     d_keep.close()
+    if args.mode == "mixed" and not args.no_next:
+        out["next_rows"] = next_rows(ctx, args)
     if args.mode == "mixed" and not args.no_serve:
         out["serving"] = serve_bench(ctx, args.c4, args.c4_steps, args.c5, args.c5_steps, args.gangs)
         for k, v in out["serving"].items():
