@@ -204,7 +204,9 @@ def step_errors(ctx, ps64, d_mixed, sealed):
 def serve_bench(ctx, n_c4: int, steps_c4: int, n_c5: int, steps_c5: int, gangs: int = 0):
     """Continuous batching samples: c4 (uneven rooms all submitted at once)
     and c5 (a seeded Poisson stream at 50 requests/s), mixed MG-PCG form;
-    `gangs` groups of merged launches (0: every scene on its own stream)."""
+    `gangs` groups of merged launches (0: every scene on its own stream).
+    Both in arrival order, and c5 again with the smallest requests admitted
+    first."""
     from ff_room import capi
     from ff_room.generator import workload_c4, workload_c5
     from ff_room.serve import Request, Server, summary
@@ -224,6 +226,11 @@ def serve_bench(ctx, n_c4: int, steps_c4: int, n_c5: int, steps_c5: int, gangs: 
                      sample=f"first {n_c5} requests of the seeded stream "
                      f"(50 req/s, grids 24^3..256x204x80, <= {steps_c5} steps or "
                      f"max|du|/dt < 1e-4), 16 slots")
+    srv = Server(ctx, slots=16, gangs=gangs, policy="shortest", **kw)
+    done = srv.run([Request(c, steps_c5, t) for t, c in stream], realtime=True)
+    res["c5_shortest_first"] = dict(summary(done), gangs=gangs,
+                                    sample="the c5 sample, admitted smallest work (cells x step "
+                                           "budget) first")
     ctx.serve_gangs(0)
     return res
 

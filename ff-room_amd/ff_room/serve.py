@@ -1,7 +1,10 @@
 """Serving: a stream of scene-solve requests, continuously batched on one GPU.
 
 A request is a scene (SceneConfig), a step budget and an arrival time.
-Requests wait in arrival order. The head of the queue is admitted when a slot
+Requests wait in arrival order (policy "fifo"), or ordered by their work,
+cells x step budget, smallest first (policy "shortest": lower median latency
+under load, at the price of making large requests wait behind later small
+ones). The head of the queue is admitted when a slot
 is free and its device footprint (``ffr_scene_footprint``) fits the context's
 HBM budget beside the scenes already resident; the scene is then built (flags,
 boundary lists, initial temperature: ``device.prepare``), created on the
@@ -76,9 +79,13 @@ class Server:
     scene on its own stream."""
 
     def __init__(self, ctx: capi.Context, slots: int = 16, hbm_budget: Optional[int] = None,
-                 keep_fields: bool = False, tuned: bool = True, gangs: int = 0, **overrides):
+                 keep_fields: bool = False, tuned: bool = True, gangs: int = 0,
+                 policy: str = "fifo", **overrides):
+        if policy not in ("fifo", "shortest"):
+            raise ValueError(f"unknown admission policy {policy!r} (fifo, shortest)")
         self.ctx = ctx
         self.slots = slots
+        self.policy = policy
         ctx.serve_config(slots)
         ctx.serve_gangs(gangs)
         info = ctx.device_info()
@@ -101,7 +108,15 @@ class Server:
         self.pending.append((i, req, None))
         return i
 
+    @staticmethod
+    def work(req: Request) -> int:
+        """Cells x step budget: the admission key of policy "shortest"."""
+        g = req.config.room.grid
+        return int(g[0]) * int(g[1]) * int(g[2]) * int(req.max_steps)
+
     def _admit(self):
+        if self.policy == "shortest" and len(self.pending) > 1:
+            self.pending = deque(sorted(self.pending, key=lambda e: (self.work(e[1]), e[0])))
         while self.pending and len(self.active) < self.slots:
             i, req, ps = self.pending[0]
             if ps is None:

@@ -120,3 +120,25 @@ def test_summary_statistics():
     assert s["p50_latency_s"] == pytest.approx(np.percentile(np.arange(1, 101), 50))
     assert s["p99_latency_s"] == pytest.approx(np.percentile(np.arange(1, 101), 99))
     assert s["cell_updates_per_s"] == pytest.approx(100 * 10 * 100 / 100.0)
+
+
+def test_shortest_policy_admits_smallest_work_first(monkeypatch):
+    from types import SimpleNamespace
+    ctx = _FakeCtx()
+    monkeypatch.setattr(serve.device, "prepare", lambda cfg, **kw: _FakePS(cfg.need, cfg.cells))
+    monkeypatch.setattr(serve.device, "create", lambda c, ps: _FakeScene(c, ps))
+
+    def req(cells, steps, need=100):
+        room = SimpleNamespace(grid=(cells, 1, 1))
+        return serve.Request(SimpleNamespace(room=room, need=need, cells=cells), steps, 0.0)
+
+    reqs = [req(1000, 10), req(10, 5), req(100, 10), req(10, 50)]  # work 10000, 50, 1000, 500
+    srv = serve.Server(ctx, slots=1, hbm_budget=1000, tuned=False, policy="shortest")
+    done = srv.run(reqs, realtime=False)
+    order = [c.index for c in sorted(done, key=lambda c: c.admitted)]
+    assert order == [1, 3, 2, 0]
+    srv = serve.Server(ctx, slots=1, hbm_budget=1000, tuned=False)  # fifo keeps arrival order
+    done = srv.run(reqs, realtime=False)
+    assert [c.index for c in sorted(done, key=lambda c: c.admitted)] == [0, 1, 2, 3]
+    with pytest.raises(ValueError):
+        serve.Server(ctx, policy="random")
