@@ -53,7 +53,11 @@ enum {
 enum {
     FFR_POISSON_AUTO = 0,  /* reference choice: CG, or MG-PCG when use_multigrid */
     FFR_POISSON_CG = 1,    /* unpreconditioned CG, PoissonSolver.cpp:93-158 */
-    FFR_POISSON_MGPCG = 2  /* geometric-multigrid-preconditioned CG */
+    FFR_POISSON_MGPCG = 2  /* geometric-multigrid-preconditioned CG: an FP64 outer
+                              residual of the reference operator around inner
+                              solves; on a stretched grid (varying dxv/dyv/dzv)
+                              the inner solves run on the operator scaled row by
+                              row by the relative cell volume, which is symmetric */
 };
 
 /* FanBC, BoundaryManager.hpp:8-13 */
