@@ -103,8 +103,9 @@ struct Geo {
     const float *cxf, *cyf, *czf, *icxf, *icyf, *iczf, *lapf;
     const uint16_t *nbm;               // neighbour codes (see NB_*)
     // FP32 per-axis factors of the multigrid's symmetric level-0 operator on a
-    // stretched grid (relative widths w | face factors t, mg_kernels.hip)
-    const float *mgw, *mgt;
+    // stretched grid: one {w, t below, t above, 0} per index of x | y | z
+    // (mg_kernels.hip)
+    const float4 *mgf;
     const uint16_t *fcode;             // face mask codes (see FC_*), u | v | w
     double sxd, syd, szd;              // uniform-grid coupling 1/h^2 per axis (0 if not uniform)
     float sx, sy, sz;

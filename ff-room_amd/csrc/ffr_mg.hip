@@ -77,10 +77,10 @@ size_t mg_layout(char *base, const Dims &D, bool bf16, bool dense, Multigrid *M,
     float *z0 = (float *)take(8ull * H);
     uint16_t *nbs = (uint16_t *)take(4ull * H);
     const size_t na = (size_t)D.n[0][0] + D.n[0][1] + D.n[0][2];
-    float *mw = (float *)take(4 * na), *mt = (float *)take(4 * (na + 3));
+    float4 *mf = (float4 *)take(16 * na);
     if (M) {
         M->e = e; M->ri = ri; M->pi = pi; M->qi = qi; M->sc = sc;
-        M->mgw = mw; M->mgt = mt;
+        M->mgf = mf;
         M->lv[0].z = z0;
         M->lv[0].r = ri;
         M->lv[0].nbs = nbs;
@@ -122,16 +122,15 @@ double mean(const double *h, int n)
     return s / n;
 }
 
-// Level-0 factors of a stretched grid (see Multigrid::mgw / mgt): per axis,
-// w[i] = d[i] / m and t[i] = 1 / (0.5 (d[i-1] + d[i]) m), the centre distance
-// of PoissonSolver.cpp:34-49, for 1 <= i < n; t[0] = t[n] = 0.
-void stretch_factors(const double *d, int n, std::vector<float> &w, std::vector<float> &t)
+// Level-0 factors of a stretched grid (see Multigrid::mgf), one axis: w[i] =
+// d[i] / m and t(i) = 1 / (0.5 (d[i-1] + d[i]) m), the centre distance of
+// PoissonSolver.cpp:34-49, of the face below i for 1 <= i < n (0 at both
+// walls); entry i holds {w[i], t(i), t(i+1), 0}.
+void stretch_factors(const double *d, int n, std::vector<float4> &f)
 {
     const double m = mean(d, n);
-    for (int i = 0; i < n; i++) w.push_back((float)(d[i] / m));
-    t.push_back(0.f);
-    for (int i = 1; i < n; i++) t.push_back((float)(1.0 / (0.5 * (d[i - 1] + d[i]) * m)));
-    t.push_back(0.f);
+    auto t = [&](int i) { return (i <= 0 || i >= n) ? 0.f : (float)(1.0 / (0.5 * (d[i - 1] + d[i]) * m)); };
+    for (int i = 0; i < n; i++) f.push_back(make_float4((float)(d[i] / m), t(i), t(i + 1), 0.f));
 }
 
 }  // namespace
@@ -323,17 +322,14 @@ int mg_create(ffr_scene *s)
     M.coarse_sweeps = s->prm.mg_coarse_sweeps > 0 ? s->prm.mg_coarse_sweeps : 8;
     hipStream_t str = s->stream;
     HIP_CHECK_CTX(s->ctx, hipMemsetAsync(M.arena, 0, bytes, str), FFR_E_DEVICE);
-    s->geo.mgw = M.mgw;
-    s->geo.mgt = M.mgt;
-    std::vector<float> fw, ft;  // (stay zero on a uniform grid, where no kernel reads them)
+    s->geo.mgf = M.mgf;
+    std::vector<float4> ff;  // (stay zero on a uniform grid, where no kernel reads them)
     if (!s->geo.uniform) {
-        stretch_factors(s->dxv.data(), s->nx, fw, ft);
-        stretch_factors(s->dyv.data(), s->ny, fw, ft);
-        stretch_factors(s->dzv.data(), s->nz, fw, ft);
-        HIP_CHECK_CTX(s->ctx, hipMemcpyAsync(M.mgw, fw.data(), 4 * fw.size(), hipMemcpyHostToDevice, str),
-                      FFR_E_DEVICE);
-        HIP_CHECK_CTX(s->ctx, hipMemcpyAsync(M.mgt, ft.data(), 4 * ft.size(), hipMemcpyHostToDevice, str),
-                      FFR_E_DEVICE);
+        stretch_factors(s->dxv.data(), s->nx, ff);
+        stretch_factors(s->dyv.data(), s->ny, ff);
+        stretch_factors(s->dzv.data(), s->nz, ff);
+        HIP_CHECK_CTX(s->ctx, hipMemcpyAsync(M.mgf, ff.data(), sizeof(float4) * ff.size(),
+                                             hipMemcpyHostToDevice, str), FFR_E_DEVICE);
     }
     mg_launch_split_codes(str, s->geo, M.lv[0]);
     for (int l = 0; l + 1 < D.levels; l++) mg_launch_build(str, s->geo, M.lv[l], M.lv[l + 1], l == 0);

@@ -82,10 +82,10 @@ struct Multigrid {
     // level-0 vectors of the inner PCG (FP32) and the outer correction; sc
     // holds the level-0 residual of the V-cycle's restriction
     float *e, *ri, *pi, *qi, *sc;
-    // per-axis factors of level 0 on a stretched grid: w = d / mean width
-    // (nx + ny + nz) and t = 1 / (centre distance * mean width) of the face
-    // below each index (nx+1 + ny+1 + nz+1, 0 at the walls)
-    float *mgw, *mgt;
+    // per-axis factors of level 0 on a stretched grid, per index of x | y | z:
+    // {w = d / mean width, t of the face below, t of the face above, 0}, with
+    // t = 1 / (centre distance * mean width), 0 at the walls
+    float4 *mgf;
     int pre_sweeps = 2, level_sweeps = 2, coarse_sweeps = 8;
     // levels tail_first .. levels-1 run as one single-workgroup kernel with
     // their z and r in LDS (-1: none); tail_lds = its LDS bytes; the

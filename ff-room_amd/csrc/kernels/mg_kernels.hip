@@ -95,9 +95,9 @@ __device__ __forceinline__ float diag0m(const Geo &g, uint16_t m)
 // coefficient becomes w_b w_c / (h_f m_a) (the other two axes' relative
 // widths, m_a the face axis' mean width): the same from both sides. The inner
 // solve runs on that symmetric, volume-weighted form with the outer residual
-// weighted alike, which has the same solution; per axis, Geo.mgw holds w
-// (n values) and Geo.mgt holds t = 1/(h_f m_a) of the face below index i
-// (n + 1 values, 0 at both walls), so the couplings of a cell are t w w.
+// weighted alike, which has the same solution; Geo.mgf holds, per index of
+// each axis, {w, t below, t above} with t = 1/(h_f m_a) (0 at the walls), so
+// a cell's couplings are t w w from three 16-byte loads.
 template <bool ST>
 struct Cof {
     float f[6];  // -x +x -y +y -z +z
@@ -113,15 +113,14 @@ __device__ __forceinline__ Cof<ST> cof_at(const Geo &g, uint32_t c)
         c = c < g.nc ? c : g.nc - 1;  // slots past the end: any in-range cell
         const uint32_t i = fdiv(c, g.f_nynz), rem = c - i * (uint32_t)(g.ny * g.nz);
         const uint32_t j = fdiv(rem, g.f_nz), k = rem - j * (uint32_t)g.nz;
-        const float *W = g.mgw, *tx = g.mgt, *ty = g.mgt + g.nx + 1, *tz = g.mgt + g.nx + g.ny + 2;
-        const float wx = W[i], wy = W[g.nx + j], wz = W[g.nx + g.ny + k];
-        const float yz = wy * wz, xz = wx * wz, xy = wx * wy;
-        r.f[0] = tx[i] * yz;
-        r.f[1] = tx[i + 1] * yz;
-        r.f[2] = ty[j] * xz;
-        r.f[3] = ty[j + 1] * xz;
-        r.f[4] = tz[k] * xy;
-        r.f[5] = tz[k + 1] * xy;
+        const float4 X = g.mgf[i], Y = g.mgf[g.nx + j], Z = g.mgf[g.nx + g.ny + k];
+        const float yz = Y.x * Z.x, xz = X.x * Z.x, xy = X.x * Y.x;
+        r.f[0] = X.y * yz;
+        r.f[1] = X.z * yz;
+        r.f[2] = Y.y * xz;
+        r.f[3] = Y.z * xz;
+        r.f[4] = Z.y * xy;
+        r.f[5] = Z.z * xy;
     }
     return r;
 }
@@ -1701,7 +1700,7 @@ struct KB_dc_residual {
         if constexpr (ST) {
             const uint32_t i = fdiv(c, g.f_nynz), rem = c - i * (uint32_t)(g.ny * g.nz);
             const uint32_t j = fdiv(rem, g.f_nz), k = rem - j * (uint32_t)g.nz;
-            const double vol = (double)g.mgw[i] * (double)g.mgw[g.nx + j] * (double)g.mgw[g.nx + g.ny + k];
+            const double vol = (double)g.mgf[i].x * (double)g.mgf[g.nx + j].x * (double)g.mgf[g.nx + g.ny + k].x;
             const float rf = (float)(rv * vol);
             r32[sc] = rf;
             if (red_slot) z0[sc] = rf / diag_t<ST>(g, nb, 0.f, cof_at<ST>(g, c));
