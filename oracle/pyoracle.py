@@ -474,6 +474,9 @@ class RefLBM:
         self.lib.ref_lbm_step.argtypes = [C.c_void_p, C.POINTER(_RefResult)]
         self.scene = scene
         self.h = self.lib.ref_create(scene.nx, scene.ny, scene.nz, scene.lx, scene.ly, scene.lz)
+        if scene.faces is not None:
+            self._faces = [np.ascontiguousarray(f, dtype=np.float64) for f in scene.faces]
+            self.lib.ref_set_face_coords(self.h, *[_ptr(f) for f in self._faces])
         for f in scene.fans:
             self.lib.ref_add_fan(self.h, *[int(x) for x in f[:6]], *[float(x) for x in f[6:9]])
         for o in scene.openings:

@@ -18,7 +18,7 @@ from conftest import GOLDEN
 from ff_room import SceneConfig, SolverBridge, capi
 from oracle.pyoracle import OracleLBM
 from oracle.scene_flags import build_scene
-from scenes import BASIC_ROOM, SMALL_LW, SMALL_THERMAL, variant
+from scenes import BASIC_ROOM, SMALL_LW, SMALL_THERMAL, stretched, variant
 
 pytestmark = pytest.mark.gpu
 
@@ -132,6 +132,23 @@ def test_lattice_fused_and_separate_forms(ctx, fan_at_outflow):
     table = ctx.prof_read()
     ctx.prof_enable(False)
     assert ("lbm_relax" in table) == fan_at_outflow, sorted(table)
+    fd, fo = d.download(), o.fields()
+    for k in ("u", "v", "w", "p"):
+        assert np.array_equal(fd[k], fo[k]), k
+    d.close()
+
+
+def test_lattice_on_stretched_grid_equals_oracle(ctx):
+    """A stretched scene: the lattice steps as on a uniform one, and the
+    divergence it reports divides by the stretched widths (LBMSolver.cpp:390-392)."""
+    sc = stretched(build_scene(SMALL_THERMAL), 1.15)
+    tau, dt = 0.8, 0.01
+    d = _device(ctx, sc, tau, dt)
+    d.lbm_attach(tau, dt, 0.0, sc.lx / sc.nx)
+    o = OracleLBM(sc, tau, dt, 0.0)
+    for _ in range(15):
+        rd, ro = d.lbm_step(), o.step()
+        assert np.array_equal(_res(rd), _res(ro)), (rd, ro)
     fd, fo = d.download(), o.fields()
     for k in ("u", "v", "w", "p"):
         assert np.array_equal(fd[k], fo[k]), k

@@ -19,7 +19,7 @@ import pytest
 from conftest import GOLDEN
 from oracle.pyoracle import INFLOW, OUTFLOW, SOLID, OracleLBM, RefLBM, SceneArrays, ref_available
 from oracle.scene_flags import build_scene
-from scenes import BASIC_ROOM, SMALL_LW, SMALL_THERMAL
+from scenes import BASIC_ROOM, SMALL_LW, SMALL_THERMAL, stretched
 
 CASES = {"thermal": (SMALL_THERMAL, 0.8, 0.01), "basic": (BASIC_ROOM, 0.6, 0.01)}
 
@@ -67,9 +67,12 @@ def _edge_room(fan_at_outflow: bool):
 
 
 @pytest.mark.skipif(not ref_available(), reason="oracle/_ref not built (reference tree absent)")
-@pytest.mark.parametrize("which", ["lw", "random", "edge", "edge_fan_at_outflow"])
+@pytest.mark.parametrize("which", ["lw", "random", "edge", "edge_fan_at_outflow", "stretched"])
 def test_oracle_lbm_matches_reference(which):
+    """(stretched: the lattice itself is uniform, but the divergence it reports
+    divides by the stretched widths, LBMSolver.cpp:390-392)"""
     sc = {"lw": lambda: build_scene(SMALL_LW), "random": _random_room,
+          "stretched": lambda: stretched(build_scene(SMALL_THERMAL), 1.15),
           "edge": lambda: _edge_room(False), "edge_fan_at_outflow": lambda: _edge_room(True)}[which]()
     o, r = OracleLBM(sc, 0.7, 0.005, 0.0), RefLBM(sc, 0.7, 0.005, 0.0)
     for _ in range(15):
