@@ -137,3 +137,26 @@ def test_gangs_equal_alone_bitwise(ctx, gangs):
     # five mixed scenes: fewer launches than their own streams would issue
     # (all five share launches with one gang; 2 + 2 + 1 with three)
     assert launches < (0.6 if gangs == 1 else 0.9) * ops, (launches, ops)
+
+
+@pytest.mark.parametrize("gangs", [0, 1])
+def test_stretched_scenes_served_equal_alone_bitwise(ctx, gangs):
+    """Stretched rooms (the volume-weighted level-0 kernels) beside uniform
+    ones, on their own streams and in one gang: each equals its alone run."""
+    base = dict(dt=0.005, max_steps=K, convergence_tol=0.0, advection="quick", buoyancy=True,
+                beta=3.4e-3, alpha=2.1e-5, T_initial=26.0)
+    cfgs = [random_scene(107, grid=(40, 32, 20), n_fans=2, n_obstacles=3, precision="mixed", **base),
+            random_scene(108, grid=(40, 32, 20), n_fans=1, n_obstacles=2, precision="mixed", **base),
+            random_scene(109, grid=(24, 36, 18), n_fans=1, n_obstacles=1, precision="mixed", **base)]
+    cfgs[0].room.stretch = 1.04
+    cfgs[2].room.stretch = 1.06
+    alone = [_alone(ctx, c, K, tuned=True) for c in cfgs]
+    srv = Server(ctx, slots=len(cfgs), keep_fields=True, gangs=gangs)
+    done = srv.run([Request(c, K) for c in cfgs], realtime=False)
+    ctx.serve_gangs(0)
+    for (fa, la), c in zip(alone, sorted(done, key=lambda c: c.index)):
+        assert c.steps == K
+        for k in ("u", "v", "w", "p", "T"):
+            assert np.array_equal(fa[k], c.fields[k]), (c.index, k)
+        assert la["pressure_residual"] == c.last["pressure_residual"], c.index
+        assert not c.last["nonfinite"]
