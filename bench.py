@@ -201,12 +201,15 @@ def step_errors(ctx, ps64, d_mixed, sealed):
     return out
 
 
-def serve_bench(ctx, n_c4: int, steps_c4: int, n_c5: int, steps_c5: int, gangs: int = 0):
+def serve_bench(ctx, n_c4: int, steps_c4: int, n_c5: int, steps_c5: int, gangs: int = 0,
+                slots: int = 6):
     """Continuous batching samples: c4 (uneven rooms all submitted at once)
     and c5 (a seeded Poisson stream at 50 requests/s), mixed MG-PCG form;
     `gangs` groups of merged launches (0: every scene on its own stream).
     Both in arrival order, and c5 again with the smallest requests admitted
-    first."""
+    first. `slots` scenes are resident at once: 6 measured best on MI355X
+    (c4 2.0e8 cell-updates/s, p50 0.66 s against 1.9e8, 0.81 s with 16; c5
+    the same throughput, p50 3.0 s against 3.9 s)."""
     from ff_room import capi
     from ff_room.generator import workload_c4, workload_c5
     from ff_room.serve import Request, Server, summary
@@ -214,19 +217,19 @@ def serve_bench(ctx, n_c4: int, steps_c4: int, n_c5: int, steps_c5: int, gangs: 
     kw = dict(precision=capi.PREC_MIXED, poisson_method=capi.POISSON_MGPCG,
               poisson_tol=1e-10, poisson_max_iter=500)
     cfgs = workload_c4(n_scenes=n_c4, steps=steps_c4, precision="mixed")
-    srv = Server(ctx, slots=16, gangs=gangs, **kw)
+    srv = Server(ctx, slots=slots, gangs=gangs, **kw)
     done = srv.run([Request(c, steps_c4) for c in cfgs], realtime=False)
     res["c4"] = dict(summary(done), gangs=gangs, gang_stats=ctx.gang_stats(),
                      sample=f"{n_c4} seeded rooms (grids 32x32x16..160x128x64), "
-                     f"{steps_c4} steps each, all submitted at t=0, 16 slots")
+                     f"{steps_c4} steps each, all submitted at t=0, {slots} slots")
     stream = list(workload_c5(n_requests=n_c5, steps=steps_c5, precision="mixed"))
-    srv = Server(ctx, slots=16, gangs=gangs, **kw)
+    srv = Server(ctx, slots=slots, gangs=gangs, **kw)
     done = srv.run([Request(c, steps_c5, t) for t, c in stream], realtime=True)
     res["c5"] = dict(summary(done), gangs=gangs, gang_stats=ctx.gang_stats(),
                      sample=f"first {n_c5} requests of the seeded stream "
                      f"(50 req/s, grids 24^3..256x204x80, <= {steps_c5} steps or "
-                     f"max|du|/dt < 1e-4), 16 slots")
-    srv = Server(ctx, slots=16, gangs=gangs, policy="shortest", **kw)
+                     f"max|du|/dt < 1e-4), {slots} slots")
+    srv = Server(ctx, slots=slots, gangs=gangs, policy="shortest", **kw)
     done = srv.run([Request(c, steps_c5, t) for t, c in stream], realtime=True)
     res["c5_shortest_first"] = dict(summary(done), gangs=gangs,
                                     sample="the c5 sample, admitted smallest work (cells x step "
@@ -448,6 +451,7 @@ def main():
                     help="replay inner iterations from HIP graphs (0 for tracing profilers)")
     ap.add_argument("--gangs", type=int, default=0,
                     help="serving samples: groups of merged launches (0: a stream per scene)")
+    ap.add_argument("--slots", type=int, default=6, help="scenes resident at once when serving")
     ap.add_argument("--c4", type=int, default=64, help="rooms in the batched sample")
     ap.add_argument("--c4-steps", type=int, default=20)
     ap.add_argument("--c5", type=int, default=40, help="requests in the stream sample")
@@ -555,7 +559,8 @@ def main():
     if args.mode == "mixed" and not args.no_next:
         out["next_rows"] = next_rows(ctx, args)
     if args.mode == "mixed" and not args.no_serve:
-        out["serving"] = serve_bench(ctx, args.c4, args.c4_steps, args.c5, args.c5_steps, args.gangs)
+        out["serving"] = serve_bench(ctx, args.c4, args.c4_steps, args.c5, args.c5_steps, args.gangs,
+                                     args.slots)
         for k, v in out["serving"].items():
             log(f"serving {k}: {v['cell_updates_per_s']:.4g} cell-updates/s, p50 "
                 f"{v['p50_latency_s']:.3f} s, p99 {v['p99_latency_s']:.3f} s over {v['requests']} requests")

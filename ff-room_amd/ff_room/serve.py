@@ -78,7 +78,7 @@ class Server:
     in that many groups of merged launches (ffr_serve_gangs); 0 runs every
     scene on its own stream."""
 
-    def __init__(self, ctx: capi.Context, slots: int = 16, hbm_budget: Optional[int] = None,
+    def __init__(self, ctx: capi.Context, slots: int = 6, hbm_budget: Optional[int] = None,
                  keep_fields: bool = False, tuned: bool = True, gangs: int = 0,
                  policy: str = "fifo", **overrides):
         if policy not in ("fifo", "shortest"):
