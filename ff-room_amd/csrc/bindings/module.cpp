@@ -1,9 +1,11 @@
 // module.cpp - the `_ffroom_core` extension module, with every name and
-// signature of the reference's bindings (cpp/src/bindings/module.cpp:15-210)
-// except LBMSolver, which this build does not provide. Field views are
-// zero-copy and writable as in the reference; because the state lives on the
-// GPU, taking a view first pulls the current fields into the Grid and marks
-// them modified, so edits made through it reach the GPU before the next step.
+// signature of the reference's bindings (cpp/src/bindings/module.cpp:15-210),
+// LBMSolver included. Field views are zero-copy and writable as in the
+// reference. The state lives on the GPU, so taking a view first pulls the
+// current fields into the Grid and marks them modified; while any view is
+// alive, every step leaves the Grid vectors current and uploads them again
+// before the next step, so a view held across steps shows the latest fields
+// and edits made through it reach the GPU, as the reference's live vectors do.
 #include <pybind11/functional.h>
 #include <pybind11/numpy.h>
 #include <pybind11/pybind11.h>
@@ -22,12 +24,26 @@ namespace py = pybind11;
 using namespace ffroom;
 
 namespace {
+// Base object of a field view: keeps the Grid alive and counts the view in
+// Grid::live_views until the array is released.
+struct ViewOwner {
+    py::object grid;
+    Grid *g;
+};
+
 template <typename T>
 py::array_t<double> view(Grid &g, std::vector<double> &vec, std::vector<ssize_t> shape)
 {
     g.sync_to_host();
     g.mark_host_modified();
-    return py::array_t<double>(shape, vec.data(), py::cast(g));
+    auto *owner = new ViewOwner{py::cast(g), &g};
+    g.live_views++;
+    py::capsule base(owner, [](void *p) {
+        auto *o = static_cast<ViewOwner *>(p);
+        o->g->live_views--;
+        delete o;
+    });
+    return py::array_t<double>(shape, vec.data(), base);
 }
 }  // namespace
 

@@ -83,7 +83,7 @@ int ffr_lbm_attach(ffr_scene *s, double tau, double dt, double convergence_tol, 
         return FFR_E_STATE;
     }
     HIP_CHECK_CTX(c, hipSetDevice(c->device), FFR_E_DEVICE);
-    hipStreamSynchronize(s->stream);
+    HIP_CHECK_CTX(c, hipStreamSynchronize(s->stream), FFR_E_DEVICE);
     lbm_release(s);
     const size_t nc = s->nc, bytes = (2 * 19 * nc + 3 * nc) * sizeof(double) + nc * sizeof(uint32_t);
     if (c->in_use + bytes > c->budget) {

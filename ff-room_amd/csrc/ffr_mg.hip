@@ -549,11 +549,26 @@ static bool graphs_ready(ffr_scene *s)
     return M.g_iter && M.g_start;
 }
 
+// A replay the runtime refused: the scene drops its graphs and launches
+// eagerly from then on. The iteration state is on the device, so an
+// iteration is either replayed or launched, never half of each.
+static void drop_graphs(ffr_scene *s)
+{
+    Multigrid &M = s->mg;
+    if (M.g_iter) hipGraphExecDestroy(M.g_iter);
+    if (M.g_start) hipGraphExecDestroy(M.g_start);
+    M.g_iter = M.g_start = nullptr;
+    hipGetLastError();
+}
+
 static void launch_iterations(ffr_scene *s, int k)
 {
     if (k > 0 && graphs_ready(s)) {
-        for (int i = 0; i < k; i++) hipGraphLaunch(s->mg.g_iter, s->stream);
-        return;
+        int i = 0;
+        while (i < k && hipGraphLaunch(s->mg.g_iter, s->stream) == hipSuccess) i++;
+        if (i == k) return;
+        drop_graphs(s);
+        k -= i;
     }
     if (k > 0 && t_rec) {  // recorded once, repeated with the same arguments
         const size_t b = t_rec->ops.size();
@@ -567,8 +582,11 @@ static void launch_iterations(ffr_scene *s, int k)
 // inner solve: z = M r, p = z, then k CG steps
 static void enqueue_inner(ffr_scene *s, int k)
 {
-    if (graphs_ready(s)) hipGraphLaunch(s->mg.g_start, s->stream);
-    else inner_start(s);
+    const bool replay = graphs_ready(s);
+    if (!replay || hipGraphLaunch(s->mg.g_start, s->stream) != hipSuccess) {
+        if (replay) drop_graphs(s);
+        inner_start(s);
+    }
     launch_iterations(s, k);
 }
 

@@ -519,8 +519,17 @@ int ffr_scene_set_params(ffr_scene *s, const ffr_params *p)
         return FFR_E_STATE;
     }
     const bool need_mg = scene_uses_mg(*p);
+    // the hierarchy (levels, storage, sweep counts, coarsest inverse, tail
+    // layout and captured graphs) is built from these fields at creation
+    const ffr_params &o = s->prm;
+    const bool mg_changed = o.mg_levels != p->mg_levels || o.mg_bf16 != p->mg_bf16 ||
+                            o.mg_pre_sweeps != p->mg_pre_sweeps ||
+                            o.mg_level_sweeps != p->mg_level_sweeps ||
+                            o.mg_coarse_sweeps != p->mg_coarse_sweeps ||
+                            o.mg_tail_kb != p->mg_tail_kb;
     s->prm = *p;
     s->pred = SolvePred{};  // iteration counts of other settings do not carry over
+    if (s->mg.levels && mg_changed) mg_destroy(s);
     if (need_mg && !s->mg.levels) return mg_create(s);
     return FFR_OK;
 }
@@ -642,6 +651,65 @@ int ffr_apply_laplacian(ffr_scene *s, const double *x, double *Ax)
     launch_laplacian(st, s->geo, s->d, s->Ad);
     HIP_CHECK_CTX(s->ctx, hipMemcpyAsync(Ax, s->Ad, s->nc * 8, hipMemcpyDeviceToHost, st), FFR_E_DEVICE);
     HIP_CHECK_CTX(s->ctx, hipStreamSynchronize(st), FFR_E_DEVICE);
+    return FFR_OK;
+}
+
+// The scene's u, v, w copied into its u*, v*, w* buffers: the single-phase
+// entry points below take the current velocity as the predictor's output.
+static int state_to_star(ffr_scene *s)
+{
+    const size_t sz = s->fp64 ? 8 : 4;
+    hipStream_t st = s->stream;
+    HIP_CHECK_CTX(s->ctx, hipMemcpyAsync(s->us, s->u, s->nu * sz, hipMemcpyDeviceToDevice, st), FFR_E_DEVICE);
+    HIP_CHECK_CTX(s->ctx, hipMemcpyAsync(s->vs, s->v, s->nv * sz, hipMemcpyDeviceToDevice, st), FFR_E_DEVICE);
+    HIP_CHECK_CTX(s->ctx, hipMemcpyAsync(s->ws, s->w, s->nw * sz, hipMemcpyDeviceToDevice, st), FFR_E_DEVICE);
+    return FFR_OK;
+}
+
+int ffr_build_rhs(ffr_scene *s, double *rhs)
+{
+    if (!s || !rhs) return FFR_E_INVALID;
+    hipSetDevice(s->ctx->device);
+    if (task_busy(s)) {
+        s->ctx->fail("ffr_build_rhs: the scene has a step in flight");
+        return FFR_E_STATE;
+    }
+    hipStream_t st = s->stream;
+    const StepParams sp = make_step_params(s->prm);
+    int rc = state_to_star(s);
+    if (rc != FFR_OK) return rc;
+    if (s->fp64) launch_rhs<double>(st, s->geo, sp, (double *)s->us, (double *)s->vs, (double *)s->ws, s->b);
+    else launch_rhs<float>(st, s->geo, sp, (float *)s->us, (float *)s->vs, (float *)s->ws, s->b);
+    HIP_CHECK_CTX(s->ctx, hipGetLastError(), FFR_E_DEVICE);
+    HIP_CHECK_CTX(s->ctx, hipMemcpyAsync(rhs, s->b, s->nc * 8, hipMemcpyDeviceToHost, st), FFR_E_DEVICE);
+    HIP_CHECK_CTX(s->ctx, hipStreamSynchronize(st), FFR_E_DEVICE);
+    return FFR_OK;
+}
+
+int ffr_apply_correction(ffr_scene *s, const double *p, double *velocity_change)
+{
+    if (!s || !p) return FFR_E_INVALID;
+    hipSetDevice(s->ctx->device);
+    if (task_busy(s)) {
+        s->ctx->fail("ffr_apply_correction: the scene has a step in flight");
+        return FFR_E_STATE;
+    }
+    hipStream_t st = s->stream;
+    const StepParams sp = make_step_params(s->prm);
+    HIP_CHECK_CTX(s->ctx, hipMemcpyAsync(s->p, p, s->nc * 8, hipMemcpyHostToDevice, st), FFR_E_DEVICE);
+    HIP_CHECK_CTX(s->ctx, hipMemsetAsync(&s->rec->vmax_bits, 0, sizeof(unsigned long long), st), FFR_E_DEVICE);
+    int rc = state_to_star(s);
+    if (rc != FFR_OK) return rc;
+    if (s->fp64)
+        launch_correct<double>(st, s->geo, sp, (double *)s->us, (double *)s->vs, (double *)s->ws, s->p,
+                               (double *)s->u, (double *)s->v, (double *)s->w, s->rec);
+    else
+        launch_correct<float>(st, s->geo, sp, (float *)s->us, (float *)s->vs, (float *)s->ws, s->p,
+                              (float *)s->u, (float *)s->v, (float *)s->w, s->rec);
+    HIP_CHECK_CTX(s->ctx, hipGetLastError(), FFR_E_DEVICE);
+    HIP_CHECK_CTX(s->ctx, hipMemcpyAsync(s->h_rec, s->rec, sizeof(StepRec), hipMemcpyDeviceToHost, st), FFR_E_DEVICE);
+    HIP_CHECK_CTX(s->ctx, hipStreamSynchronize(st), FFR_E_DEVICE);
+    if (velocity_change) *velocity_change = bits_to_double(s->h_rec->vmax_bits);
     return FFR_OK;
 }
 

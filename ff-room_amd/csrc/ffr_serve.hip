@@ -161,9 +161,10 @@ struct ffr_ctx::Server {
                                         [&](uint64_t id) { return tickets[id].done; }),
                          members[g].end());
         if (recs.empty()) return FFR_OK;
-        const int rc = G.launch_round(recs);
+        int rc = G.launch_round(recs);
         if (rc != FFR_OK) {
-            hipStreamSynchronize(G.stream);
+            // the members' launches that did go out finish before they are failed
+            if (hipStreamSynchronize(G.stream) != hipSuccess) rc = FFR_E_DEVICE;
             for (const uint64_t id : members[g]) {
                 tickets[id].scene->task.phase = StepTask::IDLE;
                 finish(tickets[id], rc);

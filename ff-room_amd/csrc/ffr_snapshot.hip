@@ -127,12 +127,28 @@ int ffr_snapshot_begin(ffr_scene *s, int kind, int *slot)
                     bytes, c->budget - c->in_use);
             return FFR_E_NOMEM;
         }
-        HIP_CHECK_CTX(c, hipMalloc(&q.dev, bytes), FFR_E_NOMEM);
+        // taken into locals and given to the slot only when every part exists
+        void *dev = nullptr, *host = nullptr;
+        hipEvent_t copied = nullptr, landed = nullptr;
+        hipError_t e = hipMalloc(&dev, bytes);
+        if (e == hipSuccess) e = hipHostMalloc(&host, bytes);
+        if (e == hipSuccess) e = hipEventCreateWithFlags(&copied, hipEventDisableTiming);
+        if (e == hipSuccess) e = hipEventCreateWithFlags(&landed, hipEventDisableTiming);
+        if (e != hipSuccess) {
+            if (landed) hipEventDestroy(landed);
+            if (copied) hipEventDestroy(copied);
+            if (host) hipHostFree(host);
+            if (dev) hipFree(dev);
+            c->fail("ffr_snapshot_begin: allocating a %zu-byte slot failed: %s", bytes,
+                    hipGetErrorString(e));
+            return FFR_E_NOMEM;
+        }
+        q.dev = dev;
+        q.host = host;
+        q.copied = copied;
+        q.landed = landed;
         c->in_use += bytes;
         s->snap_bytes = bytes;
-        HIP_CHECK_CTX(c, hipHostMalloc(&q.host, bytes), FFR_E_NOMEM);
-        HIP_CHECK_CTX(c, hipEventCreateWithFlags(&q.copied, hipEventDisableTiming), FFR_E_DEVICE);
-        HIP_CHECK_CTX(c, hipEventCreateWithFlags(&q.landed, hipEventDisableTiming), FFR_E_DEVICE);
     }
     if (kind == FFR_SNAP_CELLS) {
         double *vel = (double *)q.dev, *speed = vel + 3 * s->nc, *Tc = speed + s->nc;

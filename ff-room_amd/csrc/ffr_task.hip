@@ -188,6 +188,18 @@ static void build_result(ffr_scene *s)
 
 bool task_busy(const ffr_scene *s) { return s->task.phase != StepTask::IDLE; }
 
+// After a failed enqueue: waits for the scene's stream so that the scene
+// stays usable; a stream that cannot be drained is reported as a device error.
+static int idle_after_error(ffr_scene *s, int rc)
+{
+    const hipError_t e = hipStreamSynchronize(s->stream);
+    if (e != hipSuccess) {
+        s->ctx->fail("hipStreamSynchronize after status %d failed: %s", rc, hipGetErrorString(e));
+        return FFR_E_DEVICE;
+    }
+    return rc;
+}
+
 int task_begin(ffr_scene *s, bool solve_only)
 {
     if (task_busy(s)) {
@@ -205,11 +217,7 @@ int task_begin(ffr_scene *s, bool solve_only)
         else enqueue_predict<float>(s);
     }
     const int rc = solve_begin(s);
-    if (rc != FFR_OK) {
-        // leave the stream idle so that the scene stays usable
-        hipStreamSynchronize(s->stream);
-        return rc;
-    }
+    if (rc != FFR_OK) return idle_after_error(s, rc);
     t.phase = StepTask::SOLVE;
     return FFR_OK;
 }
@@ -227,8 +235,8 @@ int task_resume(ffr_scene *s, bool *done)
             t.phase = StepTask::RESULT;
         }
         if (rc != FFR_OK) {
-            if (!t_rec) hipStreamSynchronize(s->stream);
             t.phase = StepTask::IDLE;
+            if (!t_rec) rc = idle_after_error(s, rc);
         }
         return rc;
     }

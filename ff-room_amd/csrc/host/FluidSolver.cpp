@@ -128,7 +128,9 @@ StepResult FluidSolver::step()
     ffr_check(ffr_step(dev_->get(), &r), "ffr_step");
     host_stale_ = true;
     step_ = r.step;
-    if (params_.sync_host_every_step) sync_to_host();
+    if (params_.sync_host_every_step || grid_.live_views > 0) sync_to_host();
+    // a held view may be written before the next step: upload it then
+    if (grid_.live_views > 0) host_dirty_ = true;
     StepResult out;
     out.step = r.step;
     out.velocity_change = r.velocity_change;

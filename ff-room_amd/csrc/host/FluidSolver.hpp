@@ -44,7 +44,7 @@ struct FluidSolverParams {
     PoissonMethod poisson_method = PoissonMethod::AUTO;
     int mg_levels = 0;
     bool mg_bf16 = true;
-    double inner_tol = 1e-3;
+    double inner_tol = 1e-4;
     int inner_max_iter = 60;
     bool sync_host_every_step = false;  // download the fields after every step()
 };

@@ -51,6 +51,11 @@ struct Grid {
     // attached solver rebuilds its device geometry before the next step.
     uint64_t geometry_version = 0;
     DeviceMirror *attached = nullptr;
+    // Field views (the Python u/v/w/p/T arrays over these vectors) still
+    // alive: while there are any, a step leaves the vectors current and
+    // uploads them again before the next step, so a held view behaves as the
+    // reference's view of its live vector.
+    int live_views = 0;
     // Makes the host vectors current (pulls from the GPU if needed).
     void sync_to_host();
     // Declares that host vectors were (or may be) written: the next step

@@ -72,6 +72,10 @@ StepResult LBMSolver::step()
     ffr_check(ffr_lbm_step(dev_->get(), &r), "ffr_lbm_step");
     host_stale_ = true;
     step_ = r.step;
+    if (grid_.live_views > 0) {  // held views stay current; their edits go up next step
+        sync_to_host();
+        host_dirty_ = true;
+    }
     StepResult out;
     out.step = r.step;
     out.velocity_change = r.velocity_change;

@@ -140,6 +140,8 @@ SIGNATURES = {
     "ffr_T_mean": (_I, [_P, _dp]),
     "ffr_apply_laplacian": (_I, [_P, _dp, _dp]),
     "ffr_poisson_solve": (_I, [_P, _dp, _dp, C.POINTER(SolveInfo)]),
+    "ffr_build_rhs": (_I, [_P, _dp]),
+    "ffr_apply_correction": (_I, [_P, _dp, _dp]),
     "ffr_submit": (_I, [_P, _P, C.c_int, C.POINTER(C.c_uint64)]),
     "ffr_poll": (_I, [_P, C.c_uint64, C.POINTER(StepResult), C.POINTER(C.c_int)]),
     "ffr_drain": (_I, [_P]),
@@ -440,6 +442,22 @@ class DeviceScene:
                        "ffr_poisson_solve")
         return p, dict(residual=info.residual, iterations=info.iterations,
                        outer_iterations=info.outer_iterations)
+
+    def build_rhs(self) -> np.ndarray:
+        """The step's right-hand side from the current u, v, w taken as u*
+        (ffr_build_rhs)."""
+        out = np.empty(self.nx * self.ny * self.nz)
+        self.ctx.check(self.ctx._l.ffr_build_rhs(self.h, _ptr(out)), "ffr_build_rhs")
+        return out
+
+    def apply_correction(self, p: np.ndarray) -> float:
+        """Projection of the current u, v, w (taken as u*) with pressure p and
+        the post-solve masks (ffr_apply_correction); returns max |u - u*|."""
+        p = np.ascontiguousarray(p, dtype=np.float64)
+        dv = C.c_double()
+        self.ctx.check(self.ctx._l.ffr_apply_correction(self.h, _ptr(p), C.byref(dv)),
+                       "ffr_apply_correction")
+        return dv.value
 
     def divergence_max(self) -> float:
         d = C.c_double()

@@ -245,6 +245,20 @@ int ffr_apply_laplacian(ffr_scene *scene, const double *x, double *Ax);
 /* Solves A p = rhs with the scene's poisson settings; p is written back. */
 int ffr_poisson_solve(ffr_scene *scene, const double *rhs, double *p, ffr_solve_info *info);
 
+/* ---- single phases of the step, on the scene's current state ------------
+ * Each takes the scene's current u, v, w as the predictor's output u*, v*, w*
+ * and runs the step's own kernel for one phase (between steps only):
+ * ffr_build_rhs: rhs = -(rho/dt) div(u*) on non-SOLID cells, 0 on SOLID
+ *   (FluidSolver::build_poisson_rhs, FluidSolver.cpp:311-330), nx*ny*nz.
+ * ffr_apply_correction: p (host, nx*ny*nz) becomes the scene's pressure, and
+ *   u, v, w := u* - (dt/rho) grad p on interior faces, then the no-slip, fan
+ *   and outflow masks (FluidSolver::apply_correction, FluidSolver.cpp:332-371,
+ *   then BoundaryManager::apply_noslip / apply_inflow / apply_outflow as
+ *   FluidSolver::step applies them after the solve, FluidSolver.cpp:511-515);
+ *   *velocity_change (may be NULL) = max |u - u*| over all faces. */
+int ffr_build_rhs(ffr_scene *scene, double *rhs);
+int ffr_apply_correction(ffr_scene *scene, const double *p, double *velocity_change);
+
 /* ---- lattice Boltzmann (LBMSolver, LBMSolver.hpp:17-39) ------------------
  * The reference's second solver method: D3Q19, BGK with relaxation time tau,
  * bounce-back at walls and SOLID cells with the reference's direction

@@ -31,6 +31,29 @@ def snapped_axis(vec) -> int:
     return best
 
 
+def stretched_faces(n: int, length: float, ratio: float):
+    """scene.py:34-59 (_stretched_faces): n+1 face coordinates. Uniform
+    i*L/n when |ratio-1| < 1e-6; otherwise the first n//2 widths form the
+    geometric series dx1*ratio^i with dx1 = (L/2)(ratio-1)/(ratio^(n//2)-1),
+    the next n//2 widths repeat the realised ones in reverse, an odd n splits
+    the first right-half cell at its midpoint, and every face is multiplied by
+    L / (last face)."""
+    if abs(ratio - 1.0) < 1e-6:
+        return [i * length / n for i in range(n + 1)]
+    m = n // 2
+    dx1 = (length / 2.0) * (ratio - 1.0) / (ratio ** m - 1.0)
+    xs = [0.0]
+    for i in range(m):
+        xs.append(xs[-1] + dx1 * ratio ** i)
+    widths = [xs[i + 1] - xs[i] for i in range(m)]
+    for wd in widths[::-1]:
+        xs.append(xs[-1] + wd)
+    if n % 2 == 1:
+        xs.insert(m + 1, (xs[m] + xs[m + 1]) / 2.0)
+    k = length / xs[-1]
+    return [x * k for x in xs]
+
+
 def build_scene(cfg: dict) -> SceneArrays:
     room = cfg["room"]
     (lx, ly, lz), (nx, ny, nz) = room["size"], room["grid"]

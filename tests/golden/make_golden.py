@@ -6,6 +6,7 @@ oracle/_ref/libffref.so (with -ffp-contract=off, serial).
 
 Fixtures (numpy .npz, FP64):
   c1_upwind_cg.npz     basic_room (configuration c1): fields after steps 1, 10, 100
+                       and 500 (the configured run length)
   small_thermal.npz    12x10x8 QUICK + buoyancy + two openings, 20 steps
   small_lw.npz         12x10x8 Lax-Wendroff, 10 steps
   poisson_tiny.npz     6x5x4 mixed-flag grid: rhs, CG p, multigrid p
@@ -14,6 +15,7 @@ Fixtures (numpy .npz, FP64):
                        basic_room: fields after 25 steps, every step's result
 
     python tests/golden/make_golden.py lbm     # only lbm_small.npz
+    python tests/golden/make_golden.py c1      # only c1_upwind_cg.npz
 """
 from __future__ import annotations
 
@@ -91,6 +93,7 @@ def couette():
     np.savez_compressed(os.path.join(OUT, "couette.npz"), result=_res(r), **f)
 
 
+C1_STEPS = (1, 10, 100, 500)
 LBM_CASES = (("thermal", SMALL_THERMAL, 0.8, 0.01), ("basic", BASIC_ROOM, 0.6, 0.01))
 LBM_STEPS = 25
 
@@ -113,7 +116,10 @@ def main():
     if sys.argv[1:] == ["lbm"]:
         lbm_small()
         return
-    trajectory(BASIC_ROOM, (1, 10, 100), "c1_upwind_cg.npz")
+    if sys.argv[1:] == ["c1"]:
+        trajectory(BASIC_ROOM, C1_STEPS, "c1_upwind_cg.npz")
+        return
+    trajectory(BASIC_ROOM, C1_STEPS, "c1_upwind_cg.npz")
     trajectory(SMALL_THERMAL, (20,), "small_thermal.npz")
     trajectory(SMALL_LW, (10,), "small_lw.npz")
     poisson_tiny()

@@ -39,6 +39,37 @@ SMALL_LW = {
 }
 
 
+# examples/summer_cooling.yaml of the reference (30x22x12): two openings on
+# opposite walls, a fan blowing -y, two boxes on the floor.
+SUMMER_COOLING = {
+    "room": {"size": [6.0, 4.5, 2.5], "grid": [30, 22, 12]},
+    "fans": [{"name": "fan_at_window", "position": [3.0, 3.8, 0.9], "direction": [270.0, 0.0],
+              "velocity": 2.5, "radius": 0.15}],
+    "obstacles": [{"name": "bed", "bbox_min": [0.3, 0.3, 0.0], "bbox_max": [2.0, 2.5, 0.5]},
+                  {"name": "desk", "bbox_min": [3.5, 0.3, 0.0], "bbox_max": [5.2, 1.5, 0.75]}],
+    "openings": [{"name": "window_north", "wall": "north", "center": [3.0, 1.2],
+                  "size": [0.9, 1.1]},
+                 {"name": "door_south", "wall": "south", "center": [1.5, 1.0],
+                  "size": [0.8, 2.0]}],
+    "solver": {"dt": 0.05, "max_steps": 5000, "convergence_tol": 1.0, "T_initial": 32.0,
+               "T_outside": 25.0, "T_target": 30.0, "buoyancy": True},
+}
+
+# The c1 room with fans whose boxes meet the walls: one at the east wall
+# (its radius box reaches past y = 0 and z = Lz), one against the north wall
+# reaching past x = 0, one placed above the ceiling blowing up.
+FANS_AT_WALLS = {
+    "room": {"size": [5.0, 4.0, 2.5], "grid": [25, 20, 12]},
+    "fans": [{"name": "east", "position": [4.99, 0.12, 2.4], "direction": [0.0, 0.0],
+              "velocity": 3.0, "radius": 0.3},
+             {"name": "north", "position": [0.2, 3.95, 1.2], "direction": [90.0, 0.0],
+              "velocity": 2.0, "radius": 0.25},
+             {"name": "up", "position": [2.5, 2.0, 2.6], "direction": [0.0, 90.0],
+              "velocity": 1.5, "radius": 0.15}],
+    "solver": {"dt": 0.01},
+}
+
+
 def solver_params(cfg: dict) -> dict:
     """FluidSolverParams fields as the reference SolverBridge maps them
     (python/ff_room/solver_bridge.py:108-127, config.py defaults)."""
@@ -69,7 +100,7 @@ def stretched(sc, ratio: float):
     stay as they are: the reference maps positions to cells with
     int(x / L * N) whatever the faces."""
     import numpy as np
-    from ff_room.scene import _stretched_faces
-    sc.faces = tuple(np.array(_stretched_faces(n, L, ratio), dtype=np.float64)
+    from oracle.scene_flags import stretched_faces
+    sc.faces = tuple(np.array(stretched_faces(n, L, ratio), dtype=np.float64)
                      for n, L in ((sc.nx, sc.lx), (sc.ny, sc.ly), (sc.nz, sc.lz)))
     return sc

@@ -95,7 +95,7 @@ def test_cg_solution_matches_oracle(ctx, cfg):
 
 
 @pytest.mark.parametrize("name,cfg,steps,tol", [
-    ("c1_upwind_cg.npz", BASIC_ROOM, (1, 10, 100), 1e-5),
+    ("c1_upwind_cg.npz", BASIC_ROOM, (1, 10, 100, 500), 1e-5),
     ("small_thermal.npz", SMALL_THERMAL, (20,), 1e-5),
     ("small_lw.npz", SMALL_LW, (10,), 1e-5),
 ])

@@ -28,7 +28,7 @@ def _golden(name):
 
 
 @pytest.mark.parametrize("cfg,name,steps", [
-    (BASIC_ROOM, "c1_upwind_cg.npz", (1, 10, 100)),
+    (BASIC_ROOM, "c1_upwind_cg.npz", (1, 10, 100, 500)),
     (SMALL_THERMAL, "small_thermal.npz", (20,)),
     (SMALL_LW, "small_lw.npz", (10,)),
 ])
