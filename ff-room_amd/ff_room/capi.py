@@ -142,6 +142,7 @@ SIGNATURES = {
     "ffr_poisson_solve": (_I, [_P, _dp, _dp, C.POINTER(SolveInfo)]),
     "ffr_build_rhs": (_I, [_P, _dp]),
     "ffr_apply_correction": (_I, [_P, _dp, _dp]),
+    "ffr_poisson_solve_noalloc": (_I, [_P, _dp, _dp, C.POINTER(SolveInfo)]),
     "ffr_submit": (_I, [_P, _P, C.c_int, C.POINTER(C.c_uint64)]),
     "ffr_poll": (_I, [_P, C.c_uint64, C.POINTER(StepResult), C.POINTER(C.c_int)]),
     "ffr_drain": (_I, [_P]),
@@ -440,6 +441,18 @@ class DeviceScene:
         info = SolveInfo()
         self.ctx.check(self.ctx._l.ffr_poisson_solve(self.h, _ptr(rhs), _ptr(p), C.byref(info)),
                        "ffr_poisson_solve")
+        return p, dict(residual=info.residual, iterations=info.iterations,
+                       outer_iterations=info.outer_iterations)
+
+    def poisson_noalloc(self, rhs: np.ndarray):
+        """The CG solve with the library's fixed scratch array
+        (ffr_poisson_solve_noalloc): no device allocation."""
+        rhs = np.ascontiguousarray(rhs, dtype=np.float64)
+        p = np.empty_like(rhs)
+        info = SolveInfo()
+        self.ctx.check(self.ctx._l.ffr_poisson_solve_noalloc(self.h, _ptr(rhs), _ptr(p),
+                                                             C.byref(info)),
+                       "ffr_poisson_solve_noalloc")
         return p, dict(residual=info.residual, iterations=info.iterations,
                        outer_iterations=info.outer_iterations)
 

@@ -259,6 +259,23 @@ int ffr_poisson_solve(ffr_scene *scene, const double *rhs, double *p, ffr_solve_
 int ffr_build_rhs(ffr_scene *scene, double *rhs);
 int ffr_apply_correction(ffr_scene *scene, const double *p, double *velocity_change);
 
+/* ---- a pressure solve that allocates nothing -----------------------------
+ * No counterpart in the reference. For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): the same solve and result as ffr_poisson_solve on a
+ * scene whose pressure solver is the CG form (FFR_POISSON_CG, or AUTO without
+ * use_multigrid; FFR_E_UNSUPPORTED otherwise), with the staged right-hand
+ * side, the CG vectors, the reduction partials and the iteration scalars in
+ * a fixed device array of the library (one per GPU, about 32 MiB, held until
+ * the process ends) instead of the scene's memory. Scenes of more than
+ * FFR_NOALLOC_MAX_CELLS cells are rejected with FFR_E_INVALID. Calls on one
+ * GPU share its array: the library serialises them within the process, and
+ * calls that overlapped on one GPU would race. rhs and p are host buffers
+ * (nx*ny*nz); p may be NULL; the solution is also the scene's pressure. */
+#define FFR_NOALLOC_MAX_CELLS (1u << 20)
+int ffr_poisson_solve_noalloc(ffr_scene *scene, const double *rhs, double *p,
+                              ffr_solve_info *info);
+
 /* ---- lattice Boltzmann (LBMSolver, LBMSolver.hpp:17-39) ------------------
  * The reference's second solver method: D3Q19, BGK with relaxation time tau,
  * bounce-back at walls and SOLID cells with the reference's direction
