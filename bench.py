@@ -1,6 +1,6 @@
 """Benchmark: ff-room projection steps on one MI355X.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload c2] [--mode mixed|fp64]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload c3] [--mode mixed|fp64]
 
 A step is one full projection time step (FluidSolver::step: predictor,
 buoyancy, masks, pressure solve, projection, temperature, diagnostics) of the
@@ -112,9 +112,9 @@ def roofline(table: dict, d, workload: str):
 # residual tolerance of the plain FP64 CG form per workload: the loosest power
 # of ten whose pressure error (pressure_errors) meets 1e-6, measured on
 # MI355X: c2 4.6e-7 at 1e-8 (7.6e-6 at 1e-7); c3 7.7e-8 at 1e-9 (1.01e-6 at
-# 1e-8). The error is about the condition number times the residual, which
-# grows with the grid.
-FP64_TOL = {"c2": 1e-8, "c3": 1e-9}
+# 1e-8); the c4 rooms 5.2e-6 at 1e-8 (worst of the first 8). The error is
+# about the condition number times the residual.
+FP64_TOL = {"c2": 1e-8, "c3": 1e-9, "c4": 1e-9}
 
 
 def form_params_inplace(p, mode, workload="c2"):
@@ -238,6 +238,87 @@ def serve_bench(ctx, n_c4: int, steps_c4: int, n_c5: int, steps_c5: int, gangs: 
     return res
 
 
+def batched_forms(ctx, n_rooms: int, steps: int, n_err: int, slots: int = 6):
+    """The north-star comparison in its batched form: the same c4 rooms served
+    with continuous batching (all submitted at t = 0) once in the mixed form
+    (FP32 stencils, MG-PCG with FP64 correction to 1e-10) and once in the
+    plain FP64 CG form (the reference algorithm, CG to the loosest tolerance
+    whose pressure error meets 1e-6); aggregate cell-updates/s of each and
+    their ratio. Pressure errors of both forms on the first `n_err` rooms:
+    x* = the room's pressure after the mixed run, rhs = A x* with the FP64
+    operator, each form solving from zero (gauge-free on sealed pockets)."""
+    from types import SimpleNamespace
+    from ff_room import capi, device
+    from ff_room.generator import workload_c4
+    from ff_room.serve import Request, Server, summary
+    cfgs = workload_c4(n_scenes=n_rooms, steps=steps, precision="mixed")
+    forms = {
+        "mixed": dict(precision=capi.PREC_MIXED, poisson_method=capi.POISSON_MGPCG,
+                      poisson_tol=1e-10, poisson_max_iter=500),
+        "fp64_cg": dict(precision=capi.PREC_FP64, poisson_method=capi.POISSON_CG,
+                        poisson_tol=FP64_TOL["c4"], poisson_max_iter=200000),
+    }
+    out = {"sample": f"first {n_rooms} rooms of c4, {steps} steps each, all submitted at t=0, "
+                     f"{slots} slots", "forms": {}}
+    done_mixed = None
+    for name, kw in forms.items():
+        srv = Server(ctx, slots=slots, keep_fields=(name == "mixed"), **kw)
+        done = srv.run([Request(c, steps) for c in cfgs], realtime=False)
+        sm = summary(done)
+        out["forms"][name] = {"cell_updates_per_s": sm["cell_updates_per_s"],
+                              "wall_s": sm["wall_s"], "p50_latency_s": sm["p50_latency_s"],
+                              "p99_latency_s": sm["p99_latency_s"],
+                              "poisson_tol": kw["poisson_tol"]}
+        if name == "mixed":
+            done_mixed = done
+    out["speedup_mixed_vs_fp64_cg"] = (out["forms"]["mixed"]["cell_updates_per_s"] /
+                                       out["forms"]["fp64_cg"]["cell_updates_per_s"])
+    errs = {"mixed": [], "fp64_cg": []}
+    for c, cfg in zip(done_mixed[:n_err], cfgs[:n_err]):
+        xs = c.fields["p"]
+        scenes = {k: device.create(ctx, device.prepare(cfg, on_device=True, **kw))
+                  for k, kw in forms.items()}
+        rhs = scenes["fp64_cg"].laplacian(xs)
+        flags = scenes["fp64_cg"].cell_type()
+        g = cfg.room.grid
+        view = SimpleNamespace(flags=flags, nx=g[0], ny=g[1], nz=g[2])
+        sealed = sealed_components(view)
+        for k, d in scenes.items():
+            x, _ = d.poisson(rhs)
+            errs[k].append(gauge_free_error(x, xs, view, sealed))
+            d.close()
+    out["p_error_max"] = {k: max(v) for k, v in errs.items()}
+    out["p_error_rooms"] = n_err
+    return out
+
+
+def c1_reference_single(steps: int = 500):
+    """The reference's own FluidSolver (oracle/_ref) on c1 (basic_room,
+    25x20x12, upwind, CG tol 1e-6 capped at 1000 iterations), single-threaded
+    as its default `parallel: false`, all `steps` steps (convergence off)."""
+    from oracle.pyoracle import RefSolver, ref_available
+    from oracle.scene_flags import build_scene
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from scenes import BASIC_ROOM, solver_params
+    if not ref_available():
+        return None
+    params = solver_params(BASIC_ROOM)
+    params["convergence_tol"] = 0.0
+    sc = build_scene(BASIC_ROOM)
+    ref = RefSolver(sc, params, use_openmp=False)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        r = ref.step()
+    wall = time.perf_counter() - t0
+    cells = sc.nx * sc.ny * sc.nz
+    return {"value": cells * steps / wall, "unit": "cell-updates/s", "cores": 1,
+            "kind": "reference", "wall_s": wall, "steps": steps,
+            "sample": f"c1 (basic_room 25x20x12, upwind, CG tol 1e-6, cap 1000), all {steps} "
+                      f"steps, the reference's FluidSolver single-threaded (its default); "
+                      f"the reference docs quote 2.25 s for this room on unstated hardware",
+            "last_cg_residual": r["pressure_residual"]}
+
+
 def _scene_arrays(ps):
     """The prepared scene in the oracle's layout (stretched faces included)."""
     from oracle.pyoracle import SceneArrays
@@ -309,13 +390,14 @@ def lattice_bench(ctx, cfg, desc, ps, args):
         if ref_available():
             ref = RefLBM(_scene_arrays(ps), tau, dt, 0.0)
             ref.step()  # warm-up
+            n = 5 if ps.cells < 10_000_000 else 1  # about 10 s of serial work either way
             t0 = time.perf_counter()
-            for _ in range(5):
+            for _ in range(n):
                 ref.step()
             tr = time.perf_counter() - t0
-            out["cpu_baseline"] = {"value": ps.cells * 5 / tr, "unit": "cell-updates/s", "cores": 1,
+            out["cpu_baseline"] = {"value": ps.cells * n / tr, "unit": "cell-updates/s", "cores": 1,
                                    "kind": "reference",
-                                   "sample": f"5 lattice steps after one warm-up step of the same "
+                                   "sample": f"{n} lattice step(s) after one warm-up step of the same "
                                              f"scene, the reference's LBMSolver (serial: it has no "
                                              f"threading); {tr:.2f} s",
                                    "nproc": os.cpu_count(), "cpu_model": _cpu_model()}
@@ -345,9 +427,12 @@ def next_rows(ctx, args):
     rows["lattice"]["roofline"] = {k: lat["roofline"][k] for k in ("kernel", "achieved", "frac", "traffic")}
     if "cpu_baseline" in lat:
         rows["lattice"]["cpu_baseline"] = lat["cpu_baseline"]
-    cfg, desc = workload(args.workload, "mixed", 1.01)
+    # the stretched room on the c2 grid: the reference CG on the stretched
+    # operator converges slowly, and c3's FP64 comparison step would take minutes
+    sw = "c2"
+    cfg, desc = workload(sw, "mixed", 1.01)
     ps = device.prepare(cfg)
-    form_params_inplace(ps.params, "mixed", args.workload)
+    form_params_inplace(ps.params, "mixed", sw)
     ps.params.graphs = args.graphs
     tuned = autotune.lookup(ps.nx, ps.ny, ps.nz, ps.params.precision) if not args.no_tuning else None
     if tuned:
@@ -356,7 +441,7 @@ def next_rows(ctx, args):
     wall, res, _ = timed_steps(d, ctx, args.steps, args.warmup, prof=False)
     d.close()
     ps64 = device.prepare(cfg)
-    form_params_inplace(ps64.params, "fp64", args.workload)
+    form_params_inplace(ps64.params, "fp64", sw)
     d = device.create(ctx, ps64)
     w64, r64, _ = timed_steps(d, ctx, 1, 1, prof=False)
     d.close()
@@ -370,6 +455,40 @@ def next_rows(ctx, args):
                          "speedup_vs_fp64_cg": v / v64}
     log(f"stretched: {v:.4g} cell-updates/s, fp64 CG {v64:.4g} -> {v / v64:.1f}x")
     return rows
+
+
+def workload_row(ctx, name: str, args):
+    """The mixed and the plain FP64 CG forms on another single-scene workload
+    (as `--workload name`): cell-updates/s of `--row-steps` mixed steps after
+    the warm-up, the FP64 CG form over `--fp64-steps`, and their ratio."""
+    from ff_room import autotune, device
+    cfg, desc = workload(name, "mixed")
+    ps = device.prepare(cfg)
+    form_params_inplace(ps.params, "mixed", name)
+    ps.params.graphs = args.graphs
+    tuned = autotune.lookup(ps.nx, ps.ny, ps.nz, ps.params.precision) if not args.no_tuning else None
+    if tuned:
+        autotune.apply(ps.params, tuned)
+    d = device.create(ctx, ps)
+    wall, res, _ = timed_steps(d, ctx, args.row_steps, args.warmup, prof=False)
+    ms, nbytes = d.prof_kernel("mg_smooth_l0", 200)
+    d.close()
+    ps64 = device.prepare(cfg)
+    form_params_inplace(ps64.params, "fp64", name)
+    d = device.create(ctx, ps64)
+    w64, r64, _ = timed_steps(d, ctx, args.fp64_steps, 1, prof=False)
+    d.close()
+    v, v64 = ps.cells * args.row_steps / wall, ps64.cells * args.fp64_steps / w64
+    iters = [r["poisson_iters"] for r in res]
+    return {"workload": desc, "value": v, "unit": "cell-updates/s", "steps": args.row_steps,
+            "ms_per_step": 1e3 * wall / args.row_steps, "inner_iterations": [min(iters), max(iters)],
+            "fp64_cg": {"value": v64, "steps": args.fp64_steps, "poisson_tol": FP64_TOL[name],
+                        "iterations": [r["poisson_iters"] for r in r64]},
+            "speedup_vs_fp64_cg": v / v64,
+            "mg_smooth_l0": {"mean_launch_us": ms * 1e3, "GBps": nbytes / (ms / 1e3) / 1e9,
+                             "frac": nbytes / (ms / 1e3) / HBM_PEAK,
+                             "note": "working set inside the 256 MB Infinity Cache"
+                             if name == "c2" else ""}}
 
 
 def cpu_baseline(ps, steps: int = 2, single: bool = False):
@@ -430,9 +549,9 @@ def cpu_baseline(ps, steps: int = 2, single: bool = False):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", default="c2")
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--workload", default="c3")
     ap.add_argument("--mode", default="mixed", choices=["mixed", "fp64"])
     ap.add_argument("--method", default="projection", choices=["projection", "lbm"],
                     help="lbm: time the D3Q19 lattice solver on the workload's scene instead")
@@ -452,8 +571,16 @@ def main():
     ap.add_argument("--gangs", type=int, default=0,
                     help="serving samples: groups of merged launches (0: a stream per scene)")
     ap.add_argument("--slots", type=int, default=6, help="scenes resident at once when serving")
+    ap.add_argument("--rows", default="c2",
+                    help="other single-scene workloads measured beside the headline (comma list)")
+    ap.add_argument("--row-steps", type=int, default=20)
+    ap.add_argument("--batched", type=int, default=64,
+                    help="c4 rooms timed in both forms with continuous batching (0: skip)")
+    ap.add_argument("--batched-steps", type=int, default=20)
+    ap.add_argument("--c1-reference", type=int, default=500,
+                    help="steps of the reference's single-threaded c1 run (0: skip)")
     ap.add_argument("--c4", type=int, default=64, help="rooms in the batched sample")
-    ap.add_argument("--c4-steps", type=int, default=20)
+    ap.add_argument("--c4-steps", type=int, default=300)
     ap.add_argument("--c5", type=int, default=40, help="requests in the stream sample")
     ap.add_argument("--c5-steps", type=int, default=200)
     ap.add_argument("--kernel-only", default=None,
@@ -525,7 +652,11 @@ def main():
         "dtype": "f32+bf16 stencils/multigrid, f64 outer residual" if args.mode == "mixed" else "f64",
         "data": "synthetic (seeded scene generator, zero initial velocity)",
         "config": {"workload": desc, "cells": ps.cells, "mode": args.mode,
-                   "poisson_tol": ps.params.poisson_tol, "tuning": tuned},
+                   "poisson_tol": ps.params.poisson_tol, "tuning": tuned,
+                   "warm_start": int(ps.params.warm_start) if args.mode == "mixed" else 0,
+                   "note": "the mixed form's step solve starts from the previous step's "
+                           "pressure; the FP64 CG form starts from 0 as the reference does"
+                           if args.mode == "mixed" else ""},
         "roofline": roof,
         "kernels": {k: {"launches": v["launches"], "mean_us": 1e3 * v["total_ms"] / v["launches"],
                         "GBps": (v["bytes_per_launch"] / (v["total_ms"] / v["launches"] / 1e3) / 1e9
@@ -556,6 +687,20 @@ def main():
         log(f"fp64 CG: {v64:.4g} cell-updates/s -> mixed/fp64 = {value / v64:.1f}x")
 
     d_keep.close()
+    if args.mode == "mixed" and not args.no_compare:
+        rows = [r for r in args.rows.split(",") if r and r != args.workload]
+        if rows:
+            out["rows"] = {}
+        for r in rows:
+            out["rows"][r] = workload_row(ctx, r, args)
+            log(f"row {r}: {out['rows'][r]['value']:.4g} cell-updates/s, "
+                f"{out['rows'][r]['speedup_vs_fp64_cg']:.1f}x the FP64 CG form")
+        if args.batched > 0:
+            out["batched_c4"] = batched_forms(ctx, args.batched, args.batched_steps, 8, args.slots)
+            b = out["batched_c4"]
+            log(f"batched c4: mixed {b['forms']['mixed']['cell_updates_per_s']:.4g}, fp64 CG "
+                f"{b['forms']['fp64_cg']['cell_updates_per_s']:.4g} -> "
+                f"{b['speedup_mixed_vs_fp64_cg']:.1f}x; p errors {b['p_error_max']}")
     if args.mode == "mixed" and not args.no_next:
         out["next_rows"] = next_rows(ctx, args)
     if args.mode == "mixed" and not args.no_serve:
@@ -569,6 +714,11 @@ def main():
         log("timing the reference CPU solver on a bounded sample ...")
         out["cpu_baseline"] = cpu_baseline(device.prepare(cfg), single=args.cpu_single)
         log(f"cpu: {out['cpu_baseline']['value']:.4g} cell-updates/s")
+        if args.c1_reference > 0:
+            c1 = c1_reference_single(args.c1_reference)
+            if c1 is not None:
+                out["cpu_baseline"]["c1_single_thread"] = c1
+                log(f"cpu c1 single thread: {c1['steps']} steps in {c1['wall_s']:.2f} s")
     ctx.close()
     print(json.dumps(out), flush=True)
 
