@@ -75,14 +75,16 @@ void launch_correct(hipStream_t s, const Geo &g, const StepParams &sp, const R *
 template <typename R>
 void launch_divmax(hipStream_t s, const Geo &g, const R *u, const R *v, const R *w,
                    StepRec *rec);
+// temperature and tsum return the block count of their launch: the number of
+// SLOT_TSUM partials they wrote
 template <typename R>
-void launch_temperature(hipStream_t s, const Geo &g, const StepParams &sp, const R *u,
-                        const R *v, const R *w, const R *T, R *Tn, StepRec *rec,
-                        RedScratch red);
+unsigned launch_temperature(hipStream_t s, const Geo &g, const StepParams &sp, const R *u,
+                            const R *v, const R *w, const R *T, R *Tn, StepRec *rec,
+                            RedScratch red);
 template <typename R>
-void launch_tsum(hipStream_t s, const Geo &g, const R *T, StepRec *rec, RedScratch red);
-// rec->tsum := sum of the SLOT_TSUM partials written by temperature / tsum.
-void launch_step_finalize(hipStream_t s, const Geo &g, StepRec *rec, RedScratch red);
+unsigned launch_tsum(hipStream_t s, const Geo &g, const R *T, StepRec *rec, RedScratch red);
+// rec->tsum := sum of the G SLOT_TSUM partials written by temperature / tsum.
+void launch_step_finalize(hipStream_t s, StepRec *rec, RedScratch red, unsigned G);
 template <typename R>
 void launch_opening_T(hipStream_t s, const Geo &g, const R *u, const R *v, const R *w, R *T);
 template <typename R>

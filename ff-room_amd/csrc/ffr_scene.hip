@@ -24,6 +24,15 @@ bool valid_desc(const ffr_scene_desc *d, char *err, size_t errn)
                            (uint64_t)d->nx * (d->ny + 1) * d->nz +
                            (uint64_t)d->nx * d->ny * (d->nz + 1);
     if (faces >= (1ull << 31)) { snprintf(err, errn, "grid too large for 32-bit indexing"); return false; }
+    // the kernels read fields through buffer resources with 32-bit byte offsets
+    const uint64_t most = std::max({(uint64_t)(d->nx + 1) * d->ny * d->nz,
+                                    (uint64_t)d->nx * (d->ny + 1) * d->nz,
+                                    (uint64_t)d->nx * d->ny * (d->nz + 1)});
+    if (most * sizeof(double) >= (1ull << 31)) {
+        snprintf(err, errn, "grid too large: a face array of %llu elements exceeds 2 GiB in FP64",
+                 (unsigned long long)most);
+        return false;
+    }
     if (d->n_fans > kMaxFans) { snprintf(err, errn, "at most %d fans per scene", kMaxFans); return false; }
     if (!d->dxv || !d->dyv || !d->dzv) {
         snprintf(err, errn, "spacings are required");
@@ -633,9 +642,9 @@ int ffr_T_mean(ffr_scene *s, double *out)
     if (!s || !out) return FFR_E_INVALID;
     hipSetDevice(s->ctx->device);
     hipStream_t st = s->stream;
-    if (s->fp64) launch_tsum<double>(st, s->geo, (double *)s->T, s->rec, s->red);
-    else launch_tsum<float>(st, s->geo, (float *)s->T, s->rec, s->red);
-    launch_step_finalize(st, s->geo, s->rec, s->red);
+    const unsigned G = s->fp64 ? launch_tsum<double>(st, s->geo, (double *)s->T, s->rec, s->red)
+                               : launch_tsum<float>(st, s->geo, (float *)s->T, s->rec, s->red);
+    launch_step_finalize(st, s->rec, s->red, G);
     HIP_CHECK_CTX(s->ctx, hipMemcpyAsync(s->h_rec, s->rec, sizeof(StepRec), hipMemcpyDeviceToHost, st), FFR_E_DEVICE);
     HIP_CHECK_CTX(s->ctx, hipStreamSynchronize(st), FFR_E_DEVICE);
     *out = s->n_fluid > 0 ? s->h_rec->tsum / (double)s->n_fluid : s->prm.T_initial;  // :458-468

@@ -139,9 +139,10 @@ static int enqueue_finish(ffr_scene *s)
             ProfScope ps(ctx, st, "opening_T", 0);
             launch_opening_T<R>(st, g, u, v, w, T);
         }
+        unsigned G = 0;  // blocks that wrote the T-sum partials
         {
             ProfScope ps(ctx, st, "temperature", nc * 2 * sz + faces * sz + nc);
-            launch_temperature<R>(st, g, sp, u, v, w, T, Tn, s->rec, s->red);
+            G = launch_temperature<R>(st, g, sp, u, v, w, T, Tn, s->rec, s->red);
         }
         if (g.n_openings) {
             ProfScope ps(ctx, st, "opening_T", 0);
@@ -150,9 +151,9 @@ static int enqueue_finish(ffr_scene *s)
         std::swap(s->T, s->Tn);
         if (s->opening_touches_fluid) {
             ProfScope ps(ctx, st, "tsum", nc * sz + nc);
-            launch_tsum<R>(st, g, (R *)s->T, s->rec, s->red);
+            G = launch_tsum<R>(st, g, (R *)s->T, s->rec, s->red);
         }
-        launch_step_finalize(st, g, s->rec, s->red);
+        launch_step_finalize(st, s->rec, s->red, G);
     } else {
         ProfScope ps(ctx, st, "divmax", faces * sz + nc);
         launch_divmax<R>(st, g, u, v, w, s->rec);

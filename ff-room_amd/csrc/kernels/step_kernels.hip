@@ -82,18 +82,67 @@ __device__ __forceinline__ R adv_lw(R m, R c, R p, R vel, R dt, R h, R ih)
     return by_h(fr - fl, h, ih);
 }
 
-// Scheme switch and diffusion for one face (:132-162). nb = {-x,+x,-y,+y,-z,+z}.
+// The FP32 form regroups the same terms for fused multiply-adds (it is
+// checked against the FP64 form within FP32 rounding, not bit for bit):
+// QUICK's 8 (right - left) is 3(c+p) - 7m + mm for vel >= 0 and, in the
+// reference's other branch, 3(p-c) + m - mm; Lax-Wendroff's fr - fl is
+// vel (p - m)/2 - vel cr (p - 2c + m)/2.
+__device__ __forceinline__ float adv_upwind(float m, float c, float p, float vel, float, float ih)
+{
+    return vel * (vel > 0.0f ? c - m : p - c) * ih;
+}
+__device__ __forceinline__ float adv_quick(float mm, float m, float c, float p, float vel, float,
+                                           float ih)
+{
+    const float pos = fmaf(3.0f, c + p, fmaf(-7.0f, m, mm));
+    const float neg = fmaf(3.0f, p - c, m - mm);
+    return vel * (vel >= 0.0f ? pos : neg) * (0.125f * ih);
+}
+__device__ __forceinline__ float adv_lw(float m, float c, float p, float vel, float dt, float,
+                                        float ih)
+{
+    const float cr = vel * dt * ih;
+    const float curv = fmaf(-2.0f, c, p + m);
+    return 0.5f * vel * fmaf(-cr, curv, p - m) * ih;
+}
+// nu * sum over axes of (nb- - 2c + nb+) / h^2
 template <typename R>
-__device__ __forceinline__ R face_update(int scheme, bool qok, const R nb[6], const R q[3], R c,
-                                         const R vel[3], const R h[3], const R ih[3],
-                                         const R i2[3], R dt, R nu)
+__device__ __forceinline__ R diffusion(const R nb[6], R c, const R i2[3], R nu)
+{
+    return nu * (i2[0] * (nb[0] - R(2) * c + nb[1]) + i2[1] * (nb[2] - R(2) * c + nb[3]) +
+                 i2[2] * (nb[4] - R(2) * c + nb[5]));
+}
+__device__ __forceinline__ float diffusion(const float nb[6], float c, const float i2[3], float nu)
+{
+    float acc = i2[0] * fmaf(-2.0f, c, nb[0] + nb[1]);
+    acc = fmaf(i2[1], fmaf(-2.0f, c, nb[2] + nb[3]), acc);
+    acc = fmaf(i2[2], fmaf(-2.0f, c, nb[4] + nb[5]), acc);
+    return nu * acc;
+}
+// c + dt (-adv + dif)
+template <typename R>
+__device__ __forceinline__ R advance(R c, R dt, R adv, R dif)
+{
+    return c + dt * (-adv + dif);
+}
+__device__ __forceinline__ float advance(float c, float dt, float adv, float dif)
+{
+    return fmaf(dt, dif - adv, c);
+}
+
+// Scheme and diffusion for one face (:132-162), the scheme fixed at compile
+// time. nb = {-x,+x,-y,+y,-z,+z}; q = the far upwind values of QUICK.
+template <int SCHEME, typename R>
+__device__ __forceinline__ R face_update_s(bool qok, const R nb[6], const R q[3], R c,
+                                           const R vel[3], const R h[3], const R ih[3],
+                                           const R i2[3], R dt, R nu)
 {
     R adv;
-    if (scheme == QUICK && qok) {
+    if (SCHEME == QUICK && qok) {
         adv = adv_quick(q[0], nb[0], c, nb[1], vel[0], h[0], ih[0]) +
               adv_quick(q[1], nb[2], c, nb[3], vel[1], h[1], ih[1]) +
               adv_quick(q[2], nb[4], c, nb[5], vel[2], h[2], ih[2]);
-    } else if (scheme == LAX_WENDROFF) {
+    } else if (SCHEME == LAX_WENDROFF) {
         adv = adv_lw(nb[0], c, nb[1], vel[0], dt, h[0], ih[0]) +
               adv_lw(nb[2], c, nb[3], vel[1], dt, h[1], ih[1]) +
               adv_lw(nb[4], c, nb[5], vel[2], dt, h[2], ih[2]);
@@ -102,9 +151,7 @@ __device__ __forceinline__ R face_update(int scheme, bool qok, const R nb[6], co
               adv_upwind(nb[2], c, nb[3], vel[1], h[1], ih[1]) +
               adv_upwind(nb[4], c, nb[5], vel[2], h[2], ih[2]);
     }
-    R dif = nu * (i2[0] * (nb[0] - R(2) * c + nb[1]) + i2[1] * (nb[2] - R(2) * c + nb[3]) +
-                  i2[2] * (nb[4] - R(2) * c + nb[5]));
-    return c + dt * (-adv + dif);
+    return advance(c, dt, adv, diffusion(nb, c, i2, nu));
 }
 
 __device__ __forceinline__ double fan_v(const FanDev &f, int a, double) { return f.vel[a]; }
@@ -268,118 +315,183 @@ __device__ __forceinline__ R apply_mask(const Geo &g, uint16_t code, int axis, R
 }
 
 // ---- K1: predictor -------------------------------------------------------
-template <typename R, bool UNI>
-__device__ __forceinline__ R predict_u(const Geo &g, const StepParams &sp, const Axes<R> &A,
-                                       const R *u, const R *v, const R *w, int i, int j, int k,
-                                       uint32_t e)
+// The predictor marches planes of cells along x (the slowest index). A block
+// owns a tile of TJ x TK cells in (y, z) and a run of x planes; each thread
+// owns one (j, k) column of the tile and updates the low u, v and w faces of
+// its cell at every plane. The tile's plane of u, v, w (and T for buoyancy),
+// with a halo of two rows/columns below and one above, is staged in LDS, so
+// the y and z neighbours are LDS reads at fixed offsets; the x neighbours of
+// the thread's own column stay in registers as the march advances. Two LDS
+// buffers hold planes i and i+1, and the loads of plane i+2 are issued before
+// plane i is computed. Halo entries outside the domain hold 0, the value the
+// reference gives a missing neighbour (Q5); in-domain values the reference
+// replaces by 0 (the boundary faces one step inside, FluidSolver.cpp:110-130,
+// 192-202, 259-269) keep their selects. The QUICK reads that the reference
+// takes through a wrapped flat index (u(i, j-2, k) at j = 1 and u(i, j, k-2)
+// at k = 1, Q3) are global loads of that flat index.
+namespace ptile {
+constexpr int TK = 32, TJ = 8;              // cells of a tile in z and y
+constexpr int RK = TK + 3, RJ = TJ + 3;     // with the halo: 2 below, 1 above
+constexpr int PLANE = RK * RJ;              // LDS elements of one plane of one field
+constexpr int SLOTS = (PLANE + kBlock - 1) / kBlock;
+static_assert(TK * TJ == kBlock, "one cell column per thread");
+}  // namespace ptile
+
+// Tiling of the cell grid for the marching kernels: ntk x ntj tiles in
+// (z, y), nic runs of ic planes in x; tile index k fastest, then y, then the
+// run, so the contiguous tile ranges of one XCD share their halos in L2.
+struct PlaneTiles {
+    uint32_t ntk, ntj, nic, ic;
+};
+
+inline PlaneTiles plane_tiles(const Geo &g, uint32_t min_planes)
 {
-    const int nx = g.nx, ny = g.ny, nz = g.nz;
-    const uint32_t si = (uint32_t)ny * nz, sj = nz;
-    const R c = u[e];
-    if (i == 0 || i == nx) return c;  // boundary faces copied (:164-169)
-    R nb[6], q[3] = {R(0), R(0), R(0)}, vel[3];
-    const int ia = UNI ? 0 : i, ja = UNI ? 0 : j, ka = UNI ? 0 : k;  // uniform: one value per axis
-    const R h[3] = {A.hx[ia], A.hy[ja], A.hz[ka]};
-    const R ih[3] = {A.ihx[ia], A.ihy[ja], A.ihz[ka]};
-    const R i2[3] = {A.i2x[ia], A.i2y[ja], A.i2z[ka]};
-    nb[0] = i > 1 ? u[e - si] : R(0);
-    nb[1] = i < nx - 1 ? u[e + si] : R(0);
-    nb[2] = j > 0 ? u[e - sj] : R(0);
-    nb[3] = j < ny - 1 ? u[e + sj] : R(0);
-    nb[4] = k > 0 ? u[e - 1] : R(0);
-    nb[5] = k < nz - 1 ? u[e + 1] : R(0);
-    vel[0] = c;
-    vel[1] = j > 0 ? R(0.5) * (v[vidx(g, i - 1, j, k)] + v[vidx(g, i, j, k)]) : R(0);
-    vel[2] = k > 0 ? R(0.5) * (w[widx(g, i - 1, j, k)] + w[widx(g, i, j, k)]) : R(0);
-    const bool qok = i >= 2 && i <= nx - 2 && j >= 1 && j <= ny - 2 && k >= 1 && k <= nz - 2;
-    if (sp.scheme == QUICK && qok) {
-        q[0] = u[e - 2 * si];
-        q[1] = u[e - 2 * sj];  // flat index, wraps into row j-1 at j=1 (:139)
-        q[2] = u[e - 2];       // wraps at k=1 (:141)
-    }
-    return face_update<R>(sp.scheme, qok, nb, q, c, vel, h, ih, i2, (R)sp.dt, (R)sp.nu);
+    PlaneTiles t;
+    t.ntk = (uint32_t)(g.nz + ptile::TK - 1) / ptile::TK;
+    t.ntj = (uint32_t)(g.ny + ptile::TJ - 1) / ptile::TJ;
+    const uint32_t t2 = t.ntk * t.ntj;
+    const uint32_t want = (4096 + t2 - 1) / t2;  // runs for about 4096 tiles
+    const uint32_t most = std::max<uint32_t>(1, (uint32_t)g.nx / min_planes);
+    const uint32_t runs = std::max<uint32_t>(1, std::min(want, most));
+    t.ic = ((uint32_t)g.nx + runs - 1) / runs;
+    t.nic = ((uint32_t)g.nx + t.ic - 1) / t.ic;
+    return t;
+}
+inline uint32_t plane_tiles_grid(const PlaneTiles &t)
+{
+    return std::min<uint32_t>(t.ntk * t.ntj * t.nic, kMaxBlocksCap);
 }
 
-template <typename R, bool UNI>
-__device__ __forceinline__ R predict_v(const Geo &g, const StepParams &sp, const Axes<R> &A,
-                                       const R *u, const R *v, const R *w, int i, int j, int k,
-                                       uint32_t e)
+// Where one load slot of a thread lies in the tile's plane: its offsets in
+// the u/T, v and w planes (k fastest), and whether that element exists in
+// each array. Offsets of elements that do not exist are never used.
+struct SlotMap {
+    uint32_t ou, ov, ow;
+    bool in_u, in_v, in_w;
+};
+__device__ __forceinline__ SlotMap slot_map(const Geo &g, int j0, int k0, int e)
 {
-    const int nx = g.nx, ny = g.ny, nz = g.nz;
-    const uint32_t si = (uint32_t)(ny + 1) * nz, sj = nz;
-    const R c = v[e];
-    if (j == 0 || j == ny) return c;  // (:232-236)
-    R nb[6], q[3] = {R(0), R(0), R(0)}, vel[3];
-    const int ia = UNI ? 0 : i, ja = UNI ? 0 : j, ka = UNI ? 0 : k;  // uniform: one value per axis
-    const R h[3] = {A.hx[ia], A.hy[ja], A.hz[ka]};
-    const R ih[3] = {A.ihx[ia], A.ihy[ja], A.ihz[ka]};
-    const R i2[3] = {A.i2x[ia], A.i2y[ja], A.i2z[ka]};
-    vel[0] = i > 0 ? R(0.5) * (u[uidx(g, i, j - 1, k)] + u[uidx(g, i, j, k)]) : R(0);
-    nb[0] = i > 0 ? v[e - si] : R(0);
-    nb[1] = i < nx - 1 ? v[e + si] : R(0);
-    nb[2] = j > 1 ? v[e - sj] : R(0);
-    nb[3] = j < ny - 1 ? v[e + sj] : R(0);
-    vel[1] = c;
-    vel[2] = k > 0 ? R(0.5) * (w[widx(g, i, j - 1, k)] + w[widx(g, i, j, k)]) : R(0);
-    nb[4] = k > 0 ? v[e - 1] : R(0);
-    nb[5] = k < nz - 1 ? v[e + 1] : R(0);
-    const bool qok = i >= 1 && i <= nx - 2 && j >= 2 && j <= ny - 2 && k >= 1 && k <= nz - 2;
-    if (sp.scheme == QUICK && qok) {
-        q[0] = v[e - si];      // adjacent value as the far upwind one (:208)
-        q[1] = v[e - 2 * sj];
-        q[2] = v[e - 1];       // adjacent (:211)
-    }
-    return face_update<R>(sp.scheme, qok, nb, q, c, vel, h, ih, i2, (R)sp.dt, (R)sp.nu);
+    SlotMap m;
+    const int jj = j0 - 2 + e / ptile::RK, kk = k0 - 2 + e % ptile::RK;
+    const bool on = e < ptile::PLANE;  // the last slot of a thread may lie past the plane
+    const bool jin = on && jj >= 0 && jj < g.ny, kin = on && kk >= 0 && kk < g.nz;
+    m.in_u = jin && kin;
+    m.in_v = on && jj >= 0 && jj <= g.ny && kin;
+    m.in_w = jin && on && kk >= 0 && kk <= g.nz;
+    const uint32_t jc = (uint32_t)std::max(jj, 0), kc = (uint32_t)std::max(kk, 0);
+    m.ou = jc * g.nz + kc;
+    m.ov = m.ou;
+    m.ow = jc * (g.nz + 1) + kc;
+    return m;
 }
 
-template <typename R, bool UNI>
-__device__ __forceinline__ R predict_w(const Geo &g, const StepParams &sp, const Axes<R> &A,
-                                       const R *u, const R *v, const R *w, const R *T, int i,
-                                       int j, int k, uint32_t e)
+template <typename R>
+struct PlaneRegs {
+    R u[ptile::SLOTS], v[ptile::SLOTS], w[ptile::SLOTS], t[ptile::SLOTS];
+};
+
+// Raw buffer loads at a byte offset: an offset outside the array reads 0
+// (the hardware range check), so a missing element costs no branch.
+constexpr uint32_t kNoElement = 0xFFFFFFF0u;  // a byte offset past any array below 4 GiB
+__device__ __forceinline__ float buf_ld(__amdgpu_buffer_rsrc_t r, uint32_t off, float)
 {
-    const int nx = g.nx, ny = g.ny, nz = g.nz;
-    const uint32_t si = (uint32_t)ny * (nz + 1), sj = nz + 1;
-    const R c = w[e];
-    if (k == 0 || k == nz) return c;  // (:299-303)
-    R nb[6], q[3] = {R(0), R(0), R(0)}, vel[3];
-    const int ia = UNI ? 0 : i, ja = UNI ? 0 : j, ka = UNI ? 0 : k;  // uniform: one value per axis
-    const R h[3] = {A.hx[ia], A.hy[ja], A.hz[ka]};
-    const R ih[3] = {A.ihx[ia], A.ihy[ja], A.ihz[ka]};
-    const R i2[3] = {A.i2x[ia], A.i2y[ja], A.i2z[ka]};
-    vel[0] = i > 0 ? R(0.5) * (u[uidx(g, i, j, k - 1)] + u[uidx(g, i, j, k)]) : R(0);
-    nb[0] = i > 0 ? w[e - si] : R(0);
-    nb[1] = i < nx - 1 ? w[e + si] : R(0);
-    vel[1] = j > 0 ? R(0.5) * (v[vidx(g, i, j, k - 1)] + v[vidx(g, i, j, k)]) : R(0);
-    nb[2] = j > 0 ? w[e - sj] : R(0);
-    nb[3] = j < ny - 1 ? w[e + sj] : R(0);
-    nb[4] = k > 1 ? w[e - 1] : R(0);
-    nb[5] = k < nz - 1 ? w[e + 1] : R(0);
-    vel[2] = c;
-    const bool qok = i >= 1 && i <= nx - 2 && j >= 1 && j <= ny - 2 && k >= 2 && k <= nz - 2;
-    if (sp.scheme == QUICK && qok) {
-        q[0] = w[e - si];  // adjacent (:275)
-        q[1] = w[e - sj];  // adjacent (:276)
-        q[2] = w[e - 2];
-    }
-    R val = face_update<R>(sp.scheme, qok, nb, q, c, vel, h, ih, i2, (R)sp.dt, (R)sp.nu);
-    if (sp.buoyancy) {  // (:373-386), interior z-faces, before the BC masks
-        const uint32_t cc = cidx(g, i, j, k);
-        const R lo = T[cc - 1], hi = T[cc];
-        val += (R)sp.buoy_coeff * (R(0.5) * (lo + hi) - (R)sp.T_ref);
-    }
-    return val;
+    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, (int)off, 0, 0));
+}
+__device__ __forceinline__ double buf_ld(__amdgpu_buffer_rsrc_t r, uint32_t off, double)
+{
+    return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, (int)off, 0, 0));
 }
 
-// One thread per cell updates the u, v and w faces of that cell's low side
-// (index (i,j,k) in each face array), so the three arrays' neighbourhoods of
-// one region are fetched once into one cache; threads past the cells take the
-// remaining boundary faces (u at i = nx, v at j = ny, w at k = nz).
-template <typename R, bool UNI>
+// Buffer resources of the arrays a march reads.
+struct MarchBufs {
+    __amdgpu_buffer_rsrc_t u, v, w, t, f;
+};
+template <typename R>
+__device__ __forceinline__ MarchBufs march_bufs(const Geo &g, const R *u, const R *v, const R *w,
+                                                const R *T)
+{
+    return {buf_rsrc(u, g.nu * sizeof(R)), buf_rsrc(v, g.nv * sizeof(R)),
+            buf_rsrc(w, g.nw * sizeof(R)), buf_rsrc(T, T ? g.nc * sizeof(R) : 0u),
+            buf_rsrc(g.flags, g.nc)};
+}
+
+// Issues the loads of plane p of u, v, w (and T) for this thread's slots; 0
+// where the element does not exist.
+template <typename R>
+__device__ __forceinline__ void plane_load(const Geo &g, const SlotMap *sm, int p, bool with_t,
+                                           const MarchBufs &b, PlaneRegs<R> &r)
+{
+    const bool pu = p >= 0 && p <= g.nx, pc = p >= 0 && p < g.nx;
+    const uint32_t bu = (uint32_t)p * g.ny * g.nz, bv = (uint32_t)p * (g.ny + 1) * g.nz;
+    const uint32_t bw = (uint32_t)p * g.ny * (g.nz + 1);
+    constexpr uint32_t sz = sizeof(R);
+#pragma unroll
+    for (int s = 0; s < ptile::SLOTS; s++) {
+        const SlotMap &m = sm[s];
+        r.u[s] = buf_ld(b.u, (pu && m.in_u) ? (bu + m.ou) * sz : kNoElement, R(0));
+        r.v[s] = buf_ld(b.v, (pc && m.in_v) ? (bv + m.ov) * sz : kNoElement, R(0));
+        r.w[s] = buf_ld(b.w, (pc && m.in_w) ? (bw + m.ow) * sz : kNoElement, R(0));
+        r.t[s] = with_t ? buf_ld(b.t, (pc && m.in_u) ? (bu + m.ou) * sz : kNoElement, R(0)) : R(0);
+    }
+}
+
+template <typename R>
+__device__ __forceinline__ void plane_store(const PlaneRegs<R> &r, bool with_t, R *su, R *sv, R *sw,
+                                            R *st)
+{
+#pragma unroll
+    for (int s = 0; s < ptile::SLOTS; s++) {
+        const int e = threadIdx.x + s * kBlock;
+        if (s + 1 < ptile::SLOTS || e < ptile::PLANE) {
+            su[e] = r.u[s];
+            sv[e] = r.v[s];
+            sw[e] = r.w[s];
+            if (with_t) st[e] = r.t[s];
+        }
+    }
+}
+
+// The per-face values of the thread's cell on plane p that do not come from
+// the staged planes: the three face mask codes and, for QUICK, the two reads
+// the reference takes through a wrapped flat index (Q3): u(p, j-1, nz-1) for
+// k = 1 and u(p-1, ny-1, k) for j = 1. Loaded one plane ahead.
+template <typename R>
+struct FaceExtras {
+    uint32_t cuv, cw;  // codes: u | v << 16, w
+    R wk, wj;
+};
+template <typename R, int SCHEME>
+__device__ __forceinline__ FaceExtras<R> face_extras(const Geo &g, const MarchBufs &b,
+                                                     __amdgpu_buffer_rsrc_t codes, bool act, int p,
+                                                     int j, int k)
+{
+    FaceExtras<R> x;
+    const bool in = act && p < g.nx;
+    const uint32_t eu = uidx(g, p, j, k), ev = vidx(g, p, j, k), ew = widx(g, p, j, k);
+    const uint32_t cu = __builtin_amdgcn_raw_buffer_load_b16(codes, (int)(in ? eu * 2u : kNoElement), 0, 0);
+    const uint32_t cv = __builtin_amdgcn_raw_buffer_load_b16(codes, (int)(in ? (g.nu + ev) * 2u : kNoElement), 0, 0);
+    x.cw = __builtin_amdgcn_raw_buffer_load_b16(codes, (int)(in ? (g.nu + g.nv + ew) * 2u : kNoElement), 0, 0);
+    x.cuv = cu | (cv << 16);
+    constexpr uint32_t sz = sizeof(R);
+    if (SCHEME == QUICK) {
+        x.wk = buf_ld(b.u, (in && k == 1 && j >= 1) ? (eu - 2u) * sz : kNoElement, R(0));
+        x.wj = buf_ld(b.u, (in && j == 1 && p >= 1) ? (eu - 2u * (uint32_t)g.nz) * sz : kNoElement,
+                      R(0));
+    } else {
+        x.wk = x.wj = R(0);
+    }
+    return x;
+}
+
+template <typename R, bool UNI, int SCHEME>
 struct KB_predict {
     static __device__ __forceinline__ void run(const VB &vb, const Geo &g, const StepParams &sp,
-                                               const R *u, const R *v, const R *w, const R *T,
-                                               R *us, R *vs, R *ws, StepRec *rec)
+                                               const PlaneTiles pt, const R *u, const R *v,
+                                               const R *w, const R *T, R *us, R *vs, R *ws,
+                                               StepRec *rec)
 {
+    using namespace ptile;
+    __shared__ R sU[2][PLANE], sV[2][PLANE], sW[2][PLANE], sT[2][PLANE];
     if (vb.raw == 0 && threadIdx.x == 0) {  // start-of-step diagnostics reset
         rec->vmax_bits = 0ull;
         rec->divmax_bits = 0ull;
@@ -387,57 +499,191 @@ struct KB_predict {
         rec->nonfinite = 0;
     }
     const Axes<R> A = axes_of(g, R(0));
+    const int nx = g.nx, ny = g.ny, nz = g.nz;
     const uint16_t *cu = g.fcode, *cv = g.fcode + g.nu, *cw = g.fcode + g.nu + g.nv;
-    const uint32_t nxu = (uint32_t)g.ny * g.nz, nyv = (uint32_t)g.nx * g.nz;
-    const uint32_t total = g.nc + nxu + nyv + (uint32_t)g.nx * g.ny;
-    uint32_t b, end;
-    block_span_of(total, vb.blk, vb.G, b, end);
-    for (uint32_t t = b + threadIdx.x; t < end; t += kBlock) {
-        if (t < g.nc) {
-            const uint32_t i = fdiv(t, g.f_nynz), rem = t - i * nxu;
-            const uint32_t j = fdiv(rem, g.f_nz), k = rem - j * g.nz;
-            const uint32_t eu = t, ev = vidx(g, i, j, k), ew = widx(g, i, j, k);
-            const R vu = predict_u<R, UNI>(g, sp, A, u, v, w, i, j, k, eu);
-            const R vv = predict_v<R, UNI>(g, sp, A, u, v, w, i, j, k, ev);
-            const R vw = predict_w<R, UNI>(g, sp, A, u, v, w, T, i, j, k, ew);
-            us[eu] = apply_mask<R>(g, cu[eu], 0, vu);
-            vs[ev] = apply_mask<R>(g, cv[ev], 1, vv);
-            ws[ew] = apply_mask<R>(g, cw[ew], 2, vw);
-        } else if (t < g.nc + nxu) {  // u at i = nx: copied, then masked
-            const uint32_t e = g.nc + (t - g.nc);
-            us[e] = apply_mask<R>(g, cu[e], 0, u[e]);
-        } else if (t < g.nc + nxu + nyv) {  // v at j = ny
-            const uint32_t q = t - g.nc - nxu, i = fdiv(q, g.f_nz), k = q - i * g.nz;
-            const uint32_t e = vidx(g, i, g.ny, k);
-            vs[e] = apply_mask<R>(g, cv[e], 1, v[e]);
-        } else {  // w at k = nz
-            const uint32_t q = t - g.nc - nxu - nyv, i = q / (uint32_t)g.ny, j = q - i * g.ny;
-            const uint32_t e = widx(g, i, j, g.nz);
-            ws[e] = apply_mask<R>(g, cw[e], 2, w[e]);
+    const __amdgpu_buffer_rsrc_t codes = buf_rsrc(g.fcode, (g.nu + g.nv + g.nw) * 2u);
+    const bool with_t = sp.buoyancy != 0;
+    const R dt = (R)sp.dt, nu = (R)sp.nu;
+    const int tk = threadIdx.x % TK, tj = threadIdx.x / TK;
+    const int own = (tj + 2) * RK + tk + 2;  // the thread's cell in the LDS plane
+    const MarchBufs mb = march_bufs(g, u, v, w, with_t ? T : (const R *)nullptr);
+    const uint32_t ntiles = pt.ntk * pt.ntj * pt.nic;
+    for (uint32_t tile = vb.blk; tile < ntiles; tile += vb.G) {
+        const uint32_t rest = tile / pt.ntk;
+        const int k0 = (int)(tile - rest * pt.ntk) * TK;
+        const int j0 = (int)(rest % pt.ntj) * TJ;
+        const int i0 = (int)(rest / pt.ntj * pt.ic);
+        const int i1 = std::min(nx, i0 + (int)pt.ic);
+        const int j = j0 + tj, k = k0 + tk;
+        const bool act = j < ny && k < nz;
+        SlotMap sm[SLOTS];
+#pragma unroll
+        for (int s = 0; s < SLOTS; s++) sm[s] = slot_map(g, j0, k0, threadIdx.x + s * kBlock);
+        // y and z spacings of the column, fixed along the march
+        const int ja = UNI ? 0 : (act ? j : 0), ka = UNI ? 0 : (act ? k : 0);
+        const R hy = A.hy[ja], hz = A.hz[ka], ihy = A.ihy[ja], ihz = A.ihz[ka];
+        const R i2y = A.i2y[ja], i2z = A.i2z[ka];
+        // the column's values on the planes before the run (x neighbours)
+        R um2 = R(0), um1 = R(0), vm1 = R(0), wm1 = R(0);
+        if (act && i0 >= 2) um2 = u[uidx(g, i0 - 2, j, k)];
+        if (act && i0 >= 1) {
+            um1 = u[uidx(g, i0 - 1, j, k)];
+            vm1 = v[vidx(g, i0 - 1, j, k)];
+            wm1 = w[widx(g, i0 - 1, j, k)];
+        }
+        FaceExtras<R> ex = face_extras<R, SCHEME>(g, mb, codes, act, i0, j, k);
+        PlaneRegs<R> nxt;
+        plane_load(g, sm, i0, with_t, mb, nxt);
+        plane_store(nxt, with_t, sU[i0 & 1], sV[i0 & 1], sW[i0 & 1], sT[i0 & 1]);
+        plane_load(g, sm, i0 + 1, with_t, mb, nxt);
+        plane_store(nxt, with_t, sU[(i0 + 1) & 1], sV[(i0 + 1) & 1], sW[(i0 + 1) & 1],
+                    sT[(i0 + 1) & 1]);
+        __syncthreads();
+        for (int i = i0; i < i1; i++) {
+            const bool more = i + 2 <= i1;  // plane i+2 is read by this run
+            if (more) plane_load(g, sm, i + 2, with_t, mb, nxt);
+            const FaceExtras<R> exn = face_extras<R, SCHEME>(g, mb, codes, act, i + 1, j, k);
+            const R *Uc = sU[i & 1], *Vc = sV[i & 1], *Wc = sW[i & 1], *Tc = sT[i & 1];
+            const R *Un = sU[(i + 1) & 1], *Vn = sV[(i + 1) & 1], *Wn = sW[(i + 1) & 1];
+            const R uc = Uc[own], vc = Vc[own], wc = Wc[own];
+            if (act) {
+                const int ia = UNI ? 0 : i;
+                const R h[3] = {A.hx[ia], hy, hz};
+                const R ih[3] = {A.ihx[ia], ihy, ihz};
+                const R i2[3] = {A.i2x[ia], i2y, i2z};
+                const uint32_t eu = uidx(g, i, j, k), ev = vidx(g, i, j, k), ew = widx(g, i, j, k);
+                // u face (i, j, k), FluidSolver.cpp:83-162
+                R ru = uc;
+                if (i > 0) {
+                    R nb[6], q[3] = {R(0), R(0), R(0)}, vel[3];
+                    nb[0] = i > 1 ? um1 : R(0);
+                    nb[1] = i < nx - 1 ? Un[own] : R(0);
+                    nb[2] = Uc[own - RK];
+                    nb[3] = Uc[own + RK];
+                    nb[4] = Uc[own - 1];
+                    nb[5] = Uc[own + 1];
+                    vel[0] = uc;
+                    vel[1] = j > 0 ? R(0.5) * (vm1 + vc) : R(0);
+                    vel[2] = k > 0 ? R(0.5) * (wm1 + wc) : R(0);
+                    const bool qok = i >= 2 && i <= nx - 2 && j >= 1 && j <= ny - 2 && k >= 1 &&
+                                     k <= nz - 2;
+                    if (SCHEME == QUICK && qok) {
+                        q[0] = um2;
+                        q[1] = j >= 2 ? Uc[own - 2 * RK] : ex.wj;  // Q3: wrapped flat index
+                        q[2] = k >= 2 ? Uc[own - 2] : ex.wk;       // Q3
+                    }
+                    ru = face_update_s<SCHEME, R>(qok, nb, q, uc, vel, h, ih, i2, dt, nu);
+                }
+                us[eu] = apply_mask<R>(g, (uint16_t)ex.cuv, 0, ru);
+                // v face (i, j, k), :164-230
+                R rv = vc;
+                if (j > 0) {
+                    R nb[6], q[3] = {R(0), R(0), R(0)}, vel[3];
+                    vel[0] = i > 0 ? R(0.5) * (Uc[own - RK] + uc) : R(0);
+                    nb[0] = i > 0 ? vm1 : R(0);
+                    nb[1] = i < nx - 1 ? Vn[own] : R(0);
+                    nb[2] = j > 1 ? Vc[own - RK] : R(0);
+                    nb[3] = j < ny - 1 ? Vc[own + RK] : R(0);
+                    vel[1] = vc;
+                    vel[2] = k > 0 ? R(0.5) * (Wc[own - RK] + wc) : R(0);
+                    nb[4] = Vc[own - 1];
+                    nb[5] = Vc[own + 1];
+                    const bool qok = i >= 1 && i <= nx - 2 && j >= 2 && j <= ny - 2 && k >= 1 &&
+                                     k <= nz - 2;
+                    if (SCHEME == QUICK && qok) {
+                        q[0] = vm1;             // adjacent (:208)
+                        q[1] = Vc[own - 2 * RK];
+                        q[2] = Vc[own - 1];     // adjacent (:211)
+                    }
+                    rv = face_update_s<SCHEME, R>(qok, nb, q, vc, vel, h, ih, i2, dt, nu);
+                }
+                vs[ev] = apply_mask<R>(g, (uint16_t)(ex.cuv >> 16), 1, rv);
+                // w face (i, j, k), :232-297, and buoyancy (:373-386)
+                R rw = wc;
+                if (k > 0) {
+                    R nb[6], q[3] = {R(0), R(0), R(0)}, vel[3];
+                    vel[0] = i > 0 ? R(0.5) * (Uc[own - 1] + uc) : R(0);
+                    nb[0] = i > 0 ? wm1 : R(0);
+                    nb[1] = i < nx - 1 ? Wn[own] : R(0);
+                    vel[1] = j > 0 ? R(0.5) * (Vc[own - 1] + vc) : R(0);
+                    nb[2] = Wc[own - RK];
+                    nb[3] = Wc[own + RK];
+                    nb[4] = k > 1 ? Wc[own - 1] : R(0);
+                    nb[5] = k < nz - 1 ? Wc[own + 1] : R(0);
+                    vel[2] = wc;
+                    const bool qok = i >= 1 && i <= nx - 2 && j >= 1 && j <= ny - 2 && k >= 2 &&
+                                     k <= nz - 2;
+                    if (SCHEME == QUICK && qok) {
+                        q[0] = wm1;             // adjacent (:275)
+                        q[1] = Wc[own - RK];    // adjacent (:276)
+                        q[2] = Wc[own - 2];
+                    }
+                    rw = face_update_s<SCHEME, R>(qok, nb, q, wc, vel, h, ih, i2, dt, nu);
+                    if (with_t)
+                        rw += (R)sp.buoy_coeff * (R(0.5) * (Tc[own - 1] + Tc[own]) - (R)sp.T_ref);
+                }
+                ws[ew] = apply_mask<R>(g, (uint16_t)ex.cw, 2, rw);
+                // the boundary faces past the last cells: copied, then masked
+                if (j == ny - 1) {
+                    const uint32_t e = ev + (uint32_t)nz;
+                    vs[e] = apply_mask<R>(g, cv[e], 1, Vc[own + RK]);
+                }
+                if (k == nz - 1) {
+                    const uint32_t e = ew + 1u;
+                    ws[e] = apply_mask<R>(g, cw[e], 2, Wc[own + 1]);
+                }
+                if (i == nx - 1) {
+                    const uint32_t e = eu + (uint32_t)ny * nz;
+                    us[e] = apply_mask<R>(g, cu[e], 0, Un[own]);
+                }
+            }
+            um2 = um1;
+            um1 = uc;
+            vm1 = vc;
+            wm1 = wc;
+            ex = exn;
+            __syncthreads();  // plane i is read by every thread
+            if (more) plane_store(nxt, with_t, sU[i & 1], sV[i & 1], sW[i & 1], sT[i & 1]);
+            __syncthreads();
         }
     }
 }
 };
-template <typename R, bool UNI>
-__global__ void __launch_bounds__(kBlock) k_predict(Geo g, StepParams sp, const R *u,
+template <typename R, bool UNI, int SCHEME>
+__global__ void __launch_bounds__(kBlock) k_predict(Geo g, StepParams sp, PlaneTiles pt, const R *u,
                                                     const R *v, const R *w, const R *T, R *us,
                                                     R *vs, R *ws, StepRec *rec)
 {
-    KB_predict<R, UNI>::run(vb_here(), g, sp, u, v, w, T, us, vs, ws, rec);
+    KB_predict<R, UNI, SCHEME>::run(vb_here(), g, sp, pt, u, v, w, T, us, vs, ws, rec);
+}
+
+template <typename R, bool UNI, int SCHEME>
+static void launch_predict_s(hipStream_t s, const Geo &g, const StepParams &sp, const R *u,
+                             const R *v, const R *w, const R *T, R *us, R *vs, R *ws, StepRec *rec)
+{
+    const PlaneTiles pt = plane_tiles(g, 8);
+    FFR_GANG((KB_predict<R, UNI, SCHEME>), GB_PLAIN, (k_predict<R, UNI, SCHEME>),
+             dim3(plane_tiles_grid(pt)), dim3(kBlock), 0, s, g, sp, pt, u, v, w, T, us, vs, ws, rec);
+}
+
+template <typename R, bool UNI>
+static void launch_predict_u(hipStream_t s, const Geo &g, const StepParams &sp, const R *u,
+                             const R *v, const R *w, const R *T, R *us, R *vs, R *ws, StepRec *rec)
+{
+    if (sp.scheme == QUICK)
+        launch_predict_s<R, UNI, QUICK>(s, g, sp, u, v, w, T, us, vs, ws, rec);
+    else if (sp.scheme == LAX_WENDROFF)
+        launch_predict_s<R, UNI, LAX_WENDROFF>(s, g, sp, u, v, w, T, us, vs, ws, rec);
+    else
+        launch_predict_s<R, UNI, UPWIND1>(s, g, sp, u, v, w, T, us, vs, ws, rec);
 }
 
 template <typename R>
 void launch_predict(hipStream_t s, const Geo &g, const StepParams &sp, const R *u, const R *v,
                     const R *w, const R *T, R *us, R *vs, R *ws, StepRec *rec)
 {
-    const uint64_t n = (uint64_t)g.nc + (uint64_t)g.ny * g.nz + (uint64_t)g.nx * g.nz +
-                       (uint64_t)g.nx * g.ny;
-    if (g.uniform)
-        FFR_GANG((KB_predict<R, true>), GB_PLAIN, (k_predict<R, true>), dim3(grid_for(n)),
-                 dim3(kBlock), 0, s, g, sp, u, v, w, T, us, vs, ws, rec);
-    else
-        FFR_GANG((KB_predict<R, false>), GB_PLAIN, (k_predict<R, false>), dim3(grid_for(n)),
-                 dim3(kBlock), 0, s, g, sp, u, v, w, T, us, vs, ws, rec);
+    if (g.uniform) launch_predict_u<R, true>(s, g, sp, u, v, w, T, us, vs, ws, rec);
+    else launch_predict_u<R, false>(s, g, sp, u, v, w, T, us, vs, ws, rec);
 }
 
 // ---- K4: Poisson right-hand side ----------------------------------------
@@ -620,95 +866,247 @@ void launch_divmax(hipStream_t s, const Geo &g, const R *u, const R *v, const R 
 }
 
 // ---- K8: temperature transport (FluidSolver.cpp:388-456) ---------------
-template <typename R, bool UNI>
+// The same march as the predictor: the tile's plane of T and of the cell
+// flags, halo included, in LDS; the column's T and flags of the two planes
+// behind and the u face ahead in registers. A neighbour outside the domain
+// is staged as a SOLID cell, so the reference's Neumann rule - the cell's own
+// T for an outside or SOLID neighbour (:417-422) - is one test. The cell's
+// v and w faces are read directly (the pair of a face and the next one is
+// one coalesced run of the array).
+template <typename R>
+__device__ __forceinline__ R t_diffusion(R xm, R xp, R ym, R yp, R zm, R zp, R c, const R h[3],
+                                         const R i2[3], R al)
+{
+    return al * (by_h2(xm - R(2) * c + xp, h[0], i2[0]) + by_h2(ym - R(2) * c + yp, h[1], i2[1]) +
+                 by_h2(zm - R(2) * c + zp, h[2], i2[2]));
+}
+__device__ __forceinline__ float t_diffusion(float xm, float xp, float ym, float yp, float zm,
+                                             float zp, float c, const float *, const float i2[3],
+                                             float al)
+{
+    float acc = i2[0] * fmaf(-2.0f, c, xm + xp);
+    acc = fmaf(i2[1], fmaf(-2.0f, c, ym + yp), acc);
+    acc = fmaf(i2[2], fmaf(-2.0f, c, zm + zp), acc);
+    return al * acc;
+}
+
+template <typename R>
+__device__ __forceinline__ void cell_plane_load(const Geo &g, const SlotMap *sm, int p,
+                                                const MarchBufs &b, R *t, uint8_t *f)
+{
+    const bool pc = p >= 0 && p < g.nx;
+    const uint32_t base = (uint32_t)p * g.ny * g.nz;
+#pragma unroll
+    for (int s = 0; s < ptile::SLOTS; s++) {
+        const bool in = pc && sm[s].in_u;
+        const uint32_t c = base + sm[s].ou;
+        t[s] = buf_ld(b.t, in ? c * (uint32_t)sizeof(R) : kNoElement, R(0));
+        const uint8_t fl = __builtin_amdgcn_raw_buffer_load_b8(b.f, (int)(in ? c : kNoElement), 0, 0);
+        f[s] = in ? fl : (uint8_t)SOLID;
+    }
+}
+
+template <typename R>
+__device__ __forceinline__ void cell_plane_store(const R *t, const uint8_t *f, R *st, uint8_t *sf)
+{
+#pragma unroll
+    for (int s = 0; s < ptile::SLOTS; s++) {
+        const int e = threadIdx.x + s * kBlock;
+        if (s + 1 < ptile::SLOTS || e < ptile::PLANE) {
+            st[e] = t[s];
+            sf[e] = f[s];
+        }
+    }
+}
+
+// The faces of the thread's cell on plane p that transport T: u on its high
+// side (the low one is the previous plane's), v and w on both sides. Loaded
+// one plane ahead.
+template <typename R>
+struct CellFaces {
+    R uh, v0, v1, w0, w1;
+};
+template <typename R>
+__device__ __forceinline__ CellFaces<R> cell_faces(const Geo &g, const MarchBufs &b, bool act,
+                                                   int p, int j, int k)
+{
+    CellFaces<R> f;
+    const bool in = act && p < g.nx;
+    constexpr uint32_t sz = sizeof(R);
+    const uint32_t eu = uidx(g, p + 1, j, k), ev = vidx(g, p, j, k), ew = widx(g, p, j, k);
+    f.uh = buf_ld(b.u, in ? eu * sz : kNoElement, R(0));
+    f.v0 = buf_ld(b.v, in ? ev * sz : kNoElement, R(0));
+    f.v1 = buf_ld(b.v, in ? (ev + (uint32_t)g.nz) * sz : kNoElement, R(0));
+    f.w0 = buf_ld(b.w, in ? ew * sz : kNoElement, R(0));
+    f.w1 = buf_ld(b.w, in ? (ew + 1u) * sz : kNoElement, R(0));
+    return f;
+}
+
+template <typename R, bool UNI, int SCHEME>
 struct KB_temperature {
     static __device__ __forceinline__ void run(const VB &vb, const Geo &g, const StepParams &sp,
-                                               const R *u, const R *v, const R *w, const R *T,
-                                               R *Tn, StepRec *rec, const RedScratch &red)
+                                               const PlaneTiles pt, const R *u, const R *v,
+                                               const R *w, const R *T, R *Tn, StepRec *rec,
+                                               const RedScratch &red)
 {
+    using namespace ptile;
+    __shared__ R sT[2][PLANE];
+    __shared__ uint8_t sF[2][PLANE];
     __shared__ double lds[kBlock / kWave];
-    const uint32_t blk = vb.blk;
     double dmax = 0.0, tsum = 0.0;
     bool bad = false;
     const int nx = g.nx, ny = g.ny, nz = g.nz;
     const Axes<R> A = axes_of(g, R(0));
-    uint32_t sb, se;
-    block_span_of(g.nc, blk, vb.G, sb, se);
-    for (uint32_t c = sb + threadIdx.x; c < se; c += kBlock) {
-        if (g.flags[c] != FLUID) {  // non-FLUID cells are frozen (:402-405)
-            Tn[c] = T[c];
-            continue;
+    const R dt = (R)sp.dt, al = (R)sp.alpha_T;
+    const int tk = threadIdx.x % TK, tj = threadIdx.x / TK;
+    const int own = (tj + 2) * RK + tk + 2;
+    const MarchBufs mb = march_bufs(g, u, v, w, T);
+    const uint32_t ntiles = pt.ntk * pt.ntj * pt.nic;
+    for (uint32_t tile = vb.blk; tile < ntiles; tile += vb.G) {
+        const uint32_t rest = tile / pt.ntk;
+        const int k0 = (int)(tile - rest * pt.ntk) * TK;
+        const int j0 = (int)(rest % pt.ntj) * TJ;
+        const int i0 = (int)(rest / pt.ntj * pt.ic);
+        const int i1 = std::min(nx, i0 + (int)pt.ic);
+        const int j = j0 + tj, k = k0 + tk;
+        const bool act = j < ny && k < nz;
+        SlotMap sm[SLOTS];
+#pragma unroll
+        for (int s = 0; s < SLOTS; s++) sm[s] = slot_map(g, j0, k0, threadIdx.x + s * kBlock);
+        const int ja = UNI ? 0 : (act ? j : 0), ka = UNI ? 0 : (act ? k : 0);
+        const R hy = A.hy[ja], hz = A.hz[ka], ihy = A.ihy[ja], ihz = A.ihz[ka];
+        const R i2y = A.i2y[ja], i2z = A.i2z[ka];
+        R tm2 = R(0), tm1 = R(0), ul = R(0);
+        uint8_t fm2 = SOLID, fm1 = SOLID;
+        if (act) {
+            if (i0 >= 2) {
+                tm2 = T[cidx(g, i0 - 2, j, k)];
+                fm2 = g.flags[cidx(g, i0 - 2, j, k)];
+            }
+            if (i0 >= 1) {
+                tm1 = T[cidx(g, i0 - 1, j, k)];
+                fm1 = g.flags[cidx(g, i0 - 1, j, k)];
+            }
+            ul = u[uidx(g, i0, j, k)];
         }
-        const uint32_t i = fdiv(c, g.f_nynz), rem = c - i * (ny * nz);
-        const uint32_t j = fdiv(rem, g.f_nz), k = rem - j * nz;
-        const int ia = UNI ? 0 : i, ja = UNI ? 0 : j, ka = UNI ? 0 : k;
-        const R hx = A.hx[ia], hy = A.hy[ja], hz = A.hz[ka];
-        const R ihx = A.ihx[ia], ihy = A.ihy[ja], ihz = A.ihz[ka];
-        const R i2x = A.i2x[ia], i2y = A.i2y[ja], i2z = A.i2z[ka];
-        const R Tc = T[c];
-        const uint32_t ev = vidx(g, i, j, k), ew = widx(g, i, j, k);
-        const R u0 = u[c], u1 = u[c + ny * nz], v0 = v[ev], v1 = v[ev + nz], w0 = w[ew],
-                w1 = w[ew + 1];
-        const R uc = R(0.5) * (u0 + u1), vc = R(0.5) * (v0 + v1), wc = R(0.5) * (w0 + w1);
-        // neighbour T: Neumann (Tc) outside the domain and at SOLID (:417-422)
-        auto tn = [&](int a, int bb, int d) -> R {
-            if (a < 0 || a >= nx || bb < 0 || bb >= ny || d < 0 || d >= nz) return Tc;
-            const uint32_t q = cidx(g, a, bb, d);
-            return g.flags[q] == SOLID ? Tc : T[q];
-        };
-        const int ii = i, jj = j, kk = k;
-        const R xm = tn(ii - 1, jj, kk), xp = tn(ii + 1, jj, kk);
-        const R ym = tn(ii, jj - 1, kk), yp = tn(ii, jj + 1, kk);
-        const R zm = tn(ii, jj, kk - 1), zp = tn(ii, jj, kk + 1);
-        const R dt = (R)sp.dt;
-        R adv;
-        if (sp.scheme == QUICK && ii >= 1 && ii <= nx - 2 && jj >= 1 && jj <= ny - 2 && kk >= 1 &&
-            kk <= nz - 2) {
-            const R xmm = tn(ii - 2, jj, kk), ymm = tn(ii, jj - 2, kk), zmm = tn(ii, jj, kk - 2);
-            adv = adv_quick(xmm, xm, Tc, xp, uc, hx, ihx) + adv_quick(ymm, ym, Tc, yp, vc, hy, ihy) +
-                  adv_quick(zmm, zm, Tc, zp, wc, hz, ihz);
-        } else if (sp.scheme == LAX_WENDROFF) {
-            adv = adv_lw(xm, Tc, xp, uc, dt, hx, ihx) + adv_lw(ym, Tc, yp, vc, dt, hy, ihy) +
-                  adv_lw(zm, Tc, zp, wc, dt, hz, ihz);
-        } else {
-            adv = adv_upwind(xm, Tc, xp, uc, hx, ihx) + adv_upwind(ym, Tc, yp, vc, hy, ihy) +
-                  adv_upwind(zm, Tc, zp, wc, hz, ihz);
+        CellFaces<R> cf = cell_faces<R>(g, mb, act, i0, j, k);
+        R nt[SLOTS];
+        uint8_t nf[SLOTS];
+        cell_plane_load(g, sm, i0, mb, nt, nf);
+        cell_plane_store(nt, nf, sT[i0 & 1], sF[i0 & 1]);
+        cell_plane_load(g, sm, i0 + 1, mb, nt, nf);
+        cell_plane_store(nt, nf, sT[(i0 + 1) & 1], sF[(i0 + 1) & 1]);
+        __syncthreads();
+        for (int i = i0; i < i1; i++) {
+            const bool more = i + 2 <= i1;
+            if (more) cell_plane_load(g, sm, i + 2, mb, nt, nf);
+            const CellFaces<R> cfn = cell_faces<R>(g, mb, act, i + 1, j, k);
+            const R *Tc_ = sT[i & 1], *Tn_ = sT[(i + 1) & 1];
+            const uint8_t *Fc = sF[i & 1], *Fn = sF[(i + 1) & 1];
+            const R tc = Tc_[own];
+            const uint8_t fc = Fc[own];
+            const R uh = cf.uh;  // u(i+1, j, k)
+            if (act) {
+                const uint32_t c = cidx(g, i, j, k);
+                if (fc != FLUID) {  // non-FLUID cells are frozen (:402-405)
+                    Tn[c] = tc;
+                } else {
+                    // Neumann at SOLID and outside the domain (:417-422)
+                    auto tn = [&](R t, uint8_t f) -> R { return f == SOLID ? tc : t; };
+                    const R xm = tn(tm1, fm1), xp = tn(Tn_[own], Fn[own]);
+                    const R ym = tn(Tc_[own - RK], Fc[own - RK]), yp = tn(Tc_[own + RK], Fc[own + RK]);
+                    const R zm = tn(Tc_[own - 1], Fc[own - 1]), zp = tn(Tc_[own + 1], Fc[own + 1]);
+                    const int ia = UNI ? 0 : i;
+                    const R h[3] = {A.hx[ia], hy, hz};
+                    const R ih[3] = {A.ihx[ia], ihy, ihz};
+                    const R i2[3] = {A.i2x[ia], i2y, i2z};
+                    const R u0 = ul, u1 = uh, v0 = cf.v0, v1 = cf.v1, w0 = cf.w0, w1 = cf.w1;
+                    const R uc = R(0.5) * (u0 + u1), vc = R(0.5) * (v0 + v1), wc = R(0.5) * (w0 + w1);
+                    R adv;
+                    if (SCHEME == QUICK && i >= 1 && i <= nx - 2 && j >= 1 && j <= ny - 2 &&
+                        k >= 1 && k <= nz - 2) {
+                        const R xmm = tn(tm2, fm2), ymm = tn(Tc_[own - 2 * RK], Fc[own - 2 * RK]);
+                        const R zmm = tn(Tc_[own - 2], Fc[own - 2]);
+                        adv = adv_quick(xmm, xm, tc, xp, uc, h[0], ih[0]) +
+                              adv_quick(ymm, ym, tc, yp, vc, h[1], ih[1]) +
+                              adv_quick(zmm, zm, tc, zp, wc, h[2], ih[2]);
+                    } else if (SCHEME == LAX_WENDROFF) {
+                        adv = adv_lw(xm, tc, xp, uc, dt, h[0], ih[0]) +
+                              adv_lw(ym, tc, yp, vc, dt, h[1], ih[1]) +
+                              adv_lw(zm, tc, zp, wc, dt, h[2], ih[2]);
+                    } else {
+                        adv = adv_upwind(xm, tc, xp, uc, h[0], ih[0]) +
+                              adv_upwind(ym, tc, yp, vc, h[1], ih[1]) +
+                              adv_upwind(zm, tc, zp, wc, h[2], ih[2]);
+                    }
+                    const R dif = t_diffusion(xm, xp, ym, yp, zm, zp, tc, h, i2, al);
+                    const R tnew = advance(tc, dt, adv, dif);
+                    Tn[c] = tnew;
+                    tsum += (double)tnew;
+                    bad |= !isfinite((double)tnew);
+                    const int iy = UNI ? 0 : j, iz = UNI ? 0 : k;
+                    dmax = fmax(dmax, fabs(by_h64<R>((double)u1 - (double)u0, g.hx[ia], g.ihx[ia]) +
+                                           by_h64<R>((double)v1 - (double)v0, g.hy[iy], g.ihy[iy]) +
+                                           by_h64<R>((double)w1 - (double)w0, g.hz[iz], g.ihz[iz])));
+                }
+            }
+            tm2 = tm1;
+            tm1 = tc;
+            fm2 = fm1;
+            fm1 = fc;
+            ul = uh;
+            cf = cfn;
+            __syncthreads();
+            if (more) cell_plane_store(nt, nf, sT[i & 1], sF[i & 1]);
+            __syncthreads();
         }
-        const R al = (R)sp.alpha_T;
-        const R dif = al * (by_h2(xm - R(2) * Tc + xp, hx, i2x) + by_h2(ym - R(2) * Tc + yp, hy, i2y) +
-                            by_h2(zm - R(2) * Tc + zp, hz, i2z));
-        const R tnew = Tc + dt * (-adv + dif);
-        Tn[c] = tnew;
-        tsum += (double)tnew;
-        bad |= !isfinite((double)tnew);
-        dmax = fmax(dmax, fabs(by_h64<R>((double)u1 - (double)u0, g.hx[ia], g.ihx[ia]) +
-                               by_h64<R>((double)v1 - (double)v0, g.hy[ja], g.ihy[ja]) +
-                               by_h64<R>((double)w1 - (double)w0, g.hz[ka], g.ihz[ka])));
     }
     if (bad) rec->nonfinite = 1;
     const double m = block_max(dmax, lds);
     if (threadIdx.x == 0 && m > 0.0) atomic_max_nonneg(&rec->divmax_bits, m);
-    block_partial(tsum, red.partials + SLOT_TSUM * kMaxBlocksCap, blk, lds);
+    block_partial(tsum, red.partials + SLOT_TSUM * kMaxBlocksCap, vb.blk, lds);
 }
 };
-template <typename R, bool UNI>
-__global__ void __launch_bounds__(kBlock) k_temperature(Geo g, StepParams sp, const R *u,
-                                                        const R *v, const R *w, const R *T,
-                                                        R *Tn, StepRec *rec, RedScratch red)
+template <typename R, bool UNI, int SCHEME>
+__global__ void __launch_bounds__(kBlock) k_temperature(Geo g, StepParams sp, PlaneTiles pt,
+                                                        const R *u, const R *v, const R *w,
+                                                        const R *T, R *Tn, StepRec *rec,
+                                                        RedScratch red)
 {
-    KB_temperature<R, UNI>::run(vb_here(), g, sp, u, v, w, T, Tn, rec, red);
+    KB_temperature<R, UNI, SCHEME>::run(vb_here(), g, sp, pt, u, v, w, T, Tn, rec, red);
+}
+
+template <typename R, bool UNI, int SCHEME>
+static unsigned launch_temperature_s(hipStream_t s, const Geo &g, const StepParams &sp,
+                                     const R *u, const R *v, const R *w, const R *T, R *Tn,
+                                     StepRec *rec, RedScratch red)
+{
+    const PlaneTiles pt = plane_tiles(g, 8);
+    const unsigned G = plane_tiles_grid(pt);
+    FFR_GANG((KB_temperature<R, UNI, SCHEME>), GB_PLAIN, (k_temperature<R, UNI, SCHEME>), dim3(G),
+             dim3(kBlock), 0, s, g, sp, pt, u, v, w, T, Tn, rec, red);
+    return G;
+}
+
+template <typename R, bool UNI>
+static unsigned launch_temperature_u(hipStream_t s, const Geo &g, const StepParams &sp,
+                                     const R *u, const R *v, const R *w, const R *T, R *Tn,
+                                     StepRec *rec, RedScratch red)
+{
+    if (sp.scheme == QUICK)
+        return launch_temperature_s<R, UNI, QUICK>(s, g, sp, u, v, w, T, Tn, rec, red);
+    if (sp.scheme == LAX_WENDROFF)
+        return launch_temperature_s<R, UNI, LAX_WENDROFF>(s, g, sp, u, v, w, T, Tn, rec, red);
+    return launch_temperature_s<R, UNI, UPWIND1>(s, g, sp, u, v, w, T, Tn, rec, red);
 }
 
 template <typename R>
-void launch_temperature(hipStream_t s, const Geo &g, const StepParams &sp, const R *u,
-                        const R *v, const R *w, const R *T, R *Tn, StepRec *rec, RedScratch red)
+unsigned launch_temperature(hipStream_t s, const Geo &g, const StepParams &sp, const R *u,
+                            const R *v, const R *w, const R *T, R *Tn, StepRec *rec,
+                            RedScratch red)
 {
-    if (g.uniform)
-        FFR_GANG((KB_temperature<R, true>), GB_PLAIN, (k_temperature<R, true>),
-                 dim3(grid_for(g.nc)), dim3(kBlock), 0, s, g, sp, u, v, w, T, Tn, rec, red);
-    else
-        FFR_GANG((KB_temperature<R, false>), GB_PLAIN, (k_temperature<R, false>),
-                 dim3(grid_for(g.nc)), dim3(kBlock), 0, s, g, sp, u, v, w, T, Tn, rec, red);
+    if (g.uniform) return launch_temperature_u<R, true>(s, g, sp, u, v, w, T, Tn, rec, red);
+    return launch_temperature_u<R, false>(s, g, sp, u, v, w, T, Tn, rec, red);
 }
 
 // Sum of T over FLUID cells (used when an opening overwrites FLUID cells).
@@ -734,13 +1132,15 @@ __global__ void __launch_bounds__(kBlock) k_tsum(Geo g, const R *T, StepRec *rec
 }
 
 template <typename R>
-void launch_tsum(hipStream_t s, const Geo &g, const R *T, StepRec *rec, RedScratch red)
+unsigned launch_tsum(hipStream_t s, const Geo &g, const R *T, StepRec *rec, RedScratch red)
 {
-    FFR_GANG((KB_tsum<R>), GB_PLAIN, k_tsum<R>, dim3(grid_for(g.nc)), dim3(kBlock), 0, s, g, T, rec,
-             red);
+    const unsigned G = grid_for(g.nc);
+    FFR_GANG((KB_tsum<R>), GB_PLAIN, k_tsum<R>, dim3(G), dim3(kBlock), 0, s, g, T, rec, red);
+    return G;
 }
 
-// rec->tsum from the per-block partials of temperature / tsum (one block).
+// rec->tsum from the per-block partials of temperature / tsum (one block);
+// G is the block count of the launch that wrote them.
 struct KB_step_finalize {
     static __device__ __forceinline__ void run(const VB &vb, StepRec *rec, const RedScratch &red,
                                                uint32_t G)
@@ -755,10 +1155,10 @@ __global__ void __launch_bounds__(kBlock) k_step_finalize(StepRec *rec, RedScrat
     KB_step_finalize::run(vb_here(), rec, red, G);
 }
 
-void launch_step_finalize(hipStream_t s, const Geo &g, StepRec *rec, RedScratch red)
+void launch_step_finalize(hipStream_t s, StepRec *rec, RedScratch red, unsigned G)
 {
     FFR_GANG((KB_step_finalize), GB_PLAIN, k_step_finalize, dim3(1), dim3(kBlock), 0, s, rec, red,
-             grid_for(g.nc));
+             G);
 }
 
 // ---- opening temperatures (BoundaryManager.cpp:137-178) -----------------
@@ -860,10 +1260,11 @@ void launch_copy64(hipStream_t s, const double *src, double *dst, uint64_t n)
                                     StepRec *);                                                \
     template void launch_divmax<R>(hipStream_t, const Geo &, const R *, const R *, const R *,  \
                                    StepRec *);                                                 \
-    template void launch_temperature<R>(hipStream_t, const Geo &, const StepParams &,          \
-                                        const R *, const R *, const R *, const R *, R *,       \
-                                        StepRec *, RedScratch);                                \
-    template void launch_tsum<R>(hipStream_t, const Geo &, const R *, StepRec *, RedScratch);  \
+    template unsigned launch_temperature<R>(hipStream_t, const Geo &, const StepParams &,      \
+                                            const R *, const R *, const R *, const R *, R *,   \
+                                            StepRec *, RedScratch);                            \
+    template unsigned launch_tsum<R>(hipStream_t, const Geo &, const R *, StepRec *,           \
+                                     RedScratch);                                              \
     template void launch_opening_T<R>(hipStream_t, const Geo &, const R *, const R *,          \
                                       const R *, R *);                                         \
     template void launch_rhs<R>(hipStream_t, const Geo &, const StepParams &, const R *,       \
