@@ -574,9 +574,9 @@ def main():
     ap.add_argument("--rows", default="c2",
                     help="other single-scene workloads measured beside the headline (comma list)")
     ap.add_argument("--row-steps", type=int, default=20)
-    ap.add_argument("--batched", type=int, default=64,
+    ap.add_argument("--batched", type=int, default=32,
                     help="c4 rooms timed in both forms with continuous batching (0: skip)")
-    ap.add_argument("--batched-steps", type=int, default=20)
+    ap.add_argument("--batched-steps", type=int, default=60)
     ap.add_argument("--c1-reference", type=int, default=500,
                     help="steps of the reference's single-threaded c1 run (0: skip)")
     ap.add_argument("--c4", type=int, default=64, help="rooms in the batched sample")

@@ -736,13 +736,15 @@ int ffr_prof_kernel(ffr_scene *s, const char *name, int reps, double *mean_ms, d
     double b = 0.0;
     const bool mg = s->mg.levels > 0;
     if (k == "predict") {
-        b = faces * 2 * sz + nc + (s->prm.buoyancy ? nc * sz : 0);
+        // u, v, w and their face codes read, u*, v*, w* written, T read with buoyancy
+        b = faces * (2 * sz + 2) + (s->prm.buoyancy ? nc * sz : 0);
         if (s->fp64)
             launch = [&] { launch_predict<double>(st, g, sp, (double *)s->u, (double *)s->v, (double *)s->w, (double *)s->T, (double *)s->us, (double *)s->vs, (double *)s->ws, s->rec); };
         else
             launch = [&] { launch_predict<float>(st, g, sp, (float *)s->u, (float *)s->v, (float *)s->w, (float *)s->T, (float *)s->us, (float *)s->vs, (float *)s->ws, s->rec); };
     } else if (k == "correct") {  // u = u* - dt/rho grad p from the last step's u* and p
-        b = faces * 3 * sz + nc * 8 + nc;
+        // u*, v*, w*, u^n and the face codes read, u written, p read
+        b = faces * (3 * sz + 2) + nc * 8;
         if (s->fp64)
             launch = [&] { launch_correct<double>(st, g, sp, (double *)s->us, (double *)s->vs, (double *)s->ws, s->p, (double *)s->u, (double *)s->v, (double *)s->w, s->rec); };
         else

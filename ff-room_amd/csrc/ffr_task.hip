@@ -109,7 +109,7 @@ static void enqueue_predict(ffr_scene *s)
     const double faces = (double)(s->nu + s->nv + s->nw), nc = (double)s->nc, sz = sizeof(R);
     R *us = (R *)s->us, *vs = (R *)s->vs, *ws = (R *)s->ws;
     {   // K1 predictor + buoyancy + pre-projection masks
-        ProfScope ps(ctx, st, "predict", faces * 2 * sz + nc + (P.buoyancy ? nc * sz : 0));
+        ProfScope ps(ctx, st, "predict", faces * (2 * sz + 2) + (P.buoyancy ? nc * sz : 0));
         launch_predict<R>(st, s->geo, sp, (R *)s->u, (R *)s->v, (R *)s->w, (R *)s->T, us, vs, ws,
                           s->rec);
     }
@@ -130,7 +130,7 @@ static int enqueue_finish(ffr_scene *s)
     R *u = (R *)s->u, *v = (R *)s->v, *w = (R *)s->w;
     const double faces = (double)(s->nu + s->nv + s->nw), nc = (double)s->nc, sz = sizeof(R);
     {   // K6 projection + masks + outflow + velocity change
-        ProfScope ps(ctx, st, "correct", faces * 3 * sz + nc * 8 + nc);
+        ProfScope ps(ctx, st, "correct", faces * (3 * sz + 2) + nc * 8);
         launch_correct<R>(st, g, sp, (R *)s->us, (R *)s->vs, (R *)s->ws, s->p, u, v, w, s->rec);
     }
     if (P.thermal) {
