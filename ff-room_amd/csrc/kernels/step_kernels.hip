@@ -532,16 +532,19 @@ struct KB_predict {
             wm1 = w[widx(g, i0 - 1, j, k)];
         }
         FaceExtras<R> ex = face_extras<R, SCHEME>(g, mb, codes, act, i0, j, k);
-        PlaneRegs<R> nxt;
-        plane_load(g, sm, i0, with_t, mb, nxt);
-        plane_store(nxt, with_t, sU[i0 & 1], sV[i0 & 1], sW[i0 & 1], sT[i0 & 1]);
-        plane_load(g, sm, i0 + 1, with_t, mb, nxt);
-        plane_store(nxt, with_t, sU[(i0 + 1) & 1], sV[(i0 + 1) & 1], sW[(i0 + 1) & 1],
+        // planes i0 and i0+1 in LDS, plane i0+2 in flight; each plane step
+        // issues the loads of the plane three ahead, so two planes' loads are
+        // in flight while one is computed
+        PlaneRegs<R> ra, rb;
+        plane_load(g, sm, i0, with_t, mb, ra);
+        plane_store(ra, with_t, sU[i0 & 1], sV[i0 & 1], sW[i0 & 1], sT[i0 & 1]);
+        plane_load(g, sm, i0 + 1, with_t, mb, rb);
+        plane_store(rb, with_t, sU[(i0 + 1) & 1], sV[(i0 + 1) & 1], sW[(i0 + 1) & 1],
                     sT[(i0 + 1) & 1]);
+        if (i0 + 2 <= i1) plane_load(g, sm, i0 + 2, with_t, mb, ra);
         __syncthreads();
-        for (int i = i0; i < i1; i++) {
-            const bool more = i + 2 <= i1;  // plane i+2 is read by this run
-            if (more) plane_load(g, sm, i + 2, with_t, mb, nxt);
+        auto plane = [&](int i, PlaneRegs<R> &ld, const PlaneRegs<R> &pend) {
+            if (i + 3 <= i1) plane_load(g, sm, i + 3, with_t, mb, ld);
             const FaceExtras<R> exn = face_extras<R, SCHEME>(g, mb, codes, act, i + 1, j, k);
             const R *Uc = sU[i & 1], *Vc = sV[i & 1], *Wc = sW[i & 1], *Tc = sT[i & 1];
             const R *Un = sU[(i + 1) & 1], *Vn = sV[(i + 1) & 1], *Wn = sW[(i + 1) & 1];
@@ -643,8 +646,12 @@ struct KB_predict {
             wm1 = wc;
             ex = exn;
             __syncthreads();  // plane i is read by every thread
-            if (more) plane_store(nxt, with_t, sU[i & 1], sV[i & 1], sW[i & 1], sT[i & 1]);
+            if (i + 2 <= i1) plane_store(pend, with_t, sU[i & 1], sV[i & 1], sW[i & 1], sT[i & 1]);
             __syncthreads();
+        };
+        for (int i = i0; i < i1; i += 2) {  // the two register sets alternate
+            plane(i, rb, ra);
+            if (i + 1 < i1) plane(i + 1, ra, rb);
         }
     }
 }
