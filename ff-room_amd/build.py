@@ -25,7 +25,7 @@ ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
 
 HIP_SOURCES = [
     "ffr_capi.hip", "ffr_scene.hip", "ffr_task.hip", "ffr_mg.hip", "ffr_serve.hip", "ffr_gang.hip",
-    "ffr_snapshot.hip", "ffr_lbm.hip", "ffr_noalloc.hip",
+    "ffr_snapshot.hip", "ffr_lbm.hip", "ffr_noalloc.hip", "ffr_pool.hip",
     "kernels/step_kernels.hip", "kernels/cg_kernels.hip", "kernels/mg_kernels.hip",
     "kernels/lbm_kernels.hip",
 ]

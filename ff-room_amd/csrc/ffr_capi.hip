@@ -27,6 +27,7 @@ using namespace ffr;
 // ---- errors ----------------------------------------------------------------
 void ffr_ctx::fail(const char *fmt, ...)
 {
+    std::lock_guard<std::mutex> hold(err_mu);
     va_list ap;
     va_start(ap, fmt);
     vsnprintf(err, sizeof err, fmt, ap);
@@ -117,10 +118,6 @@ int ffr_ctx_create(int device, size_t hbm_budget, ffr_ctx **out)
     ffr::g_resident_blocks = (unsigned)std::max(1, prop.multiProcessorCount) * 8u * 4u /
                              (unsigned)(kBlock / kWave);
     c->budget = hbm_budget ? hbm_budget : (size_t)prop.totalGlobalMem * 9 / 10;
-    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
-        delete c;
-        return FFR_E_DEVICE;
-    }
     *out = c;
     return FFR_OK;
 }
@@ -133,8 +130,8 @@ void ffr_ctx_destroy(ffr_ctx *c)
     // scenes still alive go with their context
     const std::set<ffr_scene *> left = c->scenes;
     for (ffr_scene *s : left) ffr_scene_destroy(s);
+    c->pool_trim();
     c->prof_clear();
-    if (c->stream) hipStreamDestroy(c->stream);
     delete c;
 }
 

@@ -56,10 +56,7 @@ bool patch_reads_inflow(const ffr_scene *s)
 
 void lbm_release(ffr_scene *s)
 {
-    if (s->lbm.arena) {
-        hipFree(s->lbm.arena);
-        s->ctx->in_use -= s->lbm.bytes;
-    }
+    s->ctx->dev_give(s->lbm.arena, s->lbm.bytes);
     s->lbm = LbmState{};
 }
 }  // namespace ffr
@@ -92,9 +89,11 @@ int ffr_lbm_attach(ffr_scene *s, double tau, double dt, double convergence_tol, 
         return FFR_E_NOMEM;
     }
     LbmState &L = s->lbm;
-    HIP_CHECK_CTX(c, hipMalloc(&L.arena, bytes), FFR_E_NOMEM);
-    c->in_use += bytes;
-    L.bytes = bytes;
+    L.arena = c->dev_take(bytes, &L.bytes);
+    if (!L.arena) {
+        c->fail("ffr_lbm_attach: hipMalloc of %zu bytes failed", bytes);
+        return FFR_E_NOMEM;
+    }
     L.f = (double *)L.arena;
     L.f2 = L.f + 19 * nc;
     L.cu = L.f2 + 19 * nc;

@@ -263,12 +263,11 @@ int mg_create(ffr_scene *s)
         s->ctx->fail("multigrid hierarchy needs %zu bytes beyond the context budget", bytes);
         return FFR_E_NOMEM;
     }
-    if (hipMalloc(&M.arena, bytes) != hipSuccess) {
+    M.arena = s->ctx->dev_take(bytes, &M.bytes);
+    if (!M.arena) {
         s->ctx->fail("hipMalloc of %zu bytes for the multigrid hierarchy failed", bytes);
         return FFR_E_NOMEM;
     }
-    M.bytes = bytes;
-    s->ctx->in_use += bytes;
     MgState *st = nullptr;
     mg_layout((char *)M.arena, D, bf16, dense, &M, &st);
     s->mgs = st;
@@ -348,9 +347,8 @@ void mg_destroy(ffr_scene *s)
     if (s->mg.g_start) hipGraphExecDestroy(s->mg.g_start);
     s->mg.g_iter = s->mg.g_start = nullptr;
     s->mg.graphs_tried = false;
-    if (s->mg.arena) {
-        hipFree(s->mg.arena);
-        s->ctx->in_use -= s->mg.bytes;
+    if (s->mg.arena) {  // the scene's stream is idle: the block goes back to the cache
+        s->ctx->dev_give(s->mg.arena, s->mg.bytes);
         s->mg.arena = nullptr;
     }
     s->mg.levels = 0;

@@ -243,26 +243,35 @@ int ffr_scene_create(ffr_ctx *ctx, const ffr_scene_desc *d, const ffr_params *pr
     s->outflows.assign(d->outflows, d->outflows + d->n_outflows);
     s->openings.assign(d->openings, d->openings + d->n_openings);
     // a stream per scene: the serving loop overlaps the steps of many scenes
-    if (hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess) {
-        ctx->fail("hipStreamCreate failed");
-        delete s;
+    // (stream, event, pinned and device blocks come from the context's cache)
+    s->stream = ctx->stream_take();
+    s->task.ev = ctx->event_take();
+    s->h_block = ctx->pinned_take();
+    if (!s->stream || !s->task.ev || !s->h_block) {
+        ctx->fail("creating the scene's stream, event or pinned mirror failed");
+        ffr_scene_destroy(s);
         return FFR_E_DEVICE;
     }
     s->own_stream = true;
+    {
+        char *h = static_cast<char *>(s->h_block);
+        s->h_rec = reinterpret_cast<StepRec *>(h);
+        s->h_cg = reinterpret_cast<CgState *>(h + pinned_slot(sizeof(StepRec)));
+        s->h_mgs = reinterpret_cast<MgState *>(h + pinned_slot(sizeof(StepRec)) + pinned_slot(sizeof(CgState)));
+    }
 
     const size_t bytes = layout(nullptr, d, prm, nullptr);
     if (ctx->in_use + bytes > ctx->budget) {
         ctx->fail("scene needs %zu bytes, context budget has %zu free", bytes, ctx->budget - ctx->in_use);
-        delete s;
+        ffr_scene_destroy(s);
         return FFR_E_NOMEM;
     }
-    if (hipMalloc(&s->arena, bytes) != hipSuccess) {
+    s->arena = ctx->dev_take(bytes, &s->arena_bytes);
+    if (!s->arena) {
         ctx->fail("hipMalloc of %zu bytes failed", bytes);
-        delete s;
+        ffr_scene_destroy(s);
         return FFR_E_NOMEM;
     }
-    s->arena_bytes = bytes;
-    ctx->in_use += bytes;
     ArenaPtrs a;
     layout((char *)s->arena, d, prm, &a);
     if (!d->cell_type) {  // the flags are marked on the device; the host keeps a copy
@@ -278,9 +287,7 @@ int ffr_scene_create(ffr_ctx *ctx, const ffr_scene_desc *d, const ffr_params *pr
         if (hipMemcpyAsync(s->flags.data(), a.flags, s->nc, hipMemcpyDeviceToHost, s->stream) != hipSuccess ||
             hipStreamSynchronize(s->stream) != hipSuccess) {
             ctx->fail("marking the cell flags failed");
-            hipFree(s->arena);
-            ctx->in_use -= bytes;
-            delete s;
+            ffr_scene_destroy(s);
             return FFR_E_DEVICE;
         }
     }
@@ -398,9 +405,7 @@ int ffr_scene_create(ffr_ctx *ctx, const ffr_scene_desc *d, const ffr_params *pr
     ok &= hipStreamSynchronize(st) == hipSuccess;
     if (!ok) {
         ctx->fail("scene upload failed");
-        hipFree(s->arena);
-        ctx->in_use -= bytes;
-        delete s;
+        ffr_scene_destroy(s);
         return FFR_E_DEVICE;
     }
 
@@ -467,13 +472,6 @@ int ffr_scene_create(ffr_ctx *ctx, const ffr_scene_desc *d, const ffr_params *pr
     s->red.partials = a.partials;
     s->red.tickets = a.tickets;
 
-    if (hipHostMalloc((void **)&s->h_rec, sizeof(StepRec)) != hipSuccess ||
-        hipHostMalloc((void **)&s->h_cg, sizeof(CgState)) != hipSuccess ||
-        hipHostMalloc((void **)&s->h_mgs, sizeof(MgState)) != hipSuccess) {
-        ctx->fail("pinned host allocation failed");
-        ffr_scene_destroy(s);
-        return FFR_E_NOMEM;
-    }
     if (scene_uses_mg(s->prm)) {
         int rc = mg_create(s);
         if (rc != FFR_OK) {
@@ -490,21 +488,23 @@ void ffr_scene_destroy(ffr_scene *s)
 {
     if (!s) return;
     hipSetDevice(s->ctx->device);
-    s->ctx->scenes.erase(s);
-    serve_forget(s->ctx, s);
-    hipStreamSynchronize(s->stream);
+    ffr_ctx *c = s->ctx;
+    c->scenes.erase(s);
+    serve_forget(c, s);  // waits for a gang stream that may still read the scene
+    // the blocks go back to the context's cache only once the stream is idle;
+    // a stream that reports an error is not trusted with another scene
+    const bool idle = !s->stream || hipStreamSynchronize(s->stream) == hipSuccess;
+    if (!idle) hipDeviceSynchronize();
     snapshot_release(s);
     lbm_release(s);
-    if (s->task.ev) hipEventDestroy(s->task.ev);
     mg_destroy(s);
-    if (s->arena) {
-        hipFree(s->arena);
-        s->ctx->in_use -= s->arena_bytes;
+    c->dev_give(s->arena, s->arena_bytes);
+    c->event_give(s->task.ev);
+    c->pinned_give(s->h_block);
+    if (s->own_stream) {
+        if (idle) c->stream_give(s->stream);
+        else hipStreamDestroy(s->stream);
     }
-    if (s->h_rec) hipHostFree(s->h_rec);
-    if (s->h_cg) hipHostFree(s->h_cg);
-    if (s->h_mgs) hipHostFree(s->h_mgs);
-    if (s->own_stream) hipStreamDestroy(s->stream);
     delete s;
 }
 

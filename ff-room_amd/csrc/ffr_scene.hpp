@@ -6,6 +6,7 @@
 #include <map>
 #include <set>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -14,13 +15,20 @@
 #include "ffr_lbm.hpp"
 #include "ffr_mg.hpp"
 
+namespace ffr {
+// one pinned host block per scene holds its StepRec, CgState and MgState mirrors
+constexpr size_t pinned_slot(size_t b) { return (b + 255) / 256 * 256; }
+constexpr size_t kPinnedBlock =
+    pinned_slot(sizeof(StepRec)) + pinned_slot(sizeof(CgState)) + pinned_slot(sizeof(MgState));
+}  // namespace ffr
+
 struct ffr_ctx {
     int device = 0;
     hipDeviceProp_t prop{};
     size_t budget = 0;  // HBM bytes this context may hold
     size_t in_use = 0;
-    hipStream_t stream = nullptr;
     char err[512] = {0};
+    std::mutex err_mu;  // serving threads may fail at once
     void fail(const char *fmt, ...);
 
     // profiling
@@ -46,6 +54,30 @@ struct ffr_ctx {
 
     // scenes created on this context (destroyed with it)
     std::set<ffr_scene *> scenes;
+
+    // Device blocks, pinned host blocks, streams and events of destroyed
+    // scenes, kept for the scenes created next (ffr_pool.hip). hipFree and
+    // hipHostFree wait for the whole device, so freeing them each time a
+    // request ends would stall every other scene's work in a serving loop.
+    // `in_use` counts the blocks scenes hold; cached device blocks are freed,
+    // largest first, when a new block would not fit the budget beside them.
+    struct Pool {
+        std::mutex mu;
+        std::multimap<size_t, void *> dev;  // free device blocks by size
+        size_t dev_bytes = 0;
+        std::vector<void *> pinned;         // free blocks of kPinnedBlock bytes
+        std::vector<hipStream_t> streams;
+        std::vector<hipEvent_t> events;
+    } pool;
+    void *dev_take(size_t bytes, size_t *got);  // nullptr: allocation failed
+    void dev_give(void *p, size_t bytes);
+    void *pinned_take();
+    void pinned_give(void *p);
+    hipStream_t stream_take();
+    void stream_give(hipStream_t st);
+    hipEvent_t event_take();
+    void event_give(hipEvent_t ev);
+    size_t pool_trim();  // frees every cached block; returns the device bytes freed
 
     // serving (ffr_serve.hip)
     struct Server;
@@ -126,7 +158,8 @@ struct ffr_scene {
     ffr::Multigrid mg{};          // multigrid hierarchy (when allocated)
     ffr::MgState *mgs = nullptr;  // device scalars of the multigrid solve
 
-    // pinned host mirrors
+    // pinned host mirrors (three slots of one kPinnedBlock-byte block)
+    void *h_block = nullptr;
     ffr::StepRec *h_rec = nullptr;
     ffr::CgState *h_cg = nullptr;
     ffr::MgState *h_mgs = nullptr;

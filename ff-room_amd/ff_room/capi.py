@@ -121,6 +121,8 @@ SIGNATURES = {
     "ffr_last_error": (C.c_char_p, [_P]),
     "ffr_device_query": (_I, [_P, C.POINTER(DeviceInfo)]),
     "ffr_ctx_bytes_in_use": (C.c_size_t, [_P]),
+    "ffr_ctx_bytes_cached": (C.c_size_t, [_P]),
+    "ffr_ctx_trim": (C.c_size_t, [_P]),
     "ffr_scene_footprint": (C.c_size_t, [C.POINTER(SceneDesc), C.POINTER(Params)]),
     "ffr_scene_create": (_I, [_P, C.POINTER(SceneDesc), C.POINTER(Params), C.POINTER(_P)]),
     "ffr_scene_destroy": (_V, [_P]),
@@ -147,6 +149,7 @@ SIGNATURES = {
     "ffr_poll": (_I, [_P, C.c_uint64, C.POINTER(StepResult), C.POINTER(C.c_int)]),
     "ffr_drain": (_I, [_P]),
     "ffr_serve_config": (_I, [_P, C.c_int]),
+    "ffr_serve_threads": (_I, [_P, C.c_int]),
     "ffr_serve_gangs": (_I, [_P, C.c_int]),
     "ffr_serve_gang_stats": (_I, [_P, C.c_void_p]),
     "ffr_progress": (_I, [_P, C.POINTER(C.c_int)]),
@@ -229,9 +232,21 @@ class Context:
     def bytes_in_use(self) -> int:
         return self._l.ffr_ctx_bytes_in_use(self.h)
 
+    def bytes_cached(self) -> int:
+        """Device bytes of destroyed scenes the context keeps for new ones."""
+        return self._l.ffr_ctx_bytes_cached(self.h)
+
+    def trim(self) -> int:
+        """Frees the cached blocks; returns the device bytes freed."""
+        return self._l.ffr_ctx_trim(self.h)
+
     # ---- serving (continuous batching of many scenes) ----
     def serve_config(self, slots: int):
         self.check(self._l.ffr_serve_config(self.h, slots), "ffr_serve_config")
+
+    def serve_threads(self, threads: int):
+        """Host threads advancing the served scenes (ffr_serve_threads)."""
+        self.check(self._l.ffr_serve_threads(self.h, threads), "ffr_serve_threads")
 
     def serve_gangs(self, gangs: int):
         """Merged launches over groups of scenes (0: every scene on its own stream)."""

@@ -192,6 +192,16 @@ void ffr_ctx_destroy(ffr_ctx *ctx);
 const char *ffr_last_error(const ffr_ctx *ctx);
 int ffr_device_query(ffr_ctx *ctx, ffr_device_info *out);
 size_t ffr_ctx_bytes_in_use(const ffr_ctx *ctx);
+/* Device blocks of destroyed scenes stay cached in their context for the
+ * scenes created next (hipFree would wait for the whole device, stalling the
+ * other scenes' streams in a serving loop); they are freed, largest first,
+ * when a new block would not fit the budget beside them, and with the
+ * context. ffr_ctx_bytes_cached is the device bytes cached; ffr_ctx_trim
+ * frees every cached block, pinned block, stream and event and returns the
+ * device bytes it freed. No counterpart in the reference, whose scenes live
+ * in host memory. */
+size_t ffr_ctx_bytes_cached(const ffr_ctx *ctx);
+size_t ffr_ctx_trim(ffr_ctx *ctx);
 
 /* ---- scenes ------------------------------------------------------------ */
 /* Device bytes a scene with this geometry and these parameters occupies. */
@@ -304,6 +314,16 @@ int ffr_lbm_step(ffr_scene *scene, ffr_step_result *out);
  * admission: ffr_scene_create returns FFR_E_NOMEM beyond the context budget;
  * destroy finished scenes to admit new ones. */
 int ffr_serve_config(ffr_ctx *ctx, int slots);
+/* Host threads (no counterpart in the reference; 1..64, default 8) that run
+ * the active tickets: a scene's step is a few hundred kernel launches and
+ * host checks, and issuing them for many scenes from one thread limits the
+ * served steps per second. With more than one thread each active ticket is
+ * run by one worker thread at a time (waiting on the device between its
+ * launches), and ffr_progress / ffr_poll / ffr_drain admit, collect and
+ * report; with 1, those calls advance every ticket themselves. Results do
+ * not depend on the count. Gangs (below) are advanced by the calling
+ * thread. */
+int ffr_serve_threads(ffr_ctx *ctx, int threads);
 /* Gangs (no counterpart in the reference): with gangs >= 1, the active
  * mixed-precision MG-PCG scenes are spread over `gangs` groups, and each
  * group advances its scenes in rounds of merged launches: one launch runs the

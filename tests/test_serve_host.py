@@ -45,6 +45,9 @@ class _FakeCtx:
     def serve_gangs(self, gangs):
         self.gangs = gangs
 
+    def serve_threads(self, threads):
+        self.threads = threads
+
     def device_info(self):
         return {"hbm_bytes": 10_000}
 
