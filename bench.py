@@ -344,7 +344,7 @@ def lattice_bench(ctx, cfg, desc, ps, args):
     for _ in range(args.warmup):
         d.lbm_step()
     if args.kernel_only:  # as for the projection step: the kernel, then the calibration copy
-        for name in (args.kernel_only, "copy"):
+        for name in [*args.kernel_only.split(","), "copy"]:
             ms, nbytes = d.prof_kernel(name, 20)
             print(json.dumps({"kernel": name, "mean_us": ms * 1e3, "bytes": nbytes}), flush=True)
         d.close()
@@ -584,7 +584,7 @@ def main():
     ap.add_argument("--c5", type=int, default=40, help="requests in the stream sample")
     ap.add_argument("--c5-steps", type=int, default=200)
     ap.add_argument("--kernel-only", default=None,
-                    help="time one kernel (for rocprofv3 --pmc runs) and exit")
+                    help="time the named kernels (comma list; for rocprofv3 --pmc runs) and exit")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported: scenes are "
@@ -615,12 +615,12 @@ def main():
             autotune.apply(ps.params, tuned)
             log(f"tuning: {tuned}")
     if args.kernel_only:
-        # the named kernel, then the library's plain FP64 copy of nx*ny*nz
+        # the named kernels, then the library's plain FP64 copy of nx*ny*nz
         # doubles, whose memory counters calibrate the kernel's
         d = device.create(ctx, ps)
         for _ in range(2):
             d.step()
-        for name in (args.kernel_only, "copy"):
+        for name in [*args.kernel_only.split(","), "copy"]:
             ms, nbytes = d.prof_kernel(name, 20)
             print(json.dumps({"kernel": name, "mean_us": ms * 1e3, "bytes": nbytes}), flush=True)
         d.close()

@@ -198,6 +198,25 @@ __device__ __forceinline__ void block_span_of(uint32_t n, uint32_t blk, uint32_t
     e = (uint32_t)(e64 < n ? e64 : n);
 }
 
+// The chunks (256 elements) of n split over the blocks of G so that the
+// blocks of one XCD (a contiguous range of logical blocks, see xcd_remap)
+// share one contiguous range of chunks and walk it interleaved: block slot s
+// of the XCD's g blocks takes chunks first + s, first + s + g, ... < end. The
+// XCD's resident blocks then advance through their range together, so data
+// a stencil reads again one plane later is still in that XCD's L2.
+__device__ __forceinline__ void xcd_chunks(uint32_t n, uint32_t blk, uint32_t G, uint32_t &first,
+                                           uint32_t &step, uint32_t &end)
+{
+    const uint32_t q = G / kXcds, r = G % kXcds;
+    const uint32_t x = blk < r * (q + 1) ? blk / (q + 1) : r + (blk - r * (q + 1)) / (q ? q : 1u);
+    const uint32_t base = x < r ? x * (q + 1) : r * (q + 1) + (x - r) * q;
+    const uint32_t gx = x < r ? q + 1 : q;
+    const uint64_t chunks = (n + kBlock - 1) / kBlock;
+    first = (uint32_t)(chunks * base / G) + (blk - base);
+    end = (uint32_t)(chunks * (base + gx) / G);
+    step = gx;
+}
+
 // Blocks of a grid of G split over the u, v and w face arrays in proportion
 // to their sizes (each at least one), so a block works on one array only.
 struct FaceSections {
@@ -315,6 +334,10 @@ __device__ __forceinline__ __amdgpu_buffer_rsrc_t buf_rsrc(const void *p, uint32
 __device__ __forceinline__ float buf_ldf(__amdgpu_buffer_rsrc_t r, uint32_t i)
 {
     return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, (int)(i * 4u), 0, 0));
+}
+__device__ __forceinline__ double buf_ldd(__amdgpu_buffer_rsrc_t r, uint32_t i)
+{
+    return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, (int)(i * 8u), 0, 0));
 }
 __device__ __forceinline__ uint16_t buf_ldu16(__amdgpu_buffer_rsrc_t r, uint32_t i)
 {

@@ -125,24 +125,75 @@ void launch_cg_init(hipStream_t s, const Geo &g, const double *b, double *x, dou
     FFR_LAUNCH(k_cg_start, dim3(1), dim3(kBlock), 0, s, cg, red, G, max_iter, tol);
 }
 
-// Ad = A d and partials of d.Ad (:120-126)
+// Ad = A d and partials of d.Ad (:120-126). Each thread takes two cells (of
+// the chunks xcd_chunks gives its block) and issues all their loads (code, d and the six
+// neighbours, through buffer resources: out of range reads give 0) before
+// using any; the row is lap_row's, face by face in the same order, with the
+// values of faces that do not couple dropped by the code.
 template <bool UNI>
 __global__ void __launch_bounds__(kBlock) k_cg_spmv(Geo g, const double *__restrict__ d,
                                                     double *__restrict__ Ad, CgState *cg,
                                                     RedScratch red)
 {
+    constexpr int U = 2;
     __shared__ double lds[kBlock / kWave];
     if (cg->done) return;
     const uint32_t blk = xcd_block();
+    const uint32_t si = (uint32_t)g.ny * g.nz, sj = g.nz;
+    const auto bd = buf_rsrc(d, 8u * g.nc);
     double dq = 0.0;
-    uint32_t sb, se;
-    block_span(g.nc, blk, sb, se);
-    for (uint32_t c = sb + threadIdx.x; c < se; c += kBlock) {
-        const uint16_t m = g.nbm[c];
-        const double dc = d[c];
-        const double a = (m & NB_SELF_SOLID) ? dc : lap_row<UNI>(g, d, c, m);
-        Ad[c] = a;
-        dq += dc * a;
+    uint32_t ch0, cst, cend;
+    xcd_chunks(g.nc, blk, gridDim.x, ch0, cst, cend);
+    for (uint32_t ch = ch0; ch < cend; ch += U * cst) {
+        uint16_t m[U];
+        double dc[U], v[U][6];
+        uint32_t cc[U];
+        bool in[U];
+#pragma unroll
+        for (int t = 0; t < U; t++) {
+            const uint32_t ct = (ch + t * cst) * kBlock + threadIdx.x;
+            in[t] = ch + t * cst < cend && ct < g.nc;
+            const uint32_t c = in[t] ? ct : 0u;
+            cc[t] = c;
+            m[t] = g.nbm[c];
+            dc[t] = d[c];
+            const uint32_t q[6] = {c - si, c + si, c - sj, c + sj, c - 1u, c + 1u};
+#pragma unroll
+            for (int f = 0; f < 6; f++) v[t][f] = buf_ldd(bd, q[f]);
+        }
+#pragma unroll
+        for (int t = 0; t < U; t++) {
+            if (!in[t]) continue;
+            const uint32_t c = cc[t];
+            double sc[6];
+            if (UNI) {
+                sc[0] = sc[1] = g.sxd;
+                sc[2] = sc[3] = g.syd;
+                sc[4] = sc[5] = g.szd;
+            } else {
+                const uint32_t i = fdiv(c, g.f_nynz), rem = c - i * si;
+                const uint32_t j = fdiv(rem, g.f_nz), k = rem - j * sj;
+                const double *L = g.lap;
+                sc[0] = L[i]; sc[1] = L[g.nx + i];
+                sc[2] = L[2 * g.nx + j]; sc[3] = L[2 * g.nx + g.ny + j];
+                sc[4] = L[2 * g.nx + 2 * g.ny + k]; sc[5] = L[2 * g.nx + 2 * g.ny + g.nz + k];
+            }
+            double a;
+            if (m[t] & NB_SELF_SOLID) {
+                a = dc[t];
+            } else {
+                double diag = 0.0, off = 0.0;
+#pragma unroll
+                for (int q = 0; q < 6; q++) {
+                    if (m[t] & (NB_UNK0 << q)) { diag += sc[q]; off += sc[q] * v[t][q]; }
+                    else if (m[t] & (NB_OUT0 << q)) diag += 2.0 * sc[q];
+                }
+                if (diag < 1e-30) diag = 1.0;
+                a = diag * dc[t] - off;
+            }
+            Ad[c] = a;
+            dq += dc[t] * a;
+        }
     }
     // no other block of this kernel reads the counter
     if (blk == 0 && threadIdx.x == 0) cg->started += 1;

@@ -306,13 +306,15 @@ __global__ void __launch_bounds__(kBlock) k_mg_build(Geo g, MgLevel F, MgLevel C
 // load of a cell is issued before any returns; the gathers go through buffer
 // resources, so neighbours beyond either end of the other array read 0 and
 // values of faces that do not couple are dropped by the code. Each thread
-// takes kU cells a block-stride apart. Interior cells (six unknown
+// takes kU cells per round from its block's chunks (xcd_chunks: the blocks
+// of an XCD walk one range together, so the other colour's values a pass
+// gathers one plane later are still in that XCD's L2). Interior cells (six unknown
 // neighbours, none OUTFLOW) skip the per-face selects and the diagonal
 // count; their arithmetic is the general formula's with every face present.
 constexpr int kU = 4;
 // cells per thread of the gather kernels with the most values in flight
 // (the visiting order of a thread's cells, and so its partial sums, is the
-// same for any unroll: s, s + kBlock, s + 2 kBlock, ...)
+// same for any unroll: its chunks in order)
 constexpr int kU2 = 2;
 constexpr uint16_t kInterior = 0x007F;  // NB_SELF + six NB_UNK0 bits, no NB_OUT0 bit
 constexpr uint16_t kFaceBits = 0x1FFF;
@@ -404,15 +406,15 @@ struct KB_mg_smooth0 {
                bro = buf_rsrc(ro, 4u * cnt_o);
     const uint32_t blk = vb.blk;
     double acc = 0.0;
-    uint32_t sb, se;
-    block_span_of(cnt, blk, vb.G, sb, se);
-    for (uint32_t h0 = sb + threadIdx.x; h0 < se; h0 += kU * kBlock) {
+    uint32_t ch0, cst, cend;
+    xcd_chunks(cnt, blk, vb.G, ch0, cst, cend);
+    for (uint32_t ch = ch0; ch < cend; ch += kU * cst) {
         uint16_t nb[kU];
         float rv[kU], zv[kU][6], rov[kU], zov[kU];
         uint32_t cnat[kU];  // natural cell index (stretched grids)
 #pragma unroll
         for (int t = 0; t < kU; t++) {
-            const uint32_t h = h0 + t * kBlock;  // reads past cnt return 0
+            const uint32_t h = (ch + t * cst) * kBlock + threadIdx.x;  // reads past cnt return 0
             const uint32_t cn = 2 * h + split_parity(L, h, col);
             nb[t] = buf_ldu16(bcode, h);
             rv[t] = buf_ldf(br, h);
@@ -426,8 +428,8 @@ struct KB_mg_smooth0 {
         if (late(stop)) return;
 #pragma unroll
         for (int t = 0; t < kU; t++) {
-            const uint32_t h = h0 + t * kBlock;
-            if (h >= se) break;
+            const uint32_t h = (ch + t * cst) * kBlock + threadIdx.x;
+            if (ch + t * cst >= cend) break;  // h past cnt: the store is dropped, rv and zov are 0
             float v = 0.f;
             if (nb[t] & NB_SELF) {
                 const Cof<ST> cf = cof_at<ST>(g, cnat[t]);
@@ -662,15 +664,15 @@ struct KB_mg_resid0 {
     const auto bcode = buf_rsrc(L.nbs, 2u * g.nc), bz = buf_rsrc(z, 4u * g.nc),
                br = buf_rsrc(r, 4u * g.nc), bred = buf_rsrc(z, 4u * H),
                bblack = buf_rsrc(z + H, 4u * (g.nc - H)), bres = buf_rsrc(res, 4u * g.nc);
-    uint32_t sb, se;
-    block_span_of(g.nc, vb.blk, vb.G, sb, se);
-    for (uint32_t s0 = sb + threadIdx.x; s0 < se; s0 += kU2 * kBlock) {
+    uint32_t ch0, cst, cend;
+    xcd_chunks(g.nc, vb.blk, vb.G, ch0, cst, cend);
+    for (uint32_t ch = ch0; ch < cend; ch += kU2 * cst) {
         uint16_t nb[kU2];
         float rv[kU2], zs[kU2], zv[kU2][6];
         uint32_t cnat[kU2];  // natural cell index (stretched grids)
 #pragma unroll
         for (int t = 0; t < kU2; t++) {
-            const uint32_t s = s0 + t * kBlock;
+            const uint32_t s = (ch + t * cst) * kBlock + threadIdx.x;  // reads past nc return 0
             const uint32_t col = s >= H ? 1u : 0u;
             const uint32_t h = s - (col ? H : 0u);
             const uint32_t cn = 2 * h + split_parity(L, h, col);
@@ -683,8 +685,8 @@ struct KB_mg_resid0 {
         if (late(stop)) return;
 #pragma unroll
         for (int t = 0; t < kU2; t++) {
-            const uint32_t s = s0 + t * kBlock;
-            if (s >= se) break;
+            const uint32_t s = (ch + t * cst) * kBlock + threadIdx.x;
+            if (ch + t * cst >= cend) break;  // stores past nc are dropped
             float v = 0.f;
             if (nb[t] & NB_SELF) {
                 const Cof<ST> cf = cof_at<ST>(g, cnat[t]);
@@ -781,15 +783,15 @@ struct KB_mg_post_black0 {
     const auto bcode = buf_rsrc(L.nbs + H, 2u * cnt), br = buf_rsrc((const float *)L.r + H, 4u * cnt),
                bred = buf_rsrc(L.z, 4u * H), bzc = buf_rsrc(zc, (uint32_t)sizeof(TC) * C.n),
                bz = buf_rsrc(zb, 4u * cnt);
-    uint32_t sb, se;
-    block_span_of(cnt, vb.blk, vb.G, sb, se);
-    for (uint32_t h0 = sb + threadIdx.x; h0 < se; h0 += kU2 * kBlock) {
+    uint32_t ch0, cst, cend;
+    xcd_chunks(cnt, vb.blk, vb.G, ch0, cst, cend);
+    for (uint32_t ch = ch0; ch < cend; ch += kU2 * cst) {
         uint16_t nb[kU2];
         float rv[kU2], zv[kU2][6];
         uint32_t cnat[kU2];  // natural cell index (stretched grids)
 #pragma unroll
         for (int t = 0; t < kU2; t++) {
-            const uint32_t h = h0 + t * kBlock;  // reads past cnt return 0
+            const uint32_t h = (ch + t * cst) * kBlock + threadIdx.x;  // reads past cnt return 0
             nb[t] = buf_ldu16(bcode, h);
             rv[t] = buf_ldf(br, h);
             const uint32_t c = 2 * h + split_parity(L, h, 1u);
@@ -807,8 +809,8 @@ struct KB_mg_post_black0 {
         if (late(stop)) return;
 #pragma unroll
         for (int t = 0; t < kU2; t++) {
-            const uint32_t h = h0 + t * kBlock;
-            if (h >= se) break;
+            const uint32_t h = (ch + t * cst) * kBlock + threadIdx.x;
+            if (ch + t * cst >= cend) break;  // stores past cnt are dropped
             float v = 0.f;
             if (nb[t] & NB_SELF) {
                 const Cof<ST> cf = cof_at<ST>(g, cnat[t]);
@@ -1314,15 +1316,15 @@ struct KB_pcg_dirq {
                pb = buf_rsrc(p + H, 4u * nb_black), qr = buf_rsrc(q, 4u * H),
                qb = buf_rsrc(q + H, 4u * nb_black);
     double acc = 0.0;
-    uint32_t hb, he;
-    block_span_of(H, blk, vb.G, hb, he);
-    for (uint32_t h0 = hb + threadIdx.x; h0 < he; h0 += kU2 * kBlock) {
+    uint32_t ch0, cst, cend;
+    xcd_chunks(H, blk, vb.G, ch0, cst, cend);
+    for (uint32_t ch = ch0; ch < cend; ch += kU2 * cst) {
         uint16_t nr[kU2], nk[kU2];
         float zrv[kU2], rv[kU2], prv[kU2], qrv[kU2], zbv[kU2], pbv[kU2], qbv[kU2], zn[kU2][6];
         uint32_t cnat[kU2];  // natural cell index (stretched grids)
 #pragma unroll
         for (int t = 0; t < kU2; t++) {
-            const uint32_t h = h0 + t * kBlock;
+            const uint32_t h = (ch + t * cst) * kBlock + threadIdx.x;  // reads past an end return 0
             const uint32_t cn = 2 * h + split_parity(L, h, 1u);
             nr[t] = buf_ldu16(cr, h);
             zrv[t] = buf_ldf(zr, h);
@@ -1340,8 +1342,8 @@ struct KB_pcg_dirq {
         }
 #pragma unroll
         for (int t = 0; t < kU2; t++) {
-            const uint32_t h = h0 + t * kBlock;
-            if (h >= he) break;
+            const uint32_t h = (ch + t * cst) * kBlock + threadIdx.x;
+            if (ch + t * cst >= cend) break;  // slots past H: zero reads, dropped stores
             const float azr = (nr[t] & NB_SELF) ? rv[t] : 0.f;
             float azb = 0.f;
             if (nk[t] & NB_SELF) {
@@ -1660,9 +1662,43 @@ __device__ __forceinline__ void row64(const Geo &g, uint32_t c, uint16_t m, F va
     off = of;
 }
 
+// The same row from the six neighbour values already gathered (faces in the
+// order -x, +x, -y, +y, -z, +z; a value whose face does not couple is not
+// used), (i, j, k) = ijk for the per-face scales of a stretched grid.
+template <bool ST>
+__device__ __forceinline__ void row64v(const Geo &g, const uint32_t ijk[3], uint16_t m,
+                                       const double val[6], double &diag, double &off)
+{
+    double sc[6];
+    if constexpr (ST) {
+        const double *L = g.lap;
+        sc[0] = L[ijk[0]]; sc[1] = L[g.nx + ijk[0]];
+        sc[2] = L[2 * g.nx + ijk[1]]; sc[3] = L[2 * g.nx + g.ny + ijk[1]];
+        sc[4] = L[2 * g.nx + 2 * g.ny + ijk[2]]; sc[5] = L[2 * g.nx + 2 * g.ny + g.nz + ijk[2]];
+    } else {
+        sc[0] = sc[1] = g.sxd;
+        sc[2] = sc[3] = g.syd;
+        sc[4] = sc[5] = g.szd;
+    }
+    double dg = 0.0, of = 0.0;
+#pragma unroll
+    for (int q = 0; q < 6; q++) {
+        if (m & (NB_UNK0 << q)) { dg += sc[q]; of += sc[q] * val[q]; }
+        else if (m & (NB_OUT0 << q)) dg += 2.0 * sc[q];
+    }
+    diag = dg < 1e-30 ? 1.0 : dg;
+    off = of;
+}
+
 // x_out = x_in + e on unknowns; r = b - A x_out (FP64); r32 = (float) r,
 // weighted by the cell's relative volume on a stretched grid (the inner
-// solve's symmetric form); partials of ||r||^2
+// solve's symmetric form); partials of ||r||^2.
+// Each thread takes kU2 cells of its block's chunks (xcd_chunks) and issues every load of
+// both (code, b, x, own e, the six x and e neighbours) before using any: the
+// colour of a cell, which places its e values in the split layout, comes
+// from its coordinates rather than from its code, and the neighbour gathers
+// go through buffer resources (out of range reads give 0; values of faces
+// that do not couple are dropped by the code, in the row's face order).
 template <bool ST>
 struct KB_dc_residual {
     static __device__ __forceinline__ void run(const VB &vb, const Geo &g,
@@ -1672,44 +1708,82 @@ struct KB_dc_residual {
                                                const RedScratch &red)
 {
     __shared__ double lds[kBlock / kWave];
-    if (m->solve_done) return;
+    // the solve's stop flag, tested after the first loads (see stop_flags)
+    const uint32_t stop = __builtin_amdgcn_raw_buffer_load_b32(
+        buf_rsrc(m, sizeof(MgState)), (int)offsetof(MgState, solve_done), 0, 0);
     // x_in is the buffer of the current pass parity, x_out the other one
     const double *__restrict__ xin = (m->passes & 1) ? xb : xa;
     double *__restrict__ xout = (m->passes & 1) ? xa : xb;
     const uint32_t blk = vb.blk;
+    const uint32_t si = (uint32_t)g.ny * g.nz, sj = g.nz, H = (g.nc + 1) / 2;
+    const auto bx = buf_rsrc(xin, 8u * g.nc), be = buf_rsrc(e, 4u * g.nc);
+    const auto bred = buf_rsrc(e, 4u * H), bblack = buf_rsrc(e + H, 4u * (g.nc - H));
     double acc = 0.0;
-    uint32_t sb, se;
-    block_span_of(g.nc, blk, vb.G, sb, se);
-    const uint32_t H = (g.nc + 1) / 2;
-    for (uint32_t c = sb + threadIdx.x; c < se; c += kBlock) {
-        const uint16_t nb = g.nbm[c];
-        const uint32_t sc = sown(c, nb, H);  // e and r32: split layout
-        const bool red_slot = !(nb & NB_COLOR);
-        if (!(nb & NB_SELF)) {
-            xout[c] = 0.0;
-            r32[sc] = 0.f;
-            if (red_slot) z0[sc] = 0.f;
-            continue;
+    uint32_t ch0, cstep, cend;
+    xcd_chunks(g.nc, blk, vb.G, ch0, cstep, cend);
+    for (uint32_t ch = ch0; ch < cend; ch += kU2 * cstep) {
+        const uint32_t c0 = ch * kBlock + threadIdx.x;
+        uint16_t nb[kU2];
+        double bv[kU2], xv[kU2], xn[kU2][6];
+        float ev[kU2], en[kU2][6];
+        uint32_t cc[kU2], col[kU2], ijk[kU2][3];
+        bool in[kU2];
+#pragma unroll
+        for (int t = 0; t < kU2; t++) {
+            const uint32_t ct = (ch + t * cstep) * kBlock + threadIdx.x;
+            in[t] = ch + t * cstep < cend && ct < g.nc;
+            const uint32_t c = in[t] ? ct : (c0 < g.nc ? c0 : g.nc - 1);
+            const uint32_t i = fdiv(c, g.f_nynz), rem = c - i * si;
+            const uint32_t j = fdiv(rem, g.f_nz), k = rem - j * sj;
+            cc[t] = c;
+            col[t] = (i + j + k) & 1u;
+            ijk[t][0] = i; ijk[t][1] = j; ijk[t][2] = k;
+            nb[t] = g.nbm[c];
+            bv[t] = b[c];
+            xv[t] = xin[c];
+            ev[t] = e[(col[t] ? H : 0u) + (c >> 1)];
+            const auto bo = col[t] ? bred : bblack;  // the other colour's e
+            const uint32_t q[6] = {c - si, c + si, c - sj, c + sj, c - 1u, c + 1u};
+#pragma unroll
+            for (int f = 0; f < 6; f++) {
+                xn[t][f] = buf_ldd(bx, q[f]);
+                en[t][f] = buf_ldf(bo, q[f] >> 1);
+            }
         }
-        const float *eo = e + sother(nb, H);
-        double diag, off;
-        row64<ST>(g, c, nb, [&](uint32_t q) { return xin[q] + (double)eo[q >> 1]; }, diag, off);
-        const double xc = xin[c] + (double)e[sc];
-        const double rv = b[c] - (diag * xc - off);
-        xout[c] = xc;
-        if constexpr (ST) {
-            const uint32_t i = fdiv(c, g.f_nynz), rem = c - i * (uint32_t)(g.ny * g.nz);
-            const uint32_t j = fdiv(rem, g.f_nz), k = rem - j * (uint32_t)g.nz;
-            const double vol = (double)g.mgf[i].x * (double)g.mgf[g.nx + j].x * (double)g.mgf[g.nx + g.ny + k].x;
-            const float rf = (float)(rv * vol);
-            r32[sc] = rf;
-            if (red_slot) z0[sc] = rf / diag_t<ST>(g, nb, 0.f, cof_at<ST>(g, c));
-        } else {
-            const float rf = (float)rv;
-            r32[sc] = rf;
-            if (red_slot) z0[sc] = rf / diag0m(g, nb);  // the V-cycle's first (red, z = 0) sweep
+        if (late(stop)) return;
+#pragma unroll
+        for (int t = 0; t < kU2; t++) {
+            if (!in[t]) continue;
+            const uint32_t c = cc[t];
+            const uint32_t sc = (col[t] ? H : 0u) + (c >> 1);  // e and r32: split layout
+            const bool red_slot = col[t] == 0u;
+            if (!(nb[t] & NB_SELF)) {
+                xout[c] = 0.0;
+                r32[sc] = 0.f;
+                if (red_slot) z0[sc] = 0.f;
+                continue;
+            }
+            double vals[6];
+#pragma unroll
+            for (int f = 0; f < 6; f++) vals[f] = xn[t][f] + (double)en[t][f];
+            double diag, off;
+            row64v<ST>(g, ijk[t], nb[t], vals, diag, off);
+            const double xc = xv[t] + (double)ev[t];
+            const double rv = bv[t] - (diag * xc - off);
+            xout[c] = xc;
+            if constexpr (ST) {
+                const double vol = (double)g.mgf[ijk[t][0]].x * (double)g.mgf[g.nx + ijk[t][1]].x *
+                                   (double)g.mgf[g.nx + g.ny + ijk[t][2]].x;
+                const float rf = (float)(rv * vol);
+                r32[sc] = rf;
+                if (red_slot) z0[sc] = rf / diag_t<ST>(g, nb[t], 0.f, cof_at<ST>(g, c));
+            } else {
+                const float rf = (float)rv;
+                r32[sc] = rf;
+                if (red_slot) z0[sc] = rf / diag0m(g, nb[t]);  // the V-cycle's first (red, z = 0) sweep
+            }
+            acc += rv * rv;
         }
-        acc += rv * rv;
     }
     block_partial(acc, pslot(red, SLOT_UPDATE), blk, lds);
 }

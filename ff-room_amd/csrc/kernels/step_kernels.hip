@@ -761,47 +761,89 @@ __device__ __forceinline__ R corrected(const Geo &g, const StepParams &sp, const
 // fp64: x -= dt_rho*gp exactly as the reference (:345); the generic version
 // rounds dt_rho*gp to R first, which for R = double is the same operation.
 
-// One block section per face array (u, v, w), one face per thread.
+// One thread per cell: the low u, v and w faces of cell (i, j, k) (flat
+// indices c, c + i nz and c + i ny + j), so p[c] is read once for its three
+// faces and the x, y, z neighbours below come from the same or a recent
+// plane; the cells of the last plane, row and column also take the high
+// boundary face of their axis. Each face is computed exactly as corrected()
+// does; an outflow face copies the corrected face one step inside through
+// the general path.
 template <typename R>
 struct KB_correct {
+    static __device__ __forceinline__ R face(const Geo &g, const StepParams &sp, const R *fs,
+                                             const double *p, int axis, int i, int j, int k,
+                                             uint32_t e, uint32_t step, uint16_t code, R x,
+                                             double pc, double pm)
+    {
+        if (code & (FC_OUT_LO | FC_OUT_HI)) {  // outflow patch: the face one step inside
+            const int d = (code & FC_OUT_LO) ? 1 : -1;
+            const uint32_t e2 = d > 0 ? e + step : e - step;
+            const uint16_t *cod = g.fcode + (axis >= 1 ? g.nu : 0u) + (axis == 2 ? g.nv : 0u);
+            return corrected<R>(g, sp, fs, p, axis, i + d * (axis == 0), j + d * (axis == 1),
+                                k + d * (axis == 2), e2, cod[e2]);
+        }
+        if (!(code & FC_BOUND)) {
+            const double gp = axis == 0   ? by_h64<R>(pc - pm, g.cx[i], g.icx[i])
+                              : axis == 1 ? by_h64<R>(pc - pm, g.cy[j], g.icy[j])
+                                          : by_h64<R>(pc - pm, g.cz[k], g.icz[k]);
+            x = x - (R)(sp.dt_rho * gp);
+        }
+        return apply_mask<R>(g, code, axis, x);
+    }
+
     static __device__ __forceinline__ void run(const VB &vb, const Geo &g, const StepParams &sp,
-                                               const FaceSections &fsec, const R *us, const R *vs,
-                                               const R *ws, const double *p, R *u, R *v, R *w,
-                                               StepRec *rec)
+                                               const R *__restrict__ us, const R *__restrict__ vs,
+                                               const R *__restrict__ ws, const double *__restrict__ p,
+                                               R *__restrict__ u, R *__restrict__ v,
+                                               R *__restrict__ w, StepRec *rec)
 {
     __shared__ double lds[kBlock / kWave];
     double dv = 0.0;
     bool bad = false;
-    uint32_t lb, G, sb, se;
-    const int axis = face_section_of(fsec, vb.blk, vb.G, lb, G);  // uniform per block
-    // all three counts loaded, then combined: a select between the loads
-    // would become one load at a selected offset into g, which keeps a copy
-    // of g in scratch memory
-    const uint32_t nu = g.nu, nv = g.nv, nw = g.nw;
-    const uint32_t n = nu + (axis >= 1 ? nv - nu : 0u) + (axis == 2 ? nw - nv : 0u);
-    const uint16_t *code = g.fcode + (axis >= 1 ? g.nu : 0u) + (axis == 2 ? g.nv : 0u);
-    const R *fs = axis == 0 ? us : (axis == 1 ? vs : ws);
-    R *out = axis == 0 ? u : (axis == 1 ? v : w);
-    // index step to the next face along the axis
-    const uint32_t step = axis == 0 ? (uint32_t)g.ny * g.nz : (axis == 1 ? (uint32_t)g.nz : 1u);
-    block_span_of(n, lb, G, sb, se);
-    for (uint32_t e = sb + threadIdx.x; e < se; e += kBlock) {
-        const uint16_t c = code[e];
-        int i, j, k;
-        face_ijk(g, axis, e, i, j, k);
-        R val;
-        if (c & (FC_OUT_LO | FC_OUT_HI)) {  // outflow patch: the face one step inside
-            const int d = (c & FC_OUT_LO) ? 1 : -1;
-            const uint32_t e2 = d > 0 ? e + step : e - step;
-            val = corrected<R>(g, sp, fs, p, axis, i + d * (axis == 0), j + d * (axis == 1),
-                               k + d * (axis == 2), e2, code[e2]);
-        } else {
-            val = corrected<R>(g, sp, fs, p, axis, i, j, k, e, c);
-        }
-        const R old = out[e];
+    const uint32_t si = (uint32_t)g.ny * g.nz, nz = (uint32_t)g.nz;
+    const uint16_t *cu = g.fcode, *cv = g.fcode + g.nu, *cw = g.fcode + g.nu + g.nv;
+    auto put = [&](R *out, uint32_t e, R val, R old) {
         out[e] = val;
         dv = fmax(dv, fabs((double)val - (double)old));  // fmax drops NaN as std::max does
         bad |= !isfinite((double)val);
+    };
+    uint32_t ch0, cstep, cend;
+    xcd_chunks(g.nc, vb.blk, vb.G, ch0, cstep, cend);
+    for (uint32_t ch = ch0; ch < cend; ch += cstep) {
+        const uint32_t c = ch * kBlock + threadIdx.x;
+        if (c >= g.nc) break;
+        const uint32_t i = fdiv(c, g.f_nynz), rem = c - i * si;
+        const uint32_t j = fdiv(rem, g.f_nz), k = rem - j * nz;
+        const uint32_t ev = c + i * nz, ew = c + i * (uint32_t)g.ny + j;
+        // every load of the cell's three faces before any use (neighbours
+        // below a wall read the cell itself: those faces are boundary faces)
+        const uint16_t ku = cu[c], kv = cv[ev], kw = cw[ew];
+        const double pc = p[c], pxm = p[i ? c - si : c], pym = p[j ? c - nz : c],
+                     pzm = p[k ? c - 1 : c];
+        const R xu = us[c], xv = vs[ev], xw = ws[ew];
+        const R ou = u[c], ov = v[ev], ow = w[ew];
+        // the high w face of the column's last cell (a row end falls in most
+        // waves, so its loads go out with the others)
+        const bool top = k + 1 == nz;
+        const uint32_t eh = top ? ew + 1 : ew;
+        const uint16_t kh = cw[eh];
+        const R xh = ws[eh], oh = w[eh];
+        const R fu = face(g, sp, us, p, 0, i, j, k, c, si, ku, xu, pc, pxm);
+        const R fv = face(g, sp, vs, p, 1, i, j, k, ev, nz, kv, xv, pc, pym);
+        const R fw = face(g, sp, ws, p, 2, i, j, k, ew, 1u, kw, xw, pc, pzm);
+        put(u, c, fu, ou);
+        put(v, ev, fv, ov);
+        put(w, ew, fw, ow);
+        // the high boundary faces (no gradient; masks or an outflow copy)
+        if (top) put(w, eh, face(g, sp, ws, p, 2, i, j, k + 1, eh, 1u, kh, xh, 0.0, 0.0), oh);
+        if (i + 1 == (uint32_t)g.nx) {
+            const uint32_t e = c + si;
+            put(u, e, face(g, sp, us, p, 0, i + 1, j, k, e, si, cu[e], us[e], 0.0, 0.0), u[e]);
+        }
+        if (j + 1 == (uint32_t)g.ny) {
+            const uint32_t e = ev + nz;
+            put(v, e, face(g, sp, vs, p, 1, i, j + 1, k, e, nz, cv[e], vs[e], 0.0, 0.0), v[e]);
+        }
     }
     if (bad) rec->nonfinite = 1;
     const double m = block_max(dv, lds);
@@ -809,23 +851,19 @@ struct KB_correct {
 }
 };
 template <typename R>
-__global__ void __launch_bounds__(kBlock) k_correct(Geo g, StepParams sp, FaceSections fsec,
-                                                    const R *us, const R *vs, const R *ws,
-                                                    const double *p, R *u, R *v, R *w,
-                                                    StepRec *rec)
+__global__ void __launch_bounds__(kBlock) k_correct(Geo g, StepParams sp, const R *us,
+                                                    const R *vs, const R *ws, const double *p,
+                                                    R *u, R *v, R *w, StepRec *rec)
 {
-    KB_correct<R>::run(vb_here(), g, sp, fsec, us, vs, ws, p, u, v, w, rec);
+    KB_correct<R>::run(vb_here(), g, sp, us, vs, ws, p, u, v, w, rec);
 }
 
 template <typename R>
 void launch_correct(hipStream_t s, const Geo &g, const StepParams &sp, const R *us, const R *vs,
                     const R *ws, const double *p, R *u, R *v, R *w, StepRec *rec)
 {
-    const uint64_t n = (uint64_t)g.nu + g.nv + g.nw;
-    const uint32_t G = std::max(3u, grid_for(n));
-    const FaceSections fs = face_sections(g.nu, g.nv, g.nw, G);
-    FFR_GANG((KB_correct<R>), GB_PLAIN, k_correct<R>, dim3(G), dim3(kBlock), 0, s, g, sp, fs, us,
-             vs, ws, p, u, v, w, rec);
+    FFR_GANG((KB_correct<R>), GB_PLAIN, k_correct<R>, dim3(grid_for(g.nc)), dim3(kBlock), 0, s,
+             g, sp, us, vs, ws, p, u, v, w, rec);
 }
 
 // ---- divergence (FluidSolver.cpp:470-484) --------------------------------
