@@ -576,13 +576,15 @@ def main():
     ap.add_argument("--row-steps", type=int, default=20)
     ap.add_argument("--batched", type=int, default=32,
                     help="c4 rooms timed in both forms with continuous batching (0: skip)")
-    ap.add_argument("--batched-steps", type=int, default=60)
+    ap.add_argument("--batched-steps", type=int, default=300)
     ap.add_argument("--c1-reference", type=int, default=500,
                     help="steps of the reference's single-threaded c1 run (0: skip)")
     ap.add_argument("--c4", type=int, default=64, help="rooms in the batched sample")
     ap.add_argument("--c4-steps", type=int, default=300)
     ap.add_argument("--c5", type=int, default=40, help="requests in the stream sample")
     ap.add_argument("--c5-steps", type=int, default=200)
+    ap.add_argument("--kernel-warm", type=int, default=2,
+                    help="steps before the --kernel-only launches")
     ap.add_argument("--kernel-only", default=None,
                     help="time the named kernels (comma list; for rocprofv3 --pmc runs) and exit")
     args = ap.parse_args()
@@ -618,7 +620,7 @@ def main():
         # the named kernels, then the library's plain FP64 copy of nx*ny*nz
         # doubles, whose memory counters calibrate the kernel's
         d = device.create(ctx, ps)
-        for _ in range(2):
+        for _ in range(args.kernel_warm):
             d.step()
         for name in [*args.kernel_only.split(","), "copy"]:
             ms, nbytes = d.prof_kernel(name, 20)

@@ -103,7 +103,11 @@ size_t mg_layout(char *base, const Dims &D, bool bf16, bool dense, Multigrid *M,
         float *cx = (float *)take(4ull * n), *cy = (float *)take(4ull * n), *cz = (float *)take(4ull * n);
         float *dg = (float *)take(4ull * n), *rd = (float *)take(4ull * n);
         float *gx = (float *)take(4ull * n), *gy = (float *)take(4ull * n), *gz = (float *)take(4ull * n);
-        void *z = take(vs * n), *r = take(vs * n);
+        // z with si zero pads at both ends (the tail reads a cell's six
+        // neighbours without bounds tests when the level runs from global memory)
+        const size_t si = (size_t)D.n[l][1] * D.n[l][2];
+        char *zp = take(vs * (n + 2 * si));
+        void *z = zp ? zp + vs * si : nullptr, *r = take(vs * n);
         if (M) {
             MgLevel &L = M->lv[l];
             L.active = act; L.cx = cx; L.cy = cy; L.cz = cz; L.diag = dg; L.rdiag = rd;
@@ -315,6 +319,12 @@ int mg_create(ffr_scene *s)
             M.tail_first = u;
         }
         M.tail_lds = M.tail_first >= 0 ? acc + scratch(M.tail_first) : 0;
+        // the finer levels of at most mg_tail_cells cells join the tail with
+        // their vectors and operator left in global memory
+        M.tail_lds_from = M.tail_first;
+        if (s->prm.mg_tail_cells > 0 && M.tail_first > 1)
+            while (M.tail_first > 1 && M.lv[M.tail_first - 1].n <= (uint32_t)s->prm.mg_tail_cells)
+                M.tail_first--;
     }
     M.pre_sweeps = std::max(1, s->prm.mg_pre_sweeps);
     M.level_sweeps = std::max(1, s->prm.mg_level_sweeps);
@@ -390,9 +400,12 @@ static bool vcycle(ffr_scene *s, int l, bool first_red_done, bool dot = false)
         for (int k = l; k < M.levels; k++) {
             tl.lv[k] = M.lv[k];
             tl.off[k] = off;
-            off += (uint32_t)tail_level_bytes(M.lv[k], sizeof(TC), k >= M.tail_staged);
+            if (k >= M.tail_lds_from)  // levels before it run from global memory
+                off += (uint32_t)tail_level_bytes(M.lv[k], sizeof(TC), k >= M.tail_staged);
         }
-        tl.scratch_off = off;  // FP32 restriction scratch of the first level's size
+        tl.scratch_off = off;  // FP32 restriction scratch of the first LDS level's size
+        tl.gscratch = M.sc;    // level 0's restriction scratch, free while the tail runs
+        tl.lds_from = M.tail_lds_from;
         tl.first = l;
         tl.last = M.levels - 1;
         tl.staged_from = M.tail_staged;
@@ -743,3 +756,16 @@ double mg_prof_launch(ffr_scene *s, const char *name, std::function<void()> *lau
 }
 
 }  // namespace ffr
+
+extern "C" int ffr_scene_mg_layout(const ffr_scene *s, int *levels, int *tail_first,
+                                   int *tail_lds_from, unsigned int *cells)
+{
+    if (!s || !levels) return FFR_E_INVALID;
+    const ffr::Multigrid &M = s->mg;
+    *levels = M.levels;
+    if (tail_first) *tail_first = M.levels ? M.tail_first : -1;
+    if (tail_lds_from) *tail_lds_from = M.levels ? M.tail_lds_from : -1;
+    if (cells)
+        for (int l = 0; l < M.levels; l++) cells[l] = M.lv[l].n;
+    return FFR_OK;
+}

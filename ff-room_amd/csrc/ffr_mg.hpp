@@ -87,10 +87,12 @@ struct Multigrid {
     // t = 1 / (centre distance * mean width), 0 at the walls
     float4 *mgf;
     int pre_sweeps = 2, level_sweeps = 2, coarse_sweeps = 8;
-    // levels tail_first .. levels-1 run as one single-workgroup kernel with
-    // their z and r in LDS (-1: none); tail_lds = its LDS bytes; the
-    // operators of levels tail_staged .. levels-1 are staged in LDS too
-    int tail_first = -1, tail_staged = -1;
+    // levels tail_first .. levels-1 run as one single-workgroup kernel;
+    // levels tail_first .. tail_lds_from-1 keep z, r and the operator in
+    // global memory (mg_tail_cells), the others their z and r in LDS (-1:
+    // no tail); tail_lds = its LDS bytes; the operators of levels
+    // tail_staged .. levels-1 are staged in LDS too
+    int tail_first = -1, tail_staged = -1, tail_lds_from = -1;
     size_t tail_lds = 0;
     // dense (generalised) inverse of the coarsest level's operator, FP32,
     // row-major n x n, n <= kCoarsest: the tail's coarse solve is one
@@ -109,7 +111,10 @@ struct Multigrid {
 // so a cell reads its six neighbours without bounds tests: the coupling
 // across a wall is 0) and its r (vs bytes each). The first tail level may be unstaged: its
 // operator is then read from global memory (it stays in L2) and only z and
-// r take LDS.
+// r take LDS. Levels before lds_from (mg_tail_cells) keep everything in
+// global memory: their z arrays carry si zero pads at both ends in the arena
+// as the LDS copies do, and their passes are ordered by the workgroup's
+// barriers like the LDS ones (one workgroup, one CU).
 constexpr int kTailThreads = 1024;
 // the hierarchy stops at a level of at most this many cells; such a coarsest
 // level is solved with a dense inverse
@@ -149,8 +154,10 @@ inline size_t tail_level_bytes(const MgLevel &L, size_t vs, bool staged)
 struct TailLevels {
     MgLevel lv[kMaxLevels];
     uint32_t off[kMaxLevels];
-    uint32_t scratch_off; // FP32 residuals of a level being restricted (first level's size)
+    uint32_t scratch_off; // FP32 residuals of an LDS level being restricted (size of lds_from)
+    float *gscratch;      // FP32 residuals of a global level being restricted (>= its n)
     int first, last;
+    int lds_from;         // first level with its z and r in LDS (levels before: global)
     int staged_from;      // first level whose operator is in LDS (first or first + 1)
     int sweeps, coarse_sweeps;
     const float *ginv;    // dense inverse of the coarsest operator (n_last^2), or null

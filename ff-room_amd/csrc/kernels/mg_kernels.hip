@@ -935,9 +935,15 @@ __global__ void __launch_bounds__(1024) k_mg_coarse(Geo g, MgLevel L, T *z, cons
 // residual, piecewise-constant prolongation, symmetric sweeps on the coarsest
 // level). Each level's operator is staged in LDS once and its z and r live in
 // LDS as T; phases are separated by workgroup barriers. Input r and output z
-// of level tl.first are exchanged with global memory.
-template <typename T>
+// of level tl.first are exchanged with global memory. Levels before
+// tl.lds_from (scenes served side by side: mg_tail_cells) run the same
+// phases on their global arrays - one CU and one launch for what would be
+// several launches per level - ordered by the same workgroup barriers.
+// A view's type says where its vectors are (GL: global), so the LDS levels'
+// accesses stay LDS instructions.
+template <typename T, bool GL>
 struct TailView {  // one level of the tail
+    using elem = T;
     uint32_t n, H, si, sj;
     int nx, ny, nz;
     int shx, shy, shz;
@@ -947,13 +953,13 @@ struct TailView {  // one level of the tail
     unsigned char *base;     // the level's LDS region
     TailCarve tc;
     const MgLevel *G;
-    T *z, *r;  // LDS; z[-si .. n + si) readable, the pads 0
+    T *z, *r;  // LDS (global when GL); z[-si .. n + si) readable, the pads 0
 };
 
-template <typename T>
-__device__ __forceinline__ TailView<T> tail_view(const TailLevels &tl, unsigned char *smem, int l)
+template <typename T, bool GL>
+__device__ __forceinline__ TailView<T, GL> tail_view(const TailLevels &tl, unsigned char *smem, int l)
 {
-    TailView<T> v;
+    TailView<T, GL> v;
     const MgLevel &L = tl.lv[l];
     v.n = L.n;
     v.H = (L.n + 1) / 2;
@@ -971,12 +977,20 @@ __device__ __forceinline__ TailView<T> tail_view(const TailLevels &tl, unsigned 
     v.f_nynz = L.f_nynz;
     v.f_hz = L.f_hz;
     v.f_hyz = L.f_hyz;
-    v.staged = l >= tl.staged_from;
-    v.tc = tail_carve(v.n, v.si, v.sj, (uint32_t)sizeof(T), v.staged);
-    v.base = smem + tl.off[l];
     v.G = &tl.lv[l];
-    v.z = reinterpret_cast<T *>(v.base + v.tc.z);
-    v.r = reinterpret_cast<T *>(v.base + v.tc.r);
+    if constexpr (GL) {
+        v.staged = false;
+        v.base = nullptr;
+        v.tc = TailCarve{};
+        v.z = reinterpret_cast<T *>(L.z);
+        v.r = reinterpret_cast<T *>(L.r);
+    } else {
+        v.staged = l >= tl.staged_from;
+        v.tc = tail_carve(v.n, v.si, v.sj, (uint32_t)sizeof(T), v.staged);
+        v.base = smem + tl.off[l];
+        v.z = reinterpret_cast<T *>(v.base + v.tc.z);
+        v.r = reinterpret_cast<T *>(v.base + v.tc.r);
+    }
     return v;
 }
 
@@ -986,8 +1000,8 @@ struct TailOp {
     const float *cx, *cy, *cz, *diag, *rdiag;
     const uint8_t *act;
 };
-template <bool ST, typename T>
-__device__ __forceinline__ TailOp tail_op(const TailView<T> &L)
+template <bool ST, typename V>
+__device__ __forceinline__ TailOp tail_op(const V &L)
 {
     if (ST)
         return {reinterpret_cast<const float *>(L.base + L.tc.cx),
@@ -1000,8 +1014,8 @@ __device__ __forceinline__ TailOp tail_op(const TailView<T> &L)
 
 // natural index of slot m of colour col (cells 2m and 2m + 1 always differ
 // in colour; the parity formula is level 0's split-layout one)
-template <typename T>
-__device__ __forceinline__ uint32_t tail_cell(const TailView<T> &L, uint32_t m, uint32_t col)
+template <typename V>
+__device__ __forceinline__ uint32_t tail_cell(const V &L, uint32_t m, uint32_t col)
 {
     uint32_t b = col;
     if (L.par_nz) b ^= fdiv(m, L.f_hz) & 1u;
@@ -1013,9 +1027,8 @@ __device__ __forceinline__ uint32_t tail_cell(const TailView<T> &L, uint32_t m, 
 // faces on walls are 0, and the upper neighbours' couplings past the end
 // read staged zero pads or, for an operator in global memory, the next
 // array of the arena (finite) times a zero pad of x
-template <typename T>
-__device__ __forceinline__ float tail_row(const TailView<T> &L, const TailOp &o, const T *x,
-                                          uint32_t c)
+template <typename V, typename T>
+__device__ __forceinline__ float tail_row(const V &L, const TailOp &o, const T *x, uint32_t c)
 {
     const T *xc = x + c;
     float of = 0.f;
@@ -1037,8 +1050,8 @@ __device__ __forceinline__ float tail_row(const TailView<T> &L, const TailOp &o,
 // value, not whatever LDS held. Every load is issued for all U cells of a
 // thread before any is used (out-of-range slots read cell 0 and store
 // nothing).
-template <int U, bool ST, typename T>
-__device__ __forceinline__ void tail_pass_u(const TailView<T> &L, uint32_t color, int zero_start,
+template <int U, bool ST, typename V>
+__device__ __forceinline__ void tail_pass_u(const V &L, uint32_t color, int zero_start,
                                             uint32_t tid, uint32_t nthr)
 {
     const TailOp o = tail_op<ST>(L);
@@ -1068,8 +1081,8 @@ __device__ __forceinline__ void tail_pass_u(const TailView<T> &L, uint32_t color
 // The per-cell phases are instruction-bound: a thread takes 1, 2 or 4 cells
 // per round, as few as cover the level, so no slot is computed for nothing
 // while the loads of a thread's cells are still issued together.
-template <bool ST, typename T>
-__device__ __forceinline__ void tail_pass(const TailView<T> &L, uint32_t color, int zero_start,
+template <bool ST, typename V>
+__device__ __forceinline__ void tail_pass(const V &L, uint32_t color, int zero_start,
                                           uint32_t tid, uint32_t nthr)
 {
     if (L.H <= nthr) tail_pass_u<1, ST>(L, color, zero_start, tid, nthr);
@@ -1077,8 +1090,8 @@ __device__ __forceinline__ void tail_pass(const TailView<T> &L, uint32_t color, 
     else tail_pass_u<4, ST>(L, color, zero_start, tid, nthr);
 }
 
-template <typename T>
-__device__ __forceinline__ void tail_smooth(const TailView<T> &L, uint32_t color, int zero_start)
+template <typename V>
+__device__ __forceinline__ void tail_smooth(const V &L, uint32_t color, int zero_start)
 {
     if (L.staged) tail_pass<true>(L, color, zero_start, threadIdx.x, kTailThreads);
     else tail_pass<false>(L, color, zero_start, threadIdx.x, kTailThreads);
@@ -1086,8 +1099,8 @@ __device__ __forceinline__ void tail_smooth(const TailView<T> &L, uint32_t color
 }
 
 // residuals r - A z of level F into res (FP32), all cells
-template <int U, bool ST, typename T>
-__device__ __forceinline__ void tail_residual_u(const TailView<T> &F, float *res)
+template <int U, bool ST, typename V>
+__device__ __forceinline__ void tail_residual_u(const V &F, float *res)
 {
     const TailOp o = tail_op<ST>(F);
     for (uint32_t c0 = threadIdx.x; c0 < F.n; c0 += U * kTailThreads) {
@@ -1107,8 +1120,8 @@ __device__ __forceinline__ void tail_residual_u(const TailView<T> &F, float *res
         }
     }
 }
-template <bool ST, typename T>
-__device__ __forceinline__ void tail_residual(const TailView<T> &F, float *res)
+template <bool ST, typename V>
+__device__ __forceinline__ void tail_residual(const V &F, float *res)
 {
     if (F.n <= kTailThreads) tail_residual_u<1, ST>(F, res);
     else if (F.n <= 2 * kTailThreads) tail_residual_u<2, ST>(F, res);
@@ -1116,8 +1129,8 @@ __device__ __forceinline__ void tail_residual(const TailView<T> &F, float *res)
 }
 
 // F.z += prolonged C.z on unknown cells
-template <int U, bool ST, typename T>
-__device__ __forceinline__ void tail_prolong_u(const TailView<T> &F, const TailView<T> &C)
+template <int U, bool ST, typename VF, typename VC>
+__device__ __forceinline__ void tail_prolong_u(const VF &F, const VC &C)
 {
     const TailOp o = tail_op<ST>(F);
     for (uint32_t c0 = threadIdx.x; c0 < F.n; c0 += U * kTailThreads) {
@@ -1141,12 +1154,60 @@ __device__ __forceinline__ void tail_prolong_u(const TailView<T> &F, const TailV
         }
     }
 }
-template <bool ST, typename T>
-__device__ __forceinline__ void tail_prolong(const TailView<T> &F, const TailView<T> &C)
+template <bool ST, typename VF, typename VC>
+__device__ __forceinline__ void tail_prolong(const VF &F, const VC &C)
 {
     if (F.n <= kTailThreads) tail_prolong_u<1, ST>(F, C);
     else if (F.n <= 2 * kTailThreads) tail_prolong_u<2, ST>(F, C);
     else tail_prolong_u<4, ST>(F, C);
+}
+
+// rc = P^T res: each coarse cell of C sums its fine children
+template <typename VC>
+__device__ __forceinline__ void tail_restrict_sum(const MgLevel &F, const VC &C, const float *res)
+{
+    for (uint32_t I = threadIdx.x; I < C.n; I += kTailThreads) {
+        int ic, jc, kc;
+        decode(I, C.f_nynz, C.f_nz, C.ny, C.nz, ic, jc, kc);
+        const int i0 = ic << C.shx, j0 = jc << C.shy, k0 = kc << C.shz;
+        const int i1 = min(i0 + (1 << C.shx) - 1, F.nx - 1),
+                  j1 = min(j0 + (1 << C.shy) - 1, F.ny - 1),
+                  k1 = min(k0 + (1 << C.shz) - 1, F.nz - 1);
+        float sum = 0.f;
+        for (int a = i0; a <= i1; a++)
+            for (int b = j0; b <= j1; b++)
+                for (int d = k0; d <= k1; d++) sum += res[((uint32_t)a * F.ny + b) * F.nz + d];
+        stv(C.r, I, sum);
+    }
+}
+
+// one level of the way down: pre-smoothing, the residual into res (loads
+// issued together), then each coarse cell sums its children
+template <typename VF, typename VC>
+__device__ __forceinline__ void tail_down(const TailLevels &tl, const VF &F, const VC &C, float *res)
+{
+    for (int sw = 0; sw < tl.sweeps; sw++) {
+        tail_smooth(F, 0, sw == 0);
+        tail_smooth(F, 1, 0);
+    }
+    if (F.staged) tail_residual<true>(F, res);
+    else tail_residual<false>(F, res);
+    __syncthreads();
+    tail_restrict_sum(*F.G, C, res);
+    __syncthreads();
+}
+
+// one level of the way up: prolongation, then post-smoothing
+template <typename VF, typename VC>
+__device__ __forceinline__ void tail_up(const TailLevels &tl, const VF &F, const VC &C)
+{
+    if (F.staged) tail_prolong<true>(F, C);
+    else tail_prolong<false>(F, C);
+    __syncthreads();
+    for (int sw = 0; sw < tl.sweeps; sw++) {
+        tail_smooth(F, 1, 0);
+        tail_smooth(F, 0, 0);
+    }
 }
 
 template <typename T>
@@ -1155,9 +1216,10 @@ struct KB_mg_tail {
 {
     extern __shared__ __align__(16) unsigned char tail_smem[];
     if (stopped(m)) return;
-    for (int l = tl.first; l <= tl.last; l++) {  // stage the operators, clear the pads
+    const int lf = max(tl.first, tl.lds_from);  // the first level in LDS
+    for (int l = lf; l <= tl.last; l++) {  // stage the operators, clear the pads
         const MgLevel &G = tl.lv[l];
-        const TailView<T> L = tail_view<T>(tl, tail_smem, l);
+        const TailView<T, false> L = tail_view<T, false>(tl, tail_smem, l);
         if (L.staged) {
             float *cx = reinterpret_cast<float *>(L.base + L.tc.cx),
                   *cy = reinterpret_cast<float *>(L.base + L.tc.cy),
@@ -1182,44 +1244,30 @@ struct KB_mg_tail {
             stv(L.z + L.n, c, 0.f);
         }
     }
-    {
-        const TailView<T> F = tail_view<T>(tl, tail_smem, tl.first);
+    if (tl.first >= tl.lds_from) {  // the first level's r into LDS
+        const TailView<T, false> F = tail_view<T, false>(tl, tail_smem, tl.first);
         const T *rg = (const T *)tl.lv[tl.first].r;
         for (uint32_t c = threadIdx.x; c < F.n; c += kTailThreads) F.r[c] = rg[c];
     }
     __syncthreads();
-    // down: pre-smoothing and restriction
+    float *const lres = reinterpret_cast<float *>(tail_smem + tl.scratch_off);
+    // down: pre-smoothing and restriction (global levels' residuals go to
+    // the global scratch, LDS levels' to the LDS one)
     for (int l = tl.first; l < tl.last; l++) {
-        const TailView<T> F = tail_view<T>(tl, tail_smem, l), C = tail_view<T>(tl, tail_smem, l + 1);
-        for (int sw = 0; sw < tl.sweeps; sw++) {
-            tail_smooth<T>(F, 0, sw == 0);
-            tail_smooth<T>(F, 1, 0);
-        }
-        // residuals of the fine level into the scratch (loads issued
-        // together), then each coarse cell sums its children
-        float *res = reinterpret_cast<float *>(tail_smem + tl.scratch_off);
-        if (F.staged) tail_residual<true>(F, res);
-        else tail_residual<false>(F, res);
-        __syncthreads();
-        for (uint32_t I = threadIdx.x; I < C.n; I += kTailThreads) {
-            int ic, jc, kc;
-            decode(I, C.f_nynz, C.f_nz, C.ny, C.nz, ic, jc, kc);
-            const int i0 = ic << C.shx, j0 = jc << C.shy, k0 = kc << C.shz;
-            const int i1 = min(i0 + (1 << C.shx) - 1, F.nx - 1),
-                      j1 = min(j0 + (1 << C.shy) - 1, F.ny - 1),
-                      k1 = min(k0 + (1 << C.shz) - 1, F.nz - 1);
-            float sum = 0.f;
-            for (int a = i0; a <= i1; a++)
-                for (int b = j0; b <= j1; b++)
-                    for (int d = k0; d <= k1; d++) sum += res[((uint32_t)a * F.ny + b) * F.nz + d];
-            stv(C.r, I, sum);
-        }
-        __syncthreads();
+        if (l + 1 < tl.lds_from)
+            tail_down(tl, tail_view<T, true>(tl, tail_smem, l), tail_view<T, true>(tl, tail_smem, l + 1),
+                      tl.gscratch);
+        else if (l < tl.lds_from)
+            tail_down(tl, tail_view<T, true>(tl, tail_smem, l), tail_view<T, false>(tl, tail_smem, l + 1),
+                      tl.gscratch);
+        else
+            tail_down(tl, tail_view<T, false>(tl, tail_smem, l), tail_view<T, false>(tl, tail_smem, l + 1),
+                      lres);
     }
     // coarsest: z = G r with the dense inverse, or (red, black)^s then
     // (black, red)^s from z = 0
     {
-        const TailView<T> L = tail_view<T>(tl, tail_smem, tl.last);
+        const TailView<T, false> L = tail_view<T, false>(tl, tail_smem, tl.last);
         if (tl.ginv) {
             // thread i forms row i; G is symmetric, so reading G[j][i] for
             // consecutive i is a coalesced read of row j; r_j is broadcast
@@ -1254,17 +1302,15 @@ struct KB_mg_tail {
     }
     // up: prolongation and post-smoothing
     for (int l = tl.last - 1; l >= tl.first; l--) {
-        const TailView<T> F = tail_view<T>(tl, tail_smem, l), C = tail_view<T>(tl, tail_smem, l + 1);
-        if (F.staged) tail_prolong<true>(F, C);
-        else tail_prolong<false>(F, C);
-        __syncthreads();
-        for (int sw = 0; sw < tl.sweeps; sw++) {
-            tail_smooth<T>(F, 1, 0);
-            tail_smooth<T>(F, 0, 0);
-        }
+        if (l + 1 < tl.lds_from)
+            tail_up(tl, tail_view<T, true>(tl, tail_smem, l), tail_view<T, true>(tl, tail_smem, l + 1));
+        else if (l < tl.lds_from)
+            tail_up(tl, tail_view<T, true>(tl, tail_smem, l), tail_view<T, false>(tl, tail_smem, l + 1));
+        else
+            tail_up(tl, tail_view<T, false>(tl, tail_smem, l), tail_view<T, false>(tl, tail_smem, l + 1));
     }
-    {
-        const TailView<T> F = tail_view<T>(tl, tail_smem, tl.first);
+    if (tl.first >= tl.lds_from) {  // the first level's z back to global memory
+        const TailView<T, false> F = tail_view<T, false>(tl, tail_smem, tl.first);
         T *zg = (T *)tl.lv[tl.first].z;
         for (uint32_t c = threadIdx.x; c < F.n; c += kTailThreads) zg[c] = F.z[c];
     }

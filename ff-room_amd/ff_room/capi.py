@@ -74,7 +74,7 @@ class Params(C.Structure):
                 ("mg_bf16", C.c_int), ("mg_pre_sweeps", C.c_int), ("mg_level_sweeps", C.c_int),
                 ("mg_coarse_sweeps", C.c_int),
                 ("inner_tol", C.c_double), ("inner_max_iter", C.c_int), ("graphs", C.c_int),
-                ("mg_tail_kb", C.c_int), ("warm_start", C.c_int)]
+                ("mg_tail_kb", C.c_int), ("warm_start", C.c_int), ("mg_tail_cells", C.c_int)]
 
 
 class StepResult(C.Structure):
@@ -123,6 +123,8 @@ SIGNATURES = {
     "ffr_ctx_bytes_in_use": (C.c_size_t, [_P]),
     "ffr_ctx_bytes_cached": (C.c_size_t, [_P]),
     "ffr_ctx_trim": (C.c_size_t, [_P]),
+    "ffr_scene_mg_layout": (_I, [_P, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int),
+                                 C.POINTER(C.c_uint)]),
     "ffr_scene_footprint": (C.c_size_t, [C.POINTER(SceneDesc), C.POINTER(Params)]),
     "ffr_scene_create": (_I, [_P, C.POINTER(SceneDesc), C.POINTER(Params), C.POINTER(_P)]),
     "ffr_scene_destroy": (_V, [_P]),
@@ -496,6 +498,16 @@ class DeviceScene:
         d = C.c_double()
         self.ctx.check(self.ctx._l.ffr_T_mean(self.h, C.byref(d)), "ffr_T_mean")
         return d.value
+
+    def mg_layout(self) -> dict:
+        """The multigrid hierarchy: levels, tail_first (-1: no single-workgroup
+        kernel), tail_lds_from and the cells of each level."""
+        lv, tf, tl = C.c_int(), C.c_int(), C.c_int()
+        cells = (C.c_uint * 12)()
+        self.ctx.check(self.ctx._l.ffr_scene_mg_layout(self.h, C.byref(lv), C.byref(tf), C.byref(tl),
+                                                       cells), "ffr_scene_mg_layout")
+        return {"levels": lv.value, "tail_first": tf.value, "tail_lds_from": tl.value,
+                "cells": [cells[i] for i in range(lv.value)]}
 
     def prof_kernel(self, name: str, reps: int = 200):
         """(mean launch ms, algorithmic bytes) of `reps` back-to-back launches."""

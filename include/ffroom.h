@@ -141,6 +141,11 @@ typedef struct {
     int warm_start;      /* mixed MG-PCG: a step's pressure solve starts from the previous
                             step's pressure (1) or from 0 as the reference CG does (0);
                             ffr_poisson_solve always starts from 0 */
+    int mg_tail_cells;   /* levels >= 1 of at most this many cells also run inside the
+                            single-workgroup kernel, with their vectors and operator in
+                            global memory: one launch per V-cycle for all of them instead of
+                            several per level (fewer dispatches for small scenes served side
+                            by side); 0 = only the levels that fit its LDS budget */
 } ffr_params;
 
 /* StepResult, FluidSolver.hpp:40-47, plus solver statistics. */
@@ -204,6 +209,14 @@ size_t ffr_ctx_bytes_cached(const ffr_ctx *ctx);
 size_t ffr_ctx_trim(ffr_ctx *ctx);
 
 /* ---- scenes ------------------------------------------------------------ */
+/* The multigrid hierarchy of a scene that uses one (levels = 0 otherwise):
+ * the level count, the first level the single-workgroup kernel runs (-1:
+ * none), the first of those with its vectors in LDS (mg_tail_cells puts the
+ * ones before it in global memory), and the cells of each level (up to 12
+ * entries; NULL to skip). No counterpart in the reference (its multigrid,
+ * PoissonSolver.cpp:164-505, has a fixed level list): for tests and tuning. */
+int ffr_scene_mg_layout(const ffr_scene *scene, int *levels, int *tail_first, int *tail_lds_from,
+                        unsigned int *cells);
 /* Device bytes a scene with this geometry and these parameters occupies. */
 size_t ffr_scene_footprint(const ffr_scene_desc *desc, const ffr_params *params);
 /* Copies the geometry to the device and allocates the state (zero fields). */
