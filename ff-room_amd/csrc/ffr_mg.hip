@@ -427,7 +427,8 @@ static bool vcycle(ffr_scene *s, int l, bool first_red_done, bool dot = false)
     // 2 B) or operator (coarse: active + 3 couplings + rdiag), z written
     // (level 0, split layout: r, code and z of the colour, z of the other colour)
     const double bsm = l0 ? n * (4 + 4 + 2 + 4) / 2 : n * (2 * sizeof(TC) + 1 + 16 + sizeof(TC));
-    const double bres = l0 ? n * (4 + 4 + 2) + C.n * (double)sizeof(TC)
+    // (level 0: the residual of the red cells only, then P^T of it)
+    const double bres = l0 ? n / 2 * (4 + 2 + 4 + 4) + n / 2 * 4 + n / 2 * 4 + C.n * (double)sizeof(TC)
                            : n * (2 * sizeof(TC) + 1 + 16) + C.n * (double)sizeof(TC);
     const double bpost = bsm + C.n * (double)sizeof(TC);
     const int sweeps = l0 ? M.pre_sweeps : M.level_sweeps;
@@ -725,9 +726,10 @@ double mg_prof_launch(ffr_scene *s, const char *name, std::function<void()> *lau
         return (double)(g.nc - L0.H) * (4 + 4 + 2 + 4);
     }
     if (k == "mg_resid0" && M.levels > 1) {  // the first pass of the level-0 restriction
+        // red cells: r, code and z read, the residual written; black z read
         float *res = M.sc;
         *launch = [=] { mg_launch_resid0(st, g, L0, res, m); };
-        return n * (4 + 4 + 2 + 4);
+        return n / 2 * (4 + 2 + 4 + 4) + n / 2 * 4;
     }
     if (k == "pcg_dirq" && main_form(M)) {
         float *r = M.ri, *p = M.pi, *q = M.qi;

@@ -248,15 +248,23 @@ __global__ void __launch_bounds__(kBlock) k_lbm_stream(Geo g, const uint32_t *__
 // Fused form: streaming and moments as k_lbm_stream, then the next step's
 // inflow equilibrium (INFLOW cells) or BGK collision (FLUID, OUTFLOW) on the
 // streamed values, stored in f2; block 0 clears the step's diagnostics.
-// Eight waves per SIMD (64 VGPRs, the 19 streamed values partly spilled to
-// scratch) beat the unconstrained 111 VGPRs at four waves: 419 -> 366 us on
-// c2, measured; five and six waves fall in between.
+// The collision needs all 19 streamed values after the moments: kLbmLds of
+// them wait in the thread's own LDS slots (no barrier: each thread reads
+// back only what it wrote), the rest in registers, and the loads and the
+// stores go out kLbmGroup directions at a time (scheduling barriers), so
+// eight waves per SIMD fit 64 VGPRs with a few spilled values (36 B per lane,
+// against 76 before). 10 x 256 x 8 B = 20 KiB per block, 160 KiB for the
+// eight blocks of a CU. Measured on MI355X (mean of 20 launches): c2 370 ->
+// 323 us, c3 2793 -> 2461 us; with 11..14 values in LDS (no spills, but six
+// or seven waves by LDS or VGPRs) 408-417 us on c2.
+constexpr int kLbmLds = 10, kLbmGroup = 4;
 __global__ void __launch_bounds__(kBlock, 8) k_lbm_stream_collide(Geo g, const uint32_t *__restrict__ mask,
                                                                 const double *__restrict__ f,
                                                                 double *__restrict__ f2, double *cu,
                                                                 double *p, double l2s, double s2l,
                                                                 double inv_tau, StepRec *rec)
 {
+    __shared__ double sv[kLbmLds][kBlock];
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         rec->vmax_bits = 0ull;
         rec->divmax_bits = 0ull;
@@ -273,7 +281,7 @@ __global__ void __launch_bounds__(kBlock, 8) k_lbm_stream_collide(Geo g, const u
             cu[2ull * nc + c] = 0.0;
             continue;
         }
-        double v[19];
+        double v[19 - kLbmLds];
         double r = 0.0, x = 0.0, y = 0.0, z = 0.0;
 #pragma unroll
         for (int q = 0; q < 19; q++) {
@@ -284,7 +292,11 @@ __global__ void __launch_bounds__(kBlock, 8) k_lbm_stream_collide(Geo g, const u
             }
             const int b = lbm_opp(q);
             if (!(m & (1u << b))) a += f[(size_t)b * nc + c];
-            v[q] = a;
+            if (q < kLbmLds) sv[q][threadIdx.x] = a;
+            else v[q - kLbmLds] = a;
+            // at most a group of directions' loads in flight at once, so
+            // their data and addresses fit the registers
+            if (q % kLbmGroup == kLbmGroup - 1) __builtin_amdgcn_sched_barrier(0);
             r += a;
             x += a * lbm_ex(q);
             y += a * lbm_ey(q);
@@ -306,12 +318,17 @@ __global__ void __launch_bounds__(kBlock, 8) k_lbm_stream_collide(Geo g, const u
                 for (int q = 0; q < 19; q++) f2[(size_t)q * nc + c] = feq(q, 1.0, vx, vy, vz);
             } else {
 #pragma unroll
-                for (int q = 0; q < 19; q++) f2[(size_t)q * nc + c] = v[q];
+                for (int q = 0; q < 19; q++)
+                    f2[(size_t)q * nc + c] = q < kLbmLds ? sv[q][threadIdx.x] : v[q - kLbmLds];
             }
             continue;
         }
 #pragma unroll
-        for (int q = 0; q < 19; q++) f2[(size_t)q * nc + c] = v[q] + inv_tau * (feq(q, r, x, y, z) - v[q]);
+        for (int q = 0; q < 19; q++) {
+            const double vq = q < kLbmLds ? sv[q][threadIdx.x] : v[q - kLbmLds];
+            f2[(size_t)q * nc + c] = vq + inv_tau * (feq(q, r, x, y, z) - vq);
+            if (q % kLbmGroup == kLbmGroup - 1) __builtin_amdgcn_sched_barrier(0);
+        }
     }
 }
 

@@ -646,11 +646,14 @@ __global__ void __launch_bounds__(kBlock) k_mg_restrict(Geo g, MgLevel F, MgLeve
 }
 
 // ---- level-0 restriction in two passes --------------------------------------
-// (1) res = r - A z on every slot of the split layout (one thread per slot,
-//     the smoother's gather pattern); (2) rc = P^T res, one thread per coarse
-//     cell summing its up-to-8 children, whose split slots are contiguous
-//     across consecutive coarse cells. res lives in the inner PCG's q, which
-//     is not in use while the preconditioner runs.
+// (1) res = r - A z on the red slots of the split layout (one thread per slot,
+//     the smoother's gather pattern). Pre-smoothing ends with a black pass,
+//     which leaves r - A z = 0 on every black cell (up to that pass's
+//     rounding: z_black = (r + offdiag z_red) / diag), so only the red half is
+//     formed; (2) rc = P^T res, one thread per coarse cell summing its
+//     up-to-8 children (black children count 0), whose split slots are
+//     contiguous across consecutive coarse cells. res lives in the level-0
+//     scratch, which is not in use while the preconditioner runs.
 template <bool ST>
 struct KB_mg_resid0 {
     static __device__ __forceinline__ void run(const VB &vb, const Geo &g, const MgLevel &L,
@@ -661,38 +664,36 @@ struct KB_mg_resid0 {
     const uint32_t stop = stop_flags(m);
     const uint32_t H = L.H, si = (uint32_t)g.ny * g.nz, sj = g.nz;
     const float dint = diag0m(g, kInterior);
-    const auto bcode = buf_rsrc(L.nbs, 2u * g.nc), bz = buf_rsrc(z, 4u * g.nc),
-               br = buf_rsrc(r, 4u * g.nc), bred = buf_rsrc(z, 4u * H),
-               bblack = buf_rsrc(z + H, 4u * (g.nc - H)), bres = buf_rsrc(res, 4u * g.nc);
+    const auto bcode = buf_rsrc(L.nbs, 2u * H), bz = buf_rsrc(z, 4u * H),
+               br = buf_rsrc(r, 4u * H), bblack = buf_rsrc(z + H, 4u * (g.nc - H)),
+               bres = buf_rsrc(res, 4u * H);
     uint32_t ch0, cst, cend;
-    xcd_chunks(g.nc, vb.blk, vb.G, ch0, cst, cend);
+    xcd_chunks(H, vb.blk, vb.G, ch0, cst, cend);
     for (uint32_t ch = ch0; ch < cend; ch += kU2 * cst) {
         uint16_t nb[kU2];
         float rv[kU2], zs[kU2], zv[kU2][6];
         uint32_t cnat[kU2];  // natural cell index (stretched grids)
 #pragma unroll
         for (int t = 0; t < kU2; t++) {
-            const uint32_t s = (ch + t * cst) * kBlock + threadIdx.x;  // reads past nc return 0
-            const uint32_t col = s >= H ? 1u : 0u;
-            const uint32_t h = s - (col ? H : 0u);
-            const uint32_t cn = 2 * h + split_parity(L, h, col);
-            nb[t] = buf_ldu16(bcode, s);
-            rv[t] = buf_ldf(br, s);
-            zs[t] = buf_ldf(bz, s);
+            const uint32_t h = (ch + t * cst) * kBlock + threadIdx.x;  // reads past H return 0
+            const uint32_t cn = 2 * h + split_parity(L, h, 0u);
+            nb[t] = buf_ldu16(bcode, h);
+            rv[t] = buf_ldf(br, h);
+            zs[t] = buf_ldf(bz, h);
             cnat[t] = cn;
-            nb_gather(col ? bred : bblack, cn, si, sj, zv[t]);
+            nb_gather(bblack, cn, si, sj, zv[t]);
         }
         if (late(stop)) return;
 #pragma unroll
         for (int t = 0; t < kU2; t++) {
-            const uint32_t s = (ch + t * cst) * kBlock + threadIdx.x;
-            if (ch + t * cst >= cend) break;  // stores past nc are dropped
+            const uint32_t h = (ch + t * cst) * kBlock + threadIdx.x;
+            if (ch + t * cst >= cend) break;  // stores past H are dropped
             float v = 0.f;
             if (nb[t] & NB_SELF) {
                 const Cof<ST> cf = cof_at<ST>(g, cnat[t]);
                 v = rv[t] - (diag_t<ST>(g, nb[t], dint, cf) * zs[t] - nb_sum_t<ST>(g, nb[t], zv[t], cf));
             }
-            buf_stf(bres, s, v);
+            buf_stf(bres, h, v);
         }
     }
 }
@@ -714,7 +715,7 @@ struct KB_mg_restrict_sum0 {
 {
     const uint32_t stop = stop_flags(m);
     const uint32_t H = F.H;
-    const auto bred = buf_rsrc(res, 4u * H), bblack = buf_rsrc(res + H, 4u * (g.nc - H));
+    const auto bred = buf_rsrc(res, 4u * H);
     uint32_t sb, se;
     block_span_of(C.n, vb.blk, vb.G, sb, se);
     for (uint32_t I = sb + threadIdx.x; I < se; I += kBlock) {
@@ -728,9 +729,10 @@ struct KB_mg_restrict_sum0 {
             const bool in = qx <= C.shx && qy <= C.shy && qz <= C.shz && a < F.nx && b < F.ny &&
                             d < F.nz;
             const uint32_t c = ((uint32_t)a * F.ny + b) * F.nz + d;
-            // out-of-range children read past the buffer (0)
-            const uint32_t h = in ? (c >> 1) : 0xFFFFFFFFu >> 3;
-            part[q] = ((a + b + d) & 1) ? buf_ldf(bblack, h) : buf_ldf(bred, h);
+            // out-of-range children read past the buffer (0); black children
+            // have no residual after the pre-smoothing's last (black) pass
+            const uint32_t h = in && !((a + b + d) & 1) ? (c >> 1) : 0xFFFFFFFFu >> 3;
+            part[q] = buf_ldf(bred, h);
         }
         if (late(stop)) return;
         // the xor-tree order of k_mg_restrict's lane sum
@@ -1965,7 +1967,7 @@ void mg_launch_smooth(hipStream_t s, const Geo &g, const MgLevel &L, bool l0, in
 template <bool ST>
 void resid0_launch(hipStream_t s, const Geo &g, const MgLevel &L0, float *res, const MgState *m)
 {
-    FFR_GANG((KB_mg_resid0<ST>), GB_RES, (k_mg_resid0<ST>), dim3(grid_resident(g.nc)), dim3(kBlock),
+    FFR_GANG((KB_mg_resid0<ST>), GB_RES, (k_mg_resid0<ST>), dim3(grid_resident(L0.H)), dim3(kBlock),
              0, s, g, L0, (const float *)L0.z, (const float *)L0.r, res, m);
 }
 
