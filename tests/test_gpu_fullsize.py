@@ -2,7 +2,8 @@
 properties that do not depend on size, where the oracle would take minutes:
 every pressure solve meets its tolerance, the state stays finite, two runs
 give the same bits, and one mixed step from a given state stays within 1e-6
-(relative L2 of the velocities) of the FP64 step from the same state. The
+(relative L2 of p, u, v, w and T) of the FP64 step from the same state, on
+c2 and on c3. The
 lattice and the stretched grid are checked the same way at c2 size.
 """
 import numpy as np
@@ -46,27 +47,39 @@ def test_fullsize_mixed_steps_converge_and_repeat(ctx, name, steps):
     assert runs[0][0] == runs[1][0]
 
 
-def test_fullsize_mixed_step_matches_fp64_step(ctx):
-    """One step of each form from the same c2 state (the mixed form's after
-    three steps; FP32 values are exact in FP64)."""
-    cfg = workload_c2(precision="mixed")
+@pytest.mark.parametrize("name,tol", [("c2", 1e-8), ("c3", 1e-9)])
+def test_fullsize_mixed_step_matches_fp64_step(ctx, name, tol):
+    """One step of each form from the same state (the mixed form's after
+    three steps; FP32 values are exact in FP64), the FP64 form's CG run to the
+    loosest tolerance whose pressure error meets 1e-6 on that room: p, u, v, w
+    and T within 1e-6 relative L2 (p gauge-free: c3 holds a sealed pocket, on
+    which each form's constant offset is removed)."""
+    from types import SimpleNamespace
+
+    from bench import gauge_free_error, sealed_components
+    cfg = (workload_c2 if name == "c2" else workload_c3)(precision="mixed")
     ps = device.prepare(cfg, on_device=True)
     dm = device.create(ctx, ps, _params(ps, True))
     for _ in range(3):
         dm.step()
     st = dm.download()
-    d64 = device.create(ctx, ps, _params(ps, False))
+    p64 = _params(ps, False)
+    p64.poisson_tol = tol
+    d64 = device.create(ctx, ps, p64)
     d64.upload(**st)
     r64 = d64.step()
     ref = d64.download()
+    flags = d64.cell_type()
     d64.close()
     dm.step()
     got = dm.download()
     dm.close()
-    assert r64["pressure_residual"] < 1e-8
+    assert r64["pressure_residual"] < tol
     for k in ("u", "v", "w", "T"):
         err = np.linalg.norm(got[k] - ref[k]) / max(np.linalg.norm(ref[k]), 1e-300)
         assert err < 1e-6, (k, err)
+    view = SimpleNamespace(flags=flags, nx=ps.nx, ny=ps.ny, nz=ps.nz)
+    assert gauge_free_error(got["p"].copy(), ref["p"], view, sealed_components(view)) < 1e-6
 
 
 def test_fullsize_stretched_mixed_steps_converge(ctx):
