@@ -3,11 +3,13 @@
 // K1  predict:    u*, v*, w* (advection + diffusion, FluidSolver.cpp:83-309)
 //                 + buoyancy on w* (:373-386) + no-slip and fan masks
 //                 (BoundaryManager.cpp:60-130) from the scene's per-face mask
-//                 codes, one face per thread, each block on one face array.
+//                 codes; one thread per cell column marching x planes, the
+//                 tile's y/z neighbourhood staged in LDS.
 // K4  rhs:        b = -(rho/dt) div(u*) (:311-330).
 // K6  correct:    u = u* - (dt/rho) grad p (:332-371) + no-slip, fan and
 //                 outflow masks (BoundaryManager.cpp:60-130, 180-220)
-//                 + max |u^{n+1}-u^n| (:527-534), in place over u^n.
+//                 + max |u^{n+1}-u^n| (:527-534), in place over u^n; one
+//                 thread per cell's three low faces.
 // K8  temperature: T transport (:388-456) + max |div u| over FLUID (:470-484)
 //                 + sum of T over FLUID (:458-468).
 //     opening_T:  BoundaryManager.cpp:137-178.
