@@ -374,6 +374,9 @@ static const char *kRestrictName[kMaxLevels] = {
     "mg_restrict_l0", "mg_restrict_l1", "mg_restrict_l2", "mg_restrict_l3", "mg_restrict_l4",
     "mg_restrict_l5", "mg_restrict_l6", "mg_restrict_l7", "mg_restrict_l8", "mg_restrict_l9",
     "mg_restrict_l10", "mg_restrict_l11"};
+static const char *kResidName[kMaxLevels] = {
+    "mg_resid_l0", "mg_resid_l1", "mg_resid_l2", "mg_resid_l3", "mg_resid_l4", "mg_resid_l5",
+    "mg_resid_l6", "mg_resid_l7", "mg_resid_l8", "mg_resid_l9", "mg_resid_l10", "mg_resid_l11"};
 static const char *kPostName[kMaxLevels] = {
     "mg_post_l0", "mg_post_l1", "mg_post_l2", "mg_post_l3", "mg_post_l4", "mg_post_l5",
     "mg_post_l6", "mg_post_l7", "mg_post_l8", "mg_post_l9", "mg_post_l10", "mg_post_l11"};
@@ -427,9 +430,13 @@ static bool vcycle(ffr_scene *s, int l, bool first_red_done, bool dot = false)
     // 2 B) or operator (coarse: active + 3 couplings + rdiag), z written
     // (level 0, split layout: r, code and z of the colour, z of the other colour)
     const double bsm = l0 ? n * (4 + 4 + 2 + 4) / 2 : n * (2 * sizeof(TC) + 1 + 16 + sizeof(TC));
-    // (level 0: the residual of the red cells only, then P^T of it)
-    const double bres = l0 ? n / 2 * (4 + 2 + 4 + 4) + n / 2 * 4 + n / 2 * 4 + C.n * (double)sizeof(TC)
-                           : n * (2 * sizeof(TC) + 1 + 16) + C.n * (double)sizeof(TC);
+    // fused restriction (small coarse levels): the fine level's r, z and
+    // operator read, the coarse r written; two passes: the residual (level 0:
+    // red cells' r, code and z, the black z, the residual written; coarse:
+    // r, z, operator, residual written), then P^T of it
+    const double bres = n * (2 * sizeof(TC) + 1 + 16) + C.n * (double)sizeof(TC);
+    const double bresid = l0 ? n / 2 * (4 + 2 + 4 + 4) + n / 2 * 4 : n * (2 * sizeof(TC) + 1 + 16 + 4);
+    const double bsum = (l0 ? n / 2 * 4 : n * 4) + C.n * (double)sizeof(TC);
     const double bpost = bsm + C.n * (double)sizeof(TC);
     const int sweeps = l0 ? M.pre_sweeps : M.level_sweeps;
     const RedScratch red = s->red;
@@ -445,10 +452,20 @@ static bool vcycle(ffr_scene *s, int l, bool first_red_done, bool dot = false)
     // the next level's first sweep rides on the restriction unless that level
     // is the single-workgroup tail or the coarsest (both start from z = 0)
     const bool red0 = l + 1 != M.tail_first && l + 1 != M.levels - 1;
-    {
+    auto restrict_pass = [&](int pass) {
+        if (l0) mg_launch_restrict<float, TC>(st, g, L, C, true, red0, m, M.sc, pass);
+        else mg_launch_restrict<TC, TC>(st, g, L, C, false, red0, m, M.sc, pass);
+    };
+    if (restrict_two_pass(l0, L.n)) {  // each pass in its own profiler scope
+        {
+            ProfScope ps(s->ctx, st, kResidName[l], bresid);
+            restrict_pass(1);
+        }
+        ProfScope ps(s->ctx, st, kRestrictName[l], bsum);
+        restrict_pass(2);
+    } else {
         ProfScope ps(s->ctx, st, kRestrictName[l], bres);
-        if (l0) mg_launch_restrict<float, TC>(st, g, L, C, true, red0, m, M.sc);
-        else mg_launch_restrict<TC, TC>(st, g, L, C, false, red0, m);
+        restrict_pass(0);
     }
     vcycle<TC>(s, l + 1, red0);
     {

@@ -49,6 +49,11 @@
 namespace ffr {
 
 constexpr int kMaxLevels = 12;
+// coarse levels of at least this many cells restrict in two passes (the
+// residual one thread per fine cell, then the child sums); smaller ones in
+// one fused launch; level 0 always in two passes
+constexpr uint32_t kCoarseTwoPassMin = 1u << 20;
+inline bool restrict_two_pass(bool l0, uint32_t n) { return l0 || n >= kCoarseTwoPassMin; }
 
 struct MgLevel {
     int nx, ny, nz;

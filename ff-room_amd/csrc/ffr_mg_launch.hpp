@@ -10,10 +10,12 @@ void mg_launch_split_codes(hipStream_t s, const Geo &g, const MgLevel &L0);
 template <typename T>
 void mg_launch_smooth(hipStream_t s, const Geo &g, const MgLevel &L, bool l0, int color,
                       int zero_start, const MgState *m, const RedScratch *dot = nullptr);
-// level 0 with `scratch` (nc floats): residual pass + child-sum pass
+// level 0 with `scratch` (nc floats), and coarse levels of at least
+// kCoarseTwoPassMin cells with it (restrict_two_pass): residual pass +
+// child-sum pass; `pass` 1 or 2 launches only that pass of those (0: all)
 template <typename TF, typename TC>
 void mg_launch_restrict(hipStream_t s, const Geo &g, const MgLevel &F, const MgLevel &C, bool l0,
-                        bool red0, const MgState *m, float *scratch = nullptr);
+                        bool red0, const MgState *m, float *scratch = nullptr, int pass = 0);
 // the level-0 residual pass of the restriction alone (profiling)
 void mg_launch_resid0(hipStream_t s, const Geo &g, const MgLevel &L0, float *res, const MgState *m);
 // prolongation fused with the first post-smoothing (black) sweep on F

@@ -645,6 +645,92 @@ __global__ void __launch_bounds__(kBlock) k_mg_restrict(Geo g, MgLevel F, MgLeve
     KB_mg_restrict<TF, TC, L0, ST>::run(vb_here(), g, F, C, z, r, rc, zc, red0, m);
 }
 
+// ---- coarse-level restriction in two passes (large levels) ------------------
+// The fused kernel above gives each (coarse cell, child) pair a lane, so a
+// wave reads its fine level in short runs from two planes and two rows; on a
+// large level the residual is instead formed one thread per fine cell in the
+// natural order (coalesced) into an FP32 scratch, then summed per coarse cell
+// in the fused kernel's lane order (the same residual formula and sum order).
+template <typename T>
+struct KB_mg_cresid {
+    static __device__ __forceinline__ void run(const VB &vb, const MgLevel &F, const T *__restrict__ z,
+                                               const T *__restrict__ r, float *__restrict__ res,
+                                               const MgState *m)
+{
+    const uint32_t stop = stop_flags(m);
+    const T *tag = nullptr;
+    const uint32_t si = (uint32_t)F.ny * F.nz, sj = F.nz;
+    const CoarseOp<T> o = coarse_op<T>(F);
+    const auto bz = buf_rsrc(z, (uint32_t)sizeof(T) * F.n), br = buf_rsrc(r, (uint32_t)sizeof(T) * F.n),
+               bres = buf_rsrc(res, 4u * F.n);
+    uint32_t ch0, cst, cend;
+    xcd_chunks(F.n, vb.blk, vb.G, ch0, cst, cend);
+    for (uint32_t ch = ch0; ch < cend; ch += kU2 * cst) {
+        float v[kU2];
+        uint8_t a[kU2];
+#pragma unroll
+        for (int t = 0; t < kU2; t++) {  // past n: zero reads, dropped stores
+            const uint32_t c = (ch + t * cst) * kBlock + threadIdx.x;
+            a[t] = buf_ldu8(o.act, c);
+            v[t] = bldv(br, c, tag) - (buf_ldf(o.dg, c) * bldv(bz, c, tag) - coarse_row<T>(o, bz, c, si, sj));
+        }
+        if (late(stop)) return;
+#pragma unroll
+        for (int t = 0; t < kU2; t++) {
+            if (ch + t * cst >= cend) break;
+            buf_stf(bres, (ch + t * cst) * kBlock + threadIdx.x, a[t] ? v[t] : 0.f);
+        }
+    }
+}
+};
+template <typename T>
+__global__ void __launch_bounds__(kBlock) k_mg_cresid(MgLevel F, const T *__restrict__ z,
+                                                      const T *__restrict__ r, float *__restrict__ res,
+                                                      const MgState *m)
+{
+    KB_mg_cresid<T>::run(vb_here(), F, z, r, res, m);
+}
+
+template <typename TC>
+struct KB_mg_csum {
+    static __device__ __forceinline__ void run(const VB &vb, const MgLevel &F, const MgLevel &C,
+                                               const float *__restrict__ res, TC *__restrict__ rc,
+                                               TC *__restrict__ zc, int red0, const MgState *m)
+{
+    const uint32_t stop = stop_flags(m);
+    const auto bres = buf_rsrc(res, 4u * F.n);
+    uint32_t sb, se;
+    block_span_of(C.n, vb.blk, vb.G, sb, se);
+    for (uint32_t I = sb + threadIdx.x; I < se; I += kBlock) {
+        int ic, jc, kc;
+        decode(I, C.f_nynz, C.f_nz, C.ny, C.nz, ic, jc, kc);
+        float part[8];
+#pragma unroll
+        for (int q = 0; q < 8; q++) {  // child (qx, qy, qz): the fused kernel's lane q
+            const int qx = q >> 2, qy = (q >> 1) & 1, qz = q & 1;
+            const int a = (ic << C.shx) + qx, b = (jc << C.shy) + qy, d = (kc << C.shz) + qz;
+            const bool in = qx <= C.shx && qy <= C.shy && qz <= C.shz && a < F.nx && b < F.ny &&
+                            d < F.nz;
+            part[q] = buf_ldf(bres, in ? ((uint32_t)a * F.ny + b) * F.nz + d : 0xFFFFFFFFu >> 2);
+        }
+        if (late(stop)) return;
+        // the xor-tree order of the fused kernel's lane sum (lanes q ^ 1, q ^ 2, q ^ 4)
+        const float s01 = part[0] + part[1], s23 = part[2] + part[3];
+        const float s45 = part[4] + part[5], s67 = part[6] + part[7];
+        const float v = (s01 + s23) + (s45 + s67);
+        stv(rc, I, v);
+        if (red0) coarse_first_red<TC>(C, zc, I, ic, jc, kc, v);
+    }
+}
+};
+template <typename TC>
+__global__ void __launch_bounds__(kBlock) k_mg_csum(MgLevel F, MgLevel C, const float *__restrict__ res,
+                                                    TC *__restrict__ rc, TC *__restrict__ zc, int red0,
+                                                    const MgState *m)
+{
+    KB_mg_csum<TC>::run(vb_here(), F, C, res, rc, zc, red0, m);
+}
+
 // ---- level-0 restriction in two passes --------------------------------------
 // (1) res = r - A z on the red slots of the split layout (one thread per slot,
 //     the smoother's gather pattern). Pre-smoothing ends with a black pass,
@@ -1982,18 +2068,26 @@ void restrict0_launch(hipStream_t s, const Geo &g, const MgLevel &F, const MgLev
 
 template <typename TF, typename TC>
 void mg_launch_restrict(hipStream_t s, const Geo &g, const MgLevel &F, const MgLevel &C, bool l0,
-                        bool red0, const MgState *m, float *scratch)
+                        bool red0, const MgState *m, float *scratch, int pass)
 {
     if (l0 && scratch) {  // two passes through the scratch residual
-        mg_launch_resid0(s, g, F, scratch, m);
-        FFR_GANG((KB_mg_restrict_sum0<TC>), GB_PLAIN, (k_mg_restrict_sum0<TC>), dim3(grid_for(C.n)),
-                 dim3(kBlock), 0, s, g, F, C, (const float *)scratch, (TC *)C.r, (TC *)C.z,
-                 (int)red0, m);
+        if (pass != 2) mg_launch_resid0(s, g, F, scratch, m);
+        if (pass != 1)
+            FFR_GANG((KB_mg_restrict_sum0<TC>), GB_PLAIN, (k_mg_restrict_sum0<TC>), dim3(grid_for(C.n)),
+                     dim3(kBlock), 0, s, g, F, C, (const float *)scratch, (TC *)C.r, (TC *)C.z,
+                     (int)red0, m);
     } else if (l0 && g.uniform)
         restrict0_launch<TF, TC, false>(s, g, F, C, red0, m);
     else if (l0)
         restrict0_launch<TF, TC, true>(s, g, F, C, red0, m);
-    else
+    else if (scratch && restrict_two_pass(false, F.n)) {  // residual, then the child sums
+        if (pass != 2)
+            FFR_GANG((KB_mg_cresid<TF>), GB_PLAIN, (k_mg_cresid<TF>), dim3(grid_for(F.n)),
+                     dim3(kBlock), 0, s, F, (const TF *)F.z, (const TF *)F.r, scratch, m);
+        if (pass != 1)
+            FFR_GANG((KB_mg_csum<TC>), GB_PLAIN, (k_mg_csum<TC>), dim3(grid_for(C.n)), dim3(kBlock),
+                     0, s, F, C, (const float *)scratch, (TC *)C.r, (TC *)C.z, (int)red0, m);
+    } else
         FFR_GANG((KB_mg_restrict<TF, TC, false, false>), GB_PLAIN, (k_mg_restrict<TF, TC, false, false>),
                  dim3(grid_for(8ull * C.n)), dim3(kBlock), 0, s, g, F, C, (const TF *)F.z,
                  (const TF *)F.r, (TC *)C.r, (TC *)C.z, (int)red0, m);
@@ -2067,9 +2161,9 @@ void mg_launch_coarse(hipStream_t s, const Geo &g, const MgLevel &L, bool l0, in
 
 template void mg_launch_smooth<float>(hipStream_t, const Geo &, const MgLevel &, bool, int, int, const MgState *, const RedScratch *);
 template void mg_launch_smooth<__hip_bfloat16>(hipStream_t, const Geo &, const MgLevel &, bool, int, int, const MgState *, const RedScratch *);
-template void mg_launch_restrict<float, float>(hipStream_t, const Geo &, const MgLevel &, const MgLevel &, bool, bool, const MgState *, float *);
-template void mg_launch_restrict<float, __hip_bfloat16>(hipStream_t, const Geo &, const MgLevel &, const MgLevel &, bool, bool, const MgState *, float *);
-template void mg_launch_restrict<__hip_bfloat16, __hip_bfloat16>(hipStream_t, const Geo &, const MgLevel &, const MgLevel &, bool, bool, const MgState *, float *);
+template void mg_launch_restrict<float, float>(hipStream_t, const Geo &, const MgLevel &, const MgLevel &, bool, bool, const MgState *, float *, int);
+template void mg_launch_restrict<float, __hip_bfloat16>(hipStream_t, const Geo &, const MgLevel &, const MgLevel &, bool, bool, const MgState *, float *, int);
+template void mg_launch_restrict<__hip_bfloat16, __hip_bfloat16>(hipStream_t, const Geo &, const MgLevel &, const MgLevel &, bool, bool, const MgState *, float *, int);
 template void mg_launch_post_black<float, float>(hipStream_t, const Geo &, const MgLevel &, const MgLevel &, bool, const MgState *);
 template void mg_launch_post_black<float, __hip_bfloat16>(hipStream_t, const Geo &, const MgLevel &, const MgLevel &, bool, const MgState *);
 template void mg_launch_post_black<__hip_bfloat16, __hip_bfloat16>(hipStream_t, const Geo &, const MgLevel &, const MgLevel &, bool, const MgState *);
