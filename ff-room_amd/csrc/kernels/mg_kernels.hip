@@ -1832,7 +1832,9 @@ __device__ __forceinline__ void row64v(const Geo &g, const uint32_t ijk[3], uint
 // colour of a cell, which places its e values in the split layout, comes
 // from its coordinates rather than from its code, and the neighbour gathers
 // go through buffer resources (out of range reads give 0; values of faces
-// that do not couple are dropped by the code, in the row's face order).
+// that do not couple are dropped by the code, in the row's face order). An
+// interior row (six unknown neighbours, uniform grid) sums its neighbours per
+// axis before scaling, fewer FP64 operations than the general row.
 template <bool ST>
 struct KB_dc_residual {
     static __device__ __forceinline__ void run(const VB &vb, const Geo &g,
@@ -1850,6 +1852,8 @@ struct KB_dc_residual {
     double *__restrict__ xout = (m->passes & 1) ? xa : xb;
     const uint32_t blk = vb.blk;
     const uint32_t si = (uint32_t)g.ny * g.nz, sj = g.nz, H = (g.nc + 1) / 2;
+    // the diagonal of an interior row, summed in row64v's face order
+    const double dint64 = ((((g.sxd + g.sxd) + g.syd) + g.syd) + g.szd) + g.szd;
     const auto bx = buf_rsrc(xin, 8u * g.nc), be = buf_rsrc(e, 4u * g.nc);
     const auto bred = buf_rsrc(e, 4u * H), bblack = buf_rsrc(e + H, 4u * (g.nc - H));
     double acc = 0.0;
@@ -1901,7 +1905,12 @@ struct KB_dc_residual {
 #pragma unroll
             for (int f = 0; f < 6; f++) vals[f] = xn[t][f] + (double)en[t][f];
             double diag, off;
-            row64v<ST>(g, ijk[t], nb[t], vals, diag, off);
+            if (!ST && (nb[t] & kFaceBits) == kInterior) {  // six unknown neighbours
+                diag = dint64;
+                off = g.sxd * (vals[0] + vals[1]) + g.syd * (vals[2] + vals[3]) + g.szd * (vals[4] + vals[5]);
+            } else {
+                row64v<ST>(g, ijk[t], nb[t], vals, diag, off);
+            }
             const double xc = xv[t] + (double)ev[t];
             const double rv = bv[t] - (diag * xc - off);
             xout[c] = xc;
