@@ -5,7 +5,7 @@ served side by side: one launch per V-cycle for all coarse levels.
 Pins: the dense solution of the same system (tiny semi-coarsened grids, the
 hierarchy's tail forced to start at level 1 with its LDS levels after it), and
 on a c4-sized room the solve of the per-level launches (both to a residual of
-1e-11: relative L2 <= 1e-9). The hierarchy introspection (ffr_scene_mg_layout)
+1e-11: relative L2 <= 1e-8, the condition number times the residual). The hierarchy introspection (ffr_scene_mg_layout)
 shows that the global levels are in use.
 """
 import numpy as np
@@ -52,7 +52,7 @@ def test_global_tail_matches_per_level_launches_on_c4_rooms(ctx, room):
     """A consistent right-hand side b = A x* (FP64 operator; c4 rooms can hold
     sealed pockets, where A is singular and x is defined up to a constant):
     both hierarchies reach the 1e-11 residual and their solutions agree
-    gauge-free to 1e-9."""
+    gauge-free to 1e-8 (the condition number, ~1e3, times the residual)."""
     from types import SimpleNamespace
 
     from bench import gauge_free_error, sealed_components
@@ -79,4 +79,5 @@ def test_global_tail_matches_per_level_launches_on_c4_rooms(ctx, room):
         sols[cells] = x
         view = SimpleNamespace(flags=flags, nx=ps.nx, ny=ps.ny, nz=ps.nz)
     sealed = sealed_components(view)
-    assert gauge_free_error(sols[1 << 17].copy(), sols[0], view, sealed) < 1e-9
+    assert gauge_free_error(sols[1 << 17].copy(), sols[0], view, sealed) < 1e-8
+
