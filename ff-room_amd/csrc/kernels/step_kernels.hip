@@ -698,25 +698,51 @@ void launch_predict(hipStream_t s, const Geo &g, const StepParams &sp, const R *
 // ---- K4: Poisson right-hand side ----------------------------------------
 template <typename R>
 struct KB_rhs {
+    // two cells per thread from the block's chunks (xcd_chunks: the u faces
+    // one plane up are then read again from L2), every load of both issued
+    // before either is used; a SOLID cell's loads are made and dropped
+    static constexpr int U = 2;
     static __device__ __forceinline__ void run(const VB &vb, const Geo &g, const StepParams &sp,
-                                               const R *us, const R *vs, const R *ws, double *b)
+                                               const R *__restrict__ us, const R *__restrict__ vs,
+                                               const R *__restrict__ ws, double *__restrict__ b)
 {
-    uint32_t sb, se;
-    block_span_of(g.nc, vb.blk, vb.G, sb, se);
-    for (uint32_t c = sb + threadIdx.x; c < se; c += kBlock) {
-        if (g.flags[c] == SOLID) {
-            b[c] = 0.0;
-            continue;
+    const uint32_t si = (uint32_t)g.ny * g.nz, nz = g.nz;
+    uint32_t ch0, cst, cend;
+    xcd_chunks(g.nc, vb.blk, vb.G, ch0, cst, cend);
+    for (uint32_t ch = ch0; ch < cend; ch += U * cst) {
+        uint32_t cc[U], ii[U], jj[U], kk[U];
+        bool in[U];
+        uint8_t fl[U];
+        R u0[U], u1[U], v0[U], v1[U], w0[U], w1[U];
+#pragma unroll
+        for (int t = 0; t < U; t++) {
+            const uint32_t ct = (ch + t * cst) * kBlock + threadIdx.x;
+            in[t] = ch + t * cst < cend && ct < g.nc;
+            const uint32_t c = in[t] ? ct : 0u;
+            const uint32_t i = fdiv(c, g.f_nynz), rem = c - i * si;
+            const uint32_t j = fdiv(rem, g.f_nz), k = rem - j * nz;
+            const uint32_t ev = c + i * nz, ew = c + i * (uint32_t)g.ny + j;
+            cc[t] = c; ii[t] = i; jj[t] = j; kk[t] = k;
+            fl[t] = g.flags[c];
+            u0[t] = us[c]; u1[t] = us[c + si];
+            v0[t] = vs[ev]; v1[t] = vs[ev + nz];
+            w0[t] = ws[ew]; w1[t] = ws[ew + 1];
         }
-        const uint32_t i = fdiv(c, g.f_nynz), rem = c - i * (g.ny * g.nz);
-        const uint32_t j = fdiv(rem, g.f_nz), k = rem - j * g.nz;
-        const double du = (double)us[c + g.ny * g.nz] - (double)us[c];
-        const uint32_t ev = vidx(g, i, j, k), ew = widx(g, i, j, k);
-        const double dv = (double)vs[ev + g.nz] - (double)vs[ev];
-        const double dw = (double)ws[ew + 1] - (double)ws[ew];
-        const double div = by_h64<R>(du, g.hx[i], g.ihx[i]) + by_h64<R>(dv, g.hy[j], g.ihy[j]) +
-                           by_h64<R>(dw, g.hz[k], g.ihz[k]);
-        b[c] = sp.rhs_factor * div;
+#pragma unroll
+        for (int t = 0; t < U; t++) {
+            if (!in[t]) continue;
+            if (fl[t] == SOLID) {
+                b[cc[t]] = 0.0;
+                continue;
+            }
+            const double du = (double)u1[t] - (double)u0[t];
+            const double dv = (double)v1[t] - (double)v0[t];
+            const double dw = (double)w1[t] - (double)w0[t];
+            const double div = by_h64<R>(du, g.hx[ii[t]], g.ihx[ii[t]]) +
+                               by_h64<R>(dv, g.hy[jj[t]], g.ihy[jj[t]]) +
+                               by_h64<R>(dw, g.hz[kk[t]], g.ihz[kk[t]]);
+            b[cc[t]] = sp.rhs_factor * div;
+        }
     }
 }
 };
