@@ -86,6 +86,7 @@ void ffr_params_default(ffr_params *p)
     p->mg_tail_kb = 0;
     p->warm_start = 1;
     p->mg_tail_cells = 0;
+    p->mg_tail_groups = 0;
 }
 
 int ffr_ctx_create(int device, size_t hbm_budget, ffr_ctx **out)

@@ -325,6 +325,16 @@ int mg_create(ffr_scene *s)
         if (s->prm.mg_tail_cells > 0 && M.tail_first > 1)
             while (M.tail_first > 1 && M.lv[M.tail_first - 1].n <= (uint32_t)s->prm.mg_tail_cells)
                 M.tail_first--;
+        // workgroups sharing the global levels (mg_tail_groups; 0: one per
+        // kTailGroupCells cells of the largest), on an XCD of their own per scene
+        M.tail_groups = 1;
+        if (M.tail_first >= 1 && M.tail_first < M.tail_lds_from) {
+            int grp = s->prm.mg_tail_groups;
+            if (grp <= 0)
+                grp = (int)((M.lv[M.tail_first].n + kTailGroupCells - 1) / kTailGroupCells);
+            M.tail_groups = std::max(1, std::min(grp, kTailGroupsMax));
+        }
+        M.tail_xcd = (int)(s->ctx->tail_seq++ % (unsigned)kXcds);
     }
     M.pre_sweeps = std::max(1, s->prm.mg_pre_sweeps);
     M.level_sweeps = std::max(1, s->prm.mg_level_sweeps);
@@ -415,6 +425,11 @@ static bool vcycle(ffr_scene *s, int l, bool first_red_done, bool dot = false)
         tl.sweeps = M.level_sweeps;
         tl.coarse_sweeps = M.coarse_sweeps;
         tl.ginv = M.ginv;
+        MgState *ms = s->mgs;
+        tl.groups = M.tail_groups;
+        tl.xcd = M.tail_xcd;
+        tl.gsync = ms->gsync;
+        tl.fault = &ms->tail_fault;
         ProfScope ps(s->ctx, st, "mg_tail", 0);
         mg_launch_tail<TC>(st, tl, M.tail_lds, m);
         return false;
@@ -692,6 +707,17 @@ int mg_continue(ffr_scene *s, bool *finished)
     StepTask &t = s->task;
     const MgState &h = *s->h_mgs;
     *finished = false;
+    if (h.tail_fault) {
+        // a tail group's barrier timed out: the solve is void; the scene's
+        // tails run as one workgroup from now on (its graphs hold the group)
+        drop_graphs(s);
+        s->mg.tail_groups = 1;
+        HIP_CHECK_CTX(s->ctx, hipMemsetAsync(s->mgs->gsync, 0, 2 * sizeof(unsigned) + sizeof(int),
+                                             s->stream), FFR_E_DEVICE);
+        s->ctx->fail("a multigrid tail group barrier timed out; the scene's tails now run as one "
+                     "workgroup");
+        return FFR_E_DEVICE;
+    }
     if (t.inner_wait >= 0) {  // an inner solve grown under host control
         if (!h.solve_done && !h.inner_done && h.inner_it < max_inner_of(s)) {
             launch_iterations(s, 2);
@@ -787,4 +813,10 @@ extern "C" int ffr_scene_mg_layout(const ffr_scene *s, int *levels, int *tail_fi
     if (cells)
         for (int l = 0; l < M.levels; l++) cells[l] = M.lv[l].n;
     return FFR_OK;
+}
+
+extern "C" int ffr_scene_mg_tail_groups(const ffr_scene *s)
+{
+    if (!s) return FFR_E_INVALID;
+    return s->mg.levels ? s->mg.tail_groups : 0;
 }

@@ -99,6 +99,9 @@ struct Multigrid {
     // tail_staged .. levels-1 are staged in LDS too
     int tail_first = -1, tail_staged = -1, tail_lds_from = -1;
     size_t tail_lds = 0;
+    // workgroups that share the tail's global levels (1: one workgroup), all
+    // on XCD tail_xcd (see TailLevels)
+    int tail_groups = 1, tail_xcd = 0;
     // dense (generalised) inverse of the coarsest level's operator, FP32,
     // row-major n x n, n <= kCoarsest: the tail's coarse solve is one
     // matrix-vector product (null when levels == 1)
@@ -128,6 +131,14 @@ constexpr size_t kTailLdsDefault = 96 * 1024, kTailLdsMax = 150 * 1024;
 // an unstaged first tail level is taken only up to this many cells: beyond
 // it, one workgroup's passes cost more than the kernel launches they replace
 constexpr uint32_t kTailUnstagedMax = 16384;
+// At most this many workgroups share a tail: each takes a whole CU (1024
+// threads and the tail's LDS), and four tails running at once (one per
+// hardware queue) on one XCD then still fit its 32 CUs, so a group never
+// waits for a workgroup that cannot become resident. Automatic choice: one
+// workgroup per kTailGroupCells cells of the first (largest) global level.
+constexpr int kTailGroupsMax = 8;
+constexpr uint32_t kTailGroupCells = 16384;
+constexpr unsigned kTailSpinMax = 1u << 21;
 struct TailCarve {
     uint32_t cx, cy, cz, diag, rdiag, act, z, r, end;  // byte offsets in the level's region
 };
@@ -166,6 +177,15 @@ struct TailLevels {
     int staged_from;      // first level whose operator is in LDS (first or first + 1)
     int sweeps, coarse_sweeps;
     const float *ginv;    // dense inverse of the coarsest operator (n_last^2), or null
+    // groups > 1: the global levels' phases are shared by that many
+    // workgroups, blocks b of the launch with b % kXcds == xcd (one XCD, so
+    // one L2), ordered by a barrier on the counters gsync[0] (arrivals) and
+    // gsync[1] (departures, the last one resets both); the LDS levels run on
+    // the first of them. A barrier that waits longer than kTailSpinMax polls
+    // sets *fault and stops waiting, so every workgroup still ends.
+    int groups, xcd;
+    unsigned *gsync;
+    int *fault;
 };
 static_assert(sizeof(TailLevels) <= 4096, "kernel arguments are limited to 4 KiB");
 
@@ -184,6 +204,10 @@ struct MgState {
     int total_iters, max_outer, max_inner, passes;
     int total_cap, pad;
     int hist[kMgHist];
+    // the multigrid tail's group barrier (TailLevels::gsync) and its fault
+    // flag (a barrier that timed out); zero between launches
+    unsigned gsync[2];
+    int tail_fault, pad2;
 };
 
 }  // namespace ffr

@@ -535,7 +535,8 @@ int ffr_scene_set_params(ffr_scene *s, const ffr_params *p)
                             o.mg_pre_sweeps != p->mg_pre_sweeps ||
                             o.mg_level_sweeps != p->mg_level_sweeps ||
                             o.mg_coarse_sweeps != p->mg_coarse_sweeps ||
-                            o.mg_tail_kb != p->mg_tail_kb || o.mg_tail_cells != p->mg_tail_cells;
+                            o.mg_tail_kb != p->mg_tail_kb || o.mg_tail_cells != p->mg_tail_cells ||
+                            o.mg_tail_groups != p->mg_tail_groups;
     s->prm = *p;
     s->pred = SolvePred{};  // iteration counts of other settings do not carry over
     if (s->mg.levels && mg_changed) mg_destroy(s);

@@ -6,6 +6,7 @@
 #include <map>
 #include <set>
 #include <memory>
+#include <atomic>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -54,6 +55,9 @@ struct ffr_ctx {
 
     // scenes created on this context (destroyed with it)
     std::set<ffr_scene *> scenes;
+    // hierarchies built so far: spreads the scenes' multigrid tail groups over
+    // the XCDs (hierarchies are built by serving worker threads too)
+    std::atomic<unsigned> tail_seq{0};
 
     // Device blocks, pinned host blocks, streams and events of destroyed
     // scenes, kept for the scenes created next (ffr_pool.hip). hipFree and

@@ -1178,55 +1178,103 @@ __device__ __forceinline__ void tail_pass(const V &L, uint32_t color, int zero_s
     else tail_pass_u<4, ST>(L, color, zero_start, tid, nthr);
 }
 
-template <typename V>
-__device__ __forceinline__ void tail_smooth(const V &L, uint32_t color, int zero_start)
+// Who runs a tail phase, and the barrier that ends it: the threads of one
+// workgroup (LDS levels, or a tail of one workgroup) or of the tail's
+// workgroups on one XCD (global levels, TailLevels::groups > 1). The group
+// barrier needs no device-scope fence: the workgroups share the XCD's L2, a
+// wave's stores are in it once its vmcnt has drained, the arrival counter is
+// an L2 atomic, and the CU's L1 is invalidated after the wait so that no line
+// another workgroup has since written is read stale (an agent-scope release
+// would also write the L2 back to HBM: measured 2.4-9.2 us per barrier of 1-8
+// workgroups on MI355X, against 0.6-1.0 us this way).
+struct TailSync {
+    uint32_t tid, nthr;  // this thread's index among the phase's threads, their count
+    unsigned *ctr;       // group arrival counter (null: one workgroup)
+    int *fault;
+    unsigned G, k;       // workgroups; barriers passed (thread 0)
+    bool broken;         // a wait timed out (thread 0): later barriers only arrive
+    __device__ __forceinline__ void sync()
+    {
+        if (!ctr) {
+            __syncthreads();
+            return;
+        }
+        __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0): this wave's stores are in the L2
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            k++;
+            __hip_atomic_fetch_add(ctr, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if (!broken) {
+                const unsigned target = G * k;
+                unsigned polls = 0;
+                while (__hip_atomic_load(ctr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
+                    __builtin_amdgcn_s_sleep(2);
+                    if (++polls == kTailSpinMax) {
+                        broken = true;
+                        __hip_atomic_store(fault, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                        break;
+                    }
+                }
+            }
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");  // the CU's L1 invalidated
+        }
+        __syncthreads();
+    }
+};
+__device__ __forceinline__ TailSync tail_solo()
 {
-    if (L.staged) tail_pass<true>(L, color, zero_start, threadIdx.x, kTailThreads);
-    else tail_pass<false>(L, color, zero_start, threadIdx.x, kTailThreads);
-    __syncthreads();
+    return TailSync{threadIdx.x, (uint32_t)kTailThreads, nullptr, nullptr, 1u, 0u, false};
+}
+
+template <typename V>
+__device__ __forceinline__ void tail_smooth(const V &L, uint32_t color, int zero_start, TailSync &ts)
+{
+    if (L.staged) tail_pass<true>(L, color, zero_start, ts.tid, ts.nthr);
+    else tail_pass<false>(L, color, zero_start, ts.tid, ts.nthr);
+    ts.sync();
 }
 
 // residuals r - A z of level F into res (FP32), all cells
 template <int U, bool ST, typename V>
-__device__ __forceinline__ void tail_residual_u(const V &F, float *res)
+__device__ __forceinline__ void tail_residual_u(const V &F, float *res, uint32_t tid, uint32_t nthr)
 {
     const TailOp o = tail_op<ST>(F);
-    for (uint32_t c0 = threadIdx.x; c0 < F.n; c0 += U * kTailThreads) {
+    for (uint32_t c0 = tid; c0 < F.n; c0 += U * nthr) {
         float v[U];
         uint8_t a[U];
 #pragma unroll
         for (int t = 0; t < U; t++) {
-            const uint32_t c = c0 + (uint32_t)t * kTailThreads;
+            const uint32_t c = c0 + (uint32_t)t * nthr;
             const uint32_t cc = c < F.n ? c : 0u;
             a[t] = o.act[cc];
             v[t] = ldv(F.r, cc) - (o.diag[cc] * ldv(F.z, cc) - tail_row(F, o, F.z, cc));
         }
 #pragma unroll
         for (int t = 0; t < U; t++) {
-            const uint32_t c = c0 + (uint32_t)t * kTailThreads;
+            const uint32_t c = c0 + (uint32_t)t * nthr;
             if (c < F.n) res[c] = a[t] ? v[t] : 0.f;
         }
     }
 }
 template <bool ST, typename V>
-__device__ __forceinline__ void tail_residual(const V &F, float *res)
+__device__ __forceinline__ void tail_residual(const V &F, float *res, uint32_t tid, uint32_t nthr)
 {
-    if (F.n <= kTailThreads) tail_residual_u<1, ST>(F, res);
-    else if (F.n <= 2 * kTailThreads) tail_residual_u<2, ST>(F, res);
-    else tail_residual_u<4, ST>(F, res);
+    if (F.n <= nthr) tail_residual_u<1, ST>(F, res, tid, nthr);
+    else if (F.n <= 2 * nthr) tail_residual_u<2, ST>(F, res, tid, nthr);
+    else tail_residual_u<4, ST>(F, res, tid, nthr);
 }
 
 // F.z += prolonged C.z on unknown cells
 template <int U, bool ST, typename VF, typename VC>
-__device__ __forceinline__ void tail_prolong_u(const VF &F, const VC &C)
+__device__ __forceinline__ void tail_prolong_u(const VF &F, const VC &C, uint32_t tid, uint32_t nthr)
 {
     const TailOp o = tail_op<ST>(F);
-    for (uint32_t c0 = threadIdx.x; c0 < F.n; c0 += U * kTailThreads) {
+    for (uint32_t c0 = tid; c0 < F.n; c0 += U * nthr) {
         float v[U];
         uint8_t a[U];
 #pragma unroll
         for (int t = 0; t < U; t++) {
-            const uint32_t c = c0 + (uint32_t)t * kTailThreads;
+            const uint32_t c = c0 + (uint32_t)t * nthr;
             const uint32_t cc = c < F.n ? c : 0u;
             int i, j, k;
             decode(cc, F.f_nynz, F.f_nz, F.ny, F.nz, i, j, k);
@@ -1237,24 +1285,25 @@ __device__ __forceinline__ void tail_prolong_u(const VF &F, const VC &C)
         }
 #pragma unroll
         for (int t = 0; t < U; t++) {
-            const uint32_t c = c0 + (uint32_t)t * kTailThreads;
+            const uint32_t c = c0 + (uint32_t)t * nthr;
             if (c < F.n && a[t]) stv(F.z, c, v[t]);
         }
     }
 }
 template <bool ST, typename VF, typename VC>
-__device__ __forceinline__ void tail_prolong(const VF &F, const VC &C)
+__device__ __forceinline__ void tail_prolong(const VF &F, const VC &C, uint32_t tid, uint32_t nthr)
 {
-    if (F.n <= kTailThreads) tail_prolong_u<1, ST>(F, C);
-    else if (F.n <= 2 * kTailThreads) tail_prolong_u<2, ST>(F, C);
-    else tail_prolong_u<4, ST>(F, C);
+    if (F.n <= nthr) tail_prolong_u<1, ST>(F, C, tid, nthr);
+    else if (F.n <= 2 * nthr) tail_prolong_u<2, ST>(F, C, tid, nthr);
+    else tail_prolong_u<4, ST>(F, C, tid, nthr);
 }
 
 // rc = P^T res: each coarse cell of C sums its fine children
 template <typename VC>
-__device__ __forceinline__ void tail_restrict_sum(const MgLevel &F, const VC &C, const float *res)
+__device__ __forceinline__ void tail_restrict_sum(const MgLevel &F, const VC &C, const float *res,
+                                                  uint32_t tid, uint32_t nthr)
 {
-    for (uint32_t I = threadIdx.x; I < C.n; I += kTailThreads) {
+    for (uint32_t I = tid; I < C.n; I += nthr) {
         int ic, jc, kc;
         decode(I, C.f_nynz, C.f_nz, C.ny, C.nz, ic, jc, kc);
         const int i0 = ic << C.shx, j0 = jc << C.shy, k0 = kc << C.shz;
@@ -1272,29 +1321,30 @@ __device__ __forceinline__ void tail_restrict_sum(const MgLevel &F, const VC &C,
 // one level of the way down: pre-smoothing, the residual into res (loads
 // issued together), then each coarse cell sums its children
 template <typename VF, typename VC>
-__device__ __forceinline__ void tail_down(const TailLevels &tl, const VF &F, const VC &C, float *res)
+__device__ __forceinline__ void tail_down(const TailLevels &tl, const VF &F, const VC &C, float *res,
+                                          TailSync &ts)
 {
     for (int sw = 0; sw < tl.sweeps; sw++) {
-        tail_smooth(F, 0, sw == 0);
-        tail_smooth(F, 1, 0);
+        tail_smooth(F, 0, sw == 0, ts);
+        tail_smooth(F, 1, 0, ts);
     }
-    if (F.staged) tail_residual<true>(F, res);
-    else tail_residual<false>(F, res);
-    __syncthreads();
-    tail_restrict_sum(*F.G, C, res);
-    __syncthreads();
+    if (F.staged) tail_residual<true>(F, res, ts.tid, ts.nthr);
+    else tail_residual<false>(F, res, ts.tid, ts.nthr);
+    ts.sync();
+    tail_restrict_sum(*F.G, C, res, ts.tid, ts.nthr);
+    ts.sync();
 }
 
 // one level of the way up: prolongation, then post-smoothing
 template <typename VF, typename VC>
-__device__ __forceinline__ void tail_up(const TailLevels &tl, const VF &F, const VC &C)
+__device__ __forceinline__ void tail_up(const TailLevels &tl, const VF &F, const VC &C, TailSync &ts)
 {
-    if (F.staged) tail_prolong<true>(F, C);
-    else tail_prolong<false>(F, C);
-    __syncthreads();
+    if (F.staged) tail_prolong<true>(F, C, ts.tid, ts.nthr);
+    else tail_prolong<false>(F, C, ts.tid, ts.nthr);
+    ts.sync();
     for (int sw = 0; sw < tl.sweeps; sw++) {
-        tail_smooth(F, 1, 0);
-        tail_smooth(F, 0, 0);
+        tail_smooth(F, 1, 0, ts);
+        tail_smooth(F, 0, 0, ts);
     }
 }
 
@@ -1304,103 +1354,150 @@ struct KB_mg_tail {
 {
     extern __shared__ __align__(16) unsigned char tail_smem[];
     if (stopped(m)) return;
-    const int lf = max(tl.first, tl.lds_from);  // the first level in LDS
-    for (int l = lf; l <= tl.last; l++) {  // stage the operators, clear the pads
-        const MgLevel &G = tl.lv[l];
-        const TailView<T, false> L = tail_view<T, false>(tl, tail_smem, l);
-        if (L.staged) {
-            float *cx = reinterpret_cast<float *>(L.base + L.tc.cx),
-                  *cy = reinterpret_cast<float *>(L.base + L.tc.cy),
-                  *cz = reinterpret_cast<float *>(L.base + L.tc.cz),
-                  *dg = reinterpret_cast<float *>(L.base + L.tc.diag),
-                  *rd = reinterpret_cast<float *>(L.base + L.tc.rdiag);
-            uint8_t *act = L.base + L.tc.act;
-            for (uint32_t c = threadIdx.x; c < L.n; c += kTailThreads) {
-                cx[c] = G.cx[c];
-                cy[c] = G.cy[c];
-                cz[c] = G.cz[c];
-                dg[c] = G.diag[c];
-                rd[c] = G.rdiag[c];
-                act[c] = G.active[c];
-            }
-            for (uint32_t c = L.n + threadIdx.x; c < L.n + L.si; c += kTailThreads) cx[c] = 0.f;
-            for (uint32_t c = L.n + threadIdx.x; c < L.n + L.sj; c += kTailThreads) cy[c] = 0.f;
-            if (threadIdx.x == 0) cz[L.n] = 0.f;
-        }
-        for (uint32_t c = threadIdx.x; c < L.si; c += kTailThreads) {
-            stv(L.z - L.si, c, 0.f);  // the pads below and above the level
-            stv(L.z + L.n, c, 0.f);
-        }
+    // a group (global levels shared by G workgroups on XCD tl.xcd): this
+    // block's member index w; member 0 also runs the LDS levels
+    const unsigned G = tl.groups > 1 && tl.first < tl.lds_from ? (unsigned)tl.groups : 1u;
+    uint32_t w = 0;
+    if (G > 1) {
+        if ((int)(blockIdx.x % kXcds) != tl.xcd) return;
+        w = blockIdx.x / kXcds;
     }
-    if (tl.first >= tl.lds_from) {  // the first level's r into LDS
-        const TailView<T, false> F = tail_view<T, false>(tl, tail_smem, tl.first);
-        const T *rg = (const T *)tl.lv[tl.first].r;
-        for (uint32_t c = threadIdx.x; c < F.n; c += kTailThreads) F.r[c] = rg[c];
+    const bool lead = w == 0;
+    TailSync solo = tail_solo();
+    TailSync grp = solo;
+    if (G > 1) {
+        grp.tid = w * kTailThreads + threadIdx.x;
+        grp.nthr = G * kTailThreads;
+        grp.ctr = tl.gsync;
+        grp.fault = tl.fault;
+        grp.G = G;
+    }
+    const int lf = max(tl.first, tl.lds_from);  // the first level in LDS
+    if (lead) {
+        for (int l = lf; l <= tl.last; l++) {  // stage the operators, clear the pads
+            const MgLevel &Lv = tl.lv[l];
+            const TailView<T, false> L = tail_view<T, false>(tl, tail_smem, l);
+            if (L.staged) {
+                float *cx = reinterpret_cast<float *>(L.base + L.tc.cx),
+                      *cy = reinterpret_cast<float *>(L.base + L.tc.cy),
+                      *cz = reinterpret_cast<float *>(L.base + L.tc.cz),
+                      *dg = reinterpret_cast<float *>(L.base + L.tc.diag),
+                      *rd = reinterpret_cast<float *>(L.base + L.tc.rdiag);
+                uint8_t *act = L.base + L.tc.act;
+                for (uint32_t c = threadIdx.x; c < L.n; c += kTailThreads) {
+                    cx[c] = Lv.cx[c];
+                    cy[c] = Lv.cy[c];
+                    cz[c] = Lv.cz[c];
+                    dg[c] = Lv.diag[c];
+                    rd[c] = Lv.rdiag[c];
+                    act[c] = Lv.active[c];
+                }
+                for (uint32_t c = L.n + threadIdx.x; c < L.n + L.si; c += kTailThreads) cx[c] = 0.f;
+                for (uint32_t c = L.n + threadIdx.x; c < L.n + L.sj; c += kTailThreads) cy[c] = 0.f;
+                if (threadIdx.x == 0) cz[L.n] = 0.f;
+            }
+            for (uint32_t c = threadIdx.x; c < L.si; c += kTailThreads) {
+                stv(L.z - L.si, c, 0.f);  // the pads below and above the level
+                stv(L.z + L.n, c, 0.f);
+            }
+        }
+        if (tl.first >= tl.lds_from) {  // the first level's r into LDS
+            const TailView<T, false> F = tail_view<T, false>(tl, tail_smem, tl.first);
+            const T *rg = (const T *)tl.lv[tl.first].r;
+            for (uint32_t c = threadIdx.x; c < F.n; c += kTailThreads) F.r[c] = rg[c];
+        }
     }
     __syncthreads();
     float *const lres = reinterpret_cast<float *>(tail_smem + tl.scratch_off);
-    // down: pre-smoothing and restriction (global levels' residuals go to
-    // the global scratch, LDS levels' to the LDS one)
-    for (int l = tl.first; l < tl.last; l++) {
-        if (l + 1 < tl.lds_from)
+    // down: pre-smoothing and restriction of the global levels (residuals to
+    // the global scratch); a group restricts the last of them into the
+    // global r of the first LDS level, which its lead then loads into LDS
+    for (int l = tl.first; l < tl.lds_from && l < tl.last; l++) {
+        if (l + 1 < tl.lds_from || G > 1)
             tail_down(tl, tail_view<T, true>(tl, tail_smem, l), tail_view<T, true>(tl, tail_smem, l + 1),
-                      tl.gscratch);
-        else if (l < tl.lds_from)
+                      tl.gscratch, grp);
+        else
             tail_down(tl, tail_view<T, true>(tl, tail_smem, l), tail_view<T, false>(tl, tail_smem, l + 1),
-                      tl.gscratch);
-        else
+                      tl.gscratch, grp);
+    }
+    if (G > 1 && lead) {  // the group's restriction into the first LDS level
+        const TailView<T, false> C = tail_view<T, false>(tl, tail_smem, lf);
+        const T *rg = (const T *)tl.lv[lf].r;
+        for (uint32_t c = threadIdx.x; c < C.n; c += kTailThreads) C.r[c] = rg[c];
+        __syncthreads();
+    }
+    if (lead) {
+        // the LDS levels (LDS residual scratch)
+        for (int l = lf; l < tl.last; l++)
             tail_down(tl, tail_view<T, false>(tl, tail_smem, l), tail_view<T, false>(tl, tail_smem, l + 1),
-                      lres);
-    }
-    // coarsest: z = G r with the dense inverse, or (red, black)^s then
-    // (black, red)^s from z = 0
-    {
-        const TailView<T, false> L = tail_view<T, false>(tl, tail_smem, tl.last);
-        if (tl.ginv) {
-            // thread i forms row i; G is symmetric, so reading G[j][i] for
-            // consecutive i is a coalesced read of row j; r_j is broadcast
-            const uint32_t i = threadIdx.x;
-            float acc = 0.f;
-            if (i < L.n) {
-                const float *gcol = tl.ginv + i;
-                uint32_t j = 0;
-                for (; j + 8 <= L.n; j += 8) {
-                    float gv[8];
+                      lres, solo);
+        // coarsest: z = G r with the dense inverse, or (red, black)^s then
+        // (black, red)^s from z = 0
+        {
+            const TailView<T, false> L = tail_view<T, false>(tl, tail_smem, tl.last);
+            if (tl.ginv) {
+                // thread i forms row i; G is symmetric, so reading G[j][i] for
+                // consecutive i is a coalesced read of row j; r_j is broadcast
+                const uint32_t i = threadIdx.x;
+                float acc = 0.f;
+                if (i < L.n) {
+                    const float *gcol = tl.ginv + i;
+                    uint32_t j = 0;
+                    for (; j + 8 <= L.n; j += 8) {
+                        float gv[8];
 #pragma unroll
-                    for (int t = 0; t < 8; t++) gv[t] = gcol[(size_t)(j + t) * L.n];
+                        for (int t = 0; t < 8; t++) gv[t] = gcol[(size_t)(j + t) * L.n];
 #pragma unroll
-                    for (int t = 0; t < 8; t++) acc += gv[t] * ldv(L.r, j + t);
+                        for (int t = 0; t < 8; t++) acc += gv[t] * ldv(L.r, j + t);
+                    }
+                    for (; j < L.n; j++) acc += gcol[(size_t)j * L.n] * ldv(L.r, j);
+                    stv(L.z, i, acc);
                 }
-                for (; j < L.n; j++) acc += gcol[(size_t)j * L.n] * ldv(L.r, j);
-                stv(L.z, i, acc);
+                __syncthreads();
+            } else if (threadIdx.x < kWave) {
+                // one wave: its LDS accesses are ordered, so passes need no
+                // workgroup barrier (a wavefront fence orders them for the compiler)
+                for (int pass = 0; pass < 4 * tl.coarse_sweeps; pass++) {
+                    const int half = pass / (2 * tl.coarse_sweeps);
+                    const uint32_t col = (uint32_t)(((pass & 1) ^ half) & 1);
+                    if (L.staged) tail_pass<true>(L, col, pass == 0, threadIdx.x, kWave);
+                    else tail_pass<false>(L, col, pass == 0, threadIdx.x, kWave);
+                    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+                }
             }
-            __syncthreads();
-        } else if (threadIdx.x < kWave) {
-            // one wave: its LDS accesses are ordered, so passes need no
-            // workgroup barrier (a wavefront fence orders them for the compiler)
-            for (int pass = 0; pass < 4 * tl.coarse_sweeps; pass++) {
-                const int half = pass / (2 * tl.coarse_sweeps);
-                const uint32_t col = (uint32_t)(((pass & 1) ^ half) & 1);
-                if (L.staged) tail_pass<true>(L, col, pass == 0, threadIdx.x, kWave);
-                else tail_pass<false>(L, col, pass == 0, threadIdx.x, kWave);
-                __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-            }
+            if (!tl.ginv) __syncthreads();
         }
-        if (!tl.ginv) __syncthreads();
+        for (int l = tl.last - 1; l >= lf; l--)
+            tail_up(tl, tail_view<T, false>(tl, tail_smem, l), tail_view<T, false>(tl, tail_smem, l + 1),
+                    solo);
+        if (G > 1) {  // the first LDS level's z to its global array, for the group's prolongation
+            const TailView<T, false> C = tail_view<T, false>(tl, tail_smem, lf);
+            T *zg = (T *)tl.lv[lf].z;
+            for (uint32_t c = threadIdx.x; c < C.n; c += kTailThreads) zg[c] = C.z[c];
+        }
     }
-    // up: prolongation and post-smoothing
-    for (int l = tl.last - 1; l >= tl.first; l--) {
-        if (l + 1 < tl.lds_from)
-            tail_up(tl, tail_view<T, true>(tl, tail_smem, l), tail_view<T, true>(tl, tail_smem, l + 1));
-        else if (l < tl.lds_from)
-            tail_up(tl, tail_view<T, true>(tl, tail_smem, l), tail_view<T, false>(tl, tail_smem, l + 1));
+    // up: prolongation and post-smoothing of the global levels
+    if (G > 1) grp.sync();  // the lead's z of the first LDS level is in global memory
+    for (int l = min(tl.lds_from, tl.last) - 1; l >= tl.first; l--) {
+        if (l + 1 < tl.lds_from || G > 1)
+            tail_up(tl, tail_view<T, true>(tl, tail_smem, l), tail_view<T, true>(tl, tail_smem, l + 1), grp);
         else
-            tail_up(tl, tail_view<T, false>(tl, tail_smem, l), tail_view<T, false>(tl, tail_smem, l + 1));
+            tail_up(tl, tail_view<T, true>(tl, tail_smem, l), tail_view<T, false>(tl, tail_smem, l + 1), grp);
     }
-    if (tl.first >= tl.lds_from) {  // the first level's z back to global memory
+    if (tl.first >= tl.lds_from && lead) {  // the first level's z back to global memory
         const TailView<T, false> F = tail_view<T, false>(tl, tail_smem, tl.first);
         T *zg = (T *)tl.lv[tl.first].z;
         for (uint32_t c = threadIdx.x; c < F.n; c += kTailThreads) zg[c] = F.z[c];
+    }
+    if (G > 1 && threadIdx.x == 0) {
+        // every member has left its last wait once it departs: the last one
+        // to depart resets the counters for the next launch
+        const unsigned d = __hip_atomic_fetch_add(tl.gsync + 1, 1u, __ATOMIC_RELAXED,
+                                                  __HIP_MEMORY_SCOPE_AGENT);
+        if (d == G - 1) {
+            __hip_atomic_store(tl.gsync, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_store(tl.gsync + 1, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
     }
 }
 };
@@ -2149,8 +2246,16 @@ int mg_tail_allow_lds()
 template <typename T>
 void mg_launch_tail(hipStream_t s, const TailLevels &tl, size_t lds_bytes, const MgState *m)
 {
-    FFR_GANG((KB_mg_tail<T>), GB_WIDE, k_mg_tail<T>, dim3(1), dim3(kTailThreads), lds_bytes, s, tl,
-             m);
+    if (t_rec) {  // a gang member's tail runs as one workgroup
+        TailLevels one = tl;
+        one.groups = 1;
+        FFR_GANG((KB_mg_tail<T>), GB_WIDE, k_mg_tail<T>, dim3(1), dim3(kTailThreads), lds_bytes, s,
+                 one, m);
+        return;
+    }
+    // a group: kXcds blocks per member, of which the one on XCD tl.xcd stays
+    const unsigned grid = tl.groups > 1 && tl.first < tl.lds_from ? kXcds * (unsigned)tl.groups : 1u;
+    FFR_LAUNCH(k_mg_tail<T>, dim3(grid), dim3(kTailThreads), lds_bytes, s, tl, m);
 }
 
 template <typename T>

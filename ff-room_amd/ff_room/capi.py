@@ -74,7 +74,8 @@ class Params(C.Structure):
                 ("mg_bf16", C.c_int), ("mg_pre_sweeps", C.c_int), ("mg_level_sweeps", C.c_int),
                 ("mg_coarse_sweeps", C.c_int),
                 ("inner_tol", C.c_double), ("inner_max_iter", C.c_int), ("graphs", C.c_int),
-                ("mg_tail_kb", C.c_int), ("warm_start", C.c_int), ("mg_tail_cells", C.c_int)]
+                ("mg_tail_kb", C.c_int), ("warm_start", C.c_int), ("mg_tail_cells", C.c_int),
+                ("mg_tail_groups", C.c_int)]
 
 
 class StepResult(C.Structure):
@@ -125,6 +126,7 @@ SIGNATURES = {
     "ffr_ctx_trim": (C.c_size_t, [_P]),
     "ffr_scene_mg_layout": (_I, [_P, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int),
                                  C.POINTER(C.c_uint)]),
+    "ffr_scene_mg_tail_groups": (_I, [_P]),
     "ffr_scene_footprint": (C.c_size_t, [C.POINTER(SceneDesc), C.POINTER(Params)]),
     "ffr_scene_create": (_I, [_P, C.POINTER(SceneDesc), C.POINTER(Params), C.POINTER(_P)]),
     "ffr_scene_destroy": (_V, [_P]),
@@ -506,8 +508,11 @@ class DeviceScene:
         cells = (C.c_uint * 12)()
         self.ctx.check(self.ctx._l.ffr_scene_mg_layout(self.h, C.byref(lv), C.byref(tf), C.byref(tl),
                                                        cells), "ffr_scene_mg_layout")
+        groups = self.ctx._l.ffr_scene_mg_tail_groups(self.h)
+        if groups < 0:
+            self.ctx.check(groups, "ffr_scene_mg_tail_groups")
         return {"levels": lv.value, "tail_first": tf.value, "tail_lds_from": tl.value,
-                "cells": [cells[i] for i in range(lv.value)]}
+                "cells": [cells[i] for i in range(lv.value)], "tail_groups": groups}
 
     def prof_kernel(self, name: str, reps: int = 200):
         """(mean launch ms, algorithmic bytes) of `reps` back-to-back launches."""

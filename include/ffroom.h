@@ -146,6 +146,10 @@ typedef struct {
                             global memory: one launch per V-cycle for all of them instead of
                             several per level (fewer dispatches for small scenes served side
                             by side); 0 = only the levels that fit its LDS budget */
+    int mg_tail_groups;  /* workgroups (1..8) that share those global levels, all on one XCD
+                            and ordered by a barrier in its L2 (the LDS levels run on one of
+                            them); 0 = one per 16384 cells of the largest global level.
+                            Results are the same bits for every count */
 } ffr_params;
 
 /* StepResult, FluidSolver.hpp:40-47, plus solver statistics. */
@@ -217,6 +221,10 @@ size_t ffr_ctx_trim(ffr_ctx *ctx);
  * PoissonSolver.cpp:164-505, has a fixed level list): for tests and tuning. */
 int ffr_scene_mg_layout(const ffr_scene *scene, int *levels, int *tail_first, int *tail_lds_from,
                         unsigned int *cells);
+/* Workgroups that run the single-workgroup kernel's global levels together
+ * (1 when it has none, or one workgroup; 0 without a hierarchy); negative on
+ * error. No counterpart in the reference: for tests and tuning. */
+int ffr_scene_mg_tail_groups(const ffr_scene *scene);
 /* Device bytes a scene with this geometry and these parameters occupies. */
 size_t ffr_scene_footprint(const ffr_scene_desc *desc, const ffr_params *params);
 /* Copies the geometry to the device and allocates the state (zero fields). */
