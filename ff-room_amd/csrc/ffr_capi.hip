@@ -84,7 +84,7 @@ void ffr_params_default(ffr_params *p)
     p->inner_max_iter = 60;
     p->graphs = 1;
     p->mg_tail_kb = 0;
-    p->warm_start = 1;
+    p->warm_start = 3;
     p->mg_tail_cells = 0;
     p->mg_tail_groups = 0;
 }

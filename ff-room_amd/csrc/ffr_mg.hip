@@ -689,9 +689,17 @@ int mg_begin(ffr_scene *s)
     {
         // a step starts from the pressure of the previous step (warm_start);
         // a bare solve, like PoissonSolver::solve, from 0
-        const bool warm = s->prm.warm_start != 0 && !t.solve_only;
-        ProfScope ps(s->ctx, st, "dc_init", n * (8 + 8 + 4 + 1));
-        dc_launch_init(st, g, s->b, s->p, M.e, warm, s->red);
+        // (warm_start 2 and 3: extrapolated linearly or quadratically from
+        // the pressures of the last two or three steps, as far as the scene
+        // has them)
+        const int ws = std::max(0, std::min(3, s->prm.warm_start));
+        const bool warm = ws != 0 && !t.solve_only;
+        const bool keep = warm && ws >= 2 && s->p_hist;
+        const int mode = warm ? std::min(ws, 1 + (keep ? s->p_hist_n : 0)) : 0;
+        double *h1 = keep ? s->p_hist : nullptr, *h2 = keep && ws == 3 ? s->p_hist2 : nullptr;
+        ProfScope ps(s->ctx, st, "dc_init", n * (8 + 8 + 4 + 1) + (h1 ? n * 16 : 0) + (h2 ? n * 16 : 0));
+        dc_launch_init(st, g, s->b, s->p, M.e, mode, h1, h2, s->red);
+        s->p_hist_n = keep ? std::min(2, s->p_hist_n + 1) : 0;
     }
     dc_launch_start(st, g, s->mgs, s->red, s->prm.poisson_tol, /*max_outer=*/50, max_inner_of(s),
                     s->prm.inner_tol, std::max(1, s->prm.poisson_max_iter));

@@ -117,7 +117,7 @@ struct ArenaPtrs {
     OpeningDev *opens;
     uint8_t *wall;
     char *u, *v, *w, *us, *vs, *ws, *T, *Tn;
-    double *p, *b, *r, *d, *Ad, *scratch;
+    double *p, *b, *r, *d, *Ad, *scratch, *p_hist, *p_hist2;
     StepRec *rec;
     CgState *cg;
     double *partials;
@@ -155,6 +155,8 @@ size_t layout(char *base, const ffr_scene_desc *d, const ffr_params *p, ArenaPtr
     a.d = (double *)take(8 * nc);
     a.Ad = (double *)take(8 * nc);
     a.scratch = (p->precision == FFR_PREC_MIXED) ? (double *)take(8 * std::max({nu, nv, nw})) : nullptr;
+    a.p_hist = (p->precision == FFR_PREC_MIXED) ? (double *)take(8 * nc) : nullptr;
+    a.p_hist2 = (p->precision == FFR_PREC_MIXED) ? (double *)take(8 * nc) : nullptr;
     a.rec = (StepRec *)take(sizeof(StepRec));
     a.cg = (CgState *)take(sizeof(CgState));
     a.partials = (double *)take(sizeof(double) * kMaxBlocksCap * kSlots);
@@ -468,6 +470,8 @@ int ffr_scene_create(ffr_ctx *ctx, const ffr_scene_desc *d, const ffr_params *pr
     s->T = a.T; s->Tn = a.Tn;
     s->p = a.p; s->b = a.b; s->r = a.r; s->d = a.d; s->Ad = a.Ad;
     s->scratch64 = a.scratch;
+    s->p_hist = a.p_hist;
+    s->p_hist2 = a.p_hist2;
     s->rec = a.rec; s->cg = a.cg;
     s->red.partials = a.partials;
     s->red.tickets = a.tickets;
@@ -539,6 +543,7 @@ int ffr_scene_set_params(ffr_scene *s, const ffr_params *p)
                             o.mg_tail_groups != p->mg_tail_groups;
     s->prm = *p;
     s->pred = SolvePred{};  // iteration counts of other settings do not carry over
+    s->p_hist_n = 0;
     if (s->mg.levels && mg_changed) mg_destroy(s);
     if (need_mg && !s->mg.levels) return mg_create(s);
     return FFR_OK;
@@ -561,6 +566,9 @@ int ffr_scene_upload(ffr_scene *s, const double *u, const double *v, const doubl
     hipStream_t st = s->stream;
     const double *src[5] = {u, v, w, p, T};
     void *dst[5] = {s->u, s->v, s->w, s->p, s->T};
+    // an uploaded p takes the place of the last step's pressure; the older
+    // ones stay (a host mirror pushed back unchanged then steps exactly as
+    // if it had not been)
     const uint64_t cnt[5] = {s->nu, s->nv, s->nw, s->nc, s->nc};
     for (int q = 0; q < 5; q++) {
         if (!src[q]) continue;
@@ -707,6 +715,7 @@ int ffr_apply_correction(ffr_scene *s, const double *p, double *velocity_change)
     hipStream_t st = s->stream;
     const StepParams sp = make_step_params(s->prm);
     HIP_CHECK_CTX(s->ctx, hipMemcpyAsync(s->p, p, s->nc * 8, hipMemcpyHostToDevice, st), FFR_E_DEVICE);
+    s->p_hist_n = 0;
     HIP_CHECK_CTX(s->ctx, hipMemsetAsync(&s->rec->vmax_bits, 0, sizeof(unsigned long long), st), FFR_E_DEVICE);
     int rc = state_to_star(s);
     if (rc != FFR_OK) return rc;

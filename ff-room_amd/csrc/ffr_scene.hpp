@@ -159,6 +159,10 @@ struct ffr_scene {
     ffr::CgState *cg = nullptr;
     ffr::RedScratch red{};
     double *scratch64 = nullptr;  // nc doubles for host transfers of fp32 state
+    // mixed form: the pressures of the two steps before the last (warm_start
+    // 2 and 3 extrapolate from them); p_hist_n of them are valid
+    double *p_hist = nullptr, *p_hist2 = nullptr;
+    int p_hist_n = 0;
     ffr::Multigrid mg{};          // multigrid hierarchy (when allocated)
     ffr::MgState *mgs = nullptr;  // device scalars of the multigrid solve
 

@@ -139,7 +139,11 @@ typedef struct {
     int mg_tail_kb;      /* LDS budget (KiB, <= 150) of the single-workgroup kernel that runs
                             the smallest multigrid levels; 0 = 96 */
     int warm_start;      /* mixed MG-PCG: a step's pressure solve starts from the previous
-                            step's pressure (1) or from 0 as the reference CG does (0);
+                            step's pressure (1), from its linear extrapolation in time
+                            2 p_n - p_{n-1} (2) or quadratic one 3 p_n - 3 p_{n-1} + p_{n-2}
+                            (3, the default) (the first steps after creation or new params
+                            use as many past pressures as there are; an uploaded p takes the
+                            place of p_n), or from 0 as the reference CG does (0);
                             ffr_poisson_solve always starts from 0 */
     int mg_tail_cells;   /* levels >= 1 of at most this many cells also run inside the
                             single-workgroup kernel, with their vectors and operator in

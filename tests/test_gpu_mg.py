@@ -135,14 +135,15 @@ def _workload_scene(grid, seed=2):
 
 
 def test_warm_start_meets_the_same_tolerance(ctx):
-    """A step's mixed pressure solve started from the previous pressure meets
-    the same relative residual as one started from 0 (the reference CG's
-    start, PoissonSolver.cpp:96), and the trajectories agree to the solve's
-    accuracy; a bare solve starts from 0 either way."""
+    """A step's mixed pressure solve started from the previous pressure, or
+    extrapolated from the last two, meets the same relative residual as one
+    started from 0 (the reference CG's start, PoissonSolver.cpp:96), and the
+    trajectories agree to the solve's accuracy; a bare solve starts from 0
+    either way."""
     from ff_room import device
     ps = _workload_scene((64, 52, 20))
     out = []
-    for warm in (0, 1):
+    for warm in (0, 1, 2, 3):
         pm = capi.default_params(**{k: getattr(ps.params, k) for k, _ in capi.Params._fields_})
         pm.precision, pm.poisson_method, pm.poisson_tol, pm.poisson_max_iter = 1, 2, 1e-10, 500
         pm.warm_start = warm
@@ -158,7 +159,11 @@ def test_warm_start_meets_the_same_tolerance(ctx):
     m = ps.flags != SOLID
     for k in ("u", "v", "w"):
         assert rel_l2(out[1][k], out[0][k]) < 1e-6, k
+        assert rel_l2(out[2][k], out[0][k]) < 1e-6, k
+        assert rel_l2(out[3][k], out[0][k]) < 1e-6, k
     assert rel_l2(out[1]["p"][m], out[0]["p"][m]) < 1e-6
+    assert rel_l2(out[2]["p"][m], out[0]["p"][m]) < 1e-6
+    assert rel_l2(out[3]["p"][m], out[0]["p"][m]) < 1e-6
 
 
 def test_block_cap_beyond_the_resident_grid(ctx):
