@@ -203,7 +203,7 @@ void ffr_ctx::prof_resolve()
         auto &e = prof_map[pe.name];
         e.launches++;
         e.total_ms += ms;
-        e.bytes = pe.bytes;
+        e.bytes += pe.bytes;  // launches of one name may move different bytes
         ev_pool.push_back(pe.a);
         ev_pool.push_back(pe.b);
     }
@@ -244,7 +244,8 @@ int ffr_prof_read(ffr_ctx *c, ffr_prof_entry *out, int cap, int *n_out)
             std::snprintf(out[n].name, sizeof out[n].name, "%s", kv.first.c_str());
             out[n].launches = kv.second.launches;
             out[n].total_ms = kv.second.total_ms;
-            out[n].bytes_per_launch = kv.second.bytes;
+            out[n].bytes_per_launch =
+                kv.second.launches ? kv.second.bytes / (double)kv.second.launches : 0.0;
         }
         n++;
     }

@@ -191,7 +191,7 @@ typedef struct {
     char name[48];
     long launches;
     double total_ms;
-    double bytes_per_launch; /* algorithmic bytes of the last launch */
+    double bytes_per_launch; /* mean algorithmic bytes per launch */
 } ffr_prof_entry;
 
 typedef void (*ffr_step_callback)(const ffr_step_result *, void *user);
