@@ -206,10 +206,13 @@ def serve_bench(ctx, n_c4: int, steps_c4: int, n_c5: int, steps_c5: int, gangs: 
     """Continuous batching samples: c4 (uneven rooms all submitted at once)
     and c5 (a seeded Poisson stream at 50 requests/s), mixed MG-PCG form;
     `gangs` groups of merged launches (0: every scene on its own stream).
-    Both in arrival order, and c5 again with the smallest requests admitted
-    first. `slots` scenes are resident at once: 6 measured best on MI355X
-    (c4 2.0e8 cell-updates/s, p50 0.66 s against 1.9e8, 0.81 s with 16; c5
-    the same throughput, p50 3.0 s against 3.9 s)."""
+    c4 is one batch, admitted largest work first (measured on MI355X, 32
+    rooms x 60 steps: 3.6-3.7e8 cell-updates/s against 3.0-3.4e8 in arrival
+    order, whose last large rooms run alone); c5 in arrival order, and again
+    with the smallest requests admitted first. `slots` scenes are resident at
+    once: 6 measured best on MI355X (c4 2.0e8 cell-updates/s, p50 0.66 s
+    against 1.9e8, 0.81 s with 16; c5 the same throughput, p50 3.0 s against
+    3.9 s)."""
     from ff_room import capi
     from ff_room.generator import workload_c4, workload_c5
     from ff_room.serve import Request, Server, summary
@@ -217,11 +220,12 @@ def serve_bench(ctx, n_c4: int, steps_c4: int, n_c5: int, steps_c5: int, gangs: 
     kw = dict(precision=capi.PREC_MIXED, poisson_method=capi.POISSON_MGPCG,
               poisson_tol=1e-10, poisson_max_iter=500)
     cfgs = workload_c4(n_scenes=n_c4, steps=steps_c4, precision="mixed")
-    srv = Server(ctx, slots=slots, gangs=gangs, **kw)
+    srv = Server(ctx, slots=slots, gangs=gangs, policy="longest", **kw)
     done = srv.run([Request(c, steps_c4) for c in cfgs], realtime=False)
     res["c4"] = dict(summary(done), gangs=gangs, gang_stats=ctx.gang_stats(),
                      sample=f"{n_c4} seeded rooms (grids 32x32x16..160x128x64), "
-                     f"{steps_c4} steps each, all submitted at t=0, {slots} slots")
+                     f"{steps_c4} steps each, all submitted at t=0, {slots} slots, "
+                     f"largest work admitted first")
     stream = list(workload_c5(n_requests=n_c5, steps=steps_c5, precision="mixed"))
     srv = Server(ctx, slots=slots, gangs=gangs, **kw)
     done = srv.run([Request(c, steps_c5, t) for t, c in stream], realtime=True)
@@ -259,10 +263,11 @@ def batched_forms(ctx, n_rooms: int, steps: int, n_err: int, slots: int = 6):
                         poisson_tol=FP64_TOL["c4"], poisson_max_iter=200000),
     }
     out = {"sample": f"first {n_rooms} rooms of c4, {steps} steps each, all submitted at t=0, "
-                     f"{slots} slots", "forms": {}}
+                     f"{slots} slots, largest work (cells x steps) admitted first in both forms",
+           "forms": {}}
     done_mixed = None
     for name, kw in forms.items():
-        srv = Server(ctx, slots=slots, keep_fields=(name == "mixed"), **kw)
+        srv = Server(ctx, slots=slots, keep_fields=(name == "mixed"), policy="longest", **kw)
         done = srv.run([Request(c, steps) for c in cfgs], realtime=False)
         sm = summary(done)
         out["forms"][name] = {"cell_updates_per_s": sm["cell_updates_per_s"],

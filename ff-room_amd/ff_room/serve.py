@@ -4,7 +4,9 @@ A request is a scene (SceneConfig), a step budget and an arrival time.
 Requests wait in arrival order (policy "fifo"), or ordered by their work,
 cells x step budget, smallest first (policy "shortest": lower median latency
 under load, at the price of making large requests wait behind later small
-ones). The head of the queue is admitted when a slot
+ones) or largest first (policy "longest": for a batch submitted at once, the
+largest scenes start first, so that the end of the batch is not one large
+scene running alone). The head of the queue is admitted when a slot
 is free and its device footprint (``ffr_scene_footprint``) fits the context's
 HBM budget beside the scenes already resident; the scene is then built (flags,
 boundary lists, initial temperature: ``device.prepare``), created on the
@@ -82,8 +84,8 @@ class Server:
     def __init__(self, ctx: capi.Context, slots: int = 6, hbm_budget: Optional[int] = None,
                  keep_fields: bool = False, tuned: bool = True, gangs: int = 0,
                  policy: str = "fifo", threads: Optional[int] = None, **overrides):
-        if policy not in ("fifo", "shortest"):
-            raise ValueError(f"unknown admission policy {policy!r} (fifo, shortest)")
+        if policy not in ("fifo", "shortest", "longest"):
+            raise ValueError(f"unknown admission policy {policy!r} (fifo, shortest, longest)")
         self.ctx = ctx
         self.slots = slots
         self.policy = policy
@@ -117,13 +119,14 @@ class Server:
 
     @staticmethod
     def work(req: Request) -> int:
-        """Cells x step budget: the admission key of policy "shortest"."""
+        """Cells x step budget: the admission key of policies "shortest" and "longest"."""
         g = req.config.room.grid
         return int(g[0]) * int(g[1]) * int(g[2]) * int(req.max_steps)
 
     def _admit(self):
-        if self.policy == "shortest" and len(self.pending) > 1:
-            self.pending = deque(sorted(self.pending, key=lambda e: (self.work(e[1]), e[0])))
+        if self.policy != "fifo" and len(self.pending) > 1:
+            sign = 1 if self.policy == "shortest" else -1
+            self.pending = deque(sorted(self.pending, key=lambda e: (sign * self.work(e[1]), e[0])))
         while self.pending and len(self.active) < self.slots:
             i, req, ps = self.pending[0]
             if ps is None:

@@ -125,7 +125,7 @@ def test_summary_statistics():
     assert s["cell_updates_per_s"] == pytest.approx(100 * 10 * 100 / 100.0)
 
 
-def test_shortest_policy_admits_smallest_work_first(monkeypatch):
+def test_work_policies_admit_by_work(monkeypatch):
     from types import SimpleNamespace
     ctx = _FakeCtx()
     monkeypatch.setattr(serve.device, "prepare", lambda cfg, **kw: _FakePS(cfg.need, cfg.cells))
@@ -140,6 +140,9 @@ def test_shortest_policy_admits_smallest_work_first(monkeypatch):
     done = srv.run(reqs, realtime=False)
     order = [c.index for c in sorted(done, key=lambda c: c.admitted)]
     assert order == [1, 3, 2, 0]
+    srv = serve.Server(ctx, slots=1, hbm_budget=1000, tuned=False, policy="longest")
+    done = srv.run(reqs, realtime=False)
+    assert [c.index for c in sorted(done, key=lambda c: c.admitted)] == [0, 2, 3, 1]
     srv = serve.Server(ctx, slots=1, hbm_budget=1000, tuned=False)  # fifo keeps arrival order
     done = srv.run(reqs, realtime=False)
     assert [c.index for c in sorted(done, key=lambda c: c.admitted)] == [0, 1, 2, 3]
