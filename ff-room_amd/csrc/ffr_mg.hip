@@ -692,13 +692,24 @@ int mg_begin(ffr_scene *s)
         // (warm_start 2 and 3: extrapolated linearly or quadratically from
         // the pressures of the last two or three steps, as far as the scene
         // has them)
+        // With a history, the start is written to the spare buffer (p_hist2,
+        // which holds p_{n-2} or nothing yet) and the three buffers rotate:
+        // the spare becomes the solve's x, x's buffer (p_n) p_{n-1}, and
+        // p_{n-1}'s buffer the next spare.
         const int ws = std::max(0, std::min(3, s->prm.warm_start));
         const bool warm = ws != 0 && !t.solve_only;
         const bool keep = warm && ws >= 2 && s->p_hist;
         const int mode = warm ? std::min(ws, 1 + (keep ? s->p_hist_n : 0)) : 0;
-        double *h1 = keep ? s->p_hist : nullptr, *h2 = keep && ws == 3 ? s->p_hist2 : nullptr;
-        ProfScope ps(s->ctx, st, "dc_init", n * (8 + 8 + 4 + 1) + (h1 ? n * 16 : 0) + (h2 ? n * 16 : 0));
-        dc_launch_init(st, g, s->b, s->p, M.e, mode, h1, h2, s->red);
+        ProfScope ps(s->ctx, st, "dc_init",
+                     n * (8 + 8 + 4 + 1) + (keep ? n * 8 * (mode == 1 ? 1 : mode) : 0));
+        dc_launch_init(st, g, s->b, s->p, M.e, mode, keep ? s->p_hist : nullptr,
+                       keep ? s->p_hist2 : nullptr, s->red);
+        if (keep) {
+            double *const x = s->p_hist2;
+            s->p_hist2 = s->p_hist;
+            s->p_hist = s->p;
+            s->p = x;
+        }
         s->p_hist_n = keep ? std::min(2, s->p_hist_n + 1) : 0;
     }
     dc_launch_start(st, g, s->mgs, s->red, s->prm.poisson_tol, /*max_outer=*/50, max_inner_of(s),

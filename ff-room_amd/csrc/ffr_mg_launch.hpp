@@ -44,12 +44,12 @@ void pcg_launch_update(hipStream_t s, const Geo &g, const MgLevel &L0, float *e,
 void pcg_launch_check(hipStream_t s, uint32_t n, MgState *m, RedScratch red);
 void pcg_launch_dot_rz(hipStream_t s, uint32_t n, const float *r, const float *z, const MgState *m,
                        RedScratch red);
-// warm 0: x starts at 0; 1: x keeps its values (the previous pressure p_n);
-// 2: x = 2 p_n - h1; 3: x = 3 p_n - 3 h1 + h2 (extrapolated in time from
-// the pressures of the steps before, h1 = p_{n-1}, h2 = p_{n-2}); with h1
-// (null: none) the history shifts by one step (h2 only when given)
+// warm 0: x starts at 0; 1: x keeps its values (the previous pressure p_n).
+// With `out`: the start goes there instead (warm 1: p_n; 2: 2 p_n - h1; 3:
+// 3 p_n - 3 h1 + out's old values, extrapolated in time from the pressures of
+// the steps before, h1 = p_{n-1}, out = p_{n-2}), and x keeps p_n
 void dc_launch_init(hipStream_t s, const Geo &g, const double *b, double *x, float *e, int warm,
-                    double *h1, double *h2, RedScratch red);
+                    const double *h1, double *out, RedScratch red);
 void dc_launch_start(hipStream_t s, const Geo &g, MgState *m, RedScratch red, double outer_tol,
                      int max_outer, int max_inner, double inner_tol, int total_cap);
 void dc_launch_residual(hipStream_t s, const Geo &g, const double *b, double *xa, double *xb,

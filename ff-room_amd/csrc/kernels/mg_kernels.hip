@@ -1801,14 +1801,17 @@ __global__ void __launch_bounds__(kBlock) k_pcg_dot_rz(uint32_t n, const float *
 // x = 0 (PoissonSolver.cpp:96) or, warm, the pressure x already holds (the
 // previous step's: SOLID cells 0, OUTFLOW values never read as unknowns);
 // e = 0; partials of ||b||^2 over non-SOLID cells (PoissonSolver.cpp:101-113)
-// warm 0: x = 0; 1: x keeps the previous step's pressure p_n; 2: x =
-// 2 p_n - p_{n-1}; 3: x = 3 p_n - 3 p_{n-1} + p_{n-2} (linear and quadratic
-// extrapolation in time, p_{n-1} in h1, p_{n-2} in h2). With h1, warm >= 1
-// shifts the history: h2 = p_{n-1} (with h2), h1 = p_n.
+// The solve's start, with x holding p_n, the last step's pressure:
+// warm 0: x = 0; 1: x keeps p_n. With `out` (a spare buffer of the
+// scene's, which the host then makes the solve's x, while x's buffer keeps
+// p_n as history): warm 1: out = p_n; 2: out = 2 p_n - p_{n-1}; 3: out =
+// 3 p_n - 3 p_{n-1} + p_{n-2} (linear and quadratic extrapolation in time,
+// p_{n-1} in h1; p_{n-2} is out's own old content). Nothing is copied to
+// shift the history: the host rotates the three buffers' roles.
 struct KB_dc_init {
     static __device__ __forceinline__ void run(const VB &vb, const Geo &g, const double *b,
-                                               double *x, float *e, int warm, double *h1,
-                                               double *h2, const RedScratch &red)
+                                               double *x, float *e, int warm, const double *h1,
+                                               double *out, const RedScratch &red)
 {
     __shared__ double lds[kBlock / kWave];
     const uint32_t blk = vb.blk;
@@ -1818,12 +1821,11 @@ struct KB_dc_init {
     for (uint32_t c = sb + threadIdx.x; c < se; c += kBlock) {
         if (!warm) {
             x[c] = 0.0;
-        } else if (h1) {
-            const double pn = x[c], p1 = h1[c];
-            if (warm == 2) x[c] = 2.0 * pn - p1;
-            if (warm == 3) x[c] = 3.0 * (pn - p1) + h2[c];
-            if (h2) h2[c] = p1;
-            h1[c] = pn;
+        } else if (out) {
+            const double pn = x[c];
+            if (warm == 1) out[c] = pn;
+            else if (warm == 2) out[c] = 2.0 * pn - h1[c];
+            else out[c] = 3.0 * (pn - h1[c]) + out[c];
         }
         e[c] = 0.f;
         if (g.flags[c] != SOLID) acc += b[c] * b[c];
@@ -1832,9 +1834,10 @@ struct KB_dc_init {
 }
 };
 __global__ void __launch_bounds__(kBlock) k_dc_init(Geo g, const double *b, double *x, float *e,
-                                                    int warm, double *h1, double *h2, RedScratch red)
+                                                    int warm, const double *h1, double *out,
+                                                    RedScratch red)
 {
-    KB_dc_init::run(vb_here(), g, b, x, e, warm, h1, h2, red);
+    KB_dc_init::run(vb_here(), g, b, x, e, warm, h1, out, red);
 }
 
 struct KB_dc_start {
@@ -2346,10 +2349,10 @@ void pcg_launch_dot_rz(hipStream_t s, uint32_t n, const float *r, const float *z
              z, m, red);
 }
 void dc_launch_init(hipStream_t s, const Geo &g, const double *b, double *x, float *e, int warm,
-                    double *h1, double *h2, RedScratch red)
+                    const double *h1, double *out, RedScratch red)
 {
     FFR_GANG((KB_dc_init), GB_PLAIN, k_dc_init, dim3(grid_for(g.nc)), dim3(kBlock), 0, s, g, b, x,
-             e, warm, h1, h2, red);
+             e, warm, h1, out, red);
 }
 void dc_launch_start(hipStream_t s, const Geo &g, MgState *m, RedScratch red, double outer_tol,
                      int max_outer, int max_inner, double inner_tol, int total_cap)
