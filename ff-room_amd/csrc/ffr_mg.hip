@@ -751,8 +751,12 @@ int mg_continue(ffr_scene *s, bool *finished)
         return mg_schedule(s);
     }
     {
-        ProfScope ps(s->ctx, s->stream, "dc_finalize", (double)s->geo.nc * (8 + 8 + 1));
-        dc_launch_finalize(s->stream, s->geo, s->b, s->p, s->d, s->p, s->prm.use_multigrid, s->mgs);
+        ProfScope ps(s->ctx, s->stream, "dc_finalize", (double)s->geo.nc * 2);  // codes; OUTFLOW rows
+        // the last pass wrote its x into s->d after an odd number of passes:
+        // the two buffers swap roles, so that x is p in place
+        if (h.passes & 1) std::swap(s->p, s->d);
+        dc_launch_finalize(s->stream, s->geo, s->b, s->p, s->d, s->p, s->prm.use_multigrid, true,
+                           s->mgs);
     }
     t.iters = h.total_iters;
     t.outer = h.passes;

@@ -55,6 +55,9 @@ void dc_launch_start(hipStream_t s, const Geo &g, MgState *m, RedScratch red, do
 void dc_launch_residual(hipStream_t s, const Geo &g, const double *b, double *xa, double *xb,
                         const float *e, float *r32, float *z0, const MgState *m, RedScratch red);
 void dc_launch_check(hipStream_t s, const Geo &g, MgState *m, RedScratch red);
+// p from the last outer pass's output (in xa or xb by the pass count); with
+// `inplace` that output is p itself (xa == p) and only OUTFLOW rows are written
 void dc_launch_finalize(hipStream_t s, const Geo &g, const double *b, const double *xa,
-                        const double *xb, double *p, int pin_outflow, const MgState *m);
+                        const double *xb, double *p, int pin_outflow, bool inplace,
+                        const MgState *m);
 }  // namespace ffr

@@ -2086,25 +2086,28 @@ __global__ void __launch_bounds__(kBlock) k_dc_check(MgState *m, RedScratch red,
 }
 
 // p: unknowns from x; OUTFLOW rows solved from their neighbours (or pinned
-// to 0 for the reference multigrid's convention); SOLID cells 0.
+// to 0 for the reference multigrid's convention); SOLID cells 0. In place
+// (xa == p holds the last pass's output, which the host knows from the pass
+// count): only the OUTFLOW rows are written; every outer pass has left 0 at
+// the SOLID and OUTFLOW cells of its output.
 template <bool ST>
 struct KB_dc_finalize {
     static __device__ __forceinline__ void run(const VB &vb, const Geo &g,
                                                const double *__restrict__ b, const double *xa,
                                                const double *xb, double *p, int pin_outflow,
-                                               const MgState *m)
+                                               int inplace, const MgState *m)
 {
-    const double *x = (m->passes & 1) ? xb : xa;  // the last pass's output
+    const double *x = inplace ? p : ((m->passes & 1) ? xb : xa);  // the last pass's output
     uint32_t sb, se;
     block_span_of(g.nc, vb.blk, vb.G, sb, se);
     for (uint32_t c = sb + threadIdx.x; c < se; c += kBlock) {
         const uint16_t nb = g.nbm[c];
         if (nb & NB_SELF) {
-            p[c] = x[c];
+            if (!inplace) p[c] = x[c];
             continue;
         }
         if ((nb & NB_SELF_SOLID) || pin_outflow) {
-            p[c] = 0.0;
+            if (!inplace) p[c] = 0.0;
             continue;
         }
         double diag, off;
@@ -2116,10 +2119,10 @@ struct KB_dc_finalize {
 template <bool ST>
 __global__ void __launch_bounds__(kBlock) k_dc_finalize(Geo g, const double *__restrict__ b,
                                                         const double *xa, const double *xb,
-                                                        double *p, int pin_outflow,
+                                                        double *p, int pin_outflow, int inplace,
                                                         const MgState *m)
 {
-    KB_dc_finalize<ST>::run(vb_here(), g, b, xa, xb, p, pin_outflow, m);
+    KB_dc_finalize<ST>::run(vb_here(), g, b, xa, xb, p, pin_outflow, inplace, m);
 }
 
 // ---- launchers ------------------------------------------------------------------
@@ -2376,14 +2379,15 @@ void dc_launch_check(hipStream_t s, const Geo &g, MgState *m, RedScratch red)
              grid_for(g.nc));
 }
 void dc_launch_finalize(hipStream_t s, const Geo &g, const double *b, const double *xa,
-                        const double *xb, double *p, int pin_outflow, const MgState *m)
+                        const double *xb, double *p, int pin_outflow, bool inplace,
+                        const MgState *m)
 {
     if (g.uniform)
         FFR_GANG((KB_dc_finalize<false>), GB_PLAIN, (k_dc_finalize<false>), dim3(grid_for(g.nc)),
-                 dim3(kBlock), 0, s, g, b, xa, xb, p, pin_outflow, m);
+                 dim3(kBlock), 0, s, g, b, xa, xb, p, pin_outflow, inplace ? 1 : 0, m);
     else
         FFR_GANG((KB_dc_finalize<true>), GB_PLAIN, (k_dc_finalize<true>), dim3(grid_for(g.nc)),
-                 dim3(kBlock), 0, s, g, b, xa, xb, p, pin_outflow, m);
+                 dim3(kBlock), 0, s, g, b, xa, xb, p, pin_outflow, inplace ? 1 : 0, m);
 }
 
 }  // namespace ffr
