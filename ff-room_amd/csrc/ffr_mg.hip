@@ -641,9 +641,12 @@ static void enqueue_pass(ffr_scene *s)
     const Geo &g = s->geo;
     const double n = (double)g.nc;
     {
-        ProfScope ps(s->ctx, st, "dc_residual", n * (8 + 8 + 4 + 1 + 8 + 4));
+        // the solve's first pass: no correction to add yet (x read, not written)
+        const bool first = s->task.passes_enq == 0;
+        ProfScope ps(s->ctx, st, "dc_residual",
+                     first ? n * (8 + 8 + 1 + 4 + 2) : n * (8 + 8 + 4 + 1 + 8 + 4 + 2));
         dc_launch_residual(st, g, s->b, s->p, s->d, s->mg.e, s->mg.ri, (float *)s->mg.lv[0].z,
-                           s->mgs, s->red);
+                           s->mgs, s->red, first);
     }
     dc_launch_check(st, g, s->mgs, s->red);
     s->task.passes_enq++;
@@ -701,8 +704,8 @@ int mg_begin(ffr_scene *s)
         const bool keep = warm && ws >= 2 && s->p_hist;
         const int mode = warm ? std::min(ws, 1 + (keep ? s->p_hist_n : 0)) : 0;
         ProfScope ps(s->ctx, st, "dc_init",
-                     n * (8 + 8 + 4 + 1) + (keep ? n * 8 * (mode == 1 ? 1 : mode) : 0));
-        dc_launch_init(st, g, s->b, s->p, M.e, mode, keep ? s->p_hist : nullptr,
+                     n * (8 + 8 + 1) + (keep ? n * 8 * (mode == 1 ? 1 : mode) : 0));
+        dc_launch_init(st, g, s->b, s->p, mode, keep ? s->p_hist : nullptr,
                        keep ? s->p_hist2 : nullptr, s->red);
         if (keep) {
             double *const x = s->p_hist2;
@@ -752,9 +755,9 @@ int mg_continue(ffr_scene *s, bool *finished)
     }
     {
         ProfScope ps(s->ctx, s->stream, "dc_finalize", (double)s->geo.nc * 2);  // codes; OUTFLOW rows
-        // the last pass wrote its x into s->d after an odd number of passes:
-        // the two buffers swap roles, so that x is p in place
-        if (h.passes & 1) std::swap(s->p, s->d);
+        // the last pass wrote its x into s->d after an even, non-zero number
+        // of passes: the two buffers swap roles, so that x is p in place
+        if (h.passes > 0 && !(h.passes & 1)) std::swap(s->p, s->d);
         dc_launch_finalize(s->stream, s->geo, s->b, s->p, s->d, s->p, s->prm.use_multigrid, true,
                            s->mgs);
     }
@@ -816,7 +819,7 @@ double mg_prof_launch(ffr_scene *s, const char *name, std::function<void()> *lau
         float *e = M.e, *r = M.ri;
         RedScratch red = s->red;
         float *z0 = (float *)L0.z;
-        *launch = [=] { dc_launch_residual(st, g, b, x0, x1, e, r, z0, m, red); };
+        *launch = [=] { dc_launch_residual(st, g, b, x0, x1, e, r, z0, m, red, false); };
         return n * (8 + 8 + 4 + 2 + 8 + 4);
     }
     *launch = [] {};

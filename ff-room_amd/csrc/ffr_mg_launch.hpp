@@ -48,12 +48,15 @@ void pcg_launch_dot_rz(hipStream_t s, uint32_t n, const float *r, const float *z
 // With `out`: the start goes there instead (warm 1: p_n; 2: 2 p_n - h1; 3:
 // 3 p_n - 3 h1 + out's old values, extrapolated in time from the pressures of
 // the steps before, h1 = p_{n-1}, out = p_{n-2}), and x keeps p_n
-void dc_launch_init(hipStream_t s, const Geo &g, const double *b, double *x, float *e, int warm,
+void dc_launch_init(hipStream_t s, const Geo &g, const double *b, double *x, int warm,
                     const double *h1, double *out, RedScratch red);
 void dc_launch_start(hipStream_t s, const Geo &g, MgState *m, RedScratch red, double outer_tol,
                      int max_outer, int max_inner, double inner_tol, int total_cap);
+// one outer pass (x += e, the FP64 residual); `first`: the solve's first pass
+// (e is 0 then, and x stays in xa)
 void dc_launch_residual(hipStream_t s, const Geo &g, const double *b, double *xa, double *xb,
-                        const float *e, float *r32, float *z0, const MgState *m, RedScratch red);
+                        const float *e, float *r32, float *z0, const MgState *m, RedScratch red,
+                        bool first);
 void dc_launch_check(hipStream_t s, const Geo &g, MgState *m, RedScratch red);
 // p from the last outer pass's output (in xa or xb by the pass count); with
 // `inplace` that output is p itself (xa == p) and only OUTFLOW rows are written

@@ -1800,7 +1800,9 @@ __global__ void __launch_bounds__(kBlock) k_pcg_dot_rz(uint32_t n, const float *
 // ---- outer FP64 defect correction -------------------------------------------
 // x = 0 (PoissonSolver.cpp:96) or, warm, the pressure x already holds (the
 // previous step's: SOLID cells 0, OUTFLOW values never read as unknowns);
-// e = 0; partials of ||b||^2 over non-SOLID cells (PoissonSolver.cpp:101-113)
+// partials of ||b||^2 over non-SOLID cells (PoissonSolver.cpp:101-113); the
+// outer correction e needs no clearing: the first outer pass does not read it
+// and each inner solve's first update writes it
 // The solve's start, with x holding p_n, the last step's pressure:
 // warm 0: x = 0; 1: x keeps p_n. With `out` (a spare buffer of the
 // scene's, which the host then makes the solve's x, while x's buffer keeps
@@ -1810,7 +1812,7 @@ __global__ void __launch_bounds__(kBlock) k_pcg_dot_rz(uint32_t n, const float *
 // shift the history: the host rotates the three buffers' roles.
 struct KB_dc_init {
     static __device__ __forceinline__ void run(const VB &vb, const Geo &g, const double *b,
-                                               double *x, float *e, int warm, const double *h1,
+                                               double *x, int warm, const double *h1,
                                                double *out, const RedScratch &red)
 {
     __shared__ double lds[kBlock / kWave];
@@ -1827,17 +1829,15 @@ struct KB_dc_init {
             else if (warm == 2) out[c] = 2.0 * pn - h1[c];
             else out[c] = 3.0 * (pn - h1[c]) + out[c];
         }
-        e[c] = 0.f;
         if (g.flags[c] != SOLID) acc += b[c] * b[c];
     }
     block_partial(acc, pslot(red, SLOT_INIT), blk, lds);
 }
 };
-__global__ void __launch_bounds__(kBlock) k_dc_init(Geo g, const double *b, double *x, float *e,
-                                                    int warm, const double *h1, double *out,
-                                                    RedScratch red)
+__global__ void __launch_bounds__(kBlock) k_dc_init(Geo g, const double *b, double *x, int warm,
+                                                    const double *h1, double *out, RedScratch red)
 {
-    KB_dc_init::run(vb_here(), g, b, x, e, warm, h1, out, red);
+    KB_dc_init::run(vb_here(), g, b, x, warm, h1, out, red);
 }
 
 struct KB_dc_start {
@@ -1948,7 +1948,12 @@ __device__ __forceinline__ void row64v(const Geo &g, const uint32_t ijk[3], uint
 // that do not couple are dropped by the code, in the row's face order). An
 // interior row (six unknown neighbours, uniform grid) sums its neighbours per
 // axis before scaling, fewer FP64 operations than the general row.
-template <bool ST>
+//
+// P0: the solve's first pass, which the host knows it launches: e is 0 then,
+// so x stays where it is (xa), e is not read and x not written. Pass k >= 1
+// reads x from xa for odd k and from xb for even k and writes the other, so
+// after P passes x is in xb exactly when P is even.
+template <bool ST, bool P0>
 struct KB_dc_residual {
     static __device__ __forceinline__ void run(const VB &vb, const Geo &g,
                                                const double *__restrict__ b, double *xa, double *xb,
@@ -1961,8 +1966,9 @@ struct KB_dc_residual {
     const uint32_t stop = __builtin_amdgcn_raw_buffer_load_b32(
         buf_rsrc(m, sizeof(MgState)), (int)offsetof(MgState, solve_done), 0, 0);
     // x_in is the buffer of the current pass parity, x_out the other one
-    const double *__restrict__ xin = (m->passes & 1) ? xb : xa;
-    double *__restrict__ xout = (m->passes & 1) ? xa : xb;
+    const bool odd = P0 || (m->passes & 1);
+    const double *__restrict__ xin = odd ? xa : xb;
+    double *__restrict__ xout = odd ? xb : xa;  // not written by P0
     const uint32_t blk = vb.blk;
     const uint32_t si = (uint32_t)g.ny * g.nz, sj = g.nz, H = (g.nc + 1) / 2;
     // the diagonal of an interior row, summed in row64v's face order
@@ -1992,13 +1998,13 @@ struct KB_dc_residual {
             nb[t] = g.nbm[c];
             bv[t] = b[c];
             xv[t] = xin[c];
-            ev[t] = e[(col[t] ? H : 0u) + (c >> 1)];
+            ev[t] = P0 ? 0.f : e[(col[t] ? H : 0u) + (c >> 1)];
             const auto bo = col[t] ? bred : bblack;  // the other colour's e
             const uint32_t q[6] = {c - si, c + si, c - sj, c + sj, c - 1u, c + 1u};
 #pragma unroll
             for (int f = 0; f < 6; f++) {
                 xn[t][f] = buf_ldd(bx, q[f]);
-                en[t][f] = buf_ldf(bo, q[f] >> 1);
+                en[t][f] = P0 ? 0.f : buf_ldf(bo, q[f] >> 1);
             }
         }
         if (late(stop)) return;
@@ -2009,7 +2015,7 @@ struct KB_dc_residual {
             const uint32_t sc = (col[t] ? H : 0u) + (c >> 1);  // e and r32: split layout
             const bool red_slot = col[t] == 0u;
             if (!(nb[t] & NB_SELF)) {
-                xout[c] = 0.0;
+                if (!P0) xout[c] = 0.0;
                 r32[sc] = 0.f;
                 if (red_slot) z0[sc] = 0.f;
                 continue;
@@ -2026,7 +2032,7 @@ struct KB_dc_residual {
             }
             const double xc = xv[t] + (double)ev[t];
             const double rv = bv[t] - (diag * xc - off);
-            xout[c] = xc;
+            if (!P0) xout[c] = xc;
             if constexpr (ST) {
                 const double vol = (double)g.mgf[ijk[t][0]].x * (double)g.mgf[g.nx + ijk[t][1]].x *
                                    (double)g.mgf[g.nx + g.ny + ijk[t][2]].x;
@@ -2044,7 +2050,7 @@ struct KB_dc_residual {
     block_partial(acc, pslot(red, SLOT_UPDATE), blk, lds);
 }
 };
-template <bool ST>
+template <bool ST, bool P0>
 __global__ void __launch_bounds__(kBlock) k_dc_residual(Geo g, const double *__restrict__ b,
                                                         double *xa, double *xb,
                                                         const float *__restrict__ e,
@@ -2052,7 +2058,7 @@ __global__ void __launch_bounds__(kBlock) k_dc_residual(Geo g, const double *__r
                                                         float *__restrict__ z0, const MgState *m,
                                                         RedScratch red)
 {
-    KB_dc_residual<ST>::run(vb_here(), g, b, xa, xb, e, r32, z0, m, red);
+    KB_dc_residual<ST, P0>::run(vb_here(), g, b, xa, xb, e, r32, z0, m, red);
 }
 
 // outer stopping test: ||r|| / ||b|| < tol (one block)
@@ -2088,8 +2094,7 @@ __global__ void __launch_bounds__(kBlock) k_dc_check(MgState *m, RedScratch red,
 // p: unknowns from x; OUTFLOW rows solved from their neighbours (or pinned
 // to 0 for the reference multigrid's convention); SOLID cells 0. In place
 // (xa == p holds the last pass's output, which the host knows from the pass
-// count): only the OUTFLOW rows are written; every outer pass has left 0 at
-// the SOLID and OUTFLOW cells of its output.
+// count): the unknowns are not touched, only SOLID and OUTFLOW cells written.
 template <bool ST>
 struct KB_dc_finalize {
     static __device__ __forceinline__ void run(const VB &vb, const Geo &g,
@@ -2097,7 +2102,8 @@ struct KB_dc_finalize {
                                                const double *xb, double *p, int pin_outflow,
                                                int inplace, const MgState *m)
 {
-    const double *x = inplace ? p : ((m->passes & 1) ? xb : xa);  // the last pass's output
+    // the last pass's output: xb after an even, non-zero number of passes
+    const double *x = inplace ? p : ((m->passes > 0 && !(m->passes & 1)) ? xb : xa);
     uint32_t sb, se;
     block_span_of(g.nc, vb.blk, vb.G, sb, se);
     for (uint32_t c = sb + threadIdx.x; c < se; c += kBlock) {
@@ -2107,7 +2113,7 @@ struct KB_dc_finalize {
             continue;
         }
         if ((nb & NB_SELF_SOLID) || pin_outflow) {
-            if (!inplace) p[c] = 0.0;
+            p[c] = 0.0;
             continue;
         }
         double diag, off;
@@ -2351,11 +2357,11 @@ void pcg_launch_dot_rz(hipStream_t s, uint32_t n, const float *r, const float *z
     FFR_GANG((KB_pcg_dot_rz), GB_PLAIN, k_pcg_dot_rz, dim3(grid_for(n)), dim3(kBlock), 0, s, n, r,
              z, m, red);
 }
-void dc_launch_init(hipStream_t s, const Geo &g, const double *b, double *x, float *e, int warm,
+void dc_launch_init(hipStream_t s, const Geo &g, const double *b, double *x, int warm,
                     const double *h1, double *out, RedScratch red)
 {
     FFR_GANG((KB_dc_init), GB_PLAIN, k_dc_init, dim3(grid_for(g.nc)), dim3(kBlock), 0, s, g, b, x,
-             e, warm, h1, out, red);
+             warm, h1, out, red);
 }
 void dc_launch_start(hipStream_t s, const Geo &g, MgState *m, RedScratch red, double outer_tol,
                      int max_outer, int max_inner, double inner_tol, int total_cap)
@@ -2364,13 +2370,21 @@ void dc_launch_start(hipStream_t s, const Geo &g, MgState *m, RedScratch red, do
              grid_for(g.nc), outer_tol, max_outer, max_inner, inner_tol, total_cap);
 }
 void dc_launch_residual(hipStream_t s, const Geo &g, const double *b, double *xa, double *xb,
-                        const float *e, float *r32, float *z0, const MgState *m, RedScratch red)
+                        const float *e, float *r32, float *z0, const MgState *m, RedScratch red,
+                        bool first)
 {
-    if (g.uniform)
-        FFR_GANG((KB_dc_residual<false>), GB_PLAIN, (k_dc_residual<false>), dim3(grid_for(g.nc)),
+    const dim3 grid(grid_for(g.nc));
+    if (g.uniform && first)
+        FFR_GANG((KB_dc_residual<false, true>), GB_PLAIN, (k_dc_residual<false, true>), grid,
+                 dim3(kBlock), 0, s, g, b, xa, xb, e, r32, z0, m, red);
+    else if (g.uniform)
+        FFR_GANG((KB_dc_residual<false, false>), GB_PLAIN, (k_dc_residual<false, false>), grid,
+                 dim3(kBlock), 0, s, g, b, xa, xb, e, r32, z0, m, red);
+    else if (first)
+        FFR_GANG((KB_dc_residual<true, true>), GB_PLAIN, (k_dc_residual<true, true>), grid,
                  dim3(kBlock), 0, s, g, b, xa, xb, e, r32, z0, m, red);
     else
-        FFR_GANG((KB_dc_residual<true>), GB_PLAIN, (k_dc_residual<true>), dim3(grid_for(g.nc)),
+        FFR_GANG((KB_dc_residual<true, false>), GB_PLAIN, (k_dc_residual<true, false>), grid,
                  dim3(kBlock), 0, s, g, b, xa, xb, e, r32, z0, m, red);
 }
 void dc_launch_check(hipStream_t s, const Geo &g, MgState *m, RedScratch red)
