@@ -242,7 +242,7 @@ def serve_bench(ctx, n_c4: int, steps_c4: int, n_c5: int, steps_c5: int, gangs: 
     return res
 
 
-def batched_forms(ctx, n_rooms: int, steps: int, n_err: int, slots: int = 6):
+def batched_forms(ctx, n_rooms: int, steps: int, n_err: int, slots: int = 6, repeats: int = 3):
     """The north-star comparison in its batched form: the same c4 rooms served
     with continuous batching (all submitted at t = 0) once in the mixed form
     (FP32 stencils, MG-PCG with FP64 correction to 1e-10) and once in the
@@ -264,7 +264,9 @@ def batched_forms(ctx, n_rooms: int, steps: int, n_err: int, slots: int = 6):
     }
     out = {"sample": f"first {n_rooms} rooms of c4, {steps} steps each, all submitted at t=0, "
                      f"{slots} slots, largest work (cells x steps) admitted first in both forms, "
-                     f"after an untimed warm-up batch of one step per room",
+                     f"after an untimed warm-up batch of one step per room; the mixed form's "
+                     f"batch runs {repeats} times (its served rate varies from run to run by up "
+                     f"to 25%) and the run of median rate is reported, all rates listed",
            "forms": {}}
     done_mixed = None
     for name, kw in forms.items():
@@ -272,13 +274,18 @@ def batched_forms(ctx, n_rooms: int, steps: int, n_err: int, slots: int = 6):
         # streams for these room sizes (as the headline's warm-up steps)
         Server(ctx, slots=slots, policy="longest", **kw).run([Request(c, 1) for c in cfgs],
                                                              realtime=False)
-        srv = Server(ctx, slots=slots, keep_fields=(name == "mixed"), policy="longest", **kw)
-        done = srv.run([Request(c, steps) for c in cfgs], realtime=False)
-        sm = summary(done)
+        runs = []
+        for _ in range(repeats if name == "mixed" else 1):
+            srv = Server(ctx, slots=slots, keep_fields=(name == "mixed"), policy="longest", **kw)
+            done = srv.run([Request(c, steps) for c in cfgs], realtime=False)
+            runs.append((summary(done), done))
+        runs.sort(key=lambda r: r[0]["cell_updates_per_s"])
+        sm, done = runs[len(runs) // 2]
         out["forms"][name] = {"cell_updates_per_s": sm["cell_updates_per_s"],
                               "wall_s": sm["wall_s"], "p50_latency_s": sm["p50_latency_s"],
                               "p99_latency_s": sm["p99_latency_s"],
-                              "poisson_tol": kw["poisson_tol"]}
+                              "poisson_tol": kw["poisson_tol"],
+                              "runs_cell_updates_per_s": [r[0]["cell_updates_per_s"] for r in runs]}
         if name == "mixed":
             done_mixed = done
     out["speedup_mixed_vs_fp64_cg"] = (out["forms"]["mixed"]["cell_updates_per_s"] /
