@@ -2104,9 +2104,17 @@ struct KB_dc_finalize {
 {
     // the last pass's output: xb after an even, non-zero number of passes
     const double *x = inplace ? p : ((m->passes > 0 && !(m->passes & 1)) ? xb : xa);
+    // no pass: the right-hand side was 0 (||b|| < 1e-15), and the solution
+    // is 0, as the reference CG returns from its zero start, whatever start
+    // the solve was given
+    const bool zero = m->passes == 0;
     uint32_t sb, se;
     block_span_of(g.nc, vb.blk, vb.G, sb, se);
     for (uint32_t c = sb + threadIdx.x; c < se; c += kBlock) {
+        if (zero) {
+            p[c] = 0.0;
+            continue;
+        }
         const uint16_t nb = g.nbm[c];
         if (nb & NB_SELF) {
             if (!inplace) p[c] = x[c];

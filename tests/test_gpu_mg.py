@@ -134,6 +134,31 @@ def _workload_scene(grid, seed=2):
     return device.prepare(cfg)
 
 
+def test_zero_rhs_gives_zero_pressure_from_any_start(ctx):
+    """A step whose right-hand side is 0 (a still room, no fan) returns p = 0,
+    as the reference CG does from its zero start (PoissonSolver.cpp:96; a
+    norm of b below 1e-30 ends it), whatever pressure the warm start
+    carried in: here a random one uploaded before the step."""
+    nx, ny, nz = 12, 10, 8
+    f = np.zeros((nx, ny, nz), np.uint8)
+    f[-1] = OUTFLOW
+    f[3:5, 2:6, 0:3] = SOLID
+    sc = SceneArrays(nx, ny, nz, 1.2, 1.0, 0.8, f.reshape(-1))
+    for warm in (0, 1, 3):
+        d = device_scene(ctx, sc, capi_params({}, precision=capi.PREC_MIXED,
+                                              poisson_method=capi.POISSON_MGPCG,
+                                              poisson_tol=1e-10, poisson_max_iter=500,
+                                              warm_start=warm))
+        d.upload(p=np.random.default_rng(5).standard_normal(nx * ny * nz))
+        for _ in range(2):
+            r = d.step()
+            out = d.download()
+            assert r["outer_iters"] == 0 and r["poisson_iters"] == 0, (warm, r)
+            assert np.all(out["p"] == 0.0), warm
+            assert all(np.all(out[k] == 0.0) for k in ("u", "v", "w")), warm
+        d.close()
+
+
 def test_warm_start_meets_the_same_tolerance(ctx):
     """A step's mixed pressure solve started from the previous pressure, or
     extrapolated from the last two, meets the same relative residual as one
