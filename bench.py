@@ -673,8 +673,9 @@ def main():
         "config": {"workload": desc, "cells": ps.cells, "mode": args.mode,
                    "poisson_tol": ps.params.poisson_tol, "tuning": tuned,
                    "warm_start": int(ps.params.warm_start) if args.mode == "mixed" else 0,
-                   "note": "the mixed form's step solve starts from the previous step's "
-                           "pressure; the FP64 CG form starts from 0 as the reference does"
+                   "note": "the mixed form's step solve starts from the pressure "
+                           "extrapolated from the last three steps (warm_start 3); the FP64 "
+                           "CG form starts from 0 as the reference does"
                            if args.mode == "mixed" else ""},
         "roofline": roof,
         "kernels": {k: {"launches": v["launches"], "mean_us": 1e3 * v["total_ms"] / v["launches"],
