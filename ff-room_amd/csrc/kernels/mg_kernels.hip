@@ -1703,9 +1703,10 @@ __global__ void __launch_bounds__(kBlock) k_pcg_spmv(Geo g, MgLevel L, const flo
 template <bool ST>
 struct KB_pcg_update {
     static __device__ __forceinline__ void run(const VB &vb, const Geo &g, const MgLevel &L,
-                                               float *e, float *r, const float *p, const float *q,
-                                               float *z, MgState *m, const RedScratch &red,
-                                               uint32_t gq)
+                                               float *__restrict__ e, float *__restrict__ r,
+                                               const float *__restrict__ p,
+                                               const float *__restrict__ q, float *__restrict__ z,
+                                               MgState *m, const RedScratch &red, uint32_t gq)
 {
     __shared__ double lds[kBlock / kWave];
     if (stopped(m)) return;
@@ -1725,12 +1726,15 @@ struct KB_pcg_update {
     const float dint = diag0m(g, kInterior);
     double acc = 0.0;
     for (uint32_t c = sb + threadIdx.x; c < se; c += kBlock) {
-        e[c] = (first ? 0.f : e[c]) + alpha * p[c];
-        const float rv = r[c] - alpha * q[c];
+        // the four distinct vectors (restrict) and a red slot's code: every
+        // load goes out before a store (black slots read slot 0's code, unused)
+        const float pv = p[c], qv = q[c], r0 = r[c], e0 = first ? 0.f : e[c];
+        const uint16_t nb = L.nbs[c < L.H ? c : 0u];
+        e[c] = e0 + alpha * pv;
+        const float rv = r0 - alpha * qv;
         r[c] = rv;
         acc += (double)rv * (double)rv;
         if (c < L.H) {  // red slot
-            const uint16_t nb = L.nbs[c];
             const Cof<ST> cf = cof_at<ST>(g, 2 * c + split_parity(L, c, 0u));
             z[c] = (nb & NB_SELF) ? rv / diag_t<ST>(g, nb, dint, cf) : 0.f;
         }
